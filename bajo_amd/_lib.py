@@ -1,0 +1,125 @@
+"""ctypes binding of ``libbajo_hip.so`` (the C ABI declared in include/bajo_hip.h).
+
+The library is built in-tree by ``python -m bajo_amd.build``.  There is no
+fallback: if the shared object is missing every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_float, c_int32, c_uint32, c_uint64, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libbajo_hip.so"
+
+BAJO_OK = 0
+BAJO_TRACE_CLOSEST_HIT = 0
+BAJO_TRACE_ANY_HIT = 1
+BAJO_BUILD_LBVH = 0
+
+
+class BajoError(RuntimeError):
+    """A non-zero bajo status (argument error, HIP error or device status)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"bajo status {status}: {message}")
+        self.status = status
+
+
+class BuildTimings(ctypes.Structure):
+    _fields_ = [
+        ("bounds_ns", c_uint64),
+        ("morton_ns", c_uint64),
+        ("sort_ns", c_uint64),
+        ("topology_ns", c_uint64),
+        ("refit_ns", c_uint64),
+        ("total_ns", c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BvhInfo(ctypes.Structure):
+    _fields_ = [
+        ("prim_count", c_uint64),
+        ("node_count", c_uint64),
+        ("root", c_uint32),
+        ("reserved", c_uint32),
+        ("nodes", c_void_p),
+        ("triangles", c_void_p),
+        ("sorted_keys", c_void_p),
+        ("leaf_parent", c_void_p),
+        ("leaf_counts", c_void_p),
+        ("scene_bounds", c_void_p),
+    ]
+
+
+# name -> (restype, argtypes); the exported surface of include/bajo_hip.h
+SIGNATURES = {
+    "bajo_ctx_create": (c_int32, [c_int32, c_void_p, POINTER(c_void_p)]),
+    "bajo_ctx_destroy": (c_int32, [c_void_p]),
+    "bajo_ctx_synchronize": (c_int32, [c_void_p]),
+    "bajo_last_error": (c_char_p, []),
+    "bajo_status_string": (c_char_p, [c_int32]),
+    "bajo_device_name": (c_int32, [c_int32, c_char_p, c_int32]),
+    "bajo_sort_workspace_bytes": (c_uint64, [c_uint64, c_int32, c_int32]),
+    "bajo_device_radix_sort_pairs_u64_u32": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, c_uint64]),
+    "bajo_device_radix_sort_pairs_u32_u32": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, c_uint64]),
+    "bajo_device_radix_sort_keys_u64": (c_int32, [c_void_p, c_void_p, c_uint64, c_void_p, c_uint64]),
+    "bajo_device_radix_sort_keys_u32": (c_int32, [c_void_p, c_void_p, c_uint64, c_void_p, c_uint64]),
+    "bajo_onesweep_radix_sort_keys_u64": (
+        c_int32,
+        [c_void_p, c_void_p, c_uint64, c_int32, c_int32, c_void_p, c_uint64],
+    ),
+    "bajo_onesweep_radix_sort_pairs_u64_u32": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_uint64, c_int32, c_int32, c_void_p, c_uint64],
+    ),
+    "bajo_build_triangle_bvh": (
+        c_int32,
+        [c_void_p, c_void_p, c_uint64, c_int32, POINTER(BuildTimings), POINTER(c_void_p)],
+    ),
+    "bajo_bvh_free": (c_int32, [c_void_p]),
+    "bajo_bvh_get_info": (c_int32, [c_void_p, POINTER(BvhInfo)]),
+    "bajo_bvh_copy_binary": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
+    "bajo_trace_rays": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_uint64, c_void_p]),
+    "bajo_trace_camera": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, c_void_p],
+    ),
+    "bajo_camera_make": (
+        c_int32,
+        [POINTER(c_float), POINTER(c_float), POINTER(c_float), c_float, c_float, POINTER(c_float)],
+    ),
+    "bajo_scene_soup": (c_int32, [c_uint64, c_uint64, c_uint64, c_float, c_float, c_void_p]),
+    "bajo_scene_icosphere": (c_int32, [c_uint32, c_uint64, c_uint64, c_float, c_float, c_void_p]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads the library once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m bajo_amd.build`")
+        handle = ctypes.CDLL(str(LIB_PATH), mode=ctypes.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != BAJO_OK:
+        msg = lib().bajo_last_error().decode(errors="replace")
+        if not msg:
+            msg = lib().bajo_status_string(status).decode()
+        raise BajoError(status, msg)

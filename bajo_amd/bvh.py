@@ -1,0 +1,201 @@
+"""Triangle BVH build and traversal with bajo's ``bajo/bvh/gpu`` interface.
+
+Mirrors ``build_triangle_bvh`` / ``build_triangle_bvh_measured``
+(bvh/gpu/triangle_bvh.mojo:617-646) and ``GpuTriangleBvh.launch_rays`` /
+``launch_camera`` / ``launch_camera_instrumented`` (:520-614), plus bajo's
+``Camera`` record (bvh/camera.mojo:19-231).  Buffers follow bajo's ABIs:
+vertices f32[9N], rays f32[8R] field-major, hits f32[8R] AoS, camera
+f32[20 * views], traversal stats u32[5R].
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .context import DeviceContext, default_context, ptr, require_device
+
+EMPTY_LANE = 0xFFFFFFFF
+LBVH_SENTINEL = 0xFFFFFFFF
+RAY_STRIDE = 8
+HIT_STRIDE = 8
+CAMERA_STRIDE = 20
+STATS_STRIDE = 5
+F32_MAX = float(np.finfo(np.float32).max)
+
+
+class TRACE:
+    """bvh/constants.mojo:9-13."""
+
+    CLOSEST_HIT = _lib.BAJO_TRACE_CLOSEST_HIT
+    ANY_HIT = _lib.BAJO_TRACE_ANY_HIT
+
+
+class GpuBvhBuildMethod:
+    LBVH = _lib.BAJO_BUILD_LBVH
+
+
+@dataclass
+class Camera:
+    """bajo's pinhole camera (bvh/camera.mojo:86-106), flattened to 20 floats."""
+
+    origin: tuple
+    target: tuple
+    world_up: tuple = (0.0, 1.0, 0.0)
+    fov_scale: float = 0.75
+    focus_dist: float = 1.0
+
+    def flatten(self) -> np.ndarray:
+        f3 = ctypes.c_float * 3
+        out = (ctypes.c_float * CAMERA_STRIDE)()
+        _lib.check(
+            _lib.lib().bajo_camera_make(
+                f3(*self.origin), f3(*self.target), f3(*self.world_up), self.fov_scale, self.focus_dist, out
+            )
+        )
+        return np.frombuffer(out, dtype=np.float32).copy()
+
+
+class GpuTriangleBvh:
+    """Ready-to-trace triangle BVH (bvh/gpu/triangle_bvh.mojo:494-614)."""
+
+    def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, prim_count: int):
+        self._handle = handle
+        self.ctx = ctx
+        self.prim_count = prim_count
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if not self._handle:
+            raise RuntimeError("BVH already freed")
+        return self._handle
+
+    def info(self) -> _lib.BvhInfo:
+        out = _lib.BvhInfo()
+        _lib.check(_lib.lib().bajo_bvh_get_info(self.handle, ctypes.byref(out)))
+        return out
+
+    def rebuild(self, vertices: torch.Tensor, timings: bool = False):
+        """Rebuilds in place over new vertices, reusing device memory when it fits."""
+        n = _check_vertices(vertices, self.ctx)
+        t = _lib.BuildTimings() if timings else None
+        handle = ctypes.c_void_p(self.handle.value)
+        _lib.check(
+            _lib.lib().bajo_build_triangle_bvh(
+                self.ctx.handle, ptr(vertices), n, GpuBvhBuildMethod.LBVH, ctypes.byref(t) if t else None,
+                ctypes.byref(handle),
+            )
+        )
+        self._handle = handle
+        self.prim_count = n
+        return t.as_dict() if t else None
+
+    def launch_rays(self, rays, hits, ray_count=None, mode=TRACE.CLOSEST_HIT, stats=None) -> None:
+        """launch_rays[mode] (bvh/gpu/triangle_bvh.mojo:587-614)."""
+        r = rays.numel() // RAY_STRIDE if ray_count is None else int(ray_count)
+        for name, buf, stride in (("rays", rays, RAY_STRIDE), ("hits", hits, HIT_STRIDE)):
+            require_device(buf, self.ctx, name)
+            if buf.numel() < r * stride:
+                raise ValueError(f"{name} buffer is too short for {r} rays")
+        if rays.numel() != r * RAY_STRIDE:
+            raise ValueError("rays must hold exactly 8 field-major planes of ray_count entries")
+        if stats is not None and stats.numel() < r * STATS_STRIDE:
+            raise ValueError("stats buffer is too short")
+        _lib.check(
+            _lib.lib().bajo_trace_rays(self.ctx.handle, self.handle, int(mode), ptr(rays), ptr(hits), r, ptr(stats))
+        )
+
+    def launch_camera(self, camera_params, hits, ray_count, width, height, stats=None) -> None:
+        """launch_camera (bvh/gpu/triangle_bvh.mojo:520-548)."""
+        require_device(camera_params, self.ctx, "camera_params")
+        require_device(hits, self.ctx, "hits")
+        views = -(-int(ray_count) // (int(width) * int(height)))
+        if camera_params.numel() < views * CAMERA_STRIDE:
+            raise ValueError("camera parameter buffer is too short")
+        if hits.numel() < int(ray_count) * HIT_STRIDE:
+            raise ValueError("hit output buffer is too short")
+        if stats is not None and stats.numel() < int(ray_count) * STATS_STRIDE:
+            raise ValueError("traversal stats output buffer is too short")
+        _lib.check(
+            _lib.lib().bajo_trace_camera(
+                self.ctx.handle, self.handle, ptr(camera_params), ptr(hits), int(ray_count), int(width), int(height),
+                ptr(stats),
+            )
+        )
+
+    def launch_camera_instrumented(self, camera_params, hits, stats, ray_count, width, height) -> None:
+        """launch_camera_instrumented (bvh/gpu/triangle_bvh.mojo:550-585)."""
+        self.launch_camera(camera_params, hits, ray_count, width, height, stats=stats)
+
+    def binary(self) -> dict:
+        """Host copy of the binary LBVH in bajo's GpuBinaryBoundsBvh layout."""
+        n = self.prim_count
+        m = max(n - 1, 0)
+        out = {
+            "sorted_keys": np.empty(n, np.uint64),
+            "leaf_ids": np.empty(n, np.uint32),
+            "node_meta": np.empty(4 * m, np.uint32),
+            "node_bounds": np.empty(12 * m, np.float32),
+            "leaf_parent": np.empty(n, np.uint32),
+            "leaf_counts": np.empty(m, np.uint32),
+            "scene_bounds": np.empty(12, np.float32),
+        }
+        p = {k: v.ctypes.data_as(ctypes.c_void_p) for k, v in out.items()}
+        _lib.check(
+            _lib.lib().bajo_bvh_copy_binary(
+                self.ctx.handle, self.handle, p["sorted_keys"], p["leaf_ids"], p["node_meta"], p["node_bounds"],
+                p["leaf_parent"], p["leaf_counts"], p["scene_bounds"],
+            )
+        )
+        out["root"] = self.info().root
+        return out
+
+    def free(self) -> None:
+        if self._handle:
+            _lib.lib().bajo_bvh_free(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _check_vertices(vertices: torch.Tensor, ctx: DeviceContext) -> int:
+    require_device(vertices, ctx, "vertices")
+    if vertices.dtype != torch.float32:
+        raise ValueError("vertices must be float32")
+    if vertices.numel() % 9 != 0:
+        raise ValueError("triangle vertex buffer must contain complete triangle records")
+    n = vertices.numel() // 9
+    if n == 0:
+        raise ValueError("standalone triangle BVH requires nonempty input")
+    return n
+
+
+def build_triangle_bvh(vertices: torch.Tensor, ctx: DeviceContext | None = None, timings: dict | None = None):
+    """build_triangle_bvh[WORLD, 2, 1, LBVH] (bvh/gpu/triangle_bvh.mojo:617-630)."""
+    ctx = ctx or default_context(vertices.device.index or 0)
+    n = _check_vertices(vertices, ctx)
+    t = _lib.BuildTimings() if timings is not None else None
+    handle = ctypes.c_void_p()
+    _lib.check(
+        _lib.lib().bajo_build_triangle_bvh(
+            ctx.handle, ptr(vertices), n, GpuBvhBuildMethod.LBVH, ctypes.byref(t) if t else None, ctypes.byref(handle)
+        )
+    )
+    if t is not None:
+        timings.update(t.as_dict())
+    return GpuTriangleBvh(handle, ctx, n)
+
+
+def build_triangle_bvh_measured(vertices: torch.Tensor, ctx: DeviceContext | None = None):
+    """build_triangle_bvh_measured (bvh/gpu/triangle_bvh.mojo:633-646): returns (bvh, timings)."""
+    t: dict = {}
+    bvh = build_triangle_bvh(vertices, ctx=ctx, timings=t)
+    return bvh, t

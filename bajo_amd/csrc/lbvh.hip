@@ -1,0 +1,439 @@
+// LBVH builder for gfx950.
+//
+// Replaces build_triangle_bvh[..., LBVH] (bvh/gpu/triangle_bvh.mojo:617-727)
+// and build_binary_bvh_with_lbvh (bvh/gpu/builder/lbvh.mojo:332-424).  The
+// stages and their bajo counterparts:
+//   1. scene bounds: root box and centroid box in one pass over the
+//      vertices (bajo writes per-triangle boxes, then reduces them in a
+//      host-driven multi-level loop, binary_layout.mojo:128-204,383-415).
+//      min/max are folded through order-preserving integer atomics, so the
+//      result is independent of block order.
+//   2. Morton keys (lbvh.mojo:30-62): key = (morton30 << 32) | prim.  The
+//      kernel also fills the digit histogram of the high word for the sort
+//      and clears its look-back table, so the sort needs no histogram pass.
+//   3. onesweep sort of the high 32 bits only: keys are written in prim
+//      order and the sort is stable, which yields the same order as a full
+//      64-bit sort (and as bajo's stable (segment, code) pair sort with
+//      value = index, lbvh.mojo:94-113) in 4 passes instead of 8.
+//   4. Karras topology (lbvh.mojo:164-329), one thread per internal node.
+//   5. atomic bottom-up refit (lbvh.mojo:116-161): the second arriver at a
+//      node merges both child boxes and climbs.  The same pass packs each
+//      sorted leaf's triangle (v0, e1, e2, prim) for traversal.
+#include <string.h>
+
+#include "bvh_internal.h"
+#include "sort_internal.h"
+
+namespace bj {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSortBeginBit = 32;
+constexpr int kSortEndBit = 64;
+constexpr int kSortPasses = (kSortEndBit - kSortBeginBit) / 8;
+
+BJ_D float ld(const float* p) { return __builtin_nontemporal_load(p); }
+
+BJ_D void load_tri(const float* __restrict__ v, uint64_t i, v3& a, v3& b, v3& c) {
+  const float* t = v + 9 * i;
+  a = mk(ld(t + 0), ld(t + 1), ld(t + 2));
+  b = mk(ld(t + 3), ld(t + 4), ld(t + 5));
+  c = mk(ld(t + 6), ld(t + 7), ld(t + 8));
+}
+
+__global__ void __launch_bounds__(256) lbvh_init_kernel(uint32_t* acc, uint32_t* hist, uint32_t* tickets) {
+  uint32_t t = threadIdx.x;
+  if (t < 12) {
+    // (+FLT_MAX, -FLT_MAX) is bajo's empty box (core/aabb.mojo:28-35).
+    bool is_min = (t % 6) < 3;
+    acc[t] = f2ord(is_min ? kFltMax : -kFltMax);
+  }
+  for (uint32_t i = t; i < kSortPasses * 256; i += blockDim.x) hist[i] = 0;
+  if (t < 8) tickets[t] = 0;
+}
+
+// Wave-level min/max folds (select semantics; exact for all non-NaN inputs).
+BJ_D float wave_min(float x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x = fmin_sel(x, __shfl_xor(x, off, 64));
+  return x;
+}
+BJ_D float wave_max(float x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x = fmax_sel(x, __shfl_xor(x, off, 64));
+  return x;
+}
+
+__global__ void __launch_bounds__(kBlock) scene_bounds_kernel(const float* __restrict__ vertices, uint64_t n,
+                                                              uint32_t* __restrict__ acc) {
+  float r[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) r[k] = ((k % 6) < 3) ? kFltMax : -kFltMax;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    v3 a, b, c;
+    load_tri(vertices, i, a, b, c);
+    v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
+    v3 ce = scale(add(lo, hi), 0.5f);
+    r[0] = fmin_sel(r[0], lo.x); r[1] = fmin_sel(r[1], lo.y); r[2] = fmin_sel(r[2], lo.z);
+    r[3] = fmax_sel(r[3], hi.x); r[4] = fmax_sel(r[4], hi.y); r[5] = fmax_sel(r[5], hi.z);
+    r[6] = fmin_sel(r[6], ce.x); r[7] = fmin_sel(r[7], ce.y); r[8] = fmin_sel(r[8], ce.z);
+    r[9] = fmax_sel(r[9], ce.x); r[10] = fmax_sel(r[10], ce.y); r[11] = fmax_sel(r[11], ce.z);
+  }
+  __shared__ float s[kBlock / 64][12];
+  const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    float v = ((k % 6) < 3) ? wave_min(r[k]) : wave_max(r[k]);
+    if (lane == 0) s[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    int k = threadIdx.x;
+    bool is_min = (k % 6) < 3;
+    float v = s[0][k];
+    for (int w = 1; w < kBlock / 64; ++w) v = is_min ? fmin_sel(v, s[w][k]) : fmax_sel(v, s[w][k]);
+    if (is_min) atomicMin(&acc[k], f2ord(v));
+    else atomicMax(&acc[k], f2ord(v));
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) morton_kernel(const float* __restrict__ vertices, uint64_t n,
+                                                        const uint32_t* __restrict__ acc, float* __restrict__ scene,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ hist,
+                                                        uint32_t* __restrict__ lookback, uint64_t lookback_words) {
+  __shared__ uint32_t s_hist[kBlock / 64][kSortPasses * 256];
+  const uint32_t wave = threadIdx.x >> 6;
+  for (uint32_t i = threadIdx.x; i < (kBlock / 64) * kSortPasses * 256; i += blockDim.x) (&s_hist[0][0])[i] = 0;
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t gstride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = gtid; i < lookback_words; i += gstride) lookback[i] = 0;
+  if (gtid < 12) scene[gtid] = ord2f(acc[gtid]);
+  // bvh/gpu/builder/lbvh.mojo:46-52: cmin and safe_inv(extent) of the centroid box.
+  const v3 cmin = mk(ord2f(acc[6]), ord2f(acc[7]), ord2f(acc[8]));
+  const v3 cmax = mk(ord2f(acc[9]), ord2f(acc[10]), ord2f(acc[11]));
+  const v3 inv = safe_inv(sub(cmax, cmin));
+  __syncthreads();
+  for (uint64_t i = gtid; i < n; i += gstride) {
+    v3 a, b, c;
+    load_tri(vertices, i, a, b, c);
+    v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
+    v3 q = mul(sub(scale(add(lo, hi), 0.5f), cmin), inv);
+    uint32_t code = morton3(q.x, q.y, q.z);
+    keys[i] = ((uint64_t)code << 32) | (uint64_t)(uint32_t)i;
+#pragma unroll
+    for (int p = 0; p < kSortPasses; ++p) atomicAdd(&s_hist[wave][p * 256 + ((code >> (8 * p)) & 0xFFu)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kSortPasses * 256; i += blockDim.x) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) c += s_hist[w][i];
+    if (c) atomicAdd(&hist[i], c);
+  }
+}
+
+// Karras common prefix over (code << 32 | sorted index), -1 outside [0, n).
+BJ_D int karras_delta(const uint64_t* __restrict__ keys, int64_t n, int64_t i, uint32_t code_i, int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t code_j = (uint32_t)(keys[j] >> 32);
+  uint64_t x = ((uint64_t)(code_i ^ code_j) << 32) | (uint64_t)((uint32_t)i ^ (uint32_t)j);
+  return x == 0 ? 64 : __builtin_clzll(x);
+}
+
+__global__ void __launch_bounds__(kBlock) karras_kernel(const uint64_t* __restrict__ keys, uint64_t n_u,
+                                                        BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_parent,
+                                                        uint32_t* __restrict__ leaf_counts,
+                                                        uint32_t* __restrict__ flags) {
+  const int64_t n = (int64_t)n_u;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const uint32_t ci = (uint32_t)(keys[i] >> 32);
+  // bvh/gpu/builder/lbvh.mojo:164-191: direction and range.
+  int d_next = karras_delta(keys, n, i, ci, i + 1);
+  int d_prev = karras_delta(keys, n, i, ci, i - 1);
+  int64_t d = d_next < d_prev ? -1 : 1;
+  int delta_min = karras_delta(keys, n, i, ci, i - d);
+  int64_t lmax = 2;
+  while (karras_delta(keys, n, i, ci, i + lmax * d) > delta_min) {
+    lmax <<= 1;
+    if (lmax > n * 2) break;
+  }
+  int64_t l = 0;
+  for (int64_t t = lmax >> 1; t > 0; t >>= 1)
+    if (karras_delta(keys, n, i, ci, i + (l + t) * d) > delta_min) l += t;
+  int64_t j = i + l * d;
+  int64_t first = i < j ? i : j, last = i < j ? j : i;
+  // bvh/gpu/builder/lbvh.mojo:194-218: split by binary search.
+  const uint32_t cf = (uint32_t)(keys[first] >> 32);
+  int node_prefix = karras_delta(keys, n, first, cf, last);
+  int64_t split = first, step = last - first;
+  while (step > 1) {
+    step = (step + 1) >> 1;
+    int64_t ns = split + step;
+    if (ns < last && karras_delta(keys, n, first, cf, ns) > node_prefix) split = ns;
+  }
+  flags[i] = 0;
+  leaf_counts[i] = (uint32_t)(last - first + 1);
+  if (first == 0 && last == n - 1) nodes[i].parent = kSentinel;
+  uint32_t left, right;
+  if (split == first) {
+    left = kLeafFlag | (uint32_t)split;
+    leaf_parent[split] = (uint32_t)i;
+  } else {
+    left = (uint32_t)split;
+    nodes[split].parent = (uint32_t)i;
+  }
+  if (split + 1 == last) {
+    right = kLeafFlag | (uint32_t)(split + 1);
+    leaf_parent[split + 1] = (uint32_t)i;
+  } else {
+    right = (uint32_t)(split + 1);
+    nodes[split + 1].parent = (uint32_t)i;
+  }
+  nodes[i].left = left;
+  nodes[i].right = right;
+  nodes[i].fence = (uint32_t)last;
+}
+
+__global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__ vertices,
+                                                       const uint64_t* __restrict__ keys, uint64_t n,
+                                                       BjNode* __restrict__ nodes,
+                                                       uint32_t* __restrict__ leaf_parent,
+                                                       uint32_t* __restrict__ flags, BjTri* __restrict__ tris) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t prim = (uint32_t)keys[s];
+  v3 a, b, c;
+  load_tri(vertices, prim, a, b, c);
+  v3 e1 = sub(b, a), e2 = sub(c, a);
+  float4* out = reinterpret_cast<float4*>(tris + s);
+  out[0] = make_float4(a.x, a.y, a.z, __builtin_bit_cast(float, prim));
+  out[1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+  out[2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+  if (n < 2) {
+    leaf_parent[0] = kSentinel;  // single-leaf segment, lbvh.mojo:246-248
+    return;
+  }
+  // bvh/gpu/triangle_bvh.mojo:711-712: leaf box = vmin(vmin(v0,v1),v2).
+  v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
+  uint32_t cur = kLeafFlag | (uint32_t)s;
+  uint32_t parent = leaf_parent[s];
+  // A Karras hierarchy over 64-bit keys is at most 64 levels deep; the index
+  // and depth checks only keep a corrupted hierarchy from leaving the arrays.
+  for (int depth = 0; parent != kSentinel && parent < n - 1 && depth < 128; ++depth) {
+    BjNode* node = nodes + parent;
+    const uint32_t left = node->left, right = node->right;  // written by the topology launch
+    const bool is_left = cur == left;
+    if (!is_left && cur != right) break;
+    float* dst = node->b + (is_left ? 0 : 6);
+    dst[0] = lo.x; dst[1] = lo.y; dst[2] = lo.z;
+    dst[3] = hi.x; dst[4] = hi.y; dst[5] = hi.z;
+    // Release our half, acquire the sibling's: the second arriver merges.
+    uint32_t old = __hip_atomic_fetch_add(flags + parent, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) break;
+    const float* bb = node->b;
+    v3 llo = mk(bb[0], bb[1], bb[2]), lhi = mk(bb[3], bb[4], bb[5]);
+    v3 rlo = mk(bb[6], bb[7], bb[8]), rhi = mk(bb[9], bb[10], bb[11]);
+    lo = vmin(llo, rlo);
+    hi = vmax(lhi, rhi);
+    cur = parent;
+    parent = node->parent;
+  }
+}
+
+int32_t ensure_capacity(bajo_bvh* h, uint64_t n) {
+  SortPlan plan = make_sort_plan(n, 8, 0, kSortBeginBit, kSortEndBit);
+  if (n <= h->capacity && plan.total <= h->sort_ws_bytes) return BAJO_OK;
+  hipFree(h->nodes); hipFree(h->tris); hipFree(h->keys); hipFree(h->leaf_parent); hipFree(h->leaf_counts);
+  hipFree(h->flags); hipFree(h->acc); hipFree(h->scene); hipFree(h->sort_ws);
+  h->nodes = nullptr; h->tris = nullptr; h->keys = nullptr; h->leaf_parent = nullptr; h->leaf_counts = nullptr;
+  h->flags = nullptr; h->acc = nullptr; h->scene = nullptr; h->sort_ws = nullptr;
+  h->capacity = 0;
+  h->sort_ws_bytes = 0;
+  uint64_t internal = n > 1 ? n - 1 : 1;
+  BJ_HIP(hipMalloc(&h->nodes, internal * sizeof(BjNode)));
+  BJ_HIP(hipMalloc(&h->tris, n * sizeof(BjTri)));
+  BJ_HIP(hipMalloc(&h->keys, n * sizeof(uint64_t)));
+  BJ_HIP(hipMalloc(&h->leaf_parent, n * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&h->leaf_counts, internal * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&h->flags, internal * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&h->acc, 16 * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&h->scene, 16 * sizeof(float)));
+  BJ_HIP(hipMalloc(&h->sort_ws, plan.total));
+  h->capacity = n;
+  h->sort_ws_bytes = plan.total;
+  return BAJO_OK;
+}
+
+void free_bvh(bajo_bvh* h) {
+  if (!h) return;
+  hipFree(h->nodes); hipFree(h->tris); hipFree(h->keys); hipFree(h->leaf_parent); hipFree(h->leaf_counts);
+  hipFree(h->flags); hipFree(h->acc); hipFree(h->scene); hipFree(h->sort_ws);
+  delete h;
+}
+
+struct StageEvents {
+  hipEvent_t ev[6] = {};
+  bool on = false;
+  int32_t create(bool enable) {
+    on = enable;
+    if (!on) return BAJO_OK;
+    for (auto& e : ev) BJ_HIP(hipEventCreate(&e));
+    return BAJO_OK;
+  }
+  void record(int k, hipStream_t s) {
+    if (on) hipEventRecord(ev[k], s);
+  }
+  ~StageEvents() {
+    if (on)
+      for (auto& e : ev) hipEventDestroy(e);
+  }
+};
+
+int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n, bajo_build_timings* timings) {
+  int32_t rc = ensure_capacity(h, n);
+  if (rc) return rc;
+  h->n = n;
+  h->root = n == 1 ? (kLeafFlag | 0u) : 0u;
+  SortPlan plan = make_sort_plan(n, 8, 0, kSortBeginBit, kSortEndBit);
+  char* ws = (char*)h->sort_ws;
+  uint32_t* hist = (uint32_t*)(ws + plan.off_hist);
+  uint32_t* lookback = (uint32_t*)(ws + plan.off_lookback);
+  uint32_t* tickets = (uint32_t*)(ws + plan.off_tickets);
+  const uint64_t lookback_words = (uint64_t)plan.passes * plan.tiles * 256;
+  hipStream_t st = ctx->stream;
+  StageEvents ev;
+  rc = ev.create(timings != nullptr);
+  if (rc) return rc;
+
+  unsigned stream_blocks = div_up(n, (uint64_t)kBlock * 8);
+  if (stream_blocks > 4096) stream_blocks = 4096;
+  if (stream_blocks < 1) stream_blocks = 1;
+
+  ev.record(0, st);
+  lbvh_init_kernel<<<1, 256, 0, st>>>(h->acc, hist, tickets);
+  scene_bounds_kernel<<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->acc);
+  BJ_HIP(hipGetLastError());
+  ev.record(1, st);
+  morton_kernel<<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->acc, h->scene, h->keys, hist, lookback,
+                                                  lookback_words);
+  BJ_HIP(hipGetLastError());
+  ev.record(2, st);
+  rc = onesweep_run(ctx, plan, h->keys, nullptr, ws, true);
+  if (rc) return rc;
+  ev.record(3, st);
+  if (n > 1) {
+    karras_kernel<<<div_up(n - 1, kBlock), kBlock, 0, st>>>(h->keys, n, h->nodes, h->leaf_parent, h->leaf_counts,
+                                                           h->flags);
+    BJ_HIP(hipGetLastError());
+  }
+  ev.record(4, st);
+  refit_kernel<<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent, h->flags,
+                                                     h->tris);
+  BJ_HIP(hipGetLastError());
+  ev.record(5, st);
+  if (timings) {
+    BJ_HIP(hipEventSynchronize(ev.ev[5]));
+    float ms[5];
+    for (int k = 0; k < 5; ++k) BJ_HIP(hipEventElapsedTime(&ms[k], ev.ev[k], ev.ev[k + 1]));
+    timings->bounds_ns = (uint64_t)(ms[0] * 1e6);
+    timings->morton_ns = (uint64_t)(ms[1] * 1e6);
+    timings->sort_ns = (uint64_t)(ms[2] * 1e6);
+    timings->topology_ns = (uint64_t)(ms[3] * 1e6);
+    timings->refit_ns = (uint64_t)(ms[4] * 1e6);
+    float total;
+    BJ_HIP(hipEventElapsedTime(&total, ev.ev[0], ev.ev[5]));
+    timings->total_ns = (uint64_t)(total * 1e6);
+  }
+  return BAJO_OK;
+}
+
+}  // namespace
+}  // namespace bj
+
+extern "C" {
+
+int32_t bajo_build_triangle_bvh(bajo_ctx* ctx, const float* vertices, uint64_t n, int32_t method,
+                                bajo_build_timings* timings, bajo_bvh** out) {
+  BJ_REQUIRE(ctx != nullptr && out != nullptr, "bajo_build_triangle_bvh: ctx/out is NULL");
+  BJ_REQUIRE(method == BAJO_BUILD_LBVH, "bajo_build_triangle_bvh: unsupported build method %d", method);
+  BJ_REQUIRE(n > 0, "bajo_build_triangle_bvh: standalone triangle BVH requires nonempty input");
+  BJ_REQUIRE(vertices != nullptr, "bajo_build_triangle_bvh: vertices is NULL");
+  BJ_REQUIRE(n <= bj::kSortMaxKeys, "bajo_build_triangle_bvh: n=%llu exceeds 2^30-1 triangles",
+             (unsigned long long)n);
+  bajo_bvh* h = *out;
+  bool fresh = h == nullptr;
+  if (fresh) {
+    h = new bajo_bvh();
+    memset(h, 0, sizeof(*h));
+    h->device = ctx->device;
+  } else {
+    BJ_REQUIRE(h->device == ctx->device, "bajo_build_triangle_bvh: handle belongs to device %d", h->device);
+  }
+  int32_t rc = bj::build_into(ctx, h, vertices, n, timings);
+  if (rc) {
+    if (fresh) bj::free_bvh(h);
+    return rc;
+  }
+  *out = h;
+  return BAJO_OK;
+}
+
+int32_t bajo_bvh_free(bajo_bvh* bvh) {
+  bj::free_bvh(bvh);
+  return BAJO_OK;
+}
+
+int32_t bajo_bvh_get_info(const bajo_bvh* h, bajo_bvh_info* out) {
+  BJ_REQUIRE(h != nullptr && out != nullptr, "bajo_bvh_get_info: NULL argument");
+  out->prim_count = h->n;
+  out->node_count = h->n > 0 ? h->n - 1 : 0;
+  out->root = h->root;
+  out->reserved = 0;
+  out->nodes = h->nodes;
+  out->triangles = h->tris;
+  out->sorted_keys = h->keys;
+  out->leaf_parent = h->leaf_parent;
+  out->leaf_counts = h->leaf_counts;
+  out->scene_bounds = h->scene;
+  return BAJO_OK;
+}
+
+int32_t bajo_bvh_copy_binary(bajo_ctx* ctx, const bajo_bvh* h, uint64_t* sorted_keys, uint32_t* leaf_ids,
+                             uint32_t* node_meta, float* node_bounds, uint32_t* leaf_parent, uint32_t* leaf_counts,
+                             float* scene_bounds) {
+  BJ_REQUIRE(ctx != nullptr && h != nullptr, "bajo_bvh_copy_binary: NULL argument");
+  const uint64_t n = h->n, m = n > 0 ? n - 1 : 0;
+  BJ_HIP(hipStreamSynchronize(ctx->stream));
+  uint64_t* keys = (uint64_t*)malloc(n * sizeof(uint64_t));
+  BJ_REQUIRE(keys != nullptr, "bajo_bvh_copy_binary: host allocation failed");
+  hipError_t e = hipMemcpy(keys, h->keys, n * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) { free(keys); return bj::hip_fail(e, "hipMemcpy(keys)", __FILE__, __LINE__); }
+  if (sorted_keys) memcpy(sorted_keys, keys, n * sizeof(uint64_t));
+  if (leaf_ids)
+    for (uint64_t i = 0; i < n; ++i) leaf_ids[i] = (uint32_t)keys[i];
+  free(keys);
+  if ((node_meta || node_bounds) && m > 0) {
+    BjNode* nodes = (BjNode*)malloc(m * sizeof(BjNode));
+    BJ_REQUIRE(nodes != nullptr, "bajo_bvh_copy_binary: host allocation failed");
+    e = hipMemcpy(nodes, h->nodes, m * sizeof(BjNode), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { free(nodes); return bj::hip_fail(e, "hipMemcpy(nodes)", __FILE__, __LINE__); }
+    for (uint64_t i = 0; i < m; ++i) {
+      if (node_bounds) memcpy(node_bounds + 12 * i, nodes[i].b, 12 * sizeof(float));
+      if (node_meta) {
+        node_meta[4 * i + 0] = nodes[i].parent;
+        node_meta[4 * i + 1] = nodes[i].left;
+        node_meta[4 * i + 2] = nodes[i].right;
+        node_meta[4 * i + 3] = nodes[i].fence;
+      }
+    }
+    free(nodes);
+  }
+  if (leaf_parent) BJ_HIP(hipMemcpy(leaf_parent, h->leaf_parent, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (leaf_counts && m > 0) BJ_HIP(hipMemcpy(leaf_counts, h->leaf_counts, m * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (scene_bounds) BJ_HIP(hipMemcpy(scene_bounds, h->scene, 12 * sizeof(float), hipMemcpyDeviceToHost));
+  return BAJO_OK;
+}
+
+}  // extern "C"

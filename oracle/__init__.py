@@ -1,0 +1,239 @@
+"""CPU oracle -- test infrastructure only.
+
+ctypes wrapper of ``oracle/liboracle.so``, the plain-C restatement of bajo's
+BVH algorithms (orc_lbvh.c, orc_trace.c).  Imported by the tests, the smoke
+check and the CPU-baseline leg of bench.py, as the checker; the product
+package ``bajo_amd`` never imports it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from ctypes import POINTER, c_float, c_int, c_int64, c_uint32, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+SENTINEL = 0xFFFFFFFF
+LEAF_FLAG = 0x80000000
+
+_SIG = {
+    "orc_expand_bits": (c_uint32, [c_uint32]),
+    "orc_morton": (c_uint32, [c_float, c_float, c_float]),
+    "orc_common_prefix": (c_int, [c_uint32, c_uint32, c_uint32, c_uint32]),
+    "orc_triangle_bounds": (None, [c_void_p, c_uint64, c_void_p]),
+    "orc_scene_bounds": (None, [c_void_p, c_uint64, c_void_p]),
+    "orc_morton_codes": (None, [c_void_p, c_uint64, c_void_p, c_void_p]),
+    "orc_radix_sort_pairs_u64": (c_int, [c_void_p, c_void_p, c_uint64, c_int, c_int]),
+    "orc_radix_sort_morton_pairs": (c_int, [c_void_p, c_void_p, c_uint64]),
+    "orc_lbvh_find_split": (c_int64, [c_void_p, c_int64, c_int64, c_int64]),
+    "orc_karras_topology": (c_uint32, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p]),
+    "orc_refit": (None, [c_uint32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "orc_build_lbvh": (
+        c_uint32,
+        [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
+    "orc_build_lbvh_topdown": (c_uint32, [c_void_p, c_void_p, c_void_p, c_uint64, c_int, c_void_p, c_void_p, c_void_p]),
+    "orc_canonical_karras": (c_uint64, [c_uint32, c_void_p, c_void_p, c_void_p]),
+    "orc_canonical_topdown": (c_uint64, [c_void_p, c_void_p, c_void_p]),
+    "orc_camera_make": (None, [c_void_p, c_void_p, c_void_p, c_float, c_float, c_void_p]),
+    "orc_camera_rays": (None, [c_void_p, c_uint64, c_uint32, c_uint32, c_void_p]),
+    "orc_trace_bvh2": (
+        None,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_uint64, c_int, c_void_p, c_void_p],
+    ),
+    "orc_trace_bvh2_mt": (
+        None,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_uint64, c_int, c_void_p, c_int],
+    ),
+    "orc_trace_brute": (None, [c_void_p, c_uint64, c_void_p, c_uint64, c_void_p]),
+    "orc_intersect_aabb": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_void_p]),
+    "orc_intersect_tri_eager": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_void_p]),
+    "orc_intersect_tri_scaled": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_void_p],
+    ),
+    "orc_rcp_direction": (None, [c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+def build() -> Path:
+    """Compiles liboracle.so with the Makefile beside this file."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        h = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIG.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _p(a: np.ndarray) -> c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(c_void_p)
+
+
+def f32(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+
+
+# ---- LBVH -----------------------------------------------------------------
+
+def triangle_bounds(vertices: np.ndarray) -> np.ndarray:
+    v = f32(vertices)
+    n = v.size // 9
+    out = np.empty(6 * n, np.float32)
+    lib().orc_triangle_bounds(_p(v), n, _p(out))
+    return out
+
+
+def scene_bounds(bounds6: np.ndarray) -> np.ndarray:
+    b = f32(bounds6)
+    out = np.empty(12, np.float32)
+    lib().orc_scene_bounds(_p(b), b.size // 6, _p(out))
+    return out
+
+
+def morton_codes(bounds6: np.ndarray, centroid_box6: np.ndarray) -> np.ndarray:
+    b = f32(bounds6)
+    cb = f32(centroid_box6)
+    out = np.empty(b.size // 6, np.uint32)
+    lib().orc_morton_codes(_p(b), b.size // 6, _p(cb), _p(out))
+    return out
+
+
+def radix_sort_pairs_u64(keys: np.ndarray, values: np.ndarray | None, begin_bit=0, end_bit=64):
+    k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+    v = None if values is None else np.ascontiguousarray(values, dtype=np.uint32).copy()
+    rc = lib().orc_radix_sort_pairs_u64(_p(k), _p(v) if v is not None else None, k.size, begin_bit, end_bit)
+    assert rc == 0
+    return k, v
+
+
+def build_lbvh(vertices: np.ndarray) -> dict:
+    """Single-segment LBVH in bajo's GPU binary layout (GpuBinaryBoundsBvh)."""
+    v = f32(vertices)
+    n = v.size // 9
+    m = max(n - 1, 0)
+    out = {
+        "sorted_keys": np.empty(n, np.uint64),
+        "leaf_ids": np.empty(n, np.uint32),
+        "node_meta": np.empty(max(4 * m, 1), np.uint32),
+        "leaf_parent": np.empty(n, np.uint32),
+        "leaf_counts": np.empty(max(m, 1), np.uint32),
+        "node_bounds": np.empty(max(12 * m, 1), np.float32),
+        "scene_bounds": np.empty(12, np.float32),
+    }
+    root = lib().orc_build_lbvh(
+        _p(v), n, _p(out["sorted_keys"]), _p(out["leaf_ids"]), _p(out["node_meta"]), _p(out["leaf_parent"]),
+        _p(out["leaf_counts"]), _p(out["node_bounds"]), _p(out["scene_bounds"]),
+    )
+    out["node_meta"] = out["node_meta"][: 4 * m]
+    out["leaf_counts"] = out["leaf_counts"][:m]
+    out["node_bounds"] = out["node_bounds"][: 12 * m]
+    out["root"] = int(root)
+    return out
+
+
+def karras_topology(sorted_codes: np.ndarray):
+    c = np.ascontiguousarray(sorted_codes, dtype=np.uint32)
+    n = c.size
+    m = max(n - 1, 1)
+    meta = np.empty(4 * m, np.uint32)
+    parent = np.empty(max(n, 1), np.uint32)
+    counts = np.empty(m, np.uint32)
+    root = lib().orc_karras_topology(_p(c), n, _p(meta), _p(parent), _p(counts))
+    return int(root), meta[: 4 * (n - 1)], parent[:n], counts[: n - 1]
+
+
+def build_lbvh_topdown(sorted_codes, sorted_ids, bounds6, leaf_size=1) -> dict:
+    c = np.ascontiguousarray(sorted_codes, dtype=np.uint32)
+    ids = np.ascontiguousarray(sorted_ids, dtype=np.uint32)
+    b = f32(bounds6)
+    n = c.size
+    cap = max(2 * n - 1, 1)
+    boxes = np.empty(6 * cap, np.float32)
+    first = np.empty(cap, np.uint32)
+    count = np.empty(cap, np.uint32)
+    used = lib().orc_build_lbvh_topdown(_p(c), _p(ids), _p(b), n, leaf_size, _p(boxes), _p(first), _p(count))
+    return {"boxes": boxes[: 6 * used], "first": first[:used], "count": count[:used], "node_count": int(used)}
+
+
+def canonical_karras(root: int, node_meta: np.ndarray, leaf_counts: np.ndarray) -> np.ndarray:
+    meta = np.ascontiguousarray(node_meta, dtype=np.uint32)
+    cnt = np.ascontiguousarray(leaf_counts, dtype=np.uint32)
+    m = meta.size // 4
+    out = np.empty(max(3 * m, 3), np.uint32)
+    k = lib().orc_canonical_karras(root, _p(meta) if meta.size else None, _p(cnt) if cnt.size else None, _p(out))
+    return out[: 3 * k].reshape(-1, 3)
+
+
+def canonical_topdown(tree: dict) -> np.ndarray:
+    n_nodes = tree["node_count"]
+    out = np.empty(max(3 * n_nodes, 3), np.uint32)
+    k = lib().orc_canonical_topdown(_p(tree["first"]), _p(tree["count"]), _p(out))
+    return out[: 3 * k].reshape(-1, 3)
+
+
+# ---- rays -------------------------------------------------------------------
+
+def camera(origin, target, up=(0.0, 1.0, 0.0), fov_scale=0.75, focus_dist=1.0) -> np.ndarray:
+    out = np.empty(20, np.float32)
+    lib().orc_camera_make(_p(f32(origin)), _p(f32(target)), _p(f32(up)), fov_scale, focus_dist, _p(out))
+    return out
+
+
+def camera_rays(cams: np.ndarray, ray_count: int, width: int, height: int) -> np.ndarray:
+    c = f32(cams)
+    out = np.empty(8 * ray_count, np.float32)
+    lib().orc_camera_rays(_p(c), ray_count, width, height, _p(out))
+    return out
+
+
+def trace_bvh2(vertices, lbvh: dict, rays: np.ndarray, mode: int = 0, stats: bool = False, threads: int = 1):
+    v = f32(vertices)
+    r = f32(rays)
+    R = r.size // 8
+    hits = np.empty(8 * R, np.float32)
+    meta = lbvh["node_meta"] if lbvh["node_meta"].size else np.zeros(4, np.uint32)
+    nb = lbvh["node_bounds"] if lbvh["node_bounds"].size else np.zeros(12, np.float32)
+    if threads > 1 and not stats:
+        lib().orc_trace_bvh2_mt(_p(v), _p(lbvh["leaf_ids"]), _p(meta), _p(nb), lbvh["root"], _p(r), R, mode, _p(hits), threads)
+        return hits
+    st = np.empty(5 * R, np.uint32) if stats else None
+    lib().orc_trace_bvh2(
+        _p(v), _p(lbvh["leaf_ids"]), _p(meta), _p(nb), lbvh["root"], _p(r), R, mode, _p(hits),
+        _p(st) if st is not None else None,
+    )
+    return (hits, st) if stats else hits
+
+
+def trace_brute(vertices, rays: np.ndarray) -> np.ndarray:
+    v = f32(vertices)
+    r = f32(rays)
+    R = r.size // 8
+    hits = np.empty(8 * R, np.float32)
+    lib().orc_trace_brute(_p(v), v.size // 9, _p(r), R, _p(hits))
+    return hits
+
+
+def hit_prims(hits: np.ndarray) -> np.ndarray:
+    return hits.reshape(-1, 8)[:, 2].view(np.uint32)
+
+
+def hit_t(hits: np.ndarray) -> np.ndarray:
+    return hits.reshape(-1, 8)[:, 7]

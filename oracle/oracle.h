@@ -1,0 +1,63 @@
+/*
+ * CPU oracle -- test infrastructure only.
+ *
+ * A plain-C restatement of bajo's CPU/GPU BVH algorithms (see the file
+ * headers of orc_lbvh.c, orc_trace.c and orc_render.c for the reference
+ * lines each function follows).  Loaded by the tests, the smoke check and
+ * the CPU-baseline leg of the benchmark; never by the product library.
+ */
+#ifndef BAJO_ORACLE_H
+#define BAJO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* core/morton.mojo */
+uint32_t orc_expand_bits(uint32_t n);
+uint32_t orc_morton(float x, float y, float z);
+int orc_common_prefix(uint32_t ca, uint32_t ia, uint32_t cb, uint32_t ib);
+
+/* LBVH build pieces */
+void orc_triangle_bounds(const float *v, uint64_t n, float *bounds6);
+void orc_scene_bounds(const float *bounds6, uint64_t n, float *out12);
+void orc_morton_codes(const float *bounds6, uint64_t n, const float *centroid_box6, uint32_t *codes);
+int orc_radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, uint64_t n, int begin_bit, int end_bit);
+int orc_radix_sort_morton_pairs(uint32_t *codes, uint32_t *ids, uint64_t n);
+int64_t orc_lbvh_find_split(const uint32_t *codes, int64_t n, int64_t first, int64_t last);
+uint32_t orc_karras_topology(const uint32_t *sorted_codes, uint64_t n, uint32_t *node_meta, uint32_t *leaf_parent,
+                             uint32_t *leaf_counts);
+void orc_refit(uint32_t root, const float *leaf_bounds, const uint32_t *leaf_ids, const uint32_t *node_meta,
+               float *node_bounds);
+uint32_t orc_build_lbvh(const float *vertices, uint64_t n, uint64_t *sorted_keys, uint32_t *leaf_ids,
+                        uint32_t *node_meta, uint32_t *leaf_parent, uint32_t *leaf_counts, float *node_bounds,
+                        float *scene12);
+uint32_t orc_build_lbvh_topdown(const uint32_t *sorted_codes, const uint32_t *sorted_ids, const float *bounds6,
+                                uint64_t n, int leaf_size, float *boxes6, uint32_t *first, uint32_t *count);
+uint64_t orc_canonical_karras(uint32_t root, const uint32_t *node_meta, const uint32_t *leaf_counts, uint32_t *out);
+uint64_t orc_canonical_topdown(const uint32_t *first, const uint32_t *count, uint32_t *out);
+
+/* ray queries */
+void orc_camera_make(const float *origin3, const float *target3, const float *up3, float fov_scale,
+                     float focus_dist, float *out20);
+void orc_camera_rays(const float *cams, uint64_t R, uint32_t w, uint32_t h, float *rays);
+void orc_trace_bvh2(const float *vertices, const uint32_t *leaf_ids, const uint32_t *node_meta,
+                    const float *node_bounds, uint32_t root, const float *rays, uint64_t R, int mode, float *hits,
+                    uint32_t *stats);
+void orc_trace_bvh2_mt(const float *vertices, const uint32_t *leaf_ids, const uint32_t *node_meta,
+                       const float *node_bounds, uint32_t root, const float *rays, uint64_t R, int mode,
+                       float *hits, int threads);
+void orc_trace_brute(const float *vertices, uint64_t n, const float *rays, uint64_t R, float *hits);
+int orc_intersect_aabb(const float *o3, const float *rcp3, const float *box6, float t_max, float *t_out);
+int orc_intersect_tri_eager(const float *o3, const float *d3, const float *v0, const float *e1, const float *e2,
+                            float t_max, float t_min, float *tuv3);
+int orc_intersect_tri_scaled(const float *o3, const float *d3, const float *v0, const float *e1, const float *e2,
+                             float t_max, float t_min, float *dtuv4);
+void orc_rcp_direction(const float *d3, float *out3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

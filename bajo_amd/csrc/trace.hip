@@ -183,7 +183,9 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const BjNode* __rest
     } else if (root != kSentinel) {
       uint32_t cur = root;
       for (;;) {
-        if (cur >= node_count) { atomicOr(status, (uint32_t)BAJO_STATUS_BAD_NODE); break; }
+        // A valid hierarchy visits each node at most once per ray; the cap only
+        // ends a ray early (with a status word) if the hierarchy is corrupted.
+        if (cur >= node_count || n_visit > node_count) { atomicOr(status, (uint32_t)BAJO_STATUS_BAD_NODE); break; }
         n_visit++;
         const float4* np = reinterpret_cast<const float4*>(nodes + cur);
         float4 q0 = np[0], q1 = np[1], q2 = np[2];

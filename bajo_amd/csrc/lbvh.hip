@@ -226,17 +226,35 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
     const uint32_t left = node->left, right = node->right;  // written by the topology launch
     const bool is_left = cur == left;
     if (!is_left && cur != right) break;
-    float* dst = node->b + (is_left ? 0 : 6);
-    dst[0] = lo.x; dst[1] = lo.y; dst[2] = lo.z;
-    dst[3] = hi.x; dst[4] = hi.y; dst[5] = hi.z;
-    // Release our half, acquire the sibling's: the second arriver merges.
-    uint32_t old = __hip_atomic_fetch_add(flags + parent, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    // Publish this child's box with write-through (agent-scope) stores and
+    // wait until they are performed before the arrival counter is bumped.
+    // This is the release half of the hand-off without an L2 writeback: the
+    // shared data itself never sits in a non-coherent cache line.
+    float* mine = node->b + (is_left ? 0 : 6);
+    const float vals[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) __hip_atomic_store(mine + k, vals[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const uint32_t old = __hip_atomic_fetch_add(flags + parent, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) break;
-    const float* bb = node->b;
-    v3 llo = mk(bb[0], bb[1], bb[2]), lhi = mk(bb[3], bb[4], bb[5]);
-    v3 rlo = mk(bb[6], bb[7], bb[8]), rhi = mk(bb[9], bb[10], bb[11]);
-    lo = vmin(llo, rlo);
-    hi = vmax(lhi, rhi);
+    // Second arriver: the sibling's stores completed before its increment,
+    // which returned before ours; coherent loads read them.
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const float* other = node->b + (is_left ? 6 : 0);
+    float o[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[k] = __hip_atomic_load(other + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v3 olo = mk(o[0], o[1], o[2]), ohi = mk(o[3], o[4], o[5]);
+    // _load_and_union_node_bounds (binary_layout.mojo:95-101): merge(left, right).
+    if (is_left) {
+      lo = vmin(lo, olo);
+      hi = vmax(hi, ohi);
+    } else {
+      lo = vmin(olo, lo);
+      hi = vmax(ohi, hi);
+    }
     cur = parent;
     parent = node->parent;
   }

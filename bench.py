@@ -80,6 +80,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument(
+        "--copy-calibration-mib",
+        type=int,
+        default=0,
+        help="after the run, copy a buffer of this many MiB once (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -200,6 +206,14 @@ def main() -> int:
         line["cpu_baseline"] = cpu_baseline(vertices, rays, min(args.cpu_sample, R), threads)
     print(json.dumps(line), flush=True)
     bvh.free()
+    if args.copy_calibration_mib:
+        nbytes = args.copy_calibration_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy of {nbytes} bytes done", file=sys.stderr, flush=True)
     return 0
 
 

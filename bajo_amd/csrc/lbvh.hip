@@ -34,6 +34,14 @@ constexpr int kSortPasses = (kSortEndBit - kSortBeginBit) / 8;
 
 BJ_D float ld(const float* p) { return __builtin_nontemporal_load(p); }
 
+BJ_D uint64_t pack2(float a, float b) {
+  return (uint64_t)__builtin_bit_cast(uint32_t, a) | ((uint64_t)__builtin_bit_cast(uint32_t, b) << 32);
+}
+BJ_D void unpack2(uint64_t w, float& a, float& b) {
+  a = __builtin_bit_cast(float, (uint32_t)w);
+  b = __builtin_bit_cast(float, (uint32_t)(w >> 32));
+}
+
 BJ_D void load_tri(const float* __restrict__ v, uint64_t i, v3& a, v3& b, v3& c) {
   const float* t = v + 9 * i;
   a = mk(ld(t + 0), ld(t + 1), ld(t + 2));
@@ -230,10 +238,12 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
     // wait until they are performed before the arrival counter is bumped.
     // This is the release half of the hand-off without an L2 writeback: the
     // shared data itself never sits in a non-coherent cache line.
-    float* mine = node->b + (is_left ? 0 : 6);
-    const float vals[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
-#pragma unroll
-    for (int k = 0; k < 6; ++k) __hip_atomic_store(mine + k, vals[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Boxes move as three 8-byte words (child halves start at byte 0 and 24).
+    uint64_t* mine = reinterpret_cast<uint64_t*>(node->b + (is_left ? 0 : 6));
+    const uint64_t w0 = pack2(lo.x, lo.y), w1 = pack2(lo.z, hi.x), w2 = pack2(hi.y, hi.z);
+    __hip_atomic_store(mine + 0, w0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(mine + 1, w1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(mine + 2, w2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -242,10 +252,11 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
     // Second arriver: the sibling's stores completed before its increment,
     // which returned before ours; coherent loads read them.
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const float* other = node->b + (is_left ? 6 : 0);
+    const uint64_t* other = reinterpret_cast<const uint64_t*>(node->b + (is_left ? 6 : 0));
     float o[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) o[k] = __hip_atomic_load(other + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unpack2(__hip_atomic_load(other + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), o[0], o[1]);
+    unpack2(__hip_atomic_load(other + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), o[2], o[3]);
+    unpack2(__hip_atomic_load(other + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), o[4], o[5]);
     v3 olo = mk(o[0], o[1], o[2]), ohi = mk(o[3], o[4], o[5]);
     // _load_and_union_node_bounds (binary_layout.mojo:95-101): merge(left, right).
     if (is_left) {

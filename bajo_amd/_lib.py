@@ -55,6 +55,38 @@ class BvhInfo(ctypes.Structure):
     ]
 
 
+class RenderSettings(ctypes.Structure):
+    _fields_ = [
+        ("image_width", c_uint32),
+        ("image_height", c_uint32),
+        ("samples_per_pixel", c_uint32),
+        ("max_depth", c_uint32),
+        ("rng_seed", c_uint64),
+        ("algorithm", c_int32),
+        ("path_capacity", c_uint32),
+    ]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [
+        ("extension_rays", c_uint64),
+        ("shadow_rays", c_uint64),
+        ("incoherent_rays", c_uint64),
+        ("rays_per_bounce", c_uint64 * 16),
+        ("status", c_uint32),
+        ("traversal_status", c_uint32),
+        ("extension_node_visits", c_uint64),
+        ("extension_leaf_tests", c_uint64),
+        ("shadow_node_visits", c_uint64),
+        ("shadow_leaf_tests", c_uint64),
+        ("total_ms", c_float),
+        ("extend_ms", c_float),
+        ("shadow_ms", c_float),
+        ("extend_launches", c_uint32),
+        ("shadow_launches", c_uint32),
+    ]
+
+
 # name -> (restype, argtypes); the exported surface of include/bajo_hip.h
 SIGNATURES = {
     "bajo_ctx_create": (c_int32, [c_int32, c_void_p, POINTER(c_void_p)]),
@@ -95,6 +127,19 @@ SIGNATURES = {
         c_int32,
         [POINTER(c_float), POINTER(c_float), POINTER(c_float), c_float, c_float, POINTER(c_float)],
     ),
+    "bajo_scene_create": (
+        c_int32,
+        [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, POINTER(c_void_p)],
+    ),
+    "bajo_scene_rebuild": (c_int32, [c_void_p, c_void_p, POINTER(BuildTimings)]),
+    "bajo_scene_free": (c_int32, [c_void_p]),
+    "bajo_scene_get_bvh": (c_int32, [c_void_p, POINTER(c_void_p)]),
+    "bajo_scene_light_table": (c_int32, [c_void_p, c_void_p, c_void_p, POINTER(c_uint32), POINTER(c_float)]),
+    "bajo_render": (
+        c_int32,
+        [c_void_p, c_void_p, POINTER(RenderSettings), POINTER(c_float), c_void_p, POINTER(RenderStats)],
+    ),
+    "bajo_rng_stage_floats": (c_int32, [c_uint64, c_uint32, c_uint32, c_uint32, c_void_p]),
     "bajo_scene_soup": (c_int32, [c_uint64, c_uint64, c_uint64, c_float, c_float, c_void_p]),
     "bajo_scene_icosphere": (c_int32, [c_uint32, c_uint64, c_uint64, c_float, c_float, c_void_p]),
 }

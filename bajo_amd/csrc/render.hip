@@ -1,0 +1,821 @@
+// Wavefront path tracer for triangle scenes on gfx950.
+//
+// Replaces bajo's GPU wavefront (render_gpu / enqueue_render_gpu_triangles,
+// rt/gpu/render.mojo:322-420, rt/gpu/triangle_path.mojo:143-235, chunk
+// scheduler rt/gpu/resources.mojo:187-285) for Lambertian + emissive
+// triangle scenes and RENDER.PATH / NORMALS / AO / NEE / MIS:
+//   primary  rt/gpu/common_kernels.mojo:22-94
+//   extend   rt/gpu/scene_trace.mojo:41-412 (closest hit, sky, emission,
+//            NEE candidate rt/gpu/path_shading.mojo:261-403, inline
+//            Lambertian + roulette :463-535, routing :538-648)
+//   shadow   rt/gpu/scene_trace.mojo:517-624
+//   advance  rt/gpu/wavefront_contract.mojo:441-457 (no host sync)
+//   resolve  rt/gpu/common_kernels.mojo:97-136
+// Queues keep bajo's field-major planes (rt/wavefront_contract.mojo:41-99).
+//
+// MI355X design: the extend and shadow kernels are persistent waves that
+// fetch 64-path batches; queue appends are wave-aggregated (one atomic per
+// wave, slots ranked by ballot) and happen after the per-path work has
+// reconverged; HBM is large enough that a whole 1920x1080x8 frame is one
+// chunk, so a frame costs max_depth x (extend, shadow, advance) launches.
+#include <string.h>
+
+#include <vector>
+
+#include "rt_math.h"
+#include "traverse.h"
+
+namespace {
+using namespace bj;
+
+constexpr int kRtBlock = kTraceBlock;
+constexpr uint32_t kMatLambertian = 0, kMatMetal = 1, kMatDielectric = 2, kMatEmissive = 3;
+constexpr uint32_t kSurfaceIndexMask = (1u << 28) - 1;
+constexpr uint32_t kPrimTriangle = 1;
+constexpr uint32_t kPathDelta = 0x80000000u, kPathIdMask = 0x7FFFFFFFu;
+constexpr int kLightStride = 14;
+constexpr int kPathPlanes = 12, kShadowPlanes = 11, kSamplePlanes = 3;
+// counter words (bajo's WAVE_COUNTER plus two fetch cursors)
+constexpr int kActive = 0, kNext = 1, kShade = 2, kShadow = 3, kStatus = 4, kFetchExt = 5, kFetchShadow = 6;
+constexpr int kCounterWords = 16;
+constexpr uint32_t kStatusPathOverflow = 1, kStatusShadowOverflow = 3;
+constexpr int kMaxBounces = 64;
+// stats buffer: 2 x kMaxBounces per-bounce ray counts, then traversal work
+// totals (extension node visits, extension leaf tests, shadow node visits,
+// shadow leaf tests)
+constexpr int kWorkBase = 2 * kMaxBounces;
+constexpr int kStatsWords = kWorkBase + 4;
+
+struct SceneView {
+  BvhView bvh;
+  const uint32_t* surfaces;
+  const float* lambertian;
+  const float* emissive;
+  const float* light_fields;
+  const uint32_t* light_kinds;
+  uint32_t light_count;
+  float total_light_weight;
+};
+
+struct Cam {
+  float v[20];
+};
+
+struct Queues {
+  const uint32_t* src_ids;
+  const float* src_fields;
+  uint32_t* dst_ids;
+  float* dst_fields;
+  uint32_t* shadow_ids;
+  float* shadow_fields;
+  float* sample;
+  uint32_t* counters;
+  uint64_t capacity;
+  uint32_t sample_base;
+};
+
+BJ_D void accumulate(float* sample, uint64_t cap, uint32_t slot, v3 c) {
+  sample[0 * cap + slot] += c.x;
+  sample[1 * cap + slot] += c.y;
+  sample[2 * cap + slot] += c.z;
+}
+
+// rt/lighting.mojo:7-20.
+BJ_D float power_heuristic(float a, float b) {
+  float a2 = a * a, b2 = b * b, den = a2 + b2;
+  return den > 0.0f ? a2 / den : 0.0f;
+}
+
+// ---- primary (rt/gpu/common_kernels.mojo:22-94, bvh/camera.mojo:156-208) ----
+__global__ void __launch_bounds__(kRtBlock) primary_kernel(Cam cam, uint32_t* __restrict__ ids,
+                                                           float* __restrict__ fields, float* __restrict__ sample,
+                                                           uint64_t cap, uint32_t active, uint32_t sample_base,
+                                                           uint32_t width, uint32_t height, uint32_t spp,
+                                                           uint64_t seed) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= active) return;
+  const uint32_t path_id = sample_base + (uint32_t)idx;
+  const uint32_t pixel = path_id / spp;
+  const uint32_t px = pixel % width, py = pixel / width;
+  StageRng rng(seed, path_id, 0u);
+  // random_in_unit_disk (core/random.mojo:56-61), then the pixel jitter
+  float s, c;
+  sincos_turns(rng.next(), s, c);
+  const float rr = sqrtf(rng.next());
+  const float lens_u = rr * c, lens_v = rr * s;
+  const float pu = rng.next(), pv = rng.next();
+  const float* k = cam.v;
+  const float fw = (float)width, fh = (float)height;
+  const float aspect = fw / fh;
+  const float sx = (((float)px + pu) / fw) * 2.0f - 1.0f;
+  const float sy = 1.0f - (((float)py + pv) / fh) * 2.0f;
+  const v3 origin = mk(k[0], k[1], k[2]);
+  const v3 fwd = mk(k[3], k[4], k[5]), right = mk(k[6], k[7], k[8]), up = mk(k[9], k[10], k[11]);
+  const float fov = k[12], focus = k[13];
+  const v3 plane = add(add(fwd, scale(right, sx * aspect * fov)), scale(up, sy * fov));
+  const v3 focal = add(origin, scale(plane, focus));
+  const v3 ro = add(add(origin, scale(mk(k[14], k[15], k[16]), lens_u)), scale(mk(k[17], k[18], k[19]), lens_v));
+  const v3 d = normalize(sub(focal, ro));
+  ids[idx] = path_id | kPathDelta;
+  const float f[kPathPlanes] = {ro.x, ro.y, ro.z, kT_MinSecondary, d.x, d.y, d.z, kFltMax, 1.0f, 1.0f, 1.0f, 0.0f};
+#pragma unroll
+  for (int p = 0; p < kPathPlanes; ++p) fields[p * cap + idx] = f[p];
+#pragma unroll
+  for (int p = 0; p < kSamplePlanes; ++p) sample[p * cap + idx] = 0.0f;
+}
+
+// Resets the queue counters for a chunk; the status words (kStatus and the
+// traversal status at word 8) and the per-bounce ray counts are cleared
+// only for the first chunk of a frame.
+__global__ void init_counters_kernel(uint32_t* counters, uint32_t active, unsigned long long* stats, int first) {
+  const uint32_t t = threadIdx.x;
+  if (t < kCounterWords) {
+    const bool status_word = t == kStatus || t == 8;
+    if (!status_word || first) counters[t] = t == kActive ? active : 0u;
+  }
+  if (first)
+    for (uint32_t i = t; i < kStatsWords; i += blockDim.x) stats[i] = 0;
+}
+
+__global__ void advance_kernel(uint32_t* counters, unsigned long long* stats, uint32_t bounce, uint32_t cap) {
+  if (threadIdx.x != 0) return;
+  if (bounce < kMaxBounces) {
+    stats[2 * bounce + 0] += counters[kActive];
+    stats[2 * bounce + 1] += counters[kShadow];
+  }
+  // appends past the arena were dropped (and flagged); never read past it
+  const uint32_t next = counters[kNext];
+  counters[kActive] = next < cap ? next : cap;
+  counters[kNext] = 0;
+  counters[kShade] = 0;
+  counters[kShadow] = 0;
+  counters[kFetchExt] = 0;
+  counters[kFetchShadow] = 0;
+}
+
+struct NextPath {
+  uint32_t id;
+  float f[kPathPlanes];
+};
+struct ShadowWork {
+  uint32_t id;
+  float f[kShadowPlanes];
+};
+
+// Adds per-lane traversal work to two global totals, one atomic per wave.
+BJ_D void wave_add_work(unsigned long long* dst, uint64_t a, uint64_t b) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  if (lane_id() == 0) {
+    atomicAdd(dst + 0, (unsigned long long)a);
+    atomicAdd(dst + 1, (unsigned long long)b);
+  }
+}
+
+// Wave-aggregated append: one atomic per wave, slots in lane order.
+BJ_D uint32_t wave_reserve(uint32_t* counter, bool want) {
+  const uint64_t m = ballot(want);
+  if (m == 0) return 0xFFFFFFFFu;
+  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__builtin_popcountll(m));
+  base = __shfl(base, (int)leader, 64);
+  return want ? base + mask_rank(m) : 0xFFFFFFFFu;
+}
+
+// rt/gpu/path_shading.mojo:261-403 (triangle lights only).
+template <int kAlg>
+BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uint32_t bounce, v3 point, v3 normal,
+                       v3 albedo, v3 throughput, ShadowWork& sw) {
+  if (sc.light_count == 0 || !(sc.total_light_weight > 0.0f)) return false;
+  StageRng rng(seed, path_id, rng_light_stage(bounce));
+  const float alias_sample = rng.next() * (float)sc.light_count;
+  uint32_t sel = (uint32_t)alias_sample;
+  if (sel >= sc.light_count) sel = sc.light_count - 1;
+  const float frac = alias_sample - (float)sel;
+  const uint32_t packed = sc.light_kinds[sel];
+  if (frac > sc.light_fields[sel * kLightStride + 13]) sel = packed >> 4;
+  const float* L = sc.light_fields + (uint64_t)sel * kLightStride;
+  const v3 p0 = mk(L[0], L[1], L[2]), p1 = mk(L[3], L[4], L[5]), p2 = mk(L[6], L[7], L[8]);
+  const v3 e1 = sub(p1, p0), e2 = sub(p2, p0);
+  const v3 av = cross(e1, e2);
+  const float tas = length2(av);
+  if (tas <= 0.0f) return false;
+  const float sq = sqrtf(tas);
+  const v3 ln = mk(av.x / sq, av.y / sq, av.z / sq);
+  const float root_u = sqrtf(rng.next());
+  const float b1 = root_u * (1.0f - rng.next());
+  const float b2 = root_u - b1;
+  const v3 lp = add(add(p0, scale(e1, b1)), scale(e2, b2));
+  const v3 to_light = sub(lp, point);
+  const float d2 = length2(to_light);
+  if (d2 <= 1.0e-8f) return false;
+  const float dist = sqrtf(d2);
+  const v3 dir = mk(to_light.x / dist, to_light.y / dist, to_light.z / dist);
+  const float scos = fmax_sel(dot(normal, dir), 0.0f);
+  const float lcos = fmax_sel(dot(ln, mk(-dir.x, -dir.y, -dir.z)), 0.0f);
+  if (scos <= 0.0f || lcos <= 0.0f) return false;
+  const float tmax = dist - 0.002f;
+  if (tmax <= 0.001f) return false;
+  const v3 E = mk(L[10], L[11], L[12]);
+  const float light_pdf = d2 * (E.x + E.y + E.z) / (3.0f * lcos * sc.total_light_weight);
+  // _evaluate_lambertian (rt/shading.mojo:13-27)
+  const v3 value = scale(albedo, kInvPi);
+  const float bsdf_pdf = fmax_sel(dot(normal, normalize(dir)), 0.0f) * kInvPi;
+  // _direct_light_scale (rt/lighting.mojo:23-38)
+  const bool ok = light_pdf > 0.0f && bsdf_pdf > 0.0f;
+  float weight = 1.0f;
+  if constexpr (kAlg == BAJO_RENDER_MIS) weight = power_heuristic(light_pdf, bsdf_pdf);
+  const float sc_ = ok ? scos * weight / light_pdf : 0.0f;
+  if (!(sc_ > 0.0f)) return false;
+  const v3 contrib = scale(mul(mul(throughput, value), E), sc_);
+  const float f[kShadowPlanes] = {point.x, point.y, point.z, 0.001f, dir.x, dir.y, dir.z, tmax,
+                                  contrib.x, contrib.y, contrib.z};
+  for (int p = 0; p < kShadowPlanes; ++p) sw.f[p] = f[p];
+  sw.id = path_id;
+  return true;
+}
+
+template <int kAlg>
+__global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
+                                                          uint32_t* __restrict__ spill,
+                                                          unsigned long long* __restrict__ work) {
+  __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
+  const uint32_t lane = lane_id();
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
+  const uint64_t cap = q.capacity;
+  const uint32_t active = q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap;
+  uint64_t work_visits = 0, work_leaves = 0;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(q.counters + kFetchExt, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= active) break;
+    const uint32_t idx = base + lane;
+    bool want_next = false, want_shadow = false;
+    NextPath np;
+    ShadowWork sw;
+    if (idx < active) {
+      const uint32_t idw = q.src_ids[idx];
+      const uint32_t path_id = idw & kPathIdMask;
+      const bool delta = (idw & kPathDelta) != 0;
+      float f[kPathPlanes];
+#pragma unroll
+      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[p * cap + idx];
+      const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
+      const v3 T = mk(f[8], f[9], f[10]);
+      const float path_pdf = f[11];
+      const uint32_t slot = path_id - q.sample_base;
+      TraceCounters tc;
+      const Hit h = traverse<BAJO_TRACE_CLOSEST_HIT, true>(sc.bvh, ray, st, &tc);
+      work_visits += tc.visits;
+      work_leaves += tc.leaves;
+      const bool found = h.prim != kEmpty && h.t < ray.tmax;
+      if (!found) {
+        if constexpr (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS)
+          accumulate(q.sample, cap, slot, mul(T, sky_color(ray.d)));
+      } else {
+        // rt/geometry.mojo:20-30: orient the outward normal against the ray
+        const bool front = dot(ray.d, h.n) < 0.0f;
+        const v3 n = front ? h.n : mk(-h.n.x, -h.n.y, -h.n.z);
+        const v3 point = add(ray.o, scale(ray.d, h.t));
+        const uint32_t surf = sc.surfaces[h.prim];
+        const uint32_t kind = surf >> 28, mi = surf & kSurfaceIndexMask;
+        if constexpr (kAlg == BAJO_RENDER_NORMALS) {
+          accumulate(q.sample, cap, slot, scale(add(n, mk(1.0f, 1.0f, 1.0f)), 0.5f));
+        } else if constexpr (kAlg == BAJO_RENDER_AO) {
+          // rt/gpu/path_shading.mojo:651-667: hemisphere ray, t_max 4
+          StageRng rng(seed, path_id, 1u);
+          float s, c;
+          sincos_turns(rng.next(), s, c);
+          const float z = 1.0f - 2.0f * rng.next();
+          const float r = sqrtf(1.0f - z * z);
+          v3 dir = mk(r * c, r * s, z);
+          if (dot(dir, n) < 0.0f) dir = mk(-dir.x, -dir.y, -dir.z);
+          const float fs[kShadowPlanes] = {point.x, point.y, point.z, 0.001f, dir.x, dir.y, dir.z, 4.0f, 0, 0, 0};
+          for (int p = 0; p < kShadowPlanes; ++p) sw.f[p] = fs[p];
+          sw.id = path_id;
+          want_shadow = true;
+        } else {
+          v3 albedo = mk(0, 0, 0);
+          if (kind == kMatLambertian)
+            albedo = mk(sc.lambertian[3 * mi + 0], sc.lambertian[3 * mi + 1], sc.lambertian[3 * mi + 2]);
+          if constexpr (kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
+            if (kind == kMatLambertian) want_shadow = direct_light<kAlg>(sc, seed, path_id, bounce, point, n, albedo, T, sw);
+          }
+          if (kind == kMatEmissive) {
+            if (front) {
+              const v3 E = mk(sc.emissive[3 * mi + 0], sc.emissive[3 * mi + 1], sc.emissive[3 * mi + 2]);
+              float w = 1.0f;
+              if constexpr (kAlg == BAJO_RENDER_NEE) {
+                if (bounce > 0 && !delta) w = 0.0f;
+              } else if constexpr (kAlg == BAJO_RENDER_MIS) {
+                if (bounce > 0 && !delta) {
+                  const float lc = fmax_sel(dot(n, normalize(mk(-ray.d.x, -ray.d.y, -ray.d.z))), 0.0f);
+                  float lpdf = 0.0f;
+                  if (lc > 0.0f && sc.total_light_weight > 0.0f) {
+                    const float d2 = h.t * h.t * length2(ray.d);
+                    lpdf = d2 * (E.x + E.y + E.z) / (3.0f * lc * sc.total_light_weight);
+                  }
+                  w = power_heuristic(path_pdf, lpdf);
+                }
+              }
+              accumulate(q.sample, cap, slot, scale(mul(T, E), w));
+            }
+          } else if (kind == kMatLambertian) {
+            // _sample_lambertian (rt/shading.mojo:92-118)
+            StageRng rng(seed, path_id, rng_bsdf_stage(bounce));
+            const float ru = rng.next(), rv = rng.next();
+            float s, c;
+            sincos_turns(ru, s, c);
+            const float z = 1.0f - 2.0f * rv;
+            const float radius = sqrtf(fmax_sel(1.0f - z * z, 0.0f));
+            v3 sd = add(n, mk(radius * c, radius * s, z));
+            if (fabsf(sd.x) < 1e-8f && fabsf(sd.y) < 1e-8f && fabsf(sd.z) < 1e-8f) sd = n;
+            const v3 dir = normalize(sd);
+            const float pdf = fmax_sel(dot(n, normalize(dir)), 0.0f) * kInvPi;
+            v3 Tn = mul(T, albedo);
+            // russian_roulette (rt/common.mojo:43-67), depth = bounce + 1
+            bool alive = true;
+            if (bounce + 1u >= 5u) {
+              const float mx = fmax_sel(Tn.x, fmax_sel(Tn.y, Tn.z));
+              if (mx <= 0.0f) {
+                alive = false;
+              } else {
+                const float survival = fmin_sel(fmax_sel(mx, 0.05f), 0.95f);
+                StageRng rr(seed, path_id, rng_roulette_stage(bounce));
+                if (rr.next() >= survival) alive = false;
+                else Tn = mk(Tn.x / survival, Tn.y / survival, Tn.z / survival);
+              }
+            }
+            if (alive) {
+              np.id = path_id;
+              const float nf[kPathPlanes] = {ray.o.x + h.t * ray.d.x, ray.o.y + h.t * ray.d.y, ray.o.z + h.t * ray.d.z,
+                                             kT_MinSecondary, dir.x, dir.y, dir.z, kFltMax, Tn.x, Tn.y, Tn.z, pdf};
+              for (int p = 0; p < kPathPlanes; ++p) np.f[p] = nf[p];
+              want_next = true;
+            }
+          }
+          // metal and dielectric surfaces are not part of this renderer's scenes: absorbed
+        }
+      }
+    }
+    const uint32_t ns = wave_reserve(q.counters + kNext, want_next);
+    if (want_next) {
+      if (ns < cap) {
+        q.dst_ids[ns] = np.id;
+#pragma unroll
+        for (int p = 0; p < kPathPlanes; ++p) q.dst_fields[p * cap + ns] = np.f[p];
+      } else {
+        atomicMax(q.counters + kStatus, kStatusPathOverflow);
+      }
+    }
+    const uint32_t ss = wave_reserve(q.counters + kShadow, want_shadow);
+    if (want_shadow) {
+      if (ss < cap) {
+        q.shadow_ids[ss] = sw.id;
+#pragma unroll
+        for (int p = 0; p < kShadowPlanes; ++p) q.shadow_fields[p * cap + ss] = sw.f[p];
+      } else {
+        atomicMax(q.counters + kStatus, kStatusShadowOverflow);
+      }
+    }
+  }
+  wave_add_work(work, work_visits, work_leaves);
+}
+
+template <int kAlg>
+__global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t* __restrict__ spill,
+                                                          unsigned long long* __restrict__ work) {
+  __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
+  const uint32_t lane = lane_id();
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
+  const uint64_t cap = q.capacity;
+  const uint32_t count = q.counters[kShadow] < cap ? q.counters[kShadow] : (uint32_t)cap;
+  uint64_t work_visits = 0, work_leaves = 0;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(q.counters + kFetchShadow, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= count) break;
+    const uint32_t idx = base + lane;
+    if (idx >= count) continue;
+    float f[kShadowPlanes];
+#pragma unroll
+    for (int p = 0; p < kShadowPlanes; ++p) f[p] = q.shadow_fields[p * cap + idx];
+    const uint32_t slot = q.shadow_ids[idx] - q.sample_base;
+    const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
+    TraceCounters tc;
+    const Hit h = traverse<BAJO_TRACE_ANY_HIT, true>(sc.bvh, ray, st, &tc);
+    work_visits += tc.visits;
+    work_leaves += tc.leaves;
+    const bool occluded = h.prim == kEmpty && h.t == 0.0f;
+    if constexpr (kAlg == BAJO_RENDER_AO) {
+      const float v = occluded ? 0.08f : 1.0f;
+      accumulate(q.sample, cap, slot, mk(v, v, v));
+    } else {
+      if (!occluded) accumulate(q.sample, cap, slot, mk(f[8], f[9], f[10]));
+    }
+  }
+  wave_add_work(work, work_visits, work_leaves);
+}
+
+__global__ void __launch_bounds__(kRtBlock) resolve_kernel(const float* __restrict__ sample, float* __restrict__ pixels,
+                                                           uint64_t cap, uint32_t pixel_base, uint32_t pixel_count,
+                                                           uint32_t spp) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pixel_count) return;
+  float r = 0.0f, g = 0.0f, b = 0.0f;
+  const uint64_t s0 = (uint64_t)p * spp;
+  for (uint64_t s = s0; s < s0 + spp; ++s) {
+    r += sample[0 * cap + s];
+    g += sample[1 * cap + s];
+    b += sample[2 * cap + s];
+  }
+  const float k = 1.0f / (float)spp;
+  const uint64_t o = 3ull * (pixel_base + p);
+  pixels[o + 0] = r * k;
+  pixels[o + 1] = g * k;
+  pixels[o + 2] = b * k;
+}
+
+}  // namespace
+
+// ---- scene + arena ---------------------------------------------------------
+struct bajo_scene {
+  int device = 0;
+  uint64_t n = 0;
+  float* vertices = nullptr;
+  bajo_bvh* bvh = nullptr;
+  uint32_t* surfaces = nullptr;
+  float* lambertian = nullptr;
+  float* emissive = nullptr;
+  float* light_fields = nullptr;
+  uint32_t* light_kinds = nullptr;
+  uint32_t light_count = 0;
+  float total_light_weight = 0.0f;
+  // reusable render arena
+  uint64_t cap = 0;
+  uint32_t* ids[2] = {nullptr, nullptr};
+  float* fields[2] = {nullptr, nullptr};
+  uint32_t* shadow_ids = nullptr;
+  float* shadow_fields = nullptr;
+  float* sample = nullptr;
+  uint32_t* counters = nullptr;
+  unsigned long long* stats = nullptr;
+  uint32_t* spill = nullptr;
+  uint64_t spill_threads = 0;
+};
+
+namespace {
+
+void free_arena(bajo_scene* s) {
+  hipFree(s->ids[0]); hipFree(s->ids[1]); hipFree(s->fields[0]); hipFree(s->fields[1]);
+  hipFree(s->shadow_ids); hipFree(s->shadow_fields); hipFree(s->sample); hipFree(s->counters); hipFree(s->stats);
+  hipFree(s->spill);
+  s->ids[0] = s->ids[1] = nullptr;
+  s->fields[0] = s->fields[1] = nullptr;
+  s->shadow_ids = nullptr; s->shadow_fields = nullptr; s->sample = nullptr; s->counters = nullptr;
+  s->stats = nullptr; s->spill = nullptr;
+  s->cap = 0;
+  s->spill_threads = 0;
+}
+
+void free_scene(bajo_scene* s) {
+  if (!s) return;
+  free_arena(s);
+  bajo_bvh_free(s->bvh);
+  hipFree(s->vertices); hipFree(s->surfaces); hipFree(s->lambertian); hipFree(s->emissive);
+  hipFree(s->light_fields); hipFree(s->light_kinds);
+  delete s;
+}
+
+template <typename K>
+int32_t grid_of(K kernel, unsigned* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  BJ_HIP(hipGetDevice(&dev));
+  BJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kRtBlock, 0));
+  *grid = (unsigned)(cus * (per_cu < 1 ? 1 : per_cu));
+  return BAJO_OK;
+}
+
+int32_t ensure_arena(bajo_scene* s, uint64_t cap, uint64_t spill_threads) {
+  if (cap <= s->cap && spill_threads <= s->spill_threads) return BAJO_OK;
+  free_arena(s);
+  for (int k = 0; k < 2; ++k) {
+    BJ_HIP(hipMalloc(&s->ids[k], cap * sizeof(uint32_t)));
+    BJ_HIP(hipMalloc(&s->fields[k], cap * kPathPlanes * sizeof(float)));
+  }
+  BJ_HIP(hipMalloc(&s->shadow_ids, cap * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&s->shadow_fields, cap * kShadowPlanes * sizeof(float)));
+  BJ_HIP(hipMalloc(&s->sample, cap * kSamplePlanes * sizeof(float)));
+  BJ_HIP(hipMalloc(&s->counters, kCounterWords * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&s->stats, kStatsWords * sizeof(unsigned long long)));
+  BJ_HIP(hipMalloc(&s->spill, spill_threads * kSpill * sizeof(uint32_t)));
+  s->cap = cap;
+  s->spill_threads = spill_threads;
+  return BAJO_OK;
+}
+
+// LightStore.build_alias_table (rt/types.mojo:207-247): Walker-Vose with
+// LIFO small/large work lists, in FP32.
+void build_alias(const std::vector<float>& w, float total, std::vector<float>& prob, std::vector<uint32_t>& alias) {
+  const size_t n = w.size();
+  prob.assign(n, 1.0f);
+  alias.assign(n, 0u);
+  if (n == 0 || !(total > 0.0f)) return;
+  std::vector<float> scaled(n);
+  std::vector<size_t> small, large;
+  for (size_t i = 0; i < n; ++i) {
+    alias[i] = (uint32_t)i;
+    scaled[i] = w[i] * (float)n / total;
+    if (scaled[i] < 1.0f) small.push_back(i);
+    else large.push_back(i);
+  }
+  while (!small.empty() && !large.empty()) {
+    size_t si = small.back(); small.pop_back();
+    size_t li = large.back(); large.pop_back();
+    prob[si] = scaled[si];
+    alias[si] = (uint32_t)li;
+    scaled[li] = scaled[li] + scaled[si] - 1.0f;
+    if (scaled[li] < 1.0f) small.push_back(li);
+    else large.push_back(li);
+  }
+  while (!small.empty()) { size_t i = small.back(); small.pop_back(); prob[i] = 1.0f; alias[i] = (uint32_t)i; }
+  while (!large.empty()) { size_t i = large.back(); large.pop_back(); prob[i] = 1.0f; alias[i] = (uint32_t)i; }
+}
+
+struct LaunchEvents {
+  std::vector<hipEvent_t> ext, sh;  // start/end pairs
+  hipEvent_t begin = nullptr, end = nullptr;
+  bool on = false;
+  hipEvent_t make() {
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+  }
+  ~LaunchEvents() {
+    for (auto e : ext) hipEventDestroy(e);
+    for (auto e : sh) hipEventDestroy(e);
+    if (begin) hipEventDestroy(begin);
+    if (end) hipEventDestroy(end);
+  }
+};
+
+template <int kAlg>
+int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st, const float* camera, float* pixels,
+                   LaunchEvents* ev) {
+  const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
+  uint64_t cap = st->path_capacity ? st->path_capacity : total;
+  if (cap > total) cap = total;
+  cap = (cap / st->samples_per_pixel) * st->samples_per_pixel;
+  if (cap == 0) cap = st->samples_per_pixel;
+  unsigned g_ext = 0, g_sh = 0;
+  int32_t rc = grid_of(extend_kernel<kAlg>, &g_ext);
+  if (rc) return rc;
+  rc = grid_of(shadow_kernel<kAlg>, &g_sh);
+  if (rc) return rc;
+  const unsigned need = div_up(cap, kRtBlock);
+  if (g_ext > need) g_ext = need;
+  if (g_sh > need) g_sh = need;
+  if (g_ext < 1) g_ext = 1;
+  if (g_sh < 1) g_sh = 1;
+  const uint64_t spill_threads = (uint64_t)(g_ext > g_sh ? g_ext : g_sh) * kRtBlock;
+  rc = ensure_arena(s, cap, spill_threads);
+  if (rc) return rc;
+  hipStream_t stream = ctx->stream;
+  Cam cam;
+  memcpy(cam.v, camera, sizeof(cam.v));
+  bajo_bvh_info info;
+  bajo_bvh_get_info(s->bvh, &info);
+  SceneView sv;
+  sv.bvh = BvhView{(const BjNode*)info.nodes, (const BjTri*)info.triangles, info.root, (uint32_t)info.node_count,
+                   (uint32_t)info.prim_count, s->counters + 8};
+  sv.surfaces = s->surfaces;
+  sv.lambertian = s->lambertian;
+  sv.emissive = s->emissive;
+  sv.light_fields = s->light_fields;
+  sv.light_kinds = s->light_kinds;
+  sv.light_count = s->light_count;
+  sv.total_light_weight = s->total_light_weight;
+  const uint32_t depth = (kAlg == BAJO_RENDER_NORMALS || kAlg == BAJO_RENDER_AO) ? 1u : st->max_depth;
+  for (uint64_t begin = 0; begin < total; begin += cap) {
+    const uint32_t chunk = (uint32_t)((total - begin) < cap ? (total - begin) : cap);
+    init_counters_kernel<<<1, 64, 0, stream>>>(s->counters, chunk, s->stats, begin == 0);
+    primary_kernel<<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(
+        cam, s->ids[0], s->fields[0], s->sample, cap, chunk, (uint32_t)begin, st->image_width, st->image_height,
+        st->samples_per_pixel, st->rng_seed);
+    BJ_HIP(hipGetLastError());
+    for (uint32_t b = 0; b < depth; ++b) {
+      Queues q{s->ids[b & 1], s->fields[b & 1], s->ids[(b + 1) & 1], s->fields[(b + 1) & 1], s->shadow_ids,
+               s->shadow_fields, s->sample, s->counters, cap, (uint32_t)begin};
+      if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
+      extend_kernel<kAlg><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, s->spill, s->stats + kWorkBase);
+      BJ_HIP(hipGetLastError());
+      if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
+      if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
+        if (ev) { ev->sh.push_back(ev->make()); hipEventRecord(ev->sh.back(), stream); }
+        shadow_kernel<kAlg><<<g_sh, kRtBlock, 0, stream>>>(sv, q, s->spill, s->stats + kWorkBase + 2);
+        BJ_HIP(hipGetLastError());
+        if (ev) { ev->sh.push_back(ev->make()); hipEventRecord(ev->sh.back(), stream); }
+      }
+      advance_kernel<<<1, 64, 0, stream>>>(s->counters, s->stats, b, (uint32_t)cap);
+    }
+    const uint32_t pixel_count = chunk / st->samples_per_pixel;
+    resolve_kernel<<<div_up(pixel_count, kRtBlock), kRtBlock, 0, stream>>>(
+        s->sample, pixels, cap, (uint32_t)(begin / st->samples_per_pixel), pixel_count, st->samples_per_pixel);
+    BJ_HIP(hipGetLastError());
+  }
+  return BAJO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bajo_scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
+                          const float* lambertian, uint32_t n_lambertian, const float* emissive, uint32_t n_emissive,
+                          bajo_scene** out) {
+  BJ_REQUIRE(ctx && out, "bajo_scene_create: NULL ctx/out");
+  BJ_REQUIRE(n > 0 && vertices && surfaces, "bajo_scene_create: a scene needs triangles and their surfaces");
+  BJ_REQUIRE(n < (1ull << 31), "bajo_scene_create: too many triangles");
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t kind = surfaces[i] >> 28, mi = surfaces[i] & kSurfaceIndexMask;
+    BJ_REQUIRE(kind == kMatLambertian || kind == kMatEmissive,
+               "bajo_scene_create: triangle %llu has material kind %u (only Lambertian and emissive surfaces)",
+               (unsigned long long)i, kind);
+    BJ_REQUIRE(kind != kMatLambertian || mi < n_lambertian, "bajo_scene_create: Lambertian index out of range");
+    BJ_REQUIRE(kind != kMatEmissive || mi < n_emissive, "bajo_scene_create: emissive index out of range");
+  }
+  bajo_scene* s = new bajo_scene();
+  s->device = ctx->device;
+  s->n = n;
+  auto fail = [&](int32_t rc) { free_scene(s); return rc; };
+#define SC_HIP(call)                                              \
+  do {                                                            \
+    hipError_t e_ = (call);                                       \
+    if (e_ != hipSuccess) return fail(hip_fail(e_, #call, __FILE__, __LINE__)); \
+  } while (0)
+  SC_HIP(hipMalloc(&s->vertices, n * 9 * sizeof(float)));
+  SC_HIP(hipMemcpy(s->vertices, vertices, n * 9 * sizeof(float), hipMemcpyHostToDevice));
+  SC_HIP(hipMalloc(&s->surfaces, n * sizeof(uint32_t)));
+  SC_HIP(hipMemcpy(s->surfaces, surfaces, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  SC_HIP(hipMalloc(&s->lambertian, (n_lambertian ? n_lambertian : 1) * 3 * sizeof(float)));
+  if (n_lambertian) SC_HIP(hipMemcpy(s->lambertian, lambertian, n_lambertian * 3 * sizeof(float), hipMemcpyHostToDevice));
+  SC_HIP(hipMalloc(&s->emissive, (n_emissive ? n_emissive : 1) * 3 * sizeof(float)));
+  if (n_emissive) SC_HIP(hipMemcpy(s->emissive, emissive, n_emissive * 3 * sizeof(float), hipMemcpyHostToDevice));
+  // _build_light_store (rt/types.mojo:490-506): one light per emissive
+  // triangle, weight = area * mean radiance.
+  std::vector<float> lf, weights;
+  std::vector<uint32_t> tri_of_light;
+  float total = 0.0f;
+  for (uint64_t i = 0; i < n; ++i) {
+    if ((surfaces[i] >> 28) != kMatEmissive) continue;
+    const float* t = vertices + 9 * i;
+    const v3 p0 = mk(t[0], t[1], t[2]), p1 = mk(t[3], t[4], t[5]), p2 = mk(t[6], t[7], t[8]);
+    const float area = 0.5f * sqrtf(length2(cross(sub(p1, p0), sub(p2, p0))));
+    const float* e = emissive + 3 * (surfaces[i] & kSurfaceIndexMask);
+    const float imp = fmax_sel((e[0] + e[1] + e[2]) / 3.0f, 0.0f);
+    const float w = area * imp;
+    if (w > 0.0f) {
+      weights.push_back(w);
+      tri_of_light.push_back((uint32_t)i);
+      total += w;
+    }
+  }
+  std::vector<float> prob;
+  std::vector<uint32_t> alias;
+  build_alias(weights, total, prob, alias);
+  const uint32_t nl = (uint32_t)weights.size();
+  std::vector<uint32_t> kinds(nl ? nl : 1, 0u);
+  lf.assign((nl ? nl : 1) * kLightStride, 0.0f);
+  for (uint32_t l = 0; l < nl; ++l) {
+    const float* t = vertices + 9 * (uint64_t)tri_of_light[l];
+    const float* e = emissive + 3 * (surfaces[tri_of_light[l]] & kSurfaceIndexMask);
+    float* o = lf.data() + (size_t)l * kLightStride;
+    for (int k = 0; k < 9; ++k) o[k] = t[k];
+    o[9] = 0.0f;
+    o[10] = e[0]; o[11] = e[1]; o[12] = e[2];
+    o[13] = prob[l];
+    kinds[l] = (alias[l] << 4) | kPrimTriangle;
+  }
+  s->light_count = nl;
+  s->total_light_weight = total;
+  SC_HIP(hipMalloc(&s->light_fields, lf.size() * sizeof(float)));
+  SC_HIP(hipMemcpy(s->light_fields, lf.data(), lf.size() * sizeof(float), hipMemcpyHostToDevice));
+  SC_HIP(hipMalloc(&s->light_kinds, kinds.size() * sizeof(uint32_t)));
+  SC_HIP(hipMemcpy(s->light_kinds, kinds.data(), kinds.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+#undef SC_HIP
+  int32_t rc = bajo_build_triangle_bvh(ctx, s->vertices, n, BAJO_BUILD_LBVH, nullptr, &s->bvh);
+  if (rc) return fail(rc);
+  *out = s;
+  return BAJO_OK;
+}
+
+int32_t bajo_scene_rebuild(bajo_ctx* ctx, bajo_scene* s, bajo_build_timings* timings) {
+  BJ_REQUIRE(ctx && s, "bajo_scene_rebuild: NULL argument");
+  return bajo_build_triangle_bvh(ctx, s->vertices, s->n, BAJO_BUILD_LBVH, timings, &s->bvh);
+}
+
+int32_t bajo_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float* out) {
+  BJ_REQUIRE(out || count == 0, "bajo_rng_stage_floats: NULL output");
+  StageRng rng(seed, path_id, stage);
+  for (uint32_t i = 0; i < count; ++i) out[i] = rng.next();
+  return BAJO_OK;
+}
+
+int32_t bajo_scene_free(bajo_scene* s) {
+  free_scene(s);
+  return BAJO_OK;
+}
+
+int32_t bajo_scene_get_bvh(const bajo_scene* s, bajo_bvh** out) {
+  BJ_REQUIRE(s && out, "bajo_scene_get_bvh: NULL argument");
+  *out = s->bvh;
+  return BAJO_OK;
+}
+
+int32_t bajo_scene_light_table(const bajo_scene* s, float* fields, uint32_t* kinds, uint32_t* count,
+                               float* total_weight) {
+  BJ_REQUIRE(s && count && total_weight, "bajo_scene_light_table: NULL argument");
+  *count = s->light_count;
+  *total_weight = s->total_light_weight;
+  if (s->light_count && fields)
+    BJ_HIP(hipMemcpy(fields, s->light_fields, (size_t)s->light_count * kLightStride * sizeof(float), hipMemcpyDeviceToHost));
+  if (s->light_count && kinds)
+    BJ_HIP(hipMemcpy(kinds, s->light_kinds, (size_t)s->light_count * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return BAJO_OK;
+}
+
+int32_t bajo_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st, const float* camera, float* pixels,
+                    bajo_render_stats* stats) {
+  BJ_REQUIRE(ctx && s && st && camera && pixels, "bajo_render: NULL argument");
+  BJ_REQUIRE(st->image_width > 0 && st->image_height > 0 && st->samples_per_pixel > 0,
+             "bajo_render: image size and samples per pixel must be positive");
+  BJ_REQUIRE(st->max_depth <= kMaxBounces, "bajo_render: max_depth above %d", kMaxBounces);
+  const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
+  BJ_REQUIRE(total < (1ull << 31), "bajo_render: %llu paths exceed the 31-bit path id", (unsigned long long)total);
+  int32_t rc;
+  LaunchEvents events;
+  LaunchEvents* ev = stats ? &events : nullptr;
+  if (ev) {
+    ev->begin = ev->make();
+    ev->end = ev->make();
+    BJ_HIP(hipEventRecord(ev->begin, ctx->stream));
+  }
+  switch (st->algorithm) {
+    case BAJO_RENDER_PATH: rc = run_render<BAJO_RENDER_PATH>(ctx, s, st, camera, pixels, ev); break;
+    case BAJO_RENDER_NORMALS: rc = run_render<BAJO_RENDER_NORMALS>(ctx, s, st, camera, pixels, ev); break;
+    case BAJO_RENDER_AO: rc = run_render<BAJO_RENDER_AO>(ctx, s, st, camera, pixels, ev); break;
+    case BAJO_RENDER_NEE: rc = run_render<BAJO_RENDER_NEE>(ctx, s, st, camera, pixels, ev); break;
+    case BAJO_RENDER_MIS: rc = run_render<BAJO_RENDER_MIS>(ctx, s, st, camera, pixels, ev); break;
+    default: BJ_REQUIRE(false, "bajo_render: unknown algorithm %d", st->algorithm);
+  }
+  if (rc) return rc;
+  if (stats) {
+    BJ_HIP(hipEventRecord(ev->end, ctx->stream));
+    memset(stats, 0, sizeof(*stats));
+    unsigned long long host[kStatsWords];
+    uint32_t counters[kCounterWords];
+    BJ_HIP(hipMemcpyAsync(host, s->stats, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+    BJ_HIP(hipMemcpyAsync(counters, s->counters, sizeof(counters), hipMemcpyDeviceToHost, ctx->stream));
+    BJ_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    BJ_HIP(hipEventElapsedTime(&ms, ev->begin, ev->end));
+    stats->total_ms = ms;
+    for (size_t i = 0; i + 1 < ev->ext.size(); i += 2) {
+      BJ_HIP(hipEventElapsedTime(&ms, ev->ext[i], ev->ext[i + 1]));
+      stats->extend_ms += ms;
+    }
+    for (size_t i = 0; i + 1 < ev->sh.size(); i += 2) {
+      BJ_HIP(hipEventElapsedTime(&ms, ev->sh[i], ev->sh[i + 1]));
+      stats->shadow_ms += ms;
+    }
+    stats->extend_launches = (uint32_t)(ev->ext.size() / 2);
+    stats->shadow_launches = (uint32_t)(ev->sh.size() / 2);
+    stats->extension_node_visits = host[kWorkBase + 0];
+    stats->extension_leaf_tests = host[kWorkBase + 1];
+    stats->shadow_node_visits = host[kWorkBase + 2];
+    stats->shadow_leaf_tests = host[kWorkBase + 3];
+    for (int b = 0; b < kMaxBounces; ++b) {
+      stats->extension_rays += host[2 * b];
+      stats->shadow_rays += host[2 * b + 1];
+      if (b >= 1) stats->incoherent_rays += host[2 * b];
+      stats->incoherent_rays += b >= 1 ? host[2 * b + 1] : 0;
+      if (b < 16) stats->rays_per_bounce[b] = host[2 * b];
+    }
+    stats->status = counters[kStatus];
+    stats->traversal_status = counters[8];
+    if (stats->status) return (int32_t)(BAJO_STATUS_WAVE_BASE + stats->status);
+    if (stats->traversal_status) return (int32_t)stats->traversal_status;
+  }
+  return BAJO_OK;
+}
+
+}  // extern "C"

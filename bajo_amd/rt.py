@@ -1,0 +1,207 @@
+"""Wavefront path tracing with bajo's ``bajo/rt`` interface.
+
+Mirrors ``render_gpu[ALGORITHM](settings, camera, world) -> RenderResult``
+(rt/gpu/render.mojo:322-420), ``RenderSettings`` (rt/types.mojo:324-357),
+the ``RENDER`` / ``MAT`` enums (rt/types.mojo:24-57) and ``SceneData`` for
+triangle scenes with Lambertian and emissive surfaces (rt/types.mojo:382-526).
+The renderer runs entirely in libbajo_hip.so (bajo_amd/csrc/render.hip).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .context import DeviceContext, default_context, ptr, require_device
+
+
+class RENDER:
+    PATH = 0
+    NORMALS = 1
+    AO = 2
+    NEE = 3
+    MIS = 4
+
+
+class MAT:
+    LAMBERTIAN = 0
+    METAL = 1
+    DIELECTRIC = 2
+    EMISSIVE = 3
+
+
+SURFACE_KIND_SHIFT = 28
+
+
+def surface_id(kind: int, index: int) -> int:
+    """SurfaceId packing: kind in the top 4 bits, material index below."""
+    return (int(kind) << SURFACE_KIND_SHIFT) | int(index)
+
+
+@dataclass
+class RenderSettings:
+    image_width: int
+    image_height: int
+    samples_per_pixel: int
+    rng_seed: int
+    max_depth: int = 8
+    path_capacity: int = 0  # paths in flight per chunk; 0 = the whole frame
+
+    def as_c(self, algorithm: int) -> _lib.RenderSettings:
+        return _lib.RenderSettings(
+            self.image_width,
+            self.image_height,
+            self.samples_per_pixel,
+            self.max_depth,
+            self.rng_seed,
+            algorithm,
+            self.path_capacity,
+        )
+
+
+@dataclass
+class SceneData:
+    """Host description of a triangle scene."""
+
+    vertices: np.ndarray  # f32[9N]
+    surfaces: np.ndarray  # u32[N], surface_id(kind, index)
+    lambertian: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[3L] albedo
+    emissive: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[3E] radiance
+
+    @property
+    def triangle_count(self) -> int:
+        return self.vertices.size // 9
+
+
+class GpuScene:
+    """Device-resident scene: vertices, BVH, materials and the light table."""
+
+    def __init__(self, scene: SceneData, ctx: DeviceContext | None = None):
+        self.ctx = ctx or default_context(0)
+        v = np.ascontiguousarray(scene.vertices, dtype=np.float32)
+        s = np.ascontiguousarray(scene.surfaces, dtype=np.uint32)
+        lamb = np.ascontiguousarray(scene.lambertian, dtype=np.float32).reshape(-1)
+        emit = np.ascontiguousarray(scene.emissive, dtype=np.float32).reshape(-1)
+        if s.size != v.size // 9:
+            raise ValueError("one surface id per triangle is required")
+        handle = ctypes.c_void_p()
+        _lib.check(
+            _lib.lib().bajo_scene_create(
+                self.ctx.handle,
+                v.ctypes.data_as(ctypes.c_void_p),
+                v.size // 9,
+                s.ctypes.data_as(ctypes.c_void_p),
+                lamb.ctypes.data_as(ctypes.c_void_p),
+                lamb.size // 3,
+                emit.ctypes.data_as(ctypes.c_void_p),
+                emit.size // 3,
+                ctypes.byref(handle),
+            )
+        )
+        self._handle = handle
+        self.triangle_count = v.size // 9
+
+    @property
+    def handle(self):
+        if not self._handle:
+            raise RuntimeError("scene already freed")
+        return self._handle
+
+    def rebuild(self, timings: bool = False):
+        t = _lib.BuildTimings() if timings else None
+        _lib.check(_lib.lib().bajo_scene_rebuild(self.ctx.handle, self.handle, ctypes.byref(t) if t else None))
+        return t.as_dict() if t else None
+
+    def light_table(self):
+        count = ctypes.c_uint32()
+        total = ctypes.c_float()
+        _lib.check(_lib.lib().bajo_scene_light_table(self.handle, None, None, ctypes.byref(count), ctypes.byref(total)))
+        fields = np.empty(14 * max(count.value, 1), np.float32)
+        kinds = np.empty(max(count.value, 1), np.uint32)
+        _lib.check(
+            _lib.lib().bajo_scene_light_table(
+                self.handle, fields.ctypes.data_as(ctypes.c_void_p), kinds.ctypes.data_as(ctypes.c_void_p),
+                ctypes.byref(count), ctypes.byref(total),
+            )
+        )
+        return fields[: 14 * count.value], kinds[: count.value], total.value
+
+    def free(self):
+        if getattr(self, "_handle", None):
+            _lib.lib().bajo_scene_free(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class RenderResult:
+    pixels: torch.Tensor  # f32[H, W, 3] on the device
+    stats: dict | None = None
+
+    def numpy(self) -> np.ndarray:
+        torch.cuda.synchronize(self.pixels.device)
+        return self.pixels.cpu().numpy()
+
+
+def enqueue_render_gpu(
+    settings: RenderSettings, camera: np.ndarray, scene: GpuScene, algorithm: int = RENDER.PATH,
+    pixels: torch.Tensor | None = None, with_stats: bool = False,
+) -> RenderResult:
+    """Asynchronous render (rt/gpu/render.mojo:61-235): enqueues and returns."""
+    ctx = scene.ctx
+    w, h = settings.image_width, settings.image_height
+    if pixels is None:
+        pixels = torch.empty((h, w, 3), dtype=torch.float32, device=ctx.torch_device)
+    require_device(pixels, ctx, "pixels")
+    if pixels.numel() < 3 * w * h:
+        raise ValueError("pixel buffer too small")
+    cam = (ctypes.c_float * 20)(*[float(x) for x in np.asarray(camera, np.float32).reshape(-1)[:20]])
+    st = settings.as_c(algorithm)
+    stats = _lib.RenderStats() if with_stats else None
+    _lib.check(
+        _lib.lib().bajo_render(ctx.handle, scene.handle, ctypes.byref(st), cam, ptr(pixels), ctypes.byref(stats) if stats else None)
+    )
+    out = None
+    if stats is not None:
+        out = {name: getattr(stats, name) for name, _ in _lib.RenderStats._fields_}
+        out["rays_per_bounce"] = list(stats.rays_per_bounce)
+    return RenderResult(pixels, out)
+
+
+def render_gpu(settings: RenderSettings, camera: np.ndarray, world, algorithm: int = RENDER.PATH, ctx=None) -> RenderResult:
+    """render_gpu[ALGORITHM](settings, camera, world) (rt/gpu/render.mojo:322-420)."""
+    scene = world if isinstance(world, GpuScene) else GpuScene(world, ctx)
+    return enqueue_render_gpu(settings, camera, scene, algorithm, with_stats=True)
+
+
+def rng_stage_floats(seed: int, path_id: int, stage: int, count: int) -> np.ndarray:
+    """Host view of one path stage's random stream (the same numbers the kernels draw)."""
+    out = np.empty(count, np.float32)
+    _lib.check(_lib.lib().bajo_rng_stage_floats(seed, path_id, stage, count, out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+# rt/wavefront_contract.mojo:107-134
+def wavefront_rng_subsequence(path_id: int, stage: int) -> int:
+    return (int(path_id) << 32) | int(stage)
+
+
+def wavefront_rng_stage(bounce: int) -> int:
+    return (0 << 30) | (bounce + 1)
+
+
+def wavefront_rng_roulette_stage(bounce: int) -> int:
+    return (1 << 30) | (bounce + 1)
+
+
+def wavefront_rng_light_stage(bounce: int) -> int:
+    return (2 << 30) | (bounce + 1)

@@ -38,3 +38,65 @@ def icosphere(n: int, frequency: int = 458, noise_seed: int = 2028, amplitude: f
 def bounds(vertices: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     v = vertices.reshape(-1, 3)
     return v.min(axis=0), v.max(axis=0)
+
+
+# ---- render scenes ---------------------------------------------------------------
+
+def quad(p0, p1, p2, p3) -> np.ndarray:
+    """Two triangles (p0, p1, p2), (p0, p2, p3); normal = (p1-p0) x (p2-p0)."""
+    return np.array([*p0, *p1, *p2, *p0, *p2, *p3], np.float32)
+
+
+def _scene(tri_blocks, kinds, lambertian, emissive):
+    from .rt import MAT, SceneData, surface_id
+
+    vertices = np.concatenate([np.asarray(b, np.float32).reshape(-1) for b in tri_blocks])
+    surfaces = np.concatenate(kinds).astype(np.uint32)
+    assert surfaces.size * 9 == vertices.size
+    return SceneData(vertices, surfaces, np.asarray(lambertian, np.float32).reshape(-1), np.asarray(emissive, np.float32).reshape(-1))
+
+
+def icosphere_scene(n_sphere: int = 4_194_304, frequency: int = 458, albedo_seed: int = 2029, emission: float = 15.0):
+    """Displaced icosphere (Lambertian, albedo per triangle U[0.2, 0.8]^3) lit
+    by one downward-facing quad light of the given emission above it."""
+    from .rt import MAT, surface_id
+
+    sphere = icosphere(n_sphere, frequency=frequency)
+    albedo = np.random.default_rng(albedo_seed).uniform(0.2, 0.8, (n_sphere, 3)).astype(np.float32)
+    light = quad((-0.5, 1.6, -0.5), (0.5, 1.6, -0.5), (0.5, 1.6, 0.5), (-0.5, 1.6, 0.5))
+    kinds = [
+        (np.uint32(MAT.LAMBERTIAN) << 28) | np.arange(n_sphere, dtype=np.uint32),
+        np.full(2, surface_id(MAT.EMISSIVE, 0), np.uint32),
+    ]
+    return _scene([sphere, light], kinds, albedo, [emission, emission, emission])
+
+
+def box_scene(n_soup: int = 120, seed: int = 5, emission: float = 12.0):
+    """Small closed-ish box (floor, back wall, two side walls) with a soup of
+    small triangles inside and a quad light under the ceiling; for parity and
+    radiance tests."""
+    from .rt import MAT, surface_id
+
+    floor = quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))
+    back = quad((-1, 0, 1), (-1, 2, 1), (1, 2, 1), (1, 0, 1))
+    left = quad((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1))
+    right = quad((1, 0, 1), (1, 2, 1), (1, 2, -1), (1, 0, -1))
+    light = quad((-0.3, 1.9, -0.3), (0.3, 1.9, -0.3), (0.3, 1.9, 0.3), (-0.3, 1.9, 0.3))
+    rng = np.random.default_rng(seed)
+    v0 = rng.uniform([-0.8, 0.05, -0.6], [0.8, 1.4, 0.8], (n_soup, 3))
+    e = rng.uniform(-0.25, 0.25, (n_soup, 2, 3))
+    soup_t = np.concatenate([v0, v0 + e[:, 0], v0 + e[:, 1]], axis=1).astype(np.float32)
+    walls = np.concatenate([floor, back, left, right])
+    n_walls = walls.size // 9
+    lamb = np.concatenate([
+        np.array([[0.7, 0.7, 0.7], [0.7, 0.7, 0.7], [0.6, 0.2, 0.2], [0.2, 0.6, 0.2]], np.float32),
+        rng.uniform(0.2, 0.8, (n_soup, 3)).astype(np.float32),
+    ])
+    wall_mat = np.repeat(np.arange(4, dtype=np.uint32), 2)
+    kinds = [
+        (np.uint32(MAT.LAMBERTIAN) << 28) | wall_mat,
+        (np.uint32(MAT.LAMBERTIAN) << 28) | (4 + np.arange(n_soup, dtype=np.uint32)),
+        np.full(2, surface_id(MAT.EMISSIVE, 0), np.uint32),
+    ]
+    assert n_walls == 8
+    return _scene([walls, soup_t, light], kinds, lamb, [emission, emission, emission])

@@ -2,15 +2,17 @@
 """Benchmark of the MI355X BVH / ray-tracing hot path.
 
 Workload (one GPU): a displaced geodesic icosphere of 4,194,304 FP32
-triangles (frequency 458, radial value-noise amplitude 0.05).  One step =
-LBVH build (bounds, Morton, onesweep sort, Karras, refit) followed by one
-batch of 1920 x 1080 x 8 = 16,588,800 incoherent closest-hit rays (origins
-uniform in the scene box, directions uniform on the sphere).  Inputs are
-resident in HBM before timing starts.
+triangles (frequency 458, radial value-noise amplitude 0.05, Lambertian
+albedo per triangle U[0.2, 0.8]^3) under one quad light of emission 15.  One
+step = LBVH build (bounds, Morton, onesweep sort, Karras, refit) followed by
+the wavefront path tracer at 1920 x 1080, 8 spp, max depth 5, run with NEE
+and with PATH.  Scene data is resident in HBM before timing starts.
 
-Prints one JSON line with whole-step MRays/s as `value`, the build Mtris/s
-and sort GB/s beside it, the roofline of the dominant kernel (traversal,
-measured live with HIP events) and the CPU oracle timed on the host.
+Prints one JSON line: `value` = rays traced per second over the whole step
+(build included), with the incoherent-ray rate (bounce >= 1 extension and
+shadow rays over the render kernels), the build Mtris/s, the roofline of the
+dominant kernel (the NEE extend kernel, timed live with HIP events) and the
+CPU oracle's rate on a bounded sample of the same render.
 """
 
 from __future__ import annotations
@@ -28,57 +30,58 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "MRays/s incoherent @4M tris; build Mtris/s; equal-time MSE guided vs plain"
-N_TRIS = 4_194_304
-WIDTH, HEIGHT, SPP = 1920, 1080, 8
+N_SPHERE = 4_194_304
+WIDTH, HEIGHT, SPP, DEPTH, SEED = 1920, 1080, 8, 5, 2026
+FOV_SCALE = 0.41421356  # tan(22.5 deg)
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 64  # both child boxes + children, one node record
+NODE_BYTES = 64  # one binary node: both child boxes + children
 TRI_BYTES = 48  # v0, e1, e2, prim
-RAY_IO_BYTES = 64  # 32 B packed ray in + 32 B hit out
-BUILD_BYTES_PER_TRI = 452  # minimal binary-LBVH traffic per triangle
-SORT_BYTES_PER_KEY_PASS = 16  # 8 B read + 8 B write of a 64-bit key
+PATH_BYTES = 52  # 12 float planes + id
+SHADOW_BYTES = 48  # 11 float planes + id
+HIT_MATERIAL_BYTES = 16  # surface id + albedo
+SORT_BYTES_PER_KEY_PASS = 16
 
 
-def incoherent_rays(n: int, lo: np.ndarray, hi: np.ndarray, seed: int) -> np.ndarray:
-    rng = np.random.default_rng(seed)
-    rays = np.empty((8, n), np.float32)
-    for k in range(3):
-        rays[k] = lo[k] + (hi[k] - lo[k]) * rng.random(n, dtype=np.float32)
-    d = rng.standard_normal((3, n), dtype=np.float32)
-    d /= np.linalg.norm(d, axis=0, keepdims=True)
-    rays[3] = 1e-4
-    rays[4:7] = d
-    rays[7] = np.finfo(np.float32).max
-    return rays.reshape(-1)
+def extend_bytes(st: dict) -> float:
+    """Algorithmic bytes moved by the extend launches of one render."""
+    ext = st["extension_rays"]
+    continued = ext - st["rays_per_bounce"][0]  # paths written to the next queue
+    return (
+        ext * (PATH_BYTES + HIT_MATERIAL_BYTES)
+        + st["extension_node_visits"] * NODE_BYTES
+        + st["extension_leaf_tests"] * TRI_BYTES
+        + continued * PATH_BYTES
+        + st["shadow_rays"] * SHADOW_BYTES
+    )
 
 
-def cpu_baseline(vertices: np.ndarray, rays: np.ndarray, sample: int, threads: int) -> dict:
+def cpu_baseline(scene, cam, threads: int) -> dict:
     import oracle
 
+    w, h = WIDTH // 4, HEIGHT // 4
     t0 = time.perf_counter()
-    lbvh = oracle.build_lbvh(vertices)
+    oracle.build_lbvh(scene.vertices)
     build_s = time.perf_counter() - t0
-    sub = rays.reshape(8, -1)[:, :sample].reshape(-1)
     t0 = time.perf_counter()
-    oracle.trace_bvh2(vertices, lbvh, sub, mode=0, threads=threads)
-    trace_s = time.perf_counter() - t0
+    _, rays = oracle.render(scene.vertices, scene.surfaces, scene.lambertian, scene.emissive, w, h, SPP, DEPTH, SEED, 3, cam, threads)
+    dt = time.perf_counter() - t0 - build_s  # orc_render builds its own LBVH first
+    total = int(rays[0] + rays[1])
     return {
-        "value": sample / trace_s / 1e6,
+        "value": total / dt / 1e6,
         "unit": "MRays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample} of the step's incoherent rays traced by the C oracle BVH2 tracer on {threads} threads; "
-        f"oracle LBVH build of all {len(vertices) // 9} triangles on 1 thread took {build_s:.2f} s "
-        f"({len(vertices) // 9 / build_s / 1e6:.2f} Mtris/s)",
-        "build_mtris_s_1core": len(vertices) // 9 / build_s / 1e6,
+        "sample": f"C oracle NEE render of the same scene and camera at {w}x{h}x{SPP} spp, depth {DEPTH}: "
+        f"{total} rays in {dt:.2f} s on {threads} threads (its single-thread LBVH build, {build_s:.2f} s = "
+        f"{len(scene.vertices) // 9 / build_s / 1e6:.1f} Mtris/s, subtracted)",
     }
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument(
         "--copy-calibration-mib",
@@ -90,7 +93,7 @@ def main() -> int:
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: this build has no multi-GPU path (a replicated BVH with "
-            "ray ranges split per GPU is described in DESIGN.md). Run with --gpus 1.",
+            "pixel ranges split per GPU is described in DESIGN.md). Run with --gpus 1.",
             file=sys.stderr,
         )
         return 2
@@ -98,35 +101,34 @@ def main() -> int:
     import torch
 
     from bajo_amd import scenes
-    from bajo_amd.bvh import TRACE, build_triangle_bvh
+    from bajo_amd.bvh import Camera
     from bajo_amd.context import default_context
+    from bajo_amd.rt import RENDER, GpuScene, RenderSettings, enqueue_render_gpu
 
     ctx = default_context(0)
     dev = ctx.torch_device
     print(f"[bench] device {ctx.name()}", file=sys.stderr, flush=True)
-
-    vertices = scenes.icosphere(N_TRIS)
-    lo, hi = scenes.bounds(vertices)
-    R = WIDTH * HEIGHT * SPP
-    rays = incoherent_rays(R, lo, hi, seed=2026)
-    d_vertices = torch.from_numpy(vertices).to(dev)
-    d_rays = torch.from_numpy(rays).to(dev)
-    d_hits = torch.empty(8 * R, dtype=torch.float32, device=dev)
+    scene = scenes.icosphere_scene(N_SPHERE)
+    gscene = GpuScene(scene, ctx)
+    cam = Camera((0.0, 0.0, -3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), FOV_SCALE).flatten()
+    settings = RenderSettings(WIDTH, HEIGHT, SPP, SEED, DEPTH)
+    pixels = {a: torch.empty((HEIGHT, WIDTH, 3), dtype=torch.float32, device=dev) for a in (RENDER.NEE, RENDER.PATH)}
     torch.cuda.synchronize()
-    print("[bench] inputs resident", file=sys.stderr, flush=True)
-
-    bvh = build_triangle_bvh(d_vertices, ctx=ctx)
+    print(f"[bench] scene resident: {gscene.triangle_count} triangles", file=sys.stderr, flush=True)
     stream = torch.cuda.current_stream(dev)
 
     def step(ev=None):
         if ev is not None:
             ev[0].record(stream)
-        bvh.rebuild(d_vertices)
+        gscene.rebuild()
         if ev is not None:
             ev[1].record(stream)
-        bvh.launch_rays(d_rays, d_hits, R, TRACE.CLOSEST_HIT)
+        out = {}
+        for alg in (RENDER.NEE, RENDER.PATH):
+            out[alg] = enqueue_render_gpu(settings, cam, gscene, alg, pixels=pixels[alg], with_stats=True).stats
         if ev is not None:
             ev[2].record(stream)
+        return out
 
     for _ in range(args.warmup):
         step()
@@ -134,37 +136,34 @@ def main() -> int:
     print("[bench] warmup done", file=sys.stderr, flush=True)
 
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    stats = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        step(events[k])
+        stats.append(step(events[k]))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    build_ms = np.array([e[0].elapsed_time(e[1]) for e in events])
-    trace_ms = np.array([e[1].elapsed_time(e[2]) for e in events])
     ms_per_step = wall * 1e3 / args.steps
+    build_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
+    render_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
     print(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
 
-    # per-stage build breakdown (one extra measured build, outside the timed region)
-    stages = bvh.rebuild(d_vertices, timings=True)
-    # traversal counters from the instrumented kernel over the same rays
-    stats = torch.empty(5 * R, dtype=torch.int32, device=dev)
-    bvh.launch_rays(d_rays, d_hits, R, TRACE.CLOSEST_HIT, stats=stats)
-    st = stats.view(-1, 5).to(torch.float64)
-    visits = float(st[:, 0].mean())
-    leaf_tests = float(st[:, 3].mean())
-    max_stack = int(stats.view(-1, 5)[:, 4].max())
-    hit_frac = float((d_hits.view(torch.int32).view(-1, 8)[:, 2] != -1).float().mean())
-
-    bytes_per_ray = RAY_IO_BYTES + visits * NODE_BYTES + leaf_tests * TRI_BYTES
-    trace_avg_s = float(trace_ms.mean()) / 1e3
-    achieved = bytes_per_ray * R / trace_avg_s / 1e9
-    build_s = float(build_ms.mean()) / 1e3
-    sort_s = stages["sort_ns"] / 1e9
+    last = stats[-1]
+    nee, path = last[RENDER.NEE], last[RENDER.PATH]
+    rays_per_step = sum(s["extension_rays"] + s["shadow_rays"] for s in last.values())
+    incoherent = sum(s["incoherent_rays"] for s in last.values())
+    kernel_ms = sum(float(np.mean([st[a]["extend_ms"] + st[a]["shadow_ms"] for st in stats])) for a in last)
+    ext_ms_nee = float(np.mean([st[RENDER.NEE]["extend_ms"] for st in stats]))
+    ext_launches = nee["extend_launches"]
+    ext_bytes = extend_bytes(nee)
+    achieved = ext_bytes / (ext_ms_nee / 1e3) / 1e9
+    stages = gscene.rebuild(timings=True)
+    img = pixels[RENDER.NEE].float().mean(dim=(0, 1)).tolist()
+    keep = ("extension_rays", "shadow_rays", "incoherent_rays", "extend_ms", "shadow_ms", "total_ms")
 
     line = {
         "metric": METRIC,
-        "value": R / (ms_per_step / 1e3) / 1e6,
+        "value": rays_per_step / (ms_per_step / 1e3) / 1e6,
         "unit": "MRays/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -176,36 +175,41 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "icosphere-4M: LBVH build + 1920x1080x8 incoherent closest-hit rays per step",
-            "triangles": N_TRIS,
-            "rays_per_step": R,
-            "rays": "origins U(scene box), directions U(S^2), t in [1e-4, FLT_MAX]",
-            "bvh": "binary LBVH (Karras), 64 B nodes, 48 B sorted triangles",
+            "workload": "icosphere-4M + quad light: LBVH build, then wavefront NEE and PATH renders "
+            "at 1920x1080x8 spp, max depth 5 (Lambertian albedo U[0.2,0.8]^3, emission 15)",
+            "triangles": gscene.triangle_count,
+            "paths_per_render": WIDTH * HEIGHT * SPP,
         },
-        "trace_mrays_s": R / trace_avg_s / 1e6,
-        "trace_ms": float(trace_ms.mean()),
-        "build_ms": float(build_ms.mean()),
-        "build_mtris_s": N_TRIS / build_s / 1e6,
+        "rays_per_step": rays_per_step,
+        "incoherent_mrays_s": incoherent / (kernel_ms / 1e3) / 1e6,
+        "render_mrays_s": rays_per_step / (render_ms / 1e3) / 1e6,
+        "nee": {k: nee[k] for k in keep},
+        "path": {k: path[k] for k in keep},
+        "nee_rays_per_bounce": nee["rays_per_bounce"][:DEPTH],
+        "nee_image_mean_rgb": img,
+        "build_ms": build_ms,
+        "build_mtris_s": gscene.triangle_count / (build_ms / 1e3) / 1e6,
         "build_stages_ms": {k: v / 1e6 for k, v in stages.items()},
-        "sort_gbs": N_TRIS * 4 * SORT_BYTES_PER_KEY_PASS / sort_s / 1e9,
-        "build_gbs_algorithmic": N_TRIS * BUILD_BYTES_PER_TRI / build_s / 1e9,
-        "traversal": {"node_visits": visits, "leaf_tests": leaf_tests, "max_stack": max_stack, "hit_fraction": hit_frac},
+        "sort_gbs": gscene.triangle_count * 4 * SORT_BYTES_PER_KEY_PASS / (stages["sort_ns"] / 1e9) / 1e9,
         "roofline": {
-            "kernel": "trace_kernel<closest-hit>",
+            "kernel": "extend_kernel<NEE> (closest hit + NEE candidate + Lambertian bounce)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "bytes_per_ray": bytes_per_ray,
+            "launches_per_render": ext_launches,
+            "avg_launch_ms": ext_ms_nee / max(ext_launches, 1),
+            "bytes_per_render": ext_bytes,
+            "node_visits_per_ray": nee["extension_node_visits"] / nee["extension_rays"],
+            "leaf_tests_per_ray": nee["extension_leaf_tests"] / nee["extension_rays"],
         },
     }
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(vertices, rays, min(args.cpu_sample, R), threads)
+        line["cpu_baseline"] = cpu_baseline(scene, cam, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)
-    bvh.free()
+    gscene.free()
     if args.copy_calibration_mib:
         nbytes = args.copy_calibration_mib << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)

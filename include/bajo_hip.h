@@ -30,6 +30,10 @@ extern "C" {
 #define BAJO_ERR_UNSUPPORTED (-4)
 #define BAJO_STATUS_STACK_OVERFLOW 1 /* a traversal needed more than 64 stack entries */
 #define BAJO_STATUS_BAD_NODE 2       /* a traversal met a node index outside the hierarchy */
+/* WAVE_STATUS words of the renderer, rt/wavefront_contract.mojo:25-29, offset by 10 */
+#define BAJO_STATUS_WAVE_BASE 10
+#define BAJO_STATUS_PATH_OVERFLOW 11
+#define BAJO_STATUS_SHADOW_OVERFLOW 13
 
 /* Trace modes, bvh/constants.mojo:9-13 (TRACE.CLOSEST_HIT / ANY_HIT). */
 #define BAJO_TRACE_CLOSEST_HIT 0
@@ -142,6 +146,71 @@ int32_t bajo_trace_camera(bajo_ctx* ctx, const bajo_bvh* bvh, const float* camer
  * flattened to 20 floats (bvh/camera.mojo:210-231).  Host-side helper. */
 int32_t bajo_camera_make(const float* origin3, const float* target3, const float* up3, float fov_scale,
                          float focus_dist, float* out20);
+
+/* ---- wavefront renderer: bajo/rt/gpu ------------------------------------
+ * RENDER enum, rt/types.mojo:51-57. */
+#define BAJO_RENDER_PATH 0
+#define BAJO_RENDER_NORMALS 1
+#define BAJO_RENDER_AO 2
+#define BAJO_RENDER_NEE 3
+#define BAJO_RENDER_MIS 4
+/* MAT enum (rt/types.mojo:24-29); a surface id is kind << 28 | index. */
+#define BAJO_MAT_LAMBERTIAN 0
+#define BAJO_MAT_EMISSIVE 3
+
+typedef struct bajo_scene bajo_scene;
+
+/* RenderSettings (rt/types.mojo:324-357) plus the path arena capacity of
+ * GpuRtRenderTarget (rt/gpu/resources.mojo:56-99; 0 = the whole frame). */
+typedef struct {
+  uint32_t image_width, image_height, samples_per_pixel, max_depth;
+  uint64_t rng_seed;
+  int32_t algorithm;
+  uint32_t path_capacity;
+} bajo_render_settings;
+
+/* Rays traced by one render (read back from device counters). */
+typedef struct {
+  uint64_t extension_rays;      /* closest-hit rays, all bounces        */
+  uint64_t shadow_rays;         /* any-hit rays (NEE / AO)              */
+  uint64_t incoherent_rays;     /* extension + shadow rays of bounce >= 1 */
+  uint64_t rays_per_bounce[16]; /* extension rays per bounce            */
+  uint32_t status;              /* WAVE_STATUS (0 = ok)                 */
+  uint32_t traversal_status;    /* BAJO_STATUS_* of the traversals      */
+  uint64_t extension_node_visits, extension_leaf_tests; /* traversal work */
+  uint64_t shadow_node_visits, shadow_leaf_tests;
+  float total_ms;               /* HIP events on the context stream      */
+  float extend_ms;              /* sum over extend launches              */
+  float shadow_ms;              /* sum over shadow launches              */
+  uint32_t extend_launches, shadow_launches;
+} bajo_render_stats;
+
+/* SceneData for triangle scenes (rt/types.mojo:382-526): host arrays of
+ * vertices f32[9N], per-triangle surface ids u32[N], Lambertian albedos
+ * f32[3L] and emissive radiances f32[3E].  Uploads them, builds the LBVH
+ * and the Walker alias light table (rt/types.mojo:207-247, weight = area x
+ * mean radiance). */
+int32_t bajo_scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
+                          const float* lambertian, uint32_t n_lambertian, const float* emissive,
+                          uint32_t n_emissive, bajo_scene** out);
+/* Rebuilds the scene's BVH from its device-resident vertices. */
+int32_t bajo_scene_rebuild(bajo_ctx* ctx, bajo_scene* scene, bajo_build_timings* timings);
+int32_t bajo_scene_free(bajo_scene* scene);
+int32_t bajo_scene_get_bvh(const bajo_scene* scene, bajo_bvh** out);
+/* Host copy of the light table: fields f32[14 * count] (p0, p1, p2, radius,
+ * emission, alias probability), kinds u32[count] (alias << 4 | prim kind). */
+int32_t bajo_scene_light_table(const bajo_scene* scene, float* fields, uint32_t* kinds, uint32_t* count,
+                               float* total_weight);
+/* render_gpu[ALGORITHM] (rt/gpu/render.mojo:322-420) into device pixels
+ * f32[3 * width * height]; `camera` is a host f32[20] record.  Enqueued on
+ * the context stream; with `stats` it synchronizes and returns the ray
+ * counts and any queue status (bajo raises those in download_gpu_pixels,
+ * rt/gpu/resources.mojo:268-285). */
+int32_t bajo_render(bajo_ctx* ctx, bajo_scene* scene, const bajo_render_settings* settings, const float* camera,
+                    float* pixels, bajo_render_stats* stats);
+/* Host view of one stage's random stream (tests and tools): `count` floats
+ * of the Philox stream keyed by (seed, path_id, stage). */
+int32_t bajo_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float* out);
 
 /* ---- synthetic scenes (host) -------------------------------------------
  * Seeded PCG32 (O'Neill 2014) generators for the benchmark workloads.  All

@@ -57,7 +57,25 @@ _SIG = {
         [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_void_p],
     ),
     "orc_rcp_direction": (None, [c_void_p, c_void_p]),
+    "orc_philox4x32_10": (None, [c_void_p, c_uint32, c_uint32, c_void_p]),
+    "orc_rng_stage_floats": (None, [c_uint64, c_uint32, c_uint32, c_uint32, c_void_p]),
+    "orc_sincos_turns": (None, [c_float, c_void_p, c_void_p]),
+    "orc_alias_table": (None, [c_void_p, c_uint32, c_float, c_void_p, c_void_p]),
+    "orc_light_table": (None, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "orc_render": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "orc_render_fp64": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
 }
+
+
+class RenderSettings(ctypes.Structure):
+    _fields_ = [
+        ("width", c_uint32),
+        ("height", c_uint32),
+        ("spp", c_uint32),
+        ("max_depth", c_uint32),
+        ("seed", c_uint64),
+        ("algorithm", c_int),
+    ]
 
 _lib = None
 
@@ -237,3 +255,74 @@ def hit_prims(hits: np.ndarray) -> np.ndarray:
 
 def hit_t(hits: np.ndarray) -> np.ndarray:
     return hits.reshape(-1, 8)[:, 7]
+
+
+# ---- renderer -----------------------------------------------------------------
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    out = np.empty(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c), int(key[0]), int(key[1]), _p(out))
+    return out
+
+
+def rng_stage_floats(seed: int, path_id: int, stage: int, count: int) -> np.ndarray:
+    out = np.empty(count, np.float32)
+    lib().orc_rng_stage_floats(seed, path_id, stage, count, _p(out))
+    return out
+
+
+def sincos_turns(u: float):
+    s = np.zeros(1, np.float32)
+    c = np.zeros(1, np.float32)
+    lib().orc_sincos_turns(u, _p(s), _p(c))
+    return float(s[0]), float(c[0])
+
+
+def alias_table(weights) -> tuple[np.ndarray, np.ndarray]:
+    w = f32(weights)
+    n = w.size
+    total = np.float32(0)
+    for x in w:
+        total = np.float32(total + x)
+    prob = np.empty(max(n, 1), np.float32)
+    alias = np.empty(max(n, 1), np.uint32)
+    lib().orc_alias_table(_p(w), n, float(total), _p(prob), _p(alias))
+    return prob[:n], alias[:n]
+
+
+def light_table(vertices, surfaces, emissive):
+    v = f32(vertices)
+    n = v.size // 9
+    surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
+    e = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
+    fields = np.empty(14 * max(n, 1), np.float32)
+    kinds = np.empty(max(n, 1), np.uint32)
+    count = np.zeros(1, np.uint32)
+    total = np.zeros(1, np.float32)
+    lib().orc_light_table(_p(v), n, _p(surf), _p(e), _p(fields), _p(kinds), _p(count), _p(total))
+    k = int(count[0])
+    return fields[: 14 * k], kinds[:k], float(total[0])
+
+
+def render(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1):
+    v = f32(vertices)
+    surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
+    lamb = f32(lambertian) if len(lambertian) else np.zeros(3, np.float32)
+    emit = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
+    st = RenderSettings(width, height, spp, max_depth, seed, algorithm)
+    pixels = np.empty(3 * width * height, np.float32)
+    rays = np.zeros(2, np.uint64)
+    lib().orc_render(_p(v), v.size // 9, _p(surf), _p(lamb), _p(emit), ctypes.byref(st), _p(f32(cam20)), _p(pixels), _p(rays), threads)
+    return pixels, rays
+
+
+def render_fp64(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1):
+    v = f32(vertices)
+    surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
+    lamb = f32(lambertian) if len(lambertian) else np.zeros(3, np.float32)
+    emit = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
+    st = RenderSettings(width, height, spp, max_depth, seed, algorithm)
+    pixels = np.empty(3 * width * height, np.float64)
+    lib().orc_render_fp64(_p(v), v.size // 9, _p(surf), _p(lamb), _p(emit), ctypes.byref(st), _p(f32(cam20)), _p(pixels), threads)
+    return pixels

@@ -57,6 +57,25 @@ int orc_intersect_tri_scaled(const float *o3, const float *d3, const float *v0, 
                              float t_max, float t_min, float *dtuv4);
 void orc_rcp_direction(const float *d3, float *out3);
 
+/* wavefront renderer (orc_render.c) */
+typedef struct {
+    uint32_t width, height, spp, max_depth;
+    uint64_t seed;
+    int algorithm; /* 0 PATH, 1 NORMALS, 2 AO, 3 NEE, 4 MIS */
+} orc_render_settings;
+void orc_philox4x32_10(const uint32_t *ctr, uint32_t k0, uint32_t k1, uint32_t *out);
+void orc_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float *out);
+void orc_sincos_turns(float u, float *s, float *c);
+void orc_alias_table(const float *w, uint32_t n, float total, float *prob, uint32_t *alias);
+void orc_light_table(const float *v, uint64_t n, const uint32_t *surf, const float *emissive, float *fields,
+                     uint32_t *kinds, uint32_t *count, float *total);
+int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
+               const float *emissive, const orc_render_settings *st, const float *cam20, float *pixels,
+               uint64_t *ray_counts, int threads);
+int orc_render_fp64(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
+                    const float *emissive, const orc_render_settings *st, const float *cam20, double *pixels,
+                    int threads);
+
 #ifdef __cplusplus
 }
 #endif

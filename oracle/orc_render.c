@@ -1,0 +1,620 @@
+/*
+ * CPU oracle -- test infrastructure only (see orc_core.h).
+ *
+ * Restates bajo's wavefront path tracer for Lambertian + emissive triangle
+ * scenes, one path at a time, in the stage order the GPU wavefront uses
+ * (extend -> shadow per bounce):
+ *   - stage streams, (path_id << 32) | stage   rt/wavefront_contract.mojo:107-134, rt/common.mojo:37-40
+ *   - primary ray (lens disk, jitter)           rt/gpu/common_kernels.mojo:22-94, bvh/camera.mojo:156-208
+ *   - sky on miss                               rt/common.mojo:24-33
+ *   - oriented normal                           rt/geometry.mojo:20-30
+ *   - NEE candidate (alias pick, area sample)   rt/gpu/path_shading.mojo:261-403
+ *   - emission weights (PATH / NEE / MIS)       rt/gpu/path_shading.mojo:570-610, rt/lighting.mojo:7-38
+ *   - Lambertian sample + roulette              rt/shading.mojo:92-118, rt/common.mojo:43-67
+ *   - AO ray / NORMALS                          rt/gpu/path_shading.mojo:651-667, :556-563
+ *   - light table (Walker alias, LIFO lists)    rt/types.mojo:207-247, :490-506, :527-537
+ *   - resolve                                   rt/gpu/common_kernels.mojo:97-136
+ * The random generator is this repository's own Philox4x32-10 stream
+ * definition (bajo's Mojo stdlib Random[10] mapping is not available).
+ *
+ * orc_render_fp64 is an independent double-precision estimator of the same
+ * images (own PCG32 streams, brute-force double intersection), used to check
+ * radiance statistically.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+#include "orc_internal.h"
+
+/* ---- Philox4x32-10 (Salmon et al., SC'11) ---------------------------------- */
+void orc_philox4x32_10(const uint32_t *ctr, uint32_t k0, uint32_t k1, uint32_t *out) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+typedef struct { uint32_t k0, k1, s0, s1, block; float buf[4]; int used; } orc_rng;
+
+static void rng_init(orc_rng *r, uint64_t seed, uint32_t path_id, uint32_t stage) {
+    r->k0 = (uint32_t)seed; r->k1 = (uint32_t)(seed >> 32); r->s0 = stage; r->s1 = path_id;
+    r->block = 0; r->used = 4;
+}
+static float rng_next(orc_rng *r) {
+    if (r->used >= 4) {
+        uint32_t c[4] = {r->block, 0u, r->s0, r->s1}, o[4];
+        orc_philox4x32_10(c, r->k0, r->k1, o);
+        r->block++;
+        for (int i = 0; i < 4; ++i) r->buf[i] = (float)(o[i] >> 8) * (1.0f / 16777216.0f);
+        r->used = 0;
+    }
+    return r->buf[r->used++];
+}
+
+void orc_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float *out) {
+    orc_rng r;
+    rng_init(&r, seed, path_id, stage);
+    for (uint32_t i = 0; i < count; ++i) out[i] = rng_next(&r);
+}
+
+static uint32_t stage_bsdf(uint32_t b) { return (0u << 30) | (b + 1u); }
+static uint32_t stage_roulette(uint32_t b) { return (1u << 30) | (b + 1u); }
+static uint32_t stage_light(uint32_t b) { return (2u << 30) | (b + 1u); }
+
+/* sin/cos(2 pi u): quarter-turn reduction, Taylor series with FMAs. */
+void orc_sincos_turns(float u, float *s, float *c) {
+    float q = rintf(u * 4.0f);
+    float r = u - q * 0.25f;
+    float x = r * 6.28318530717958647692f;
+    float x2 = x * x;
+    float ps = fmaf(x2, 2.75573192239858906526e-6f, -1.98412698412698412698e-4f);
+    ps = fmaf(x2, ps, 8.33333333333333333333e-3f);
+    ps = fmaf(x2, ps, -1.66666666666666666667e-1f);
+    float sn = fmaf(x * x2, ps, x);
+    float pc = fmaf(x2, -2.75573192239858906526e-7f, 2.48015873015873015873e-5f);
+    pc = fmaf(x2, pc, -1.38888888888888888889e-3f);
+    pc = fmaf(x2, pc, 4.16666666666666666667e-2f);
+    pc = fmaf(x2, pc, -0.5f);
+    float cs = fmaf(x2, pc, 1.0f);
+    switch (((int)q) & 3) {
+        case 0: *s = sn; *c = cs; break;
+        case 1: *s = cs; *c = -sn; break;
+        case 2: *s = -sn; *c = -cs; break;
+        default: *s = -cs; *c = sn; break;
+    }
+}
+
+/* ---- light table ------------------------------------------------------------- */
+#define L_STRIDE 14
+#define MAT_LAMB 0u
+#define MAT_EMIT 3u
+#define IDX_MASK ((1u << 28) - 1u)
+
+void orc_alias_table(const float *w, uint32_t n, float total, float *prob, uint32_t *alias) {
+    for (uint32_t i = 0; i < n; ++i) { prob[i] = 1.0f; alias[i] = 0; }
+    if (n == 0 || !(total > 0.0f)) return;
+    float *scaled = (float *)malloc(n * sizeof(float));
+    uint32_t *small = (uint32_t *)malloc(n * sizeof(uint32_t)), *large = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint32_t ns = 0, nl = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        alias[i] = i;
+        scaled[i] = w[i] * (float)n / total;
+        if (scaled[i] < 1.0f) small[ns++] = i; else large[nl++] = i;
+    }
+    while (ns > 0 && nl > 0) {
+        uint32_t si = small[--ns], li = large[--nl];
+        prob[si] = scaled[si];
+        alias[si] = li;
+        scaled[li] = scaled[li] + scaled[si] - 1.0f;
+        if (scaled[li] < 1.0f) small[ns++] = li; else large[nl++] = li;
+    }
+    while (ns > 0) { uint32_t i = small[--ns]; prob[i] = 1.0f; alias[i] = i; }
+    while (nl > 0) { uint32_t i = large[--nl]; prob[i] = 1.0f; alias[i] = i; }
+    free(scaled); free(small); free(large);
+}
+
+typedef struct {
+    float *fields;  /* 14 per light */
+    uint32_t *kinds;
+    uint32_t count;
+    float total;
+} lights_t;
+
+static void build_lights(const float *v, uint64_t n, const uint32_t *surf, const float *emissive, lights_t *L) {
+    uint32_t cnt = 0;
+    for (uint64_t i = 0; i < n; ++i) if ((surf[i] >> 28) == MAT_EMIT) cnt++;
+    float *w = (float *)malloc((cnt ? cnt : 1) * sizeof(float));
+    uint32_t *tri = (uint32_t *)malloc((cnt ? cnt : 1) * sizeof(uint32_t));
+    uint32_t k = 0;
+    float total = 0.0f;
+    for (uint64_t i = 0; i < n; ++i) {
+        if ((surf[i] >> 28) != MAT_EMIT) continue;
+        const float *t = v + 9 * i;
+        orc_v3 p0 = orc_v(t[0], t[1], t[2]), p1 = orc_v(t[3], t[4], t[5]), p2 = orc_v(t[6], t[7], t[8]);
+        orc_v3 cr = orc_cross(orc_sub(p1, p0), orc_sub(p2, p0));
+        float area = 0.5f * sqrtf(orc_dot(cr, cr));
+        const float *e = emissive + 3 * (surf[i] & IDX_MASK);
+        float imp = orc_fmax((e[0] + e[1] + e[2]) / 3.0f, 0.0f);
+        float wt = area * imp;
+        if (wt > 0.0f) { w[k] = wt; tri[k] = (uint32_t)i; total += wt; k++; }
+    }
+    float *prob = (float *)malloc((k ? k : 1) * sizeof(float));
+    uint32_t *alias = (uint32_t *)malloc((k ? k : 1) * sizeof(uint32_t));
+    orc_alias_table(w, k, total, prob, alias);
+    L->fields = (float *)calloc((k ? k : 1) * L_STRIDE, sizeof(float));
+    L->kinds = (uint32_t *)calloc(k ? k : 1, sizeof(uint32_t));
+    for (uint32_t l = 0; l < k; ++l) {
+        const float *t = v + 9 * (uint64_t)tri[l];
+        const float *e = emissive + 3 * (surf[tri[l]] & IDX_MASK);
+        float *o = L->fields + (size_t)l * L_STRIDE;
+        for (int j = 0; j < 9; ++j) o[j] = t[j];
+        o[10] = e[0]; o[11] = e[1]; o[12] = e[2];
+        o[13] = prob[l];
+        L->kinds[l] = (alias[l] << 4) | 1u;
+    }
+    L->count = k;
+    L->total = total;
+    free(w); free(tri); free(prob); free(alias);
+}
+
+void orc_light_table(const float *v, uint64_t n, const uint32_t *surf, const float *emissive, float *fields,
+                     uint32_t *kinds, uint32_t *count, float *total) {
+    lights_t L;
+    build_lights(v, n, surf, emissive, &L);
+    if (fields) memcpy(fields, L.fields, (size_t)L.count * L_STRIDE * sizeof(float));
+    if (kinds) memcpy(kinds, L.kinds, (size_t)L.count * sizeof(uint32_t));
+    *count = L.count;
+    *total = L.total;
+    free(L.fields); free(L.kinds);
+}
+
+/* ---- one path ------------------------------------------------------------------- */
+typedef struct {
+    orc_bvh bvh;
+    const uint32_t *surf;
+    const float *lamb, *emit;
+    lights_t lights;
+    orc_render_settings st;
+    float cam[20];
+} scene_t;
+
+static float power_heuristic(float a, float b) {
+    float a2 = a * a, b2 = b * b, d = a2 + b2;
+    return d > 0.0f ? a2 / d : 0.0f;
+}
+
+static orc_v3 sky(orc_v3 d) {
+    orc_v3 u = orc_normalize(d);
+    float a = 0.5f * (u.y + 1.0f);
+    return orc_add(orc_scale(orc_v(1.0f, 1.0f, 1.0f), 1.0f - a), orc_scale(orc_v(0.5f, 0.7f, 1.0f), a));
+}
+
+static void acc(float *s, orc_v3 c) { s[0] += c.x; s[1] += c.y; s[2] += c.z; }
+
+/* Returns 1 with a shadow ray + contribution, rt/gpu/path_shading.mojo:261-403. */
+static int direct_light(const scene_t *sc, uint32_t path_id, uint32_t bounce, orc_v3 point, orc_v3 normal,
+                        orc_v3 albedo, orc_v3 T, orc_v3 *dir_out, float *tmax_out, orc_v3 *contrib) {
+    const lights_t *L = &sc->lights;
+    if (L->count == 0 || !(L->total > 0.0f)) return 0;
+    orc_rng rng;
+    rng_init(&rng, sc->st.seed, path_id, stage_light(bounce));
+    float as = rng_next(&rng) * (float)L->count;
+    uint32_t sel = (uint32_t)as;
+    if (sel >= L->count) sel = L->count - 1;
+    float frac = as - (float)sel;
+    uint32_t packed = L->kinds[sel];
+    if (frac > L->fields[sel * L_STRIDE + 13]) sel = packed >> 4;
+    const float *f = L->fields + (size_t)sel * L_STRIDE;
+    orc_v3 p0 = orc_v(f[0], f[1], f[2]), p1 = orc_v(f[3], f[4], f[5]), p2 = orc_v(f[6], f[7], f[8]);
+    orc_v3 e1 = orc_sub(p1, p0), e2 = orc_sub(p2, p0);
+    orc_v3 av = orc_cross(e1, e2);
+    float tas = orc_dot(av, av);
+    if (tas <= 0.0f) return 0;
+    float sq = sqrtf(tas);
+    orc_v3 ln = orc_v(av.x / sq, av.y / sq, av.z / sq);
+    float root_u = sqrtf(rng_next(&rng));
+    float b1 = root_u * (1.0f - rng_next(&rng));
+    float b2 = root_u - b1;
+    orc_v3 lp = orc_add(orc_add(p0, orc_scale(e1, b1)), orc_scale(e2, b2));
+    orc_v3 tl = orc_sub(lp, point);
+    float d2 = orc_dot(tl, tl);
+    if (d2 <= 1.0e-8f) return 0;
+    float dist = sqrtf(d2);
+    orc_v3 dir = orc_v(tl.x / dist, tl.y / dist, tl.z / dist);
+    float scos = orc_fmax(orc_dot(normal, dir), 0.0f);
+    float lcos = orc_fmax(orc_dot(ln, orc_v(-dir.x, -dir.y, -dir.z)), 0.0f);
+    if (scos <= 0.0f || lcos <= 0.0f) return 0;
+    float tmax = dist - 0.002f;
+    if (tmax <= 0.001f) return 0;
+    orc_v3 E = orc_v(f[10], f[11], f[12]);
+    float light_pdf = d2 * (E.x + E.y + E.z) / (3.0f * lcos * L->total);
+    orc_v3 value = orc_scale(albedo, 0.3183098861837907f);
+    float bsdf_pdf = orc_fmax(orc_dot(normal, orc_normalize(dir)), 0.0f) * 0.3183098861837907f;
+    int ok = light_pdf > 0.0f && bsdf_pdf > 0.0f;
+    float w = sc->st.algorithm == 4 ? power_heuristic(light_pdf, bsdf_pdf) : 1.0f;
+    float s = ok ? scos * w / light_pdf : 0.0f;
+    if (!(s > 0.0f)) return 0;
+    *contrib = orc_scale(orc_mul(orc_mul(T, value), E), s);
+    *dir_out = dir;
+    *tmax_out = tmax;
+    return 1;
+}
+
+static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint64_t *rays) {
+    const orc_render_settings *st = &sc->st;
+    const float *k = sc->cam;
+    uint32_t pixel = path_id / st->spp, px = pixel % st->width, py = pixel / st->width;
+    orc_rng rng;
+    rng_init(&rng, st->seed, path_id, 0u);
+    float s, c;
+    orc_sincos_turns(rng_next(&rng), &s, &c);
+    float rr = sqrtf(rng_next(&rng));
+    float lu = rr * c, lv = rr * s;
+    float pu = rng_next(&rng), pv = rng_next(&rng);
+    float fw = (float)st->width, fh = (float)st->height;
+    float aspect = fw / fh;
+    float sx = (((float)px + pu) / fw) * 2.0f - 1.0f;
+    float sy = 1.0f - (((float)py + pv) / fh) * 2.0f;
+    orc_v3 origin = orc_v(k[0], k[1], k[2]);
+    orc_v3 plane = orc_add(orc_add(orc_v(k[3], k[4], k[5]), orc_scale(orc_v(k[6], k[7], k[8]), sx * aspect * k[12])),
+                           orc_scale(orc_v(k[9], k[10], k[11]), sy * k[12]));
+    orc_v3 focal = orc_add(origin, orc_scale(plane, k[13]));
+    orc_v3 ro = orc_add(orc_add(origin, orc_scale(orc_v(k[14], k[15], k[16]), lu)), orc_scale(orc_v(k[17], k[18], k[19]), lv));
+    orc_v3 o = ro, d = orc_normalize(orc_sub(focal, ro));
+    float tmin = 0.001f;
+    orc_v3 T = orc_v(1.0f, 1.0f, 1.0f);
+    float path_pdf = 0.0f;
+    int delta = 1;
+    int alg = st->algorithm;
+    uint32_t depth = (alg == 1 || alg == 2) ? 1u : st->max_depth;
+    sample[0] = sample[1] = sample[2] = 0.0f;
+    for (uint32_t b = 0; b < depth; ++b) {
+        if (rays) rays[0]++;
+        orc_hit h = orc_trace_closest(&sc->bvh, o, d, tmin, ORC_FLT_MAX, NULL);
+        int found = h.prim != ORC_EMPTY && h.t < ORC_FLT_MAX;
+        int shadow = 0, next = 0;
+        orc_v3 sh_o = orc_v(0, 0, 0), sh_d = orc_v(0, 0, 0), sh_c = orc_v(0, 0, 0);
+        float sh_tmax = 0.0f;
+        orc_v3 no = orc_v(0, 0, 0), nd = orc_v(0, 0, 0), nT = T;
+        float npdf = 0.0f;
+        if (!found) {
+            if (alg == 0 || alg == 3 || alg == 4) acc(sample, orc_mul(T, sky(d)));
+        } else {
+            int front = orc_dot(d, h.n) < 0.0f;
+            orc_v3 n = front ? h.n : orc_v(-h.n.x, -h.n.y, -h.n.z);
+            orc_v3 point = orc_add(o, orc_scale(d, h.t));
+            uint32_t surf = sc->surf[h.prim], kind = surf >> 28, mi = surf & IDX_MASK;
+            if (alg == 1) {
+                acc(sample, orc_scale(orc_add(n, orc_v(1.0f, 1.0f, 1.0f)), 0.5f));
+            } else if (alg == 2) {
+                orc_rng ar;
+                rng_init(&ar, st->seed, path_id, 1u);
+                float as_, ac_;
+                orc_sincos_turns(rng_next(&ar), &as_, &ac_);
+                float z = 1.0f - 2.0f * rng_next(&ar);
+                float r = sqrtf(1.0f - z * z);
+                orc_v3 dir = orc_v(r * ac_, r * as_, z);
+                if (orc_dot(dir, n) < 0.0f) dir = orc_v(-dir.x, -dir.y, -dir.z);
+                shadow = 1; sh_o = point; sh_d = dir; sh_tmax = 4.0f;
+            } else {
+                orc_v3 albedo = orc_v(0, 0, 0);
+                if (kind == MAT_LAMB) albedo = orc_v(sc->lamb[3 * mi], sc->lamb[3 * mi + 1], sc->lamb[3 * mi + 2]);
+                if ((alg == 3 || alg == 4) && kind == MAT_LAMB) {
+                    shadow = direct_light(sc, path_id, b, point, n, albedo, T, &sh_d, &sh_tmax, &sh_c);
+                    sh_o = point;
+                }
+                if (kind == MAT_EMIT) {
+                    if (front) {
+                        orc_v3 E = orc_v(sc->emit[3 * mi], sc->emit[3 * mi + 1], sc->emit[3 * mi + 2]);
+                        float w = 1.0f;
+                        if (alg == 3 && b > 0 && !delta) w = 0.0f;
+                        if (alg == 4 && b > 0 && !delta) {
+                            float lc = orc_fmax(orc_dot(n, orc_normalize(orc_v(-d.x, -d.y, -d.z))), 0.0f);
+                            float lpdf = 0.0f;
+                            if (lc > 0.0f && sc->lights.total > 0.0f) {
+                                float d2 = h.t * h.t * orc_dot(d, d);
+                                lpdf = d2 * (E.x + E.y + E.z) / (3.0f * lc * sc->lights.total);
+                            }
+                            w = power_heuristic(path_pdf, lpdf);
+                        }
+                        acc(sample, orc_scale(orc_mul(T, E), w));
+                    }
+                } else if (kind == MAT_LAMB) {
+                    orc_rng br;
+                    rng_init(&br, st->seed, path_id, stage_bsdf(b));
+                    float ru = rng_next(&br), rv = rng_next(&br);
+                    float ss, cc;
+                    orc_sincos_turns(ru, &ss, &cc);
+                    float z = 1.0f - 2.0f * rv;
+                    float radius = sqrtf(orc_fmax(1.0f - z * z, 0.0f));
+                    orc_v3 sd = orc_add(n, orc_v(radius * cc, radius * ss, z));
+                    if (fabsf(sd.x) < 1e-8f && fabsf(sd.y) < 1e-8f && fabsf(sd.z) < 1e-8f) sd = n;
+                    orc_v3 dir = orc_normalize(sd);
+                    float pdf = orc_fmax(orc_dot(n, orc_normalize(dir)), 0.0f) * 0.3183098861837907f;
+                    orc_v3 Tn = orc_mul(T, albedo);
+                    int alive = 1;
+                    if (b + 1u >= 5u) {
+                        float mx = orc_fmax(Tn.x, orc_fmax(Tn.y, Tn.z));
+                        if (mx <= 0.0f) alive = 0;
+                        else {
+                            float survival = orc_fmin(orc_fmax(mx, 0.05f), 0.95f);
+                            orc_rng rr2;
+                            rng_init(&rr2, st->seed, path_id, stage_roulette(b));
+                            if (rng_next(&rr2) >= survival) alive = 0;
+                            else Tn = orc_v(Tn.x / survival, Tn.y / survival, Tn.z / survival);
+                        }
+                    }
+                    if (alive) {
+                        next = 1;
+                        no = orc_v(o.x + h.t * d.x, o.y + h.t * d.y, o.z + h.t * d.z);
+                        nd = dir; nT = Tn; npdf = pdf;
+                    }
+                }
+            }
+        }
+        if (shadow) {
+            if (rays) rays[1]++;
+            orc_hit sh = orc_trace_any(&sc->bvh, sh_o, sh_d, 0.001f, sh_tmax);
+            int occluded = sh.prim == ORC_EMPTY && sh.t == 0.0f;
+            if (alg == 2) { float v = occluded ? 0.08f : 1.0f; acc(sample, orc_v(v, v, v)); }
+            else if (!occluded) acc(sample, sh_c);
+        }
+        if (!next) break;
+        o = no; d = nd; T = nT; path_pdf = npdf; tmin = 0.001f; delta = 0;
+    }
+}
+
+typedef struct { const scene_t *sc; uint64_t first, last; float *samples; uint64_t rays[2]; } job_t;
+
+static void *render_worker(void *p) {
+    job_t *j = (job_t *)p;
+    for (uint64_t id = j->first; id < j->last; ++id) render_path(j->sc, (uint32_t)id, j->samples + 3 * id, j->rays);
+    return NULL;
+}
+
+int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
+               const float *emissive, const orc_render_settings *st, const float *cam20, float *pixels,
+               uint64_t *ray_counts, int threads) {
+    uint64_t m = n > 1 ? n - 1 : 1;
+    uint64_t *keys = (uint64_t *)malloc(n * sizeof(uint64_t));
+    uint32_t *ids = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint32_t *meta = (uint32_t *)malloc(4 * m * sizeof(uint32_t));
+    uint32_t *parent = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint32_t *counts = (uint32_t *)malloc(m * sizeof(uint32_t));
+    float *nb = (float *)malloc(12 * m * sizeof(float));
+    float scene12[12];
+    uint32_t root = orc_build_lbvh(vertices, n, keys, ids, meta, parent, counts, nb, scene12);
+    scene_t sc;
+    sc.bvh.vertices = vertices; sc.bvh.leaf_ids = ids; sc.bvh.node_meta = meta; sc.bvh.node_bounds = nb; sc.bvh.root = root;
+    sc.surf = surfaces; sc.lamb = lambertian; sc.emit = emissive; sc.st = *st;
+    memcpy(sc.cam, cam20, sizeof(sc.cam));
+    build_lights(vertices, n, surfaces, emissive, &sc.lights);
+    uint64_t total = (uint64_t)st->width * st->height * st->spp;
+    float *samples = (float *)malloc(3 * total * sizeof(float));
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t tid[256];
+    job_t jobs[256];
+    for (int t = 0; t < threads; ++t) {
+        job_t j = {&sc, total * (uint64_t)t / (uint64_t)threads, total * (uint64_t)(t + 1) / (uint64_t)threads, samples, {0, 0}};
+        jobs[t] = j;
+        pthread_create(&tid[t], NULL, render_worker, &jobs[t]);
+    }
+    uint64_t rays[2] = {0, 0};
+    for (int t = 0; t < threads; ++t) { pthread_join(tid[t], NULL); rays[0] += jobs[t].rays[0]; rays[1] += jobs[t].rays[1]; }
+    float kk = 1.0f / (float)st->spp;
+    for (uint64_t p = 0; p < (uint64_t)st->width * st->height; ++p) {
+        float r = 0, g = 0, b = 0;
+        for (uint64_t s = p * st->spp; s < (p + 1) * st->spp; ++s) { r += samples[3 * s]; g += samples[3 * s + 1]; b += samples[3 * s + 2]; }
+        pixels[3 * p] = r * kk; pixels[3 * p + 1] = g * kk; pixels[3 * p + 2] = b * kk;
+    }
+    if (ray_counts) { ray_counts[0] = rays[0]; ray_counts[1] = rays[1]; }
+    free(samples); free(keys); free(ids); free(meta); free(parent); free(counts); free(nb);
+    free(sc.lights.fields); free(sc.lights.kinds);
+    return 0;
+}
+
+/* ---- FP64 reference estimator ---------------------------------------------------- */
+typedef struct { uint64_t state, inc; } pcg_t;
+static uint32_t pcg_next(pcg_t *r) {
+    uint64_t old = r->state;
+    r->state = old * 6364136223846793005ULL + r->inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u), rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+}
+static double pcg_d(pcg_t *r) { return (double)(pcg_next(r) >> 5) * (1.0 / 134217728.0); }
+
+typedef struct { double x, y, z; } d3;
+static d3 D(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static d3 dadd(d3 a, d3 b) { return D(a.x + b.x, a.y + b.y, a.z + b.z); }
+static d3 dsub(d3 a, d3 b) { return D(a.x - b.x, a.y - b.y, a.z - b.z); }
+static d3 dsc(d3 a, double s) { return D(a.x * s, a.y * s, a.z * s); }
+static d3 dmul(d3 a, d3 b) { return D(a.x * b.x, a.y * b.y, a.z * b.z); }
+static double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static d3 dcross(d3 a, d3 b) { return D(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static d3 dnorm(d3 a) { double l = sqrt(ddot(a, a)); return l > 0 ? dsc(a, 1.0 / l) : a; }
+
+typedef struct {
+    const float *v; uint64_t n; const uint32_t *surf; const float *lamb, *emit;
+    double *lw; uint32_t *ltri; uint32_t nl; double total;
+    orc_render_settings st; const float *cam;
+} scene64;
+
+/* brute-force closest hit in double (eager MT, t in (tmin, tmax)) */
+static int hit64(const scene64 *s, d3 o, d3 d, double tmin, double tmax, double *t_out, uint64_t *prim, d3 *nrm) {
+    int found = 0;
+    double best = tmax;
+    for (uint64_t i = 0; i < s->n; ++i) {
+        const float *t = s->v + 9 * i;
+        d3 v0 = D(t[0], t[1], t[2]), e1 = dsub(D(t[3], t[4], t[5]), v0), e2 = dsub(D(t[6], t[7], t[8]), v0);
+        d3 p = dcross(d, e2);
+        double det = ddot(e1, p);
+        if (fabs(det) < 1e-14) continue;
+        double inv = 1.0 / det;
+        d3 tv = dsub(o, v0);
+        double u = ddot(tv, p) * inv;
+        if (u < 0 || u > 1) continue;
+        d3 q = dcross(tv, e1);
+        double vv = ddot(d, q) * inv;
+        if (vv < 0 || u + vv > 1) continue;
+        double tt = ddot(e2, q) * inv;
+        if (tt > tmin && tt < best) { best = tt; found = 1; *prim = i; *nrm = dnorm(dcross(e1, e2)); }
+    }
+    *t_out = best;
+    return found;
+}
+
+static void render_path64(const scene64 *s, uint64_t path_id, pcg_t *rng, double *out) {
+    const orc_render_settings *st = &s->st;
+    const float *k = s->cam;
+    uint64_t pixel = path_id / st->spp, px = pixel % st->width, py = pixel / st->width;
+    double pu = pcg_d(rng), pv = pcg_d(rng);
+    double fw = st->width, fh = st->height, aspect = fw / fh;
+    double sx = ((px + pu) / fw) * 2.0 - 1.0, sy = 1.0 - ((py + pv) / fh) * 2.0;
+    d3 origin = D(k[0], k[1], k[2]);
+    d3 plane = dadd(dadd(D(k[3], k[4], k[5]), dsc(D(k[6], k[7], k[8]), sx * aspect * k[12])), dsc(D(k[9], k[10], k[11]), sy * k[12]));
+    d3 o = origin, d = dnorm(dsub(dadd(origin, dsc(plane, k[13])), origin));
+    d3 T = D(1, 1, 1);
+    int alg = st->algorithm;
+    uint32_t depth = alg == 1 || alg == 2 ? 1u : st->max_depth;
+    out[0] = out[1] = out[2] = 0;
+    double tmin = 0.001;
+    for (uint32_t b = 0; b < depth; ++b) {
+        double t;
+        uint64_t prim;
+        d3 nrm;
+        if (!hit64(s, o, d, tmin, 1e300, &t, &prim, &nrm)) {
+            if (alg == 0 || alg == 3) {
+                d3 u = dnorm(d);
+                double a = 0.5 * (u.y + 1.0);
+                d3 sk = dadd(dsc(D(1, 1, 1), 1.0 - a), dsc(D(0.5, 0.7, 1.0), a));
+                d3 c = dmul(T, sk);
+                out[0] += c.x; out[1] += c.y; out[2] += c.z;
+            }
+            return;
+        }
+        int front = ddot(d, nrm) < 0;
+        d3 n = front ? nrm : dsc(nrm, -1);
+        d3 p = dadd(o, dsc(d, t));
+        uint32_t surf = s->surf[prim], kind = surf >> 28, mi = surf & IDX_MASK;
+        if (alg == 1) { out[0] += 0.5 * (n.x + 1); out[1] += 0.5 * (n.y + 1); out[2] += 0.5 * (n.z + 1); return; }
+        if (alg == 2) {
+            double u1 = pcg_d(rng), u2 = pcg_d(rng);
+            double z = 1 - 2 * u2, r = sqrt(fmax(0.0, 1 - z * z)), th = 2 * M_PI * u1;
+            d3 dir = D(r * cos(th), r * sin(th), z);
+            if (ddot(dir, n) < 0) dir = dsc(dir, -1);
+            double tt; uint64_t pp; d3 nn;
+            double v = hit64(s, p, dir, 0.001, 4.0, &tt, &pp, &nn) ? 0.08 : 1.0;
+            out[0] += v; out[1] += v; out[2] += v;
+            return;
+        }
+        if (kind == MAT_EMIT) {
+            if (front && (alg == 0 || b == 0)) {
+                const float *e = s->emit + 3 * mi;
+                out[0] += T.x * e[0]; out[1] += T.y * e[1]; out[2] += T.z * e[2];
+            }
+            return;
+        }
+        d3 alb = D(s->lamb[3 * mi], s->lamb[3 * mi + 1], s->lamb[3 * mi + 2]);
+        if (alg == 3 && s->nl > 0) {
+            /* light chosen by power, point uniform on the triangle */
+            double xi = pcg_d(rng) * s->total, accw = 0;
+            uint32_t l = 0;
+            while (l + 1 < s->nl && accw + s->lw[l] < xi) accw += s->lw[l++];
+            const float *lt = s->v + 9 * (uint64_t)s->ltri[l];
+            d3 p0 = D(lt[0], lt[1], lt[2]), e1 = dsub(D(lt[3], lt[4], lt[5]), p0), e2 = dsub(D(lt[6], lt[7], lt[8]), p0);
+            d3 av = dcross(e1, e2);
+            double area2 = sqrt(ddot(av, av));
+            d3 ln = dsc(av, 1.0 / area2);
+            double su = sqrt(pcg_d(rng)), b1 = su * (1 - pcg_d(rng)), b2 = su - b1;
+            d3 lp = dadd(dadd(p0, dsc(e1, b1)), dsc(e2, b2));
+            d3 tl = dsub(lp, p);
+            double d2 = ddot(tl, tl), dist = sqrt(d2);
+            d3 dir = dsc(tl, 1.0 / dist);
+            double sc = ddot(n, dir), lc = -ddot(ln, dir);
+            if (sc > 0 && lc > 0 && dist - 0.002 > 0.001) {
+                double tt; uint64_t pp; d3 nn;
+                if (!hit64(s, p, dir, 0.001, dist - 0.002, &tt, &pp, &nn)) {
+                    const float *e = s->emit + 3 * (s->surf[s->ltri[l]] & IDX_MASK);
+                    double pdf_l = (s->lw[l] / s->total) * d2 / (lc * 0.5 * area2);
+                    double f = sc / (M_PI * pdf_l);
+                    out[0] += T.x * alb.x * e[0] * f; out[1] += T.y * alb.y * e[1] * f; out[2] += T.z * alb.z * e[2] * f;
+                }
+            }
+        }
+        /* cosine-weighted bounce, throughput *= albedo */
+        double u1 = pcg_d(rng), u2 = pcg_d(rng);
+        double z = 1 - 2 * u2, r = sqrt(fmax(0.0, 1 - z * z)), th = 2 * M_PI * u1;
+        d3 sd = dadd(n, D(r * cos(th), r * sin(th), z));
+        if (ddot(sd, sd) < 1e-20) sd = n;
+        T = dmul(T, alb);
+        if (b + 1 >= 5) {
+            double mx = fmax(T.x, fmax(T.y, T.z));
+            if (mx <= 0) return;
+            double surv = fmin(fmax(mx, 0.05), 0.95);
+            if (pcg_d(rng) >= surv) return;
+            T = dsc(T, 1.0 / surv);
+        }
+        o = p; d = dnorm(sd);
+    }
+}
+
+typedef struct { const scene64 *s; uint64_t first, last; double *px; uint64_t seed; } job64;
+
+static void *worker64(void *p) {
+    job64 *j = (job64 *)p;
+    const orc_render_settings *st = &j->s->st;
+    for (uint64_t pix = j->first; pix < j->last; ++pix) {
+        double sum[3] = {0, 0, 0};
+        for (uint32_t k = 0; k < st->spp; ++k) {
+            uint64_t path = pix * st->spp + k;
+            pcg_t rng = {0, ((path ^ j->seed) << 1u) | 1u};
+            pcg_next(&rng); rng.state += 0x853c49e6748fea9bULL + j->seed; pcg_next(&rng);
+            double o[3];
+            render_path64(j->s, path, &rng, o);
+            sum[0] += o[0]; sum[1] += o[1]; sum[2] += o[2];
+        }
+        for (int c = 0; c < 3; ++c) j->px[3 * pix + c] = sum[c] / st->spp;
+    }
+    return NULL;
+}
+
+/* FP64 estimate of the PATH (0), NORMALS (1), AO (2) or NEE (3) image. */
+int orc_render_fp64(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
+                    const float *emissive, const orc_render_settings *st, const float *cam20, double *pixels,
+                    int threads) {
+    scene64 s = {vertices, n, surfaces, lambertian, emissive, NULL, NULL, 0, 0.0, *st, cam20};
+    s.lw = (double *)malloc((n ? n : 1) * sizeof(double));
+    s.ltri = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    for (uint64_t i = 0; i < n; ++i) {
+        if ((surfaces[i] >> 28) != MAT_EMIT) continue;
+        const float *t = vertices + 9 * i;
+        d3 p0 = D(t[0], t[1], t[2]);
+        d3 cr = dcross(dsub(D(t[3], t[4], t[5]), p0), dsub(D(t[6], t[7], t[8]), p0));
+        const float *e = emissive + 3 * (surfaces[i] & IDX_MASK);
+        double w = 0.5 * sqrt(ddot(cr, cr)) * fmax((e[0] + (double)e[1] + e[2]) / 3.0, 0.0);
+        if (w > 0) { s.lw[s.nl] = w; s.ltri[s.nl] = (uint32_t)i; s.total += w; s.nl++; }
+    }
+    uint64_t pixels_n = (uint64_t)st->width * st->height;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t tid[256];
+    job64 jobs[256];
+    for (int t = 0; t < threads; ++t) {
+        job64 j = {&s, pixels_n * (uint64_t)t / (uint64_t)threads, pixels_n * (uint64_t)(t + 1) / (uint64_t)threads, pixels, st->seed};
+        jobs[t] = j;
+        pthread_create(&tid[t], NULL, worker64, &jobs[t]);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+    free(s.lw); free(s.ltri);
+    return 0;
+}

@@ -15,6 +15,7 @@
 
 #include "orc_core.h"
 #include "oracle.h"
+#include "orc_internal.h"
 
 /* bvh/camera.mojo:86-106 (defocus angle 0). */
 void orc_camera_make(const float *origin3, const float *target3, const float *up3, float fov_scale,
@@ -58,8 +59,6 @@ void orc_camera_rays(const float *cams, uint64_t R, uint32_t w, uint32_t h, floa
     }
 }
 
-typedef struct { float u, v; uint32_t prim, inst; orc_v3 n; float t; } orc_hit;
-
 static orc_hit hit_miss(float t) { orc_hit h = {0, 0, ORC_EMPTY, ORC_EMPTY, {0, 0, 0}, t}; return h; }
 
 static void store_hit(float *hits, uint64_t r, orc_hit h) {
@@ -69,14 +68,6 @@ static void store_hit(float *hits, uint64_t r, orc_hit h) {
     memcpy(p + 3, &h.inst, 4);
     p[4] = h.n.x; p[5] = h.n.y; p[6] = h.n.z; p[7] = h.t;
 }
-
-typedef struct {
-    const float *vertices;   /* f32[9n], original order */
-    const uint32_t *leaf_ids;
-    const uint32_t *node_meta;
-    const float *node_bounds;
-    uint32_t root;
-} orc_bvh;
 
 /* One scaled-MT leaf test that updates the closest hit. */
 static int leaf_closest(const orc_bvh *b, uint32_t sorted_leaf, orc_v3 o, orc_v3 d, float t_min, orc_hit *hit) {
@@ -108,7 +99,7 @@ static int leaf_any(const orc_bvh *b, uint32_t sorted_leaf, orc_v3 o, orc_v3 d, 
 
 #define ORC_STACK 128
 
-static orc_hit trace_closest(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, float t_max, uint32_t *stats) {
+orc_hit orc_trace_closest(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, float t_max, uint32_t *stats) {
     orc_hit hit = hit_miss(t_max);
     if (b->root == ORC_SENTINEL) return hit;
     if (b->root & ORC_LEAF_FLAG) {
@@ -159,7 +150,7 @@ static orc_hit trace_closest(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, 
     return hit;
 }
 
-static orc_hit trace_any(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, float t_max) {
+orc_hit orc_trace_any(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, float t_max) {
     orc_hit occluded = {0, 0, ORC_EMPTY, ORC_EMPTY, {0, 0, 0}, 0.0f};
     if (b->root == ORC_SENTINEL) return hit_miss(t_max);
     if (b->root & ORC_LEAF_FLAG) return leaf_any(b, b->root & ORC_INDEX_MASK, o, d, t_min, t_max) ? occluded : hit_miss(t_max);
@@ -200,7 +191,7 @@ void orc_trace_bvh2(const float *vertices, const uint32_t *leaf_ids, const uint3
         float t_min = rays[3 * R + r], t_max = rays[7 * R + r];
         uint32_t *st = NULL;
         if (stats) { st = stats + 5 * r; memset(st, 0, 5 * sizeof(uint32_t)); }
-        orc_hit h = mode == 1 ? trace_any(&b, o, d, t_min, t_max) : trace_closest(&b, o, d, t_min, t_max, st);
+        orc_hit h = mode == 1 ? orc_trace_any(&b, o, d, t_min, t_max) : orc_trace_closest(&b, o, d, t_min, t_max, st);
         store_hit(hits, r, h);
     }
 }
@@ -268,7 +259,7 @@ static void *trace_worker(void *p) {
         orc_v3 o = orc_v(j->rays[0 * R + r], j->rays[1 * R + r], j->rays[2 * R + r]);
         orc_v3 d = orc_v(j->rays[4 * R + r], j->rays[5 * R + r], j->rays[6 * R + r]);
         float t_min = j->rays[3 * R + r], t_max = j->rays[7 * R + r];
-        orc_hit h = j->mode == 1 ? trace_any(&b, o, d, t_min, t_max) : trace_closest(&b, o, d, t_min, t_max, NULL);
+        orc_hit h = j->mode == 1 ? orc_trace_any(&b, o, d, t_min, t_max) : orc_trace_closest(&b, o, d, t_min, t_max, NULL);
         store_hit(j->hits, r, h);
     }
     return NULL;

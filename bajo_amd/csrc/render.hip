@@ -137,8 +137,23 @@ __global__ void init_counters_kernel(uint32_t* counters, uint32_t active, unsign
     for (uint32_t i = t; i < kStatsWords; i += blockDim.x) stats[i] = 0;
 }
 
-__global__ void advance_kernel(uint32_t* counters, unsigned long long* stats, uint32_t bounce, uint32_t cap) {
-  if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsigned long long* stats, uint32_t bounce,
+                                                      uint32_t cap, const unsigned long long* ext_work,
+                                                      uint32_t ext_blocks, const unsigned long long* sh_work,
+                                                      uint32_t sh_blocks) {
+  __shared__ unsigned long long s_sum[4][256];
+  const uint32_t t = threadIdx.x;
+  unsigned long long a = 0, b = 0, c = 0, d = 0;
+  for (uint32_t i = t; i < ext_blocks; i += blockDim.x) { a += ext_work[2 * i]; b += ext_work[2 * i + 1]; }
+  for (uint32_t i = t; i < sh_blocks; i += blockDim.x) { c += sh_work[2 * i]; d += sh_work[2 * i + 1]; }
+  s_sum[0][t] = a; s_sum[1][t] = b; s_sum[2][t] = c; s_sum[3][t] = d;
+  __syncthreads();
+  if (t != 0) return;
+  for (int k = 0; k < 4; ++k) {
+    unsigned long long x = 0;
+    for (uint32_t i = 0; i < blockDim.x; ++i) x += s_sum[k][i];
+    stats[kWorkBase + k] += x;
+  }
   if (bounce < kMaxBounces) {
     stats[2 * bounce + 0] += counters[kActive];
     stats[2 * bounce + 1] += counters[kShadow];
@@ -162,17 +177,41 @@ struct ShadowWork {
   float f[kShadowPlanes];
 };
 
-// Adds per-lane traversal work to two global totals, one atomic per wave.
-BJ_D void wave_add_work(unsigned long long* dst, uint64_t a, uint64_t b) {
+// Block-reduces per-lane traversal work and stores the block's two totals
+// (no atomics: the advance kernel sums the per-block partials).  Every
+// thread of the block must call it.
+BJ_D void store_block_work(unsigned long long* partials, uint64_t a, uint64_t b) {
+  __shared__ unsigned long long s_work[kRtBlock / 64][2];
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     a += __shfl_xor(a, off, 64);
     b += __shfl_xor(b, off, 64);
   }
+  const uint32_t wave = threadIdx.x >> 6;
   if (lane_id() == 0) {
-    atomicAdd(dst + 0, (unsigned long long)a);
-    atomicAdd(dst + 1, (unsigned long long)b);
+    s_work[wave][0] = a;
+    s_work[wave][1] = b;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long x = 0, y = 0;
+    for (int w = 0; w < kRtBlock / 64; ++w) {
+      x += s_work[w][0];
+      y += s_work[w][1];
+    }
+    partials[2 * blockIdx.x + 0] = x;
+    partials[2 * blockIdx.x + 1] = y;
+  }
+}
+
+// Paths per fetch: one batch of 64 per atomic while the queue is short, up
+// to eight batches when every wave would otherwise fetch more than eight
+// times (fewer same-address atomics, still balanced at the tail).
+BJ_D uint32_t fetch_batch(uint32_t count) {
+  const uint32_t waves = gridDim.x * (kRtBlock / 64);
+  uint32_t k = count / (waves * 64u * 8u);
+  k = k < 1u ? 1u : (k > 8u ? 8u : k);
+  return 64u * k;
 }
 
 // Wave-aggregated append: one atomic per wave, slots in lane order.
@@ -250,12 +289,18 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
   const uint64_t cap = q.capacity;
   const uint32_t active = q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap;
   uint64_t work_visits = 0, work_leaves = 0;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(q.counters + kFetchExt, 64u);
-    base = __shfl(base, 0, 64);
-    if (base >= active) break;
-    const uint32_t idx = base + lane;
+  // blocks past the live work exit without touching the shared counters
+  const bool has_work = (uint64_t)blockIdx.x * kRtBlock < active;
+  const uint32_t batch = fetch_batch(active);
+  for (uint32_t base = 0, r = batch; has_work;) {
+    if (r >= batch) {
+      if (lane == 0) base = atomicAdd(q.counters + kFetchExt, batch);
+      base = __shfl(base, 0, 64);
+      r = 0;
+    }
+    if (base + r >= active) break;
+    const uint32_t idx = base + r + lane;
+    r += 64;
     bool want_next = false, want_shadow = false;
     NextPath np;
     ShadowWork sw;
@@ -385,7 +430,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
       }
     }
   }
-  wave_add_work(work, work_visits, work_leaves);
+  store_block_work(work, work_visits, work_leaves);
 }
 
 template <int kAlg>
@@ -398,12 +443,17 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
   const uint64_t cap = q.capacity;
   const uint32_t count = q.counters[kShadow] < cap ? q.counters[kShadow] : (uint32_t)cap;
   uint64_t work_visits = 0, work_leaves = 0;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(q.counters + kFetchShadow, 64u);
-    base = __shfl(base, 0, 64);
-    if (base >= count) break;
-    const uint32_t idx = base + lane;
+  const bool has_work = (uint64_t)blockIdx.x * kRtBlock < count;
+  const uint32_t batch = fetch_batch(count);
+  for (uint32_t base = 0, r = batch; has_work;) {
+    if (r >= batch) {
+      if (lane == 0) base = atomicAdd(q.counters + kFetchShadow, batch);
+      base = __shfl(base, 0, 64);
+      r = 0;
+    }
+    if (base + r >= count) break;
+    const uint32_t idx = base + r + lane;
+    r += 64;
     if (idx >= count) continue;
     float f[kShadowPlanes];
 #pragma unroll
@@ -422,7 +472,7 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
       if (!occluded) accumulate(q.sample, cap, slot, mk(f[8], f[9], f[10]));
     }
   }
-  wave_add_work(work, work_visits, work_leaves);
+  store_block_work(work, work_visits, work_leaves);
 }
 
 __global__ void __launch_bounds__(kRtBlock) resolve_kernel(const float* __restrict__ sample, float* __restrict__ pixels,
@@ -470,6 +520,8 @@ struct bajo_scene {
   unsigned long long* stats = nullptr;
   uint32_t* spill = nullptr;
   uint64_t spill_threads = 0;
+  unsigned long long* work_ext = nullptr;  // per-block traversal work partials
+  unsigned long long* work_sh = nullptr;
 };
 
 namespace {
@@ -478,6 +530,9 @@ void free_arena(bajo_scene* s) {
   hipFree(s->ids[0]); hipFree(s->ids[1]); hipFree(s->fields[0]); hipFree(s->fields[1]);
   hipFree(s->shadow_ids); hipFree(s->shadow_fields); hipFree(s->sample); hipFree(s->counters); hipFree(s->stats);
   hipFree(s->spill);
+  hipFree(s->work_ext);
+  hipFree(s->work_sh);
+  s->work_ext = s->work_sh = nullptr;
   s->ids[0] = s->ids[1] = nullptr;
   s->fields[0] = s->fields[1] = nullptr;
   s->shadow_ids = nullptr; s->shadow_fields = nullptr; s->sample = nullptr; s->counters = nullptr;
@@ -518,6 +573,8 @@ int32_t ensure_arena(bajo_scene* s, uint64_t cap, uint64_t spill_threads) {
   BJ_HIP(hipMalloc(&s->counters, kCounterWords * sizeof(uint32_t)));
   BJ_HIP(hipMalloc(&s->stats, kStatsWords * sizeof(unsigned long long)));
   BJ_HIP(hipMalloc(&s->spill, spill_threads * kSpill * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&s->work_ext, 2 * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
+  BJ_HIP(hipMalloc(&s->work_sh, 2 * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
   s->cap = cap;
   s->spill_threads = spill_threads;
   return BAJO_OK;
@@ -616,16 +673,18 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
       Queues q{s->ids[b & 1], s->fields[b & 1], s->ids[(b + 1) & 1], s->fields[(b + 1) & 1], s->shadow_ids,
                s->shadow_fields, s->sample, s->counters, cap, (uint32_t)begin};
       if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
-      extend_kernel<kAlg><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, s->spill, s->stats + kWorkBase);
+      extend_kernel<kAlg><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, s->spill, s->work_ext);
       BJ_HIP(hipGetLastError());
       if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
         if (ev) { ev->sh.push_back(ev->make()); hipEventRecord(ev->sh.back(), stream); }
-        shadow_kernel<kAlg><<<g_sh, kRtBlock, 0, stream>>>(sv, q, s->spill, s->stats + kWorkBase + 2);
+        shadow_kernel<kAlg><<<g_sh, kRtBlock, 0, stream>>>(sv, q, s->spill, s->work_sh);
         BJ_HIP(hipGetLastError());
         if (ev) { ev->sh.push_back(ev->make()); hipEventRecord(ev->sh.back(), stream); }
       }
-      advance_kernel<<<1, 64, 0, stream>>>(s->counters, s->stats, b, (uint32_t)cap);
+      const bool traced_shadows = kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS;
+      advance_kernel<<<1, 256, 0, stream>>>(s->counters, s->stats, b, (uint32_t)cap, s->work_ext, g_ext, s->work_sh,
+                                            traced_shadows ? g_sh : 0u);
     }
     const uint32_t pixel_count = chunk / st->samples_per_pixel;
     resolve_kernel<<<div_up(pixel_count, kRtBlock), kRtBlock, 0, stream>>>(

@@ -52,12 +52,20 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const BjNode* __rest
   Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
   const BvhView bv{nodes, tris, root, node_count, prim_count, status};
 
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(counter, 64u);
-    base = __shfl(base, 0, 64);
-    if ((uint64_t)base >= R) break;
-    const uint64_t r = (uint64_t)base + lane;
+  // Batches of 64 rays per atomic while the launch is short; up to eight
+  // batches per fetch when every wave would otherwise fetch 8+ times.
+  const uint64_t waves = (uint64_t)gridDim.x * (kTraceBlock / 64);
+  uint64_t kb = R / (waves * 64u * 8u);
+  const uint32_t batch = 64u * (uint32_t)(kb < 1 ? 1 : (kb > 8 ? 8 : kb));
+  for (uint32_t base = 0, off = batch;;) {
+    if (off >= batch) {
+      if (lane == 0) base = atomicAdd(counter, batch);
+      base = __shfl(base, 0, 64);
+      off = 0;
+    }
+    if ((uint64_t)base + off >= R) break;
+    const uint64_t r = (uint64_t)base + off + lane;
+    off += 64;
     if (r >= R) continue;
 
     Ray ray;
@@ -139,7 +147,7 @@ int32_t bajo_trace_rays(bajo_ctx* ctx, const bajo_bvh* bvh, int32_t mode, const 
   BJ_REQUIRE(ctx != nullptr && bvh != nullptr, "bajo_trace_rays: NULL ctx/bvh");
   BJ_REQUIRE(mode == BAJO_TRACE_CLOSEST_HIT || mode == BAJO_TRACE_ANY_HIT, "bajo_trace_rays: bad mode %d", mode);
   BJ_REQUIRE(ray_count > 0, "bajo_trace_rays: ray count must be positive");
-  BJ_REQUIRE(ray_count < 0xFFFFFF00ull, "bajo_trace_rays: ray count exceeds 2^32-256");
+  BJ_REQUIRE(ray_count < (1ull << 31), "bajo_trace_rays: ray count must be below 2^31");
   BJ_REQUIRE(rays != nullptr && hits != nullptr, "bajo_trace_rays: NULL rays/hits");
   BJ_REQUIRE(bvh->n > 0, "bajo_trace_rays: empty BVH");
   if (mode == BAJO_TRACE_ANY_HIT) {
@@ -154,7 +162,7 @@ int32_t bajo_trace_camera(bajo_ctx* ctx, const bajo_bvh* bvh, const float* camer
                           uint32_t width, uint32_t height, uint32_t* stats) {
   BJ_REQUIRE(ctx != nullptr && bvh != nullptr, "bajo_trace_camera: NULL ctx/bvh");
   BJ_REQUIRE(ray_count > 0 && width > 0 && height > 0, "bajo_trace_camera: launch dimensions must be positive");
-  BJ_REQUIRE(ray_count < 0xFFFFFF00ull, "bajo_trace_camera: ray count exceeds 2^32-256");
+  BJ_REQUIRE(ray_count < (1ull << 31), "bajo_trace_camera: ray count must be below 2^31");
   BJ_REQUIRE(camera != nullptr && hits != nullptr, "bajo_trace_camera: NULL camera/hits");
   BJ_REQUIRE(bvh->n > 0, "bajo_trace_camera: empty BVH");
   if (stats)

@@ -16,6 +16,8 @@ BAJO_OK = 0
 BAJO_TRACE_CLOSEST_HIT = 0
 BAJO_TRACE_ANY_HIT = 1
 BAJO_BUILD_LBVH = 0
+BAJO_WIDTH_BINARY = 2
+BAJO_WIDTH_WIDE8 = 8
 
 
 class BajoError(RuntimeError):
@@ -33,6 +35,7 @@ class BuildTimings(ctypes.Structure):
         ("sort_ns", c_uint64),
         ("topology_ns", c_uint64),
         ("refit_ns", c_uint64),
+        ("collapse_ns", c_uint64),
         ("total_ns", c_uint64),
     ]
 
@@ -45,13 +48,24 @@ class BvhInfo(ctypes.Structure):
         ("prim_count", c_uint64),
         ("node_count", c_uint64),
         ("root", c_uint32),
-        ("reserved", c_uint32),
+        ("node_width", c_uint32),
         ("nodes", c_void_p),
         ("triangles", c_void_p),
         ("sorted_keys", c_void_p),
         ("leaf_parent", c_void_p),
         ("leaf_counts", c_void_p),
         ("scene_bounds", c_void_p),
+        ("wide_node_count", c_uint64),
+        ("wide_nodes", c_void_p),
+    ]
+
+
+class BuildOptions(ctypes.Structure):
+    _fields_ = [
+        ("method", c_int32),
+        ("node_width", c_int32),
+        ("leaf_size", c_int32),
+        ("reserved", c_int32),
     ]
 
 
@@ -112,12 +126,17 @@ SIGNATURES = {
         c_int32,
         [c_void_p, c_void_p, c_uint64, c_int32, POINTER(BuildTimings), POINTER(c_void_p)],
     ),
+    "bajo_build_triangle_bvh_ex": (
+        c_int32,
+        [c_void_p, c_void_p, c_uint64, POINTER(BuildOptions), POINTER(BuildTimings), POINTER(c_void_p)],
+    ),
     "bajo_bvh_free": (c_int32, [c_void_p]),
     "bajo_bvh_get_info": (c_int32, [c_void_p, POINTER(BvhInfo)]),
     "bajo_bvh_copy_binary": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
+    "bajo_bvh_copy_wide": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, POINTER(c_uint64)]),
     "bajo_trace_rays": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_uint64, c_void_p]),
     "bajo_trace_camera": (
         c_int32,
@@ -130,6 +149,13 @@ SIGNATURES = {
     "bajo_scene_create": (
         c_int32,
         [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, POINTER(c_void_p)],
+    ),
+    "bajo_scene_create_ex": (
+        c_int32,
+        [
+            c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_uint32, c_void_p, c_uint32,
+            POINTER(BuildOptions), POINTER(c_void_p),
+        ],
     ),
     "bajo_scene_rebuild": (c_int32, [c_void_p, c_void_p, POINTER(BuildTimings)]),
     "bajo_scene_free": (c_int32, [c_void_p]),

@@ -39,6 +39,26 @@ class GpuBvhBuildMethod:
     LBVH = _lib.BAJO_BUILD_LBVH
 
 
+# One 8-wide node (96 bytes): child i spans origin + [lo, hi] * 2^(exp - 127)
+# per axis; child bit 31 = leaf of ((child >> 28) & 7) + 1 sorted triangles
+# starting at child & 0x0FFFFFFF, else a wide node index.
+WIDE_NODE_DTYPE = np.dtype(
+    [
+        ("origin", "<f4", 3),
+        ("exp", "u1", 3),
+        ("count", "u1"),
+        ("lo", "u1", (3, 8)),
+        ("hi", "u1", (3, 8)),
+        ("child", "<u4", 8),
+    ]
+)
+assert WIDE_NODE_DTYPE.itemsize == 96
+
+
+def build_options(node_width: int = 2, leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH):
+    return _lib.BuildOptions(int(method), int(node_width), int(leaf_size), 0)
+
+
 @dataclass
 class Camera:
     """bajo's pinhole camera (bvh/camera.mojo:86-106), flattened to 20 floats."""
@@ -63,10 +83,13 @@ class Camera:
 class GpuTriangleBvh:
     """Ready-to-trace triangle BVH (bvh/gpu/triangle_bvh.mojo:494-614)."""
 
-    def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, prim_count: int):
+    def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, prim_count: int, node_width: int = 2,
+                 leaf_size: int = 0):
         self._handle = handle
         self.ctx = ctx
         self.prim_count = prim_count
+        self.node_width = node_width
+        self.leaf_size = leaf_size
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -84,9 +107,10 @@ class GpuTriangleBvh:
         n = _check_vertices(vertices, self.ctx)
         t = _lib.BuildTimings() if timings else None
         handle = ctypes.c_void_p(self.handle.value)
+        opts = build_options(self.node_width, self.leaf_size)
         _lib.check(
-            _lib.lib().bajo_build_triangle_bvh(
-                self.ctx.handle, ptr(vertices), n, GpuBvhBuildMethod.LBVH, ctypes.byref(t) if t else None,
+            _lib.lib().bajo_build_triangle_bvh_ex(
+                self.ctx.handle, ptr(vertices), n, ctypes.byref(opts), ctypes.byref(t) if t else None,
                 ctypes.byref(handle),
             )
         )
@@ -154,6 +178,18 @@ class GpuTriangleBvh:
         out["root"] = self.info().root
         return out
 
+    def wide(self) -> np.ndarray:
+        """Host copy of the 8-wide nodes (WIDE_NODE_DTYPE records, root = record 0)."""
+        count = ctypes.c_uint64()
+        _lib.check(_lib.lib().bajo_bvh_copy_wide(self.ctx.handle, self.handle, None, 0, ctypes.byref(count)))
+        out = np.empty(count.value, WIDE_NODE_DTYPE)
+        _lib.check(
+            _lib.lib().bajo_bvh_copy_wide(
+                self.ctx.handle, self.handle, out.ctypes.data_as(ctypes.c_void_p), count.value, ctypes.byref(count)
+            )
+        )
+        return out
+
     def free(self) -> None:
         if self._handle:
             _lib.lib().bajo_bvh_free(self._handle)
@@ -178,24 +214,32 @@ def _check_vertices(vertices: torch.Tensor, ctx: DeviceContext) -> int:
     return n
 
 
-def build_triangle_bvh(vertices: torch.Tensor, ctx: DeviceContext | None = None, timings: dict | None = None):
-    """build_triangle_bvh[WORLD, 2, 1, LBVH] (bvh/gpu/triangle_bvh.mojo:617-630)."""
+def build_triangle_bvh(
+    vertices: torch.Tensor, ctx: DeviceContext | None = None, timings: dict | None = None, node_width: int = 2,
+    leaf_size: int = 0,
+):
+    """build_triangle_bvh[WORLD, W, L, LBVH] (bvh/gpu/triangle_bvh.mojo:617-630).
+
+    node_width 2 keeps the binary LBVH; 8 collapses it into quantized 8-wide
+    nodes whose leaf children hold up to `leaf_size` triangles (0 = 4)."""
     ctx = ctx or default_context(vertices.device.index or 0)
     n = _check_vertices(vertices, ctx)
     t = _lib.BuildTimings() if timings is not None else None
     handle = ctypes.c_void_p()
+    opts = build_options(node_width, leaf_size)
     _lib.check(
-        _lib.lib().bajo_build_triangle_bvh(
-            ctx.handle, ptr(vertices), n, GpuBvhBuildMethod.LBVH, ctypes.byref(t) if t else None, ctypes.byref(handle)
+        _lib.lib().bajo_build_triangle_bvh_ex(
+            ctx.handle, ptr(vertices), n, ctypes.byref(opts), ctypes.byref(t) if t else None, ctypes.byref(handle)
         )
     )
     if t is not None:
         timings.update(t.as_dict())
-    return GpuTriangleBvh(handle, ctx, n)
+    return GpuTriangleBvh(handle, ctx, n, node_width, leaf_size)
 
 
-def build_triangle_bvh_measured(vertices: torch.Tensor, ctx: DeviceContext | None = None):
+def build_triangle_bvh_measured(vertices: torch.Tensor, ctx: DeviceContext | None = None, node_width: int = 2,
+                                leaf_size: int = 0):
     """build_triangle_bvh_measured (bvh/gpu/triangle_bvh.mojo:633-646): returns (bvh, timings)."""
     t: dict = {}
-    bvh = build_triangle_bvh(vertices, ctx=ctx, timings=t)
+    bvh = build_triangle_bvh(vertices, ctx=ctx, timings=t, node_width=node_width, leaf_size=leaf_size)
     return bvh, t

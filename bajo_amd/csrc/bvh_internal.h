@@ -22,6 +22,25 @@ struct __attribute__((aligned(16))) BjTri {
 };
 static_assert(sizeof(BjTri) == 48, "BjTri must be 48 bytes");
 
+// 96-byte 8-wide node with quantized child boxes.  Child i spans
+// [o + lo_i * 2^e, o + hi_i * 2^e] per axis (one spare quantization step of
+// padding on each side, so the boxes stay conservative under the traversal's
+// fused dequantization).  child[i]: bit 31 = leaf; a leaf holds sorted
+// triangles [start, start + count) with start in bits 0..27 and count - 1 in
+// bits 28..30; an inner child holds a wide node index.
+struct __attribute__((aligned(16))) BjWide {
+  float ox, oy, oz;
+  uint8_t ex, ey, ez, n;  // biased FP32 exponents of the scales, child count
+  uint8_t lo_x[8], lo_y[8], lo_z[8], hi_x[8], hi_y[8], hi_z[8];
+  uint32_t child[8];
+};
+static_assert(sizeof(BjWide) == 96, "BjWide must be 96 bytes");
+
+constexpr uint32_t kWideLeafStartMask = 0x0FFFFFFFu;
+constexpr uint32_t kWideMaxLeaf = 8;      // largest leaf child the encoding holds
+constexpr uint32_t kWideDefaultLeaf = 4;  // triangles per leaf child ("fat leaf")
+constexpr uint64_t kWideMaxPrims = 1ull << 28;
+
 struct bajo_bvh {
   int device;
   uint64_t n;         // triangles
@@ -37,4 +56,18 @@ struct bajo_bvh {
   float* scene;           // 12 floats
   void* sort_ws;
   size_t sort_ws_bytes;
+  // 8-wide layout (node_width == 8): collapsed from the binary LBVH
+  int node_width;
+  BjWide* wide;
+  uint32_t* wide_src;   // binary node each wide node was opened from
+  uint32_t* wide_ctl;   // wide counter, level begin/end, status
+  uint32_t* host_ctl;   // pinned host mirror of wide_ctl
+  uint64_t wide_capacity;
+  uint32_t wide_count;
+  uint32_t wide_leaf;  // largest leaf child, 1..kWideMaxLeaf
 };
+
+namespace bj {
+// Collapses h's binary LBVH into h->wide (8-wide quantized nodes); wide.hip.
+int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h);
+}  // namespace bj

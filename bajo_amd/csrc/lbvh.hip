@@ -295,15 +295,36 @@ int32_t ensure_capacity(bajo_bvh* h, uint64_t n) {
   return BAJO_OK;
 }
 
+void free_wide(bajo_bvh* h) {
+  hipFree(h->wide); hipFree(h->wide_src); hipFree(h->wide_ctl); hipHostFree(h->host_ctl);
+  h->wide = nullptr; h->wide_src = nullptr; h->wide_ctl = nullptr; h->host_ctl = nullptr;
+  h->wide_capacity = 0;
+  h->wide_count = 0;
+}
+
+// Wide nodes: one per binary node holding more than the leaf size, plus the
+// root, so n nodes always suffice.
+int32_t ensure_wide_capacity(bajo_bvh* h, uint64_t n) {
+  if (n <= h->wide_capacity) return BAJO_OK;
+  free_wide(h);
+  BJ_HIP(hipMalloc(&h->wide, n * sizeof(BjWide)));
+  BJ_HIP(hipMalloc(&h->wide_src, n * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&h->wide_ctl, 16 * sizeof(uint32_t)));
+  BJ_HIP(hipHostMalloc(&h->host_ctl, 16 * sizeof(uint32_t), hipHostMallocDefault));
+  h->wide_capacity = n;
+  return BAJO_OK;
+}
+
 void free_bvh(bajo_bvh* h) {
   if (!h) return;
   hipFree(h->nodes); hipFree(h->tris); hipFree(h->keys); hipFree(h->leaf_parent); hipFree(h->leaf_counts);
   hipFree(h->flags); hipFree(h->acc); hipFree(h->scene); hipFree(h->sort_ws);
+  free_wide(h);
   delete h;
 }
 
 struct StageEvents {
-  hipEvent_t ev[6] = {};
+  hipEvent_t ev[7] = {};
   bool on = false;
   int32_t create(bool enable) {
     on = enable;
@@ -323,6 +344,10 @@ struct StageEvents {
 int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n, bajo_build_timings* timings) {
   int32_t rc = ensure_capacity(h, n);
   if (rc) return rc;
+  if (h->node_width == BAJO_WIDTH_WIDE8) {
+    rc = ensure_wide_capacity(h, n);
+    if (rc) return rc;
+  }
   h->n = n;
   h->root = n == 1 ? (kLeafFlag | 0u) : 0u;
   SortPlan plan = make_sort_plan(n, 8, 0, kSortBeginBit, kSortEndBit);
@@ -362,17 +387,25 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
                                                      h->tris);
   BJ_HIP(hipGetLastError());
   ev.record(5, st);
+  if (h->node_width == BAJO_WIDTH_WIDE8) {
+    rc = collapse_wide(ctx, h);
+    if (rc) return rc;
+  } else {
+    h->wide_count = 0;
+  }
+  ev.record(6, st);
   if (timings) {
-    BJ_HIP(hipEventSynchronize(ev.ev[5]));
-    float ms[5];
-    for (int k = 0; k < 5; ++k) BJ_HIP(hipEventElapsedTime(&ms[k], ev.ev[k], ev.ev[k + 1]));
+    BJ_HIP(hipEventSynchronize(ev.ev[6]));
+    float ms[6];
+    for (int k = 0; k < 6; ++k) BJ_HIP(hipEventElapsedTime(&ms[k], ev.ev[k], ev.ev[k + 1]));
     timings->bounds_ns = (uint64_t)(ms[0] * 1e6);
     timings->morton_ns = (uint64_t)(ms[1] * 1e6);
     timings->sort_ns = (uint64_t)(ms[2] * 1e6);
     timings->topology_ns = (uint64_t)(ms[3] * 1e6);
     timings->refit_ns = (uint64_t)(ms[4] * 1e6);
+    timings->collapse_ns = (uint64_t)(ms[5] * 1e6);
     float total;
-    BJ_HIP(hipEventElapsedTime(&total, ev.ev[0], ev.ev[5]));
+    BJ_HIP(hipEventElapsedTime(&total, ev.ev[0], ev.ev[6]));
     timings->total_ns = (uint64_t)(total * 1e6);
   }
   return BAJO_OK;
@@ -385,12 +418,28 @@ extern "C" {
 
 int32_t bajo_build_triangle_bvh(bajo_ctx* ctx, const float* vertices, uint64_t n, int32_t method,
                                 bajo_build_timings* timings, bajo_bvh** out) {
+  bajo_build_options opt{method, BAJO_WIDTH_BINARY, 0, 0};
+  return bajo_build_triangle_bvh_ex(ctx, vertices, n, &opt, timings, out);
+}
+
+int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_t n,
+                                   const bajo_build_options* options, bajo_build_timings* timings,
+                                   bajo_bvh** out) {
+  bajo_build_options opt = options ? *options : bajo_build_options{BAJO_BUILD_LBVH, BAJO_WIDTH_BINARY, 0, 0};
   BJ_REQUIRE(ctx != nullptr && out != nullptr, "bajo_build_triangle_bvh: ctx/out is NULL");
-  BJ_REQUIRE(method == BAJO_BUILD_LBVH, "bajo_build_triangle_bvh: unsupported build method %d", method);
+  BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH, "bajo_build_triangle_bvh: unsupported build method %d", opt.method);
+  BJ_REQUIRE(opt.node_width == BAJO_WIDTH_BINARY || opt.node_width == BAJO_WIDTH_WIDE8,
+             "bajo_build_triangle_bvh: node_width must be 2 or 8, got %d", opt.node_width);
+  if (opt.leaf_size == 0) opt.leaf_size = (int32_t)kWideDefaultLeaf;
+  BJ_REQUIRE(opt.leaf_size >= 1 && opt.leaf_size <= (int32_t)kWideMaxLeaf,
+             "bajo_build_triangle_bvh: leaf_size must be in [1, 8], got %d", opt.leaf_size);
+  BJ_REQUIRE(opt.reserved == 0, "bajo_build_triangle_bvh: reserved option field must be 0");
   BJ_REQUIRE(n > 0, "bajo_build_triangle_bvh: standalone triangle BVH requires nonempty input");
   BJ_REQUIRE(vertices != nullptr, "bajo_build_triangle_bvh: vertices is NULL");
   BJ_REQUIRE(n <= bj::kSortMaxKeys, "bajo_build_triangle_bvh: n=%llu exceeds 2^30-1 triangles",
              (unsigned long long)n);
+  BJ_REQUIRE(opt.node_width != BAJO_WIDTH_WIDE8 || n < kWideMaxPrims,
+             "bajo_build_triangle_bvh: n=%llu exceeds the 8-wide limit of 2^28-1 triangles", (unsigned long long)n);
   bajo_bvh* h = *out;
   bool fresh = h == nullptr;
   if (fresh) {
@@ -400,6 +449,8 @@ int32_t bajo_build_triangle_bvh(bajo_ctx* ctx, const float* vertices, uint64_t n
   } else {
     BJ_REQUIRE(h->device == ctx->device, "bajo_build_triangle_bvh: handle belongs to device %d", h->device);
   }
+  h->node_width = opt.node_width;
+  h->wide_leaf = (uint32_t)opt.leaf_size;
   int32_t rc = bj::build_into(ctx, h, vertices, n, timings);
   if (rc) {
     if (fresh) bj::free_bvh(h);
@@ -419,13 +470,28 @@ int32_t bajo_bvh_get_info(const bajo_bvh* h, bajo_bvh_info* out) {
   out->prim_count = h->n;
   out->node_count = h->n > 0 ? h->n - 1 : 0;
   out->root = h->root;
-  out->reserved = 0;
+  out->node_width = (uint32_t)h->node_width;
   out->nodes = h->nodes;
   out->triangles = h->tris;
   out->sorted_keys = h->keys;
   out->leaf_parent = h->leaf_parent;
   out->leaf_counts = h->leaf_counts;
   out->scene_bounds = h->scene;
+  out->wide_node_count = h->node_width == BAJO_WIDTH_WIDE8 ? h->wide_count : 0;
+  out->wide_nodes = h->node_width == BAJO_WIDTH_WIDE8 ? h->wide : nullptr;
+  return BAJO_OK;
+}
+
+int32_t bajo_bvh_copy_wide(bajo_ctx* ctx, const bajo_bvh* h, void* nodes, uint64_t capacity, uint64_t* count) {
+  BJ_REQUIRE(ctx != nullptr && h != nullptr, "bajo_bvh_copy_wide: NULL argument");
+  BJ_REQUIRE(h->node_width == BAJO_WIDTH_WIDE8, "bajo_bvh_copy_wide: the BVH was built with node width %d",
+             h->node_width);
+  if (count) *count = h->wide_count;
+  if (!nodes) return BAJO_OK;
+  BJ_REQUIRE(capacity >= h->wide_count, "bajo_bvh_copy_wide: capacity %llu < %u nodes",
+             (unsigned long long)capacity, h->wide_count);
+  BJ_HIP(hipStreamSynchronize(ctx->stream));
+  BJ_HIP(hipMemcpy(nodes, h->wide, (uint64_t)h->wide_count * sizeof(BjWide), hipMemcpyDeviceToHost));
   return BAJO_OK;
 }
 

@@ -47,7 +47,7 @@ constexpr int kWorkBase = 2 * kMaxBounces;
 constexpr int kStatsWords = kWorkBase + 4;
 
 struct SceneView {
-  BvhView bvh;
+  AccelView bvh;
   const uint32_t* surfaces;
   const float* lambertian;
   const float* emissive;
@@ -278,7 +278,7 @@ BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uin
   return true;
 }
 
-template <int kAlg>
+template <int kAlg, bool kWide>
 __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                           uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work) {
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
       const float path_pdf = f[11];
       const uint32_t slot = path_id - q.sample_base;
       TraceCounters tc;
-      const Hit h = traverse<BAJO_TRACE_CLOSEST_HIT, true>(sc.bvh, ray, st, &tc);
+      const Hit h = trace_accel<BAJO_TRACE_CLOSEST_HIT, true, kWide>(sc.bvh, ray, st, &tc);
       work_visits += tc.visits;
       work_leaves += tc.leaves;
       const bool found = h.prim != kEmpty && h.t < ray.tmax;
@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
   store_block_work(work, work_visits, work_leaves);
 }
 
-template <int kAlg>
+template <int kAlg, bool kWide>
 __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work) {
   __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
     const uint32_t slot = q.shadow_ids[idx] - q.sample_base;
     const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
     TraceCounters tc;
-    const Hit h = traverse<BAJO_TRACE_ANY_HIT, true>(sc.bvh, ray, st, &tc);
+    const Hit h = trace_accel<BAJO_TRACE_ANY_HIT, true, kWide>(sc.bvh, ray, st, &tc);
     work_visits += tc.visits;
     work_leaves += tc.leaves;
     const bool occluded = h.prim == kEmpty && h.t == 0.0f;
@@ -502,6 +502,7 @@ struct bajo_scene {
   uint64_t n = 0;
   float* vertices = nullptr;
   bajo_bvh* bvh = nullptr;
+  bajo_build_options options{};
   uint32_t* surfaces = nullptr;
   float* lambertian = nullptr;
   float* emissive = nullptr;
@@ -625,7 +626,7 @@ struct LaunchEvents {
   }
 };
 
-template <int kAlg>
+template <int kAlg, bool kWide>
 int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st, const float* camera, float* pixels,
                    LaunchEvents* ev) {
   const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
@@ -634,9 +635,9 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   cap = (cap / st->samples_per_pixel) * st->samples_per_pixel;
   if (cap == 0) cap = st->samples_per_pixel;
   unsigned g_ext = 0, g_sh = 0;
-  int32_t rc = grid_of(extend_kernel<kAlg>, &g_ext);
+  int32_t rc = grid_of(extend_kernel<kAlg, kWide>, &g_ext);
   if (rc) return rc;
-  rc = grid_of(shadow_kernel<kAlg>, &g_sh);
+  rc = grid_of(shadow_kernel<kAlg, kWide>, &g_sh);
   if (rc) return rc;
   const unsigned need = div_up(cap, kRtBlock);
   if (g_ext > need) g_ext = need;
@@ -649,11 +650,8 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   hipStream_t stream = ctx->stream;
   Cam cam;
   memcpy(cam.v, camera, sizeof(cam.v));
-  bajo_bvh_info info;
-  bajo_bvh_get_info(s->bvh, &info);
   SceneView sv;
-  sv.bvh = BvhView{(const BjNode*)info.nodes, (const BjTri*)info.triangles, info.root, (uint32_t)info.node_count,
-                   (uint32_t)info.prim_count, s->counters + 8};
+  sv.bvh = accel_view(s->bvh, s->counters + 8);
   sv.surfaces = s->surfaces;
   sv.lambertian = s->lambertian;
   sv.emissive = s->emissive;
@@ -673,12 +671,12 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
       Queues q{s->ids[b & 1], s->fields[b & 1], s->ids[(b + 1) & 1], s->fields[(b + 1) & 1], s->shadow_ids,
                s->shadow_fields, s->sample, s->counters, cap, (uint32_t)begin};
       if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
-      extend_kernel<kAlg><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, s->spill, s->work_ext);
+      extend_kernel<kAlg, kWide><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, s->spill, s->work_ext);
       BJ_HIP(hipGetLastError());
       if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
         if (ev) { ev->sh.push_back(ev->make()); hipEventRecord(ev->sh.back(), stream); }
-        shadow_kernel<kAlg><<<g_sh, kRtBlock, 0, stream>>>(sv, q, s->spill, s->work_sh);
+        shadow_kernel<kAlg, kWide><<<g_sh, kRtBlock, 0, stream>>>(sv, q, s->spill, s->work_sh);
         BJ_HIP(hipGetLastError());
         if (ev) { ev->sh.push_back(ev->make()); hipEventRecord(ev->sh.back(), stream); }
       }
@@ -701,6 +699,13 @@ extern "C" {
 int32_t bajo_scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
                           const float* lambertian, uint32_t n_lambertian, const float* emissive, uint32_t n_emissive,
                           bajo_scene** out) {
+  return bajo_scene_create_ex(ctx, vertices, n, surfaces, lambertian, n_lambertian, emissive, n_emissive, nullptr,
+                              out);
+}
+
+int32_t bajo_scene_create_ex(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
+                             const float* lambertian, uint32_t n_lambertian, const float* emissive,
+                             uint32_t n_emissive, const bajo_build_options* options, bajo_scene** out) {
   BJ_REQUIRE(ctx && out, "bajo_scene_create: NULL ctx/out");
   BJ_REQUIRE(n > 0 && vertices && surfaces, "bajo_scene_create: a scene needs triangles and their surfaces");
   BJ_REQUIRE(n < (1ull << 31), "bajo_scene_create: too many triangles");
@@ -715,6 +720,7 @@ int32_t bajo_scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, cons
   bajo_scene* s = new bajo_scene();
   s->device = ctx->device;
   s->n = n;
+  s->options = options ? *options : bajo_build_options{BAJO_BUILD_LBVH, BAJO_WIDTH_BINARY, 0, 0};
   auto fail = [&](int32_t rc) { free_scene(s); return rc; };
 #define SC_HIP(call)                                              \
   do {                                                            \
@@ -771,7 +777,7 @@ int32_t bajo_scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, cons
   SC_HIP(hipMalloc(&s->light_kinds, kinds.size() * sizeof(uint32_t)));
   SC_HIP(hipMemcpy(s->light_kinds, kinds.data(), kinds.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
 #undef SC_HIP
-  int32_t rc = bajo_build_triangle_bvh(ctx, s->vertices, n, BAJO_BUILD_LBVH, nullptr, &s->bvh);
+  int32_t rc = bajo_build_triangle_bvh_ex(ctx, s->vertices, n, &s->options, nullptr, &s->bvh);
   if (rc) return fail(rc);
   *out = s;
   return BAJO_OK;
@@ -779,7 +785,7 @@ int32_t bajo_scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, cons
 
 int32_t bajo_scene_rebuild(bajo_ctx* ctx, bajo_scene* s, bajo_build_timings* timings) {
   BJ_REQUIRE(ctx && s, "bajo_scene_rebuild: NULL argument");
-  return bajo_build_triangle_bvh(ctx, s->vertices, s->n, BAJO_BUILD_LBVH, timings, &s->bvh);
+  return bajo_build_triangle_bvh_ex(ctx, s->vertices, s->n, &s->options, timings, &s->bvh);
 }
 
 int32_t bajo_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float* out) {
@@ -828,12 +834,13 @@ int32_t bajo_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st
     ev->end = ev->make();
     BJ_HIP(hipEventRecord(ev->begin, ctx->stream));
   }
+  const bool wide = s->bvh->node_width == BAJO_WIDTH_WIDE8;
   switch (st->algorithm) {
-    case BAJO_RENDER_PATH: rc = run_render<BAJO_RENDER_PATH>(ctx, s, st, camera, pixels, ev); break;
-    case BAJO_RENDER_NORMALS: rc = run_render<BAJO_RENDER_NORMALS>(ctx, s, st, camera, pixels, ev); break;
-    case BAJO_RENDER_AO: rc = run_render<BAJO_RENDER_AO>(ctx, s, st, camera, pixels, ev); break;
-    case BAJO_RENDER_NEE: rc = run_render<BAJO_RENDER_NEE>(ctx, s, st, camera, pixels, ev); break;
-    case BAJO_RENDER_MIS: rc = run_render<BAJO_RENDER_MIS>(ctx, s, st, camera, pixels, ev); break;
+    case BAJO_RENDER_PATH: rc = wide ? run_render<BAJO_RENDER_PATH, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_PATH, false>(ctx, s, st, camera, pixels, ev); break;
+    case BAJO_RENDER_NORMALS: rc = wide ? run_render<BAJO_RENDER_NORMALS, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_NORMALS, false>(ctx, s, st, camera, pixels, ev); break;
+    case BAJO_RENDER_AO: rc = wide ? run_render<BAJO_RENDER_AO, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_AO, false>(ctx, s, st, camera, pixels, ev); break;
+    case BAJO_RENDER_NEE: rc = wide ? run_render<BAJO_RENDER_NEE, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_NEE, false>(ctx, s, st, camera, pixels, ev); break;
+    case BAJO_RENDER_MIS: rc = wide ? run_render<BAJO_RENDER_MIS, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_MIS, false>(ctx, s, st, camera, pixels, ev); break;
     default: BJ_REQUIRE(false, "bajo_render: unknown algorithm %d", st->algorithm);
   }
   if (rc) return rc;

@@ -1,4 +1,4 @@
-// BVH2 closest-hit / any-hit traversal for gfx950.
+// Closest-hit / any-hit ray launches over the binary or 8-wide BVH (gfx950).
 //
 // Replaces GpuTriangleBvh.launch_rays / launch_camera /
 // launch_camera_instrumented (bvh/gpu/triangle_bvh.mojo:520-614,730-863) and
@@ -35,22 +35,19 @@ BJ_D Ray camera_ray(const float* __restrict__ cams, uint64_t r, uint32_t w, uint
   return Ray{mk(c[0], c[1], c[2]), normalize(dir), 0.0f, kFltMax};
 }
 
-template <int kMode, bool kStats, bool kCamera>
-__global__ void __launch_bounds__(kTraceBlock) trace_kernel(const BjNode* __restrict__ nodes,
-                                                            const BjTri* __restrict__ tris, uint32_t root,
-                                                            uint32_t node_count, uint32_t prim_count,
+template <int kMode, bool kStats, bool kCamera, bool kWide>
+__global__ void __launch_bounds__(kTraceBlock) trace_kernel(const AccelView av,
                                                             const float* __restrict__ rays,
                                                             const float* __restrict__ cams, uint32_t cam_w,
                                                             uint32_t cam_h, float inv_height,
                                                             float* __restrict__ hits, uint64_t R,
                                                             uint32_t* __restrict__ stats,
                                                             uint32_t* __restrict__ counter,
-                                                            uint32_t* __restrict__ spill, uint32_t* __restrict__ status) {
+                                                            uint32_t* __restrict__ spill) {
   __shared__ uint32_t s_stack[kLdsStack * kTraceBlock];
   const uint32_t lane = lane_id();
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
-  const BvhView bv{nodes, tris, root, node_count, prim_count, status};
 
   // Batches of 64 rays per atomic while the launch is short; up to eight
   // batches per fetch when every wave would otherwise fetch 8+ times.
@@ -78,7 +75,7 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const BjNode* __rest
       ray.tmax = rays[7 * R + r];
     }
     TraceCounters tc;
-    Hit hit = traverse<kMode, kStats>(bv, ray, st, &tc);
+    Hit hit = trace_accel<kMode, kStats, kWide>(av, ray, st, &tc);
     store_hit(hits, r, hit);
     if constexpr (kStats) {
       uint32_t* s = stats + 5 * r;
@@ -98,14 +95,14 @@ struct TraceScratch {
   unsigned grid;
 };
 
-template <int kMode, bool kStats, bool kCamera>
+template <int kMode, bool kStats, bool kCamera, bool kWide>
 int32_t occupancy_grid(unsigned* grid) {
   static unsigned cached = 0;
   if (!cached) {
     int dev = 0, cus = 0, per_cu = 0;
     BJ_HIP(hipGetDevice(&dev));
     BJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<kMode, kStats, kCamera>, kTraceBlock, 0));
+    BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<kMode, kStats, kCamera, kWide>, kTraceBlock, 0));
     if (per_cu < 1) per_cu = 1;
     cached = (unsigned)(cus * per_cu);
   }
@@ -113,11 +110,11 @@ int32_t occupancy_grid(unsigned* grid) {
   return BAJO_OK;
 }
 
-template <int kMode, bool kStats, bool kCamera>
-int32_t launch(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const float* cams, uint32_t w, uint32_t hh,
-               float* hits, uint64_t R, uint32_t* stats) {
+template <int kMode, bool kStats, bool kCamera, bool kWide>
+int32_t launch_width(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const float* cams, uint32_t w, uint32_t hh,
+                     float* hits, uint64_t R, uint32_t* stats) {
   unsigned grid = 0;
-  int32_t rc = occupancy_grid<kMode, kStats, kCamera>(&grid);
+  int32_t rc = occupancy_grid<kMode, kStats, kCamera, kWide>(&grid);
   if (rc) return rc;
   uint64_t need_blocks = (R + kTraceBlock - 1) / kTraceBlock;
   if ((uint64_t)grid > need_blocks) grid = (unsigned)need_blocks;
@@ -130,11 +127,19 @@ int32_t launch(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const float*
   uint32_t* status = counter + 1;
   uint32_t* spill = (uint32_t*)((char*)scratch + 256);
   trace_init_kernel<<<1, 64, 0, ctx->stream>>>(counter, status);
-  trace_kernel<kMode, kStats, kCamera><<<grid, kTraceBlock, 0, ctx->stream>>>(
-      h->nodes, h->tris, h->root, (uint32_t)(h->n - 1), (uint32_t)h->n, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill,
-      status);
+  const AccelView av = accel_view(h, status);
+  trace_kernel<kMode, kStats, kCamera, kWide><<<grid, kTraceBlock, 0, ctx->stream>>>(
+      av, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill);
   BJ_HIP(hipGetLastError());
   return BAJO_OK;
+}
+
+template <int kMode, bool kStats, bool kCamera>
+int32_t launch(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const float* cams, uint32_t w, uint32_t hh,
+               float* hits, uint64_t R, uint32_t* stats) {
+  if (h->node_width == BAJO_WIDTH_WIDE8)
+    return launch_width<kMode, kStats, kCamera, true>(ctx, h, rays, cams, w, hh, hits, R, stats);
+  return launch_width<kMode, kStats, kCamera, false>(ctx, h, rays, cams, w, hh, hits, R, stats);
 }
 
 }  // namespace

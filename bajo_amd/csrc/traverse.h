@@ -230,4 +230,189 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
   return hit;
 }
 
+// ---- 8-wide traversal -------------------------------------------------------------
+// Plays the role of bajo's compressed-BVH8 traversal
+// (bvh/gpu/triangle_bvh.mojo:1010-1104, bvh/gpu/compressed_bounds_bvh.mojo:
+// 469-576) over the 96-byte quantized nodes built by the collapse.  Per node:
+// six 16-byte loads, the ray octant picks the near and far plane bytes, and
+// t(q) = (o + q*2^e - ray.o) * rcp is evaluated as fmaf(q, 2^e*rcp,
+// (o - ray.o)*rcp).  Leaf children hit by the ray are tested after the box
+// loop in one flattened triangle loop (so lanes diverge by the number of
+// triangles, not by child slot); inner children continue nearest first with
+// the others pushed far-to-near.
+
+struct WideView {
+  const BjWide* nodes;
+  const BjTri* tris;
+  uint32_t node_count, prim_count;
+  uint32_t* status;
+};
+
+BJ_D float ubyte_f(uint32_t word, int k) { return (float)((word >> (8 * k)) & 0xFFu); }
+
+// Child reference of a one-hot slot mask, as a chain of selects on plain
+// register values (an index-based select over an array or struct would be
+// folded into a dynamically indexed stack copy, i.e. scratch memory, when the
+// index differs between lanes).
+BJ_D uint32_t sel8(uint32_t bit, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t c4, uint32_t c5,
+                   uint32_t c6, uint32_t c7) {
+  uint32_t r = c0;
+  r = (bit & 0x02u) ? c1 : r;
+  r = (bit & 0x04u) ? c2 : r;
+  r = (bit & 0x08u) ? c3 : r;
+  r = (bit & 0x10u) ? c4 : r;
+  r = (bit & 0x20u) ? c5 : r;
+  r = (bit & 0x40u) ? c6 : r;
+  r = (bit & 0x80u) ? c7 : r;
+  return r;
+}
+
+template <int kMode, bool kStats>
+BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters* tc) {
+  uint32_t n_visit = 0, n_inner = 0, n_leaf = 0, max_sp = 0;
+  Hit hit = miss(ray.tmax);
+  const v3 rcp = rcp_dir(ray.d);
+  const bool px = rcp.x >= 0.0f, py = rcp.y >= 0.0f, pz = rcp.z >= 0.0f;
+  st.sp = 0;
+  bool occluded = false;
+  uint32_t cur = wv.node_count == 0 ? kSentinel : 0u;
+  while (cur != kSentinel) {
+    if (cur >= wv.node_count || n_visit > wv.node_count) {
+      atomicOr(wv.status, (uint32_t)BAJO_STATUS_BAD_NODE);
+      break;
+    }
+    n_visit++;
+    const uint4* np = reinterpret_cast<const uint4*>(wv.nodes + cur);
+    const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
+    const uint32_t meta = w0.w;
+    const int n = (int)(meta >> 24);
+    const float sx = __builtin_bit_cast(float, (meta & 0xFFu) << 23);
+    const float sy = __builtin_bit_cast(float, ((meta >> 8) & 0xFFu) << 23);
+    const float sz = __builtin_bit_cast(float, ((meta >> 16) & 0xFFu) << 23);
+    const float ax = (__builtin_bit_cast(float, w0.x) - ray.o.x) * rcp.x;
+    const float ay = (__builtin_bit_cast(float, w0.y) - ray.o.y) * rcp.y;
+    const float az = (__builtin_bit_cast(float, w0.z) - ray.o.z) * rcp.z;
+    const float bx = sx * rcp.x, by = sy * rcp.y, bz = sz * rcp.z;
+    // near / far plane bytes per axis for this ray octant (words: lo_x w1.xy,
+    // lo_y w1.zw, lo_z w2.xy, hi_x w2.zw, hi_y w3.xy, hi_z w3.zw)
+    const uint32_t nx0 = px ? w1.x : w2.z, nx1 = px ? w1.y : w2.w, fx0 = px ? w2.z : w1.x, fx1 = px ? w2.w : w1.y;
+    const uint32_t ny0 = py ? w1.z : w3.x, ny1 = py ? w1.w : w3.y, fy0 = py ? w3.x : w1.z, fy1 = py ? w3.y : w1.w;
+    const uint32_t nz0 = pz ? w2.x : w3.z, nz1 = pz ? w2.y : w3.w, fz0 = pz ? w3.z : w2.x, fz1 = pz ? w3.w : w2.y;
+    uint32_t c0 = w4.x, c1 = w4.y, c2 = w4.z, c3 = w4.w, c4 = w5.x, c5 = w5.y, c6 = w5.z, c7 = w5.w;
+    asm("" : "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3), "+v"(c4), "+v"(c5), "+v"(c6), "+v"(c7));
+    const float node_tmax = kMode == BAJO_TRACE_ANY_HIT ? ray.tmax : hit.t;
+    float th[8];
+    uint32_t inner_mask = 0, leaf_mask = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int bt = i & 3;
+      const uint32_t qnx = i < 4 ? nx0 : nx1, qfx = i < 4 ? fx0 : fx1;
+      const uint32_t qny = i < 4 ? ny0 : ny1, qfy = i < 4 ? fy0 : fy1;
+      const uint32_t qnz = i < 4 ? nz0 : nz1, qfz = i < 4 ? fz0 : fz1;
+      const float tnx = fmaf(ubyte_f(qnx, bt), bx, ax), tfx = fmaf(ubyte_f(qfx, bt), bx, ax);
+      const float tny = fmaf(ubyte_f(qny, bt), by, ay), tfy = fmaf(ubyte_f(qfy, bt), by, ay);
+      const float tnz = fmaf(ubyte_f(qnz, bt), bz, az), tfz = fmaf(ubyte_f(qfz, bt), bz, az);
+      const float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+      const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, node_tmax));
+      th[i] = tmin;
+      if (i < n && tmin <= tmax) {
+        if (sel8(1u << i, c0, c1, c2, c3, c4, c5, c6, c7) & kLeafFlag) leaf_mask |= 1u << i;
+        else inner_mask |= 1u << i;
+      }
+    }
+    // leaf children: one triangle per iteration across all hit leaves
+    uint32_t tri = 0, tri_end = 0;
+    while (true) {
+      if (tri == tri_end) {
+        if (!leaf_mask) break;
+        const uint32_t bit = leaf_mask & (0u - leaf_mask);
+        leaf_mask ^= bit;
+        const uint32_t ref = sel8(bit, c0, c1, c2, c3, c4, c5, c6, c7);
+        tri = ref & kWideLeafStartMask;
+        tri_end = tri + ((ref >> 28) & 7u) + 1u;
+        if (tri_end > wv.prim_count) tri_end = wv.prim_count;
+        if (tri >= tri_end) continue;
+      }
+      n_leaf++;
+      if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
+        if (leaf_any(wv.tris, tri, ray)) {
+          occluded = true;
+          break;
+        }
+      } else {
+        leaf_closest(wv.tris, tri, ray, hit);
+      }
+      ++tri;
+    }
+    if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
+      if (occluded) break;
+    } else {
+      // children entered beyond the (possibly improved) closest hit are dropped
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (th[i] > hit.t) inner_mask &= ~(1u << i);
+    }
+    if (kStats) n_inner += __builtin_popcount(inner_mask);
+    if (inner_mask) {
+      // push all but the nearest inner child, farthest first
+      bool overflow = false;
+      while (inner_mask & (inner_mask - 1u)) {
+        uint32_t far_bit = 0, far_ref = 0;
+        float far_t = -1.0f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (((inner_mask >> i) & 1u) && th[i] >= far_t) {
+            far_t = th[i];
+            far_bit = 1u << i;
+            far_ref = sel8(1u << i, c0, c1, c2, c3, c4, c5, c6, c7);
+          }
+        inner_mask ^= far_bit;
+        if (st.sp >= kMaxStack) {
+          overflow = true;
+          break;
+        }
+        st.push(far_ref);
+      }
+      if (overflow) {
+        atomicOr(wv.status, (uint32_t)BAJO_STATUS_STACK_OVERFLOW);
+        break;
+      }
+      if (kStats && (uint32_t)st.sp > max_sp) max_sp = (uint32_t)st.sp;
+      cur = sel8(inner_mask, c0, c1, c2, c3, c4, c5, c6, c7);
+      continue;
+    }
+    cur = st.sp == 0 ? kSentinel : st.pop();
+  }
+  if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
+    hit = occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
+  }
+  if constexpr (kStats) {
+    tc->visits = n_visit;
+    tc->inner = n_inner;
+    tc->leaves = n_leaf;
+    tc->max_sp = max_sp;
+  }
+  return hit;
+}
+
+// Either layout behind one call: kWide selects the 8-wide traversal.
+struct AccelView {
+  BvhView bin;
+  WideView wide;
+};
+
+// Host: the view of a built hierarchy; `status` receives traversal errors.
+inline AccelView accel_view(const bajo_bvh* h, uint32_t* status) {
+  AccelView av;
+  av.bin = BvhView{h->nodes, h->tris, h->root, (uint32_t)(h->n - 1), (uint32_t)h->n, status};
+  av.wide = WideView{h->wide, h->tris, h->node_width == BAJO_WIDTH_WIDE8 ? h->wide_count : 0u, (uint32_t)h->n, status};
+  return av;
+}
+
+template <int kMode, bool kStats, bool kWide>
+BJ_D Hit trace_accel(const AccelView& av, const Ray& ray, Stack& st, TraceCounters* tc) {
+  if constexpr (kWide) return traverse8<kMode, kStats>(av.wide, ray, st, tc);
+  else return traverse<kMode, kStats>(av.bin, ray, st, tc);
+}
+
 }  // namespace bj

@@ -1,0 +1,282 @@
+// Binary LBVH -> 8-wide BVH with quantized child boxes (gfx950).
+//
+// Plays the role of bajo's wide collapse and compressed-BVH8 encoding
+// (bvh/gpu/builder/wide_collapse.mojo:148-643, bvh/gpu/compressed_bounds_bvh.mojo:
+// 84-369): a top-down collapse that opens the child with the largest surface
+// area until a node has 8 children -- first among children holding more than
+// the leaf size, then among any inner child -- and turns inner children of
+// at most the leaf size (1..8 triangles) into leaves.  Child boxes are
+// stored as 8-bit offsets on a per-axis power-of-two grid anchored at the
+// node's lower corner.
+//
+// Unlike bajo's persistent work queue with a spin fail-safe, the collapse here
+// is level-synchronous: level k processes the wide nodes allocated while
+// processing level k-1 (a contiguous index range), so no kernel ever waits on
+// another wave.  Levels are launched in batches; the host reads the level
+// size after each batch and stops at the first empty level.  Leaf children
+// reference ranges of the sorted triangle records that the refit already
+// wrote (an LBVH subtree covers a contiguous sorted range), so no triangle
+// data is copied.
+#include "bvh_internal.h"
+
+namespace bj {
+namespace {
+
+constexpr int kCollapseBlock = 256;
+constexpr int kLevelBatch = 6;
+// wide_ctl words
+constexpr int kCtlCount = 0, kCtlBegin = 1, kCtlEnd = 2, kCtlStatus = 3;
+constexpr uint32_t kStatusOutOfNodes = 1;
+
+struct Child {
+  uint32_t ref, count;
+  float b[6];
+};
+
+BJ_D float half_area(const float* b) {
+  const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+  return dx * dy + dx * dz + dy * dz;
+}
+
+BJ_D void load_children(const BjNode* __restrict__ bnodes, const uint32_t* __restrict__ leaf_counts, uint32_t node,
+                        Child& l, Child& r) {
+  const float4* p = reinterpret_cast<const float4*>(bnodes + node);
+  const float4 a = p[0], b = p[1], c = p[2];
+  const uint4 m = reinterpret_cast<const uint4*>(bnodes + node)[3];
+  l.b[0] = a.x; l.b[1] = a.y; l.b[2] = a.z; l.b[3] = a.w; l.b[4] = b.x; l.b[5] = b.y;
+  r.b[0] = b.z; r.b[1] = b.w; r.b[2] = c.x; r.b[3] = c.y; r.b[4] = c.z; r.b[5] = c.w;
+  l.ref = m.y;
+  r.ref = m.z;
+  l.count = (l.ref & kLeafFlag) ? 1u : leaf_counts[l.ref];
+  r.count = (r.ref & kLeafFlag) ? 1u : leaf_counts[r.ref];
+}
+
+// Smallest biased exponent e with o + 254 * 2^(e-127) >= hi (one spare step).
+BJ_HD uint32_t grid_exponent(float o, float hi) {
+  const float extent = hi - o;
+  uint32_t e = 1;
+  if (extent > 0.0f) {
+    const uint32_t bits = __builtin_bit_cast(uint32_t, extent / 254.0f);
+    e = (bits >> 23) & 0xFFu;
+    if (bits & 0x7FFFFFu) e += 1;
+    if (e < 1) e = 1;
+  }
+  while (e < 253) {
+    const float s = __builtin_bit_cast(float, e << 23);
+    if (o + 254.0f * s >= hi) break;
+    ++e;
+  }
+  return e;
+}
+
+BJ_HD void quantize(float o, uint32_t e, float lo, float hi, uint8_t& qlo, uint8_t& qhi) {
+  const float s = __builtin_bit_cast(float, e << 23);
+  const float inv = __builtin_bit_cast(float, (254u - e) << 23);  // exact 2^-(e-127)
+  float fl = floorf((lo - o) * inv);
+  fl = fl < 0.0f ? 0.0f : (fl > 255.0f ? 255.0f : fl);
+  while (fl > 0.0f && o + fl * s > lo) fl -= 1.0f;
+  float fh = ceilf((hi - o) * inv);
+  fh = fh < 0.0f ? 0.0f : (fh > 255.0f ? 255.0f : fh);
+  while (fh < 255.0f && o + fh * s < hi) fh += 1.0f;
+  fl = fl > 0.0f ? fl - 1.0f : 0.0f;      // padding: one step below
+  fh = fh < 255.0f ? fh + 1.0f : 255.0f;  // and one above
+  qlo = (uint8_t)fl;
+  qhi = (uint8_t)fh;
+}
+
+__global__ void collapse_root_kernel(const float* __restrict__ scene, uint32_t n, uint32_t root_binary,
+                                     BjWide* __restrict__ wide, uint32_t* __restrict__ wide_src,
+                                     uint32_t* __restrict__ ctl) {
+  if (threadIdx.x != 0) return;
+  ctl[kCtlStatus] = 0;
+  if (n == 1) {
+    // single triangle: one wide node with one leaf child spanning the scene box
+    BjWide w;
+    w.ox = scene[0]; w.oy = scene[1]; w.oz = scene[2];
+    w.ex = (uint8_t)grid_exponent(scene[0], scene[3]);
+    w.ey = (uint8_t)grid_exponent(scene[1], scene[4]);
+    w.ez = (uint8_t)grid_exponent(scene[2], scene[5]);
+    w.n = 1;
+    for (int i = 0; i < 8; ++i) {
+      w.lo_x[i] = w.lo_y[i] = w.lo_z[i] = 255;
+      w.hi_x[i] = w.hi_y[i] = w.hi_z[i] = 0;
+      w.child[i] = 0;
+    }
+    quantize(scene[0], w.ex, scene[0], scene[3], w.lo_x[0], w.hi_x[0]);
+    quantize(scene[1], w.ey, scene[1], scene[4], w.lo_y[0], w.hi_y[0]);
+    quantize(scene[2], w.ez, scene[2], scene[5], w.lo_z[0], w.hi_z[0]);
+    w.child[0] = kLeafFlag | 0u;
+    wide[0] = w;
+    ctl[kCtlCount] = 1;
+    ctl[kCtlBegin] = 0;
+    ctl[kCtlEnd] = 0;
+    return;
+  }
+  wide_src[0] = root_binary;
+  ctl[kCtlCount] = 1;
+  ctl[kCtlBegin] = 0;
+  ctl[kCtlEnd] = 1;
+}
+
+// The next level is the nodes reserved by the last one, clipped to the
+// capacity; after a capacity failure the level range is left empty.
+__global__ void level_advance_kernel(uint32_t* __restrict__ ctl, uint32_t capacity) {
+  if (threadIdx.x != 0) return;
+  const uint32_t end = ctl[kCtlEnd];
+  uint32_t next = ctl[kCtlCount];
+  if (next > capacity) next = capacity;
+  if (ctl[kCtlStatus]) next = end;
+  ctl[kCtlBegin] = end;
+  ctl[kCtlEnd] = next;
+}
+
+__global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const BjNode* __restrict__ bnodes,
+                                                                        const uint32_t* __restrict__ leaf_counts,
+                                                                        BjWide* __restrict__ wide,
+                                                                        uint32_t* __restrict__ wide_src,
+                                                                        uint32_t* __restrict__ ctl,
+                                                                        uint32_t capacity, uint32_t max_leaf) {
+  const uint32_t begin = ctl[kCtlBegin], end = ctl[kCtlEnd];
+  const uint32_t lane = lane_id();
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // whole waves iterate together so the slot reservation below can ballot
+  for (uint32_t base = begin + (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); base < end; base += stride) {
+    const uint32_t w = base + lane;
+    const bool active = w < end;
+    Child ch[8];
+    int n = 0;
+    if (active) {
+      load_children(bnodes, leaf_counts, wide_src[w], ch[0], ch[1]);
+      n = 2;
+      // open largest-area children: first those too big for a leaf, then any inner child
+      for (int phase = 0; phase < 2; ++phase) {
+        while (n < 8) {
+          int pick = -1;
+          float best = -1.0f;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            if (i >= n) break;
+            const bool inner = !(ch[i].ref & kLeafFlag);
+            const bool eligible = inner && (phase == 0 ? ch[i].count > max_leaf : ch[i].count >= 2);
+            const float a = half_area(ch[i].b);
+            if (eligible && a > best) {
+              best = a;
+              pick = i;
+            }
+          }
+          if (pick < 0) break;
+          Child l, r;
+          load_children(bnodes, leaf_counts, ch[pick].ref, l, r);
+          ch[pick] = l;
+          ch[n++] = r;
+        }
+      }
+    }
+    // reserve wide nodes for inner children that stay inner (wave-aggregated)
+    uint32_t k = 0;
+    for (int i = 0; i < n; ++i)
+      if (!(ch[i].ref & kLeafFlag) && ch[i].count > max_leaf) ++k;
+    uint32_t incl = k;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off, 64);
+      if (lane >= (uint32_t)off) incl += y;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t first = 0;
+    if (lane == 0 && total) first = atomicAdd(ctl + kCtlCount, total);
+    first = __shfl(first, 0, 64) + (incl - k);
+    if (!active) continue;
+    if ((uint64_t)first + k > capacity) {
+      atomicOr(ctl + kCtlStatus, kStatusOutOfNodes);
+      continue;
+    }
+    // node box and per-axis grids
+    float box[6] = {kFltMax, kFltMax, kFltMax, -kFltMax, -kFltMax, -kFltMax};
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        box[a] = fmin_sel(box[a], ch[i].b[a]);
+        box[a + 3] = fmax_sel(box[a + 3], ch[i].b[a + 3]);
+      }
+    BjWide out;
+    out.ox = box[0]; out.oy = box[1]; out.oz = box[2];
+    const uint32_t ex = grid_exponent(box[0], box[3]), ey = grid_exponent(box[1], box[4]),
+                   ez = grid_exponent(box[2], box[5]);
+    out.ex = (uint8_t)ex; out.ey = (uint8_t)ey; out.ez = (uint8_t)ez;
+    out.n = (uint8_t)n;
+    uint32_t next = first;
+    for (int i = 0; i < 8; ++i) {
+      if (i < n) {
+        quantize(box[0], ex, ch[i].b[0], ch[i].b[3], out.lo_x[i], out.hi_x[i]);
+        quantize(box[1], ey, ch[i].b[1], ch[i].b[4], out.lo_y[i], out.hi_y[i]);
+        quantize(box[2], ez, ch[i].b[2], ch[i].b[5], out.lo_z[i], out.hi_z[i]);
+        const uint32_t ref = ch[i].ref;
+        if (ref & kLeafFlag) {
+          out.child[i] = kLeafFlag | (ref & kIndexMask);  // one triangle
+        } else if (ch[i].count <= max_leaf) {
+          // inner binary node over sorted range [fence - count + 1, fence]
+          const uint32_t fence = bnodes[ref].fence;
+          out.child[i] = kLeafFlag | ((ch[i].count - 1u) << 28) | (fence + 1u - ch[i].count);
+        } else {
+          out.child[i] = next;
+          wide_src[next] = ref;
+          ++next;
+        }
+      } else {
+        out.lo_x[i] = out.lo_y[i] = out.lo_z[i] = 255;
+        out.hi_x[i] = out.hi_y[i] = out.hi_z[i] = 0;
+        out.child[i] = 0;
+      }
+    }
+    const uint4* src = reinterpret_cast<const uint4*>(&out);
+    uint4* dst = reinterpret_cast<uint4*>(wide + w);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) dst[q] = src[q];
+  }
+}
+
+}  // namespace
+
+// Builds the 8-wide layout of h from its binary LBVH.  Enqueued on the
+// context stream; synchronizes once per batch of kLevelBatch levels.
+int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h) {
+  hipStream_t st = ctx->stream;
+  const uint32_t n = (uint32_t)h->n;
+  int dev = 0, cus = 0;
+  BJ_HIP(hipGetDevice(&dev));
+  BJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  collapse_root_kernel<<<1, 64, 0, st>>>(h->scene, n, 0u, h->wide, h->wide_src, h->wide_ctl);
+  BJ_HIP(hipGetLastError());
+  if (n == 1) {
+    h->wide_count = 1;
+    return BAJO_OK;
+  }
+  // grid: enough waves for the widest level (~n/3 nodes), capped at a few per CU
+  unsigned grid = div_up(n / 3 + 1, kCollapseBlock);
+  const unsigned cap_grid = (unsigned)cus * 8;
+  if (grid > cap_grid) grid = cap_grid;
+  if (grid < 1) grid = 1;
+  bool done = false;
+  for (int batch = 0; batch < 64 && !done; ++batch) {
+    for (int l = 0; l < kLevelBatch; ++l) {
+      collapse_level_kernel<<<grid, kCollapseBlock, 0, st>>>(h->nodes, h->leaf_counts, h->wide, h->wide_src,
+                                                             h->wide_ctl, (uint32_t)h->wide_capacity,
+                                                             (uint32_t)h->wide_leaf);
+      level_advance_kernel<<<1, 64, 0, st>>>(h->wide_ctl, (uint32_t)h->wide_capacity);
+    }
+    BJ_HIP(hipGetLastError());
+    BJ_HIP(hipMemcpyAsync(h->host_ctl, h->wide_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    BJ_HIP(hipStreamSynchronize(st));
+    if (h->host_ctl[kCtlStatus]) {
+      set_error("8-wide collapse ran out of node capacity");
+      return BAJO_ERR_INVALID_ARGUMENT;
+    }
+    done = h->host_ctl[kCtlBegin] == h->host_ctl[kCtlEnd];
+  }
+  // a binary tree over n <= 2^28 leaves is far shallower than 64 batches
+  BJ_REQUIRE(done, "8-wide collapse did not finish within %d levels", 64 * kLevelBatch);
+  h->wide_count = h->host_ctl[kCtlCount];
+  return BAJO_OK;
+}
+
+}  // namespace bj

@@ -80,7 +80,9 @@ class SceneData:
 class GpuScene:
     """Device-resident scene: vertices, BVH, materials and the light table."""
 
-    def __init__(self, scene: SceneData, ctx: DeviceContext | None = None):
+    def __init__(self, scene: SceneData, ctx: DeviceContext | None = None, node_width: int = 2, leaf_size: int = 0):
+        from .bvh import build_options
+
         self.ctx = ctx or default_context(0)
         v = np.ascontiguousarray(scene.vertices, dtype=np.float32)
         s = np.ascontiguousarray(scene.surfaces, dtype=np.uint32)
@@ -89,8 +91,9 @@ class GpuScene:
         if s.size != v.size // 9:
             raise ValueError("one surface id per triangle is required")
         handle = ctypes.c_void_p()
+        opts = build_options(node_width, leaf_size)
         _lib.check(
-            _lib.lib().bajo_scene_create(
+            _lib.lib().bajo_scene_create_ex(
                 self.ctx.handle,
                 v.ctypes.data_as(ctypes.c_void_p),
                 v.size // 9,
@@ -99,11 +102,13 @@ class GpuScene:
                 lamb.size // 3,
                 emit.ctypes.data_as(ctypes.c_void_p),
                 emit.size // 3,
+                ctypes.byref(opts),
                 ctypes.byref(handle),
             )
         )
         self._handle = handle
         self.triangle_count = v.size // 9
+        self.node_width = node_width
 
     @property
     def handle(self):

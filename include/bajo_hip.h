@@ -42,6 +42,22 @@ extern "C" {
 /* Build methods, bvh/gpu GpuBvhBuildMethod (LBVH only in this build). */
 #define BAJO_BUILD_LBVH 0
 
+/* Node widths (bajo's W parameter, bvh/gpu/triangle_bvh.mojo:617): 2 keeps
+ * the binary LBVH; 8 collapses it into 8-wide nodes with 8-bit quantized
+ * child boxes (bajo's wide collapse + compressed BVH8 layout,
+ * bvh/gpu/builder/wide_collapse.mojo:148-643,
+ * bvh/gpu/compressed_bounds_bvh.mojo:84-369). */
+#define BAJO_WIDTH_BINARY 2
+#define BAJO_WIDTH_WIDE8 8
+
+/* Build options (bajo's compile-time W, L and method parameters). */
+typedef struct {
+  int32_t method;     /* BAJO_BUILD_LBVH                                   */
+  int32_t node_width; /* BAJO_WIDTH_BINARY or BAJO_WIDTH_WIDE8             */
+  int32_t leaf_size;  /* width 8: most triangles per leaf child, 1..8; 0 = 4 */
+  int32_t reserved;   /* must be 0                                         */
+} bajo_build_options;
+
 typedef struct bajo_ctx bajo_ctx;
 typedef struct bajo_bvh bajo_bvh;
 
@@ -54,6 +70,7 @@ typedef struct {
   uint64_t sort_ns;     /* onesweep radix sort of the 64-bit keys     */
   uint64_t topology_ns; /* Karras hierarchy                           */
   uint64_t refit_ns;    /* bottom-up refit + sorted triangle packing  */
+  uint64_t collapse_ns; /* binary -> 8-wide collapse (width 8 only)   */
   uint64_t total_ns;
 } bajo_build_timings;
 
@@ -62,13 +79,16 @@ typedef struct {
   uint64_t prim_count;
   uint64_t node_count;       /* internal nodes = prim_count - 1         */
   uint32_t root;             /* tagged reference (bit 31 = leaf)        */
-  uint32_t reserved;
+  uint32_t node_width;       /* 2 or 8                                  */
   const void* nodes;         /* 64 B per node: bounds[12] f32, meta[4] u32 (parent,left,right,fence) */
   const void* triangles;     /* 48 B per sorted leaf: v0.xyz,prim | e1.xyz,0 | e2.xyz,0 */
   const void* sorted_keys;   /* u64[prim_count]: (morton<<32)|prim       */
   const void* leaf_parent;   /* u32[prim_count]                          */
   const void* leaf_counts;   /* u32[node_count]                          */
   const void* scene_bounds;  /* f32[12]: root box, centroid box          */
+  uint64_t wide_node_count;  /* width 8: 96-byte wide nodes, root = node 0 */
+  const void* wide_nodes;    /* width 8: origin f32[3], exponents u8[3], child count u8,
+                                lo_x,lo_y,lo_z,hi_x,hi_y,hi_z u8[8] each, child u32[8] */
 } bajo_bvh_info;
 
 /* ---- context ------------------------------------------------------------
@@ -118,6 +138,17 @@ int32_t bajo_onesweep_radix_sort_pairs_u64_u32(bajo_ctx* ctx, uint64_t* keys, ui
  * owns the hierarchy (a binary LBVH whose nodes hold both child boxes). */
 int32_t bajo_build_triangle_bvh(bajo_ctx* ctx, const float* vertices, uint64_t n, int32_t method,
                                 bajo_build_timings* timings, bajo_bvh** out);
+/* build_triangle_bvh[frame, W, L, method] with W and L chosen at run time.
+ * With node_width 8 the LBVH is collapsed top-down into 8-wide nodes: a node
+ * opens the child of largest surface area until it has 8 children, subtrees
+ * of at most leaf_size triangles become leaf children (ranges of the sorted
+ * triangle records), child boxes are stored on a per-axis power-of-two grid
+ * and padded by one grid step.  The collapse runs level by level and reads
+ * the level size back every few levels, so a width-8 build synchronizes the
+ * context stream.  `options` NULL = binary LBVH.  n < 2^28 for width 8. */
+int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_t n,
+                                   const bajo_build_options* options, bajo_build_timings* timings,
+                                   bajo_bvh** out);
 int32_t bajo_bvh_free(bajo_bvh* bvh);
 int32_t bajo_bvh_get_info(const bajo_bvh* bvh, bajo_bvh_info* out);
 /* Copies the binary LBVH into bajo's GpuBinaryBoundsBvh layout
@@ -128,6 +159,11 @@ int32_t bajo_bvh_get_info(const bajo_bvh* bvh, bajo_bvh_info* out);
 int32_t bajo_bvh_copy_binary(bajo_ctx* ctx, const bajo_bvh* bvh, uint64_t* sorted_keys, uint32_t* leaf_ids,
                              uint32_t* node_meta, float* node_bounds, uint32_t* leaf_parent,
                              uint32_t* leaf_counts, float* scene_bounds);
+/* Copies the 8-wide nodes (96 bytes each, layout in bajo_bvh_info) into a
+ * host buffer of `capacity` nodes; *count receives the node count.  Node
+ * numbering depends on the order in which waves reserve nodes, so only the
+ * hierarchy it describes is reproducible (as for bajo's wide collapse). */
+int32_t bajo_bvh_copy_wide(bajo_ctx* ctx, const bajo_bvh* bvh, void* nodes, uint64_t capacity, uint64_t* count);
 
 /* ---- traversal ----------------------------------------------------------
  * GpuTriangleBvh.launch_rays[mode], bvh/gpu/triangle_bvh.mojo:587-614:
@@ -193,6 +229,11 @@ typedef struct {
 int32_t bajo_scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
                           const float* lambertian, uint32_t n_lambertian, const float* emissive,
                           uint32_t n_emissive, bajo_scene** out);
+/* bajo_scene_create with explicit build options (NULL = binary LBVH); the
+ * renderer traverses whichever layout the options select. */
+int32_t bajo_scene_create_ex(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
+                             const float* lambertian, uint32_t n_lambertian, const float* emissive,
+                             uint32_t n_emissive, const bajo_build_options* options, bajo_scene** out);
 /* Rebuilds the scene's BVH from its device-resident vertices. */
 int32_t bajo_scene_rebuild(bajo_ctx* ctx, bajo_scene* scene, bajo_build_timings* timings);
 int32_t bajo_scene_free(bajo_scene* scene);

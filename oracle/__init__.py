@@ -62,6 +62,13 @@ _SIG = {
     "orc_sincos_turns": (None, [c_float, c_void_p, c_void_p]),
     "orc_alias_table": (None, [c_void_p, c_uint32, c_float, c_void_p, c_void_p]),
     "orc_light_table": (None, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "orc_grid_exponent": (c_uint32, [c_float, c_float]),
+    "orc_quantize": (None, [c_float, c_uint32, c_float, c_float, c_void_p, c_void_p]),
+    "orc_collapse_wide": (c_uint64, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, c_uint32, c_void_p, c_uint64]),
+    "orc_trace_wide": (
+        c_uint64,
+        [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_uint64, c_int, c_void_p, c_void_p],
+    ),
     "orc_render": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "orc_render_fp64": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
 }
@@ -75,6 +82,8 @@ class RenderSettings(ctypes.Structure):
         ("max_depth", c_uint32),
         ("seed", c_uint64),
         ("algorithm", c_int),
+        ("node_width", c_int),
+        ("leaf_size", c_int),
     ]
 
 _lib = None
@@ -305,16 +314,100 @@ def light_table(vertices, surfaces, emissive):
     return fields[: 14 * k], kinds[:k], float(total[0])
 
 
-def render(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1):
+def render(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1,
+           node_width=2, leaf_size=0):
     v = f32(vertices)
     surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
     lamb = f32(lambertian) if len(lambertian) else np.zeros(3, np.float32)
     emit = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
-    st = RenderSettings(width, height, spp, max_depth, seed, algorithm)
+    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, node_width, leaf_size)
     pixels = np.empty(3 * width * height, np.float32)
     rays = np.zeros(2, np.uint64)
-    lib().orc_render(_p(v), v.size // 9, _p(surf), _p(lamb), _p(emit), ctypes.byref(st), _p(f32(cam20)), _p(pixels), _p(rays), threads)
+    rc = lib().orc_render(
+        _p(v), v.size // 9, _p(surf), _p(lamb), _p(emit), ctypes.byref(st), _p(f32(cam20)), _p(pixels), _p(rays), threads
+    )
+    if rc != 0:
+        raise RuntimeError(f"orc_render failed ({rc})")
     return pixels, rays
+
+
+# 96-byte 8-wide node record of orc_wide.c (and of the device library).
+WIDE_NODE_DTYPE = np.dtype(
+    [
+        ("origin", "<f4", 3),
+        ("exp", "u1", 3),
+        ("count", "u1"),
+        ("lo", "u1", (3, 8)),
+        ("hi", "u1", (3, 8)),
+        ("child", "<u4", 8),
+    ]
+)
+WIDE_LEAF = 0x80000000
+WIDE_START_MASK = 0x0FFFFFFF
+
+
+def grid_exponent(o: float, hi: float) -> int:
+    return int(lib().orc_grid_exponent(ctypes.c_float(o), ctypes.c_float(hi)))
+
+
+def quantize(o: float, e: int, lo: float, hi: float):
+    a, b = ctypes.c_uint8(), ctypes.c_uint8()
+    lib().orc_quantize(ctypes.c_float(o), e, ctypes.c_float(lo), ctypes.c_float(hi), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def collapse_wide(bvh: dict, leaf_size: int = 4) -> np.ndarray:
+    """8-wide nodes collapsed from an oracle LBVH (build_lbvh's dict), breadth-first numbering."""
+    n = bvh["leaf_ids"].size
+    cap = max(n, 1)
+    out = np.zeros(cap, WIDE_NODE_DTYPE)
+    meta = np.ascontiguousarray(bvh["node_meta"], np.uint32)
+    nb = f32(bvh["node_bounds"])
+    counts = np.ascontiguousarray(bvh["leaf_counts"], np.uint32)
+    k = lib().orc_collapse_wide(
+        _p(meta) if meta.size else None, _p(nb) if nb.size else None, _p(counts) if counts.size else None, n,
+        _p(f32(bvh["scene_bounds"])), leaf_size, out.ctypes.data_as(ctypes.c_void_p), cap,
+    )
+    if k == 0:
+        raise RuntimeError("orc_collapse_wide failed")
+    return out[:k].copy()
+
+
+def trace_wide(vertices, leaf_ids, wide: np.ndarray, rays: np.ndarray, mode: int = 0, with_stats: bool = False):
+    """Traverses 8-wide nodes: (hits f32[R, 8], stats u32[R, 5] or None, stack overflows)."""
+    v = f32(vertices)
+    ids = np.ascontiguousarray(leaf_ids, np.uint32)
+    w = np.ascontiguousarray(wide, WIDE_NODE_DTYPE)
+    r = f32(rays)
+    R = r.size // 8
+    hits = np.empty(8 * R, np.float32)
+    stats = np.empty(5 * R, np.uint32) if with_stats else None
+    over = lib().orc_trace_wide(
+        _p(v), v.size // 9, _p(ids), w.ctypes.data_as(ctypes.c_void_p), w.size, _p(r), R, mode, _p(hits),
+        _p(stats) if stats is not None else None,
+    )
+    return hits.reshape(R, 8), (stats.reshape(R, 5) if stats is not None else None), int(over)
+
+
+def canonical_wide(nodes: np.ndarray) -> np.ndarray:
+    """Depth-first renumbering (children in slot order) of 8-wide nodes, so two
+    builds that differ only in node numbering compare equal."""
+    order, stack = [], [0]
+    while stack:
+        k = stack.pop()
+        order.append(k)
+        c = nodes[k]["child"][: nodes[k]["count"]]
+        inner = [int(x) for x in c if not (int(x) & WIDE_LEAF)]
+        stack.extend(reversed(inner))
+    pos = {k: i for i, k in enumerate(order)}
+    out = nodes[order].copy()
+    for i in range(out.size):
+        cnt = int(out[i]["count"])
+        for j in range(cnt):
+            c = int(out[i]["child"][j])
+            if not (c & WIDE_LEAF):
+                out[i]["child"][j] = pos[c]
+    return out
 
 
 def render_fp64(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1):
@@ -322,7 +415,7 @@ def render_fp64(vertices, surfaces, lambertian, emissive, width, height, spp, ma
     surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
     lamb = f32(lambertian) if len(lambertian) else np.zeros(3, np.float32)
     emit = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
-    st = RenderSettings(width, height, spp, max_depth, seed, algorithm)
+    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, 2, 0)
     pixels = np.empty(3 * width * height, np.float64)
     lib().orc_render_fp64(_p(v), v.size // 9, _p(surf), _p(lamb), _p(emit), ctypes.byref(st), _p(f32(cam20)), _p(pixels), threads)
     return pixels

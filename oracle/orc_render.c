@@ -394,7 +394,18 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
     float scene12[12];
     uint32_t root = orc_build_lbvh(vertices, n, keys, ids, meta, parent, counts, nb, scene12);
     scene_t sc;
+    memset(&sc.bvh, 0, sizeof(sc.bvh));
     sc.bvh.vertices = vertices; sc.bvh.leaf_ids = ids; sc.bvh.node_meta = meta; sc.bvh.node_bounds = nb; sc.bvh.root = root;
+    sc.bvh.prim_count = n;
+    void *wide = NULL;
+    if (st->node_width == 8) {
+        wide = malloc(n * 96);
+        uint64_t wc = orc_collapse_wide(meta, nb, counts, n, scene12, st->leaf_size ? (uint32_t)st->leaf_size : 4u,
+                                        wide, n);
+        if (wc == 0) { free(wide); free(keys); free(ids); free(meta); free(parent); free(counts); free(nb); return -1; }
+        sc.bvh.wide = wide;
+        sc.bvh.wide_count = wc;
+    }
     sc.surf = surfaces; sc.lamb = lambertian; sc.emit = emissive; sc.st = *st;
     memcpy(sc.cam, cam20, sizeof(sc.cam));
     build_lights(vertices, n, surfaces, emissive, &sc.lights);
@@ -418,7 +429,7 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
         pixels[3 * p] = r * kk; pixels[3 * p + 1] = g * kk; pixels[3 * p + 2] = b * kk;
     }
     if (ray_counts) { ray_counts[0] = rays[0]; ray_counts[1] = rays[1]; }
-    free(samples); free(keys); free(ids); free(meta); free(parent); free(counts); free(nb);
+    free(samples); free(keys); free(ids); free(meta); free(parent); free(counts); free(nb); free(wide);
     free(sc.lights.fields); free(sc.lights.kinds);
     return 0;
 }

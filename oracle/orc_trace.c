@@ -100,6 +100,11 @@ static int leaf_any(const orc_bvh *b, uint32_t sorted_leaf, orc_v3 o, orc_v3 d, 
 #define ORC_STACK 128
 
 orc_hit orc_trace_closest(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, float t_max, uint32_t *stats) {
+    if (b->wide) {
+        orc_hit h;
+        orc_trace_wide_query(b, 0, o, d, t_min, t_max, &h, NULL);
+        return h;
+    }
     orc_hit hit = hit_miss(t_max);
     if (b->root == ORC_SENTINEL) return hit;
     if (b->root & ORC_LEAF_FLAG) {
@@ -151,6 +156,11 @@ orc_hit orc_trace_closest(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, flo
 }
 
 orc_hit orc_trace_any(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, float t_max) {
+    if (b->wide) {
+        orc_hit h;
+        orc_trace_wide_query(b, 1, o, d, t_min, t_max, &h, NULL);
+        return h;
+    }
     orc_hit occluded = {0, 0, ORC_EMPTY, ORC_EMPTY, {0, 0, 0}, 0.0f};
     if (b->root == ORC_SENTINEL) return hit_miss(t_max);
     if (b->root & ORC_LEAF_FLAG) return leaf_any(b, b->root & ORC_INDEX_MASK, o, d, t_min, t_max) ? occluded : hit_miss(t_max);
@@ -184,7 +194,7 @@ orc_hit orc_trace_any(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, float t
 void orc_trace_bvh2(const float *vertices, const uint32_t *leaf_ids, const uint32_t *node_meta,
                     const float *node_bounds, uint32_t root, const float *rays, uint64_t R, int mode,
                     float *hits, uint32_t *stats) {
-    orc_bvh b = {vertices, leaf_ids, node_meta, node_bounds, root};
+    orc_bvh b = {vertices, leaf_ids, node_meta, node_bounds, root, NULL, 0, 0};
     for (uint64_t r = 0; r < R; ++r) {
         orc_v3 o = orc_v(rays[0 * R + r], rays[1 * R + r], rays[2 * R + r]);
         orc_v3 d = orc_v(rays[4 * R + r], rays[5 * R + r], rays[6 * R + r]);
@@ -253,7 +263,7 @@ typedef struct {
 
 static void *trace_worker(void *p) {
     trace_job *j = (trace_job *)p;
-    orc_bvh b = {j->vertices, j->leaf_ids, j->node_meta, j->node_bounds, j->root};
+    orc_bvh b = {j->vertices, j->leaf_ids, j->node_meta, j->node_bounds, j->root, NULL, 0, 0};
     uint64_t R = j->R;
     for (uint64_t r = j->first; r < j->last; ++r) {
         orc_v3 o = orc_v(j->rays[0 * R + r], j->rays[1 * R + r], j->rays[2 * R + r]);

@@ -4,7 +4,8 @@
 Workload (one GPU): a displaced geodesic icosphere of 4,194,304 FP32
 triangles (frequency 458, radial value-noise amplitude 0.05, Lambertian
 albedo per triangle U[0.2, 0.8]^3) under one quad light of emission 15.  One
-step = LBVH build (bounds, Morton, onesweep sort, Karras, refit) followed by
+step = LBVH build (bounds, Morton, onesweep sort, Karras, refit, collapse to
+8-wide nodes unless --node-width 2) followed by
 the wavefront path tracer at 1920 x 1080, 8 spp, max depth 5, run with NEE
 and with PATH.  Scene data is resident in HBM before timing starts.
 
@@ -34,7 +35,7 @@ N_SPHERE = 4_194_304
 WIDTH, HEIGHT, SPP, DEPTH, SEED = 1920, 1080, 8, 5, 2026
 FOV_SCALE = 0.41421356  # tan(22.5 deg)
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 64  # one binary node: both child boxes + children
+NODE_BYTES = {2: 64, 8: 96}  # binary node (both child boxes + children); 8-wide quantized node
 TRI_BYTES = 48  # v0, e1, e2, prim
 PATH_BYTES = 52  # 12 float planes + id
 SHADOW_BYTES = 48  # 11 float planes + id
@@ -42,13 +43,13 @@ HIT_MATERIAL_BYTES = 16  # surface id + albedo
 SORT_BYTES_PER_KEY_PASS = 16
 
 
-def extend_bytes(st: dict) -> float:
+def extend_bytes(st: dict, node_width: int) -> float:
     """Algorithmic bytes moved by the extend launches of one render."""
     ext = st["extension_rays"]
     continued = ext - st["rays_per_bounce"][0]  # paths written to the next queue
     return (
         ext * (PATH_BYTES + HIT_MATERIAL_BYTES)
-        + st["extension_node_visits"] * NODE_BYTES
+        + st["extension_node_visits"] * NODE_BYTES[node_width]
         + st["extension_leaf_tests"] * TRI_BYTES
         + continued * PATH_BYTES
         + st["shadow_rays"] * SHADOW_BYTES
@@ -83,6 +84,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--node-width", type=int, default=8, choices=(2, 8), help="BVH layout: binary or 8-wide")
+    ap.add_argument("--leaf-size", type=int, default=4, help="8-wide leaf size (triangles per leaf child)")
     ap.add_argument(
         "--copy-calibration-mib",
         type=int,
@@ -109,7 +112,7 @@ def main() -> int:
     dev = ctx.torch_device
     print(f"[bench] device {ctx.name()}", file=sys.stderr, flush=True)
     scene = scenes.icosphere_scene(N_SPHERE)
-    gscene = GpuScene(scene, ctx)
+    gscene = GpuScene(scene, ctx, node_width=args.node_width, leaf_size=args.leaf_size)
     cam = Camera((0.0, 0.0, -3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), FOV_SCALE).flatten()
     settings = RenderSettings(WIDTH, HEIGHT, SPP, SEED, DEPTH)
     pixels = {a: torch.empty((HEIGHT, WIDTH, 3), dtype=torch.float32, device=dev) for a in (RENDER.NEE, RENDER.PATH)}
@@ -155,7 +158,7 @@ def main() -> int:
     kernel_ms = sum(float(np.mean([st[a]["extend_ms"] + st[a]["shadow_ms"] for st in stats])) for a in last)
     ext_ms_nee = float(np.mean([st[RENDER.NEE]["extend_ms"] for st in stats]))
     ext_launches = nee["extend_launches"]
-    ext_bytes = extend_bytes(nee)
+    ext_bytes = extend_bytes(nee, args.node_width)
     achieved = ext_bytes / (ext_ms_nee / 1e3) / 1e9
     stages = gscene.rebuild(timings=True)
     img = pixels[RENDER.NEE].float().mean(dim=(0, 1)).tolist()
@@ -178,6 +181,7 @@ def main() -> int:
             "workload": "icosphere-4M + quad light: LBVH build, then wavefront NEE and PATH renders "
             "at 1920x1080x8 spp, max depth 5 (Lambertian albedo U[0.2,0.8]^3, emission 15)",
             "triangles": gscene.triangle_count,
+            "bvh": "8-wide quantized (leaf size %d)" % args.leaf_size if args.node_width == 8 else "binary LBVH",
             "paths_per_render": WIDTH * HEIGHT * SPP,
         },
         "rays_per_step": rays_per_step,

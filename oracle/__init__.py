@@ -69,6 +69,13 @@ _SIG = {
         c_uint64,
         [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_uint64, c_int, c_void_p, c_void_p],
     ),
+    "orc_hploc_topology": (
+        c_uint32,
+        [c_void_p, c_void_p, c_void_p, c_uint64, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p],
+    ),
+    "orc_preorder_topology": (c_uint64, [c_uint32, c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
+    "orc_preorder_binary": (c_uint64, [c_uint32, c_void_p, c_void_p, c_uint64, c_void_p]),
     "orc_render": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "orc_render_fp64": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
 }
@@ -419,3 +426,60 @@ def render_fp64(vertices, surfaces, lambertian, emissive, width, height, spp, ma
     pixels = np.empty(3 * width * height, np.float64)
     lib().orc_render_fp64(_p(v), v.size // 9, _p(surf), _p(lamb), _p(emit), ctypes.byref(st), _p(f32(cam20)), _p(pixels), threads)
     return pixels
+
+
+# ---- H-PLOC ------------------------------------------------------------------
+
+HPLOC_SEARCH_RADIUS = 8
+HPLOC_MERGING_THRESHOLD = 16
+
+
+def hploc_topology(leaf_bounds6, sorted_codes, sorted_ids, search_radius=HPLOC_SEARCH_RADIUS,
+                   merging_threshold=HPLOC_MERGING_THRESHOLD) -> dict:
+    """bajo's CPU H-PLOC topology: nodes 0..n-1 are leaves (by leaf id), inner
+    nodes n..2n-2 in creation order."""
+    b = f32(leaf_bounds6)
+    codes = np.ascontiguousarray(sorted_codes, np.uint32)
+    ids = np.ascontiguousarray(sorted_ids, np.uint32)
+    n = codes.size
+    m = 2 * n - 1
+    out = {k: np.empty(m, np.uint32) for k in ("parent", "left", "right", "leaf_id", "leaf_count")}
+    out["bounds"] = np.empty(6 * m, np.float32)
+    stats = np.zeros(6, np.uint64)
+    root = lib().orc_hploc_topology(
+        _p(b), _p(codes), _p(ids), n, search_radius, merging_threshold, _p(out["parent"]), _p(out["left"]),
+        _p(out["right"]), _p(out["leaf_id"]), _p(out["leaf_count"]), _p(out["bounds"]), _p(stats),
+    )
+    if root == SENTINEL:
+        raise RuntimeError("orc_hploc_topology failed")
+    out["root"] = int(root)
+    out["stats"] = dict(zip(
+        ("guide_nodes", "merge_calls", "merge_rounds", "hierarchical_rounds", "final_rounds", "max_cluster_count"),
+        (int(x) for x in stats),
+    ))
+    return out
+
+
+def hploc_quality(t: dict) -> float:
+    """Sum of node surface areas over the root's (bajo's HplocTopology.quality)."""
+    b = t["bounds"].reshape(-1, 6).astype(np.float64)
+    d = b[:, 3:] - b[:, :3]
+    area = 2 * (d[:, 0] * d[:, 1] + d[:, 0] * d[:, 2] + d[:, 1] * d[:, 2])
+    root = area[t["root"]]
+    return float(area.sum() / root) if root > 0 else 0.0
+
+
+def preorder_topology(t: dict) -> np.ndarray:
+    m = t["left"].size
+    out = np.empty(m, np.uint32)
+    k = lib().orc_preorder_topology(t["root"], _p(t["left"]), _p(t["right"]), _p(t["leaf_id"]), m, _p(out))
+    return out[:k]
+
+
+def preorder_binary(root: int, node_meta: np.ndarray, leaf_ids: np.ndarray) -> np.ndarray:
+    meta = np.ascontiguousarray(node_meta, np.uint32)
+    ids = np.ascontiguousarray(leaf_ids, np.uint32)
+    n = ids.size
+    out = np.empty(2 * n, np.uint32)
+    k = lib().orc_preorder_binary(root, _p(meta) if meta.size else None, _p(ids), n, _p(out))
+    return out[:k]

@@ -66,6 +66,16 @@ uint64_t orc_collapse_wide(const uint32_t *node_meta, const float *node_bounds, 
 uint64_t orc_trace_wide(const float *vertices, uint64_t n, const uint32_t *leaf_ids, const void *wide,
                         uint64_t wide_count, const float *rays, uint64_t R, int mode, float *hits, uint32_t *stats);
 
+/* H-PLOC topology (orc_hploc.c) */
+uint32_t orc_hploc_topology(const float *leaf_bounds6, const uint32_t *sorted_codes, const uint32_t *sorted_ids,
+                            uint64_t n, int search_radius, int merging_threshold, uint32_t *parent, uint32_t *left,
+                            uint32_t *right, uint32_t *leaf_id, uint32_t *leaf_count, float *bounds6,
+                            uint64_t *stats6);
+uint64_t orc_preorder_topology(uint32_t root, const uint32_t *left, const uint32_t *right, const uint32_t *leaf_id,
+                               uint64_t node_count, uint32_t *out);
+uint64_t orc_preorder_binary(uint32_t root, const uint32_t *node_meta, const uint32_t *leaf_ids, uint64_t n,
+                             uint32_t *out);
+
 /* wavefront renderer (orc_render.c) */
 typedef struct {
     uint32_t width, height, spp, max_depth;

@@ -16,6 +16,7 @@ BAJO_OK = 0
 BAJO_TRACE_CLOSEST_HIT = 0
 BAJO_TRACE_ANY_HIT = 1
 BAJO_BUILD_LBVH = 0
+BAJO_BUILD_HPLOC = 1
 BAJO_WIDTH_BINARY = 2
 BAJO_WIDTH_WIDE8 = 8
 

@@ -36,7 +36,10 @@ class TRACE:
 
 
 class GpuBvhBuildMethod:
+    """bvh/gpu/builder GpuBvhBuildMethod."""
+
     LBVH = _lib.BAJO_BUILD_LBVH
+    HPLOC = _lib.BAJO_BUILD_HPLOC
 
 
 # One 8-wide node (96 bytes): child i spans origin + [lo, hi] * 2^(exp - 127)
@@ -84,12 +87,13 @@ class GpuTriangleBvh:
     """Ready-to-trace triangle BVH (bvh/gpu/triangle_bvh.mojo:494-614)."""
 
     def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, prim_count: int, node_width: int = 2,
-                 leaf_size: int = 0):
+                 leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH):
         self._handle = handle
         self.ctx = ctx
         self.prim_count = prim_count
         self.node_width = node_width
         self.leaf_size = leaf_size
+        self.method = method
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -107,7 +111,7 @@ class GpuTriangleBvh:
         n = _check_vertices(vertices, self.ctx)
         t = _lib.BuildTimings() if timings else None
         handle = ctypes.c_void_p(self.handle.value)
-        opts = build_options(self.node_width, self.leaf_size)
+        opts = build_options(self.node_width, self.leaf_size, self.method)
         _lib.check(
             _lib.lib().bajo_build_triangle_bvh_ex(
                 self.ctx.handle, ptr(vertices), n, ctypes.byref(opts), ctypes.byref(t) if t else None,
@@ -216,17 +220,18 @@ def _check_vertices(vertices: torch.Tensor, ctx: DeviceContext) -> int:
 
 def build_triangle_bvh(
     vertices: torch.Tensor, ctx: DeviceContext | None = None, timings: dict | None = None, node_width: int = 2,
-    leaf_size: int = 0,
+    leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH,
 ):
-    """build_triangle_bvh[WORLD, W, L, LBVH] (bvh/gpu/triangle_bvh.mojo:617-630).
+    """build_triangle_bvh[WORLD, W, L, method] (bvh/gpu/triangle_bvh.mojo:617-630).
 
-    node_width 2 keeps the binary LBVH; 8 collapses it into quantized 8-wide
-    nodes whose leaf children hold up to `leaf_size` triangles (0 = 4)."""
+    method LBVH or HPLOC builds the binary hierarchy; node_width 2 keeps it,
+    8 collapses it into quantized 8-wide nodes whose leaf children hold up to
+    `leaf_size` triangles (0 = 4)."""
     ctx = ctx or default_context(vertices.device.index or 0)
     n = _check_vertices(vertices, ctx)
     t = _lib.BuildTimings() if timings is not None else None
     handle = ctypes.c_void_p()
-    opts = build_options(node_width, leaf_size)
+    opts = build_options(node_width, leaf_size, method)
     _lib.check(
         _lib.lib().bajo_build_triangle_bvh_ex(
             ctx.handle, ptr(vertices), n, ctypes.byref(opts), ctypes.byref(t) if t else None, ctypes.byref(handle)
@@ -234,12 +239,12 @@ def build_triangle_bvh(
     )
     if t is not None:
         timings.update(t.as_dict())
-    return GpuTriangleBvh(handle, ctx, n, node_width, leaf_size)
+    return GpuTriangleBvh(handle, ctx, n, node_width, leaf_size, method)
 
 
 def build_triangle_bvh_measured(vertices: torch.Tensor, ctx: DeviceContext | None = None, node_width: int = 2,
-                                leaf_size: int = 0):
+                                leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH):
     """build_triangle_bvh_measured (bvh/gpu/triangle_bvh.mojo:633-646): returns (bvh, timings)."""
     t: dict = {}
-    bvh = build_triangle_bvh(vertices, ctx=ctx, timings=t, node_width=node_width, leaf_size=leaf_size)
+    bvh = build_triangle_bvh(vertices, ctx=ctx, timings=t, node_width=node_width, leaf_size=leaf_size, method=method)
     return bvh, t

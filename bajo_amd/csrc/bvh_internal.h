@@ -65,9 +65,32 @@ struct bajo_bvh {
   uint64_t wide_capacity;
   uint32_t wide_count;
   uint32_t wide_leaf;  // largest leaf child, 1..kWideMaxLeaf
+  // H-PLOC (method BAJO_BUILD_HPLOC): guide LBVH and cluster workspace
+  int method;
+  BjNode* hp_guide;       // n-1 guide nodes (Karras over the sorted keys)
+  uint32_t* hp_first;     // n-1 first sorted position of each guide node
+  void* hp_clusters;      // n cluster entries of 32 bytes
+  uint32_t* hp_counts;    // n-1 cluster list lengths
+  uint32_t* hp_pos;       // n depth-first leaf positions
+  uint32_t* hp_ctl;       // node counter, status, root
+  uint32_t* host_hp_ctl;  // pinned host mirror of hp_ctl
+  uint64_t hp_capacity;
 };
 
 namespace bj {
-// Collapses h's binary LBVH into h->wide (8-wide quantized nodes); wide.hip.
+// Collapses h's binary hierarchy into h->wide (8-wide quantized nodes); wide.hip.
 int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h);
+// Builds h's binary hierarchy with H-PLOC over the sorted keys; hploc.hip.
+int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices);
+// Karras topology over sorted 64-bit keys (lbvh.hip); `range_first` may be NULL.
+__global__ void karras_kernel(const uint64_t* __restrict__ keys, uint64_t n_u, BjNode* __restrict__ nodes,
+                              uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ leaf_counts,
+                              uint32_t* __restrict__ flags, uint32_t* __restrict__ range_first);
+// Triangle i's vertices from the f32[9N] input.
+BJ_D inline void load_tri_vertices(const float* __restrict__ v, uint64_t i, v3& a, v3& b, v3& c) {
+  const float* t = v + 9 * i;
+  a = mk(t[0], t[1], t[2]);
+  b = mk(t[3], t[4], t[5]);
+  c = mk(t[6], t[7], t[8]);
+}
 }  // namespace bj

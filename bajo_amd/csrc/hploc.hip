@@ -1,0 +1,412 @@
+// H-PLOC binary BVH builder for gfx950.
+//
+// Plays the role of bajo's GPU H-PLOC (bvh/gpu/builder/hploc_multi_wave.mojo:
+// 187-777, hploc_binary.mojo) and produces the topology of its deterministic
+// CPU definition (bvh/cpu/builder/hploc.mojo:450-663): the LBVH over the
+// sorted Morton keys guides a bottom-up pass; the second thread to arrive at
+// a guide node concatenates the cluster lists of its two halves (left then
+// right) and, while the list holds more than the merging threshold (16; 1 at
+// the root), runs merge rounds: each cluster picks the neighbour within
+// radius 8 whose merged box has the least surface area (right before left,
+// nearer first, strict '<'), mutual pairs merge into new nodes in list
+// order, and the list is compacted in order.
+//
+// MI355X design:
+//   - one thread per sorted leaf climbs the guide hierarchy; arrival is a
+//     relaxed agent-scope counter per guide node, and every datum handed to
+//     another wave (cluster lists, their counts, output nodes) is written
+//     with write-through stores and read with coherent loads, so no L2
+//     writeback or invalidate is needed across the 8 XCDs;
+//   - a guide node that needs merging is processed by its whole wave: up to
+//     32 clusters sit one per lane, neighbour boxes move by lane shuffles,
+//     mutual pairs are found with one ballot, node indices are reserved with
+//     one atomic per round, and the list is compacted with a forward lane
+//     permute (ds_permute);
+//   - output nodes use the 64-byte binary record of the LBVH; a last pass
+//     renumbers the leaves in depth-first order (so every subtree covers a
+//     contiguous range of sorted triangles, which the 8-wide collapse and the
+//     leaf records rely on), packs the triangles in that order, and writes
+//     each node's last leaf position as its fence.
+#include "bvh_internal.h"
+#include "sort_internal.h"
+
+namespace bj {
+namespace {
+
+constexpr int kHpBlock = 256;
+constexpr int kHpRadius = 8;
+constexpr uint32_t kHpThreshold = 16;
+constexpr int kHpMaxLoop = 512;  // climb iterations per wave (depth is far smaller)
+// ctl words
+constexpr int kHpNext = 0, kHpStatus = 1, kHpRoot = 2;
+constexpr uint32_t kHpStatusNoProgress = 1, kHpStatusBadList = 2;
+
+// A cluster: subtree reference (bit 31 = leaf at a sorted position), its
+// leaf count and its box; 32 bytes.
+struct __attribute__((aligned(16))) HpEntry {
+  uint32_t ref, cnt;
+  float lo[3], hi[3];
+};
+
+BJ_D void store_wt(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+BJ_D void store_wt(float* p, float v) {
+  __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __builtin_bit_cast(uint32_t, v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+BJ_D uint32_t load_co(const uint32_t* p) {
+  return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+BJ_D float load_co(const float* p) {
+  return __builtin_bit_cast(float, load_co(reinterpret_cast<const uint32_t*>(p)));
+}
+
+BJ_D void store_entry(HpEntry* p, const HpEntry& e) {
+  store_wt(&p->ref, e.ref);
+  store_wt(&p->cnt, e.cnt);
+  for (int k = 0; k < 3; ++k) {
+    store_wt(&p->lo[k], e.lo[k]);
+    store_wt(&p->hi[k], e.hi[k]);
+  }
+}
+BJ_D HpEntry load_entry(const HpEntry* p) {
+  HpEntry e;
+  e.ref = load_co(&p->ref);
+  e.cnt = load_co(&p->cnt);
+  for (int k = 0; k < 3; ++k) {
+    e.lo[k] = load_co(&p->lo[k]);
+    e.hi[k] = load_co(&p->hi[k]);
+  }
+  return e;
+}
+
+BJ_D void publish_fence() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// bvh/cpu/builder/hploc.mojo:130-142: 2 (dx dy + dx dz + dy dz) of the merged box.
+BJ_D float merged_area(const HpEntry& a, const float* o) {
+  const float dx = fmax_sel(a.hi[0], o[3]) - fmin_sel(a.lo[0], o[0]);
+  const float dy = fmax_sel(a.hi[1], o[4]) - fmin_sel(a.lo[1], o[1]);
+  const float dz = fmax_sel(a.hi[2], o[5]) - fmin_sel(a.lo[2], o[2]);
+  return 2.0f * (dx * dy + dx * dz + dy * dz);
+}
+
+BJ_D uint32_t shfl_u(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, 64); }
+BJ_D float shfl_f(float v, int src) { return __shfl(v, src, 64); }
+BJ_D uint32_t permute_u(uint32_t dst_lane, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_ds_permute((int)(dst_lane << 2), (int)v);
+}
+BJ_D float permute_f(uint32_t dst_lane, float v) {
+  return __builtin_bit_cast(float, permute_u(dst_lane, __builtin_bit_cast(uint32_t, v)));
+}
+
+// Writes output node `idx` = (a, b) with write-through stores; sets the
+// children's parent links (inner children; leaves keep theirs in the
+// per-position parent array).
+BJ_D void write_node(BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_counts,
+                     uint32_t* __restrict__ leaf_parent, uint32_t idx, const HpEntry& a, const HpEntry& b) {
+  BjNode* nd = nodes + idx;
+  for (int k = 0; k < 3; ++k) {
+    store_wt(&nd->b[k], a.lo[k]);
+    store_wt(&nd->b[3 + k], a.hi[k]);
+    store_wt(&nd->b[6 + k], b.lo[k]);
+    store_wt(&nd->b[9 + k], b.hi[k]);
+  }
+  store_wt(&nd->left, a.ref);
+  store_wt(&nd->right, b.ref);
+  store_wt(leaf_counts + idx, a.cnt + b.cnt);
+  if (a.ref & kLeafFlag) store_wt(leaf_parent + (a.ref & kIndexMask), idx);
+  else store_wt(&nodes[a.ref].parent, idx);
+  if (b.ref & kLeafFlag) store_wt(leaf_parent + (b.ref & kIndexMask), idx);
+  else store_wt(&nodes[b.ref].parent, idx);
+}
+
+// The wave processes guide node `g` (all 64 lanes call this together).
+BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
+                       const uint32_t* __restrict__ range_first, HpEntry* __restrict__ clusters,
+                       uint32_t* __restrict__ counts, BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_counts,
+                       uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ ctl) {
+  const uint32_t lane = lane_id();
+  const uint32_t first = range_first[g], last = guide[g].fence;
+  const uint32_t gl = guide[g].left, gr = guide[g].right;
+  const uint32_t split = gl & kIndexMask;
+  const uint32_t cl = (gl & kLeafFlag) ? 1u : load_co(counts + split);
+  const uint32_t cr = (gr & kLeafFlag) ? 1u : load_co(counts + (gr & kIndexMask));
+  uint32_t c = cl + cr;
+  if (cl > kHpThreshold || cr > kHpThreshold || cl == 0 || cr == 0 || first > split || split >= last ||
+      (uint64_t)last >= n || cl > split - first + 1 || cr > last - split) {
+    if (lane == 0) atomicOr(ctl + kHpStatus, kHpStatusBadList);
+    return;
+  }
+  const bool root = first == 0 && (uint64_t)last == n - 1;
+  const uint32_t thr = root ? 1u : kHpThreshold;
+  HpEntry e{};
+  if (lane < cl) e = load_entry(clusters + first + lane);
+  else if (lane < c) e = load_entry(clusters + split + 1 + (lane - cl));
+  for (int round = 0; c > thr && round < 64; ++round) {
+    const bool live = lane < c;
+    float best_area = kFltMax;
+    int best = -1;
+    for (int r = 1; r <= kHpRadius; ++r) {
+      float o[6];
+      const int rs = (int)lane + r;  // right neighbour
+      for (int k = 0; k < 3; ++k) {
+        o[k] = shfl_f(e.lo[k], rs & 63);
+        o[3 + k] = shfl_f(e.hi[k], rs & 63);
+      }
+      if (live && rs < (int)c) {
+        const float a = merged_area(e, o);
+        if (a < best_area) {
+          best_area = a;
+          best = rs;
+        }
+      }
+      const int ls = (int)lane - r;  // left neighbour
+      for (int k = 0; k < 3; ++k) {
+        o[k] = shfl_f(e.lo[k], ls & 63);
+        o[3 + k] = shfl_f(e.hi[k], ls & 63);
+      }
+      if (live && ls >= 0) {
+        const float a = merged_area(e, o);
+        if (a < best_area) {
+          best_area = a;
+          best = ls;
+        }
+      }
+    }
+    const int nb_best = (int)shfl_u((uint32_t)best, best < 0 ? (int)lane : best);
+    bool mutual = live && best >= 0 && nb_best == (int)lane;
+    uint64_t mergers = ballot(mutual && (int)lane < best);
+    if (mergers == 0) {
+      // no mutual pair (only with degenerate boxes): merge the first two
+      // clusters (bvh/cpu/builder/hploc.mojo:470-500)
+      best = lane == 0 ? 1 : (lane == 1 ? 0 : -1);
+      mutual = lane < 2;
+      mergers = 1;
+    }
+    const uint32_t k = (uint32_t)__builtin_popcountll(mergers);
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(ctl + kHpNext, k);
+    base = shfl_u(base, 0);
+    // partner entry
+    const int src = best < 0 ? (int)lane : best;
+    HpEntry p;
+    p.ref = shfl_u(e.ref, src);
+    p.cnt = shfl_u(e.cnt, src);
+    for (int q = 0; q < 3; ++q) {
+      p.lo[q] = shfl_f(e.lo[q], src);
+      p.hi[q] = shfl_f(e.hi[q], src);
+    }
+    const bool merger = (mergers >> lane) & 1u;
+    if (merger) {
+      const uint32_t idx = base + mask_rank(mergers);
+      const bool refs_ok = ((e.ref & kLeafFlag) ? (uint64_t)(e.ref & kIndexMask) < n : (uint64_t)e.ref < n - 1) &&
+                           ((p.ref & kLeafFlag) ? (uint64_t)(p.ref & kIndexMask) < n : (uint64_t)p.ref < n - 1);
+      if ((uint64_t)idx < n - 1 && refs_ok) write_node(nodes, leaf_counts, leaf_parent, idx, e, p);
+      else atomicOr(ctl + kHpStatus, kHpStatusBadList);
+      e.ref = idx;
+      e.cnt += p.cnt;
+      for (int q = 0; q < 3; ++q) {
+        e.lo[q] = fmin_sel(e.lo[q], p.lo[q]);
+        e.hi[q] = fmax_sel(e.hi[q], p.hi[q]);
+      }
+    }
+    // compaction: survivors and new nodes keep their order; the higher
+    // member of each merged pair leaves the list
+    const bool keep = live && !(mutual && (int)lane > best);
+    const uint64_t kept = ballot(keep);
+    const uint32_t dst = keep ? mask_rank(kept) : 63u;
+    HpEntry m;
+    m.ref = permute_u(dst, e.ref);
+    m.cnt = permute_u(dst, e.cnt);
+    for (int q = 0; q < 3; ++q) {
+      m.lo[q] = permute_f(dst, e.lo[q]);
+      m.hi[q] = permute_f(dst, e.hi[q]);
+    }
+    e = m;
+    c = (uint32_t)__builtin_popcountll(kept);
+  }
+  if (c > thr) {
+    if (lane == 0) atomicOr(ctl + kHpStatus, kHpStatusNoProgress);
+    return;
+  }
+  if (lane < c) store_entry(clusters + first + lane, e);
+  if (lane == 0) {
+    store_wt(counts + g, c);
+    if (root) store_wt(ctl + kHpRoot, e.ref);
+  }
+  publish_fence();
+}
+
+__global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict__ vertices,
+                                                         const uint64_t* __restrict__ keys, uint64_t n,
+                                                         const BjNode* __restrict__ guide,
+                                                         const uint32_t* __restrict__ range_first,
+                                                         uint32_t* __restrict__ leaf_parent,
+                                                         uint32_t* __restrict__ flags, HpEntry* __restrict__ clusters,
+                                                         uint32_t* __restrict__ counts, BjNode* __restrict__ nodes,
+                                                         uint32_t* __restrict__ leaf_counts,
+                                                         uint32_t* __restrict__ ctl) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = i < n;
+  bool pending = false;
+  uint32_t cur = kLeafFlag | (uint32_t)i;
+  if (active) {
+    const uint32_t prim = (uint32_t)keys[i];
+    v3 a, b, c;
+    load_tri_vertices(vertices, prim, a, b, c);
+    // bvh/gpu/triangle_bvh.mojo:711-712: leaf box = vmin(vmin(v0,v1),v2)
+    const v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
+    HpEntry e{kLeafFlag | (uint32_t)i, 1u, {lo.x, lo.y, lo.z}, {hi.x, hi.y, hi.z}};
+    store_entry(clusters + i, e);
+  }
+  publish_fence();
+  for (int it = 0; it < kHpMaxLoop; ++it) {
+    if (active && !pending) {
+      const uint32_t parent = (cur & kLeafFlag) ? leaf_parent[cur & kIndexMask] : guide[cur].parent;
+      if (parent == kSentinel || (uint64_t)parent >= n - 1) {
+        active = false;
+      } else {
+        const uint32_t old = __hip_atomic_fetch_add(flags + parent, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) {
+          active = false;
+        } else {
+          cur = parent;
+          pending = true;
+        }
+      }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    uint64_t todo = ballot(pending);
+    while (todo) {
+      const int leader = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const uint32_t g = shfl_u(cur, leader);
+      process_node(g, n, guide, range_first, clusters, counts, nodes, leaf_counts, leaf_parent, ctl);
+      if ((int)lane_id() == leader) {
+        pending = false;
+        if (range_first[g] == 0 && (uint64_t)guide[g].fence == n - 1) active = false;
+      }
+    }
+    if (!ballot(active)) return;
+  }
+  if (active) atomicOr(ctl + kHpStatus, kHpStatusNoProgress);
+}
+
+__global__ void hploc_init_kernel(uint32_t* __restrict__ ctl) {
+  if (threadIdx.x < 4) ctl[threadIdx.x] = 0;
+}
+
+// The root gets no parent from a merge.
+__global__ void hploc_root_kernel(BjNode* __restrict__ nodes, uint64_t n, const uint32_t* __restrict__ ctl) {
+  if (threadIdx.x != 0) return;
+  const uint32_t root = ctl[kHpRoot];
+  if (!(root & kLeafFlag) && (uint64_t)root < n - 1) nodes[root].parent = kSentinel;
+}
+
+// Depth-first position of each sorted leaf: the leaves of the left subtrees
+// of every ancestor reached from the right come first.  Also writes the
+// fence (last depth-first position) of every node whose rightmost leaf this
+// is, and which side of its parent the leaf hangs on (bit 31 of `pos`).
+__global__ void __launch_bounds__(kHpBlock) dfs_position_kernel(BjNode* nodes, uint64_t n,
+                                                                const uint32_t* __restrict__ leaf_parent,
+                                                                const uint32_t* __restrict__ leaf_counts,
+                                                                uint32_t* __restrict__ pos_out) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t ref = kLeafFlag | (uint32_t)j;
+  uint32_t parent = leaf_parent[j];
+  uint32_t pos = 0;
+  bool leaf_is_right = false;
+  for (int depth = 0; parent != kSentinel && (uint64_t)parent < n - 1 && depth < 4096; ++depth) {
+    const uint32_t l = nodes[parent].left;
+    const bool from_right = l != ref;
+    if (depth == 0) leaf_is_right = from_right;
+    if (from_right) pos += (l & kLeafFlag) ? 1u : ((uint64_t)l < n - 1 ? leaf_counts[l] : 0u);
+    ref = parent;
+    parent = nodes[parent].parent;
+  }
+  pos_out[j] = pos | (leaf_is_right ? 0x80000000u : 0u);
+  // fences: ancestors of which this leaf is the rightmost
+  ref = kLeafFlag | (uint32_t)j;
+  parent = leaf_parent[j];
+  for (int depth = 0; parent != kSentinel && (uint64_t)parent < n - 1 && depth < 4096; ++depth) {
+    if (nodes[parent].right != ref) break;
+    nodes[parent].fence = pos;
+    ref = parent;
+    parent = nodes[parent].parent;
+  }
+}
+
+// Moves each leaf to its depth-first position: triangle record, key, parent
+// link, and the parent's child reference.
+__global__ void __launch_bounds__(kHpBlock) dfs_emit_kernel(const float* __restrict__ vertices,
+                                                            const uint64_t* __restrict__ keys, uint64_t n,
+                                                            const uint32_t* __restrict__ pos_in,
+                                                            const uint32_t* __restrict__ leaf_parent,
+                                                            BjNode* __restrict__ nodes, BjTri* __restrict__ tris,
+                                                            uint64_t* __restrict__ keys_out,
+                                                            uint32_t* __restrict__ leaf_parent_out) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t pw = pos_in[j];
+  const uint32_t pos = pw & 0x7FFFFFFFu;
+  if ((uint64_t)pos >= n) return;
+  const uint64_t key = keys[j];
+  const uint32_t prim = (uint32_t)key;
+  v3 a, b, c;
+  load_tri_vertices(vertices, prim, a, b, c);
+  const v3 e1 = sub(b, a), e2 = sub(c, a);
+  float4* out = reinterpret_cast<float4*>(tris + pos);
+  out[0] = make_float4(a.x, a.y, a.z, __builtin_bit_cast(float, prim));
+  out[1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+  out[2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+  keys_out[pos] = key;
+  const uint32_t parent = leaf_parent[j];
+  leaf_parent_out[pos] = parent;
+  if (parent != kSentinel && (uint64_t)parent < n - 1) {
+    if (pw & 0x80000000u) nodes[parent].right = kLeafFlag | pos;
+    else nodes[parent].left = kLeafFlag | pos;
+  }
+}
+
+}  // namespace
+
+int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
+  hipStream_t st = ctx->stream;
+  const uint64_t n = h->n;
+  if (n < 2) return BAJO_OK;  // a single leaf is its own root; the LBVH path packed it
+  HpEntry* clusters = reinterpret_cast<HpEntry*>(h->hp_clusters);
+  hploc_init_kernel<<<1, 64, 0, st>>>(h->hp_ctl);
+  karras_kernel<<<div_up(n - 1, kHpBlock), kHpBlock, 0, st>>>(h->keys, n, h->hp_guide, h->leaf_parent,
+                                                               h->leaf_counts, h->flags, h->hp_first);
+  hploc_kernel<<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
+                                                         h->leaf_parent, h->flags, clusters, h->hp_counts, h->nodes,
+                                                         h->leaf_counts, h->hp_ctl);
+  hploc_root_kernel<<<1, 64, 0, st>>>(h->nodes, n, h->hp_ctl);
+  BJ_HIP(hipGetLastError());
+  // the cluster area is free now: it holds the reordered keys and leaf parents
+  uint64_t* keys_dfs = reinterpret_cast<uint64_t*>(h->hp_clusters);
+  uint32_t* parent_dfs = reinterpret_cast<uint32_t*>(keys_dfs + n);
+  dfs_position_kernel<<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(h->nodes, n, h->leaf_parent,
+                                                                h->leaf_counts, h->hp_pos);
+  dfs_emit_kernel<<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_pos, h->leaf_parent,
+                                                            h->nodes, h->tris, keys_dfs, parent_dfs);
+  BJ_HIP(hipGetLastError());
+  BJ_HIP(hipMemcpyAsync(h->keys, keys_dfs, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+  BJ_HIP(hipMemcpyAsync(h->leaf_parent, parent_dfs, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+  BJ_HIP(hipMemcpyAsync(h->host_hp_ctl, h->hp_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  BJ_HIP(hipStreamSynchronize(st));
+  const uint32_t* c = h->host_hp_ctl;
+  BJ_REQUIRE(c[kHpStatus] == 0, "H-PLOC build failed with status %u (1 = no progress, 2 = corrupt cluster list)",
+             c[kHpStatus]);
+  BJ_REQUIRE((uint64_t)c[kHpNext] == n - 1, "H-PLOC built %u of %llu nodes", c[kHpNext],
+             (unsigned long long)(n - 1));
+  BJ_REQUIRE(!(c[kHpRoot] & kLeafFlag) && (uint64_t)c[kHpRoot] < n - 1, "H-PLOC root %u is invalid", c[kHpRoot]);
+  h->root = c[kHpRoot];
+  return BAJO_OK;
+}
+
+}  // namespace bj

@@ -149,10 +149,15 @@ BJ_D int karras_delta(const uint64_t* __restrict__ keys, int64_t n, int64_t i, u
   return x == 0 ? 64 : __builtin_clzll(x);
 }
 
+}  // namespace
+
+// Also the guide hierarchy of the H-PLOC builder, which needs each node's
+// first sorted position (`range_first`, NULL for the plain LBVH).
 __global__ void __launch_bounds__(kBlock) karras_kernel(const uint64_t* __restrict__ keys, uint64_t n_u,
                                                         BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_parent,
                                                         uint32_t* __restrict__ leaf_counts,
-                                                        uint32_t* __restrict__ flags) {
+                                                        uint32_t* __restrict__ flags,
+                                                        uint32_t* __restrict__ range_first) {
   const int64_t n = (int64_t)n_u;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
@@ -183,6 +188,7 @@ __global__ void __launch_bounds__(kBlock) karras_kernel(const uint64_t* __restri
   }
   flags[i] = 0;
   leaf_counts[i] = (uint32_t)(last - first + 1);
+  if (range_first) range_first[i] = (uint32_t)first;
   if (first == 0 && last == n - 1) nodes[i].parent = kSentinel;
   uint32_t left, right;
   if (split == first) {
@@ -203,6 +209,8 @@ __global__ void __launch_bounds__(kBlock) karras_kernel(const uint64_t* __restri
   nodes[i].right = right;
   nodes[i].fence = (uint32_t)last;
 }
+
+namespace {
 
 __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__ vertices,
                                                        const uint64_t* __restrict__ keys, uint64_t n,
@@ -315,8 +323,35 @@ int32_t ensure_wide_capacity(bajo_bvh* h, uint64_t n) {
   return BAJO_OK;
 }
 
+void free_hploc(bajo_bvh* h) {
+  hipFree(h->hp_guide); hipFree(h->hp_first); hipFree(h->hp_clusters); hipFree(h->hp_counts); hipFree(h->hp_pos);
+  hipFree(h->hp_ctl); hipHostFree(h->host_hp_ctl);
+  h->hp_guide = nullptr; h->hp_first = nullptr; h->hp_clusters = nullptr; h->hp_counts = nullptr;
+  h->hp_pos = nullptr; h->hp_ctl = nullptr; h->host_hp_ctl = nullptr;
+  h->hp_capacity = 0;
+}
+
+// H-PLOC workspace: the guide hierarchy, range starts, 32-byte cluster
+// entries (later reused for the depth-first keys and parents), list
+// lengths and leaf positions.
+int32_t ensure_hploc_capacity(bajo_bvh* h, uint64_t n) {
+  if (n <= h->hp_capacity) return BAJO_OK;
+  free_hploc(h);
+  const uint64_t internal = n > 1 ? n - 1 : 1;
+  BJ_HIP(hipMalloc(&h->hp_guide, internal * sizeof(BjNode)));
+  BJ_HIP(hipMalloc(&h->hp_first, internal * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&h->hp_clusters, n * 32));
+  BJ_HIP(hipMalloc(&h->hp_counts, internal * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&h->hp_pos, n * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&h->hp_ctl, 16 * sizeof(uint32_t)));
+  BJ_HIP(hipHostMalloc(&h->host_hp_ctl, 16 * sizeof(uint32_t), hipHostMallocDefault));
+  h->hp_capacity = n;
+  return BAJO_OK;
+}
+
 void free_bvh(bajo_bvh* h) {
   if (!h) return;
+  free_hploc(h);
   hipFree(h->nodes); hipFree(h->tris); hipFree(h->keys); hipFree(h->leaf_parent); hipFree(h->leaf_counts);
   hipFree(h->flags); hipFree(h->acc); hipFree(h->scene); hipFree(h->sort_ws);
   free_wide(h);
@@ -348,6 +383,11 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
     rc = ensure_wide_capacity(h, n);
     if (rc) return rc;
   }
+  const bool hploc = h->method == BAJO_BUILD_HPLOC && n > 1;
+  if (hploc) {
+    rc = ensure_hploc_capacity(h, n);
+    if (rc) return rc;
+  }
   h->n = n;
   h->root = n == 1 ? (kLeafFlag | 0u) : 0u;
   SortPlan plan = make_sort_plan(n, 8, 0, kSortBeginBit, kSortEndBit);
@@ -377,15 +417,22 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
   rc = onesweep_run(ctx, plan, h->keys, nullptr, ws, true);
   if (rc) return rc;
   ev.record(3, st);
-  if (n > 1) {
-    karras_kernel<<<div_up(n - 1, kBlock), kBlock, 0, st>>>(h->keys, n, h->nodes, h->leaf_parent, h->leaf_counts,
-                                                           h->flags);
+  if (hploc) {
+    // guide topology, H-PLOC clustering and the depth-first leaf order
+    ev.record(4, st);
+    rc = build_hploc(ctx, h, vertices);
+    if (rc) return rc;
+  } else {
+    if (n > 1) {
+      karras_kernel<<<div_up(n - 1, kBlock), kBlock, 0, st>>>(h->keys, n, h->nodes, h->leaf_parent,
+                                                             h->leaf_counts, h->flags, nullptr);
+      BJ_HIP(hipGetLastError());
+    }
+    ev.record(4, st);
+    refit_kernel<<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent, h->flags,
+                                                       h->tris);
     BJ_HIP(hipGetLastError());
   }
-  ev.record(4, st);
-  refit_kernel<<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent, h->flags,
-                                                     h->tris);
-  BJ_HIP(hipGetLastError());
   ev.record(5, st);
   if (h->node_width == BAJO_WIDTH_WIDE8) {
     rc = collapse_wide(ctx, h);
@@ -427,7 +474,8 @@ int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_
                                    bajo_bvh** out) {
   bajo_build_options opt = options ? *options : bajo_build_options{BAJO_BUILD_LBVH, BAJO_WIDTH_BINARY, 0, 0};
   BJ_REQUIRE(ctx != nullptr && out != nullptr, "bajo_build_triangle_bvh: ctx/out is NULL");
-  BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH, "bajo_build_triangle_bvh: unsupported build method %d", opt.method);
+  BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH || opt.method == BAJO_BUILD_HPLOC,
+             "bajo_build_triangle_bvh: unsupported build method %d", opt.method);
   BJ_REQUIRE(opt.node_width == BAJO_WIDTH_BINARY || opt.node_width == BAJO_WIDTH_WIDE8,
              "bajo_build_triangle_bvh: node_width must be 2 or 8, got %d", opt.node_width);
   if (opt.leaf_size == 0) opt.leaf_size = (int32_t)kWideDefaultLeaf;
@@ -450,6 +498,7 @@ int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_
     BJ_REQUIRE(h->device == ctx->device, "bajo_build_triangle_bvh: handle belongs to device %d", h->device);
   }
   h->node_width = opt.node_width;
+  h->method = opt.method;
   h->wide_leaf = (uint32_t)opt.leaf_size;
   int32_t rc = bj::build_into(ctx, h, vertices, n, timings);
   if (rc) {

@@ -245,7 +245,7 @@ int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h) {
   int dev = 0, cus = 0;
   BJ_HIP(hipGetDevice(&dev));
   BJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  collapse_root_kernel<<<1, 64, 0, st>>>(h->scene, n, 0u, h->wide, h->wide_src, h->wide_ctl);
+  collapse_root_kernel<<<1, 64, 0, st>>>(h->scene, n, n > 1 ? h->root : 0u, h->wide, h->wide_src, h->wide_ctl);
   BJ_HIP(hipGetLastError());
   if (n == 1) {
     h->wide_count = 1;

@@ -80,7 +80,8 @@ class SceneData:
 class GpuScene:
     """Device-resident scene: vertices, BVH, materials and the light table."""
 
-    def __init__(self, scene: SceneData, ctx: DeviceContext | None = None, node_width: int = 2, leaf_size: int = 0):
+    def __init__(self, scene: SceneData, ctx: DeviceContext | None = None, node_width: int = 2, leaf_size: int = 0,
+                 method: int = 0):
         from .bvh import build_options
 
         self.ctx = ctx or default_context(0)
@@ -91,7 +92,7 @@ class GpuScene:
         if s.size != v.size // 9:
             raise ValueError("one surface id per triangle is required")
         handle = ctypes.c_void_p()
-        opts = build_options(node_width, leaf_size)
+        opts = build_options(node_width, leaf_size, method)
         _lib.check(
             _lib.lib().bajo_scene_create_ex(
                 self.ctx.handle,

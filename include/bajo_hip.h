@@ -39,8 +39,16 @@ extern "C" {
 #define BAJO_TRACE_CLOSEST_HIT 0
 #define BAJO_TRACE_ANY_HIT 1
 
-/* Build methods, bvh/gpu GpuBvhBuildMethod (LBVH only in this build). */
+/* Build methods, bvh/gpu GpuBvhBuildMethod (bvh/gpu/builder/__init__.mojo). */
 #define BAJO_BUILD_LBVH 0
+/* H-PLOC (bvh/gpu/builder/hploc_multi_wave.mojo:187-777): LBVH-guided
+ * bottom-up clustering, search radius 8, merging threshold 16, with the
+ * topology of bajo's CPU definition (bvh/cpu/builder/hploc.mojo:855-917).
+ * Leaves are then renumbered depth-first, so in the exported binary layout
+ * every subtree covers a contiguous range of positions and `fence` is the
+ * last one (bajo leaves FENCE unset for H-PLOC); the build synchronizes the
+ * context stream once to read back its root and status. */
+#define BAJO_BUILD_HPLOC 1
 
 /* Node widths (bajo's W parameter, bvh/gpu/triangle_bvh.mojo:617): 2 keeps
  * the binary LBVH; 8 collapses it into 8-wide nodes with 8-bit quantized
@@ -52,7 +60,7 @@ extern "C" {
 
 /* Build options (bajo's compile-time W, L and method parameters). */
 typedef struct {
-  int32_t method;     /* BAJO_BUILD_LBVH                                   */
+  int32_t method;     /* BAJO_BUILD_LBVH or BAJO_BUILD_HPLOC               */
   int32_t node_width; /* BAJO_WIDTH_BINARY or BAJO_WIDTH_WIDE8             */
   int32_t leaf_size;  /* width 8: most triangles per leaf child, 1..8; 0 = 4 */
   int32_t reserved;   /* must be 0                                         */
@@ -68,8 +76,9 @@ typedef struct {
   uint64_t bounds_ns;   /* triangle + scene/centroid bounds           */
   uint64_t morton_ns;   /* Morton keys (and the fused key histogram)  */
   uint64_t sort_ns;     /* onesweep radix sort of the 64-bit keys     */
-  uint64_t topology_ns; /* Karras hierarchy                           */
-  uint64_t refit_ns;    /* bottom-up refit + sorted triangle packing  */
+  uint64_t topology_ns; /* Karras hierarchy (LBVH; H-PLOC: 0)         */
+  uint64_t refit_ns;    /* LBVH: bottom-up refit + triangle packing;
+                           H-PLOC: guide, clustering, leaf reorder      */
   uint64_t collapse_ns; /* binary -> 8-wide collapse (width 8 only)   */
   uint64_t total_ns;
 } bajo_build_timings;

@@ -64,7 +64,9 @@ _SIG = {
     "orc_light_table": (None, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "orc_grid_exponent": (c_uint32, [c_float, c_float]),
     "orc_quantize": (None, [c_float, c_uint32, c_float, c_float, c_void_p, c_void_p]),
-    "orc_collapse_wide": (c_uint64, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, c_uint32, c_void_p, c_uint64]),
+    "orc_collapse_wide": (
+        c_uint64, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_void_p, c_uint32, c_void_p, c_uint64]
+    ),
     "orc_trace_wide": (
         c_uint64,
         [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_uint64, c_int, c_void_p, c_void_p],
@@ -76,6 +78,11 @@ _SIG = {
     ),
     "orc_preorder_topology": (c_uint64, [c_uint32, c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
     "orc_preorder_binary": (c_uint64, [c_uint32, c_void_p, c_void_p, c_uint64, c_void_p]),
+    "orc_hploc_binary": (
+        c_uint32,
+        [c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p],
+    ),
     "orc_render": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "orc_render_fp64": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
 }
@@ -91,6 +98,7 @@ class RenderSettings(ctypes.Structure):
         ("algorithm", c_int),
         ("node_width", c_int),
         ("leaf_size", c_int),
+        ("method", c_int),
     ]
 
 _lib = None
@@ -322,12 +330,12 @@ def light_table(vertices, surfaces, emissive):
 
 
 def render(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1,
-           node_width=2, leaf_size=0):
+           node_width=2, leaf_size=0, method=0):
     v = f32(vertices)
     surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
     lamb = f32(lambertian) if len(lambertian) else np.zeros(3, np.float32)
     emit = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
-    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, node_width, leaf_size)
+    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, node_width, leaf_size, method)
     pixels = np.empty(3 * width * height, np.float32)
     rays = np.zeros(2, np.uint64)
     rc = lib().orc_render(
@@ -373,7 +381,8 @@ def collapse_wide(bvh: dict, leaf_size: int = 4) -> np.ndarray:
     counts = np.ascontiguousarray(bvh["leaf_counts"], np.uint32)
     k = lib().orc_collapse_wide(
         _p(meta) if meta.size else None, _p(nb) if nb.size else None, _p(counts) if counts.size else None, n,
-        _p(f32(bvh["scene_bounds"])), leaf_size, out.ctypes.data_as(ctypes.c_void_p), cap,
+        int(bvh["root"]) if n > 1 else 0, _p(f32(bvh["scene_bounds"])), leaf_size,
+        out.ctypes.data_as(ctypes.c_void_p), cap,
     )
     if k == 0:
         raise RuntimeError("orc_collapse_wide failed")
@@ -422,7 +431,7 @@ def render_fp64(vertices, surfaces, lambertian, emissive, width, height, spp, ma
     surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
     lamb = f32(lambertian) if len(lambertian) else np.zeros(3, np.float32)
     emit = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
-    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, 2, 0)
+    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, 2, 0, 0)
     pixels = np.empty(3 * width * height, np.float64)
     lib().orc_render_fp64(_p(v), v.size // 9, _p(surf), _p(lamb), _p(emit), ctypes.byref(st), _p(f32(cam20)), _p(pixels), threads)
     return pixels
@@ -483,3 +492,41 @@ def preorder_binary(root: int, node_meta: np.ndarray, leaf_ids: np.ndarray) -> n
     out = np.empty(2 * n, np.uint32)
     k = lib().orc_preorder_binary(root, _p(meta) if meta.size else None, _p(ids), n, _p(out))
     return out[:k]
+
+
+def hploc_binary(t: dict) -> dict:
+    """The topology in the binary layout with depth-first leaf positions
+    (the device H-PLOC builder's output form, up to inner-node numbering)."""
+    n = (t["left"].size + 1) // 2
+    m = max(n - 1, 1)
+    out = {
+        "node_meta": np.zeros(4 * m, np.uint32),
+        "node_bounds": np.zeros(12 * m, np.float32),
+        "leaf_ids": np.zeros(n, np.uint32),
+        "leaf_counts": np.zeros(m, np.uint32),
+        "leaf_parent": np.zeros(n, np.uint32),
+    }
+    root = lib().orc_hploc_binary(
+        t["root"], _p(t["left"]), _p(t["right"]), _p(t["leaf_id"]), _p(t["leaf_count"]), _p(t["bounds"]), n,
+        _p(out["node_meta"]), _p(out["node_bounds"]), _p(out["leaf_ids"]), _p(out["leaf_counts"]),
+        _p(out["leaf_parent"]),
+    )
+    out["root"] = int(root)
+    if n == 1:
+        for k in ("node_meta", "node_bounds", "leaf_counts"):
+            out[k] = out[k][:0]
+    return out
+
+
+def build_hploc(vertices: np.ndarray) -> dict:
+    """Oracle H-PLOC over a triangle soup: the LBVH's sorted keys, bajo's CPU
+    topology, and its depth-first binary layout (usable by trace_bvh2 and
+    collapse_wide)."""
+    v = f32(vertices)
+    lb = build_lbvh(v)
+    codes = (lb["sorted_keys"] >> np.uint64(32)).astype(np.uint32)
+    t = hploc_topology(triangle_bounds(v), codes, lb["leaf_ids"])
+    b = hploc_binary(t)
+    b["scene_bounds"] = lb["scene_bounds"]
+    b["topology"] = t
+    return b

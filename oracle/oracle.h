@@ -61,7 +61,7 @@ void orc_rcp_direction(const float *d3, float *out3);
 uint32_t orc_grid_exponent(float o, float hi);
 void orc_quantize(float o, uint32_t e, float lo, float hi, uint8_t *qlo, uint8_t *qhi);
 uint64_t orc_collapse_wide(const uint32_t *node_meta, const float *node_bounds, const uint32_t *leaf_counts,
-                           uint64_t n, const float *scene_bounds, uint32_t leaf_size, void *out_nodes,
+                           uint64_t n, uint32_t root, const float *scene_bounds, uint32_t leaf_size, void *out_nodes,
                            uint64_t capacity);
 uint64_t orc_trace_wide(const float *vertices, uint64_t n, const uint32_t *leaf_ids, const void *wide,
                         uint64_t wide_count, const float *rays, uint64_t R, int mode, float *hits, uint32_t *stats);
@@ -75,6 +75,9 @@ uint64_t orc_preorder_topology(uint32_t root, const uint32_t *left, const uint32
                                uint64_t node_count, uint32_t *out);
 uint64_t orc_preorder_binary(uint32_t root, const uint32_t *node_meta, const uint32_t *leaf_ids, uint64_t n,
                              uint32_t *out);
+uint32_t orc_hploc_binary(uint32_t root, const uint32_t *left, const uint32_t *right, const uint32_t *leaf_id,
+                          const uint32_t *leaf_count, const float *bounds6, uint64_t n, uint32_t *node_meta,
+                          float *node_bounds, uint32_t *leaf_ids, uint32_t *leaf_counts, uint32_t *leaf_parent);
 
 /* wavefront renderer (orc_render.c) */
 typedef struct {
@@ -83,6 +86,7 @@ typedef struct {
     int algorithm; /* 0 PATH, 1 NORMALS, 2 AO, 3 NEE, 4 MIS */
     int node_width; /* 2 (binary LBVH) or 8 (collapsed 8-wide, orc_wide.c); 0 = 2 */
     int leaf_size;  /* width 8: triangles per leaf child, 1..8; 0 = 4 */
+    int method;     /* 0 LBVH, 1 H-PLOC (orc_hploc.c, depth-first leaf order) */
 } orc_render_settings;
 void orc_philox4x32_10(const uint32_t *ctr, uint32_t k0, uint32_t k1, uint32_t *out);
 void orc_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float *out);

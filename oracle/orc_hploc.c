@@ -220,3 +220,48 @@ uint64_t orc_preorder_binary(uint32_t root, const uint32_t *node_meta, const uin
     free(stack);
     return k;
 }
+
+/* The topology as a binary layout with leaves renumbered depth-first (the
+ * device builder's output form): inner topology node t becomes binary node
+ * t - n, node_meta = parent, left, right, fence (last depth-first leaf
+ * position), node_bounds = left then right child box, leaf_ids by position,
+ * leaf_counts per node.  Returns the binary root (tagged when n == 1). */
+typedef struct {
+    const uint32_t *left, *right, *leaf_id, *leaf_count;
+    const float *box;
+    uint64_t n;
+    uint32_t *meta, *leaf_ids, *counts, *leaf_parent;
+    float *nb;
+    uint32_t next_pos;
+} hb_state;
+
+static uint32_t hb_ref(hb_state *s, uint32_t t, uint32_t parent, uint32_t *last_pos) {
+    if (s->left[t] == ORC_SENTINEL) {
+        uint32_t pos = s->next_pos++;
+        s->leaf_ids[pos] = s->leaf_id[t];
+        s->leaf_parent[pos] = parent;
+        *last_pos = pos;
+        return ORC_LEAF_FLAG | pos;
+    }
+    uint32_t b = t - (uint32_t)s->n;
+    uint32_t *m = s->meta + 4 * (uint64_t)b;
+    m[0] = parent;
+    uint32_t lp = 0, rp = 0;
+    m[1] = hb_ref(s, s->left[t], b, &lp);
+    m[2] = hb_ref(s, s->right[t], b, &rp);
+    m[3] = rp;
+    memcpy(s->nb + 12 * (uint64_t)b, s->box + 6 * (uint64_t)s->left[t], 6 * sizeof(float));
+    memcpy(s->nb + 12 * (uint64_t)b + 6, s->box + 6 * (uint64_t)s->right[t], 6 * sizeof(float));
+    s->counts[b] = s->leaf_count[t];
+    *last_pos = rp;
+    return b;
+}
+
+uint32_t orc_hploc_binary(uint32_t root, const uint32_t *left, const uint32_t *right, const uint32_t *leaf_id,
+                          const uint32_t *leaf_count, const float *bounds6, uint64_t n, uint32_t *node_meta,
+                          float *node_bounds, uint32_t *leaf_ids, uint32_t *leaf_counts, uint32_t *leaf_parent) {
+    hb_state s = {left, right, leaf_id, leaf_count, bounds6, n, node_meta, leaf_ids, leaf_counts, leaf_parent,
+                  node_bounds, 0};
+    uint32_t last = 0;
+    return hb_ref(&s, root, ORC_SENTINEL, &last);
+}

@@ -393,6 +393,23 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
     float *nb = (float *)malloc(12 * m * sizeof(float));
     float scene12[12];
     uint32_t root = orc_build_lbvh(vertices, n, keys, ids, meta, parent, counts, nb, scene12);
+    if (st->method == 1 && n > 1) {
+        /* H-PLOC topology over the same sorted keys, as a depth-first binary layout */
+        uint64_t m2 = 2 * n - 1;
+        float *tb = (float *)malloc(6 * n * sizeof(float));
+        uint32_t *codes = (uint32_t *)malloc(n * sizeof(uint32_t));
+        uint32_t *tp = (uint32_t *)malloc(5 * m2 * sizeof(uint32_t));
+        float *tbox = (float *)malloc(6 * m2 * sizeof(float));
+        orc_triangle_bounds(vertices, n, tb);
+        for (uint64_t i = 0; i < n; ++i) codes[i] = (uint32_t)(keys[i] >> 32);
+        uint32_t troot = orc_hploc_topology(tb, codes, ids, n, 8, 16, tp, tp + m2, tp + 2 * m2, tp + 3 * m2,
+                                            tp + 4 * m2, tbox, NULL);
+        if (troot != ORC_SENTINEL)
+            root = orc_hploc_binary(troot, tp + m2, tp + 2 * m2, tp + 3 * m2, tp + 4 * m2, tbox, n, meta, nb, ids,
+                                    counts, parent);
+        free(tb); free(codes); free(tp); free(tbox);
+        if (troot == ORC_SENTINEL) { free(keys); free(ids); free(meta); free(parent); free(counts); free(nb); return -2; }
+    }
     scene_t sc;
     memset(&sc.bvh, 0, sizeof(sc.bvh));
     sc.bvh.vertices = vertices; sc.bvh.leaf_ids = ids; sc.bvh.node_meta = meta; sc.bvh.node_bounds = nb; sc.bvh.root = root;
@@ -400,8 +417,8 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
     void *wide = NULL;
     if (st->node_width == 8) {
         wide = malloc(n * 96);
-        uint64_t wc = orc_collapse_wide(meta, nb, counts, n, scene12, st->leaf_size ? (uint32_t)st->leaf_size : 4u,
-                                        wide, n);
+        uint64_t wc = orc_collapse_wide(meta, nb, counts, n, root, scene12,
+                                        st->leaf_size ? (uint32_t)st->leaf_size : 4u, wide, n);
         if (wc == 0) { free(wide); free(keys); free(ids); free(meta); free(parent); free(counts); free(nb); return -1; }
         sc.bvh.wide = wide;
         sc.bvh.wide_count = wc;

@@ -100,7 +100,7 @@ static void empty_slots(orc_wide_node *w, int from) {
 }
 
 uint64_t orc_collapse_wide(const uint32_t *node_meta, const float *node_bounds, const uint32_t *leaf_counts,
-                           uint64_t n, const float *scene_bounds, uint32_t leaf_size, void *out_nodes,
+                           uint64_t n, uint32_t root, const float *scene_bounds, uint32_t leaf_size, void *out_nodes,
                            uint64_t capacity) {
     orc_wide_node *out = (orc_wide_node *)out_nodes;
     if (n == 0 || capacity == 0 || leaf_size < 1 || leaf_size > 8) return 0;
@@ -120,7 +120,7 @@ uint64_t orc_collapse_wide(const uint32_t *node_meta, const float *node_bounds, 
         return 1;
     }
     uint32_t *src = (uint32_t *)malloc(capacity * sizeof(uint32_t));
-    src[0] = 0; /* binary root */
+    src[0] = root; /* binary root: an inner node when n > 1 */
     uint64_t count = 1;
     for (uint64_t w = 0; w < count; ++w) {
         wchild ch[8];

@@ -99,6 +99,8 @@ class RenderStats(ctypes.Structure):
         ("shadow_ms", c_float),
         ("extend_launches", c_uint32),
         ("shadow_launches", c_uint32),
+        ("extend_bounce_ms", c_float * 16),
+        ("shadow_bounce_ms", c_float * 16),
     ]
 
 

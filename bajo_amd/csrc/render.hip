@@ -87,14 +87,16 @@ BJ_D float power_heuristic(float a, float b) {
 }
 
 // ---- primary (rt/gpu/common_kernels.mojo:22-94, bvh/camera.mojo:156-208) ----
-__global__ void __launch_bounds__(kRtBlock) primary_kernel(Cam cam, uint32_t* __restrict__ ids,
-                                                           float* __restrict__ fields, float* __restrict__ sample,
-                                                           uint64_t cap, uint32_t active, uint32_t sample_base,
-                                                           uint32_t width, uint32_t height, uint32_t spp,
-                                                           uint64_t seed) {
-  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= active) return;
-  const uint32_t path_id = sample_base + (uint32_t)idx;
+struct PrimaryArgs {
+  Cam cam;
+  uint32_t width, height, spp;
+};
+
+// The camera path of `path_id`: origin, t_min, direction, t_max, throughput
+// 1 and pdf 0 in the path-plane order of the queues.
+BJ_D void primary_path(const PrimaryArgs& pa, uint32_t path_id, uint64_t seed, float (&f)[kPathPlanes]) {
+  const Cam& cam = pa.cam;
+  const uint32_t width = pa.width, height = pa.height, spp = pa.spp;
   const uint32_t pixel = path_id / spp;
   const uint32_t px = pixel % width, py = pixel / width;
   StageRng rng(seed, path_id, 0u);
@@ -116,12 +118,9 @@ __global__ void __launch_bounds__(kRtBlock) primary_kernel(Cam cam, uint32_t* __
   const v3 focal = add(origin, scale(plane, focus));
   const v3 ro = add(add(origin, scale(mk(k[14], k[15], k[16]), lens_u)), scale(mk(k[17], k[18], k[19]), lens_v));
   const v3 d = normalize(sub(focal, ro));
-  ids[idx] = path_id | kPathDelta;
-  const float f[kPathPlanes] = {ro.x, ro.y, ro.z, kT_MinSecondary, d.x, d.y, d.z, kFltMax, 1.0f, 1.0f, 1.0f, 0.0f};
+  const float v[kPathPlanes] = {ro.x, ro.y, ro.z, kT_MinSecondary, d.x, d.y, d.z, kFltMax, 1.0f, 1.0f, 1.0f, 0.0f};
 #pragma unroll
-  for (int p = 0; p < kPathPlanes; ++p) fields[p * cap + idx] = f[p];
-#pragma unroll
-  for (int p = 0; p < kSamplePlanes; ++p) sample[p * cap + idx] = 0.0f;
+  for (int p = 0; p < kPathPlanes; ++p) f[p] = v[p];
 }
 
 // Resets the queue counters for a chunk; the status words (kStatus and the
@@ -278,9 +277,12 @@ BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uin
   return true;
 }
 
-template <int kAlg, bool kWide>
+// kFirst: bounce 0 of a chunk, whose paths are the camera paths of the
+// chunk's path ids (generated here instead of read from a queue); it also
+// clears the chunk's sample accumulators.
+template <int kAlg, bool kWide, bool kFirst>
 __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
-                                                          uint32_t* __restrict__ spill,
+                                                          PrimaryArgs pa, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work) {
   __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
   const uint32_t lane = lane_id();
@@ -305,12 +307,22 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
     NextPath np;
     ShadowWork sw;
     if (idx < active) {
-      const uint32_t idw = q.src_ids[idx];
-      const uint32_t path_id = idw & kPathIdMask;
-      const bool delta = (idw & kPathDelta) != 0;
+      uint32_t path_id;
+      bool delta;
       float f[kPathPlanes];
+      if constexpr (kFirst) {
+        path_id = q.sample_base + idx;
+        delta = true;
+        primary_path(pa, path_id, seed, f);
 #pragma unroll
-      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[p * cap + idx];
+        for (int p = 0; p < kSamplePlanes; ++p) q.sample[p * cap + idx] = 0.0f;
+      } else {
+        const uint32_t idw = q.src_ids[idx];
+        path_id = idw & kPathIdMask;
+        delta = (idw & kPathDelta) != 0;
+#pragma unroll
+        for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[p * cap + idx];
+      }
       const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
       const v3 T = mk(f[8], f[9], f[10]);
       const float path_pdf = f[11];
@@ -635,7 +647,7 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   cap = (cap / st->samples_per_pixel) * st->samples_per_pixel;
   if (cap == 0) cap = st->samples_per_pixel;
   unsigned g_ext = 0, g_sh = 0;
-  int32_t rc = grid_of(extend_kernel<kAlg, kWide>, &g_ext);
+  int32_t rc = grid_of(extend_kernel<kAlg, kWide, false>, &g_ext);
   if (rc) return rc;
   rc = grid_of(shadow_kernel<kAlg, kWide>, &g_sh);
   if (rc) return rc;
@@ -648,8 +660,11 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   rc = ensure_arena(s, cap, spill_threads);
   if (rc) return rc;
   hipStream_t stream = ctx->stream;
-  Cam cam;
-  memcpy(cam.v, camera, sizeof(cam.v));
+  PrimaryArgs pa;
+  memcpy(pa.cam.v, camera, sizeof(pa.cam.v));
+  pa.width = st->image_width;
+  pa.height = st->image_height;
+  pa.spp = st->samples_per_pixel;
   SceneView sv;
   sv.bvh = accel_view(s->bvh, s->counters + 8);
   sv.surfaces = s->surfaces;
@@ -663,15 +678,17 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   for (uint64_t begin = 0; begin < total; begin += cap) {
     const uint32_t chunk = (uint32_t)((total - begin) < cap ? (total - begin) : cap);
     init_counters_kernel<<<1, 64, 0, stream>>>(s->counters, chunk, s->stats, begin == 0);
-    primary_kernel<<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(
-        cam, s->ids[0], s->fields[0], s->sample, cap, chunk, (uint32_t)begin, st->image_width, st->image_height,
-        st->samples_per_pixel, st->rng_seed);
     BJ_HIP(hipGetLastError());
     for (uint32_t b = 0; b < depth; ++b) {
       Queues q{s->ids[b & 1], s->fields[b & 1], s->ids[(b + 1) & 1], s->fields[(b + 1) & 1], s->shadow_ids,
                s->shadow_fields, s->sample, s->counters, cap, (uint32_t)begin};
       if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
-      extend_kernel<kAlg, kWide><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, s->spill, s->work_ext);
+      if (b == 0)
+        extend_kernel<kAlg, kWide, true><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, s->spill,
+                                                                         s->work_ext);
+      else
+        extend_kernel<kAlg, kWide, false><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, s->spill,
+                                                                          s->work_ext);
       BJ_HIP(hipGetLastError());
       if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
@@ -858,10 +875,12 @@ int32_t bajo_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st
     for (size_t i = 0; i + 1 < ev->ext.size(); i += 2) {
       BJ_HIP(hipEventElapsedTime(&ms, ev->ext[i], ev->ext[i + 1]));
       stats->extend_ms += ms;
+      if (i / 2 < 16) stats->extend_bounce_ms[i / 2] = ms;
     }
     for (size_t i = 0; i + 1 < ev->sh.size(); i += 2) {
       BJ_HIP(hipEventElapsedTime(&ms, ev->sh[i], ev->sh[i + 1]));
       stats->shadow_ms += ms;
+      if (i / 2 < 16) stats->shadow_bounce_ms[i / 2] = ms;
     }
     stats->extend_launches = (uint32_t)(ev->ext.size() / 2);
     stats->shadow_launches = (uint32_t)(ev->sh.size() / 2);

@@ -143,6 +143,9 @@ __global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const Bj
   for (uint32_t base = begin + (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); base < end; base += stride) {
     const uint32_t w = base + lane;
     const bool active = w < end;
+    // The children live in registers: every index into ch[] below is a
+    // compile-time constant after unrolling (a data-dependent index would
+    // place the array in scratch memory).
     Child ch[8];
     int n = 0;
     if (active) {
@@ -150,32 +153,40 @@ __global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const Bj
       n = 2;
       // open largest-area children: first those too big for a leaf, then any inner child
       for (int phase = 0; phase < 2; ++phase) {
-        while (n < 8) {
+        for (int step = 0; step < 6 && n < 8; ++step) {
           int pick = -1;
           float best = -1.0f;
+          uint32_t pick_ref = 0;
 #pragma unroll
           for (int i = 0; i < 8; ++i) {
-            if (i >= n) break;
-            const bool inner = !(ch[i].ref & kLeafFlag);
-            const bool eligible = inner && (phase == 0 ? ch[i].count > max_leaf : ch[i].count >= 2);
-            const float a = half_area(ch[i].b);
-            if (eligible && a > best) {
-              best = a;
-              pick = i;
+            if (i < n) {
+              const bool inner = !(ch[i].ref & kLeafFlag);
+              const bool eligible = inner && (phase == 0 ? ch[i].count > max_leaf : ch[i].count >= 2);
+              const float a = half_area(ch[i].b);
+              if (eligible && a > best) {
+                best = a;
+                pick = i;
+                pick_ref = ch[i].ref;
+              }
             }
           }
           if (pick < 0) break;
           Child l, r;
-          load_children(bnodes, leaf_counts, ch[pick].ref, l, r);
-          ch[pick] = l;
-          ch[n++] = r;
+          load_children(bnodes, leaf_counts, pick_ref, l, r);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            if (i == pick) ch[i] = l;
+            if (i == n) ch[i] = r;
+          }
+          ++n;
         }
       }
     }
     // reserve wide nodes for inner children that stay inner (wave-aggregated)
     uint32_t k = 0;
-    for (int i = 0; i < n; ++i)
-      if (!(ch[i].ref & kLeafFlag) && ch[i].count > max_leaf) ++k;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (i < n && !(ch[i].ref & kLeafFlag) && ch[i].count > max_leaf) ++k;
     uint32_t incl = k;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -193,45 +204,57 @@ __global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const Bj
     }
     // node box and per-axis grids
     float box[6] = {kFltMax, kFltMax, kFltMax, -kFltMax, -kFltMax, -kFltMax};
-    for (int i = 0; i < n; ++i)
-      for (int a = 0; a < 3; ++a) {
-        box[a] = fmin_sel(box[a], ch[i].b[a]);
-        box[a + 3] = fmax_sel(box[a + 3], ch[i].b[a + 3]);
-      }
-    BjWide out;
-    out.ox = box[0]; out.oy = box[1]; out.oz = box[2];
-    const uint32_t ex = grid_exponent(box[0], box[3]), ey = grid_exponent(box[1], box[4]),
-                   ez = grid_exponent(box[2], box[5]);
-    out.ex = (uint8_t)ex; out.ey = (uint8_t)ey; out.ez = (uint8_t)ez;
-    out.n = (uint8_t)n;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (i < n)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          box[a] = fmin_sel(box[a], ch[i].b[a]);
+          box[a + 3] = fmax_sel(box[a + 3], ch[i].b[a + 3]);
+        }
+    const uint32_t e[3] = {grid_exponent(box[0], box[3]), grid_exponent(box[1], box[4]),
+                           grid_exponent(box[2], box[5])};
+    // packed byte planes: lo_x, lo_y, lo_z, hi_x, hi_y, hi_z, two words each
+    uint32_t q[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) q[j] = j < 6 ? 0xFFFFFFFFu : 0u;
+    uint32_t child[8];
     uint32_t next = first;
+#pragma unroll
     for (int i = 0; i < 8; ++i) {
+      child[i] = 0;
       if (i < n) {
-        quantize(box[0], ex, ch[i].b[0], ch[i].b[3], out.lo_x[i], out.hi_x[i]);
-        quantize(box[1], ey, ch[i].b[1], ch[i].b[4], out.lo_y[i], out.hi_y[i]);
-        quantize(box[2], ez, ch[i].b[2], ch[i].b[5], out.lo_z[i], out.hi_z[i]);
+        const int word = i >> 2, shift = 8 * (i & 3);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          uint8_t ql, qh;
+          quantize(box[a], e[a], ch[i].b[a], ch[i].b[a + 3], ql, qh);
+          q[2 * a + word] = (q[2 * a + word] & ~(0xFFu << shift)) | ((uint32_t)ql << shift);
+          q[6 + 2 * a + word] |= (uint32_t)qh << shift;
+        }
         const uint32_t ref = ch[i].ref;
         if (ref & kLeafFlag) {
-          out.child[i] = kLeafFlag | (ref & kIndexMask);  // one triangle
+          child[i] = kLeafFlag | (ref & kIndexMask);  // one triangle
         } else if (ch[i].count <= max_leaf) {
           // inner binary node over sorted range [fence - count + 1, fence]
           const uint32_t fence = bnodes[ref].fence;
-          out.child[i] = kLeafFlag | ((ch[i].count - 1u) << 28) | (fence + 1u - ch[i].count);
+          child[i] = kLeafFlag | ((ch[i].count - 1u) << 28) | (fence + 1u - ch[i].count);
         } else {
-          out.child[i] = next;
+          child[i] = next;
           wide_src[next] = ref;
           ++next;
         }
-      } else {
-        out.lo_x[i] = out.lo_y[i] = out.lo_z[i] = 255;
-        out.hi_x[i] = out.hi_y[i] = out.hi_z[i] = 0;
-        out.child[i] = 0;
       }
     }
-    const uint4* src = reinterpret_cast<const uint4*>(&out);
+    const uint32_t meta = e[0] | (e[1] << 8) | (e[2] << 16) | ((uint32_t)n << 24);
     uint4* dst = reinterpret_cast<uint4*>(wide + w);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) dst[q] = src[q];
+    dst[0] = make_uint4(__builtin_bit_cast(uint32_t, box[0]), __builtin_bit_cast(uint32_t, box[1]),
+                        __builtin_bit_cast(uint32_t, box[2]), meta);
+    dst[1] = make_uint4(q[0], q[1], q[2], q[3]);
+    dst[2] = make_uint4(q[4], q[5], q[6], q[7]);
+    dst[3] = make_uint4(q[8], q[9], q[10], q[11]);
+    dst[4] = make_uint4(child[0], child[1], child[2], child[3]);
+    dst[5] = make_uint4(child[4], child[5], child[6], child[7]);
   }
 }
 

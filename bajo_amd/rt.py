@@ -180,6 +180,8 @@ def enqueue_render_gpu(
     if stats is not None:
         out = {name: getattr(stats, name) for name, _ in _lib.RenderStats._fields_}
         out["rays_per_bounce"] = list(stats.rays_per_bounce)
+        out["extend_bounce_ms"] = list(stats.extend_bounce_ms)
+        out["shadow_bounce_ms"] = list(stats.shadow_bounce_ms)
     return RenderResult(pixels, out)
 
 

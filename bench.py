@@ -190,6 +190,11 @@ def main() -> int:
         "nee": {k: nee[k] for k in keep},
         "path": {k: path[k] for k in keep},
         "nee_rays_per_bounce": nee["rays_per_bounce"][:DEPTH],
+        "nee_extend_bounce_ms": [round(x, 4) for x in nee["extend_bounce_ms"][:DEPTH]],
+        "nee_shadow_bounce_ms": [round(x, 4) for x in nee["shadow_bounce_ms"][:DEPTH]],
+        "path_extend_bounce_ms": [round(x, 4) for x in path["extend_bounce_ms"][:DEPTH]],
+        "nee_shadow_visits_per_ray": nee["shadow_node_visits"] / max(nee["shadow_rays"], 1),
+        "nee_shadow_tests_per_ray": nee["shadow_leaf_tests"] / max(nee["shadow_rays"], 1),
         "nee_image_mean_rgb": img,
         "build_ms": build_ms,
         "build_mtris_s": gscene.triangle_count / (build_ms / 1e3) / 1e6,

@@ -228,6 +228,8 @@ typedef struct {
   float extend_ms;              /* sum over extend launches              */
   float shadow_ms;              /* sum over shadow launches              */
   uint32_t extend_launches, shadow_launches;
+  float extend_bounce_ms[16];   /* extend launch time per bounce (first chunk) */
+  float shadow_bounce_ms[16];   /* shadow launch time per bounce (first chunk) */
 } bajo_render_stats;
 
 /* SceneData for triangle scenes (rt/types.mojo:382-526): host arrays of

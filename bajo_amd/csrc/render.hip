@@ -36,7 +36,8 @@ constexpr uint32_t kPathDelta = 0x80000000u, kPathIdMask = 0x7FFFFFFFu;
 constexpr int kLightStride = 14;
 constexpr int kPathPlanes = 12, kShadowPlanes = 11, kSamplePlanes = 3;
 // counter words (bajo's WAVE_COUNTER plus two fetch cursors)
-constexpr int kActive = 0, kNext = 1, kShade = 2, kShadow = 3, kStatus = 4, kFetchExt = 5, kFetchShadow = 6;
+constexpr int kActive = 0, kNext = 1, kShade = 2, kShadow = 3, kStatus = 4, kFetchExt = 5, kFetchShadow = 6,
+              kTailDone = 7;
 constexpr int kCounterWords = 16;
 constexpr uint32_t kStatusPathOverflow = 1, kStatusShadowOverflow = 3;
 constexpr int kMaxBounces = 64;
@@ -142,6 +143,7 @@ __global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsign
                                                       uint32_t sh_blocks) {
   __shared__ unsigned long long s_sum[4][256];
   const uint32_t t = threadIdx.x;
+  if (counters[kTailDone]) return;  // the tail kernel traced and counted the rest of the chunk
   unsigned long long a = 0, b = 0, c = 0, d = 0;
   for (uint32_t i = t; i < ext_blocks; i += blockDim.x) { a += ext_work[2 * i]; b += ext_work[2 * i + 1]; }
   for (uint32_t i = t; i < sh_blocks; i += blockDim.x) { c += sh_work[2 * i]; d += sh_work[2 * i + 1]; }
@@ -280,6 +282,159 @@ BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uin
 // kFirst: bounce 0 of a chunk, whose paths are the camera paths of the
 // chunk's path ids (generated here instead of read from a queue); it also
 // clears the chunk's sample accumulators.
+// Surface facts of an extension hit (rt/geometry.mojo:20-30: the outward
+// normal is oriented against the ray).
+struct Surface {
+  bool found, front;
+  v3 n, point;
+  uint32_t kind, mi;
+};
+
+BJ_D Surface surface_of(const SceneView& sc, const Ray& ray, const Hit& h) {
+  Surface su;
+  su.found = h.prim != kEmpty && h.t < ray.tmax;
+  su.front = false;
+  su.n = mk(0, 0, 0);
+  su.point = mk(0, 0, 0);
+  su.kind = su.mi = 0;
+  if (su.found) {
+    su.front = dot(ray.d, h.n) < 0.0f;
+    su.n = su.front ? h.n : mk(-h.n.x, -h.n.y, -h.n.z);
+    su.point = add(ray.o, scale(ray.d, h.t));
+    const uint32_t surf = sc.surfaces[h.prim];
+    su.kind = surf >> 28;
+    su.mi = surf & kSurfaceIndexMask;
+  }
+  return su;
+}
+
+BJ_D v3 albedo_of(const SceneView& sc, const Surface& su) {
+  return su.kind == kMatLambertian
+             ? mk(sc.lambertian[3 * su.mi + 0], sc.lambertian[3 * su.mi + 1], sc.lambertian[3 * su.mi + 2])
+             : mk(0, 0, 0);
+}
+
+// The shadow ray an extension hit spawns: the NEE / MIS light sample
+// (rt/gpu/path_shading.mojo:261-460) or the AO hemisphere ray (:651-667).
+template <int kAlg>
+BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint32_t path_id, v3 T,
+                       const Surface& su, ShadowWork& sw) {
+  if (!su.found) return false;
+  if constexpr (kAlg == BAJO_RENDER_AO) {
+    StageRng rng(seed, path_id, 1u);
+    float s, c;
+    sincos_turns(rng.next(), s, c);
+    const float z = 1.0f - 2.0f * rng.next();
+    const float r = sqrtf(1.0f - z * z);
+    v3 dir = mk(r * c, r * s, z);
+    if (dot(dir, su.n) < 0.0f) dir = mk(-dir.x, -dir.y, -dir.z);
+    const float fs[kShadowPlanes] = {su.point.x, su.point.y, su.point.z, 0.001f, dir.x, dir.y, dir.z, 4.0f, 0, 0, 0};
+    for (int p = 0; p < kShadowPlanes; ++p) sw.f[p] = fs[p];
+    sw.id = path_id;
+    return true;
+  } else if constexpr (kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
+    if (su.kind != kMatLambertian) return false;
+    return direct_light<kAlg>(sc, seed, path_id, bounce, su.point, su.n, albedo_of(sc, su), T, sw);
+  } else {
+    return false;
+  }
+}
+
+// Radiance an extension hit adds (sky on a miss, emission with the NEE/MIS
+// weights, normals) and the next segment (Lambertian bounce with Russian
+// roulette), rt/gpu/path_shading.mojo:463-648.  Shared by the wavefront
+// extend kernel and the tail kernel so both follow one definition.
+template <int kAlg>
+BJ_D void shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, uint32_t path_id, bool delta,
+                       const Ray& ray, v3 T, float path_pdf, float t_hit, const Surface& su, bool& has_add, v3& add_c,
+                       bool& want_next, NextPath& np) {
+  has_add = false;
+  want_next = false;
+  if (!su.found) {
+    if constexpr (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
+      add_c = mul(T, sky_color(ray.d));
+      has_add = true;
+    }
+    return;
+  }
+  if constexpr (kAlg == BAJO_RENDER_NORMALS) {
+    add_c = scale(add(su.n, mk(1.0f, 1.0f, 1.0f)), 0.5f);
+    has_add = true;
+  } else if constexpr (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
+    if (su.kind == kMatEmissive) {
+      if (su.front) {
+        const v3 E = mk(sc.emissive[3 * su.mi + 0], sc.emissive[3 * su.mi + 1], sc.emissive[3 * su.mi + 2]);
+        float w = 1.0f;
+        if constexpr (kAlg == BAJO_RENDER_NEE) {
+          if (bounce > 0 && !delta) w = 0.0f;
+        } else if constexpr (kAlg == BAJO_RENDER_MIS) {
+          if (bounce > 0 && !delta) {
+            const float lc = fmax_sel(dot(su.n, normalize(mk(-ray.d.x, -ray.d.y, -ray.d.z))), 0.0f);
+            float lpdf = 0.0f;
+            if (lc > 0.0f && sc.total_light_weight > 0.0f) {
+              const float d2 = t_hit * t_hit * length2(ray.d);
+              lpdf = d2 * (E.x + E.y + E.z) / (3.0f * lc * sc.total_light_weight);
+            }
+            w = power_heuristic(path_pdf, lpdf);
+          }
+        }
+        add_c = scale(mul(T, E), w);
+        has_add = true;
+      }
+    } else if (su.kind == kMatLambertian) {
+      // _sample_lambertian (rt/shading.mojo:92-118)
+      const v3 n = su.n;
+      StageRng rng(seed, path_id, rng_bsdf_stage(bounce));
+      const float ru = rng.next(), rv = rng.next();
+      float s, c;
+      sincos_turns(ru, s, c);
+      const float z = 1.0f - 2.0f * rv;
+      const float radius = sqrtf(fmax_sel(1.0f - z * z, 0.0f));
+      v3 sd = add(n, mk(radius * c, radius * s, z));
+      if (fabsf(sd.x) < 1e-8f && fabsf(sd.y) < 1e-8f && fabsf(sd.z) < 1e-8f) sd = n;
+      const v3 dir = normalize(sd);
+      const float pdf = fmax_sel(dot(n, normalize(dir)), 0.0f) * kInvPi;
+      v3 Tn = mul(T, albedo_of(sc, su));
+      // russian_roulette (rt/common.mojo:43-67), depth = bounce + 1
+      bool alive = true;
+      if (bounce + 1u >= 5u) {
+        const float mx = fmax_sel(Tn.x, fmax_sel(Tn.y, Tn.z));
+        if (mx <= 0.0f) {
+          alive = false;
+        } else {
+          const float survival = fmin_sel(fmax_sel(mx, 0.05f), 0.95f);
+          StageRng rr(seed, path_id, rng_roulette_stage(bounce));
+          if (rr.next() >= survival) alive = false;
+          else Tn = mk(Tn.x / survival, Tn.y / survival, Tn.z / survival);
+        }
+      }
+      if (alive) {
+        np.id = path_id;
+        const float nf[kPathPlanes] = {ray.o.x + t_hit * ray.d.x, ray.o.y + t_hit * ray.d.y,
+                                       ray.o.z + t_hit * ray.d.z, kT_MinSecondary, dir.x, dir.y, dir.z, kFltMax,
+                                       Tn.x, Tn.y, Tn.z, pdf};
+        for (int p = 0; p < kPathPlanes; ++p) np.f[p] = nf[p];
+        want_next = true;
+      }
+    }
+    // metal and dielectric surfaces are not part of this renderer's scenes: absorbed
+  }
+}
+
+// Radiance a traced shadow ray adds (rt/gpu/scene_trace.mojo:517-624).
+template <int kAlg>
+BJ_D v3 shadow_radiance(bool occluded, const float* f) {
+  if constexpr (kAlg == BAJO_RENDER_AO) {
+    const float v = occluded ? 0.08f : 1.0f;
+    return mk(v, v, v);
+  } else {
+    return occluded ? mk(0.0f, 0.0f, 0.0f) : mk(f[8], f[9], f[10]);
+  }
+}
+
+// kFirst: bounce 0 of a chunk, whose paths are the camera paths of the
+// chunk's path ids (generated here instead of read from a queue); it also
+// clears the chunk's sample accumulators.
 template <int kAlg, bool kWide, bool kFirst>
 __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                           PrimaryArgs pa, uint32_t* __restrict__ spill,
@@ -289,7 +444,8 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
   const uint64_t cap = q.capacity;
-  const uint32_t active = q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap;
+  const bool tail_done = q.counters[kTailDone] != 0;
+  const uint32_t active = tail_done ? 0u : (q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap);
   uint64_t work_visits = 0, work_leaves = 0;
   // blocks past the live work exit without touching the shared counters
   const bool has_work = (uint64_t)blockIdx.x * kRtBlock < active;
@@ -303,133 +459,50 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
     if (base + r >= active) break;
     const uint32_t idx = base + r + lane;
     r += 64;
-    bool want_next = false, want_shadow = false;
-    NextPath np;
+    Surface su;
+    su.found = false;
+    Ray ray;
+    float t_hit = 0.0f;
+    uint32_t path_id = 0;
+    bool delta = false;
+    v3 T = mk(1.0f, 1.0f, 1.0f);
+    float path_pdf = 0.0f;
     ShadowWork sw;
-    if (idx < active) {
-      uint32_t path_id;
-      bool delta;
-      float f[kPathPlanes];
+    bool want_shadow = false;
+    const bool live = idx < active;
+    if (live) {
+      float f[8];
       if constexpr (kFirst) {
-        path_id = q.sample_base + idx;
-        delta = true;
-        primary_path(pa, path_id, seed, f);
+        float pf[kPathPlanes];
+        primary_path(pa, q.sample_base + idx, seed, pf);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) f[p] = pf[p];
 #pragma unroll
         for (int p = 0; p < kSamplePlanes; ++p) q.sample[p * cap + idx] = 0.0f;
       } else {
-        const uint32_t idw = q.src_ids[idx];
-        path_id = idw & kPathIdMask;
-        delta = (idw & kPathDelta) != 0;
 #pragma unroll
-        for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[p * cap + idx];
+        for (int p = 0; p < 8; ++p) f[p] = q.src_fields[p * cap + idx];
       }
-      const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
-      const v3 T = mk(f[8], f[9], f[10]);
-      const float path_pdf = f[11];
-      const uint32_t slot = path_id - q.sample_base;
+      ray = Ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
       TraceCounters tc;
       const Hit h = trace_accel<BAJO_TRACE_CLOSEST_HIT, true, kWide>(sc.bvh, ray, st, &tc);
       work_visits += tc.visits;
       work_leaves += tc.leaves;
-      const bool found = h.prim != kEmpty && h.t < ray.tmax;
-      if (!found) {
-        if constexpr (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS)
-          accumulate(q.sample, cap, slot, mul(T, sky_color(ray.d)));
+      // throughput, pdf and id are read after the traversal rather than
+      // held in registers through it
+      if constexpr (!kFirst) {
+        const uint32_t idw = q.src_ids[idx];
+        path_id = idw & kPathIdMask;
+        delta = (idw & kPathDelta) != 0;
+        T = mk(q.src_fields[8 * cap + idx], q.src_fields[9 * cap + idx], q.src_fields[10 * cap + idx]);
+        path_pdf = q.src_fields[11 * cap + idx];
       } else {
-        // rt/geometry.mojo:20-30: orient the outward normal against the ray
-        const bool front = dot(ray.d, h.n) < 0.0f;
-        const v3 n = front ? h.n : mk(-h.n.x, -h.n.y, -h.n.z);
-        const v3 point = add(ray.o, scale(ray.d, h.t));
-        const uint32_t surf = sc.surfaces[h.prim];
-        const uint32_t kind = surf >> 28, mi = surf & kSurfaceIndexMask;
-        if constexpr (kAlg == BAJO_RENDER_NORMALS) {
-          accumulate(q.sample, cap, slot, scale(add(n, mk(1.0f, 1.0f, 1.0f)), 0.5f));
-        } else if constexpr (kAlg == BAJO_RENDER_AO) {
-          // rt/gpu/path_shading.mojo:651-667: hemisphere ray, t_max 4
-          StageRng rng(seed, path_id, 1u);
-          float s, c;
-          sincos_turns(rng.next(), s, c);
-          const float z = 1.0f - 2.0f * rng.next();
-          const float r = sqrtf(1.0f - z * z);
-          v3 dir = mk(r * c, r * s, z);
-          if (dot(dir, n) < 0.0f) dir = mk(-dir.x, -dir.y, -dir.z);
-          const float fs[kShadowPlanes] = {point.x, point.y, point.z, 0.001f, dir.x, dir.y, dir.z, 4.0f, 0, 0, 0};
-          for (int p = 0; p < kShadowPlanes; ++p) sw.f[p] = fs[p];
-          sw.id = path_id;
-          want_shadow = true;
-        } else {
-          v3 albedo = mk(0, 0, 0);
-          if (kind == kMatLambertian)
-            albedo = mk(sc.lambertian[3 * mi + 0], sc.lambertian[3 * mi + 1], sc.lambertian[3 * mi + 2]);
-          if constexpr (kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
-            if (kind == kMatLambertian) want_shadow = direct_light<kAlg>(sc, seed, path_id, bounce, point, n, albedo, T, sw);
-          }
-          if (kind == kMatEmissive) {
-            if (front) {
-              const v3 E = mk(sc.emissive[3 * mi + 0], sc.emissive[3 * mi + 1], sc.emissive[3 * mi + 2]);
-              float w = 1.0f;
-              if constexpr (kAlg == BAJO_RENDER_NEE) {
-                if (bounce > 0 && !delta) w = 0.0f;
-              } else if constexpr (kAlg == BAJO_RENDER_MIS) {
-                if (bounce > 0 && !delta) {
-                  const float lc = fmax_sel(dot(n, normalize(mk(-ray.d.x, -ray.d.y, -ray.d.z))), 0.0f);
-                  float lpdf = 0.0f;
-                  if (lc > 0.0f && sc.total_light_weight > 0.0f) {
-                    const float d2 = h.t * h.t * length2(ray.d);
-                    lpdf = d2 * (E.x + E.y + E.z) / (3.0f * lc * sc.total_light_weight);
-                  }
-                  w = power_heuristic(path_pdf, lpdf);
-                }
-              }
-              accumulate(q.sample, cap, slot, scale(mul(T, E), w));
-            }
-          } else if (kind == kMatLambertian) {
-            // _sample_lambertian (rt/shading.mojo:92-118)
-            StageRng rng(seed, path_id, rng_bsdf_stage(bounce));
-            const float ru = rng.next(), rv = rng.next();
-            float s, c;
-            sincos_turns(ru, s, c);
-            const float z = 1.0f - 2.0f * rv;
-            const float radius = sqrtf(fmax_sel(1.0f - z * z, 0.0f));
-            v3 sd = add(n, mk(radius * c, radius * s, z));
-            if (fabsf(sd.x) < 1e-8f && fabsf(sd.y) < 1e-8f && fabsf(sd.z) < 1e-8f) sd = n;
-            const v3 dir = normalize(sd);
-            const float pdf = fmax_sel(dot(n, normalize(dir)), 0.0f) * kInvPi;
-            v3 Tn = mul(T, albedo);
-            // russian_roulette (rt/common.mojo:43-67), depth = bounce + 1
-            bool alive = true;
-            if (bounce + 1u >= 5u) {
-              const float mx = fmax_sel(Tn.x, fmax_sel(Tn.y, Tn.z));
-              if (mx <= 0.0f) {
-                alive = false;
-              } else {
-                const float survival = fmin_sel(fmax_sel(mx, 0.05f), 0.95f);
-                StageRng rr(seed, path_id, rng_roulette_stage(bounce));
-                if (rr.next() >= survival) alive = false;
-                else Tn = mk(Tn.x / survival, Tn.y / survival, Tn.z / survival);
-              }
-            }
-            if (alive) {
-              np.id = path_id;
-              const float nf[kPathPlanes] = {ray.o.x + h.t * ray.d.x, ray.o.y + h.t * ray.d.y, ray.o.z + h.t * ray.d.z,
-                                             kT_MinSecondary, dir.x, dir.y, dir.z, kFltMax, Tn.x, Tn.y, Tn.z, pdf};
-              for (int p = 0; p < kPathPlanes; ++p) np.f[p] = nf[p];
-              want_next = true;
-            }
-          }
-          // metal and dielectric surfaces are not part of this renderer's scenes: absorbed
-        }
+        path_id = q.sample_base + idx;
+        delta = true;
       }
-    }
-    const uint32_t ns = wave_reserve(q.counters + kNext, want_next);
-    if (want_next) {
-      if (ns < cap) {
-        q.dst_ids[ns] = np.id;
-#pragma unroll
-        for (int p = 0; p < kPathPlanes; ++p) q.dst_fields[p * cap + ns] = np.f[p];
-      } else {
-        atomicMax(q.counters + kStatus, kStatusPathOverflow);
-      }
+      su = surface_of(sc, ray, h);
+      t_hit = h.t;
+      want_shadow = shade_shadow<kAlg>(sc, seed, bounce, path_id, T, su, sw);
     }
     const uint32_t ss = wave_reserve(q.counters + kShadow, want_shadow);
     if (want_shadow) {
@@ -441,8 +514,141 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
         atomicMax(q.counters + kStatus, kStatusShadowOverflow);
       }
     }
+    bool want_next = false;
+    NextPath np;
+    if (live) {
+      bool has_add;
+      v3 add_c;
+      shade_bounce<kAlg>(sc, seed, bounce, path_id, delta, ray, T, path_pdf, t_hit, su, has_add, add_c, want_next, np);
+      if (has_add) accumulate(q.sample, cap, path_id - q.sample_base, add_c);
+    }
+    const uint32_t ns = wave_reserve(q.counters + kNext, want_next);
+    if (want_next) {
+      if (ns < cap) {
+        q.dst_ids[ns] = np.id;
+#pragma unroll
+        for (int p = 0; p < kPathPlanes; ++p) q.dst_fields[p * cap + ns] = np.f[p];
+      } else {
+        atomicMax(q.counters + kStatus, kStatusPathOverflow);
+      }
+    }
   }
   store_block_work(work, work_visits, work_leaves);
+}
+
+// Wave-aggregated 64-bit add of a per-lane count.
+BJ_D void wave_add(unsigned long long* dst, uint32_t v) {
+  uint64_t x = v;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+  if (lane_id() == 0 && x) atomicAdd(dst, (unsigned long long)x);
+}
+
+// The remaining bounces of a chunk once few paths are left.  Each thread
+// carries one path from `bounce` to the end: extension ray, shading, the
+// shadow ray traced at once, the next segment in registers.  A path's
+// sample receives the same additions in the same order as in the wavefront
+// (extension then shadow, bounce by bounce), so images are identical.  All
+// blocks of a launch take the same decision from the queue length; a
+// launch that runs marks the chunk done (tag = bounce + 1) so later extend,
+// shadow, advance and tail launches of the chunk do nothing.
+template <int kAlg, bool kWide>
+__global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
+                                                        uint32_t depth, uint32_t threshold,
+                                                        unsigned long long* __restrict__ stats,
+                                                        uint32_t* __restrict__ spill) {
+  __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
+  const uint32_t tag = q.counters[kTailDone];
+  if (tag != 0 && tag != bounce + 1u) return;
+  const uint64_t cap = q.capacity;
+  const uint32_t active = q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap;
+  if (active > threshold) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) q.counters[kTailDone] = bounce + 1u;
+  const uint32_t lane = lane_id();
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
+  uint64_t ext_visits = 0, ext_leaves = 0, sh_visits = 0, sh_leaves = 0;
+  const uint32_t nb = depth - bounce;
+  for (uint32_t base = (uint32_t)(gtid & ~63ull); base < active; base += gridDim.x * blockDim.x) {
+    const uint32_t idx = base + lane;
+    bool alive = idx < active;
+    uint32_t path_id = 0;
+    bool delta = false;
+    float f[kPathPlanes];
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t slot = 0;
+    if (alive) {
+      const uint32_t idw = q.src_ids[idx];
+      path_id = idw & kPathIdMask;
+      delta = (idw & kPathDelta) != 0;
+#pragma unroll
+      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[p * cap + idx];
+      slot = path_id - q.sample_base;
+#pragma unroll
+      for (int p = 0; p < kSamplePlanes; ++p) acc[p] = q.sample[p * cap + slot];
+    }
+    for (uint32_t k = 0; k < nb; ++k) {
+      const uint32_t b = bounce + k;
+      if (b < kMaxBounces) wave_add(stats + 2 * b, alive ? 1u : 0u);
+      bool shadow = false;
+      if (alive) {
+        const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
+        TraceCounters tc;
+        const Hit h = trace_accel<BAJO_TRACE_CLOSEST_HIT, true, kWide>(sc.bvh, ray, st, &tc);
+        ext_visits += tc.visits;
+        ext_leaves += tc.leaves;
+        const v3 T = mk(f[8], f[9], f[10]);
+        const Surface su = surface_of(sc, ray, h);
+        ShadowWork sw;
+        bool add_s = false;
+        v3 cs = mk(0, 0, 0);
+        if (shade_shadow<kAlg>(sc, seed, b, path_id, T, su, sw)) {
+          shadow = true;
+          const Ray sray{mk(sw.f[0], sw.f[1], sw.f[2]), mk(sw.f[4], sw.f[5], sw.f[6]), sw.f[3], sw.f[7]};
+          TraceCounters tcs;
+          const Hit sh = trace_accel<BAJO_TRACE_ANY_HIT, true, kWide>(sc.bvh, sray, st, &tcs);
+          sh_visits += tcs.visits;
+          sh_leaves += tcs.leaves;
+          const bool occluded = sh.prim == kEmpty && sh.t == 0.0f;
+          if (kAlg == BAJO_RENDER_AO || !occluded) {
+            cs = shadow_radiance<kAlg>(occluded, sw.f);
+            add_s = true;
+          }
+        }
+        bool has_add, want_next;
+        v3 add_c;
+        NextPath np;
+        shade_bounce<kAlg>(sc, seed, b, path_id, delta, ray, T, f[11], h.t, su, has_add, add_c, want_next, np);
+        // the same order as the wavefront: the extension's radiance, then the shadow ray's
+        if (has_add) {
+          acc[0] += add_c.x;
+          acc[1] += add_c.y;
+          acc[2] += add_c.z;
+        }
+        if (add_s) {
+          acc[0] += cs.x;
+          acc[1] += cs.y;
+          acc[2] += cs.z;
+        }
+        alive = want_next;
+        if (alive) {
+#pragma unroll
+          for (int p = 0; p < kPathPlanes; ++p) f[p] = np.f[p];
+          delta = false;
+        }
+      }
+      if (b < kMaxBounces) wave_add(stats + 2 * b + 1, shadow ? 1u : 0u);
+      if (!ballot(alive)) break;
+    }
+    if (idx < active) {
+#pragma unroll
+      for (int p = 0; p < kSamplePlanes; ++p) q.sample[p * cap + slot] = acc[p];
+    }
+  }
+  wave_add(stats + kWorkBase + 0, (uint32_t)ext_visits);
+  wave_add(stats + kWorkBase + 1, (uint32_t)ext_leaves);
+  wave_add(stats + kWorkBase + 2, (uint32_t)sh_visits);
+  wave_add(stats + kWorkBase + 3, (uint32_t)sh_leaves);
 }
 
 template <int kAlg, bool kWide>
@@ -453,7 +659,8 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
   const uint64_t cap = q.capacity;
-  const uint32_t count = q.counters[kShadow] < cap ? q.counters[kShadow] : (uint32_t)cap;
+  const uint32_t count =
+      q.counters[kTailDone] ? 0u : (q.counters[kShadow] < cap ? q.counters[kShadow] : (uint32_t)cap);
   uint64_t work_visits = 0, work_leaves = 0;
   const bool has_work = (uint64_t)blockIdx.x * kRtBlock < count;
   const uint32_t batch = fetch_batch(count);
@@ -477,12 +684,7 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
     work_visits += tc.visits;
     work_leaves += tc.leaves;
     const bool occluded = h.prim == kEmpty && h.t == 0.0f;
-    if constexpr (kAlg == BAJO_RENDER_AO) {
-      const float v = occluded ? 0.08f : 1.0f;
-      accumulate(q.sample, cap, slot, mk(v, v, v));
-    } else {
-      if (!occluded) accumulate(q.sample, cap, slot, mk(f[8], f[9], f[10]));
-    }
+    if (kAlg == BAJO_RENDER_AO || !occluded) accumulate(q.sample, cap, slot, shadow_radiance<kAlg>(occluded, f));
   }
   store_block_work(work, work_visits, work_leaves);
 }
@@ -651,12 +853,22 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   if (rc) return rc;
   rc = grid_of(shadow_kernel<kAlg, kWide>, &g_sh);
   if (rc) return rc;
+  constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
+  unsigned g_tail = 0;
+  if constexpr (kTail) {
+    rc = grid_of(tail_kernel<kAlg, kWide>, &g_tail);
+    if (rc) return rc;
+  }
   const unsigned need = div_up(cap, kRtBlock);
   if (g_ext > need) g_ext = need;
   if (g_sh > need) g_sh = need;
   if (g_ext < 1) g_ext = 1;
   if (g_sh < 1) g_sh = 1;
-  const uint64_t spill_threads = (uint64_t)(g_ext > g_sh ? g_ext : g_sh) * kRtBlock;
+  if (g_tail < 1) g_tail = 1;
+  // the tail takes over when the remaining paths fit one pass of its resident threads
+  const uint32_t tail_threshold = g_tail * kRtBlock;
+  const unsigned g_max = g_ext > g_sh ? (g_ext > g_tail ? g_ext : g_tail) : (g_sh > g_tail ? g_sh : g_tail);
+  const uint64_t spill_threads = (uint64_t)g_max * kRtBlock;
   rc = ensure_arena(s, cap, spill_threads);
   if (rc) return rc;
   hipStream_t stream = ctx->stream;
@@ -683,6 +895,13 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
       Queues q{s->ids[b & 1], s->fields[b & 1], s->ids[(b + 1) & 1], s->fields[(b + 1) & 1], s->shadow_ids,
                s->shadow_fields, s->sample, s->counters, cap, (uint32_t)begin};
       if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
+      if constexpr (kTail) {
+        if (b > 0) {
+          tail_kernel<kAlg, kWide><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, tail_threshold,
+                                                                    s->stats, s->spill);
+          BJ_HIP(hipGetLastError());
+        }
+      }
       if (b == 0)
         extend_kernel<kAlg, kWide, true><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, s->spill,
                                                                          s->work_ext);

@@ -75,10 +75,19 @@ struct Queues {
   uint32_t sample_base;
 };
 
-BJ_D void accumulate(float* sample, uint64_t cap, uint32_t slot, v3 c) {
-  sample[0 * cap + slot] += c.x;
-  sample[1 * cap + slot] += c.y;
-  sample[2 * cap + slot] += c.z;
+// Queue planes live in tiles of 64 entries (one wave): plane p of entry i is
+// at ((i / 64) * planes + p) * 64 + i % 64.  A wave touches 256 contiguous
+// bytes per plane, and every plane's offset inside a tile is a constant, so
+// a kernel needs one base address per queue instead of one per plane.
+template <int kPlanes>
+BJ_D uint64_t plane_at(uint32_t i, int p) {
+  return ((uint64_t)(i >> 6) * kPlanes + (uint64_t)p) * 64u + (i & 63u);
+}
+
+BJ_D void accumulate(float* sample, uint32_t slot, v3 c) {
+  sample[plane_at<kSamplePlanes>(slot, 0)] += c.x;
+  sample[plane_at<kSamplePlanes>(slot, 1)] += c.y;
+  sample[plane_at<kSamplePlanes>(slot, 2)] += c.z;
 }
 
 // rt/lighting.mojo:7-20.
@@ -145,8 +154,13 @@ __global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsign
   const uint32_t t = threadIdx.x;
   if (counters[kTailDone]) return;  // the tail kernel traced and counted the rest of the chunk
   unsigned long long a = 0, b = 0, c = 0, d = 0;
-  for (uint32_t i = t; i < ext_blocks; i += blockDim.x) { a += ext_work[2 * i]; b += ext_work[2 * i + 1]; }
-  for (uint32_t i = t; i < sh_blocks; i += blockDim.x) { c += sh_work[2 * i]; d += sh_work[2 * i + 1]; }
+  // blocks past the live entries exit without writing their partials
+  const uint32_t active = counters[kActive] < cap ? counters[kActive] : cap;
+  const uint32_t shadows = counters[kShadow] < cap ? counters[kShadow] : cap;
+  const uint32_t ext_live = (active + 255u) / 256u < ext_blocks ? (active + 255u) / 256u : ext_blocks;
+  const uint32_t sh_live = (shadows + 255u) / 256u < sh_blocks ? (shadows + 255u) / 256u : sh_blocks;
+  for (uint32_t i = t; i < ext_live; i += blockDim.x) { a += ext_work[2 * i]; b += ext_work[2 * i + 1]; }
+  for (uint32_t i = t; i < sh_live; i += blockDim.x) { c += sh_work[2 * i]; d += sh_work[2 * i + 1]; }
   s_sum[0][t] = a; s_sum[1][t] = b; s_sum[2][t] = c; s_sum[3][t] = d;
   __syncthreads();
   if (t != 0) return;
@@ -447,18 +461,12 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
   const bool tail_done = q.counters[kTailDone] != 0;
   const uint32_t active = tail_done ? 0u : (q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap);
   uint64_t work_visits = 0, work_leaves = 0;
-  // blocks past the live work exit without touching the shared counters
-  const bool has_work = (uint64_t)blockIdx.x * kRtBlock < active;
-  const uint32_t batch = fetch_batch(active);
-  for (uint32_t base = 0, r = batch; has_work;) {
-    if (r >= batch) {
-      if (lane == 0) base = atomicAdd(q.counters + kFetchExt, batch);
-      base = __shfl(base, 0, 64);
-      r = 0;
-    }
-    if (base + r >= active) break;
-    const uint32_t idx = base + r + lane;
-    r += 64;
+  // One thread per queue entry; the grid covers the arena, and blocks past
+  // the live entries exit at once (a persistent loop here made the compiler
+  // keep far more registers live, halving the resident waves).
+  if ((uint64_t)blockIdx.x * kRtBlock >= active) return;
+  {
+    const uint32_t idx = (uint32_t)gtid;
     Surface su;
     su.found = false;
     Ray ray;
@@ -478,10 +486,10 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
 #pragma unroll
         for (int p = 0; p < 8; ++p) f[p] = pf[p];
 #pragma unroll
-        for (int p = 0; p < kSamplePlanes; ++p) q.sample[p * cap + idx] = 0.0f;
+        for (int p = 0; p < kSamplePlanes; ++p) q.sample[plane_at<kSamplePlanes>(idx, p)] = 0.0f;
       } else {
 #pragma unroll
-        for (int p = 0; p < 8; ++p) f[p] = q.src_fields[p * cap + idx];
+        for (int p = 0; p < 8; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(idx, p)];
       }
       ray = Ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
       TraceCounters tc;
@@ -494,8 +502,8 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
         const uint32_t idw = q.src_ids[idx];
         path_id = idw & kPathIdMask;
         delta = (idw & kPathDelta) != 0;
-        T = mk(q.src_fields[8 * cap + idx], q.src_fields[9 * cap + idx], q.src_fields[10 * cap + idx]);
-        path_pdf = q.src_fields[11 * cap + idx];
+        T = mk(q.src_fields[plane_at<kPathPlanes>(idx, 8)], q.src_fields[plane_at<kPathPlanes>(idx, 9)], q.src_fields[plane_at<kPathPlanes>(idx, 10)]);
+        path_pdf = q.src_fields[plane_at<kPathPlanes>(idx, 11)];
       } else {
         path_id = q.sample_base + idx;
         delta = true;
@@ -509,7 +517,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
       if (ss < cap) {
         q.shadow_ids[ss] = sw.id;
 #pragma unroll
-        for (int p = 0; p < kShadowPlanes; ++p) q.shadow_fields[p * cap + ss] = sw.f[p];
+        for (int p = 0; p < kShadowPlanes; ++p) q.shadow_fields[plane_at<kShadowPlanes>(ss, p)] = sw.f[p];
       } else {
         atomicMax(q.counters + kStatus, kStatusShadowOverflow);
       }
@@ -520,14 +528,14 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
       bool has_add;
       v3 add_c;
       shade_bounce<kAlg>(sc, seed, bounce, path_id, delta, ray, T, path_pdf, t_hit, su, has_add, add_c, want_next, np);
-      if (has_add) accumulate(q.sample, cap, path_id - q.sample_base, add_c);
+      if (has_add) accumulate(q.sample, path_id - q.sample_base, add_c);
     }
     const uint32_t ns = wave_reserve(q.counters + kNext, want_next);
     if (want_next) {
       if (ns < cap) {
         q.dst_ids[ns] = np.id;
 #pragma unroll
-        for (int p = 0; p < kPathPlanes; ++p) q.dst_fields[p * cap + ns] = np.f[p];
+        for (int p = 0; p < kPathPlanes; ++p) q.dst_fields[plane_at<kPathPlanes>(ns, p)] = np.f[p];
       } else {
         atomicMax(q.counters + kStatus, kStatusPathOverflow);
       }
@@ -582,10 +590,10 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
       path_id = idw & kPathIdMask;
       delta = (idw & kPathDelta) != 0;
 #pragma unroll
-      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[p * cap + idx];
+      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(idx, p)];
       slot = path_id - q.sample_base;
 #pragma unroll
-      for (int p = 0; p < kSamplePlanes; ++p) acc[p] = q.sample[p * cap + slot];
+      for (int p = 0; p < kSamplePlanes; ++p) acc[p] = q.sample[plane_at<kSamplePlanes>(slot, p)];
     }
     for (uint32_t k = 0; k < nb; ++k) {
       const uint32_t b = bounce + k;
@@ -642,7 +650,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
     }
     if (idx < active) {
 #pragma unroll
-      for (int p = 0; p < kSamplePlanes; ++p) q.sample[p * cap + slot] = acc[p];
+      for (int p = 0; p < kSamplePlanes; ++p) q.sample[plane_at<kSamplePlanes>(slot, p)] = acc[p];
     }
   }
   wave_add(stats + kWorkBase + 0, (uint32_t)ext_visits);
@@ -662,21 +670,12 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
   const uint32_t count =
       q.counters[kTailDone] ? 0u : (q.counters[kShadow] < cap ? q.counters[kShadow] : (uint32_t)cap);
   uint64_t work_visits = 0, work_leaves = 0;
-  const bool has_work = (uint64_t)blockIdx.x * kRtBlock < count;
-  const uint32_t batch = fetch_batch(count);
-  for (uint32_t base = 0, r = batch; has_work;) {
-    if (r >= batch) {
-      if (lane == 0) base = atomicAdd(q.counters + kFetchShadow, batch);
-      base = __shfl(base, 0, 64);
-      r = 0;
-    }
-    if (base + r >= count) break;
-    const uint32_t idx = base + r + lane;
-    r += 64;
-    if (idx >= count) continue;
+  if ((uint64_t)blockIdx.x * kRtBlock >= count) return;
+  const uint32_t idx = (uint32_t)gtid;
+  if (idx < count) {
     float f[kShadowPlanes];
 #pragma unroll
-    for (int p = 0; p < kShadowPlanes; ++p) f[p] = q.shadow_fields[p * cap + idx];
+    for (int p = 0; p < kShadowPlanes; ++p) f[p] = q.shadow_fields[plane_at<kShadowPlanes>(idx, p)];
     const uint32_t slot = q.shadow_ids[idx] - q.sample_base;
     const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
     TraceCounters tc;
@@ -684,7 +683,7 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
     work_visits += tc.visits;
     work_leaves += tc.leaves;
     const bool occluded = h.prim == kEmpty && h.t == 0.0f;
-    if (kAlg == BAJO_RENDER_AO || !occluded) accumulate(q.sample, cap, slot, shadow_radiance<kAlg>(occluded, f));
+    if (kAlg == BAJO_RENDER_AO || !occluded) accumulate(q.sample, slot, shadow_radiance<kAlg>(occluded, f));
   }
   store_block_work(work, work_visits, work_leaves);
 }
@@ -697,9 +696,9 @@ __global__ void __launch_bounds__(kRtBlock) resolve_kernel(const float* __restri
   float r = 0.0f, g = 0.0f, b = 0.0f;
   const uint64_t s0 = (uint64_t)p * spp;
   for (uint64_t s = s0; s < s0 + spp; ++s) {
-    r += sample[0 * cap + s];
-    g += sample[1 * cap + s];
-    b += sample[2 * cap + s];
+    r += sample[plane_at<kSamplePlanes>((uint32_t)s, 0)];
+    g += sample[plane_at<kSamplePlanes>((uint32_t)s, 1)];
+    b += sample[plane_at<kSamplePlanes>((uint32_t)s, 2)];
   }
   const float k = 1.0f / (float)spp;
   const uint64_t o = 3ull * (pixel_base + p);
@@ -778,13 +777,14 @@ int32_t grid_of(K kernel, unsigned* grid) {
 int32_t ensure_arena(bajo_scene* s, uint64_t cap, uint64_t spill_threads) {
   if (cap <= s->cap && spill_threads <= s->spill_threads) return BAJO_OK;
   free_arena(s);
+  const uint64_t tiles = (cap + 63) / 64 * 64;  // plane storage is whole 64-entry tiles
   for (int k = 0; k < 2; ++k) {
     BJ_HIP(hipMalloc(&s->ids[k], cap * sizeof(uint32_t)));
-    BJ_HIP(hipMalloc(&s->fields[k], cap * kPathPlanes * sizeof(float)));
+    BJ_HIP(hipMalloc(&s->fields[k], tiles * kPathPlanes * sizeof(float)));
   }
   BJ_HIP(hipMalloc(&s->shadow_ids, cap * sizeof(uint32_t)));
-  BJ_HIP(hipMalloc(&s->shadow_fields, cap * kShadowPlanes * sizeof(float)));
-  BJ_HIP(hipMalloc(&s->sample, cap * kSamplePlanes * sizeof(float)));
+  BJ_HIP(hipMalloc(&s->shadow_fields, tiles * kShadowPlanes * sizeof(float)));
+  BJ_HIP(hipMalloc(&s->sample, tiles * kSamplePlanes * sizeof(float)));
   BJ_HIP(hipMalloc(&s->counters, kCounterWords * sizeof(uint32_t)));
   BJ_HIP(hipMalloc(&s->stats, kStatsWords * sizeof(unsigned long long)));
   BJ_HIP(hipMalloc(&s->spill, spill_threads * kSpill * sizeof(uint32_t)));
@@ -848,26 +848,19 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   if (cap > total) cap = total;
   cap = (cap / st->samples_per_pixel) * st->samples_per_pixel;
   if (cap == 0) cap = st->samples_per_pixel;
-  unsigned g_ext = 0, g_sh = 0;
-  int32_t rc = grid_of(extend_kernel<kAlg, kWide, false>, &g_ext);
-  if (rc) return rc;
-  rc = grid_of(shadow_kernel<kAlg, kWide>, &g_sh);
-  if (rc) return rc;
+  int32_t rc = BAJO_OK;
   constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
   unsigned g_tail = 0;
   if constexpr (kTail) {
     rc = grid_of(tail_kernel<kAlg, kWide>, &g_tail);
     if (rc) return rc;
   }
-  const unsigned need = div_up(cap, kRtBlock);
-  if (g_ext > need) g_ext = need;
-  if (g_sh > need) g_sh = need;
-  if (g_ext < 1) g_ext = 1;
-  if (g_sh < 1) g_sh = 1;
+  // extend and shadow launch one thread per arena entry
+  const unsigned g_ext = div_up(cap, kRtBlock), g_sh = g_ext;
   if (g_tail < 1) g_tail = 1;
   // the tail takes over when the remaining paths fit one pass of its resident threads
   const uint32_t tail_threshold = g_tail * kRtBlock;
-  const unsigned g_max = g_ext > g_sh ? (g_ext > g_tail ? g_ext : g_tail) : (g_sh > g_tail ? g_sh : g_tail);
+  const unsigned g_max = g_ext > g_tail ? g_ext : g_tail;
   const uint64_t spill_threads = (uint64_t)g_max * kRtBlock;
   rc = ensure_arena(s, cap, spill_threads);
   if (rc) return rc;
@@ -903,7 +896,7 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
         }
       }
       if (b == 0)
-        extend_kernel<kAlg, kWide, true><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, s->spill,
+        extend_kernel<kAlg, kWide, true><<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, s->spill,
                                                                          s->work_ext);
       else
         extend_kernel<kAlg, kWide, false><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, s->spill,

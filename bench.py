@@ -56,10 +56,25 @@ def extend_bytes(st: dict, node_width: int) -> float:
     )
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "render_icosphere4m_hbm_traffic.md"
+
+
+def measured_traffic() -> int | None:
+    """HBM bytes of the NEE render's extend launches, as measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE (copy-calibrated) and committed in profiles/."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("extend_kernel<NEE> HBM bytes per render:"):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(scene, cam, threads: int) -> dict:
     import oracle
 
-    w, h = WIDTH // 4, HEIGHT // 4
+    w, h = WIDTH, HEIGHT
     t0 = time.perf_counter()
     oracle.build_lbvh(scene.vertices)
     build_s = time.perf_counter() - t0
@@ -86,6 +101,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--node-width", type=int, default=8, choices=(2, 8), help="BVH layout: binary or 8-wide")
     ap.add_argument("--leaf-size", type=int, default=4, help="8-wide leaf size (triangles per leaf child)")
+    ap.add_argument("--method", choices=("lbvh", "hploc"), default="lbvh", help="binary builder")
     ap.add_argument(
         "--copy-calibration-mib",
         type=int,
@@ -112,7 +128,8 @@ def main() -> int:
     dev = ctx.torch_device
     print(f"[bench] device {ctx.name()}", file=sys.stderr, flush=True)
     scene = scenes.icosphere_scene(N_SPHERE)
-    gscene = GpuScene(scene, ctx, node_width=args.node_width, leaf_size=args.leaf_size)
+    method = {"lbvh": 0, "hploc": 1}[args.method]
+    gscene = GpuScene(scene, ctx, node_width=args.node_width, leaf_size=args.leaf_size, method=method)
     cam = Camera((0.0, 0.0, -3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), FOV_SCALE).flatten()
     settings = RenderSettings(WIDTH, HEIGHT, SPP, SEED, DEPTH)
     pixels = {a: torch.empty((HEIGHT, WIDTH, 3), dtype=torch.float32, device=dev) for a in (RENDER.NEE, RENDER.PATH)}
@@ -178,10 +195,11 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "icosphere-4M + quad light: LBVH build, then wavefront NEE and PATH renders "
+            "workload": "icosphere-4M + quad light: BVH build, then wavefront NEE and PATH renders "
             "at 1920x1080x8 spp, max depth 5 (Lambertian albedo U[0.2,0.8]^3, emission 15)",
             "triangles": gscene.triangle_count,
-            "bvh": "8-wide quantized (leaf size %d)" % args.leaf_size if args.node_width == 8 else "binary LBVH",
+            "bvh": (f"{args.method.upper()} collapsed to 8-wide quantized nodes (leaf size {args.leaf_size})"
+                    if args.node_width == 8 else f"binary {args.method.upper()}"),
             "paths_per_render": WIDTH * HEIGHT * SPP,
         },
         "rays_per_step": rays_per_step,
@@ -207,7 +225,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_note": "HBM bytes per NEE render over the extend launches, rocprofv3 FETCH_SIZE x 2.0 + "
+            "WRITE_SIZE x 1.0 (1 GiB copy calibration), profiles/render_icosphere4m_hbm_traffic.md",
             "launches_per_render": ext_launches,
             "avg_launch_ms": ext_ms_nee / max(ext_launches, 1),
             "bytes_per_render": ext_bytes,

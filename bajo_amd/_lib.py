@@ -101,6 +101,8 @@ class RenderStats(ctypes.Structure):
         ("shadow_launches", c_uint32),
         ("extend_bounce_ms", c_float * 16),
         ("shadow_bounce_ms", c_float * 16),
+        ("max_extension_visits", c_uint32 * 16),
+        ("max_shadow_visits", c_uint32 * 16),
     ]
 
 

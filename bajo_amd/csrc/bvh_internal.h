@@ -87,7 +87,7 @@ __global__ void karras_kernel(const uint64_t* __restrict__ keys, uint64_t n_u, B
                               uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ leaf_counts,
                               uint32_t* __restrict__ flags, uint32_t* __restrict__ range_first);
 // Triangle i's vertices from the f32[9N] input.
-BJ_D inline void load_tri_vertices(const float* __restrict__ v, uint64_t i, v3& a, v3& b, v3& c) {
+BJ_D void load_tri_vertices(const float* __restrict__ v, uint64_t i, v3& a, v3& b, v3& c) {
   const float* t = v + 9 * i;
   a = mk(t[0], t[1], t[2]);
   b = mk(t[3], t[4], t[5]);

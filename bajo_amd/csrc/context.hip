@@ -80,8 +80,8 @@ int32_t bajo_ctx_create(int32_t device, void* stream, bajo_ctx** out) {
 int32_t bajo_ctx_destroy(bajo_ctx* ctx) {
   if (!ctx) return BAJO_OK;
   if (ctx->scratch) {
-    hipStreamSynchronize(ctx->stream);
-    hipFree(ctx->scratch);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->scratch);
   }
   delete ctx;
   return BAJO_OK;

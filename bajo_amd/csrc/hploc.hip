@@ -38,7 +38,7 @@ constexpr int kHpRadius = 8;
 constexpr uint32_t kHpThreshold = 16;
 constexpr int kHpMaxLoop = 512;  // climb iterations per wave (depth is far smaller)
 // ctl words
-constexpr int kHpNext = 0, kHpStatus = 1, kHpRoot = 2;
+constexpr int kHpLeaves = 0, kHpStatus = 1, kHpRoot = 2;
 constexpr uint32_t kHpStatusNoProgress = 1, kHpStatusBadList = 2;
 
 // A cluster: subtree reference (bit 31 = leaf at a sorted position), its
@@ -60,22 +60,33 @@ BJ_D float load_co(const float* p) {
   return __builtin_bit_cast(float, load_co(reinterpret_cast<const uint32_t*>(p)));
 }
 
+BJ_D uint64_t pack_u(uint32_t a, uint32_t b) { return (uint64_t)a | ((uint64_t)b << 32); }
+BJ_D uint64_t pack_f(float a, float b) {
+  return pack_u(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b));
+}
+BJ_D float lo_f(uint64_t w) { return __builtin_bit_cast(float, (uint32_t)w); }
+BJ_D float hi_f(uint64_t w) { return __builtin_bit_cast(float, (uint32_t)(w >> 32)); }
+
+// An entry moves as four 8-byte words: (ref, cnt), (lo.x, lo.y), (lo.z, hi.x), (hi.y, hi.z).
 BJ_D void store_entry(HpEntry* p, const HpEntry& e) {
-  store_wt(&p->ref, e.ref);
-  store_wt(&p->cnt, e.cnt);
-  for (int k = 0; k < 3; ++k) {
-    store_wt(&p->lo[k], e.lo[k]);
-    store_wt(&p->hi[k], e.hi[k]);
-  }
+  uint64_t* w = reinterpret_cast<uint64_t*>(p);
+  __hip_atomic_store(w + 0, pack_u(e.ref, e.cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(w + 1, pack_f(e.lo[0], e.lo[1]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(w + 2, pack_f(e.lo[2], e.hi[0]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(w + 3, pack_f(e.hi[1], e.hi[2]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 BJ_D HpEntry load_entry(const HpEntry* p) {
+  uint64_t* w = reinterpret_cast<uint64_t*>(const_cast<HpEntry*>(p));
+  const uint64_t a = __hip_atomic_load(w + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t b = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t c = __hip_atomic_load(w + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t d = __hip_atomic_load(w + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   HpEntry e;
-  e.ref = load_co(&p->ref);
-  e.cnt = load_co(&p->cnt);
-  for (int k = 0; k < 3; ++k) {
-    e.lo[k] = load_co(&p->lo[k]);
-    e.hi[k] = load_co(&p->hi[k]);
-  }
+  e.ref = (uint32_t)a;
+  e.cnt = (uint32_t)(a >> 32);
+  e.lo[0] = lo_f(b); e.lo[1] = hi_f(b);
+  e.lo[2] = lo_f(c); e.hi[0] = hi_f(c);
+  e.hi[1] = lo_f(d); e.hi[2] = hi_f(d);
   return e;
 }
 
@@ -126,7 +137,7 @@ BJ_D void write_node(BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_cou
 // The wave processes guide node `g` (all 64 lanes call this together).
 BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
                        const uint32_t* __restrict__ range_first, HpEntry* __restrict__ clusters,
-                       uint32_t* __restrict__ counts, BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_counts,
+                       uint32_t* __restrict__ ids, uint32_t* __restrict__ counts, BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_counts,
                        uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ ctl) {
   const uint32_t lane = lane_id();
   const uint32_t first = range_first[g], last = guide[g].fence;
@@ -142,9 +153,28 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
   }
   const bool root = first == 0 && (uint64_t)last == n - 1;
   const uint32_t thr = root ? 1u : kHpThreshold;
+  if (c <= thr) {
+    // no merging: the list is the left list (already at `first`) followed by
+    // the right list, which moves only if the left one is shorter than its range
+    if (split + 1 != first + cl && lane < cr) {
+      const HpEntry m = load_entry(clusters + split + 1 + lane);
+      const uint32_t mid = load_co(ids + split + 1 + lane);
+      store_entry(clusters + first + cl + lane, m);
+      store_wt(ids + first + cl + lane, mid);
+    }
+    if (lane == 0) store_wt(counts + g, c);
+    publish_fence();
+    return;
+  }
   HpEntry e{};
-  if (lane < cl) e = load_entry(clusters + first + lane);
-  else if (lane < c) e = load_entry(clusters + split + 1 + (lane - cl));
+  uint32_t id = 0;
+  if (lane < cl) {
+    e = load_entry(clusters + first + lane);
+    id = load_co(ids + first + lane);
+  } else if (lane < c) {
+    e = load_entry(clusters + split + 1 + (lane - cl));
+    id = load_co(ids + split + 1 + (lane - cl));
+  }
   for (int round = 0; c > thr && round < 64; ++round) {
     const bool live = lane < c;
     float best_area = kFltMax;
@@ -186,12 +216,9 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
       mutual = lane < 2;
       mergers = 1;
     }
-    const uint32_t k = (uint32_t)__builtin_popcountll(mergers);
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(ctl + kHpNext, k);
-    base = shfl_u(base, 0);
     // partner entry
     const int src = best < 0 ? (int)lane : best;
+    const uint32_t pid = shfl_u(id, src);
     HpEntry p;
     p.ref = shfl_u(e.ref, src);
     p.cnt = shfl_u(e.cnt, src);
@@ -201,7 +228,12 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
     }
     const bool merger = (mergers >> lane) & 1u;
     if (merger) {
-      const uint32_t idx = base + mask_rank(mergers);
+      // Node numbering without a shared counter: every cluster carries the
+      // lowest sorted leaf position it holds; a merge retires the larger of
+      // the two and the new node takes index (retired - 1).  Leaf 0 is never
+      // retired, so the n-1 merges use the indices 0..n-2 exactly once.
+      const uint32_t idx = (id > pid ? id : pid) - 1u;
+      id = id < pid ? id : pid;
       const bool refs_ok = ((e.ref & kLeafFlag) ? (uint64_t)(e.ref & kIndexMask) < n : (uint64_t)e.ref < n - 1) &&
                            ((p.ref & kLeafFlag) ? (uint64_t)(p.ref & kIndexMask) < n : (uint64_t)p.ref < n - 1);
       if ((uint64_t)idx < n - 1 && refs_ok) write_node(nodes, leaf_counts, leaf_parent, idx, e, p);
@@ -221,6 +253,7 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
     HpEntry m;
     m.ref = permute_u(dst, e.ref);
     m.cnt = permute_u(dst, e.cnt);
+    id = permute_u(dst, id);
     for (int q = 0; q < 3; ++q) {
       m.lo[q] = permute_f(dst, e.lo[q]);
       m.hi[q] = permute_f(dst, e.hi[q]);
@@ -232,7 +265,10 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
     if (lane == 0) atomicOr(ctl + kHpStatus, kHpStatusNoProgress);
     return;
   }
-  if (lane < c) store_entry(clusters + first + lane, e);
+  if (lane < c) {
+    store_entry(clusters + first + lane, e);
+    store_wt(ids + first + lane, id);
+  }
   if (lane == 0) {
     store_wt(counts + g, c);
     if (root) store_wt(ctl + kHpRoot, e.ref);
@@ -246,6 +282,7 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
                                                          const uint32_t* __restrict__ range_first,
                                                          uint32_t* __restrict__ leaf_parent,
                                                          uint32_t* __restrict__ flags, HpEntry* __restrict__ clusters,
+                                                         uint32_t* __restrict__ ids,
                                                          uint32_t* __restrict__ counts, BjNode* __restrict__ nodes,
                                                          uint32_t* __restrict__ leaf_counts,
                                                          uint32_t* __restrict__ ctl) {
@@ -261,6 +298,7 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
     const v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
     HpEntry e{kLeafFlag | (uint32_t)i, 1u, {lo.x, lo.y, lo.z}, {hi.x, hi.y, hi.z}};
     store_entry(clusters + i, e);
+    store_wt(ids + i, (uint32_t)i);
   }
   publish_fence();
   for (int it = 0; it < kHpMaxLoop; ++it) {
@@ -279,12 +317,36 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
       }
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    // Nodes whose two lists together stay within the threshold need no
+    // merging: each lane finishes its own (appending the right list after
+    // the left one) and keeps climbing.  Only merging nodes take the wave.
+    if (pending) {
+      const uint32_t g = cur;
+      const uint32_t first = range_first[g], last = guide[g].fence;
+      const uint32_t gl = guide[g].left, gr = guide[g].right;
+      const uint32_t split = gl & kIndexMask;
+      const uint32_t cl = (gl & kLeafFlag) ? 1u : load_co(counts + split);
+      const uint32_t cr = (gr & kLeafFlag) ? 1u : load_co(counts + (gr & kIndexMask));
+      const bool root = first == 0 && (uint64_t)last == n - 1;
+      const bool sane = cl >= 1 && cr >= 1 && cl <= kHpThreshold && cr <= kHpThreshold && first <= split &&
+                        split < last && (uint64_t)last < n && cl <= split - first + 1 && cr <= last - split;
+      if (sane && !root && cl + cr <= kHpThreshold) {
+        if (split + 1 != first + cl)
+          for (uint32_t k = 0; k < cr; ++k) {
+            store_entry(clusters + first + cl + k, load_entry(clusters + split + 1 + k));
+            store_wt(ids + first + cl + k, load_co(ids + split + 1 + k));
+          }
+        store_wt(counts + g, cl + cr);
+        pending = false;
+      }
+    }
+    publish_fence();
     uint64_t todo = ballot(pending);
     while (todo) {
       const int leader = __builtin_ctzll(todo);
       todo &= todo - 1;
       const uint32_t g = shfl_u(cur, leader);
-      process_node(g, n, guide, range_first, clusters, counts, nodes, leaf_counts, leaf_parent, ctl);
+      process_node(g, n, guide, range_first, clusters, ids, counts, nodes, leaf_counts, leaf_parent, ctl);
       if ((int)lane_id() == leader) {
         pending = false;
         if (range_first[g] == 0 && (uint64_t)guide[g].fence == n - 1) active = false;
@@ -299,11 +361,16 @@ __global__ void hploc_init_kernel(uint32_t* __restrict__ ctl) {
   if (threadIdx.x < 4) ctl[threadIdx.x] = 0;
 }
 
-// The root gets no parent from a merge.
-__global__ void hploc_root_kernel(BjNode* __restrict__ nodes, uint64_t n, const uint32_t* __restrict__ ctl) {
+// The root gets no parent from a merge; its leaf count goes to ctl word
+// kHpLeaves for the host's completeness check.
+__global__ void hploc_root_kernel(BjNode* __restrict__ nodes, uint64_t n, const uint32_t* __restrict__ leaf_counts,
+                                  uint32_t* __restrict__ ctl) {
   if (threadIdx.x != 0) return;
   const uint32_t root = ctl[kHpRoot];
-  if (!(root & kLeafFlag) && (uint64_t)root < n - 1) nodes[root].parent = kSentinel;
+  if (!(root & kLeafFlag) && (uint64_t)root < n - 1) {
+    nodes[root].parent = kSentinel;
+    ctl[kHpLeaves] = leaf_counts[root];
+  }
 }
 
 // Depth-first position of each sorted leaf: the leaves of the left subtrees
@@ -382,10 +449,11 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
   hploc_init_kernel<<<1, 64, 0, st>>>(h->hp_ctl);
   karras_kernel<<<div_up(n - 1, kHpBlock), kHpBlock, 0, st>>>(h->keys, n, h->hp_guide, h->leaf_parent,
                                                                h->leaf_counts, h->flags, h->hp_first);
+  // hp_pos holds the clusters' lowest leaf positions until the depth-first pass
   hploc_kernel<<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
-                                                         h->leaf_parent, h->flags, clusters, h->hp_counts, h->nodes,
-                                                         h->leaf_counts, h->hp_ctl);
-  hploc_root_kernel<<<1, 64, 0, st>>>(h->nodes, n, h->hp_ctl);
+                                                         h->leaf_parent, h->flags, clusters, h->hp_pos, h->hp_counts,
+                                                         h->nodes, h->leaf_counts, h->hp_ctl);
+  hploc_root_kernel<<<1, 64, 0, st>>>(h->nodes, n, h->leaf_counts, h->hp_ctl);
   BJ_HIP(hipGetLastError());
   // the cluster area is free now: it holds the reordered keys and leaf parents
   uint64_t* keys_dfs = reinterpret_cast<uint64_t*>(h->hp_clusters);
@@ -402,8 +470,8 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
   const uint32_t* c = h->host_hp_ctl;
   BJ_REQUIRE(c[kHpStatus] == 0, "H-PLOC build failed with status %u (1 = no progress, 2 = corrupt cluster list)",
              c[kHpStatus]);
-  BJ_REQUIRE((uint64_t)c[kHpNext] == n - 1, "H-PLOC built %u of %llu nodes", c[kHpNext],
-             (unsigned long long)(n - 1));
+  BJ_REQUIRE((uint64_t)c[kHpLeaves] == n, "H-PLOC root holds %u of %llu leaves", c[kHpLeaves],
+             (unsigned long long)n);
   BJ_REQUIRE(!(c[kHpRoot] & kLeafFlag) && (uint64_t)c[kHpRoot] < n - 1, "H-PLOC root %u is invalid", c[kHpRoot]);
   h->root = c[kHpRoot];
   return BAJO_OK;

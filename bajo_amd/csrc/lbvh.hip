@@ -212,11 +212,21 @@ __global__ void __launch_bounds__(kBlock) karras_kernel(const uint64_t* __restri
 
 namespace {
 
+// Bottom-up refit, one thread per sorted leaf.  Nodes whose leaf range lies
+// inside this block's 256 leaves are merged in LDS (arrival counters and
+// child boxes in shared memory; the second arriver writes the node's two
+// child boxes with plain stores).  Only nodes whose range crosses a block
+// boundary use the cross-XCD hand-off through global memory.
 __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__ vertices,
                                                        const uint64_t* __restrict__ keys, uint64_t n,
                                                        BjNode* __restrict__ nodes,
                                                        uint32_t* __restrict__ leaf_parent,
+                                                       const uint32_t* __restrict__ leaf_counts,
                                                        uint32_t* __restrict__ flags, BjTri* __restrict__ tris) {
+  __shared__ uint32_t s_flag[kBlock];
+  __shared__ float s_box[kBlock][2][6];
+  s_flag[threadIdx.x] = 0;
+  __syncthreads();
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t prim = (uint32_t)keys[s];
@@ -231,6 +241,8 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
     leaf_parent[0] = kSentinel;  // single-leaf segment, lbvh.mojo:246-248
     return;
   }
+  const uint64_t block_first = (uint64_t)blockIdx.x * blockDim.x;
+  const uint64_t block_last = block_first + blockDim.x - 1 < n - 1 ? block_first + blockDim.x - 1 : n - 1;
   // bvh/gpu/triangle_bvh.mojo:711-712: leaf box = vmin(vmin(v0,v1),v2).
   v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
   uint32_t cur = kLeafFlag | (uint32_t)s;
@@ -242,30 +254,56 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
     const uint32_t left = node->left, right = node->right;  // written by the topology launch
     const bool is_left = cur == left;
     if (!is_left && cur != right) break;
-    // Publish this child's box with write-through (agent-scope) stores and
-    // wait until they are performed before the arrival counter is bumped.
-    // This is the release half of the hand-off without an L2 writeback: the
-    // shared data itself never sits in a non-coherent cache line.
-    // Boxes move as three 8-byte words (child halves start at byte 0 and 24).
-    uint64_t* mine = reinterpret_cast<uint64_t*>(node->b + (is_left ? 0 : 6));
-    const uint64_t w0 = pack2(lo.x, lo.y), w1 = pack2(lo.z, hi.x), w2 = pack2(hi.y, hi.z);
-    __hip_atomic_store(mine + 0, w0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(mine + 1, w1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(mine + 2, w2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const uint32_t old = __hip_atomic_fetch_add(flags + parent, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) break;
-    // Second arriver: the sibling's stores completed before its increment,
-    // which returned before ours; coherent loads read them.
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const uint64_t* other = reinterpret_cast<const uint64_t*>(node->b + (is_left ? 6 : 0));
-    float o[6];
-    unpack2(__hip_atomic_load(other + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), o[0], o[1]);
-    unpack2(__hip_atomic_load(other + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), o[2], o[3]);
-    unpack2(__hip_atomic_load(other + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), o[4], o[5]);
-    v3 olo = mk(o[0], o[1], o[2]), ohi = mk(o[3], o[4], o[5]);
+    // Karras node i is one end of its leaf range [fence + 1 - count, fence]
+    const uint64_t last = node->fence, first = last + 1 - leaf_counts[parent];
+    v3 olo, ohi;
+    if (first >= block_first && last <= block_last && parent >= block_first && parent - block_first < kBlock) {
+      // every leaf below this node belongs to this block
+      float* mine = s_box[parent - block_first][is_left ? 0 : 1];
+      mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z; mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // LDS writes done (lgkmcnt 0)
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      const uint32_t old = atomicAdd(&s_flag[parent - block_first], 1u);
+      if (old == 0) break;
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      const float* o = s_box[parent - block_first][is_left ? 1 : 0];
+      olo = mk(o[0], o[1], o[2]);
+      ohi = mk(o[3], o[4], o[5]);
+      // both child boxes of the node
+      float* l = node->b;
+      const v3 llo = is_left ? lo : olo, lhi = is_left ? hi : ohi;
+      const v3 rlo = is_left ? olo : lo, rhi = is_left ? ohi : hi;
+      reinterpret_cast<float4*>(l)[0] = make_float4(llo.x, llo.y, llo.z, lhi.x);
+      reinterpret_cast<float4*>(l)[1] = make_float4(lhi.y, lhi.z, rlo.x, rlo.y);
+      reinterpret_cast<float4*>(l)[2] = make_float4(rlo.z, rhi.x, rhi.y, rhi.z);
+    } else {
+      // Publish this child's box with write-through (agent-scope) stores and
+      // wait until they are performed before the arrival counter is bumped.
+      // This is the release half of the hand-off without an L2 writeback:
+      // the shared data itself never sits in a non-coherent cache line.
+      // Boxes move as three 8-byte words (child halves start at byte 0 and 24).
+      uint64_t* mine = reinterpret_cast<uint64_t*>(node->b + (is_left ? 0 : 6));
+      const uint64_t w0 = pack2(lo.x, lo.y), w1 = pack2(lo.z, hi.x), w2 = pack2(hi.y, hi.z);
+      __hip_atomic_store(mine + 0, w0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(mine + 1, w1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(mine + 2, w2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      const uint32_t old = __hip_atomic_fetch_add(flags + parent, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (old == 0) break;
+      // Second arriver: the sibling's stores completed before its increment,
+      // which returned before ours; coherent loads read them.
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      const uint64_t* other = reinterpret_cast<const uint64_t*>(node->b + (is_left ? 6 : 0));
+      float o[6];
+      unpack2(__hip_atomic_load(other + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), o[0], o[1]);
+      unpack2(__hip_atomic_load(other + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), o[2], o[3]);
+      unpack2(__hip_atomic_load(other + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), o[4], o[5]);
+      olo = mk(o[0], o[1], o[2]);
+      ohi = mk(o[3], o[4], o[5]);
+    }
     // _load_and_union_node_bounds (binary_layout.mojo:95-101): merge(left, right).
     if (is_left) {
       lo = vmin(lo, olo);
@@ -282,8 +320,8 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
 int32_t ensure_capacity(bajo_bvh* h, uint64_t n) {
   SortPlan plan = make_sort_plan(n, 8, 0, kSortBeginBit, kSortEndBit);
   if (n <= h->capacity && plan.total <= h->sort_ws_bytes) return BAJO_OK;
-  hipFree(h->nodes); hipFree(h->tris); hipFree(h->keys); hipFree(h->leaf_parent); hipFree(h->leaf_counts);
-  hipFree(h->flags); hipFree(h->acc); hipFree(h->scene); hipFree(h->sort_ws);
+  (void)hipFree(h->nodes); (void)hipFree(h->tris); (void)hipFree(h->keys); (void)hipFree(h->leaf_parent); (void)hipFree(h->leaf_counts);
+  (void)hipFree(h->flags); (void)hipFree(h->acc); (void)hipFree(h->scene); (void)hipFree(h->sort_ws);
   h->nodes = nullptr; h->tris = nullptr; h->keys = nullptr; h->leaf_parent = nullptr; h->leaf_counts = nullptr;
   h->flags = nullptr; h->acc = nullptr; h->scene = nullptr; h->sort_ws = nullptr;
   h->capacity = 0;
@@ -304,7 +342,7 @@ int32_t ensure_capacity(bajo_bvh* h, uint64_t n) {
 }
 
 void free_wide(bajo_bvh* h) {
-  hipFree(h->wide); hipFree(h->wide_src); hipFree(h->wide_ctl); hipHostFree(h->host_ctl);
+  (void)hipFree(h->wide); (void)hipFree(h->wide_src); (void)hipFree(h->wide_ctl); (void)hipHostFree(h->host_ctl);
   h->wide = nullptr; h->wide_src = nullptr; h->wide_ctl = nullptr; h->host_ctl = nullptr;
   h->wide_capacity = 0;
   h->wide_count = 0;
@@ -324,8 +362,8 @@ int32_t ensure_wide_capacity(bajo_bvh* h, uint64_t n) {
 }
 
 void free_hploc(bajo_bvh* h) {
-  hipFree(h->hp_guide); hipFree(h->hp_first); hipFree(h->hp_clusters); hipFree(h->hp_counts); hipFree(h->hp_pos);
-  hipFree(h->hp_ctl); hipHostFree(h->host_hp_ctl);
+  (void)hipFree(h->hp_guide); (void)hipFree(h->hp_first); (void)hipFree(h->hp_clusters); (void)hipFree(h->hp_counts); (void)hipFree(h->hp_pos);
+  (void)hipFree(h->hp_ctl); (void)hipHostFree(h->host_hp_ctl);
   h->hp_guide = nullptr; h->hp_first = nullptr; h->hp_clusters = nullptr; h->hp_counts = nullptr;
   h->hp_pos = nullptr; h->hp_ctl = nullptr; h->host_hp_ctl = nullptr;
   h->hp_capacity = 0;
@@ -352,8 +390,8 @@ int32_t ensure_hploc_capacity(bajo_bvh* h, uint64_t n) {
 void free_bvh(bajo_bvh* h) {
   if (!h) return;
   free_hploc(h);
-  hipFree(h->nodes); hipFree(h->tris); hipFree(h->keys); hipFree(h->leaf_parent); hipFree(h->leaf_counts);
-  hipFree(h->flags); hipFree(h->acc); hipFree(h->scene); hipFree(h->sort_ws);
+  (void)hipFree(h->nodes); (void)hipFree(h->tris); (void)hipFree(h->keys); (void)hipFree(h->leaf_parent); (void)hipFree(h->leaf_counts);
+  (void)hipFree(h->flags); (void)hipFree(h->acc); (void)hipFree(h->scene); (void)hipFree(h->sort_ws);
   free_wide(h);
   delete h;
 }
@@ -368,11 +406,11 @@ struct StageEvents {
     return BAJO_OK;
   }
   void record(int k, hipStream_t s) {
-    if (on) hipEventRecord(ev[k], s);
+    if (on) (void)hipEventRecord(ev[k], s);
   }
   ~StageEvents() {
     if (on)
-      for (auto& e : ev) hipEventDestroy(e);
+      for (auto& e : ev) (void)hipEventDestroy(e);
   }
 };
 
@@ -429,8 +467,8 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
       BJ_HIP(hipGetLastError());
     }
     ev.record(4, st);
-    refit_kernel<<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent, h->flags,
-                                                       h->tris);
+    refit_kernel<<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent,
+                                                       h->leaf_counts, h->flags, h->tris);
     BJ_HIP(hipGetLastError());
   }
   ev.record(5, st);

@@ -29,7 +29,8 @@ namespace {
 using namespace bj;
 
 constexpr int kRtBlock = kTraceBlock;
-constexpr uint32_t kMatLambertian = 0, kMatMetal = 1, kMatDielectric = 2, kMatEmissive = 3;
+// MAT kinds (rt/types.mojo:24-29); metal (1) and dielectric (2) are not rendered here
+constexpr uint32_t kMatLambertian = 0, kMatEmissive = 3;
 constexpr uint32_t kSurfaceIndexMask = (1u << 28) - 1;
 constexpr uint32_t kPrimTriangle = 1;
 constexpr uint32_t kPathDelta = 0x80000000u, kPathIdMask = 0x7FFFFFFFu;
@@ -45,7 +46,11 @@ constexpr int kMaxBounces = 64;
 // totals (extension node visits, extension leaf tests, shadow node visits,
 // shadow leaf tests)
 constexpr int kWorkBase = 2 * kMaxBounces;
-constexpr int kStatsWords = kWorkBase + 4;
+// then the largest node-visit count of one extension ray and of one shadow
+// ray, per bounce (kMaxBounces each)
+constexpr int kMaxVisitBase = kWorkBase + 4;
+constexpr int kWorkStride = 3;  // per-block partials: node visits, leaf tests, max visits of one ray
+constexpr int kStatsWords = kMaxVisitBase + 2 * kMaxBounces;
 
 struct SceneView {
   AccelView bvh;
@@ -150,18 +155,26 @@ __global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsign
                                                       uint32_t cap, const unsigned long long* ext_work,
                                                       uint32_t ext_blocks, const unsigned long long* sh_work,
                                                       uint32_t sh_blocks) {
-  __shared__ unsigned long long s_sum[4][256];
+  __shared__ unsigned long long s_sum[6][256];
   const uint32_t t = threadIdx.x;
   if (counters[kTailDone]) return;  // the tail kernel traced and counted the rest of the chunk
-  unsigned long long a = 0, b = 0, c = 0, d = 0;
+  unsigned long long a = 0, b = 0, c = 0, d = 0, ma = 0, mc = 0;
   // blocks past the live entries exit without writing their partials
   const uint32_t active = counters[kActive] < cap ? counters[kActive] : cap;
   const uint32_t shadows = counters[kShadow] < cap ? counters[kShadow] : cap;
   const uint32_t ext_live = (active + 255u) / 256u < ext_blocks ? (active + 255u) / 256u : ext_blocks;
   const uint32_t sh_live = (shadows + 255u) / 256u < sh_blocks ? (shadows + 255u) / 256u : sh_blocks;
-  for (uint32_t i = t; i < ext_live; i += blockDim.x) { a += ext_work[2 * i]; b += ext_work[2 * i + 1]; }
-  for (uint32_t i = t; i < sh_live; i += blockDim.x) { c += sh_work[2 * i]; d += sh_work[2 * i + 1]; }
-  s_sum[0][t] = a; s_sum[1][t] = b; s_sum[2][t] = c; s_sum[3][t] = d;
+  for (uint32_t i = t; i < ext_live; i += blockDim.x) {
+    a += ext_work[kWorkStride * i];
+    b += ext_work[kWorkStride * i + 1];
+    ma = ext_work[kWorkStride * i + 2] > ma ? ext_work[kWorkStride * i + 2] : ma;
+  }
+  for (uint32_t i = t; i < sh_live; i += blockDim.x) {
+    c += sh_work[kWorkStride * i];
+    d += sh_work[kWorkStride * i + 1];
+    mc = sh_work[kWorkStride * i + 2] > mc ? sh_work[kWorkStride * i + 2] : mc;
+  }
+  s_sum[0][t] = a; s_sum[1][t] = b; s_sum[2][t] = c; s_sum[3][t] = d; s_sum[4][t] = ma; s_sum[5][t] = mc;
   __syncthreads();
   if (t != 0) return;
   for (int k = 0; k < 4; ++k) {
@@ -170,6 +183,11 @@ __global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsign
     stats[kWorkBase + k] += x;
   }
   if (bounce < kMaxBounces) {
+    for (int k = 0; k < 2; ++k) {
+      unsigned long long x = stats[kMaxVisitBase + 2 * bounce + k];
+      for (uint32_t i = 0; i < blockDim.x; ++i) x = s_sum[4 + k][i] > x ? s_sum[4 + k][i] : x;
+      stats[kMaxVisitBase + 2 * bounce + k] = x;
+    }
     stats[2 * bounce + 0] += counters[kActive];
     stats[2 * bounce + 1] += counters[kShadow];
   }
@@ -195,27 +213,32 @@ struct ShadowWork {
 // Block-reduces per-lane traversal work and stores the block's two totals
 // (no atomics: the advance kernel sums the per-block partials).  Every
 // thread of the block must call it.
-BJ_D void store_block_work(unsigned long long* partials, uint64_t a, uint64_t b) {
-  __shared__ unsigned long long s_work[kRtBlock / 64][2];
+BJ_D void store_block_work(unsigned long long* partials, uint64_t a, uint64_t b, uint32_t m) {
+  __shared__ unsigned long long s_work[kRtBlock / 64][3];
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     a += __shfl_xor(a, off, 64);
     b += __shfl_xor(b, off, 64);
+    const uint32_t o = (uint32_t)__shfl_xor((int)m, off, 64);
+    m = o > m ? o : m;
   }
   const uint32_t wave = threadIdx.x >> 6;
   if (lane_id() == 0) {
     s_work[wave][0] = a;
     s_work[wave][1] = b;
+    s_work[wave][2] = m;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long x = 0, y = 0;
+    unsigned long long x = 0, y = 0, z = 0;
     for (int w = 0; w < kRtBlock / 64; ++w) {
       x += s_work[w][0];
       y += s_work[w][1];
+      z = s_work[w][2] > z ? s_work[w][2] : z;
     }
-    partials[2 * blockIdx.x + 0] = x;
-    partials[2 * blockIdx.x + 1] = y;
+    partials[kWorkStride * blockIdx.x + 0] = x;
+    partials[kWorkStride * blockIdx.x + 1] = y;
+    partials[kWorkStride * blockIdx.x + 2] = z;
   }
 }
 
@@ -296,6 +319,24 @@ BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uin
 // kFirst: bounce 0 of a chunk, whose paths are the camera paths of the
 // chunk's path ids (generated here instead of read from a queue); it also
 // clears the chunk's sample accumulators.
+// Wave-aggregated maximum of a per-lane count.
+BJ_D void wave_max(unsigned long long* dst, uint32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
+    v = o > v ? o : v;
+  }
+  if (lane_id() == 0 && v) atomicMax(dst, (unsigned long long)v);
+}
+
+// Wave-aggregated 64-bit add of a per-lane count.
+BJ_D void wave_add(unsigned long long* dst, uint32_t v) {
+  uint64_t x = v;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+  if (lane_id() == 0 && x) atomicAdd(dst, (unsigned long long)x);
+}
+
 // Surface facts of an extension hit (rt/geometry.mojo:20-30: the outward
 // normal is oriented against the ray).
 struct Surface {
@@ -452,9 +493,9 @@ BJ_D v3 shadow_radiance(bool occluded, const float* f) {
 template <int kAlg, bool kWide, bool kFirst>
 __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                           PrimaryArgs pa, uint32_t* __restrict__ spill,
-                                                          unsigned long long* __restrict__ work) {
+                                                          unsigned long long* __restrict__ work,
+                                                          unsigned long long* __restrict__ stats) {
   __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
-  const uint32_t lane = lane_id();
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
   const uint64_t cap = q.capacity;
@@ -541,15 +582,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
       }
     }
   }
-  store_block_work(work, work_visits, work_leaves);
-}
-
-// Wave-aggregated 64-bit add of a per-lane count.
-BJ_D void wave_add(unsigned long long* dst, uint32_t v) {
-  uint64_t x = v;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-  if (lane_id() == 0 && x) atomicAdd(dst, (unsigned long long)x);
+  store_block_work(work, work_visits, work_leaves, (uint32_t)work_visits);
 }
 
 // The remaining bounces of a chunk once few paths are left.  Each thread
@@ -599,12 +632,14 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
       const uint32_t b = bounce + k;
       if (b < kMaxBounces) wave_add(stats + 2 * b, alive ? 1u : 0u);
       bool shadow = false;
+      uint32_t ext_max = 0, sh_max = 0;
       if (alive) {
         const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
         TraceCounters tc;
         const Hit h = trace_accel<BAJO_TRACE_CLOSEST_HIT, true, kWide>(sc.bvh, ray, st, &tc);
         ext_visits += tc.visits;
         ext_leaves += tc.leaves;
+        ext_max = tc.visits;
         const v3 T = mk(f[8], f[9], f[10]);
         const Surface su = surface_of(sc, ray, h);
         ShadowWork sw;
@@ -617,6 +652,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
           const Hit sh = trace_accel<BAJO_TRACE_ANY_HIT, true, kWide>(sc.bvh, sray, st, &tcs);
           sh_visits += tcs.visits;
           sh_leaves += tcs.leaves;
+          sh_max = tcs.visits;
           const bool occluded = sh.prim == kEmpty && sh.t == 0.0f;
           if (kAlg == BAJO_RENDER_AO || !occluded) {
             cs = shadow_radiance<kAlg>(occluded, sw.f);
@@ -645,7 +681,11 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
           delta = false;
         }
       }
-      if (b < kMaxBounces) wave_add(stats + 2 * b + 1, shadow ? 1u : 0u);
+      if (b < kMaxBounces) {
+        wave_add(stats + 2 * b + 1, shadow ? 1u : 0u);
+        wave_max(stats + kMaxVisitBase + 2 * b, ext_max);
+        wave_max(stats + kMaxVisitBase + 2 * b + 1, sh_max);
+      }
       if (!ballot(alive)) break;
     }
     if (idx < active) {
@@ -660,10 +700,11 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
 }
 
 template <int kAlg, bool kWide>
-__global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t* __restrict__ spill,
-                                                          unsigned long long* __restrict__ work) {
+__global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t bounce,
+                                                          uint32_t* __restrict__ spill,
+                                                          unsigned long long* __restrict__ work,
+                                                          unsigned long long* __restrict__ stats) {
   __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
-  const uint32_t lane = lane_id();
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
   const uint64_t cap = q.capacity;
@@ -685,7 +726,7 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
     const bool occluded = h.prim == kEmpty && h.t == 0.0f;
     if (kAlg == BAJO_RENDER_AO || !occluded) accumulate(q.sample, slot, shadow_radiance<kAlg>(occluded, f));
   }
-  store_block_work(work, work_visits, work_leaves);
+  store_block_work(work, work_visits, work_leaves, (uint32_t)work_visits);
 }
 
 __global__ void __launch_bounds__(kRtBlock) resolve_kernel(const float* __restrict__ sample, float* __restrict__ pixels,
@@ -741,11 +782,11 @@ struct bajo_scene {
 namespace {
 
 void free_arena(bajo_scene* s) {
-  hipFree(s->ids[0]); hipFree(s->ids[1]); hipFree(s->fields[0]); hipFree(s->fields[1]);
-  hipFree(s->shadow_ids); hipFree(s->shadow_fields); hipFree(s->sample); hipFree(s->counters); hipFree(s->stats);
-  hipFree(s->spill);
-  hipFree(s->work_ext);
-  hipFree(s->work_sh);
+  (void)hipFree(s->ids[0]); (void)hipFree(s->ids[1]); (void)hipFree(s->fields[0]); (void)hipFree(s->fields[1]);
+  (void)hipFree(s->shadow_ids); (void)hipFree(s->shadow_fields); (void)hipFree(s->sample); (void)hipFree(s->counters); (void)hipFree(s->stats);
+  (void)hipFree(s->spill);
+  (void)hipFree(s->work_ext);
+  (void)hipFree(s->work_sh);
   s->work_ext = s->work_sh = nullptr;
   s->ids[0] = s->ids[1] = nullptr;
   s->fields[0] = s->fields[1] = nullptr;
@@ -759,8 +800,8 @@ void free_scene(bajo_scene* s) {
   if (!s) return;
   free_arena(s);
   bajo_bvh_free(s->bvh);
-  hipFree(s->vertices); hipFree(s->surfaces); hipFree(s->lambertian); hipFree(s->emissive);
-  hipFree(s->light_fields); hipFree(s->light_kinds);
+  (void)hipFree(s->vertices); (void)hipFree(s->surfaces); (void)hipFree(s->lambertian); (void)hipFree(s->emissive);
+  (void)hipFree(s->light_fields); (void)hipFree(s->light_kinds);
   delete s;
 }
 
@@ -788,8 +829,8 @@ int32_t ensure_arena(bajo_scene* s, uint64_t cap, uint64_t spill_threads) {
   BJ_HIP(hipMalloc(&s->counters, kCounterWords * sizeof(uint32_t)));
   BJ_HIP(hipMalloc(&s->stats, kStatsWords * sizeof(unsigned long long)));
   BJ_HIP(hipMalloc(&s->spill, spill_threads * kSpill * sizeof(uint32_t)));
-  BJ_HIP(hipMalloc(&s->work_ext, 2 * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
-  BJ_HIP(hipMalloc(&s->work_sh, 2 * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
+  BJ_HIP(hipMalloc(&s->work_ext, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
+  BJ_HIP(hipMalloc(&s->work_sh, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
   s->cap = cap;
   s->spill_threads = spill_threads;
   return BAJO_OK;
@@ -829,14 +870,14 @@ struct LaunchEvents {
   bool on = false;
   hipEvent_t make() {
     hipEvent_t e = nullptr;
-    hipEventCreate(&e);
+    (void)hipEventCreate(&e);
     return e;
   }
   ~LaunchEvents() {
-    for (auto e : ext) hipEventDestroy(e);
-    for (auto e : sh) hipEventDestroy(e);
-    if (begin) hipEventDestroy(begin);
-    if (end) hipEventDestroy(end);
+    for (auto e : ext) (void)hipEventDestroy(e);
+    for (auto e : sh) (void)hipEventDestroy(e);
+    if (begin) (void)hipEventDestroy(begin);
+    if (end) (void)hipEventDestroy(end);
   }
 };
 
@@ -887,7 +928,7 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
     for (uint32_t b = 0; b < depth; ++b) {
       Queues q{s->ids[b & 1], s->fields[b & 1], s->ids[(b + 1) & 1], s->fields[(b + 1) & 1], s->shadow_ids,
                s->shadow_fields, s->sample, s->counters, cap, (uint32_t)begin};
-      if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
+      if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if constexpr (kTail) {
         if (b > 0) {
           tail_kernel<kAlg, kWide><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, tail_threshold,
@@ -896,18 +937,18 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
         }
       }
       if (b == 0)
-        extend_kernel<kAlg, kWide, true><<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, s->spill,
-                                                                         s->work_ext);
+        extend_kernel<kAlg, kWide, true><<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(
+            sv, q, st->rng_seed, b, pa, s->spill, s->work_ext, s->stats);
       else
         extend_kernel<kAlg, kWide, false><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, s->spill,
-                                                                          s->work_ext);
+                                                                          s->work_ext, s->stats);
       BJ_HIP(hipGetLastError());
-      if (ev) { ev->ext.push_back(ev->make()); hipEventRecord(ev->ext.back(), stream); }
+      if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
-        if (ev) { ev->sh.push_back(ev->make()); hipEventRecord(ev->sh.back(), stream); }
-        shadow_kernel<kAlg, kWide><<<g_sh, kRtBlock, 0, stream>>>(sv, q, s->spill, s->work_sh);
+        if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
+        shadow_kernel<kAlg, kWide><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, s->spill, s->work_sh, s->stats);
         BJ_HIP(hipGetLastError());
-        if (ev) { ev->sh.push_back(ev->make()); hipEventRecord(ev->sh.back(), stream); }
+        if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
       }
       const bool traced_shadows = kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS;
       advance_kernel<<<1, 256, 0, stream>>>(s->counters, s->stats, b, (uint32_t)cap, s->work_ext, g_ext, s->work_sh,
@@ -1100,6 +1141,10 @@ int32_t bajo_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st
     stats->extension_leaf_tests = host[kWorkBase + 1];
     stats->shadow_node_visits = host[kWorkBase + 2];
     stats->shadow_leaf_tests = host[kWorkBase + 3];
+    for (int b = 0; b < 16; ++b) {
+      stats->max_extension_visits[b] = (uint32_t)host[kMaxVisitBase + 2 * b];
+      stats->max_shadow_visits[b] = (uint32_t)host[kMaxVisitBase + 2 * b + 1];
+    }
     for (int b = 0; b < kMaxBounces; ++b) {
       stats->extension_rays += host[2 * b];
       stats->shadow_rays += host[2 * b + 1];

@@ -182,6 +182,8 @@ def enqueue_render_gpu(
         out["rays_per_bounce"] = list(stats.rays_per_bounce)
         out["extend_bounce_ms"] = list(stats.extend_bounce_ms)
         out["shadow_bounce_ms"] = list(stats.shadow_bounce_ms)
+        out["max_extension_visits"] = list(stats.max_extension_visits)
+        out["max_shadow_visits"] = list(stats.max_shadow_visits)
     return RenderResult(pixels, out)
 
 

@@ -10,10 +10,11 @@ the wavefront path tracer at 1920 x 1080, 8 spp, max depth 5, run with NEE
 and with PATH.  Scene data is resident in HBM before timing starts.
 
 Prints one JSON line: `value` = rays traced per second over the whole step
-(build included), with the incoherent-ray rate (bounce >= 1 extension and
-shadow rays over the render kernels), the build Mtris/s, the roofline of the
-dominant kernel (the NEE extend kernel, timed live with HIP events) and the
-CPU oracle's rate on a bounded sample of the same render.
+(build included), with the incoherent-ray rate of the renders (bounce >= 1
+extension and shadow rays over their launches), a separate closest-hit launch
+of 16M incoherent random rays on the same scene, the build Mtris/s, the
+roofline of the dominant kernel (the NEE extend kernel, timed live with HIP
+events) and the CPU oracle's rate on the same render.
 """
 
 from __future__ import annotations
@@ -69,6 +70,61 @@ def measured_traffic() -> int | None:
     except OSError:
         return None
     return None
+
+
+def incoherent_trace(scene, ctx, node_width: int, leaf_size: int, method: int, rays: int = 1 << 24) -> dict:
+    """Closest-hit launch of `rays` incoherent rays (origins uniform in the scene
+    box, directions uniform on the sphere) over the same 4M-triangle scene and
+    BVH layout, timed with HIP events on the launch stream."""
+    import torch
+
+    from bajo_amd.bvh import build_triangle_bvh
+
+    dev = ctx.torch_device
+    v = torch.from_numpy(np.ascontiguousarray(scene.vertices, np.float32)).to(dev)
+    bvh = build_triangle_bvh(v, ctx=ctx, node_width=node_width, leaf_size=leaf_size, method=method)
+    lo = v.view(-1, 3).min(0).values
+    hi = v.view(-1, 3).max(0).values
+    g = torch.Generator(device=dev)
+    g.manual_seed(2027)
+    o = lo + (hi - lo) * torch.rand((rays, 3), generator=g, device=dev)
+    d = torch.randn((rays, 3), generator=g, device=dev)
+    d = d / d.norm(dim=1, keepdim=True)
+    r = torch.empty((8, rays), dtype=torch.float32, device=dev)
+    r[0:3] = o.T
+    r[3] = 0.0
+    r[4:7] = d.T
+    r[7] = float(np.finfo(np.float32).max)
+    r = r.reshape(-1).contiguous()
+    hits = torch.empty(8 * rays, dtype=torch.float32, device=dev)
+    st = torch.empty(5 * rays, dtype=torch.int32, device=dev)
+    bvh.launch_rays(r, hits, rays, stats=st)  # instrumented pass: node visits and triangle tests
+    s = st.view(-1, 5).to(torch.float64)
+    visits, tests = float(s[:, 0].mean()), float(s[:, 3].mean())
+    stream = torch.cuda.current_stream(dev)
+    bvh.launch_rays(r, hits, rays)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    reps = 3
+    ev[0].record(stream)
+    for _ in range(reps):
+        bvh.launch_rays(r, hits, rays)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    hit_frac = float((hits.view(-1, 8)[:, 2].view(torch.int32) != -1).float().mean())
+    bvh.free()
+    node_bytes = NODE_BYTES[node_width]
+    algo = rays * (64 + visits * node_bytes + tests * TRI_BYTES)
+    return {
+        "rays": rays,
+        "ms": ms,
+        "mrays_s": rays / (ms / 1e3) / 1e6,
+        "node_visits_per_ray": visits,
+        "tri_tests_per_ray": tests,
+        "hit_fraction": hit_frac,
+        "algorithmic_gbs": algo / (ms / 1e3) / 1e9,
+        "rays_def": "origins uniform in the scene box, directions uniform on the sphere, closest hit",
+    }
 
 
 def cpu_baseline(scene, cam, threads: int) -> dict:
@@ -172,7 +228,11 @@ def main() -> int:
     nee, path = last[RENDER.NEE], last[RENDER.PATH]
     rays_per_step = sum(s["extension_rays"] + s["shadow_rays"] for s in last.values())
     incoherent = sum(s["incoherent_rays"] for s in last.values())
-    kernel_ms = sum(float(np.mean([st[a]["extend_ms"] + st[a]["shadow_ms"] for st in stats])) for a in last)
+    # time of the bounce >= 1 launches (extend, its tail, shadow) of both renders
+    incoherent_ms = sum(
+        float(np.mean([sum(st[a]["extend_bounce_ms"][1:DEPTH]) + sum(st[a]["shadow_bounce_ms"][1:DEPTH]) for st in stats]))
+        for a in last
+    )
     ext_ms_nee = float(np.mean([st[RENDER.NEE]["extend_ms"] for st in stats]))
     ext_launches = nee["extend_launches"]
     ext_bytes = extend_bytes(nee, args.node_width)
@@ -203,7 +263,8 @@ def main() -> int:
             "paths_per_render": WIDTH * HEIGHT * SPP,
         },
         "rays_per_step": rays_per_step,
-        "incoherent_mrays_s": incoherent / (kernel_ms / 1e3) / 1e6,
+        "incoherent_mrays_s": incoherent / (incoherent_ms / 1e3) / 1e6,
+        "incoherent_note": "bounce >= 1 extension and shadow rays of both renders over their launch times",
         "render_mrays_s": rays_per_step / (render_ms / 1e3) / 1e6,
         "nee": {k: nee[k] for k in keep},
         "path": {k: path[k] for k in keep},
@@ -211,6 +272,8 @@ def main() -> int:
         "nee_extend_bounce_ms": [round(x, 4) for x in nee["extend_bounce_ms"][:DEPTH]],
         "nee_shadow_bounce_ms": [round(x, 4) for x in nee["shadow_bounce_ms"][:DEPTH]],
         "path_extend_bounce_ms": [round(x, 4) for x in path["extend_bounce_ms"][:DEPTH]],
+        "nee_max_extension_visits": nee["max_extension_visits"][:DEPTH],
+        "nee_max_shadow_visits": nee["max_shadow_visits"][:DEPTH],
         "nee_shadow_visits_per_ray": nee["shadow_node_visits"] / max(nee["shadow_rays"], 1),
         "nee_shadow_tests_per_ray": nee["shadow_leaf_tests"] / max(nee["shadow_rays"], 1),
         "nee_image_mean_rgb": img,
@@ -235,6 +298,7 @@ def main() -> int:
             "leaf_tests_per_ray": nee["extension_leaf_tests"] / nee["extension_rays"],
         },
     }
+    line["incoherent_trace"] = incoherent_trace(scene, ctx, args.node_width, args.leaf_size, method)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, cam, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)

@@ -230,6 +230,8 @@ typedef struct {
   uint32_t extend_launches, shadow_launches;
   float extend_bounce_ms[16];   /* extend launch time per bounce (first chunk) */
   float shadow_bounce_ms[16];   /* shadow launch time per bounce (first chunk) */
+  uint32_t max_extension_visits[16]; /* most nodes one extension ray visited, per bounce */
+  uint32_t max_shadow_visits[16];    /* most nodes one shadow ray visited, per bounce    */
 } bajo_render_stats;
 
 /* SceneData for triangle scenes (rt/types.mojo:382-526): host arrays of

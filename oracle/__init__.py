@@ -81,8 +81,9 @@ _SIG = {
     "orc_hploc_binary": (
         c_uint32,
         [c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p,
-         c_void_p, c_void_p],
+         c_void_p, c_void_p, c_void_p],
     ),
+    "orc_hploc_lowest_leaf_numbering": (None, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
     "orc_render": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "orc_render_fp64": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
 }
@@ -494,11 +495,25 @@ def preorder_binary(root: int, node_meta: np.ndarray, leaf_ids: np.ndarray) -> n
     return out[:k]
 
 
-def hploc_binary(t: dict) -> dict:
-    """The topology in the binary layout with depth-first leaf positions
-    (the device H-PLOC builder's output form, up to inner-node numbering)."""
+def hploc_lowest_leaf_numbering(t: dict, sorted_ids: np.ndarray) -> np.ndarray:
+    """The device builder's inner-node indices for topology `t`: a cluster is
+    named by its lowest sorted leaf position, a merge retires the larger name
+    and the new node takes index (retired - 1)."""
+    n = (t["left"].size + 1) // 2
+    out = np.empty(2 * n - 1, np.uint32)
+    ids = np.ascontiguousarray(sorted_ids, np.uint32)
+    lib().orc_hploc_lowest_leaf_numbering(_p(t["left"]), _p(t["right"]), _p(ids), n, _p(out))
+    return out
+
+
+def hploc_binary(t: dict, numbering: np.ndarray | None = None) -> dict:
+    """The topology in the binary layout with depth-first leaf positions (the
+    device H-PLOC builder's output form).  Inner nodes are numbered by
+    `numbering` (from hploc_lowest_leaf_numbering: then the layout equals the
+    device's word for word) or, when None, in creation order."""
     n = (t["left"].size + 1) // 2
     m = max(n - 1, 1)
+    num = None if numbering is None else np.ascontiguousarray(numbering, np.uint32)
     out = {
         "node_meta": np.zeros(4 * m, np.uint32),
         "node_bounds": np.zeros(12 * m, np.float32),
@@ -508,7 +523,7 @@ def hploc_binary(t: dict) -> dict:
     }
     root = lib().orc_hploc_binary(
         t["root"], _p(t["left"]), _p(t["right"]), _p(t["leaf_id"]), _p(t["leaf_count"]), _p(t["bounds"]), n,
-        _p(out["node_meta"]), _p(out["node_bounds"]), _p(out["leaf_ids"]), _p(out["leaf_counts"]),
+        _p(num) if num is not None else None, _p(out["node_meta"]), _p(out["node_bounds"]), _p(out["leaf_ids"]), _p(out["leaf_counts"]),
         _p(out["leaf_parent"]),
     )
     out["root"] = int(root)
@@ -526,7 +541,7 @@ def build_hploc(vertices: np.ndarray) -> dict:
     lb = build_lbvh(v)
     codes = (lb["sorted_keys"] >> np.uint64(32)).astype(np.uint32)
     t = hploc_topology(triangle_bounds(v), codes, lb["leaf_ids"])
-    b = hploc_binary(t)
+    b = hploc_binary(t, hploc_lowest_leaf_numbering(t, lb["leaf_ids"]))
     b["scene_bounds"] = lb["scene_bounds"]
     b["topology"] = t
     return b

@@ -76,8 +76,11 @@ uint64_t orc_preorder_topology(uint32_t root, const uint32_t *left, const uint32
 uint64_t orc_preorder_binary(uint32_t root, const uint32_t *node_meta, const uint32_t *leaf_ids, uint64_t n,
                              uint32_t *out);
 uint32_t orc_hploc_binary(uint32_t root, const uint32_t *left, const uint32_t *right, const uint32_t *leaf_id,
-                          const uint32_t *leaf_count, const float *bounds6, uint64_t n, uint32_t *node_meta,
-                          float *node_bounds, uint32_t *leaf_ids, uint32_t *leaf_counts, uint32_t *leaf_parent);
+                          const uint32_t *leaf_count, const float *bounds6, uint64_t n, const uint32_t *numbering,
+                          uint32_t *node_meta, float *node_bounds, uint32_t *leaf_ids, uint32_t *leaf_counts,
+                          uint32_t *leaf_parent);
+void orc_hploc_lowest_leaf_numbering(const uint32_t *left, const uint32_t *right, const uint32_t *sorted_ids,
+                                     uint64_t n, uint32_t *out);
 
 /* wavefront renderer (orc_render.c) */
 typedef struct {

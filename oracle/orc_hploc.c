@@ -227,7 +227,7 @@ uint64_t orc_preorder_binary(uint32_t root, const uint32_t *node_meta, const uin
  * position), node_bounds = left then right child box, leaf_ids by position,
  * leaf_counts per node.  Returns the binary root (tagged when n == 1). */
 typedef struct {
-    const uint32_t *left, *right, *leaf_id, *leaf_count;
+    const uint32_t *left, *right, *leaf_id, *leaf_count, *numbering;
     const float *box;
     uint64_t n;
     uint32_t *meta, *leaf_ids, *counts, *leaf_parent;
@@ -243,7 +243,7 @@ static uint32_t hb_ref(hb_state *s, uint32_t t, uint32_t parent, uint32_t *last_
         *last_pos = pos;
         return ORC_LEAF_FLAG | pos;
     }
-    uint32_t b = t - (uint32_t)s->n;
+    uint32_t b = s->numbering ? s->numbering[t] : t - (uint32_t)s->n;
     uint32_t *m = s->meta + 4 * (uint64_t)b;
     m[0] = parent;
     uint32_t lp = 0, rp = 0;
@@ -258,10 +258,31 @@ static uint32_t hb_ref(hb_state *s, uint32_t t, uint32_t parent, uint32_t *last_
 }
 
 uint32_t orc_hploc_binary(uint32_t root, const uint32_t *left, const uint32_t *right, const uint32_t *leaf_id,
-                          const uint32_t *leaf_count, const float *bounds6, uint64_t n, uint32_t *node_meta,
-                          float *node_bounds, uint32_t *leaf_ids, uint32_t *leaf_counts, uint32_t *leaf_parent) {
-    hb_state s = {left, right, leaf_id, leaf_count, bounds6, n, node_meta, leaf_ids, leaf_counts, leaf_parent,
-                  node_bounds, 0};
+                          const uint32_t *leaf_count, const float *bounds6, uint64_t n, const uint32_t *numbering,
+                          uint32_t *node_meta, float *node_bounds, uint32_t *leaf_ids, uint32_t *leaf_counts,
+                          uint32_t *leaf_parent) {
+    hb_state s = {left, right, leaf_id, leaf_count, numbering, bounds6, n, node_meta, leaf_ids, leaf_counts,
+                  leaf_parent, node_bounds, 0};
     uint32_t last = 0;
     return hb_ref(&s, root, ORC_SENTINEL, &last);
+}
+
+/* The device builder's inner-node numbering (a counter-free scheme with no
+ * counterpart in bajo): every cluster is named by the lowest sorted position
+ * among its leaves; a merge retires the larger of its two names and the new
+ * node takes index (retired - 1).  Position 0 is never retired, so the n-1
+ * inner nodes receive 0..n-2.  Inner topology nodes are created after their
+ * children (indices n.. in creation order), so one ascending pass suffices.
+ * sorted_ids: leaf id at each sorted position.  out: 2n-1 entries. */
+void orc_hploc_lowest_leaf_numbering(const uint32_t *left, const uint32_t *right, const uint32_t *sorted_ids,
+                                     uint64_t n, uint32_t *out) {
+    uint32_t *low = (uint32_t *)malloc((2 * n - 1) * sizeof(uint32_t));
+    for (uint64_t p = 0; p < n; ++p) low[sorted_ids[p]] = (uint32_t)p;
+    for (uint64_t t = 0; t < n; ++t) out[t] = ORC_SENTINEL;
+    for (uint64_t t = n; t < 2 * n - 1; ++t) {
+        uint32_t a = low[left[t]], b = low[right[t]];
+        low[t] = a < b ? a : b;
+        out[t] = (a > b ? a : b) - 1u;
+    }
+    free(low);
 }

@@ -404,10 +404,14 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
         for (uint64_t i = 0; i < n; ++i) codes[i] = (uint32_t)(keys[i] >> 32);
         uint32_t troot = orc_hploc_topology(tb, codes, ids, n, 8, 16, tp, tp + m2, tp + 2 * m2, tp + 3 * m2,
                                             tp + 4 * m2, tbox, NULL);
-        if (troot != ORC_SENTINEL)
-            root = orc_hploc_binary(troot, tp + m2, tp + 2 * m2, tp + 3 * m2, tp + 4 * m2, tbox, n, meta, nb, ids,
-                                    counts, parent);
-        free(tb); free(codes); free(tp); free(tbox);
+        uint32_t *num = (uint32_t *)malloc(m2 * sizeof(uint32_t));
+        if (troot != ORC_SENTINEL) {
+            /* the device builder's inner-node numbering (from the sorted ids, before they are reordered) */
+            orc_hploc_lowest_leaf_numbering(tp + m2, tp + 2 * m2, ids, n, num);
+            root = orc_hploc_binary(troot, tp + m2, tp + 2 * m2, tp + 3 * m2, tp + 4 * m2, tbox, n, num, meta, nb,
+                                    ids, counts, parent);
+        }
+        free(tb); free(codes); free(tp); free(tbox); free(num);
         if (troot == ORC_SENTINEL) { free(keys); free(ids); free(meta); free(parent); free(counts); free(nb); return -2; }
     }
     scene_t sc;

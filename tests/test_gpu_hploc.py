@@ -2,9 +2,11 @@
 topology (oracle/orc_hploc.c), the protocol of test/bvh/gpu/
 test_hploc_binary.mojo: the device topology must equal the reference
 topology exactly (child order included; compared through a preorder
-serialization of leaf ids, since inner-node numbering follows the order in
-which waves reserve nodes), node boxes must be the unions of their leaves,
-and quality must match.  Traversal over the device hierarchy is checked
+serialization of leaf ids), node boxes must be the unions of their leaves,
+and quality must match.  Beyond bajo's protocol the whole binary layout is
+compared word for word: inner nodes are numbered by the counter-free
+lowest-leaf rule, which the oracle restates, so the device output is
+deterministic and equal to the oracle's.  Traversal over the device hierarchy is checked
 bit-for-bit against the oracle traversing the same (exported) hierarchy, and
 renders against the oracle renderer on its own H-PLOC hierarchy.
 """
@@ -114,6 +116,18 @@ def test_hploc_topology_equals_reference(gpu_ctx, name, n):
     w = oracle.preorder_topology(want["topology"])
     assert a.size == 2 * n - 1
     np.testing.assert_array_equal(a, w)
+    if n >= 2:
+        assert got["root"] == want["root"]
+        for key in ("node_meta", "leaf_ids", "leaf_counts", "leaf_parent"):
+            np.testing.assert_array_equal(got[key], want[key], err_msg=key)
+        np.testing.assert_array_equal(got["node_bounds"].view(np.uint32), want["node_bounds"].view(np.uint32))
+        # keys follow their leaves to depth-first order
+        lb_keys = oracle.build_lbvh(v)["sorted_keys"] if n <= 100_000 else None
+        if lb_keys is not None:
+            code_of = np.empty(n, np.uint64)
+            code_of[(lb_keys & np.uint64(0xFFFFFFFF)).astype(np.int64)] = lb_keys >> np.uint64(32)
+            ids = want["leaf_ids"].astype(np.uint64)
+            np.testing.assert_array_equal(got["sorted_keys"], (code_of[ids.astype(np.int64)] << np.uint64(32)) | ids)
     print(name, {k: val / 1e6 for k, val in t.items()})
     bvh.free()
 

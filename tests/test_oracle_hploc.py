@@ -174,3 +174,45 @@ def test_preorder_of_lbvh_binary_layout():
     a = oracle.preorder_topology(t)
     b = oracle.preorder_binary(lb["root"], lb["node_meta"], lb["leaf_ids"])
     np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,seed", [(2, 11), (3, 12), (40, 13), (777, 14), (5000, 15)])
+def test_lowest_leaf_numbering(n, seed):
+    """The device builder's counter-free inner-node numbering, restated by the
+    oracle: a bijection onto 0..n-2, equal (after relabeling) to the
+    creation-order layout, and equal to a replay of the merges in creation
+    order where each cluster carries the set of its sorted positions."""
+    v = soup(n, seed=seed)
+    lb = oracle.build_lbvh(v)
+    codes = (lb["sorted_keys"] >> np.uint64(32)).astype(np.uint32)
+    t = oracle.hploc_topology(oracle.triangle_bounds(v), codes, lb["leaf_ids"])
+    num = oracle.hploc_lowest_leaf_numbering(t, lb["leaf_ids"])
+    inner = num[n:]
+    assert sorted(inner.tolist()) == list(range(n - 1))
+    # replay: positions held by each cluster, merged in creation order
+    pos_of_leaf = np.empty(n, np.int64)
+    pos_of_leaf[lb["leaf_ids"]] = np.arange(n)
+    held = {x: {int(pos_of_leaf[x])} for x in range(n)}
+    for x in range(n, 2 * n - 1):
+        a, b = held.pop(int(t["left"][x])), held.pop(int(t["right"][x]))
+        assert num[x] == max(min(a), min(b)) - 1
+        held[x] = a | b
+    assert list(held) == [t["root"]] and min(held[t["root"]]) == 0
+    # the same tree under both numberings
+    by_creation = oracle.hploc_binary(t)
+    by_position = oracle.hploc_binary(t, num)
+    np.testing.assert_array_equal(by_creation["leaf_ids"], by_position["leaf_ids"])
+    np.testing.assert_array_equal(
+        oracle.preorder_binary(by_creation["root"], by_creation["node_meta"], by_creation["leaf_ids"]),
+        oracle.preorder_binary(by_position["root"], by_position["node_meta"], by_position["leaf_ids"]),
+    )
+    relabel = np.empty(max(n - 1, 1), np.int64)
+    relabel[np.arange(n - 1)] = inner[np.arange(n - 1)]  # creation index i -> position index
+    m1 = by_creation["node_meta"].reshape(-1, 4)
+    m2 = by_position["node_meta"].reshape(-1, 4)
+    np.testing.assert_array_equal(m2[relabel], np.where(
+        (m1 & oracle.LEAF_FLAG) | (m1 == oracle.SENTINEL) | (np.arange(4) == 3), m1,
+        relabel[np.minimum(m1, n - 2)]).astype(np.uint32))
+    np.testing.assert_array_equal(by_position["node_bounds"].reshape(-1, 12)[relabel],
+                                  by_creation["node_bounds"].reshape(-1, 12))
+    np.testing.assert_array_equal(by_position["leaf_parent"], relabel[by_creation["leaf_parent"]])

@@ -399,22 +399,25 @@ BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint
 // weights, normals) and the next segment (Lambertian bounce with Russian
 // roulette), rt/gpu/path_shading.mojo:463-648.  Shared by the wavefront
 // extend kernel and the tail kernel so both follow one definition.
+// shade_bounce returns kShadeAdd when `add_c` holds radiance for the sample
+// and kShadeNext when `np` holds the next path segment (flags in one value:
+// bools returned through references were kept in scratch memory).
+constexpr uint32_t kShadeAdd = 1u, kShadeNext = 2u;
 template <int kAlg>
-BJ_D void shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, uint32_t path_id, bool delta,
-                       const Ray& ray, v3 T, float path_pdf, float t_hit, const Surface& su, bool& has_add, v3& add_c,
-                       bool& want_next, NextPath& np) {
-  has_add = false;
-  want_next = false;
+BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, uint32_t path_id, bool delta,
+                           const Ray& ray, v3 T, float path_pdf, float t_hit, const Surface& su, v3& add_c,
+                           NextPath& np) {
+  uint32_t out = 0;
   if (!su.found) {
     if constexpr (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
       add_c = mul(T, sky_color(ray.d));
-      has_add = true;
+      out = kShadeAdd;
     }
-    return;
+    return out;
   }
   if constexpr (kAlg == BAJO_RENDER_NORMALS) {
     add_c = scale(add(su.n, mk(1.0f, 1.0f, 1.0f)), 0.5f);
-    has_add = true;
+    out |= kShadeAdd;
   } else if constexpr (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
     if (su.kind == kMatEmissive) {
       if (su.front) {
@@ -434,7 +437,7 @@ BJ_D void shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, uint
           }
         }
         add_c = scale(mul(T, E), w);
-        has_add = true;
+        out |= kShadeAdd;
       }
     } else if (su.kind == kMatLambertian) {
       // _sample_lambertian (rt/shading.mojo:92-118)
@@ -469,11 +472,12 @@ BJ_D void shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, uint
                                        ray.o.z + t_hit * ray.d.z, kT_MinSecondary, dir.x, dir.y, dir.z, kFltMax,
                                        Tn.x, Tn.y, Tn.z, pdf};
         for (int p = 0; p < kPathPlanes; ++p) np.f[p] = nf[p];
-        want_next = true;
+        out |= kShadeNext;
       }
     }
     // metal and dielectric surfaces are not part of this renderer's scenes: absorbed
   }
+  return out;
 }
 
 // Radiance a traced shadow ray adds (rt/gpu/scene_trace.mojo:517-624).
@@ -566,10 +570,10 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
     bool want_next = false;
     NextPath np;
     if (live) {
-      bool has_add;
       v3 add_c;
-      shade_bounce<kAlg>(sc, seed, bounce, path_id, delta, ray, T, path_pdf, t_hit, su, has_add, add_c, want_next, np);
-      if (has_add) accumulate(q.sample, path_id - q.sample_base, add_c);
+      const uint32_t sf = shade_bounce<kAlg>(sc, seed, bounce, path_id, delta, ray, T, path_pdf, t_hit, su, add_c, np);
+      want_next = (sf & kShadeNext) != 0;
+      if (sf & kShadeAdd) accumulate(q.sample, path_id - q.sample_base, add_c);
     }
     const uint32_t ns = wave_reserve(q.counters + kNext, want_next);
     if (want_next) {
@@ -595,7 +599,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
 // shadow, advance and tail launches of the chunk do nothing.
 template <int kAlg, bool kWide>
 __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
-                                                        uint32_t depth, uint32_t threshold,
+                                                        uint32_t depth, uint32_t threshold, uint32_t simds,
                                                         unsigned long long* __restrict__ stats,
                                                         uint32_t* __restrict__ spill) {
   __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
@@ -610,9 +614,16 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
   Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
   uint64_t ext_visits = 0, ext_leaves = 0, sh_visits = 0, sh_leaves = 0;
   const uint32_t nb = depth - bounce;
-  for (uint32_t base = (uint32_t)(gtid & ~63ull); base < active; base += gridDim.x * blockDim.x) {
-    const uint32_t idx = base + lane;
-    bool alive = idx < active;
+  // A wave is as slow as its slowest path over all remaining bounces, so the
+  // paths are spread over about one wave per SIMD with only `group` lanes
+  // each (a power of two, at most 64) instead of packing 64 to a wave.
+  const uint32_t want = (active + simds - 1) / (simds ? simds : 1u);
+  uint32_t group = 1;
+  while (group < want && group < 64) group <<= 1;
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t gw = (uint32_t)(gtid >> 6); (uint64_t)gw * group < active; gw += waves) {
+    const uint32_t idx = gw * group + lane;
+    bool alive = lane < group && idx < active;
     uint32_t path_id = 0;
     bool delta = false;
     float f[kPathPlanes];
@@ -659,12 +670,12 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
             add_s = true;
           }
         }
-        bool has_add, want_next;
         v3 add_c;
         NextPath np;
-        shade_bounce<kAlg>(sc, seed, b, path_id, delta, ray, T, f[11], h.t, su, has_add, add_c, want_next, np);
+        const uint32_t sf = shade_bounce<kAlg>(sc, seed, b, path_id, delta, ray, T, f[11], h.t, su, add_c, np);
+        const bool want_next = (sf & kShadeNext) != 0;
         // the same order as the wavefront: the extension's radiance, then the shadow ray's
-        if (has_add) {
+        if (sf & kShadeAdd) {
           acc[0] += add_c.x;
           acc[1] += add_c.y;
           acc[2] += add_c.z;
@@ -688,7 +699,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
       }
       if (!ballot(alive)) break;
     }
-    if (idx < active) {
+    if (lane < group && idx < active) {
 #pragma unroll
       for (int p = 0; p < kSamplePlanes; ++p) q.sample[plane_at<kSamplePlanes>(slot, p)] = acc[p];
     }
@@ -891,10 +902,14 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   if (cap == 0) cap = st->samples_per_pixel;
   int32_t rc = BAJO_OK;
   constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
-  unsigned g_tail = 0;
+  unsigned g_tail = 0, simds = 0;
   if constexpr (kTail) {
     rc = grid_of(tail_kernel<kAlg, kWide>, &g_tail);
     if (rc) return rc;
+    int dev = 0, cus = 0;
+    BJ_HIP(hipGetDevice(&dev));
+    BJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    simds = 4u * (unsigned)cus;
   }
   // extend and shadow launch one thread per arena entry
   const unsigned g_ext = div_up(cap, kRtBlock), g_sh = g_ext;
@@ -932,7 +947,7 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
       if constexpr (kTail) {
         if (b > 0) {
           tail_kernel<kAlg, kWide><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, tail_threshold,
-                                                                    s->stats, s->spill);
+                                                                    simds, s->stats, s->spill);
           BJ_HIP(hipGetLastError());
         }
       }

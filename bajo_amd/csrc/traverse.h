@@ -82,18 +82,35 @@ BJ_D bool tri_scaled(const Ray& r, const TriData& tr, float t_max, float& abs_de
          ts > r.tmin * abs_det && ts < t_max * abs_det;
 }
 
-BJ_D bool leaf_closest(const BjTri* __restrict__ tris, uint32_t s, const Ray& r, Hit& hit) {
+// The closest-hit candidate held during a traversal: distance, scaled
+// barycentrics, reciprocal determinant and triangle position.  The hit record
+// (u = us * inv, v = vs * inv, primitive, unit normal of e1 x e2) is formed
+// once at the end from the same values, so it equals the record formed at
+// each accepted triangle, at a fraction of the per-triangle work.
+struct Closest {
+  float t, us, vs, inv;
+  uint32_t s;
+};
+
+BJ_D Closest no_closest(float t) { return Closest{t, 0.0f, 0.0f, 0.0f, kEmpty}; }
+
+BJ_D bool leaf_closest(const BjTri* __restrict__ tris, uint32_t s, const Ray& r, Closest& c) {
   TriData tr = load_tri(tris, s);
   float ad, ts, us, vs;
-  if (!tri_scaled(r, tr, hit.t, ad, ts, us, vs)) return false;
-  float inv = 1.0f / ad;
-  hit.t = ts * inv;
-  hit.u = us * inv;
-  hit.v = vs * inv;
-  hit.prim = tr.prim;
-  hit.inst = kEmpty;
-  hit.n = normalize(cross(tr.e1, tr.e2));
+  if (!tri_scaled(r, tr, c.t, ad, ts, us, vs)) return false;
+  const float inv = 1.0f / ad;
+  c.t = ts * inv;
+  c.us = us;
+  c.vs = vs;
+  c.inv = inv;
+  c.s = s;
   return true;
+}
+
+BJ_D Hit closest_hit(const BjTri* __restrict__ tris, const Closest& c) {
+  if (c.s == kEmpty) return miss(c.t);
+  const TriData tr = load_tri(tris, c.s);
+  return Hit{c.us * c.inv, c.vs * c.inv, tr.prim, kEmpty, normalize(cross(tr.e1, tr.e2)), c.t};
 }
 
 BJ_D bool leaf_any(const BjTri* __restrict__ tris, uint32_t s, const Ray& r) {
@@ -139,7 +156,7 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
   const BjTri* __restrict__ tris = bv.tris;
   const uint32_t root = bv.root, node_count = bv.node_count, prim_count = bv.prim_count;
   uint32_t* status = bv.status;
-  Hit hit = miss(ray.tmax);
+  Closest best = no_closest(ray.tmax);
   const v3 rcp = rcp_dir(ray.d);
   st.sp = 0;
   bool occluded = false;
@@ -147,7 +164,7 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
   if ((root & kLeafFlag) && root != kSentinel) {
     n_leaf++;
     if constexpr (kMode == BAJO_TRACE_ANY_HIT) occluded = leaf_any(tris, root & kIndexMask, ray);
-    else leaf_closest(tris, root & kIndexMask, ray, hit);
+    else leaf_closest(tris, root & kIndexMask, ray, best);
   } else if (root != kSentinel) {
     uint32_t cur = root;
     for (;;) {
@@ -158,7 +175,7 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
       const float4* np = reinterpret_cast<const float4*>(nodes + cur);
       float4 q0 = np[0], q1 = np[1], q2 = np[2];
       uint4 m = reinterpret_cast<const uint4*>(nodes + cur)[3];
-      const float node_tmax = kMode == BAJO_TRACE_ANY_HIT ? ray.tmax : hit.t;
+      const float node_tmax = kMode == BAJO_TRACE_ANY_HIT ? ray.tmax : best.t;
       float t0, t1;
       bool h0 = slab(ray.o, rcp, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, node_tmax, t0);
       bool h1 = slab(ray.o, rcp, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, node_tmax, t1);
@@ -190,11 +207,11 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
       } else {
         bool v0 = false, v1 = false;
         if (h0) {
-          if (l0) { n_leaf++; leaf_closest(tris, c0 & kIndexMask, ray, hit); }
+          if (l0) { n_leaf++; leaf_closest(tris, c0 & kIndexMask, ray, best); }
           else if (!(c0 & kLeafFlag)) { v0 = true; n_inner++; }
         }
         if (h1) {
-          if (l1) { n_leaf++; leaf_closest(tris, c1 & kIndexMask, ray, hit); }
+          if (l1) { n_leaf++; leaf_closest(tris, c1 & kIndexMask, ray, best); }
           else if (!(c1 & kLeafFlag)) { v1 = true; n_inner++; }
         }
         // bvh/gpu/trace.mojo:395-426: nearest child first, push the other.
@@ -203,23 +220,26 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
         if (v0) { near = 0; near_t = t0; }
         if (v1 && (near == -1 || t1 < near_t)) { near = 1; near_t = t1; }
         bool overflow = false;
-        if (v0 && near != 0 && t0 <= hit.t) {
+        if (v0 && near != 0 && t0 <= best.t) {
           if (st.sp >= kMaxStack) overflow = true; else st.push(c0);
         }
-        if (v1 && near != 1 && t1 <= hit.t) {
+        if (v1 && near != 1 && t1 <= best.t) {
           if (st.sp >= kMaxStack) overflow = true; else st.push(c1);
         }
         if (overflow) { atomicOr(status, (uint32_t)BAJO_STATUS_STACK_OVERFLOW); break; }
         if (kStats && (uint32_t)st.sp > max_sp) max_sp = (uint32_t)st.sp;
-        if (near != -1 && near_t <= hit.t) { cur = near == 0 ? c0 : c1; continue; }
+        if (near != -1 && near_t <= best.t) { cur = near == 0 ? c0 : c1; continue; }
         if (st.sp == 0) break;
         cur = st.pop();
       }
     }
   }
+  Hit hit;
   if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
     // bvh/gpu/trace.mojo:44-59: occlusion is reported as Hit.shadow_hit().
     hit = occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
+  } else {
+    hit = closest_hit(tris, best);
   }
   if constexpr (kStats) {
     tc->visits = n_visit;
@@ -270,7 +290,7 @@ BJ_D uint32_t sel8(uint32_t bit, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t
 template <int kMode, bool kStats>
 BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters* tc) {
   uint32_t n_visit = 0, n_inner = 0, n_leaf = 0, max_sp = 0;
-  Hit hit = miss(ray.tmax);
+  Closest best = no_closest(ray.tmax);
   const v3 rcp = rcp_dir(ray.d);
   const bool px = rcp.x >= 0.0f, py = rcp.y >= 0.0f, pz = rcp.z >= 0.0f;
   st.sp = 0;
@@ -300,9 +320,11 @@ BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters*
     const uint32_t nz0 = pz ? w2.x : w3.z, nz1 = pz ? w2.y : w3.w, fz0 = pz ? w3.z : w2.x, fz1 = pz ? w3.w : w2.y;
     uint32_t c0 = w4.x, c1 = w4.y, c2 = w4.z, c3 = w4.w, c4 = w5.x, c5 = w5.y, c6 = w5.z, c7 = w5.w;
     asm("" : "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3), "+v"(c4), "+v"(c5), "+v"(c6), "+v"(c7));
-    const float node_tmax = kMode == BAJO_TRACE_ANY_HIT ? ray.tmax : hit.t;
+    const float node_tmax = kMode == BAJO_TRACE_ANY_HIT ? ray.tmax : best.t;
     float th[8];
-    uint32_t inner_mask = 0, leaf_mask = 0;
+    // slot hits as a bit mask, built without branches (per-slot branches
+    // cost more than the slab test itself)
+    uint32_t hit_mask = 0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int bt = i & 3;
@@ -315,11 +337,12 @@ BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters*
       const float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
       const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, node_tmax));
       th[i] = tmin;
-      if (i < n && tmin <= tmax) {
-        if (sel8(1u << i, c0, c1, c2, c3, c4, c5, c6, c7) & kLeafFlag) leaf_mask |= 1u << i;
-        else inner_mask |= 1u << i;
-      }
+      hit_mask |= tmin <= tmax ? (1u << i) : 0u;
     }
+    hit_mask &= n >= 8 ? 0xFFu : (1u << n) - 1u;  // slots past the child count are never entered
+    const uint32_t leaf_bits = (c0 >> 31) | ((c1 >> 30) & 2u) | ((c2 >> 29) & 4u) | ((c3 >> 28) & 8u) |
+                               ((c4 >> 27) & 16u) | ((c5 >> 26) & 32u) | ((c6 >> 25) & 64u) | ((c7 >> 24) & 128u);
+    uint32_t leaf_mask = hit_mask & leaf_bits, inner_mask = hit_mask & ~leaf_bits;
     // leaf children: one triangle per iteration across all hit leaves
     uint32_t tri = 0, tri_end = 0;
     while (true) {
@@ -340,7 +363,7 @@ BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters*
           break;
         }
       } else {
-        leaf_closest(wv.tris, tri, ray, hit);
+        leaf_closest(wv.tris, tri, ray, best);
       }
       ++tri;
     }
@@ -350,7 +373,7 @@ BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters*
       // children entered beyond the (possibly improved) closest hit are dropped
 #pragma unroll
       for (int i = 0; i < 8; ++i)
-        if (th[i] > hit.t) inner_mask &= ~(1u << i);
+        if (th[i] > best.t) inner_mask &= ~(1u << i);
     }
     if (kStats) n_inner += __builtin_popcount(inner_mask);
     if (inner_mask) {
@@ -383,8 +406,11 @@ BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters*
     }
     cur = st.sp == 0 ? kSentinel : st.pop();
   }
+  Hit hit;
   if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
     hit = occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
+  } else {
+    hit = closest_hit(wv.tris, best);
   }
   if constexpr (kStats) {
     tc->visits = n_visit;

@@ -143,6 +143,7 @@ SIGNATURES = {
     ),
     "bajo_bvh_copy_wide": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, POINTER(c_uint64)]),
     "bajo_trace_rays": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_uint64, c_void_p]),
+    "bajo_ao_rays": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint64, c_void_p]),
     "bajo_trace_camera": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, c_void_p],

@@ -248,3 +248,23 @@ def build_triangle_bvh_measured(vertices: torch.Tensor, ctx: DeviceContext | Non
     t: dict = {}
     bvh = build_triangle_bvh(vertices, ctx=ctx, timings=t, node_width=node_width, leaf_size=leaf_size, method=method)
     return bvh, t
+
+
+def make_ao_rays(rays: torch.Tensor, hits: torch.Tensor, ray_count: int, samples: int, seed: int,
+                 out: torch.Tensor | None = None, ctx: DeviceContext | None = None) -> torch.Tensor:
+    """AO rays of a closest-hit launch (bajo's _make_ao_ray, rt/gpu/path_shading.mojo:650-665): `samples`
+    hemisphere rays per primary ray, field-major, for launch_rays(mode=TRACE.ANY_HIT).  Rays of primary
+    rays without a hit get t_max = -1."""
+    ctx = ctx or default_context(rays.device.index or 0)
+    r, s = int(ray_count), int(samples)
+    require_device(rays, ctx, "rays")
+    require_device(hits, ctx, "hits")
+    if rays.numel() != r * RAY_STRIDE or hits.numel() < r * HIT_STRIDE:
+        raise ValueError("rays must hold 8 planes of ray_count entries and hits 8 floats per ray")
+    if out is None:
+        out = torch.empty(r * s * RAY_STRIDE, dtype=torch.float32, device=rays.device)
+    require_device(out, ctx, "out")
+    if out.numel() != r * s * RAY_STRIDE:
+        raise ValueError("AO ray buffer must hold 8 planes of ray_count * samples entries")
+    _lib.check(_lib.lib().bajo_ao_rays(ctx.handle, ptr(rays), ptr(hits), r, s, int(seed), ptr(out)))
+    return out

@@ -18,6 +18,7 @@
 //     over 64-bit (code, index) keys is at most 64 levels deep, so 64 entries
 //     always suffice.
 #include "traverse.h"
+#include "rt_math.h"
 
 namespace bj {
 namespace {
@@ -142,10 +143,64 @@ int32_t launch(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const float*
   return launch_width<kMode, kStats, kCamera, false>(ctx, h, rays, cams, w, hh, hits, R, stats);
 }
 
+// Ambient-occlusion rays of closest hits: bajo's _make_ao_ray
+// (rt/gpu/path_shading.mojo:650-665) applied to a ray launch.  AO ray
+// j = r * samples + k belongs to primary ray r and draws its direction from
+// the Philox stream (seed, path j, stage 1): uniform on the sphere, flipped to
+// the side of the normal oriented against the incoming ray
+// (rt/geometry.mojo:20-30).  Origin o + t d, t in [0.001, 4.0].  A primary ray
+// without a surface hit yields a ray with t_max = -1, which no traversal can
+// hit.  Output is field-major like the input rays.
+__global__ void __launch_bounds__(256) ao_rays_kernel(const float* __restrict__ rays, const float* __restrict__ hits,
+                                                      uint64_t n, uint32_t samples, uint64_t seed,
+                                                      float* __restrict__ out) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t m = n * samples;
+  if (j >= m) return;
+  const uint64_t r = j / samples;
+  const float4* hp = reinterpret_cast<const float4*>(hits + 8 * r);
+  const float4 h0 = hp[0], h1 = hp[1];
+  const uint32_t prim = __builtin_bit_cast(uint32_t, h0.z);
+  const v3 o = mk(rays[r], rays[n + r], rays[2 * n + r]);
+  const v3 d = mk(rays[4 * n + r], rays[5 * n + r], rays[6 * n + r]);
+  const float tmax = rays[7 * n + r];
+  v3 po = mk(0.0f, 0.0f, 0.0f), dir = mk(0.0f, 0.0f, 1.0f);
+  float t_hi = -1.0f;
+  if (prim != kEmpty && h1.w < tmax) {
+    const v3 hn = mk(h1.x, h1.y, h1.z);
+    const v3 nn = dot(d, hn) < 0.0f ? hn : mk(-hn.x, -hn.y, -hn.z);
+    po = add(o, scale(d, h1.w));
+    StageRng rng(seed, (uint32_t)j, 1u);
+    float sn, cs;
+    sincos_turns(rng.next(), sn, cs);
+    const float z = 1.0f - 2.0f * rng.next();
+    const float rad = sqrtf(1.0f - z * z);
+    dir = mk(rad * cs, rad * sn, z);
+    if (dot(dir, nn) < 0.0f) dir = mk(-dir.x, -dir.y, -dir.z);
+    t_hi = 4.0f;
+  }
+  const float f[8] = {po.x, po.y, po.z, 0.001f, dir.x, dir.y, dir.z, t_hi};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) out[(uint64_t)k * m + j] = f[k];
+}
+
 }  // namespace
 }  // namespace bj
 
 extern "C" {
+
+int32_t bajo_ao_rays(bajo_ctx* ctx, const float* rays, const float* hits, uint64_t ray_count, uint32_t samples,
+                     uint64_t seed, float* ao_rays) {
+  BJ_REQUIRE(ctx != nullptr, "bajo_ao_rays: NULL ctx");
+  BJ_REQUIRE(rays != nullptr && hits != nullptr && ao_rays != nullptr, "bajo_ao_rays: NULL buffer");
+  BJ_REQUIRE(ray_count > 0 && samples > 0, "bajo_ao_rays: ray and sample counts must be positive");
+  BJ_REQUIRE(ray_count * samples < (1ull << 31), "bajo_ao_rays: ray_count * samples must be below 2^31");
+  const uint64_t m = ray_count * samples;
+  bj::ao_rays_kernel<<<(unsigned)((m + 255) / 256), 256, 0, ctx->stream>>>(rays, hits, ray_count, samples, seed,
+                                                                          ao_rays);
+  BJ_HIP(hipGetLastError());
+  return BAJO_OK;
+}
 
 int32_t bajo_trace_rays(bajo_ctx* ctx, const bajo_bvh* bvh, int32_t mode, const float* rays, float* hits,
                         uint64_t ray_count, uint32_t* stats) {

@@ -23,6 +23,21 @@ def soup(n: int, initstate: int = 2027, initseq: int = 54, edge_min=SOUP_EDGE_MI
     return out
 
 
+def soup_grid(rows: int = 64, cols: int = 64, per_copy: int = 4096, initstate: int = 2029) -> np.ndarray:
+    """A rows x cols grid of triangle soups flattened to one mesh (the 16.7M-triangle H-PLOC / 8-wide
+    workload at the default 64 x 64 x 4096): copy c = row * cols + col is soup(per_copy) from PCG32 stream
+    (initstate, initseq = c), translated by (col, row, 0) into its unit cell."""
+    out = np.empty((rows * cols, per_copy, 3, 3), dtype=np.float32)
+    for row in range(rows):
+        for col in range(cols):
+            c = row * cols + col
+            tri = soup(per_copy, initstate=initstate, initseq=c).reshape(per_copy, 3, 3)
+            tri[:, :, 0] += np.float32(col)
+            tri[:, :, 1] += np.float32(row)
+            out[c] = tri
+    return out.reshape(-1)
+
+
 def icosphere(n: int, frequency: int = 458, noise_seed: int = 2028, amplitude: float = 0.05, noise_frequency: float = 4.0):
     if n > 20 * frequency * frequency:
         raise ValueError(f"frequency {frequency} yields only {20 * frequency * frequency} triangles")

@@ -127,6 +127,152 @@ def incoherent_trace(scene, ctx, node_width: int, leaf_size: int, method: int, r
     }
 
 
+def _box_rays(v, rays: int, seed: int):
+    """`rays` rays with origins uniform in the box of vertices `v` (device f32[9N]) and directions uniform on
+    the sphere, field-major f32[8 * rays]."""
+    import torch
+
+    dev = v.device
+    lo = v.view(-1, 3).min(0).values
+    hi = v.view(-1, 3).max(0).values
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    o = lo + (hi - lo) * torch.rand((rays, 3), generator=g, device=dev)
+    d = torch.randn((rays, 3), generator=g, device=dev)
+    d = d / d.norm(dim=1, keepdim=True)
+    r = torch.empty((8, rays), dtype=torch.float32, device=dev)
+    r[0:3] = o.T
+    r[3] = 0.0
+    r[4:7] = d.T
+    r[7] = float(np.finfo(np.float32).max)
+    return r.reshape(-1).contiguous()
+
+
+def _timed(fn, stream, reps: int = 3) -> float:
+    """Mean milliseconds of `fn` over `reps` runs, HIP events on `stream` (one untimed run first)."""
+    import torch
+
+    fn()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        fn()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    return ev[0].elapsed_time(ev[1]) / reps
+
+
+def soup_grid_workload(ctx, rows: int = 64, cols: int = 64, ao_samples: int = 4) -> dict:
+    """H-PLOC (radius 8) build collapsed to 8-wide quantized nodes over a rows x cols grid of 4,096-triangle
+    soups (16.7M triangles at 64 x 64), then as many incoherent closest-hit rays as triangles and 4 AO rays
+    per hit traced any-hit (bajo's AO ray: hemisphere, t in [0.001, 4])."""
+    import torch
+
+    from bajo_amd import scenes
+    from bajo_amd.bvh import TRACE, build_triangle_bvh, make_ao_rays
+
+    dev = ctx.torch_device
+    stream = torch.cuda.current_stream(dev)
+    v = torch.from_numpy(scenes.soup_grid(rows, cols, 4096)).to(dev)
+    n = v.numel() // 9
+    bvh = build_triangle_bvh(v, ctx=ctx, node_width=8, leaf_size=4, method=1)
+    build_ms = _timed(lambda: bvh.rebuild(v), stream)
+    stages = bvh.rebuild(v, timings=True)
+    print(f"[bench] soup grid {rows}x{cols}: {n} triangles, H-PLOC + 8-wide build {build_ms:.2f} ms",
+          file=sys.stderr, flush=True)
+    R = n
+    rays = _box_rays(v, R, 2029)
+    hits = torch.empty(8 * R, dtype=torch.float32, device=dev)
+    st = torch.empty(5 * R, dtype=torch.int32, device=dev)
+    bvh.launch_rays(rays, hits, R, stats=st)
+    s = st.view(-1, 5).to(torch.float64)
+    visits, tests = float(s[:, 0].mean()), float(s[:, 3].mean())
+    del st, s
+    closest_ms = _timed(lambda: bvh.launch_rays(rays, hits, R), stream)
+    hit_frac = float((hits.view(-1, 8)[:, 2].view(torch.int32) != -1).float().mean())
+    ao = make_ao_rays(rays, hits, R, ao_samples, 2029)
+    occ = torch.empty(8 * R * ao_samples, dtype=torch.float32, device=dev)
+    gen_ms = _timed(lambda: make_ao_rays(rays, hits, R, ao_samples, 2029, out=ao), stream)
+    ao_ms = _timed(lambda: bvh.launch_rays(ao, occ, R * ao_samples, TRACE.ANY_HIT), stream)
+    o = occ.view(-1, 8)
+    live = int((ao.view(8, -1)[7] > 0).sum())
+    occluded = float(((o[:, 2].view(torch.int32) == -1) & (o[:, 7] == 0)).sum()) / max(live, 1)
+    print(f"[bench] soup grid traces: closest {closest_ms:.2f} ms, AO {ao_ms:.2f} ms", file=sys.stderr, flush=True)
+    bvh.free()
+    node_bytes = NODE_BYTES[8]
+    return {
+        "workload": f"{rows}x{cols} grid of 4096-triangle soups (PCG32 streams), H-PLOC radius 8 + 8-wide "
+        "quantized nodes (leaf size 4)",
+        "triangles": n,
+        "build_ms": build_ms,
+        "build_mtris_s": n / (build_ms / 1e3) / 1e6,
+        "build_stages_ms": {k: val / 1e6 for k, val in stages.items()},
+        "closest": {
+            "rays": R,
+            "ms": closest_ms,
+            "mrays_s": R / (closest_ms / 1e3) / 1e6,
+            "node_visits_per_ray": visits,
+            "tri_tests_per_ray": tests,
+            "hit_fraction": hit_frac,
+            "algorithmic_gbs": R * (64 + visits * node_bytes + tests * TRI_BYTES) / (closest_ms / 1e3) / 1e9,
+        },
+        "ao": {
+            "samples_per_hit": ao_samples,
+            "rays": R * ao_samples,
+            "live_rays": live,
+            "ray_gen_ms": gen_ms,
+            "ms": ao_ms,
+            "mrays_s": live / (ao_ms / 1e3) / 1e6,
+            "occluded_fraction": occluded,
+            "note": "rays of primary misses carry t_max = -1 and count in the launch but not in mrays_s",
+        },
+        "rays_def": "origins uniform in the scene box, directions uniform on the sphere",
+    }
+
+
+def soup_camera_workload(ctx, threads: int, cpu: bool) -> dict:
+    """LBVH build over 65,536 soup triangles and 1024 x 1024 pinhole primary rays from (0.5, 0.5, -1.5)
+    looking at the unit cube, closest hit; the C oracle build and trace of the same scene beside it."""
+    import torch
+
+    from bajo_amd import scenes
+    from bajo_amd.bvh import Camera, build_triangle_bvh
+
+    dev = ctx.torch_device
+    stream = torch.cuda.current_stream(dev)
+    vh = scenes.soup(65536, initstate=2027, initseq=54)
+    v = torch.from_numpy(vh).to(dev)
+    n = 65536
+    out = {"workload": "65,536-triangle soup (edges U[0.002, 0.02]), binary LBVH, 1024x1024 pinhole rays, closest hit"}
+    cam = Camera((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 0.41421356).flatten()
+    dcam = torch.from_numpy(cam).to(dev)
+    w = h = 1024
+    hits = torch.empty(8 * w * h, dtype=torch.float32, device=dev)
+    bvh = build_triangle_bvh(v, ctx=ctx)
+    build_ms = _timed(lambda: bvh.rebuild(v), stream, reps=10)
+    trace_ms = _timed(lambda: bvh.launch_camera(dcam, hits, w * h, w, h), stream, reps=10)
+    bvh.free()
+    out.update({"triangles": n, "build_ms": build_ms, "build_mtris_s": n / (build_ms / 1e3) / 1e6,
+                "rays": w * h, "trace_ms": trace_ms, "trace_mrays_s": w * h / (trace_ms / 1e3) / 1e6})
+    if cpu:
+        import oracle
+
+        t0 = time.perf_counter()
+        b = oracle.build_lbvh(vh)
+        cb = time.perf_counter() - t0
+        rays = oracle.camera_rays(cam, w * h, w, h)
+        t0 = time.perf_counter()
+        oracle.trace_bvh2(vh, b, rays)
+        ct1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oracle.trace_bvh2(vh, b, rays, threads=threads)
+        ctn = time.perf_counter() - t0
+        out["cpu"] = {"build_ms_1_thread": cb * 1e3, "trace_mrays_s_1_thread": w * h / ct1 / 1e6,
+                      f"trace_mrays_s_{threads}_threads": w * h / ctn / 1e6, "kind": "port",
+                      "sample": "C oracle LBVH build (one thread) and BVH2 closest-hit trace of the same rays"}
+    return out
+
+
 def cpu_baseline(scene, cam, threads: int) -> dict:
     import oracle
 
@@ -155,6 +301,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra-workloads", action="store_true",
+                    help="skip the 65k-soup camera and 16.7M soup-grid H-PLOC workloads")
     ap.add_argument("--node-width", type=int, default=8, choices=(2, 8), help="BVH layout: binary or 8-wide")
     ap.add_argument("--leaf-size", type=int, default=4, help="8-wide leaf size (triangles per leaf child)")
     ap.add_argument("--method", choices=("lbvh", "hploc"), default="lbvh", help="binary builder")
@@ -299,10 +447,13 @@ def main() -> int:
         },
     }
     line["incoherent_trace"] = incoherent_trace(scene, ctx, args.node_width, args.leaf_size, method)
+    gscene.free()
+    if not args.no_extra_workloads:
+        line["soup_camera_65k"] = soup_camera_workload(ctx, min(16, os.cpu_count() or 1), not args.no_cpu_baseline)
+        line["soup_grid_16m"] = soup_grid_workload(ctx)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, cam, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)
-    gscene.free()
     if args.copy_calibration_mib:
         nbytes = args.copy_calibration_mib << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)

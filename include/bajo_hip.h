@@ -187,6 +187,17 @@ int32_t bajo_trace_rays(bajo_ctx* ctx, const bajo_bvh* bvh, int32_t mode, const 
  * f32[20 * views] (bvh/camera.mojo:19-30), one raster ray per pixel. */
 int32_t bajo_trace_camera(bajo_ctx* ctx, const bajo_bvh* bvh, const float* camera, float* hits,
                           uint64_t ray_count, uint32_t width, uint32_t height, uint32_t* stats);
+/* Ambient-occlusion rays of a closest-hit launch: `samples` rays per
+ * primary ray, each bajo's _make_ao_ray (rt/gpu/path_shading.mojo:650-665):
+ * origin at the hit, direction uniform on the hemisphere of the normal
+ * oriented against the ray (Philox stream (seed, path r*samples+k, stage 1)),
+ * t in [0.001, 4.0].  rays / hits as bajo_trace_rays; ao_rays f32[8 * R *
+ * samples] field-major, ready for bajo_trace_rays(ANY_HIT).  Rays of primary
+ * rays that hit nothing get t_max = -1.  bajo builds this ray inside its AO
+ * render kernel; here it is a launch of its own so the AO rays of any ray set
+ * (the H-PLOC + 8-wide random-ray workload) can be traced. */
+int32_t bajo_ao_rays(bajo_ctx* ctx, const float* rays, const float* hits, uint64_t ray_count, uint32_t samples,
+                     uint64_t seed, float* ao_rays);
 /* Camera(origin, target, up, fov_scale, focus_dist), bvh/camera.mojo:86-106,
  * flattened to 20 floats (bvh/camera.mojo:210-231).  Host-side helper. */
 int32_t bajo_camera_make(const float* origin3, const float* target3, const float* up3, float fov_scale,

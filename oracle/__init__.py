@@ -84,6 +84,7 @@ _SIG = {
          c_void_p, c_void_p, c_void_p],
     ),
     "orc_hploc_lowest_leaf_numbering": (None, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
+    "orc_ao_rays": (None, [c_void_p, c_void_p, c_uint64, c_uint32, c_uint64, c_void_p]),
     "orc_render": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "orc_render_fp64": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
 }
@@ -545,3 +546,14 @@ def build_hploc(vertices: np.ndarray) -> dict:
     b["scene_bounds"] = lb["scene_bounds"]
     b["topology"] = t
     return b
+
+
+def ao_rays(rays: np.ndarray, hits: np.ndarray, samples: int, seed: int) -> np.ndarray:
+    """AO rays of a closest-hit launch (the renderer's AO ray per sample, rt/gpu/path_shading.mojo:650-665);
+    field-major f32[8 * R * samples]."""
+    r = f32(rays)
+    h = f32(hits)
+    n = r.size // 8
+    out = np.empty(8 * n * samples, np.float32)
+    lib().orc_ao_rays(_p(r), _p(h), n, samples, seed, _p(out))
+    return out

@@ -83,6 +83,7 @@ void orc_hploc_lowest_leaf_numbering(const uint32_t *left, const uint32_t *right
                                      uint64_t n, uint32_t *out);
 
 /* wavefront renderer (orc_render.c) */
+void orc_ao_rays(const float *rays, const float *hits, uint64_t n, uint32_t samples, uint64_t seed, float *out);
 typedef struct {
     uint32_t width, height, spp, max_depth;
     uint64_t seed;

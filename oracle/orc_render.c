@@ -620,6 +620,42 @@ static void *worker64(void *p) {
     return NULL;
 }
 
+/* AO rays of a closest-hit launch (the renderer's AO ray, bajo's
+ * _make_ao_ray, rt/gpu/path_shading.mojo:650-665, per sample): ray
+ * j = r * samples + k uses stream (seed, path j, stage 1); a primary ray
+ * without a hit yields origin 0, direction +z, t_max -1.  Field-major in and
+ * out (8 planes), hits AoS (8 floats). */
+void orc_ao_rays(const float *rays, const float *hits, uint64_t n, uint32_t samples, uint64_t seed, float *out) {
+    const uint64_t m = n * samples;
+    for (uint64_t j = 0; j < m; ++j) {
+        uint64_t r = j / samples;
+        const float *h = hits + 8 * r;
+        uint32_t prim;
+        memcpy(&prim, h + 2, 4);
+        orc_v3 o = orc_v(rays[r], rays[n + r], rays[2 * n + r]);
+        orc_v3 d = orc_v(rays[4 * n + r], rays[5 * n + r], rays[6 * n + r]);
+        float tmax = rays[7 * n + r];
+        orc_v3 po = orc_v(0, 0, 0), dir = orc_v(0, 0, 1);
+        float thi = -1.0f;
+        if (prim != ORC_EMPTY && h[7] < tmax) {
+            orc_v3 hn = orc_v(h[4], h[5], h[6]);
+            orc_v3 nn = orc_dot(d, hn) < 0.0f ? hn : orc_v(-hn.x, -hn.y, -hn.z);
+            po = orc_add(o, orc_scale(d, h[7]));
+            orc_rng ar;
+            rng_init(&ar, seed, (uint32_t)j, 1u);
+            float as_, ac_;
+            orc_sincos_turns(rng_next(&ar), &as_, &ac_);
+            float z = 1.0f - 2.0f * rng_next(&ar);
+            float rad = sqrtf(1.0f - z * z);
+            dir = orc_v(rad * ac_, rad * as_, z);
+            if (orc_dot(dir, nn) < 0.0f) dir = orc_v(-dir.x, -dir.y, -dir.z);
+            thi = 4.0f;
+        }
+        float f[8] = {po.x, po.y, po.z, 0.001f, dir.x, dir.y, dir.z, thi};
+        for (int k = 0; k < 8; ++k) out[(uint64_t)k * m + j] = f[k];
+    }
+}
+
 /* FP64 estimate of the PATH (0), NORMALS (1), AO (2) or NEE (3) image. */
 int orc_render_fp64(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
                     const float *emissive, const orc_render_settings *st, const float *cam20, double *pixels,

@@ -85,6 +85,86 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const AccelView av,
   }
 }
 
+// 8-wide launches keep every lane busy: a lane whose ray is finished takes a
+// new one as soon as a quarter of the wave is idle, instead of idling until
+// the slowest ray of its batch ends (random rays differ in node visits by an
+// order of magnitude, so fixed 64-ray batches left most lanes idle).  Each
+// wave reserves kRefillChunk ray indices per atomic and hands them to idle
+// lanes in lane order; one node visit per loop iteration.
+constexpr uint32_t kRefillLanes = 16;
+constexpr uint32_t kRefillChunk = 256;
+
+template <int kMode, bool kStats, bool kCamera>
+__global__ void __launch_bounds__(kTraceBlock) trace_refill_kernel(const WideView wv, const float* __restrict__ rays,
+                                                                   const float* __restrict__ cams, uint32_t cam_w,
+                                                                   uint32_t cam_h, float inv_height,
+                                                                   float* __restrict__ hits, uint64_t R,
+                                                                   uint32_t* __restrict__ stats,
+                                                                   uint32_t* __restrict__ counter,
+                                                                   uint32_t* __restrict__ spill) {
+  __shared__ uint32_t s_stack[kLdsStack * kTraceBlock];
+  const uint32_t lane = lane_id();
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
+  Walk8<kMode, kStats> w;
+  w.cur = kSentinel;
+  bool active = false, drained = false;
+  uint64_t r = 0, next = 0, stop = 0;  // the wave's reserved, not yet assigned indices [next, stop)
+  // short launches are spread one 64-ray batch per wave, long ones take up to
+  // kRefillChunk rays per atomic
+  const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t per_wave = (R / waves + 63u) & ~63ull;
+  const uint32_t chunk = per_wave < 64u ? 64u : (per_wave > kRefillChunk ? kRefillChunk : (uint32_t)per_wave);
+  for (;;) {
+    const uint64_t idle = ballot(!active);
+    const uint32_t n_idle = (uint32_t)__builtin_popcountll(idle);
+    if (!drained && n_idle >= kRefillLanes) {
+      if (next >= stop) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(counter, chunk);
+        base = __shfl(base, 0, 64);
+        next = base;
+        stop = (uint64_t)base + chunk < R ? (uint64_t)base + chunk : R;
+        if ((uint64_t)base >= R) drained = true;
+      }
+      if (next < stop) {
+        const uint64_t mine = next + mask_rank(idle);
+        if (!active && mine < stop) {
+          r = mine;
+          Ray ray;
+          if constexpr (kCamera) {
+            ray = camera_ray(cams, r, cam_w, cam_h, inv_height);
+          } else {
+            ray.o = mk(rays[0 * R + r], rays[1 * R + r], rays[2 * R + r]);
+            ray.tmin = rays[3 * R + r];
+            ray.d = mk(rays[4 * R + r], rays[5 * R + r], rays[6 * R + r]);
+            ray.tmax = rays[7 * R + r];
+          }
+          w.begin(wv, ray, st);
+          active = true;
+        }
+        next = next + n_idle < stop ? next + n_idle : stop;
+      }
+    }
+    if (!ballot(active)) {
+      if (drained) break;
+      continue;
+    }
+    if (active) {
+      if (!w.done()) w.step(wv, st);
+      if (w.done()) {
+        TraceCounters tc;
+        store_hit(hits, r, w.finish(wv, &tc));
+        if constexpr (kStats) {
+          uint32_t* sp = stats + 5 * r;
+          sp[0] = tc.visits; sp[1] = tc.inner; sp[2] = 0; sp[3] = tc.leaves; sp[4] = tc.max_sp;
+        }
+        active = false;
+      }
+    }
+  }
+}
+
 __global__ void trace_init_kernel(uint32_t* counter, uint32_t* status) {
   if (threadIdx.x == 0) { *counter = 0; *status = 0; }
 }
@@ -103,7 +183,12 @@ int32_t occupancy_grid(unsigned* grid) {
     int dev = 0, cus = 0, per_cu = 0;
     BJ_HIP(hipGetDevice(&dev));
     BJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<kMode, kStats, kCamera, kWide>, kTraceBlock, 0));
+    if constexpr (kWide)
+      BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_refill_kernel<kMode, kStats, kCamera>,
+                                                          kTraceBlock, 0));
+    else
+      BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<kMode, kStats, kCamera, kWide>,
+                                                          kTraceBlock, 0));
     if (per_cu < 1) per_cu = 1;
     cached = (unsigned)(cus * per_cu);
   }
@@ -129,8 +214,12 @@ int32_t launch_width(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const 
   uint32_t* spill = (uint32_t*)((char*)scratch + 256);
   trace_init_kernel<<<1, 64, 0, ctx->stream>>>(counter, status);
   const AccelView av = accel_view(h, status);
-  trace_kernel<kMode, kStats, kCamera, kWide><<<grid, kTraceBlock, 0, ctx->stream>>>(
-      av, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill);
+  if constexpr (kWide)
+    trace_refill_kernel<kMode, kStats, kCamera><<<grid, kTraceBlock, 0, ctx->stream>>>(
+        av.wide, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill);
+  else
+    trace_kernel<kMode, kStats, kCamera, kWide><<<grid, kTraceBlock, 0, ctx->stream>>>(
+        av, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill);
   BJ_HIP(hipGetLastError());
   return BAJO_OK;
 }

@@ -287,19 +287,40 @@ BJ_D uint32_t sel8(uint32_t bit, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t
   return r;
 }
 
+// The 8-wide traversal as a resumable walk: begin() takes a ray, each step()
+// visits one node (its eight slab tests, the triangles of the leaf children
+// it enters, the pushes of the inner ones) and moves to the next node;
+// done() once the stack is empty, the ray is occluded (any hit) or an error
+// ended it.  Launches that refill finished lanes with new rays call the steps
+// themselves; traverse8 runs one ray to the end.
 template <int kMode, bool kStats>
-BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters* tc) {
-  uint32_t n_visit = 0, n_inner = 0, n_leaf = 0, max_sp = 0;
-  Closest best = no_closest(ray.tmax);
-  const v3 rcp = rcp_dir(ray.d);
-  const bool px = rcp.x >= 0.0f, py = rcp.y >= 0.0f, pz = rcp.z >= 0.0f;
-  st.sp = 0;
-  bool occluded = false;
-  uint32_t cur = wv.node_count == 0 ? kSentinel : 0u;
-  while (cur != kSentinel) {
+struct Walk8 {
+  Ray ray;
+  v3 rcp;
+  bool px, py, pz, occluded;
+  Closest best;
+  uint32_t cur, n_visit, n_inner, n_leaf, max_sp;
+
+  BJ_D void begin(const WideView& wv, const Ray& r, Stack& st) {
+    ray = r;
+    best = no_closest(r.tmax);
+    rcp = rcp_dir(r.d);
+    px = rcp.x >= 0.0f;
+    py = rcp.y >= 0.0f;
+    pz = rcp.z >= 0.0f;
+    occluded = false;
+    st.sp = 0;
+    cur = wv.node_count == 0 ? kSentinel : 0u;
+    n_visit = n_inner = n_leaf = max_sp = 0;
+  }
+
+  BJ_D bool done() const { return cur == kSentinel; }
+
+  BJ_D void step(const WideView& wv, Stack& st) {
     if (cur >= wv.node_count || n_visit > wv.node_count) {
       atomicOr(wv.status, (uint32_t)BAJO_STATUS_BAD_NODE);
-      break;
+      cur = kSentinel;
+      return;
     }
     n_visit++;
     const uint4* np = reinterpret_cast<const uint4*>(wv.nodes + cur);
@@ -368,7 +389,10 @@ BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters*
       ++tri;
     }
     if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
-      if (occluded) break;
+      if (occluded) {
+        cur = kSentinel;
+        return;
+      }
     } else {
       // children entered beyond the (possibly improved) closest hit are dropped
 #pragma unroll
@@ -398,27 +422,40 @@ BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters*
       }
       if (overflow) {
         atomicOr(wv.status, (uint32_t)BAJO_STATUS_STACK_OVERFLOW);
-        break;
+        cur = kSentinel;
+        return;
       }
       if (kStats && (uint32_t)st.sp > max_sp) max_sp = (uint32_t)st.sp;
       cur = sel8(inner_mask, c0, c1, c2, c3, c4, c5, c6, c7);
-      continue;
+      return;
     }
     cur = st.sp == 0 ? kSentinel : st.pop();
+
+}
+
+  BJ_D Hit finish(const WideView& wv, TraceCounters* tc) const {
+    Hit hit;
+    if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
+      hit = occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
+    } else {
+      hit = closest_hit(wv.tris, best);
+    }
+    if constexpr (kStats) {
+      tc->visits = n_visit;
+      tc->inner = n_inner;
+      tc->leaves = n_leaf;
+      tc->max_sp = max_sp;
+    }
+    return hit;
   }
-  Hit hit;
-  if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
-    hit = occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
-  } else {
-    hit = closest_hit(wv.tris, best);
-  }
-  if constexpr (kStats) {
-    tc->visits = n_visit;
-    tc->inner = n_inner;
-    tc->leaves = n_leaf;
-    tc->max_sp = max_sp;
-  }
-  return hit;
+};
+
+template <int kMode, bool kStats>
+BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters* tc) {
+  Walk8<kMode, kStats> w;
+  w.begin(wv, ray, st);
+  while (!w.done()) w.step(wv, st);
+  return w.finish(wv, tc);
 }
 
 // Either layout behind one call: kWide selects the 8-wide traversal.

@@ -60,12 +60,12 @@ def extend_bytes(st: dict, node_width: int) -> float:
 TRAFFIC_PROFILE = ROOT / "profiles" / "render_icosphere4m_hbm_traffic.md"
 
 
-def measured_traffic() -> int | None:
-    """HBM bytes of the NEE render's extend launches, as measured with rocprofv3
-    FETCH_SIZE / WRITE_SIZE (copy-calibrated) and committed in profiles/."""
+def measured_traffic(key: str = "extend_kernel<NEE> HBM bytes per render") -> int | None:
+    """HBM bytes measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated) and committed in
+    profiles/: the NEE render's extend launches, or one 16.7M random-ray launch."""
     try:
         for line in TRAFFIC_PROFILE.read_text().splitlines():
-            if line.startswith("extend_kernel<NEE> HBM bytes per render:"):
+            if line.startswith(key + ":"):
                 return int(line.split(":")[1])
     except OSError:
         return None
@@ -81,21 +81,10 @@ def incoherent_trace(scene, ctx, node_width: int, leaf_size: int, method: int, r
     from bajo_amd.bvh import build_triangle_bvh
 
     dev = ctx.torch_device
-    v = torch.from_numpy(np.ascontiguousarray(scene.vertices, np.float32)).to(dev)
+    vh = np.ascontiguousarray(scene.vertices, np.float32)
+    v = torch.from_numpy(vh).to(dev)
     bvh = build_triangle_bvh(v, ctx=ctx, node_width=node_width, leaf_size=leaf_size, method=method)
-    lo = v.view(-1, 3).min(0).values
-    hi = v.view(-1, 3).max(0).values
-    g = torch.Generator(device=dev)
-    g.manual_seed(2027)
-    o = lo + (hi - lo) * torch.rand((rays, 3), generator=g, device=dev)
-    d = torch.randn((rays, 3), generator=g, device=dev)
-    d = d / d.norm(dim=1, keepdim=True)
-    r = torch.empty((8, rays), dtype=torch.float32, device=dev)
-    r[0:3] = o.T
-    r[3] = 0.0
-    r[4:7] = d.T
-    r[7] = float(np.finfo(np.float32).max)
-    r = r.reshape(-1).contiguous()
+    r = _box_rays(vh, rays, 2027, dev)
     hits = torch.empty(8 * rays, dtype=torch.float32, device=dev)
     st = torch.empty(5 * rays, dtype=torch.int32, device=dev)
     bvh.launch_rays(r, hits, rays, stats=st)  # instrumented pass: node visits and triangle tests
@@ -123,18 +112,20 @@ def incoherent_trace(scene, ctx, node_width: int, leaf_size: int, method: int, r
         "tri_tests_per_ray": tests,
         "hit_fraction": hit_frac,
         "algorithmic_gbs": algo / (ms / 1e3) / 1e9,
+        "hbm_bytes_measured": (hbm := measured_traffic("trace_refill_kernel HBM bytes per 16.7M-ray launch")),
+        "hbm_gbs": hbm / (ms / 1e3) / 1e9 if hbm and rays == 1 << 24 else None,
         "rays_def": "origins uniform in the scene box, directions uniform on the sphere, closest hit",
     }
 
 
-def _box_rays(v, rays: int, seed: int):
-    """`rays` rays with origins uniform in the box of vertices `v` (device f32[9N]) and directions uniform on
-    the sphere, field-major f32[8 * rays]."""
+def _box_rays(vh: np.ndarray, rays: int, seed: int, dev):
+    """`rays` rays on `dev` with origins uniform in the box of the host vertices `vh` (f32[9N]) and directions
+    uniform on the sphere, field-major f32[8 * rays]."""
     import torch
 
-    dev = v.device
-    lo = v.view(-1, 3).min(0).values
-    hi = v.view(-1, 3).max(0).values
+    pts = vh.reshape(-1, 3)
+    lo = torch.from_numpy(pts.min(axis=0)).to(dev)
+    hi = torch.from_numpy(pts.max(axis=0)).to(dev)
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     o = lo + (hi - lo) * torch.rand((rays, 3), generator=g, device=dev)
@@ -173,7 +164,8 @@ def soup_grid_workload(ctx, rows: int = 64, cols: int = 64, ao_samples: int = 4)
 
     dev = ctx.torch_device
     stream = torch.cuda.current_stream(dev)
-    v = torch.from_numpy(scenes.soup_grid(rows, cols, 4096)).to(dev)
+    vh = scenes.soup_grid(rows, cols, 4096)
+    v = torch.from_numpy(vh).to(dev)
     n = v.numel() // 9
     bvh = build_triangle_bvh(v, ctx=ctx, node_width=8, leaf_size=4, method=1)
     build_ms = _timed(lambda: bvh.rebuild(v), stream)
@@ -181,7 +173,7 @@ def soup_grid_workload(ctx, rows: int = 64, cols: int = 64, ao_samples: int = 4)
     print(f"[bench] soup grid {rows}x{cols}: {n} triangles, H-PLOC + 8-wide build {build_ms:.2f} ms",
           file=sys.stderr, flush=True)
     R = n
-    rays = _box_rays(v, R, 2029)
+    rays = _box_rays(vh, R, 2029, dev)
     hits = torch.empty(8 * R, dtype=torch.float32, device=dev)
     st = torch.empty(5 * R, dtype=torch.int32, device=dev)
     bvh.launch_rays(rays, hits, R, stats=st)

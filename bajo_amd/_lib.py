@@ -70,6 +70,21 @@ class BuildOptions(ctypes.Structure):
     ]
 
 
+class Materials(ctypes.Structure):
+    """bajo_materials: the flattened material arrays of SceneData."""
+
+    _fields_ = [
+        ("lambertian", c_void_p),
+        ("n_lambertian", c_uint32),
+        ("metal", c_void_p),
+        ("n_metal", c_uint32),
+        ("dielectric", c_void_p),
+        ("n_dielectric", c_uint32),
+        ("emissive", c_void_p),
+        ("n_emissive", c_uint32),
+    ]
+
+
 class RenderSettings(ctypes.Structure):
     _fields_ = [
         ("image_width", c_uint32),
@@ -155,6 +170,9 @@ SIGNATURES = {
     "bajo_scene_create": (
         c_int32,
         [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, POINTER(c_void_p)],
+    ),
+    "bajo_scene_create_materials": (
+        c_int32, [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, POINTER(BuildOptions), POINTER(c_void_p)],
     ),
     "bajo_scene_create_ex": (
         c_int32,

@@ -29,8 +29,8 @@ namespace {
 using namespace bj;
 
 constexpr int kRtBlock = kTraceBlock;
-// MAT kinds (rt/types.mojo:24-29); metal (1) and dielectric (2) are not rendered here
-constexpr uint32_t kMatLambertian = 0, kMatEmissive = 3;
+// MAT kinds (rt/types.mojo:24-29)
+constexpr uint32_t kMatLambertian = 0, kMatMetal = 1, kMatDielectric = 2, kMatEmissive = 3;
 constexpr uint32_t kSurfaceIndexMask = (1u << 28) - 1;
 constexpr uint32_t kPrimTriangle = 1;
 constexpr uint32_t kPathDelta = 0x80000000u, kPathIdMask = 0x7FFFFFFFu;
@@ -56,6 +56,8 @@ struct SceneView {
   AccelView bvh;
   const uint32_t* surfaces;
   const float* lambertian;
+  const float* metals;       // albedo.xyz, fuzz per metal
+  const float* dielectrics;  // refraction index per dielectric
   const float* emissive;
   const float* light_fields;
   const uint32_t* light_kinds;
@@ -266,7 +268,7 @@ BJ_D uint32_t wave_reserve(uint32_t* counter, bool want) {
 // rt/gpu/path_shading.mojo:261-403 (triangle lights only).
 template <int kAlg>
 BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uint32_t bounce, v3 point, v3 normal,
-                       v3 albedo, v3 throughput, ShadowWork& sw) {
+                       uint32_t kind, uint32_t mi, v3 ray_d, v3 throughput, ShadowWork& sw) {
   if (sc.light_count == 0 || !(sc.total_light_weight > 0.0f)) return false;
   StageRng rng(seed, path_id, rng_light_stage(bounce));
   const float alias_sample = rng.next() * (float)sc.light_count;
@@ -299,9 +301,20 @@ BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uin
   if (tmax <= 0.001f) return false;
   const v3 E = mk(L[10], L[11], L[12]);
   const float light_pdf = d2 * (E.x + E.y + E.z) / (3.0f * lcos * sc.total_light_weight);
-  // _evaluate_lambertian (rt/shading.mojo:13-27)
-  const v3 value = scale(albedo, kInvPi);
-  const float bsdf_pdf = fmax_sel(dot(normal, normalize(dir)), 0.0f) * kInvPi;
+  v3 value;
+  float bsdf_pdf;
+  if (kind == kMatMetal) {
+    // _evaluate_metal (rt/shading.mojo:30-57); a mirror gives pdf 0: no light sample
+    const float* m = sc.metals + 4 * (uint64_t)mi;
+    const BsdfEval ev = eval_metal(ray_d, normal, mk(m[0], m[1], m[2]), m[3], dir);
+    value = ev.value;
+    bsdf_pdf = ev.pdf;
+  } else {
+    // _evaluate_lambertian (rt/shading.mojo:13-27)
+    const float* a = sc.lambertian + 3 * (uint64_t)mi;
+    value = scale(mk(a[0], a[1], a[2]), kInvPi);
+    bsdf_pdf = fmax_sel(dot(normal, normalize(dir)), 0.0f) * kInvPi;
+  }
   // _direct_light_scale (rt/lighting.mojo:23-38)
   const bool ok = light_pdf > 0.0f && bsdf_pdf > 0.0f;
   float weight = 1.0f;
@@ -316,9 +329,6 @@ BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uin
   return true;
 }
 
-// kFirst: bounce 0 of a chunk, whose paths are the camera paths of the
-// chunk's path ids (generated here instead of read from a queue); it also
-// clears the chunk's sample accumulators.
 // Wave-aggregated maximum of a per-lane count.
 BJ_D void wave_max(unsigned long long* dst, uint32_t v) {
 #pragma unroll
@@ -372,7 +382,7 @@ BJ_D v3 albedo_of(const SceneView& sc, const Surface& su) {
 // The shadow ray an extension hit spawns: the NEE / MIS light sample
 // (rt/gpu/path_shading.mojo:261-460) or the AO hemisphere ray (:651-667).
 template <int kAlg>
-BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint32_t path_id, v3 T,
+BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint32_t path_id, v3 T, v3 ray_d,
                        const Surface& su, ShadowWork& sw) {
   if (!su.found) return false;
   if constexpr (kAlg == BAJO_RENDER_AO) {
@@ -388,8 +398,9 @@ BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint
     sw.id = path_id;
     return true;
   } else if constexpr (kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
-    if (su.kind != kMatLambertian) return false;
-    return direct_light<kAlg>(sc, seed, path_id, bounce, su.point, su.n, albedo_of(sc, su), T, sw);
+    // _sample_direct_light_candidate (rt/gpu/path_shading.mojo:261-403): Lambertian and metal surfaces
+    if (su.kind != kMatLambertian && su.kind != kMatMetal) return false;
+    return direct_light<kAlg>(sc, seed, path_id, bounce, su.point, su.n, su.kind, su.mi, ray_d, T, sw);
   } else {
     return false;
   }
@@ -399,6 +410,27 @@ BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint
 // weights, normals) and the next segment (Lambertian bounce with Russian
 // roulette), rt/gpu/path_shading.mojo:463-648.  Shared by the wavefront
 // extend kernel and the tail kernel so both follow one definition.
+// russian_roulette (rt/common.mojo:43-67) at depth bounce + 1: from depth 5
+// a path survives with probability max(T) clamped to [0.05, 0.95].
+BJ_D bool survives_roulette(uint64_t seed, uint32_t path_id, uint32_t bounce, v3& Tn) {
+  if (bounce + 1u < 5u) return true;
+  const float mx = fmax_sel(Tn.x, fmax_sel(Tn.y, Tn.z));
+  if (mx <= 0.0f) return false;
+  const float survival = fmin_sel(fmax_sel(mx, 0.05f), 0.95f);
+  StageRng rr(seed, path_id, rng_roulette_stage(bounce));
+  if (rr.next() >= survival) return false;
+  Tn = mk(Tn.x / survival, Tn.y / survival, Tn.z / survival);
+  return true;
+}
+
+// The next path segment from the hit point (rt/gpu/path_shading.mojo:768-782).
+BJ_D void next_path(NextPath& np, uint32_t path_id, bool delta, const Ray& ray, float t_hit, v3 dir, v3 Tn, float pdf) {
+  np.id = path_id | (delta ? kPathDelta : 0u);
+  const float nf[kPathPlanes] = {ray.o.x + t_hit * ray.d.x, ray.o.y + t_hit * ray.d.y, ray.o.z + t_hit * ray.d.z,
+                                 kT_MinSecondary, dir.x, dir.y, dir.z, kFltMax, Tn.x, Tn.y, Tn.z, pdf};
+  for (int p = 0; p < kPathPlanes; ++p) np.f[p] = nf[p];
+}
+
 // shade_bounce returns kShadeAdd when `add_c` holds radiance for the sample
 // and kShadeNext when `np` holds the next path segment (flags in one value:
 // bools returned through references were kept in scratch memory).
@@ -453,29 +485,40 @@ BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, 
       const v3 dir = normalize(sd);
       const float pdf = fmax_sel(dot(n, normalize(dir)), 0.0f) * kInvPi;
       v3 Tn = mul(T, albedo_of(sc, su));
-      // russian_roulette (rt/common.mojo:43-67), depth = bounce + 1
-      bool alive = true;
-      if (bounce + 1u >= 5u) {
-        const float mx = fmax_sel(Tn.x, fmax_sel(Tn.y, Tn.z));
-        if (mx <= 0.0f) {
-          alive = false;
-        } else {
-          const float survival = fmin_sel(fmax_sel(mx, 0.05f), 0.95f);
-          StageRng rr(seed, path_id, rng_roulette_stage(bounce));
-          if (rr.next() >= survival) alive = false;
-          else Tn = mk(Tn.x / survival, Tn.y / survival, Tn.z / survival);
-        }
-      }
-      if (alive) {
-        np.id = path_id;
-        const float nf[kPathPlanes] = {ray.o.x + t_hit * ray.d.x, ray.o.y + t_hit * ray.d.y,
-                                       ray.o.z + t_hit * ray.d.z, kT_MinSecondary, dir.x, dir.y, dir.z, kFltMax,
-                                       Tn.x, Tn.y, Tn.z, pdf};
-        for (int p = 0; p < kPathPlanes; ++p) np.f[p] = nf[p];
+      if (survives_roulette(seed, path_id, bounce, Tn)) {
+        next_path(np, path_id, false, ray, t_hit, dir, Tn, pdf);
         out |= kShadeNext;
       }
+    } else if (su.kind == kMatMetal || su.kind == kMatDielectric) {
+      // _gpu_rt_shade_one (rt/gpu/path_shading.mojo:669-783): random numbers
+      // are drawn only where the sample uses them
+      StageRng rng(seed, path_id, rng_bsdf_stage(bounce));
+      BsdfSample smp;
+      if (su.kind == kMatMetal) {
+        const float* m = sc.metals + 4 * (uint64_t)su.mi;
+        float ru = 0.0f, rv = 0.0f;
+        if (m[3] > 1.0e-4f) {
+          ru = rng.next();
+          rv = rng.next();
+        }
+        smp = sample_metal(ray.d, su.n, mk(m[0], m[1], m[2]), m[3], ru, rv);
+      } else {
+        const float ior = sc.dielectrics[su.mi];
+        const float ri = su.front ? 1.0f / ior : ior;
+        const v3 u = normalize(ray.d);
+        const float cos_t = fmin_sel(dot(mk(-u.x, -u.y, -u.z), su.n), 1.0f);
+        const float sin_t = sqrtf(fmax_sel(1.0f - cos_t * cos_t, 0.0f));
+        const float ru = ri * sin_t <= 1.0f ? rng.next() : 0.0f;
+        smp = sample_dielectric(ray.d, su.n, ior, su.front, ru);
+      }
+      if (smp.ok) {
+        v3 Tn = mul(T, smp.weight);
+        if (survives_roulette(seed, path_id, bounce, Tn)) {
+          next_path(np, path_id, smp.delta, ray, t_hit, smp.direction, Tn, smp.pdf);
+          out |= kShadeNext;
+        }
+      }
     }
-    // metal and dielectric surfaces are not part of this renderer's scenes: absorbed
   }
   return out;
 }
@@ -555,7 +598,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
       }
       su = surface_of(sc, ray, h);
       t_hit = h.t;
-      want_shadow = shade_shadow<kAlg>(sc, seed, bounce, path_id, T, su, sw);
+      want_shadow = shade_shadow<kAlg>(sc, seed, bounce, path_id, T, ray.d, su, sw);
     }
     const uint32_t ss = wave_reserve(q.counters + kShadow, want_shadow);
     if (want_shadow) {
@@ -656,7 +699,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
         ShadowWork sw;
         bool add_s = false;
         v3 cs = mk(0, 0, 0);
-        if (shade_shadow<kAlg>(sc, seed, b, path_id, T, su, sw)) {
+        if (shade_shadow<kAlg>(sc, seed, b, path_id, T, ray.d, su, sw)) {
           shadow = true;
           const Ray sray{mk(sw.f[0], sw.f[1], sw.f[2]), mk(sw.f[4], sw.f[5], sw.f[6]), sw.f[3], sw.f[7]};
           TraceCounters tcs;
@@ -689,7 +732,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
         if (alive) {
 #pragma unroll
           for (int p = 0; p < kPathPlanes; ++p) f[p] = np.f[p];
-          delta = false;
+          delta = (np.id & kPathDelta) != 0;
         }
       }
       if (b < kMaxBounces) {
@@ -770,6 +813,8 @@ struct bajo_scene {
   bajo_build_options options{};
   uint32_t* surfaces = nullptr;
   float* lambertian = nullptr;
+  float* metals = nullptr;
+  float* dielectrics = nullptr;
   float* emissive = nullptr;
   float* light_fields = nullptr;
   uint32_t* light_kinds = nullptr;
@@ -812,6 +857,7 @@ void free_scene(bajo_scene* s) {
   free_arena(s);
   bajo_bvh_free(s->bvh);
   (void)hipFree(s->vertices); (void)hipFree(s->surfaces); (void)hipFree(s->lambertian); (void)hipFree(s->emissive);
+  (void)hipFree(s->metals); (void)hipFree(s->dielectrics);
   (void)hipFree(s->light_fields); (void)hipFree(s->light_kinds);
   delete s;
 }
@@ -930,6 +976,8 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   sv.bvh = accel_view(s->bvh, s->counters + 8);
   sv.surfaces = s->surfaces;
   sv.lambertian = s->lambertian;
+  sv.metals = s->metals;
+  sv.dielectrics = s->dielectrics;
   sv.emissive = s->emissive;
   sv.light_fields = s->light_fields;
   sv.light_kinds = s->light_kinds;
@@ -991,15 +1039,30 @@ int32_t bajo_scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, cons
 int32_t bajo_scene_create_ex(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
                              const float* lambertian, uint32_t n_lambertian, const float* emissive,
                              uint32_t n_emissive, const bajo_build_options* options, bajo_scene** out) {
-  BJ_REQUIRE(ctx && out, "bajo_scene_create: NULL ctx/out");
+  bajo_materials m{lambertian, n_lambertian, nullptr, 0, nullptr, 0, emissive, n_emissive};
+  return bajo_scene_create_materials(ctx, vertices, n, surfaces, &m, options, out);
+}
+
+int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
+                                    const bajo_materials* materials, const bajo_build_options* options,
+                                    bajo_scene** out) {
+  BJ_REQUIRE(ctx && out && materials, "bajo_scene_create: NULL ctx/materials/out");
   BJ_REQUIRE(n > 0 && vertices && surfaces, "bajo_scene_create: a scene needs triangles and their surfaces");
   BJ_REQUIRE(n < (1ull << 31), "bajo_scene_create: too many triangles");
+  const float *lambertian = materials->lambertian, *metal = materials->metal, *dielectric = materials->dielectric,
+              *emissive = materials->emissive;
+  const uint32_t n_lambertian = materials->n_lambertian, n_metal = materials->n_metal,
+                 n_dielectric = materials->n_dielectric, n_emissive = materials->n_emissive;
+  BJ_REQUIRE((n_lambertian == 0 || lambertian) && (n_metal == 0 || metal) && (n_dielectric == 0 || dielectric) &&
+                 (n_emissive == 0 || emissive),
+             "bajo_scene_create: NULL material array with a nonzero count");
   for (uint64_t i = 0; i < n; ++i) {
     const uint32_t kind = surfaces[i] >> 28, mi = surfaces[i] & kSurfaceIndexMask;
-    BJ_REQUIRE(kind == kMatLambertian || kind == kMatEmissive,
-               "bajo_scene_create: triangle %llu has material kind %u (only Lambertian and emissive surfaces)",
+    BJ_REQUIRE(kind <= kMatEmissive, "bajo_scene_create: triangle %llu has material kind %u",
                (unsigned long long)i, kind);
     BJ_REQUIRE(kind != kMatLambertian || mi < n_lambertian, "bajo_scene_create: Lambertian index out of range");
+    BJ_REQUIRE(kind != kMatMetal || mi < n_metal, "bajo_scene_create: metal index out of range");
+    BJ_REQUIRE(kind != kMatDielectric || mi < n_dielectric, "bajo_scene_create: dielectric index out of range");
     BJ_REQUIRE(kind != kMatEmissive || mi < n_emissive, "bajo_scene_create: emissive index out of range");
   }
   bajo_scene* s = new bajo_scene();
@@ -1018,6 +1081,10 @@ int32_t bajo_scene_create_ex(bajo_ctx* ctx, const float* vertices, uint64_t n, c
   SC_HIP(hipMemcpy(s->surfaces, surfaces, n * sizeof(uint32_t), hipMemcpyHostToDevice));
   SC_HIP(hipMalloc(&s->lambertian, (n_lambertian ? n_lambertian : 1) * 3 * sizeof(float)));
   if (n_lambertian) SC_HIP(hipMemcpy(s->lambertian, lambertian, n_lambertian * 3 * sizeof(float), hipMemcpyHostToDevice));
+  SC_HIP(hipMalloc(&s->metals, (n_metal ? n_metal : 1) * 4 * sizeof(float)));
+  if (n_metal) SC_HIP(hipMemcpy(s->metals, metal, n_metal * 4 * sizeof(float), hipMemcpyHostToDevice));
+  SC_HIP(hipMalloc(&s->dielectrics, (n_dielectric ? n_dielectric : 1) * sizeof(float)));
+  if (n_dielectric) SC_HIP(hipMemcpy(s->dielectrics, dielectric, n_dielectric * sizeof(float), hipMemcpyHostToDevice));
   SC_HIP(hipMalloc(&s->emissive, (n_emissive ? n_emissive : 1) * 3 * sizeof(float)));
   if (n_emissive) SC_HIP(hipMemcpy(s->emissive, emissive, n_emissive * 3 * sizeof(float), hipMemcpyHostToDevice));
   // _build_light_store (rt/types.mojo:490-506): one light per emissive

@@ -116,4 +116,155 @@ BJ_HD v3 sky_color(v3 d) {
 
 BJ_HD float length2(v3 v) { return dot(v, v); }
 
+// ---- x^y for x in [0, 1] and y >= 0 (the Phong lobe of bajo's metal) --------
+// log2 x = k + ln(m) / ln 2 with x = m 2^k, m in [sqrt(1/2), sqrt(2)), and
+// ln m = 2 atanh(s), s = (m - 1) / (m + 1), by the odd series of atanh to
+// s^11 (|s| <= 0.172); 2^t with t = n + f, n = rint(t), |f| <= 1/2, and
+// e^(f ln 2) by its Taylor series to degree 8.  Plain FP32 operations and
+// fmaf only, so the host restatement gives the same bits.  Results below
+// 2^-125 are returned as 0.
+BJ_HD float log2_pos(float x) {
+  uint32_t b = __builtin_bit_cast(uint32_t, x);
+  int k = 0;
+  if (b < 0x00800000u) {  // subnormal input: scale into the normal range
+    x *= 8388608.0f;
+    b = __builtin_bit_cast(uint32_t, x);
+    k = -23;
+  }
+  k += (int)(b >> 23) - 127;
+  float m = __builtin_bit_cast(float, (b & 0x007FFFFFu) | 0x3F800000u);
+  if (m > 1.41421356237309504880f) {
+    m *= 0.5f;
+    k += 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  float p = fmaf(s2, 0.0909090909090909090909f, 0.111111111111111111111f);
+  p = fmaf(s2, p, 0.142857142857142857143f);
+  p = fmaf(s2, p, 0.2f);
+  p = fmaf(s2, p, 0.333333333333333333333f);
+  const float ln_m = 2.0f * fmaf(s * s2, p, s);
+  return fmaf(ln_m, 1.44269504088896340736f, (float)k);
+}
+
+BJ_HD float exp2_nonpos(float t) {
+  if (t < -125.0f) return 0.0f;
+  const float n = rintf(t);
+  const float x = (t - n) * 0.693147180559945309417f;
+  float p = fmaf(x, 2.48015873015873015873e-5f, 1.98412698412698412698e-4f);
+  p = fmaf(x, p, 1.38888888888888888889e-3f);
+  p = fmaf(x, p, 8.33333333333333333333e-3f);
+  p = fmaf(x, p, 4.16666666666666666667e-2f);
+  p = fmaf(x, p, 1.66666666666666666667e-1f);
+  p = fmaf(x, p, 0.5f);
+  p = fmaf(x, p, 1.0f);
+  p = fmaf(x, p, 1.0f);
+  return p * __builtin_bit_cast(float, (uint32_t)((int)n + 127) << 23);
+}
+
+BJ_HD float pow_unit(float x, float y) {
+  if (!(x > 0.0f)) return y == 0.0f ? 1.0f : 0.0f;
+  if (y == 0.0f) return 1.0f;
+  const float t = y * log2_pos(x);
+  return exp2_nonpos(t > 0.0f ? 0.0f : t);
+}
+
+// ---- bajo's metal and dielectric (rt/shading.mojo) -----------------------------
+constexpr float kTwoPi = 6.28318530717958647692f;
+
+// reflect / refract / reflectance, rt/shading.mojo:226-260.
+BJ_HD v3 reflect(v3 v, v3 n) { return sub(v, scale(n, 2.0f * dot(v, n))); }
+BJ_HD v3 refract(v3 uv, v3 n, float eta) {
+  const float cos_theta = fmin_sel(dot(mk(-uv.x, -uv.y, -uv.z), n), 1.0f);
+  const v3 perp = scale(add(uv, scale(n, cos_theta)), eta);
+  const v3 par = scale(n, -sqrtf(fabsf(1.0f - length2(perp))));
+  return add(perp, par);
+}
+BJ_HD float reflectance(float cosine, float ref_idx) {
+  const float root = (1.0f - ref_idx) / (1.0f + ref_idx);
+  const float r2 = root * root;
+  const float x = 1.0f - cosine;
+  const float x2 = x * x;
+  const float x5 = x2 * x2 * x;
+  return fmaf(1.0f - r2, x5, r2);
+}
+
+struct BsdfEval {
+  v3 value;
+  float pdf;
+  bool delta;
+};
+struct BsdfSample {
+  v3 direction, weight;
+  float pdf;
+  bool delta, ok;
+};
+
+// _evaluate_metal, rt/shading.mojo:30-57: a normalized Phong lobe around the
+// mirror direction, exponent 2 / fuzz^2 - 2; fuzz <= 1e-4 is a mirror (delta).
+BJ_HD BsdfEval eval_metal(v3 ray_d, v3 n, v3 albedo, float fuzz, v3 out_dir) {
+  const bool smooth = fuzz <= 1.0e-4f;
+  const float sf = smooth ? 1.0f : fuzz;
+  const v3 dir = normalize(out_dir);
+  const v3 refl = normalize(reflect(normalize(ray_d), n));
+  const bool surface_valid = dot(n, dir) > 0.0f;
+  const float lobe_cos = fmax_sel(dot(refl, dir), 0.0f);
+  const bool lobe_valid = lobe_cos > 0.0f;
+  const float e = fmax_sel(2.0f / (sf * sf) - 2.0f, 0.0f);
+  const float lobe = pow_unit(lobe_cos, e);
+  const bool valid = !smooth && surface_valid && lobe_valid;
+  const float vs = (e + 2.0f) * lobe / kTwoPi;
+  const float pdf = (e + 1.0f) * lobe / kTwoPi;
+  return BsdfEval{valid ? scale(albedo, vs) : mk(0.0f, 0.0f, 0.0f), valid ? pdf : 0.0f, smooth};
+}
+
+// _sample_metal, rt/shading.mojo:120-160 (cos / sin of 2 pi v by sincos_turns).
+BJ_HD BsdfSample sample_metal(v3 ray_d, v3 n, v3 albedo, float fuzz, float ru, float rv) {
+  const v3 refl = normalize(reflect(normalize(ray_d), n));
+  const bool smooth = fuzz <= 1.0e-4f;
+  const float sf = smooth ? 1.0f : fuzz;
+  const float e = fmax_sel(2.0f / (sf * sf) - 2.0f, 0.0f);
+  const float cos_t = pow_unit(ru, 1.0f / (e + 1.0f));
+  const float sin_t = sqrtf(fmax_sel(1.0f - cos_t * cos_t, 0.0f));
+  float sp, cp;
+  sincos_turns(rv, sp, cp);
+  const v3 helper = fabsf(refl.y) > 0.99f ? mk(1.0f, 0.0f, 0.0f) : mk(0.0f, 1.0f, 0.0f);
+  const v3 tangent = normalize(cross(helper, refl));
+  const v3 bitangent = cross(refl, tangent);
+  const v3 rough = normalize(add(add(scale(tangent, cp * sin_t), scale(bitangent, sp * sin_t)), scale(refl, cos_t)));
+  const v3 dir = smooth ? refl : rough;
+  const BsdfEval ev = eval_metal(ray_d, n, albedo, fuzz, dir);
+  const float scos = fmax_sel(dot(n, dir), 0.0f);
+  const bool rough_ok = ev.pdf > 0.0f && scos > 0.0f;
+  const float safe_pdf = rough_ok ? ev.pdf : 1.0f;
+  const float k = scos / safe_pdf;
+  BsdfSample out;
+  out.direction = dir;
+  out.weight = smooth ? albedo : scale(ev.value, k);
+  out.pdf = smooth ? 1.0f : ev.pdf;
+  out.delta = smooth;
+  out.ok = smooth ? dot(refl, n) > 0.0f : rough_ok;
+  return out;
+}
+
+// _sample_dielectric, rt/shading.mojo:163-196: Schlick reflection
+// probability, total internal reflection, weight 1, always delta.
+BJ_HD BsdfSample sample_dielectric(v3 ray_d, v3 n, float ior, bool front, float ru) {
+  const float ri = front ? 1.0f / ior : ior;
+  const v3 u = normalize(ray_d);
+  const float cos_t = fmin_sel(dot(mk(-u.x, -u.y, -u.z), n), 1.0f);
+  const float sin_t = sqrtf(fmax_sel(1.0f - cos_t * cos_t, 0.0f));
+  const bool cannot = ri * sin_t > 1.0f;
+  const float rp = reflectance(cos_t, ri);
+  const bool refl = cannot || rp > ru;
+  const v3 dir = normalize(refl ? reflect(u, n) : refract(u, n, ri));
+  BsdfSample out;
+  out.direction = dir;
+  out.weight = mk(1.0f, 1.0f, 1.0f);
+  out.pdf = refl ? (cannot ? 1.0f : rp) : 1.0f - rp;
+  out.delta = true;
+  out.ok = true;
+  return out;
+}
+
 }  // namespace bj

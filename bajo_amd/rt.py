@@ -3,7 +3,8 @@
 Mirrors ``render_gpu[ALGORITHM](settings, camera, world) -> RenderResult``
 (rt/gpu/render.mojo:322-420), ``RenderSettings`` (rt/types.mojo:324-357),
 the ``RENDER`` / ``MAT`` enums (rt/types.mojo:24-57) and ``SceneData`` for
-triangle scenes with Lambertian and emissive surfaces (rt/types.mojo:382-526).
+triangle scenes with Lambertian, metal, dielectric and emissive surfaces
+(rt/types.mojo:382-526).
 The renderer runs entirely in libbajo_hip.so (bajo_amd/csrc/render.hip).
 """
 
@@ -71,6 +72,8 @@ class SceneData:
     surfaces: np.ndarray  # u32[N], surface_id(kind, index)
     lambertian: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[3L] albedo
     emissive: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[3E] radiance
+    metal: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[4M] albedo, fuzz
+    dielectric: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[D] refraction index
 
     @property
     def triangle_count(self) -> int:
@@ -89,24 +92,30 @@ class GpuScene:
         s = np.ascontiguousarray(scene.surfaces, dtype=np.uint32)
         lamb = np.ascontiguousarray(scene.lambertian, dtype=np.float32).reshape(-1)
         emit = np.ascontiguousarray(scene.emissive, dtype=np.float32).reshape(-1)
+        met = np.ascontiguousarray(getattr(scene, "metal", np.zeros(0)), dtype=np.float32).reshape(-1)
+        die = np.ascontiguousarray(getattr(scene, "dielectric", np.zeros(0)), dtype=np.float32).reshape(-1)
         if s.size != v.size // 9:
             raise ValueError("one surface id per triangle is required")
+        if lamb.size % 3 or emit.size % 3 or met.size % 4:
+            raise ValueError("materials are f32[3] per Lambertian / emissive and f32[4] per metal")
         handle = ctypes.c_void_p()
         opts = build_options(node_width, leaf_size, method)
+        mats = _lib.Materials(
+            lamb.ctypes.data_as(ctypes.c_void_p), lamb.size // 3, met.ctypes.data_as(ctypes.c_void_p), met.size // 4,
+            die.ctypes.data_as(ctypes.c_void_p), die.size, emit.ctypes.data_as(ctypes.c_void_p), emit.size // 3,
+        )
         _lib.check(
-            _lib.lib().bajo_scene_create_ex(
+            _lib.lib().bajo_scene_create_materials(
                 self.ctx.handle,
                 v.ctypes.data_as(ctypes.c_void_p),
                 v.size // 9,
                 s.ctypes.data_as(ctypes.c_void_p),
-                lamb.ctypes.data_as(ctypes.c_void_p),
-                lamb.size // 3,
-                emit.ctypes.data_as(ctypes.c_void_p),
-                emit.size // 3,
+                ctypes.byref(mats),
                 ctypes.byref(opts),
                 ctypes.byref(handle),
             )
         )
+        self._keep = (lamb, emit, met, die)
         self._handle = handle
         self.triangle_count = v.size // 9
         self.node_width = node_width

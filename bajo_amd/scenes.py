@@ -115,3 +115,46 @@ def box_scene(n_soup: int = 120, seed: int = 5, emission: float = 12.0):
     ]
     assert n_walls == 8
     return _scene([walls, soup_t, light], kinds, lamb, [emission, emission, emission])
+
+
+def material_box_scene(seed: int = 7, n_soup: int = 60, emission: float = 12.0):
+    """The box scene's walls and light with all four of bajo's material kinds:
+    a glass ball (dielectric, index 1.5), a rough metal ball (fuzz 0.25), a
+    mirror panel (fuzz 0) on the back wall, and a soup of small triangles
+    cycling Lambertian / rough metal / glass; for material parity and
+    radiance tests."""
+    from .rt import MAT, SceneData, surface_id
+
+    floor = quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))
+    back = quad((-1, 0, 1), (-1, 2, 1), (1, 2, 1), (1, 0, 1))
+    left = quad((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1))
+    right = quad((1, 0, 1), (1, 2, 1), (1, 2, -1), (1, 0, -1))
+    mirror = quad((-0.5, 0.4, 0.99), (-0.5, 1.4, 0.99), (0.5, 1.4, 0.99), (0.5, 0.4, 0.99))
+    light = quad((-0.3, 1.9, -0.3), (0.3, 1.9, -0.3), (0.3, 1.9, 0.3), (-0.3, 1.9, 0.3))
+    ball = icosphere(320, frequency=4, amplitude=0.0).reshape(-1, 3)
+    glass = (ball * 0.3 + np.array([0.4, 0.3, 0.1], np.float32)).astype(np.float32).reshape(-1)
+    metal_ball = (ball * 0.25 + np.array([-0.45, 0.25, 0.3], np.float32)).astype(np.float32).reshape(-1)
+    rng = np.random.default_rng(seed)
+    v0 = rng.uniform([-0.8, 0.6, -0.6], [0.8, 1.5, 0.6], (n_soup, 3))
+    e = rng.uniform(-0.2, 0.2, (n_soup, 2, 3))
+    soup_t = np.concatenate([v0, v0 + e[:, 0], v0 + e[:, 1]], axis=1).astype(np.float32).reshape(-1)
+    lamb = np.array([[0.7, 0.7, 0.7], [0.7, 0.7, 0.7], [0.6, 0.2, 0.2], [0.2, 0.6, 0.2], [0.5, 0.4, 0.3]], np.float32)
+    metal = np.array([[0.9, 0.9, 0.9, 0.0], [0.8, 0.65, 0.3, 0.25], [0.7, 0.75, 0.8, 0.35]], np.float32)
+    dielectric = np.array([1.5], np.float32)
+    wall_mat = np.repeat(np.arange(4, dtype=np.uint32), 2)
+    cycle = np.array([surface_id(MAT.LAMBERTIAN, 4), surface_id(MAT.METAL, 2), surface_id(MAT.DIELECTRIC, 0)],
+                     np.uint32)
+    kinds = [
+        (np.uint32(MAT.LAMBERTIAN) << 28) | wall_mat,
+        np.full(2, surface_id(MAT.METAL, 0), np.uint32),
+        np.full(glass.size // 9, surface_id(MAT.DIELECTRIC, 0), np.uint32),
+        np.full(metal_ball.size // 9, surface_id(MAT.METAL, 1), np.uint32),
+        cycle[np.arange(n_soup) % 3],
+        np.full(2, surface_id(MAT.EMISSIVE, 0), np.uint32),
+    ]
+    walls = np.concatenate([floor, back, left, right])
+    vertices = np.concatenate([walls, mirror, glass, metal_ball, soup_t, light]).astype(np.float32)
+    surfaces = np.concatenate(kinds).astype(np.uint32)
+    assert surfaces.size * 9 == vertices.size
+    return SceneData(vertices, surfaces, lamb.reshape(-1), np.array([emission] * 3, np.float32), metal.reshape(-1),
+                     dielectric)

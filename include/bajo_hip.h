@@ -258,6 +258,29 @@ int32_t bajo_scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, cons
 int32_t bajo_scene_create_ex(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
                              const float* lambertian, uint32_t n_lambertian, const float* emissive,
                              uint32_t n_emissive, const bajo_build_options* options, bajo_scene** out);
+/* The material arrays of SceneData (rt/types.mojo:382-526), flattened as
+ * bajo's GPU shading view does (rt/gpu/path_shading.mojo:87-120): Lambertian
+ * albedo f32[3], metal albedo + fuzz f32[4], dielectric refraction index
+ * f32[1], emissive radiance f32[3] per material.  Surface ids select them by
+ * kind (MAT: 0 Lambertian, 1 metal, 2 dielectric, 3 emissive). */
+typedef struct bajo_materials {
+  const float* lambertian;
+  uint32_t n_lambertian;
+  const float* metal;
+  uint32_t n_metal;
+  const float* dielectric;
+  uint32_t n_dielectric;
+  const float* emissive;
+  uint32_t n_emissive;
+} bajo_materials;
+/* bajo_scene_create_ex with every material kind: metal (bajo's Phong-lobe
+ * metal, mirror at fuzz <= 1e-4) and dielectric (Schlick reflection,
+ * refraction) surfaces are shaded as in rt/gpu/path_shading.mojo:669-783
+ * and take direct light like Lambertian ones where the BSDF has a density
+ * (rough metal). */
+int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
+                                    const bajo_materials* materials, const bajo_build_options* options,
+                                    bajo_scene** out);
 /* Rebuilds the scene's BVH from its device-resident vertices. */
 int32_t bajo_scene_rebuild(bajo_ctx* ctx, bajo_scene* scene, bajo_build_timings* timings);
 int32_t bajo_scene_free(bajo_scene* scene);

@@ -85,8 +85,20 @@ _SIG = {
     ),
     "orc_hploc_lowest_leaf_numbering": (None, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
     "orc_ao_rays": (None, [c_void_p, c_void_p, c_uint64, c_uint32, c_uint64, c_void_p]),
-    "orc_render": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
-    "orc_render_fp64": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "orc_render": (
+        c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                c_void_p, c_int],
+    ),
+    "orc_render_fp64": (
+        c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                c_int],
+    ),
+    "orc_pow_unit": (c_float, [c_float, c_float]),
+    "orc_reflectance": (c_float, [c_float, c_float]),
+    "orc_reflect3": (None, [c_void_p, c_void_p, c_void_p]),
+    "orc_sample_metal": (None, [c_void_p, c_void_p, c_void_p, c_float, c_float, c_float, c_void_p]),
+    "orc_eval_metal": (None, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
+    "orc_sample_dielectric": (None, [c_void_p, c_void_p, c_float, c_int, c_float, c_void_p]),
 }
 
 
@@ -331,17 +343,26 @@ def light_table(vertices, surfaces, emissive):
     return fields[: 14 * k], kinds[:k], float(total[0])
 
 
+def _materials(lambertian, emissive, metal, dielectric):
+    def arr(a, k):
+        return f32(a) if a is not None and len(a) else np.zeros(k, np.float32)
+
+    return arr(lambertian, 3), arr(emissive, 3), arr(metal, 4), arr(dielectric, 1)
+
+
 def render(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1,
-           node_width=2, leaf_size=0, method=0):
+           node_width=2, leaf_size=0, method=0, metal=None, dielectric=None):
+    """The FP32 wavefront renderer restated (Lambertian, metal f32[4 per material], dielectric f32[1],
+    emissive): (pixels, (extension rays, shadow rays))."""
     v = f32(vertices)
     surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
-    lamb = f32(lambertian) if len(lambertian) else np.zeros(3, np.float32)
-    emit = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
+    lamb, emit, met, die = _materials(lambertian, emissive, metal, dielectric)
     st = RenderSettings(width, height, spp, max_depth, seed, algorithm, node_width, leaf_size, method)
     pixels = np.empty(3 * width * height, np.float32)
     rays = np.zeros(2, np.uint64)
     rc = lib().orc_render(
-        _p(v), v.size // 9, _p(surf), _p(lamb), _p(emit), ctypes.byref(st), _p(f32(cam20)), _p(pixels), _p(rays), threads
+        _p(v), v.size // 9, _p(surf), _p(lamb), _p(met), _p(die), _p(emit), ctypes.byref(st), _p(f32(cam20)),
+        _p(pixels), _p(rays), threads,
     )
     if rc != 0:
         raise RuntimeError(f"orc_render failed ({rc})")
@@ -428,15 +449,50 @@ def canonical_wide(nodes: np.ndarray) -> np.ndarray:
     return out
 
 
-def render_fp64(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1):
+def render_fp64(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1,
+                metal=None, dielectric=None):
     v = f32(vertices)
     surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
-    lamb = f32(lambertian) if len(lambertian) else np.zeros(3, np.float32)
-    emit = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
+    lamb, emit, met, die = _materials(lambertian, emissive, metal, dielectric)
     st = RenderSettings(width, height, spp, max_depth, seed, algorithm, 2, 0, 0)
     pixels = np.empty(3 * width * height, np.float64)
-    lib().orc_render_fp64(_p(v), v.size // 9, _p(surf), _p(lamb), _p(emit), ctypes.byref(st), _p(f32(cam20)), _p(pixels), threads)
+    lib().orc_render_fp64(_p(v), v.size // 9, _p(surf), _p(lamb), _p(met), _p(die), _p(emit), ctypes.byref(st),
+                          _p(f32(cam20)), _p(pixels), threads)
     return pixels
+
+
+# ---- materials (rt/shading.mojo) ----------------------------------------------
+
+def pow_unit(x: float, y: float) -> float:
+    return float(lib().orc_pow_unit(x, y))
+
+
+def reflectance(cosine: float, ref_idx: float) -> float:
+    return float(lib().orc_reflectance(cosine, ref_idx))
+
+
+def reflect(v, n) -> np.ndarray:
+    out = np.empty(3, np.float32)
+    lib().orc_reflect3(_p(f32(v)), _p(f32(n)), _p(out))
+    return out
+
+
+def sample_metal(ray_d, n, albedo, fuzz, ru, rv) -> dict:
+    out = np.empty(9, np.float32)
+    lib().orc_sample_metal(_p(f32(ray_d)), _p(f32(n)), _p(f32(albedo)), fuzz, ru, rv, _p(out))
+    return {"dir": out[:3], "weight": out[3:6], "pdf": float(out[6]), "delta": bool(out[7]), "ok": bool(out[8])}
+
+
+def eval_metal(ray_d, n, albedo, fuzz, out_dir) -> dict:
+    out = np.empty(5, np.float32)
+    lib().orc_eval_metal(_p(f32(ray_d)), _p(f32(n)), _p(f32(albedo)), fuzz, _p(f32(out_dir)), _p(out))
+    return {"value": out[:3], "pdf": float(out[3]), "delta": bool(out[4])}
+
+
+def sample_dielectric(ray_d, n, ior, front, ru) -> dict:
+    out = np.empty(9, np.float32)
+    lib().orc_sample_dielectric(_p(f32(ray_d)), _p(f32(n)), ior, int(front), ru, _p(out))
+    return {"dir": out[:3], "weight": out[3:6], "pdf": float(out[6]), "delta": bool(out[7]), "ok": bool(out[8])}
 
 
 # ---- H-PLOC ------------------------------------------------------------------

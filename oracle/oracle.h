@@ -99,11 +99,19 @@ void orc_alias_table(const float *w, uint32_t n, float total, float *prob, uint3
 void orc_light_table(const float *v, uint64_t n, const uint32_t *surf, const float *emissive, float *fields,
                      uint32_t *kinds, uint32_t *count, float *total);
 int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
-               const float *emissive, const orc_render_settings *st, const float *cam20, float *pixels,
-               uint64_t *ray_counts, int threads);
+               const float *metal, const float *dielectric, const float *emissive, const orc_render_settings *st,
+               const float *cam20, float *pixels, uint64_t *ray_counts, int threads);
+float orc_pow_unit(float x, float y);
+float orc_reflectance(float cosine, float ref_idx);
+void orc_reflect3(const float *v, const float *n, float *out);
+void orc_sample_metal(const float *ray_d, const float *n, const float *albedo, float fuzz, float ru, float rv,
+                      float *out);
+void orc_eval_metal(const float *ray_d, const float *n, const float *albedo, float fuzz, const float *out_dir,
+                    float *out);
+void orc_sample_dielectric(const float *ray_d, const float *n, float ior, int front, float ru, float *out);
 int orc_render_fp64(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
-                    const float *emissive, const orc_render_settings *st, const float *cam20, double *pixels,
-                    int threads);
+                    const float *metal, const float *dielectric, const float *emissive,
+                    const orc_render_settings *st, const float *cam20, double *pixels, int threads);
 
 #ifdef __cplusplus
 }

@@ -93,9 +93,176 @@ void orc_sincos_turns(float u, float *s, float *c) {
     }
 }
 
+/* ---- x^y for x in [0, 1], y >= 0 (the metal's Phong lobe) --------------------
+ * log2 via x = m 2^k, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh((m-1)/(m+1)) by
+ * its odd series to s^11; 2^t via t = n + f, n = rint(t), e^(f ln 2) by its
+ * Taylor series to degree 8; results below 2^-125 are 0.  (The renderer's
+ * own definition; FP32 operations and fmaf only.) */
+static float orc_log2_pos(float x) {
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    int k = 0;
+    if (b < 0x00800000u) { x *= 8388608.0f; memcpy(&b, &x, 4); k = -23; }
+    k += (int)(b >> 23) - 127;
+    uint32_t mb = (b & 0x007FFFFFu) | 0x3F800000u;
+    float m;
+    memcpy(&m, &mb, 4);
+    if (m > 1.41421356237309504880f) { m *= 0.5f; k += 1; }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = fmaf(s2, 0.0909090909090909090909f, 0.111111111111111111111f);
+    p = fmaf(s2, p, 0.142857142857142857143f);
+    p = fmaf(s2, p, 0.2f);
+    p = fmaf(s2, p, 0.333333333333333333333f);
+    float ln_m = 2.0f * fmaf(s * s2, p, s);
+    return fmaf(ln_m, 1.44269504088896340736f, (float)k);
+}
+
+static float orc_exp2_nonpos(float t) {
+    if (t < -125.0f) return 0.0f;
+    float n = rintf(t);
+    float x = (t - n) * 0.693147180559945309417f;
+    float p = fmaf(x, 2.48015873015873015873e-5f, 1.98412698412698412698e-4f);
+    p = fmaf(x, p, 1.38888888888888888889e-3f);
+    p = fmaf(x, p, 8.33333333333333333333e-3f);
+    p = fmaf(x, p, 4.16666666666666666667e-2f);
+    p = fmaf(x, p, 1.66666666666666666667e-1f);
+    p = fmaf(x, p, 0.5f);
+    p = fmaf(x, p, 1.0f);
+    p = fmaf(x, p, 1.0f);
+    uint32_t eb = (uint32_t)((int)n + 127) << 23;
+    float sc;
+    memcpy(&sc, &eb, 4);
+    return p * sc;
+}
+
+float orc_pow_unit(float x, float y) {
+    if (!(x > 0.0f)) return y == 0.0f ? 1.0f : 0.0f;
+    if (y == 0.0f) return 1.0f;
+    float t = y * orc_log2_pos(x);
+    return orc_exp2_nonpos(t > 0.0f ? 0.0f : t);
+}
+
+/* ---- metal and dielectric, rt/shading.mojo:30-260 ------------------------------ */
+#define TWO_PI_F 6.28318530717958647692f
+
+static orc_v3 orc_reflect(orc_v3 v, orc_v3 n) { return orc_sub(v, orc_scale(n, 2.0f * orc_dot(v, n))); }
+
+static orc_v3 orc_refract(orc_v3 uv, orc_v3 n, float eta) {
+    float cos_t = orc_fmin(orc_dot(orc_v(-uv.x, -uv.y, -uv.z), n), 1.0f);
+    orc_v3 perp = orc_scale(orc_add(uv, orc_scale(n, cos_t)), eta);
+    orc_v3 par = orc_scale(n, -sqrtf(fabsf(1.0f - orc_dot(perp, perp))));
+    return orc_add(perp, par);
+}
+
+float orc_reflectance(float cosine, float ref_idx) {
+    float root = (1.0f - ref_idx) / (1.0f + ref_idx);
+    float r2 = root * root;
+    float x = 1.0f - cosine;
+    float x2 = x * x;
+    float x5 = x2 * x2 * x;
+    return fmaf(1.0f - r2, x5, r2);
+}
+
+typedef struct { orc_v3 value; float pdf; int delta; } orc_eval;
+typedef struct { orc_v3 dir, weight; float pdf; int delta, ok; } orc_sample;
+
+/* _evaluate_metal, rt/shading.mojo:30-57 */
+static orc_eval eval_metal(orc_v3 ray_d, orc_v3 n, orc_v3 albedo, float fuzz, orc_v3 out_dir) {
+    int smooth = fuzz <= 1.0e-4f;
+    float sf = smooth ? 1.0f : fuzz;
+    orc_v3 dir = orc_normalize(out_dir);
+    orc_v3 refl = orc_normalize(orc_reflect(orc_normalize(ray_d), n));
+    int surface_valid = orc_dot(n, dir) > 0.0f;
+    float lobe_cos = orc_fmax(orc_dot(refl, dir), 0.0f);
+    int lobe_valid = lobe_cos > 0.0f;
+    float e = orc_fmax(2.0f / (sf * sf) - 2.0f, 0.0f);
+    float lobe = orc_pow_unit(lobe_cos, e);
+    int valid = !smooth && surface_valid && lobe_valid;
+    float vs = (e + 2.0f) * lobe / TWO_PI_F;
+    float pdf = (e + 1.0f) * lobe / TWO_PI_F;
+    orc_eval r = {valid ? orc_scale(albedo, vs) : orc_v(0, 0, 0), valid ? pdf : 0.0f, smooth};
+    return r;
+}
+
+/* _sample_metal, rt/shading.mojo:120-160 */
+static orc_sample sample_metal(orc_v3 ray_d, orc_v3 n, orc_v3 albedo, float fuzz, float ru, float rv) {
+    orc_v3 refl = orc_normalize(orc_reflect(orc_normalize(ray_d), n));
+    int smooth = fuzz <= 1.0e-4f;
+    float sf = smooth ? 1.0f : fuzz;
+    float e = orc_fmax(2.0f / (sf * sf) - 2.0f, 0.0f);
+    float cos_t = orc_pow_unit(ru, 1.0f / (e + 1.0f));
+    float sin_t = sqrtf(orc_fmax(1.0f - cos_t * cos_t, 0.0f));
+    float sp, cp;
+    orc_sincos_turns(rv, &sp, &cp);
+    orc_v3 helper = fabsf(refl.y) > 0.99f ? orc_v(1, 0, 0) : orc_v(0, 1, 0);
+    orc_v3 tangent = orc_normalize(orc_cross(helper, refl));
+    orc_v3 bitangent = orc_cross(refl, tangent);
+    orc_v3 rough = orc_normalize(orc_add(orc_add(orc_scale(tangent, cp * sin_t), orc_scale(bitangent, sp * sin_t)),
+                                         orc_scale(refl, cos_t)));
+    orc_v3 dir = smooth ? refl : rough;
+    orc_eval ev = eval_metal(ray_d, n, albedo, fuzz, dir);
+    float scos = orc_fmax(orc_dot(n, dir), 0.0f);
+    int rough_ok = ev.pdf > 0.0f && scos > 0.0f;
+    float safe_pdf = rough_ok ? ev.pdf : 1.0f;
+    float k = scos / safe_pdf;
+    orc_sample r;
+    r.dir = dir;
+    r.weight = smooth ? albedo : orc_scale(ev.value, k);
+    r.pdf = smooth ? 1.0f : ev.pdf;
+    r.delta = smooth;
+    r.ok = smooth ? orc_dot(refl, n) > 0.0f : rough_ok;
+    return r;
+}
+
+/* _sample_dielectric, rt/shading.mojo:163-196 */
+static orc_sample sample_dielectric(orc_v3 ray_d, orc_v3 n, float ior, int front, float ru) {
+    float ri = front ? 1.0f / ior : ior;
+    orc_v3 u = orc_normalize(ray_d);
+    float cos_t = orc_fmin(orc_dot(orc_v(-u.x, -u.y, -u.z), n), 1.0f);
+    float sin_t = sqrtf(orc_fmax(1.0f - cos_t * cos_t, 0.0f));
+    int cannot = ri * sin_t > 1.0f;
+    float rp = orc_reflectance(cos_t, ri);
+    int refl = cannot || rp > ru;
+    orc_sample r;
+    r.dir = orc_normalize(refl ? orc_reflect(u, n) : orc_refract(u, n, ri));
+    r.weight = orc_v(1, 1, 1);
+    r.pdf = refl ? (cannot ? 1.0f : rp) : 1.0f - rp;
+    r.delta = 1;
+    r.ok = 1;
+    return r;
+}
+
+/* Test hooks: out = dir.xyz, weight.xyz, pdf, delta, ok (9 floats). */
+void orc_sample_metal(const float *ray_d, const float *n, const float *albedo, float fuzz, float ru, float rv,
+                      float *out) {
+    orc_sample r = sample_metal(orc_v(ray_d[0], ray_d[1], ray_d[2]), orc_v(n[0], n[1], n[2]),
+                                orc_v(albedo[0], albedo[1], albedo[2]), fuzz, ru, rv);
+    float o[9] = {r.dir.x, r.dir.y, r.dir.z, r.weight.x, r.weight.y, r.weight.z, r.pdf, (float)r.delta, (float)r.ok};
+    memcpy(out, o, sizeof(o));
+}
+void orc_eval_metal(const float *ray_d, const float *n, const float *albedo, float fuzz, const float *out_dir,
+                    float *out) {
+    orc_eval r = eval_metal(orc_v(ray_d[0], ray_d[1], ray_d[2]), orc_v(n[0], n[1], n[2]),
+                            orc_v(albedo[0], albedo[1], albedo[2]), fuzz, orc_v(out_dir[0], out_dir[1], out_dir[2]));
+    float o[5] = {r.value.x, r.value.y, r.value.z, r.pdf, (float)r.delta};
+    memcpy(out, o, sizeof(o));
+}
+void orc_sample_dielectric(const float *ray_d, const float *n, float ior, int front, float ru, float *out) {
+    orc_sample r = sample_dielectric(orc_v(ray_d[0], ray_d[1], ray_d[2]), orc_v(n[0], n[1], n[2]), ior, front, ru);
+    float o[9] = {r.dir.x, r.dir.y, r.dir.z, r.weight.x, r.weight.y, r.weight.z, r.pdf, (float)r.delta, (float)r.ok};
+    memcpy(out, o, sizeof(o));
+}
+void orc_reflect3(const float *v, const float *n, float *out) {
+    orc_v3 r = orc_reflect(orc_v(v[0], v[1], v[2]), orc_v(n[0], n[1], n[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
 /* ---- light table ------------------------------------------------------------- */
 #define L_STRIDE 14
 #define MAT_LAMB 0u
+#define MAT_METAL 1u
+#define MAT_DIEL 2u
 #define MAT_EMIT 3u
 #define IDX_MASK ((1u << 28) - 1u)
 
@@ -181,7 +348,7 @@ void orc_light_table(const float *v, uint64_t n, const uint32_t *surf, const flo
 typedef struct {
     orc_bvh bvh;
     const uint32_t *surf;
-    const float *lamb, *emit;
+    const float *lamb, *metal, *diel, *emit;
     lights_t lights;
     orc_render_settings st;
     float cam[20];
@@ -202,7 +369,8 @@ static void acc(float *s, orc_v3 c) { s[0] += c.x; s[1] += c.y; s[2] += c.z; }
 
 /* Returns 1 with a shadow ray + contribution, rt/gpu/path_shading.mojo:261-403. */
 static int direct_light(const scene_t *sc, uint32_t path_id, uint32_t bounce, orc_v3 point, orc_v3 normal,
-                        orc_v3 albedo, orc_v3 T, orc_v3 *dir_out, float *tmax_out, orc_v3 *contrib) {
+                        uint32_t kind, uint32_t mi, orc_v3 ray_d, orc_v3 T, orc_v3 *dir_out, float *tmax_out,
+                        orc_v3 *contrib) {
     const lights_t *L = &sc->lights;
     if (L->count == 0 || !(L->total > 0.0f)) return 0;
     orc_rng rng;
@@ -237,8 +405,18 @@ static int direct_light(const scene_t *sc, uint32_t path_id, uint32_t bounce, or
     if (tmax <= 0.001f) return 0;
     orc_v3 E = orc_v(f[10], f[11], f[12]);
     float light_pdf = d2 * (E.x + E.y + E.z) / (3.0f * lcos * L->total);
-    orc_v3 value = orc_scale(albedo, 0.3183098861837907f);
-    float bsdf_pdf = orc_fmax(orc_dot(normal, orc_normalize(dir)), 0.0f) * 0.3183098861837907f;
+    orc_v3 value;
+    float bsdf_pdf;
+    if (kind == MAT_METAL) {
+        const float *m = sc->metal + 4 * (size_t)mi;
+        orc_eval ev = eval_metal(ray_d, normal, orc_v(m[0], m[1], m[2]), m[3], dir);
+        value = ev.value;
+        bsdf_pdf = ev.pdf;
+    } else {
+        const float *a = sc->lamb + 3 * (size_t)mi;
+        value = orc_scale(orc_v(a[0], a[1], a[2]), 0.3183098861837907f);
+        bsdf_pdf = orc_fmax(orc_dot(normal, orc_normalize(dir)), 0.0f) * 0.3183098861837907f;
+    }
     int ok = light_pdf > 0.0f && bsdf_pdf > 0.0f;
     float w = sc->st.algorithm == 4 ? power_heuristic(light_pdf, bsdf_pdf) : 1.0f;
     float s = ok ? scos * w / light_pdf : 0.0f;
@@ -286,6 +464,7 @@ static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint
         float sh_tmax = 0.0f;
         orc_v3 no = orc_v(0, 0, 0), nd = orc_v(0, 0, 0), nT = T;
         float npdf = 0.0f;
+        int ndelta = 0;
         if (!found) {
             if (alg == 0 || alg == 3 || alg == 4) acc(sample, orc_mul(T, sky(d)));
         } else {
@@ -308,8 +487,8 @@ static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint
             } else {
                 orc_v3 albedo = orc_v(0, 0, 0);
                 if (kind == MAT_LAMB) albedo = orc_v(sc->lamb[3 * mi], sc->lamb[3 * mi + 1], sc->lamb[3 * mi + 2]);
-                if ((alg == 3 || alg == 4) && kind == MAT_LAMB) {
-                    shadow = direct_light(sc, path_id, b, point, n, albedo, T, &sh_d, &sh_tmax, &sh_c);
+                if ((alg == 3 || alg == 4) && (kind == MAT_LAMB || kind == MAT_METAL)) {
+                    shadow = direct_light(sc, path_id, b, point, n, kind, mi, d, T, &sh_d, &sh_tmax, &sh_c);
                     sh_o = point;
                 }
                 if (kind == MAT_EMIT) {
@@ -356,7 +535,46 @@ static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint
                     if (alive) {
                         next = 1;
                         no = orc_v(o.x + h.t * d.x, o.y + h.t * d.y, o.z + h.t * d.z);
-                        nd = dir; nT = Tn; npdf = pdf;
+                        nd = dir; nT = Tn; npdf = pdf; ndelta = 0;
+                    }
+                } else if (kind == MAT_METAL || kind == MAT_DIEL) {
+                    /* rt/gpu/path_shading.mojo:669-783: numbers drawn only where used */
+                    orc_rng br;
+                    rng_init(&br, st->seed, path_id, stage_bsdf(b));
+                    orc_sample smp;
+                    if (kind == MAT_METAL) {
+                        const float *m = sc->metal + 4 * (size_t)mi;
+                        float ru = 0.0f, rv = 0.0f;
+                        if (m[3] > 1.0e-4f) { ru = rng_next(&br); rv = rng_next(&br); }
+                        smp = sample_metal(d, n, orc_v(m[0], m[1], m[2]), m[3], ru, rv);
+                    } else {
+                        float ior = sc->diel[mi];
+                        float ri = front ? 1.0f / ior : ior;
+                        orc_v3 u = orc_normalize(d);
+                        float cos_t = orc_fmin(orc_dot(orc_v(-u.x, -u.y, -u.z), n), 1.0f);
+                        float sin_t = sqrtf(orc_fmax(1.0f - cos_t * cos_t, 0.0f));
+                        float ru = ri * sin_t <= 1.0f ? rng_next(&br) : 0.0f;
+                        smp = sample_dielectric(d, n, ior, front, ru);
+                    }
+                    if (smp.ok) {
+                        orc_v3 Tn = orc_mul(T, smp.weight);
+                        int alive = 1;
+                        if (b + 1u >= 5u) {
+                            float mx = orc_fmax(Tn.x, orc_fmax(Tn.y, Tn.z));
+                            if (mx <= 0.0f) alive = 0;
+                            else {
+                                float survival = orc_fmin(orc_fmax(mx, 0.05f), 0.95f);
+                                orc_rng rr2;
+                                rng_init(&rr2, st->seed, path_id, stage_roulette(b));
+                                if (rng_next(&rr2) >= survival) alive = 0;
+                                else Tn = orc_v(Tn.x / survival, Tn.y / survival, Tn.z / survival);
+                            }
+                        }
+                        if (alive) {
+                            next = 1;
+                            no = orc_v(o.x + h.t * d.x, o.y + h.t * d.y, o.z + h.t * d.z);
+                            nd = smp.dir; nT = Tn; npdf = smp.pdf; ndelta = smp.delta;
+                        }
                     }
                 }
             }
@@ -369,7 +587,7 @@ static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint
             else if (!occluded) acc(sample, sh_c);
         }
         if (!next) break;
-        o = no; d = nd; T = nT; path_pdf = npdf; tmin = 0.001f; delta = 0;
+        o = no; d = nd; T = nT; path_pdf = npdf; tmin = 0.001f; delta = ndelta;
     }
 }
 
@@ -382,8 +600,8 @@ static void *render_worker(void *p) {
 }
 
 int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
-               const float *emissive, const orc_render_settings *st, const float *cam20, float *pixels,
-               uint64_t *ray_counts, int threads) {
+               const float *metal, const float *dielectric, const float *emissive, const orc_render_settings *st,
+               const float *cam20, float *pixels, uint64_t *ray_counts, int threads) {
     uint64_t m = n > 1 ? n - 1 : 1;
     uint64_t *keys = (uint64_t *)malloc(n * sizeof(uint64_t));
     uint32_t *ids = (uint32_t *)malloc(n * sizeof(uint32_t));
@@ -427,7 +645,7 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
         sc.bvh.wide = wide;
         sc.bvh.wide_count = wc;
     }
-    sc.surf = surfaces; sc.lamb = lambertian; sc.emit = emissive; sc.st = *st;
+    sc.surf = surfaces; sc.lamb = lambertian; sc.metal = metal; sc.diel = dielectric; sc.emit = emissive; sc.st = *st;
     memcpy(sc.cam, cam20, sizeof(sc.cam));
     build_lights(vertices, n, surfaces, emissive, &sc.lights);
     uint64_t total = (uint64_t)st->width * st->height * st->spp;
@@ -476,10 +694,70 @@ static d3 dcross(d3 a, d3 b) { return D(a.y * b.z - a.z * b.y, a.z * b.x - a.x *
 static d3 dnorm(d3 a) { double l = sqrt(ddot(a, a)); return l > 0 ? dsc(a, 1.0 / l) : a; }
 
 typedef struct {
-    const float *v; uint64_t n; const uint32_t *surf; const float *lamb, *emit;
+    const float *v; uint64_t n; const uint32_t *surf; const float *lamb, *metal, *diel, *emit;
     double *lw; uint32_t *ltri; uint32_t nl; double total;
     orc_render_settings st; const float *cam;
 } scene64;
+
+/* Russian roulette from depth 5 in double (survival max(T) in [0.05, 0.95]). */
+static int roulette64(pcg_t *rng, uint32_t b, d3 *T) {
+    if (b + 1 < 5) return 1;
+    double mx = fmax(T->x, fmax(T->y, T->z));
+    if (mx <= 0) return 0;
+    double surv = fmin(fmax(mx, 0.05), 0.95);
+    if (pcg_d(rng) >= surv) return 0;
+    *T = dsc(*T, 1.0 / surv);
+    return 1;
+}
+
+/* Metal and dielectric scattering in double precision, written from the
+ * definitions (not from the FP32 code): a Phong lobe of exponent
+ * e = 2/fuzz^2 - 2 around the mirror direction sampled by cos = u^(1/(e+1)),
+ * weight = albedo (e+2)/(e+1) cos_n (a mirror below fuzz 1e-4: weight albedo);
+ * glass: Schlick's reflectance chooses reflection, else Snell refraction,
+ * weight 1.  Returns 0 when the path is absorbed. */
+static int scatter64(const scene64 *s, uint32_t kind, uint32_t mi, d3 d, d3 n, int front, double u1, double u2,
+                     d3 *dir, d3 *weight) {
+    d3 u = dnorm(d);
+    double un = ddot(u, n);
+    d3 refl = dnorm(dsub(u, dsc(n, 2 * un)));
+    if (kind == MAT_METAL) {
+        const float *m = s->metal + 4 * (size_t)mi;
+        d3 alb = D(m[0], m[1], m[2]);
+        double fuzz = m[3];
+        if (fuzz <= 1.0e-4) {
+            if (ddot(refl, n) <= 0) return 0;
+            *dir = refl;
+            *weight = alb;
+            return 1;
+        }
+        double e = fmax(2.0 / (fuzz * fuzz) - 2.0, 0.0);
+        double ct = pow(u1, 1.0 / (e + 1.0)), st = sqrt(fmax(1 - ct * ct, 0.0)), ph = 2 * M_PI * u2;
+        d3 helper = fabs(refl.y) > 0.99 ? D(1, 0, 0) : D(0, 1, 0);
+        d3 t = dnorm(dcross(helper, refl)), bt = dcross(refl, t);
+        d3 w = dnorm(dadd(dadd(dsc(t, cos(ph) * st), dsc(bt, sin(ph) * st)), dsc(refl, ct)));
+        double cn = ddot(n, w);
+        if (cn <= 0 || ddot(refl, w) <= 0) return 0;
+        *dir = w;
+        *weight = dsc(alb, (e + 2.0) / (e + 1.0) * cn);
+        return 1;
+    }
+    double ior = s->diel[mi];
+    double ri = front ? 1.0 / ior : ior;
+    double ci = fmin(-un, 1.0), si = sqrt(fmax(1 - ci * ci, 0.0));
+    double r0 = (1 - ri) / (1 + ri);
+    r0 *= r0;
+    double R = r0 + (1 - r0) * pow(1 - ci, 5);
+    if (ri * si > 1.0 || R > u1) {
+        *dir = refl;
+    } else {
+        d3 perp = dsc(dadd(u, dsc(n, ci)), ri);
+        d3 par = dsc(n, -sqrt(fabs(1 - ddot(perp, perp))));
+        *dir = dnorm(dadd(perp, par));
+    }
+    *weight = D(1, 1, 1);
+    return 1;
+}
 
 /* brute-force closest hit in double (eager MT, t in (tmin, tmax)) */
 static int hit64(const scene64 *s, d3 o, d3 d, double tmin, double tmax, double *t_out, uint64_t *prim, d3 *nrm) {
@@ -556,6 +834,15 @@ static void render_path64(const scene64 *s, uint64_t path_id, pcg_t *rng, double
             }
             return;
         }
+        if (kind == MAT_METAL || kind == MAT_DIEL) {
+            double u1 = pcg_d(rng), u2 = pcg_d(rng);
+            d3 nd, wgt;
+            if (!scatter64(s, kind, mi, d, n, front, u1, u2, &nd, &wgt)) return;
+            T = dmul(T, wgt);
+            if (!roulette64(rng, b, &T)) return;
+            o = p; d = nd;
+            continue;
+        }
         d3 alb = D(s->lamb[3 * mi], s->lamb[3 * mi + 1], s->lamb[3 * mi + 2]);
         if (alg == 3 && s->nl > 0) {
             /* light chosen by power, point uniform on the triangle */
@@ -589,13 +876,7 @@ static void render_path64(const scene64 *s, uint64_t path_id, pcg_t *rng, double
         d3 sd = dadd(n, D(r * cos(th), r * sin(th), z));
         if (ddot(sd, sd) < 1e-20) sd = n;
         T = dmul(T, alb);
-        if (b + 1 >= 5) {
-            double mx = fmax(T.x, fmax(T.y, T.z));
-            if (mx <= 0) return;
-            double surv = fmin(fmax(mx, 0.05), 0.95);
-            if (pcg_d(rng) >= surv) return;
-            T = dsc(T, 1.0 / surv);
-        }
+        if (!roulette64(rng, b, &T)) return;
         o = p; d = dnorm(sd);
     }
 }
@@ -656,11 +937,12 @@ void orc_ao_rays(const float *rays, const float *hits, uint64_t n, uint32_t samp
     }
 }
 
-/* FP64 estimate of the PATH (0), NORMALS (1), AO (2) or NEE (3) image. */
+/* FP64 estimate of the PATH (0), NORMALS (1), AO (2) or NEE (3) image; scenes with metal or dielectric
+ * surfaces are estimated with PATH (its NEE handles Lambertian surfaces only). */
 int orc_render_fp64(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
-                    const float *emissive, const orc_render_settings *st, const float *cam20, double *pixels,
-                    int threads) {
-    scene64 s = {vertices, n, surfaces, lambertian, emissive, NULL, NULL, 0, 0.0, *st, cam20};
+                    const float *metal, const float *dielectric, const float *emissive,
+                    const orc_render_settings *st, const float *cam20, double *pixels, int threads) {
+    scene64 s = {vertices, n, surfaces, lambertian, metal, dielectric, emissive, NULL, NULL, 0, 0.0, *st, cam20};
     s.lw = (double *)malloc((n ? n : 1) * sizeof(double));
     s.ltri = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
     for (uint64_t i = 0; i < n; ++i) {

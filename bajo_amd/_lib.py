@@ -150,6 +150,10 @@ SIGNATURES = {
         c_int32,
         [c_void_p, c_void_p, c_uint64, POINTER(BuildOptions), POINTER(BuildTimings), POINTER(c_void_p)],
     ),
+    "bajo_build_binary_bounds": (
+        c_int32,
+        [c_void_p, c_void_p, c_uint64, POINTER(BuildOptions), POINTER(BuildTimings), POINTER(c_void_p)],
+    ),
     "bajo_bvh_free": (c_int32, [c_void_p]),
     "bajo_bvh_get_info": (c_int32, [c_void_p, POINTER(BvhInfo)]),
     "bajo_bvh_copy_binary": (

@@ -242,6 +242,52 @@ def build_triangle_bvh(
     return GpuTriangleBvh(handle, ctx, n, node_width, leaf_size, method)
 
 
+class GpuBinaryBoundsBvh(GpuTriangleBvh):
+    """A binary (or collapsed 8-wide) hierarchy over caller boxes: bajo's GpuBinaryBoundsBvh
+    (bvh/gpu/builder/binary_layout.mojo:303-448).  `binary()` exports bajo's layout; its leaf_ids index the
+    input boxes, through which bajo's payloads are looked up."""
+
+    def leaf_payloads(self, payloads: np.ndarray) -> np.ndarray:
+        """The payload of each sorted leaf."""
+        return np.asarray(payloads)[self.binary()["leaf_ids"]]
+
+    def rebuild(self, leaf_bounds: torch.Tensor, timings: bool = False):
+        n = _check_bounds(leaf_bounds, self.ctx)
+        t = _lib.BuildTimings() if timings else None
+        handle = ctypes.c_void_p(self.handle.value)
+        opts = build_options(self.node_width, self.leaf_size, self.method)
+        _lib.check(_lib.lib().bajo_build_binary_bounds(self.ctx.handle, ptr(leaf_bounds), n, ctypes.byref(opts),
+                                                       ctypes.byref(t) if t else None, ctypes.byref(handle)))
+        self._handle = handle
+        self.prim_count = n
+        return t.as_dict() if t else None
+
+
+def _check_bounds(leaf_bounds: torch.Tensor, ctx: DeviceContext) -> int:
+    require_device(leaf_bounds, ctx, "leaf_bounds")
+    if leaf_bounds.dtype != torch.float32:
+        raise ValueError("leaf bounds must be float32")
+    if leaf_bounds.numel() % 6 != 0 or leaf_bounds.numel() == 0:
+        raise ValueError("leaf bounds hold six floats (lo.xyz, hi.xyz) per leaf")
+    return leaf_bounds.numel() // 6
+
+
+def build_binary_bvh(leaf_bounds: torch.Tensor, ctx: DeviceContext | None = None, method: int = GpuBvhBuildMethod.LBVH,
+                     timings: dict | None = None, node_width: int = 2, leaf_size: int = 0) -> GpuBinaryBoundsBvh:
+    """build_binary_bvh[method] over GpuBinaryBoundsBvh(ctx, leaf_bounds, ...)
+    (bvh/gpu/builder/binary_builder.mojo:21-45): the LBVH or H-PLOC builder over boxes f32[6N]."""
+    ctx = ctx or default_context(leaf_bounds.device.index or 0)
+    n = _check_bounds(leaf_bounds, ctx)
+    t = _lib.BuildTimings() if timings is not None else None
+    handle = ctypes.c_void_p()
+    opts = build_options(node_width, leaf_size, method)
+    _lib.check(_lib.lib().bajo_build_binary_bounds(ctx.handle, ptr(leaf_bounds), n, ctypes.byref(opts),
+                                                   ctypes.byref(t) if t else None, ctypes.byref(handle)))
+    if t is not None:
+        timings.update(t.as_dict())
+    return GpuBinaryBoundsBvh(handle, ctx, n, node_width, leaf_size, method)
+
+
 def build_triangle_bvh_measured(vertices: torch.Tensor, ctx: DeviceContext | None = None, node_width: int = 2,
                                 leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH):
     """build_triangle_bvh_measured (bvh/gpu/triangle_bvh.mojo:633-646): returns (bvh, timings)."""

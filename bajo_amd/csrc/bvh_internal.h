@@ -75,13 +75,15 @@ struct bajo_bvh {
   uint32_t* hp_ctl;       // node counter, status, root
   uint32_t* host_hp_ctl;  // pinned host mirror of hp_ctl
   uint64_t hp_capacity;
+  // input kind: triangles f32[9N] (traceable) or leaf boxes f32[6N]
+  bool boxes;
 };
 
 namespace bj {
 // Collapses h's binary hierarchy into h->wide (8-wide quantized nodes); wide.hip.
 int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h);
 // Builds h's binary hierarchy with H-PLOC over the sorted keys; hploc.hip.
-int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices);
+int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* src);
 // Karras topology over sorted 64-bit keys (lbvh.hip); `range_first` may be NULL.
 __global__ void karras_kernel(const uint64_t* __restrict__ keys, uint64_t n_u, BjNode* __restrict__ nodes,
                               uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ leaf_counts,
@@ -92,5 +94,21 @@ BJ_D void load_tri_vertices(const float* __restrict__ v, uint64_t i, v3& a, v3& 
   a = mk(t[0], t[1], t[2]);
   b = mk(t[3], t[4], t[5]);
   c = mk(t[6], t[7], t[8]);
+}
+// Leaf i's box: the box of triangle i's vertices (bvh/gpu/triangle_bvh.mojo:
+// 711-712, vmin(vmin(v0, v1), v2)) or, for a box input, the six floats
+// lo.xyz, hi.xyz of bajo's leaf_bounds (bvh/gpu/builder/binary_layout.mojo).
+template <bool kBoxes>
+BJ_D void leaf_box(const float* __restrict__ src, uint64_t i, v3& lo, v3& hi) {
+  if constexpr (kBoxes) {
+    const float* b = src + 6 * i;
+    lo = mk(b[0], b[1], b[2]);
+    hi = mk(b[3], b[4], b[5]);
+  } else {
+    v3 a, b, c;
+    load_tri_vertices(src, i, a, b, c);
+    lo = vmin(vmin(a, b), c);
+    hi = vmax(vmax(a, b), c);
+  }
 }
 }  // namespace bj

@@ -276,6 +276,7 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
   publish_fence();
 }
 
+template <bool kBoxes>
 __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict__ vertices,
                                                          const uint64_t* __restrict__ keys, uint64_t n,
                                                          const BjNode* __restrict__ guide,
@@ -292,10 +293,8 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
   uint32_t cur = kLeafFlag | (uint32_t)i;
   if (active) {
     const uint32_t prim = (uint32_t)keys[i];
-    v3 a, b, c;
-    load_tri_vertices(vertices, prim, a, b, c);
-    // bvh/gpu/triangle_bvh.mojo:711-712: leaf box = vmin(vmin(v0,v1),v2)
-    const v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
+    v3 lo, hi;
+    leaf_box<kBoxes>(vertices, prim, lo, hi);
     HpEntry e{kLeafFlag | (uint32_t)i, 1u, {lo.x, lo.y, lo.z}, {hi.x, hi.y, hi.z}};
     store_entry(clusters + i, e);
     store_wt(ids + i, (uint32_t)i);
@@ -409,6 +408,7 @@ __global__ void __launch_bounds__(kHpBlock) dfs_position_kernel(BjNode* nodes, u
 
 // Moves each leaf to its depth-first position: triangle record, key, parent
 // link, and the parent's child reference.
+template <bool kBoxes>
 __global__ void __launch_bounds__(kHpBlock) dfs_emit_kernel(const float* __restrict__ vertices,
                                                             const uint64_t* __restrict__ keys, uint64_t n,
                                                             const uint32_t* __restrict__ pos_in,
@@ -423,13 +423,15 @@ __global__ void __launch_bounds__(kHpBlock) dfs_emit_kernel(const float* __restr
   if ((uint64_t)pos >= n) return;
   const uint64_t key = keys[j];
   const uint32_t prim = (uint32_t)key;
-  v3 a, b, c;
-  load_tri_vertices(vertices, prim, a, b, c);
-  const v3 e1 = sub(b, a), e2 = sub(c, a);
-  float4* out = reinterpret_cast<float4*>(tris + pos);
-  out[0] = make_float4(a.x, a.y, a.z, __builtin_bit_cast(float, prim));
-  out[1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
-  out[2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+  if constexpr (!kBoxes) {
+    v3 a, b, c;
+    load_tri_vertices(vertices, prim, a, b, c);
+    const v3 e1 = sub(b, a), e2 = sub(c, a);
+    float4* out = reinterpret_cast<float4*>(tris + pos);
+    out[0] = make_float4(a.x, a.y, a.z, __builtin_bit_cast(float, prim));
+    out[1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+    out[2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+  }
   keys_out[pos] = key;
   const uint32_t parent = leaf_parent[j];
   leaf_parent_out[pos] = parent;
@@ -450,9 +452,14 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
   karras_kernel<<<div_up(n - 1, kHpBlock), kHpBlock, 0, st>>>(h->keys, n, h->hp_guide, h->leaf_parent,
                                                                h->leaf_counts, h->flags, h->hp_first);
   // hp_pos holds the clusters' lowest leaf positions until the depth-first pass
-  hploc_kernel<<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
-                                                         h->leaf_parent, h->flags, clusters, h->hp_pos, h->hp_counts,
-                                                         h->nodes, h->leaf_counts, h->hp_ctl);
+  if (h->boxes)
+    hploc_kernel<true><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
+                                                                 h->leaf_parent, h->flags, clusters, h->hp_pos,
+                                                                 h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl);
+  else
+    hploc_kernel<false><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
+                                                                  h->leaf_parent, h->flags, clusters, h->hp_pos,
+                                                                  h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl);
   hploc_root_kernel<<<1, 64, 0, st>>>(h->nodes, n, h->leaf_counts, h->hp_ctl);
   BJ_HIP(hipGetLastError());
   // the cluster area is free now: it holds the reordered keys and leaf parents
@@ -460,8 +467,13 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
   uint32_t* parent_dfs = reinterpret_cast<uint32_t*>(keys_dfs + n);
   dfs_position_kernel<<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(h->nodes, n, h->leaf_parent,
                                                                 h->leaf_counts, h->hp_pos);
-  dfs_emit_kernel<<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_pos, h->leaf_parent,
-                                                            h->nodes, h->tris, keys_dfs, parent_dfs);
+  if (h->boxes)
+    dfs_emit_kernel<true><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_pos, h->leaf_parent,
+                                                                    h->nodes, h->tris, keys_dfs, parent_dfs);
+  else
+    dfs_emit_kernel<false><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_pos,
+                                                                     h->leaf_parent, h->nodes, h->tris, keys_dfs,
+                                                                     parent_dfs);
   BJ_HIP(hipGetLastError());
   BJ_HIP(hipMemcpyAsync(h->keys, keys_dfs, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
   BJ_HIP(hipMemcpyAsync(h->leaf_parent, parent_dfs, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));

@@ -49,6 +49,21 @@ BJ_D void load_tri(const float* __restrict__ v, uint64_t i, v3& a, v3& b, v3& c)
   c = mk(ld(t + 6), ld(t + 7), ld(t + 8));
 }
 
+// Leaf box with streaming (non-temporal) loads for the one-pass kernels.
+template <bool kBoxes>
+BJ_D void leaf_src(const float* __restrict__ src, uint64_t i, v3& lo, v3& hi) {
+  if constexpr (kBoxes) {
+    const float* b = src + 6 * i;
+    lo = mk(ld(b + 0), ld(b + 1), ld(b + 2));
+    hi = mk(ld(b + 3), ld(b + 4), ld(b + 5));
+  } else {
+    v3 a, b, c;
+    load_tri(src, i, a, b, c);
+    lo = vmin(vmin(a, b), c);
+    hi = vmax(vmax(a, b), c);
+  }
+}
+
 __global__ void __launch_bounds__(256) lbvh_init_kernel(uint32_t* acc, uint32_t* hist, uint32_t* tickets) {
   uint32_t t = threadIdx.x;
   if (t < 12) {
@@ -72,6 +87,7 @@ BJ_D float wave_max(float x) {
   return x;
 }
 
+template <bool kBoxes>
 __global__ void __launch_bounds__(kBlock) scene_bounds_kernel(const float* __restrict__ vertices, uint64_t n,
                                                               uint32_t* __restrict__ acc) {
   float r[12];
@@ -79,9 +95,8 @@ __global__ void __launch_bounds__(kBlock) scene_bounds_kernel(const float* __res
   for (int k = 0; k < 12; ++k) r[k] = ((k % 6) < 3) ? kFltMax : -kFltMax;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    v3 a, b, c;
-    load_tri(vertices, i, a, b, c);
-    v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
+    v3 lo, hi;
+    leaf_src<kBoxes>(vertices, i, lo, hi);
     v3 ce = scale(add(lo, hi), 0.5f);
     r[0] = fmin_sel(r[0], lo.x); r[1] = fmin_sel(r[1], lo.y); r[2] = fmin_sel(r[2], lo.z);
     r[3] = fmax_sel(r[3], hi.x); r[4] = fmax_sel(r[4], hi.y); r[5] = fmax_sel(r[5], hi.z);
@@ -106,6 +121,7 @@ __global__ void __launch_bounds__(kBlock) scene_bounds_kernel(const float* __res
   }
 }
 
+template <bool kBoxes>
 __global__ void __launch_bounds__(kBlock) morton_kernel(const float* __restrict__ vertices, uint64_t n,
                                                         const uint32_t* __restrict__ acc, float* __restrict__ scene,
                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ hist,
@@ -123,9 +139,8 @@ __global__ void __launch_bounds__(kBlock) morton_kernel(const float* __restrict_
   const v3 inv = safe_inv(sub(cmax, cmin));
   __syncthreads();
   for (uint64_t i = gtid; i < n; i += gstride) {
-    v3 a, b, c;
-    load_tri(vertices, i, a, b, c);
-    v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
+    v3 lo, hi;
+    leaf_src<kBoxes>(vertices, i, lo, hi);
     v3 q = mul(sub(scale(add(lo, hi), 0.5f), cmin), inv);
     uint32_t code = morton3(q.x, q.y, q.z);
     keys[i] = ((uint64_t)code << 32) | (uint64_t)(uint32_t)i;
@@ -217,6 +232,7 @@ namespace {
 // child boxes in shared memory; the second arriver writes the node's two
 // child boxes with plain stores).  Only nodes whose range crosses a block
 // boundary use the cross-XCD hand-off through global memory.
+template <bool kBoxes>
 __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__ vertices,
                                                        const uint64_t* __restrict__ keys, uint64_t n,
                                                        BjNode* __restrict__ nodes,
@@ -230,21 +246,27 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t prim = (uint32_t)keys[s];
-  v3 a, b, c;
-  load_tri(vertices, prim, a, b, c);
-  v3 e1 = sub(b, a), e2 = sub(c, a);
-  float4* out = reinterpret_cast<float4*>(tris + s);
-  out[0] = make_float4(a.x, a.y, a.z, __builtin_bit_cast(float, prim));
-  out[1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
-  out[2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+  v3 lo, hi;
+  if constexpr (kBoxes) {
+    leaf_src<true>(vertices, prim, lo, hi);
+  } else {
+    v3 a, b, c;
+    load_tri(vertices, prim, a, b, c);
+    v3 e1 = sub(b, a), e2 = sub(c, a);
+    float4* out = reinterpret_cast<float4*>(tris + s);
+    out[0] = make_float4(a.x, a.y, a.z, __builtin_bit_cast(float, prim));
+    out[1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+    out[2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    // bvh/gpu/triangle_bvh.mojo:711-712: leaf box = vmin(vmin(v0,v1),v2).
+    lo = vmin(vmin(a, b), c);
+    hi = vmax(vmax(a, b), c);
+  }
   if (n < 2) {
     leaf_parent[0] = kSentinel;  // single-leaf segment, lbvh.mojo:246-248
     return;
   }
   const uint64_t block_first = (uint64_t)blockIdx.x * blockDim.x;
   const uint64_t block_last = block_first + blockDim.x - 1 < n - 1 ? block_first + blockDim.x - 1 : n - 1;
-  // bvh/gpu/triangle_bvh.mojo:711-712: leaf box = vmin(vmin(v0,v1),v2).
-  v3 lo = vmin(vmin(a, b), c), hi = vmax(vmax(a, b), c);
   uint32_t cur = kLeafFlag | (uint32_t)s;
   uint32_t parent = leaf_parent[s];
   // A Karras hierarchy over 64-bit keys is at most 64 levels deep; the index
@@ -445,11 +467,16 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
 
   ev.record(0, st);
   lbvh_init_kernel<<<1, 256, 0, st>>>(h->acc, hist, tickets);
-  scene_bounds_kernel<<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->acc);
+  if (h->boxes) scene_bounds_kernel<true><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->acc);
+  else scene_bounds_kernel<false><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->acc);
   BJ_HIP(hipGetLastError());
   ev.record(1, st);
-  morton_kernel<<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->acc, h->scene, h->keys, hist, lookback,
-                                                  lookback_words);
+  if (h->boxes)
+    morton_kernel<true><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->acc, h->scene, h->keys, hist, lookback,
+                                                          lookback_words);
+  else
+    morton_kernel<false><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->acc, h->scene, h->keys, hist, lookback,
+                                                           lookback_words);
   BJ_HIP(hipGetLastError());
   ev.record(2, st);
   rc = onesweep_run(ctx, plan, h->keys, nullptr, ws, true);
@@ -467,8 +494,12 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
       BJ_HIP(hipGetLastError());
     }
     ev.record(4, st);
-    refit_kernel<<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent,
-                                                       h->leaf_counts, h->flags, h->tris);
+    if (h->boxes)
+      refit_kernel<true><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent,
+                                                               h->leaf_counts, h->flags, h->tris);
+    else
+      refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent,
+                                                                h->leaf_counts, h->flags, h->tris);
     BJ_HIP(hipGetLastError());
   }
   ev.record(5, st);
@@ -497,6 +528,9 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
 }
 
 }  // namespace
+
+int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo_build_options* options,
+                    bajo_build_timings* timings, bajo_bvh** out, bool boxes);
 }  // namespace bj
 
 extern "C" {
@@ -510,22 +544,38 @@ int32_t bajo_build_triangle_bvh(bajo_ctx* ctx, const float* vertices, uint64_t n
 int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_t n,
                                    const bajo_build_options* options, bajo_build_timings* timings,
                                    bajo_bvh** out) {
+  return bj::build_entry(ctx, vertices, n, options, timings, out, false);
+}
+
+int32_t bajo_build_binary_bounds(bajo_ctx* ctx, const float* leaf_bounds, uint64_t n,
+                                 const bajo_build_options* options, bajo_build_timings* timings, bajo_bvh** out) {
+  return bj::build_entry(ctx, leaf_bounds, n, options, timings, out, true);
+}
+
+}  // extern "C"
+
+namespace bj {
+
+int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo_build_options* options,
+                    bajo_build_timings* timings, bajo_bvh** out, bool boxes) {
+  const char* fn = boxes ? "bajo_build_binary_bounds" : "bajo_build_triangle_bvh";
+  const char* items = boxes ? "boxes" : "triangles";
   bajo_build_options opt = options ? *options : bajo_build_options{BAJO_BUILD_LBVH, BAJO_WIDTH_BINARY, 0, 0};
-  BJ_REQUIRE(ctx != nullptr && out != nullptr, "bajo_build_triangle_bvh: ctx/out is NULL");
+  BJ_REQUIRE(ctx != nullptr && out != nullptr, "%s: ctx/out is NULL", fn);
   BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH || opt.method == BAJO_BUILD_HPLOC,
-             "bajo_build_triangle_bvh: unsupported build method %d", opt.method);
+             "%s: unsupported build method %d", fn, opt.method);
   BJ_REQUIRE(opt.node_width == BAJO_WIDTH_BINARY || opt.node_width == BAJO_WIDTH_WIDE8,
-             "bajo_build_triangle_bvh: node_width must be 2 or 8, got %d", opt.node_width);
+             "%s: node_width must be 2 or 8, got %d", fn, opt.node_width);
   if (opt.leaf_size == 0) opt.leaf_size = (int32_t)kWideDefaultLeaf;
   BJ_REQUIRE(opt.leaf_size >= 1 && opt.leaf_size <= (int32_t)kWideMaxLeaf,
-             "bajo_build_triangle_bvh: leaf_size must be in [1, 8], got %d", opt.leaf_size);
-  BJ_REQUIRE(opt.reserved == 0, "bajo_build_triangle_bvh: reserved option field must be 0");
-  BJ_REQUIRE(n > 0, "bajo_build_triangle_bvh: standalone triangle BVH requires nonempty input");
-  BJ_REQUIRE(vertices != nullptr, "bajo_build_triangle_bvh: vertices is NULL");
-  BJ_REQUIRE(n <= bj::kSortMaxKeys, "bajo_build_triangle_bvh: n=%llu exceeds 2^30-1 triangles",
-             (unsigned long long)n);
+             "%s: leaf_size must be in [1, 8], got %d", fn, opt.leaf_size);
+  BJ_REQUIRE(opt.reserved == 0, "%s: reserved option field must be 0", fn);
+  BJ_REQUIRE(n > 0, "%s: a BVH needs nonempty input", fn);
+  BJ_REQUIRE(vertices != nullptr, "%s: input array is NULL", fn);
+  BJ_REQUIRE(n <= bj::kSortMaxKeys, "%s: n=%llu exceeds 2^30-1 %s", fn,
+             (unsigned long long)n, items);
   BJ_REQUIRE(opt.node_width != BAJO_WIDTH_WIDE8 || n < kWideMaxPrims,
-             "bajo_build_triangle_bvh: n=%llu exceeds the 8-wide limit of 2^28-1 triangles", (unsigned long long)n);
+             "%s: n=%llu exceeds the 8-wide limit of 2^28-1 %s", fn, (unsigned long long)n, items);
   bajo_bvh* h = *out;
   bool fresh = h == nullptr;
   if (fresh) {
@@ -533,11 +583,13 @@ int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_
     memset(h, 0, sizeof(*h));
     h->device = ctx->device;
   } else {
-    BJ_REQUIRE(h->device == ctx->device, "bajo_build_triangle_bvh: handle belongs to device %d", h->device);
+    BJ_REQUIRE(h->device == ctx->device, "%s: handle belongs to device %d", fn, h->device);
   }
+  BJ_REQUIRE(fresh || h->boxes == boxes, "bajo_build: a handle keeps its input kind (triangles or boxes)");
   h->node_width = opt.node_width;
   h->method = opt.method;
   h->wide_leaf = (uint32_t)opt.leaf_size;
+  h->boxes = boxes;
   int32_t rc = bj::build_into(ctx, h, vertices, n, timings);
   if (rc) {
     if (fresh) bj::free_bvh(h);
@@ -546,6 +598,10 @@ int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_
   *out = h;
   return BAJO_OK;
 }
+
+}  // namespace bj
+
+extern "C" {
 
 int32_t bajo_bvh_free(bajo_bvh* bvh) {
   bj::free_bvh(bvh);

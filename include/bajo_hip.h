@@ -158,6 +158,16 @@ int32_t bajo_build_triangle_bvh(bajo_ctx* ctx, const float* vertices, uint64_t n
 int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_t n,
                                    const bajo_build_options* options, bajo_build_timings* timings,
                                    bajo_bvh** out);
+/* GpuBinaryBoundsBvh(ctx, leaf_bounds, payloads, ws) + build_binary_bvh[LBVH |
+ * HPLOC] (bvh/gpu/builder/binary_layout.mojo:303-448,
+ * bvh/gpu/builder/binary_builder.mojo:21-45) for one segment: the same
+ * builders over caller boxes f32[6N] (lo.xyz, hi.xyz per leaf; centroid =
+ * (lo + hi) / 2) instead of triangles.  bajo_bvh_copy_binary exports the
+ * layout; leaf_ids index the input boxes (bajo's payloads are looked up
+ * through them).  Options as for triangles (node width 8 collapses the
+ * boxes hierarchy too).  Ray launches reject a boxes hierarchy. */
+int32_t bajo_build_binary_bounds(bajo_ctx* ctx, const float* leaf_bounds, uint64_t n,
+                                 const bajo_build_options* options, bajo_build_timings* timings, bajo_bvh** out);
 int32_t bajo_bvh_free(bajo_bvh* bvh);
 int32_t bajo_bvh_get_info(const bajo_bvh* bvh, bajo_bvh_info* out);
 /* Copies the binary LBVH into bajo's GpuBinaryBoundsBvh layout
@@ -212,6 +222,8 @@ int32_t bajo_camera_make(const float* origin3, const float* target3, const float
 #define BAJO_RENDER_MIS 4
 /* MAT enum (rt/types.mojo:24-29); a surface id is kind << 28 | index. */
 #define BAJO_MAT_LAMBERTIAN 0
+#define BAJO_MAT_METAL 1
+#define BAJO_MAT_DIELECTRIC 2
 #define BAJO_MAT_EMISSIVE 3
 
 typedef struct bajo_scene bajo_scene;

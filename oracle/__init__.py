@@ -36,6 +36,10 @@ _SIG = {
         c_uint32,
         [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
+    "orc_build_lbvh_bounds": (
+        c_uint32,
+        [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "orc_build_lbvh_topdown": (c_uint32, [c_void_p, c_void_p, c_void_p, c_uint64, c_int, c_void_p, c_void_p, c_void_p]),
     "orc_canonical_karras": (c_uint64, [c_uint32, c_void_p, c_void_p, c_void_p]),
     "orc_canonical_topdown": (c_uint64, [c_void_p, c_void_p, c_void_p]),
@@ -183,7 +187,16 @@ def radix_sort_pairs_u64(keys: np.ndarray, values: np.ndarray | None, begin_bit=
 def build_lbvh(vertices: np.ndarray) -> dict:
     """Single-segment LBVH in bajo's GPU binary layout (GpuBinaryBoundsBvh)."""
     v = f32(vertices)
-    n = v.size // 9
+    return _build_lbvh(v, v.size // 9, lib().orc_build_lbvh)
+
+
+def build_lbvh_bounds(leaf_bounds: np.ndarray) -> dict:
+    """The same build over leaf boxes f32[6N] (lo.xyz, hi.xyz)."""
+    b = f32(leaf_bounds)
+    return _build_lbvh(b, b.size // 6, lib().orc_build_lbvh_bounds)
+
+
+def _build_lbvh(v, n, fn) -> dict:
     m = max(n - 1, 0)
     out = {
         "sorted_keys": np.empty(n, np.uint64),
@@ -194,7 +207,7 @@ def build_lbvh(vertices: np.ndarray) -> dict:
         "node_bounds": np.empty(max(12 * m, 1), np.float32),
         "scene_bounds": np.empty(12, np.float32),
     }
-    root = lib().orc_build_lbvh(
+    root = fn(
         _p(v), n, _p(out["sorted_keys"]), _p(out["leaf_ids"]), _p(out["node_meta"]), _p(out["leaf_parent"]),
         _p(out["leaf_counts"]), _p(out["node_bounds"]), _p(out["scene_bounds"]),
     )
@@ -594,10 +607,15 @@ def build_hploc(vertices: np.ndarray) -> dict:
     """Oracle H-PLOC over a triangle soup: the LBVH's sorted keys, bajo's CPU
     topology, and its depth-first binary layout (usable by trace_bvh2 and
     collapse_wide)."""
-    v = f32(vertices)
-    lb = build_lbvh(v)
+    return build_hploc_bounds(triangle_bounds(f32(vertices)))
+
+
+def build_hploc_bounds(leaf_bounds: np.ndarray) -> dict:
+    """Oracle H-PLOC over leaf boxes f32[6N]."""
+    bounds = f32(leaf_bounds)
+    lb = build_lbvh_bounds(bounds)
     codes = (lb["sorted_keys"] >> np.uint64(32)).astype(np.uint32)
-    t = hploc_topology(triangle_bounds(v), codes, lb["leaf_ids"])
+    t = hploc_topology(bounds, codes, lb["leaf_ids"])
     b = hploc_binary(t, hploc_lowest_leaf_numbering(t, lb["leaf_ids"]))
     b["scene_bounds"] = lb["scene_bounds"]
     b["topology"] = t

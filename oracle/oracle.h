@@ -34,6 +34,9 @@ void orc_refit(uint32_t root, const float *leaf_bounds, const uint32_t *leaf_ids
 uint32_t orc_build_lbvh(const float *vertices, uint64_t n, uint64_t *sorted_keys, uint32_t *leaf_ids,
                         uint32_t *node_meta, uint32_t *leaf_parent, uint32_t *leaf_counts, float *node_bounds,
                         float *scene12);
+uint32_t orc_build_lbvh_bounds(const float *bounds, uint64_t n, uint64_t *sorted_keys, uint32_t *leaf_ids,
+                               uint32_t *node_meta, uint32_t *leaf_parent, uint32_t *leaf_counts, float *node_bounds,
+                               float *scene12);
 uint32_t orc_build_lbvh_topdown(const uint32_t *sorted_codes, const uint32_t *sorted_ids, const float *bounds6,
                                 uint64_t n, int leaf_size, float *boxes6, uint32_t *first, uint32_t *count);
 uint64_t orc_canonical_karras(uint32_t root, const uint32_t *node_meta, const uint32_t *leaf_counts, uint32_t *out);

@@ -200,8 +200,19 @@ uint32_t orc_build_lbvh(const float *vertices, uint64_t n, uint64_t *sorted_keys
                         uint32_t *node_meta, uint32_t *leaf_parent, uint32_t *leaf_counts, float *node_bounds,
                         float *scene12) {
     float *bounds = (float *)malloc((n ? n : 1) * 6 * sizeof(float));
-    uint32_t *codes = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
     orc_triangle_bounds(vertices, n, bounds);
+    uint32_t root = orc_build_lbvh_bounds(bounds, n, sorted_keys, leaf_ids, node_meta, leaf_parent, leaf_counts,
+                                          node_bounds, scene12);
+    free(bounds);
+    return root;
+}
+
+/* The same build over caller leaf boxes f32[6n] (lo.xyz, hi.xyz), bajo's
+ * GpuBinaryBoundsBvh input (bvh/gpu/builder/binary_layout.mojo:331-428). */
+uint32_t orc_build_lbvh_bounds(const float *bounds, uint64_t n, uint64_t *sorted_keys, uint32_t *leaf_ids,
+                               uint32_t *node_meta, uint32_t *leaf_parent, uint32_t *leaf_counts, float *node_bounds,
+                               float *scene12) {
+    uint32_t *codes = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
     orc_scene_bounds(bounds, n, scene12);
     orc_morton_codes(bounds, n, scene12 + 6, codes);
     for (uint64_t i = 0; i < n; ++i) { sorted_keys[i] = ((uint64_t)codes[i] << 32) | (uint64_t)i; leaf_ids[i] = (uint32_t)i; }
@@ -209,7 +220,6 @@ uint32_t orc_build_lbvh(const float *vertices, uint64_t n, uint64_t *sorted_keys
     for (uint64_t i = 0; i < n; ++i) codes[i] = (uint32_t)(sorted_keys[i] >> 32);
     uint32_t root = orc_karras_topology(codes, n, node_meta, leaf_parent, leaf_counts);
     orc_refit(root, bounds, leaf_ids, node_meta, node_bounds);
-    free(bounds);
     free(codes);
     return root;
 }

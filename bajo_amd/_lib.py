@@ -167,6 +167,15 @@ SIGNATURES = {
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, c_void_p],
     ),
+    "bajo_tlas_create": (
+        c_int32, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, POINTER(BuildOptions), POINTER(c_void_p)],
+    ),
+    "bajo_tlas_trace_rays": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_uint64]),
+    "bajo_tlas_trace_camera": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32]),
+    "bajo_tlas_get_top": (c_int32, [c_void_p, POINTER(c_void_p)]),
+    "bajo_tlas_copy_instance_bounds": (c_int32, [c_void_p, c_void_p]),
+    "bajo_tlas_status": (c_int32, [c_void_p, c_void_p, POINTER(c_uint32)]),
+    "bajo_tlas_free": (c_int32, [c_void_p]),
     "bajo_camera_make": (
         c_int32,
         [POINTER(c_float), POINTER(c_float), POINTER(c_float), c_float, c_float, POINTER(c_float)],

@@ -41,7 +41,8 @@ HIP_FLAGS = [
     "-packed-fp32-ops",
 ]
 
-SOURCES = ["context.hip", "sort.hip", "lbvh.hip", "hploc.hip", "wide.hip", "trace.hip", "render.hip", "scenes.cpp"]
+SOURCES = ["context.hip", "sort.hip", "lbvh.hip", "hploc.hip", "wide.hip", "trace.hip", "tlas.hip", "render.hip",
+           "scenes.cpp"]
 
 
 def _compile(src: Path) -> Path:

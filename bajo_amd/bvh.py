@@ -196,7 +196,8 @@ class GpuTriangleBvh:
 
     def free(self) -> None:
         if self._handle:
-            _lib.lib().bajo_bvh_free(self._handle)
+            if getattr(self, "_owner", None) is None:
+                _lib.lib().bajo_bvh_free(self._handle)
             self._handle = None
 
     def __del__(self):
@@ -204,6 +205,99 @@ class GpuTriangleBvh:
             self.free()
         except Exception:
             pass
+
+
+@dataclass
+class Instance:
+    """bajo's Instance (bvh/types.mojo:153-190): a BLAS placed in world space by an affine `transform`
+    (3x4 or 4x4, row-major, BLAS -> world)."""
+
+    transform: np.ndarray
+    blas_idx: int
+
+    def matrix3x4(self) -> np.ndarray:
+        m = np.asarray(self.transform, np.float32)
+        return np.ascontiguousarray(m.reshape(-1, 4)[:3], np.float32)
+
+
+class GpuTriangleTlas:
+    """build_triangle_tlas / GpuTriangleTlas (bvh/gpu/tlas.mojo:482-645): instances of triangle BVHs under a
+    top-level hierarchy.  The BLAS objects are kept alive by this object."""
+
+    def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, blases: list, instance_count: int):
+        self._handle = handle
+        self.ctx = ctx
+        self.blases = list(blases)
+        self.instance_count = instance_count
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if not self._handle:
+            raise RuntimeError("TLAS already freed")
+        return self._handle
+
+    def launch_rays(self, rays, hits, ray_count=None, mode=TRACE.CLOSEST_HIT) -> None:
+        r = rays.numel() // RAY_STRIDE if ray_count is None else int(ray_count)
+        require_device(rays, self.ctx, "rays")
+        require_device(hits, self.ctx, "hits")
+        if rays.numel() != r * RAY_STRIDE or hits.numel() < r * HIT_STRIDE:
+            raise ValueError("rays must hold 8 planes of ray_count entries and hits 8 floats per ray")
+        _lib.check(_lib.lib().bajo_tlas_trace_rays(self.ctx.handle, self.handle, int(mode), ptr(rays), ptr(hits), r))
+
+    def launch_camera(self, camera_params, hits, ray_count, width, height) -> None:
+        """GpuTriangleTlas.launch_camera (bvh/gpu/tlas.mojo:499-548)."""
+        require_device(camera_params, self.ctx, "camera_params")
+        require_device(hits, self.ctx, "hits")
+        if hits.numel() < int(ray_count) * HIT_STRIDE:
+            raise ValueError("hit output buffer is too short")
+        _lib.check(_lib.lib().bajo_tlas_trace_camera(self.ctx.handle, self.handle, ptr(camera_params), ptr(hits),
+                                                     int(ray_count), int(width), int(height)))
+
+    def top(self) -> "GpuBinaryBoundsBvh":
+        """The top-level boxes hierarchy (owned by the TLAS): its leaf_ids are instance indices."""
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().bajo_tlas_get_top(self.handle, ctypes.byref(h)))
+        view = GpuBinaryBoundsBvh(h, self.ctx, self.instance_count)
+        view._owner = self
+        return view
+
+    def instance_bounds(self) -> np.ndarray:
+        out = np.empty(6 * self.instance_count, np.float32)
+        _lib.check(_lib.lib().bajo_tlas_copy_instance_bounds(self.handle, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def status(self) -> int:
+        st = ctypes.c_uint32()
+        _lib.check(_lib.lib().bajo_tlas_status(self.ctx.handle, self.handle, ctypes.byref(st)))
+        return st.value
+
+    def free(self) -> None:
+        if self._handle:
+            _lib.lib().bajo_tlas_free(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def build_triangle_tlas(blases: list, instances: list, ctx: DeviceContext | None = None,
+                        method: int = GpuBvhBuildMethod.LBVH) -> GpuTriangleTlas:
+    """build_triangle_tlas (bvh/gpu/tlas.mojo:618-645) over triangle BVHs and Instance records."""
+    if not blases or not instances:
+        raise ValueError("a TLAS needs BLASes and instances")
+    ctx = ctx or blases[0].ctx
+    handles = (ctypes.c_void_p * len(blases))(*[b.handle.value for b in blases])
+    xf = np.ascontiguousarray(np.stack([inst.matrix3x4() for inst in instances]).reshape(-1), np.float32)
+    idx = np.ascontiguousarray([int(inst.blas_idx) for inst in instances], np.uint32)
+    opts = build_options(2, 0, method)
+    handle = ctypes.c_void_p()
+    _lib.check(_lib.lib().bajo_tlas_create(ctx.handle, handles, len(blases), xf.ctypes.data_as(ctypes.c_void_p),
+                                           idx.ctypes.data_as(ctypes.c_void_p), len(instances), ctypes.byref(opts),
+                                           ctypes.byref(handle)))
+    return GpuTriangleTlas(handle, ctx, blases, len(instances))
 
 
 def _check_vertices(vertices: torch.Tensor, ctx: DeviceContext) -> int:

@@ -458,6 +458,19 @@ BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters*
   return w.finish(wv, tc);
 }
 
+// bvh/camera.mojo:137-154, camera record bvh/camera.mojo:19-30.
+BJ_D Ray camera_ray(const float* __restrict__ cams, uint64_t r, uint32_t w, uint32_t h, float inv_height) {
+  uint64_t ppv = (uint64_t)w * h;
+  uint64_t view = r / ppv, local = r - view * ppv;
+  const float* c = cams + 20 * view;
+  uint32_t px = (uint32_t)(local % w), py = (uint32_t)(local / w);
+  float sx = (2.0f * ((float)px + 0.5f) - (float)w) * inv_height;
+  float sy = 1.0f - 2.0f * ((float)py + 0.5f) * inv_height;
+  float fov = c[12];
+  v3 dir = add(add(mk(c[3], c[4], c[5]), scale(mk(c[6], c[7], c[8]), sx * fov)), scale(mk(c[9], c[10], c[11]), sy * fov));
+  return Ray{mk(c[0], c[1], c[2]), normalize(dir), 0.0f, kFltMax};
+}
+
 // Either layout behind one call: kWide selects the 8-wide traversal.
 struct AccelView {
   BvhView bin;

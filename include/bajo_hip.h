@@ -213,6 +213,41 @@ int32_t bajo_ao_rays(bajo_ctx* ctx, const float* rays, const float* hits, uint64
 int32_t bajo_camera_make(const float* origin3, const float* target3, const float* up3, float fov_scale,
                          float focus_dist, float* out20);
 
+/* ---- instances: bajo/bvh/gpu/tlas.mojo -----------------------------------
+ * build_triangle_tlas (bvh/gpu/tlas.mojo:618-645): a top-level hierarchy
+ * over instances of triangle BVHs.  transforms f32[12 I] are row-major 3x4
+ * affines (m00 m01 m02 tx / m10 m11 m12 ty / m20 m21 m22 tz) from BLAS space
+ * to world space (bajo's Instance and Affine3, bvh/types.mojo:153-190,
+ * core/transform.mojo); blas_indices u32[I] select from blases[].  An
+ * instance's world box is its BLAS root box through the transform
+ * (AABB.apply_transform, core/aabb.mojo:101-137); the top level is
+ * bajo_build_binary_bounds over those boxes (options: method; node width 2).
+ * Host arrays in, device state out; the BLASes must outlive the TLAS. */
+typedef struct bajo_tlas bajo_tlas;
+int32_t bajo_tlas_create(bajo_ctx* ctx, const bajo_bvh* const* blases, uint32_t blas_count, const float* transforms,
+                         const uint32_t* blas_indices, uint32_t instance_count, const bajo_build_options* options,
+                         bajo_tlas** out);
+/* Rays through the instances (_trace_tlas_ray, bvh/gpu/tlas.mojo:176-233):
+ * at each instance leaf the ray is mapped into BLAS space by the inverse
+ * transform (t unchanged) and the BLAS, binary or 8-wide, is traversed; a
+ * closest hit reports inst = the instance index and the normal through the
+ * inverse transpose; an any hit is bajo's shadow hit with inst set.  rays /
+ * hits as bajo_trace_rays. */
+int32_t bajo_tlas_trace_rays(bajo_ctx* ctx, const bajo_tlas* tlas, int32_t mode, const float* rays, float* hits,
+                             uint64_t ray_count);
+/* GpuTriangleTlas.launch_camera (bvh/gpu/tlas.mojo:499-548): pinhole rays as
+ * bajo_trace_camera, closest hit. */
+int32_t bajo_tlas_trace_camera(bajo_ctx* ctx, const bajo_tlas* tlas, const float* camera, float* hits,
+                               uint64_t ray_count, uint32_t width, uint32_t height);
+/* The top-level boxes hierarchy (owned by the TLAS; bajo_bvh_copy_binary
+ * exports it, its leaf_ids are instance indices) and the instance world
+ * boxes f32[6 I] on the host. */
+int32_t bajo_tlas_get_top(const bajo_tlas* tlas, bajo_bvh** top);
+int32_t bajo_tlas_copy_instance_bounds(const bajo_tlas* tlas, float* out6);
+/* Traversal status word of the last launch (BAJO_STATUS_*), synchronizing. */
+int32_t bajo_tlas_status(bajo_ctx* ctx, const bajo_tlas* tlas, uint32_t* status);
+int32_t bajo_tlas_free(bajo_tlas* tlas);
+
 /* ---- wavefront renderer: bajo/rt/gpu ------------------------------------
  * RENDER enum, rt/types.mojo:51-57. */
 #define BAJO_RENDER_PATH 0

@@ -97,6 +97,13 @@ _SIG = {
         c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                 c_int],
     ),
+    "orc_affine_inverse": (c_int, [c_void_p, c_void_p]),
+    "orc_transform_box": (None, [c_void_p, c_void_p, c_void_p]),
+    "orc_tlas_trace": (
+        None,
+        [c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p,
+         c_uint64, c_int, c_void_p],
+    ),
     "orc_pow_unit": (c_float, [c_float, c_float]),
     "orc_reflectance": (c_float, [c_float, c_float]),
     "orc_reflect3": (None, [c_void_p, c_void_p, c_void_p]),
@@ -631,3 +638,87 @@ def ao_rays(rays: np.ndarray, hits: np.ndarray, samples: int, seed: int) -> np.n
     out = np.empty(8 * n * samples, np.float32)
     lib().orc_ao_rays(_p(r), _p(h), n, samples, seed, _p(out))
     return out
+
+
+# ---- instances ------------------------------------------------------------------
+
+class OrcBvh(ctypes.Structure):
+    """orc_bvh: one BLAS for orc_tlas_trace (binary layout, or 8-wide when `wide` is set)."""
+
+    _fields_ = [
+        ("vertices", c_void_p),
+        ("leaf_ids", c_void_p),
+        ("node_meta", c_void_p),
+        ("node_bounds", c_void_p),
+        ("root", c_uint32),
+        ("wide", c_void_p),
+        ("wide_count", c_uint64),
+        ("prim_count", c_uint64),
+    ]
+
+
+def affine_inverse(m3x4) -> tuple[np.ndarray, bool]:
+    """Affine3.inverse of a row-major 3x4: (f32[12], invertible)."""
+    a = f32(m3x4).reshape(-1)
+    o = np.empty(12, np.float32)
+    ok = lib().orc_affine_inverse(_p(a), _p(o))
+    return o, bool(ok)
+
+
+def transform_box(m3x4, box6) -> np.ndarray:
+    """AABB.apply_transform of a box (lo.xyz, hi.xyz) by a row-major 3x4."""
+    m = f32(m3x4).reshape(-1)
+    b = f32(box6).reshape(-1)
+    o = np.empty(6, np.float32)
+    lib().orc_transform_box(_p(m), _p(b), _p(o))
+    return o
+
+
+def tlas_instance_bounds(transforms: np.ndarray, blas_boxes: np.ndarray, inst_blas: np.ndarray) -> np.ndarray:
+    """World boxes f32[6I] of instances: each BLAS root box through its instance transform."""
+    xf = f32(transforms).reshape(-1, 12)
+    boxes = f32(blas_boxes).reshape(-1, 6)
+    return np.concatenate([transform_box(xf[i], boxes[int(b)]) for i, b in enumerate(inst_blas)]).astype(np.float32)
+
+
+def tlas_trace(blases: list, transforms: np.ndarray, inst_blas: np.ndarray, top: dict, rays: np.ndarray,
+               mode: int = 0) -> np.ndarray:
+    """Rays through instances of BLASes.  blases: dicts with 'vertices' and the binary layout of
+    build_lbvh / build_hploc ('leaf_ids', 'node_meta', 'node_bounds', 'root'), plus 'wide' (8-wide
+    nodes over the same leaf_ids) to traverse that layout instead; transforms f32[12I] BLAS -> world;
+    top: binary layout over the instance boxes whose leaf_ids are instance indices."""
+    keep = []
+    descs = (OrcBvh * len(blases))()
+    for k, b in enumerate(blases):
+        v = f32(b["vertices"])
+        ids = np.ascontiguousarray(b["leaf_ids"], np.uint32)
+        meta = np.ascontiguousarray(b["node_meta"] if b["node_meta"].size else np.zeros(4), np.uint32)
+        nb = f32(b["node_bounds"] if b["node_bounds"].size else np.zeros(12))
+        keep += [v, ids, meta, nb]
+        d = descs[k]
+        d.vertices, d.leaf_ids, d.node_meta, d.node_bounds = (x.ctypes.data for x in (v, ids, meta, nb))
+        d.root = int(b["root"])
+        d.prim_count = v.size // 9
+        if b.get("wide") is not None:
+            w = np.ascontiguousarray(b["wide"], WIDE_NODE_DTYPE)
+            keep.append(w)
+            d.wide = w.ctypes.data
+            d.wide_count = w.size
+    xf = f32(transforms).reshape(-1, 12)
+    inv = np.empty_like(xf)
+    for i in range(xf.shape[0]):
+        inv[i], ok = affine_inverse(xf[i])
+        if not ok:
+            raise ValueError(f"transform of instance {i} is not invertible")
+    ib = np.ascontiguousarray(inst_blas, np.uint32)
+    tmeta = np.ascontiguousarray(top["node_meta"] if top["node_meta"].size else np.zeros(4), np.uint32)
+    tnb = f32(top["node_bounds"] if top["node_bounds"].size else np.zeros(12))
+    tids = np.ascontiguousarray(top["leaf_ids"], np.uint32)
+    r = f32(rays)
+    R = r.size // 8
+    hits = np.empty(8 * R, np.float32)
+    inv = np.ascontiguousarray(inv.reshape(-1))
+    lib().orc_tlas_trace(ctypes.cast(descs, c_void_p), len(blases), _p(inv), _p(ib), ib.size, _p(tids), _p(tmeta),
+                         _p(tnb), int(top["root"]), _p(r), R, int(mode), _p(hits))
+    del keep
+    return hits

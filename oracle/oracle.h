@@ -85,6 +85,11 @@ uint32_t orc_hploc_binary(uint32_t root, const uint32_t *left, const uint32_t *r
 void orc_hploc_lowest_leaf_numbering(const uint32_t *left, const uint32_t *right, const uint32_t *sorted_ids,
                                      uint64_t n, uint32_t *out);
 
+/* instances (orc_tlas.c); orc_bvh is declared in orc_internal.h */
+int orc_affine_inverse(const float *a, float *o);
+void orc_transform_box(const float *m, const float *b, float *out);
+struct orc_bvh_s;
+
 /* wavefront renderer (orc_render.c) */
 void orc_ao_rays(const float *rays, const float *hits, uint64_t n, uint32_t samples, uint64_t seed, float *out);
 typedef struct {

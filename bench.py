@@ -265,6 +265,97 @@ def soup_camera_workload(ctx, threads: int, cpu: bool) -> dict:
     return out
 
 
+def _views(lo, hi, views: int = 3, fov_scale: float = 0.75, distance: float = 2.5) -> np.ndarray:
+    """bajo's make_camera_rays_and_params cameras (benchmark/bvh_fixtures.mojo:116-152): `views` eyes spread
+    along x in front of the box [lo, hi], all looking at its centre, flattened to 20 floats each."""
+    from bajo_amd.bvh import Camera
+
+    c = (lo + hi) * 0.5
+    e = hi - lo
+    w = max(float(e.max()), 1.0)
+    cams = []
+    for view in range(views):
+        off = view - (views - 1) * 0.5
+        eye = c + np.array([off * w * 0.30, e[1] * 0.20, -w * distance])
+        cams.append(Camera(tuple(map(float, eye)), tuple(map(float, c)), (0.0, 1.0, 0.0), fov_scale).flatten())
+    return np.concatenate(cams).astype(np.float32)
+
+
+def tlas_workload(ctx) -> dict:
+    """bajo's GPU TLAS benchmark shape (bench/bvh/bench_gpu_tlas.mojo:73-140, 920-1030, 1212-1223) on synthetic
+    meshes: translated side x side grids (spacing 2.25 x the BLAS extent) of one 67,280-triangle displaced
+    icosphere BLAS (8-wide), and a 32 x 16 grid cycling three BLASes of different sizes; 640 x 360 primary
+    rays from 3 views, closest hit through the instances.  Beside each grid, the same instances baked into
+    one flat mesh (8-wide) traced with the same rays."""
+    import torch
+
+    from bajo_amd import scenes
+    from bajo_amd.bvh import Instance, build_triangle_bvh, build_triangle_tlas
+
+    dev = ctx.torch_device
+    stream = torch.cuda.current_stream(dev)
+    W, H, V = 640, 360, 3
+    R = W * H * V
+    hits = torch.empty(8 * R, dtype=torch.float32, device=dev)
+    meshes = [scenes.icosphere(20 * f * f, frequency=f, noise_seed=2030 + f) for f in (58, 40, 82)]
+    blases = [build_triangle_bvh(torch.from_numpy(m).to(dev), ctx=ctx, node_width=8, leaf_size=4) for m in meshes]
+    boxes = [(m.reshape(-1, 3).min(0), m.reshape(-1, 3).max(0)) for m in meshes]
+
+    def run(name, xf, idx, distance=2.5):
+        tlas = build_triangle_tlas(blases, [Instance(x, i) for x, i in zip(xf, idx)])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            build_triangle_tlas(blases, [Instance(x, i) for x, i in zip(xf, idx)]).free()
+        torch.cuda.synchronize()
+        build_ms = (time.perf_counter() - t0) * 1e3 / 3  # wall clock: host inverse + boxes, upload, top build
+        lo = np.min([x[:, 3] + boxes[i][0] for x, i in zip(xf, idx)], axis=0)
+        hi = np.max([x[:, 3] + boxes[i][1] for x, i in zip(xf, idx)], axis=0)
+        cams = torch.from_numpy(_views(lo, hi, V, distance=distance)).to(dev)
+        ms = _timed(lambda: tlas.launch_camera(cams, hits, R, W, H), stream, reps=8)
+        hit_frac = float((hits.view(-1, 8)[:, 2].view(torch.int32) != -1).float().mean())
+        status = tlas.status()
+        tlas.free()
+        flat = np.concatenate([(meshes[i].reshape(-1, 3) + x[:, 3]).reshape(-1) for x, i in zip(xf, idx)])
+        fb = build_triangle_bvh(torch.from_numpy(flat).to(dev), ctx=ctx, node_width=8, leaf_size=4)
+        flat_ms = _timed(lambda: fb.launch_camera(cams, hits, R, W, H), stream, reps=8)
+        fb.free()
+        print(f"[bench] tlas {name}: {len(idx)} instances, {ms:.3f} ms ({R / ms / 1e3:.0f} MRays/s), flattened "
+              f"{flat_ms:.3f} ms", file=sys.stderr, flush=True)
+        return {"instances": len(idx), "instanced_triangles": int(flat.size // 9), "tlas_build_ms": build_ms,
+                "rays": R, "ms": ms, "mrays_s": R / (ms / 1e3) / 1e6, "hit_fraction": hit_frac, "status": status,
+                "flattened_ms": flat_ms, "flattened_mrays_s": R / (flat_ms / 1e3) / 1e6}
+
+    out = {"workload": "translated grids of a 67,280-triangle displaced icosphere (8-wide BLAS) and a 32x16 grid "
+           "over three icosphere BLASes (67,280 / 32,000 / 134,480 triangles); 640x360 primary rays x 3 views "
+           "(bajo's cameras at 2.5 scene widths; 'close': 0.5), closest hit; 'flattened' = the same instances baked "
+           "into one 8-wide BVH"}
+    ext = boxes[0][1] - boxes[0][0]
+    for side in (4, 8, 16):
+        xf, idx = [], []
+        for y in range(side):
+            for x in range(side):
+                t = ((x - (side - 1) * 0.5) * max(ext[0] * 2.25, 0.25), (y - (side - 1) * 0.5) * max(ext[1] * 2.25, 0.25), 0)
+                xf.append(np.array([[1, 0, 0, t[0]], [0, 1, 0, t[1]], [0, 0, 1, t[2]]], np.float32))
+                idx.append(0)
+        out[f"grid_{side}x{side}"] = run(f"grid {side}x{side}", xf, idx)
+    # the 16 x 16 grid from eyes at half the grid width: most rays enter instances
+    out["grid_16x16_close"] = run("grid 16x16 close", xf, idx, distance=0.5)
+    me = max(max(float((b[1] - b[0]).max()) for b in boxes), 0.25)
+    sp = max(me * 2.75, 0.25)
+    xf, idx = [], []
+    for y in range(16):
+        for x in range(32):
+            k = y * 32 + x
+            t = ((x - 15.5) * sp, (y - 7.5) * sp, ((k * 17) % 13) * me * 0.025)
+            xf.append(np.array([[1, 0, 0, t[0]], [0, 1, 0, t[1]], [0, 0, 1, t[2]]], np.float32))
+            idx.append(k % 3)
+    out["multi_blas_32x16"] = run("multi-BLAS 32x16", xf, idx)
+    for b in blases:
+        b.free()
+    return out
+
+
 def cpu_baseline(scene, cam, threads: int) -> dict:
     import oracle
 
@@ -294,10 +385,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-workloads", action="store_true",
-                    help="skip the 65k-soup camera and 16.7M soup-grid H-PLOC workloads")
+                    help="skip the 65k-soup camera, 16.7M soup-grid H-PLOC and TLAS workloads")
     ap.add_argument("--node-width", type=int, default=8, choices=(2, 8), help="BVH layout: binary or 8-wide")
     ap.add_argument("--leaf-size", type=int, default=4, help="8-wide leaf size (triangles per leaf child)")
     ap.add_argument("--method", choices=("lbvh", "hploc"), default="lbvh", help="binary builder")
+    ap.add_argument("--tlas-only", action="store_true", help="run only the TLAS workload and print its JSON")
     ap.add_argument(
         "--copy-calibration-mib",
         type=int,
@@ -323,6 +415,9 @@ def main() -> int:
     ctx = default_context(0)
     dev = ctx.torch_device
     print(f"[bench] device {ctx.name()}", file=sys.stderr, flush=True)
+    if args.tlas_only:
+        print(json.dumps({"tlas": tlas_workload(ctx)}), flush=True)
+        return 0
     scene = scenes.icosphere_scene(N_SPHERE)
     method = {"lbvh": 0, "hploc": 1}[args.method]
     gscene = GpuScene(scene, ctx, node_width=args.node_width, leaf_size=args.leaf_size, method=method)
@@ -443,6 +538,7 @@ def main() -> int:
     if not args.no_extra_workloads:
         line["soup_camera_65k"] = soup_camera_workload(ctx, min(16, os.cpu_count() or 1), not args.no_cpu_baseline)
         line["soup_grid_16m"] = soup_grid_workload(ctx)
+        line["tlas"] = tlas_workload(ctx)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, cam, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)

@@ -167,6 +167,13 @@ SIGNATURES = {
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, c_void_p],
     ),
+    "bajo_build_triangle_blas_set": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_uint32, POINTER(BuildOptions), POINTER(BuildTimings), POINTER(c_void_p)],
+    ),
+    "bajo_blas_set_count": (c_int32, [c_void_p, POINTER(c_uint32)]),
+    "bajo_blas_set_get": (c_int32, [c_void_p, c_uint32, POINTER(c_void_p)]),
+    "bajo_blas_set_free": (c_int32, [c_void_p]),
     "bajo_tlas_create": (
         c_int32, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, POINTER(BuildOptions), POINTER(c_void_p)],
     ),

@@ -207,6 +207,72 @@ class GpuTriangleBvh:
             pass
 
 
+class GpuTriangleBlasSet:
+    """GpuBlasSet of triangles (bvh/gpu/blas_storage.mojo:21-120) made by build_triangle_blas_set
+    (bvh/gpu/triangle_bvh.mojo:469-491): several meshes built in one segmented pass.  `blas(k)` is mesh k
+    as a ready-to-trace BVH that shares the set's storage (bajo's BlasDesc); it stays valid until the set
+    is freed."""
+
+    def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, counts: list, node_width: int, leaf_size: int):
+        self._handle = handle
+        self.ctx = ctx
+        self.counts = list(counts)
+        self.node_width = node_width
+        self.leaf_size = leaf_size
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if not self._handle:
+            raise RuntimeError("BLAS set already freed")
+        return self._handle
+
+    @property
+    def blas_count(self) -> int:
+        return len(self.counts)
+
+    def blas(self, k: int) -> GpuTriangleBvh:
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().bajo_blas_set_get(self.handle, int(k), ctypes.byref(h)))
+        view = GpuTriangleBvh(h, self.ctx, self.counts[k], self.node_width, self.leaf_size)
+        view._owner = self
+        return view
+
+    def blases(self) -> list:
+        return [self.blas(k) for k in range(self.blas_count)]
+
+    def free(self) -> None:
+        if self._handle:
+            _lib.lib().bajo_blas_set_free(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def build_triangle_blas_set(vertex_sets: list, ctx: DeviceContext | None = None, node_width: int = 2,
+                            leaf_size: int = 0, timings: dict | None = None) -> GpuTriangleBlasSet:
+    """build_triangle_blas_set[W, L, LBVH] (bvh/gpu/triangle_bvh.mojo:469-491): one segmented LBVH build over
+    the meshes `vertex_sets` (float32 tensors of 9 floats per triangle on one device); node_width 2 or 8."""
+    if not vertex_sets:
+        raise ValueError("a BLAS set needs at least one mesh")
+    ctx = ctx or default_context(vertex_sets[0].device.index or 0)
+    counts = [_check_vertices(v, ctx) for v in vertex_sets]
+    flat = torch.cat([v.reshape(-1) for v in vertex_sets]).contiguous()
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    t = _lib.BuildTimings() if timings is not None else None
+    handle = ctypes.c_void_p()
+    opts = build_options(node_width, leaf_size, GpuBvhBuildMethod.LBVH)
+    _lib.check(_lib.lib().bajo_build_triangle_blas_set(
+        ctx.handle, ptr(flat), offsets.ctypes.data_as(ctypes.c_void_p), len(counts), ctypes.byref(opts),
+        ctypes.byref(t) if t else None, ctypes.byref(handle)))
+    if t is not None:
+        timings.update(t.as_dict())
+    return GpuTriangleBlasSet(handle, ctx, counts, node_width, leaf_size)
+
+
 @dataclass
 class Instance:
     """bajo's Instance (bvh/types.mojo:153-190): a BLAS placed in world space by an affine `transform`

@@ -77,6 +77,23 @@ struct bajo_bvh {
   uint64_t hp_capacity;
   // input kind: triangles f32[9N] (traceable) or leaf boxes f32[6N]
   bool boxes;
+  // segmented build (a BLAS set): seg_count > 0 segments share this storage;
+  // segment s owns sorted leaves [off_s, off_{s+1}) and nodes from off_s - s
+  uint32_t seg_count;
+  uint64_t seg_capacity;
+  uint64_t* seg_off;      // device u64[S + 1]
+  uint32_t* seg_acc;      // 12 S ordered-int bounds accumulators
+  float* seg_scene;       // 12 S: root box and centroid box of each segment
+  uint32_t* seg_prim;     // n: input index of each sorted leaf
+  uint64_t seg_leaf_capacity;
+  void* seg_sort_ws;
+  size_t seg_sort_ws_bytes;
+  // traversal: index bounds of the node / triangle arrays and the 8-wide
+  // root.  A segment view of a BLAS set (owner != NULL) shares its owner's
+  // arrays; its records start at node_base / leaf_base.
+  uint32_t trav_nodes, trav_prims, wide_root;
+  const bajo_bvh* owner;
+  uint64_t node_base, leaf_base;
 };
 
 namespace bj {
@@ -84,6 +101,9 @@ namespace bj {
 int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h);
 // Builds h's binary hierarchy with H-PLOC over the sorted keys; hploc.hip.
 int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* src);
+// Builds the segments of a BLAS set into h (lbvh.hip); offsets are host u64[S + 1].
+int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const uint64_t* offsets, uint32_t S,
+                        bajo_build_timings* timings);
 // Karras topology over sorted 64-bit keys (lbvh.hip); `range_first` may be NULL.
 __global__ void karras_kernel(const uint64_t* __restrict__ keys, uint64_t n_u, BjNode* __restrict__ nodes,
                               uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ leaf_counts,

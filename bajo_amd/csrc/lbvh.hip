@@ -21,6 +21,8 @@
 //      sorted leaf's triangle (v0, e1, e2, prim) for traversal.
 #include <string.h>
 
+#include <vector>
+
 #include "bvh_internal.h"
 #include "sort_internal.h"
 
@@ -164,18 +166,15 @@ BJ_D int karras_delta(const uint64_t* __restrict__ keys, int64_t n, int64_t i, u
   return x == 0 ? 64 : __builtin_clzll(x);
 }
 
-}  // namespace
-
-// Also the guide hierarchy of the H-PLOC builder, which needs each node's
-// first sorted position (`range_first`, NULL for the plain LBVH).
-__global__ void __launch_bounds__(kBlock) karras_kernel(const uint64_t* __restrict__ keys, uint64_t n_u,
-                                                        BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_parent,
-                                                        uint32_t* __restrict__ leaf_counts,
-                                                        uint32_t* __restrict__ flags,
-                                                        uint32_t* __restrict__ range_first) {
-  const int64_t n = (int64_t)n_u;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
+// One Karras node: internal node i of the n sorted keys at `keys`.  Its
+// record goes to nodes[node_base + i]; leaf references are leaf_base + the
+// sorted position, inner references node_base + the node (both bases are 0
+// for a single hierarchy; a segmented build places each segment's nodes and
+// leaves at its own bases).
+BJ_D void karras_node(const uint64_t* __restrict__ keys, int64_t n, int64_t i, BjNode* __restrict__ nodes,
+                      uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ leaf_counts,
+                      uint32_t* __restrict__ flags, uint32_t* __restrict__ range_first, uint32_t node_base,
+                      uint32_t leaf_base) {
   const uint32_t ci = (uint32_t)(keys[i] >> 32);
   // bvh/gpu/builder/lbvh.mojo:164-191: direction and range.
   int d_next = karras_delta(keys, n, i, ci, i + 1);
@@ -201,28 +200,44 @@ __global__ void __launch_bounds__(kBlock) karras_kernel(const uint64_t* __restri
     int64_t ns = split + step;
     if (ns < last && karras_delta(keys, n, first, cf, ns) > node_prefix) split = ns;
   }
-  flags[i] = 0;
-  leaf_counts[i] = (uint32_t)(last - first + 1);
-  if (range_first) range_first[i] = (uint32_t)first;
-  if (first == 0 && last == n - 1) nodes[i].parent = kSentinel;
+  const uint32_t me = node_base + (uint32_t)i;
+  flags[me] = 0;
+  leaf_counts[me] = (uint32_t)(last - first + 1);
+  if (range_first) range_first[me] = leaf_base + (uint32_t)first;
+  if (first == 0 && last == n - 1) nodes[me].parent = kSentinel;
   uint32_t left, right;
   if (split == first) {
-    left = kLeafFlag | (uint32_t)split;
-    leaf_parent[split] = (uint32_t)i;
+    left = kLeafFlag | (leaf_base + (uint32_t)split);
+    leaf_parent[leaf_base + split] = me;
   } else {
-    left = (uint32_t)split;
-    nodes[split].parent = (uint32_t)i;
+    left = node_base + (uint32_t)split;
+    nodes[node_base + split].parent = me;
   }
   if (split + 1 == last) {
-    right = kLeafFlag | (uint32_t)(split + 1);
-    leaf_parent[split + 1] = (uint32_t)i;
+    right = kLeafFlag | (leaf_base + (uint32_t)(split + 1));
+    leaf_parent[leaf_base + split + 1] = me;
   } else {
-    right = (uint32_t)(split + 1);
-    nodes[split + 1].parent = (uint32_t)i;
+    right = node_base + (uint32_t)(split + 1);
+    nodes[node_base + split + 1].parent = me;
   }
-  nodes[i].left = left;
-  nodes[i].right = right;
-  nodes[i].fence = (uint32_t)last;
+  nodes[me].left = left;
+  nodes[me].right = right;
+  nodes[me].fence = leaf_base + (uint32_t)last;
+}
+
+}  // namespace
+
+// Also the guide hierarchy of the H-PLOC builder, which needs each node's
+// first sorted position (`range_first`, NULL for the plain LBVH).
+__global__ void __launch_bounds__(kBlock) karras_kernel(const uint64_t* __restrict__ keys, uint64_t n_u,
+                                                        BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_parent,
+                                                        uint32_t* __restrict__ leaf_counts,
+                                                        uint32_t* __restrict__ flags,
+                                                        uint32_t* __restrict__ range_first) {
+  const int64_t n = (int64_t)n_u;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  karras_node(keys, n, i, nodes, leaf_parent, leaf_counts, flags, range_first, 0u, 0u);
 }
 
 namespace {
@@ -232,10 +247,13 @@ namespace {
 // child boxes in shared memory; the second arriver writes the node's two
 // child boxes with plain stores).  Only nodes whose range crosses a block
 // boundary use the cross-XCD hand-off through global memory.
+// `gprim` (segmented builds) gives each sorted leaf's input index, whose low
+// key word is its index inside the segment; `node_total` bounds the climb.
 template <bool kBoxes>
 __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__ vertices,
-                                                       const uint64_t* __restrict__ keys, uint64_t n,
-                                                       BjNode* __restrict__ nodes,
+                                                       const uint64_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ gprim, uint64_t n,
+                                                       uint64_t node_total, BjNode* __restrict__ nodes,
                                                        uint32_t* __restrict__ leaf_parent,
                                                        const uint32_t* __restrict__ leaf_counts,
                                                        uint32_t* __restrict__ flags, BjTri* __restrict__ tris) {
@@ -246,12 +264,13 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t prim = (uint32_t)keys[s];
+  const uint64_t src = gprim ? (uint64_t)gprim[s] : (uint64_t)prim;
   v3 lo, hi;
   if constexpr (kBoxes) {
-    leaf_src<true>(vertices, prim, lo, hi);
+    leaf_src<true>(vertices, src, lo, hi);
   } else {
     v3 a, b, c;
-    load_tri(vertices, prim, a, b, c);
+    load_tri(vertices, src, a, b, c);
     v3 e1 = sub(b, a), e2 = sub(c, a);
     float4* out = reinterpret_cast<float4*>(tris + s);
     out[0] = make_float4(a.x, a.y, a.z, __builtin_bit_cast(float, prim));
@@ -271,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
   uint32_t parent = leaf_parent[s];
   // A Karras hierarchy over 64-bit keys is at most 64 levels deep; the index
   // and depth checks only keep a corrupted hierarchy from leaving the arrays.
-  for (int depth = 0; parent != kSentinel && parent < n - 1 && depth < 128; ++depth) {
+  for (int depth = 0; parent != kSentinel && parent < node_total && depth < 128; ++depth) {
     BjNode* node = nodes + parent;
     const uint32_t left = node->left, right = node->right;  // written by the topology launch
     const bool is_left = cur == left;
@@ -337,6 +356,132 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
     cur = parent;
     parent = node->parent;
   }
+}
+
+// ---- segmented builds (bajo's BLAS sets) ------------------------------------
+// Segment s owns sorted leaves [off_s, off_{s+1}) (every segment nonempty)
+// and internal nodes [off_s - s, off_{s+1} - s - 1): its hierarchy is the
+// single-hierarchy layout of its own triangles, shifted by those bases.
+
+// The segment holding position i: the last s with off[s] <= i
+// (off[0] = 0 <= i < off[S]); `shift` = 1 finds the segment of internal node i
+// through the node bases off[s] - s.
+BJ_D uint32_t segment_of(const uint64_t* __restrict__ off, uint32_t S, uint64_t i, uint64_t shift) {
+  uint32_t lo = 0, hi = S;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] - shift * mid <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kBlock) seg_init_kernel(uint32_t* __restrict__ acc, uint32_t S) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < 12ull * S; w += stride)
+    acc[w] = f2ord((w % 6) < 3 ? kFltMax : -kFltMax);
+}
+
+// Per-segment root and centroid boxes.  Lanes of a wave hold consecutive
+// triangles, so a wave's segments are contiguous runs: a segmented shuffle
+// reduction leaves each run's extrema in its first lane, which folds them
+// into the segment's accumulators (order-preserving integer atomics).
+template <bool kBoxes>
+__global__ void __launch_bounds__(kBlock) seg_bounds_kernel(const float* __restrict__ src, uint64_t n,
+                                                            const uint64_t* __restrict__ off, uint32_t S,
+                                                            uint32_t* __restrict__ acc) {
+  const uint32_t lane = lane_id();
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+    const uint64_t i = base + lane;
+    const bool active = i < n;
+    float r[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) r[k] = ((k % 6) < 3) ? kFltMax : -kFltMax;
+    uint32_t seg = kSentinel;
+    if (active) {
+      seg = segment_of(off, S, i, 0);
+      v3 lo, hi;
+      leaf_src<kBoxes>(src, i, lo, hi);
+      const v3 ce = scale(add(lo, hi), 0.5f);
+      r[0] = lo.x; r[1] = lo.y; r[2] = lo.z; r[3] = hi.x; r[4] = hi.y; r[5] = hi.z;
+      r[6] = ce.x; r[7] = ce.y; r[8] = ce.z; r[9] = ce.x; r[10] = ce.y; r[11] = ce.z;
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t os = __shfl_down(seg, d, 64);
+      const bool take = lane + d < 64 && os == seg;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        const float o = __shfl_down(r[k], d, 64);
+        if (take) r[k] = ((k % 6) < 3) ? fmin_sel(r[k], o) : fmax_sel(r[k], o);
+      }
+    }
+    const uint32_t prev = __shfl_up(seg, 1, 64);
+    if (active && (lane == 0 || prev != seg)) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        if ((k % 6) < 3) atomicMin(&acc[12 * (uint64_t)seg + k], f2ord(r[k]));
+        else atomicMax(&acc[12 * (uint64_t)seg + k], f2ord(r[k]));
+      }
+    }
+  }
+}
+
+// Sort keys (segment << 30 | morton30) with the input index as value; a
+// stable sort of these pairs orders each segment by (code, index), which is
+// the order of the single-hierarchy key (code << 32 | index) within it.
+template <bool kBoxes>
+__global__ void __launch_bounds__(kBlock) seg_morton_kernel(const float* __restrict__ src, uint64_t n,
+                                                            const uint64_t* __restrict__ off, uint32_t S,
+                                                            const uint32_t* __restrict__ acc,
+                                                            float* __restrict__ scene, uint64_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ values) {
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t w = gtid; w < 12ull * S; w += stride) scene[w] = ord2f(acc[w]);
+  for (uint64_t i = gtid; i < n; i += stride) {
+    const uint32_t seg = segment_of(off, S, i, 0);
+    const uint32_t* a = acc + 12 * (uint64_t)seg;
+    // bvh/gpu/builder/lbvh.mojo:46-52 with the segment's centroid box
+    const v3 cmin = mk(ord2f(a[6]), ord2f(a[7]), ord2f(a[8]));
+    const v3 cmax = mk(ord2f(a[9]), ord2f(a[10]), ord2f(a[11]));
+    const v3 inv = safe_inv(sub(cmax, cmin));
+    v3 lo, hi;
+    leaf_src<kBoxes>(src, i, lo, hi);
+    const v3 q = mul(sub(scale(add(lo, hi), 0.5f), cmin), inv);
+    keys[i] = ((uint64_t)seg << 30) | (uint64_t)morton3(q.x, q.y, q.z);
+    values[i] = (uint32_t)i;
+  }
+}
+
+// Sorted pairs -> the single-hierarchy keys (code << 32 | index inside the
+// segment); a one-leaf segment's leaf has no parent.
+__global__ void __launch_bounds__(kBlock) seg_keys_kernel(uint64_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ gprim, uint64_t n,
+                                                          const uint64_t* __restrict__ off,
+                                                          uint32_t* __restrict__ leaf_parent) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t key = keys[k];
+  const uint32_t seg = (uint32_t)(key >> 30);
+  const uint64_t first = off[seg];
+  keys[k] = ((key & 0x3FFFFFFFull) << 32) | (uint64_t)(gprim[k] - (uint32_t)first);
+  if (off[seg + 1] - first == 1) leaf_parent[k] = kSentinel;
+}
+
+__global__ void __launch_bounds__(kBlock) seg_karras_kernel(const uint64_t* __restrict__ keys, uint64_t n,
+                                                            const uint64_t* __restrict__ off, uint32_t S,
+                                                            BjNode* __restrict__ nodes,
+                                                            uint32_t* __restrict__ leaf_parent,
+                                                            uint32_t* __restrict__ leaf_counts,
+                                                            uint32_t* __restrict__ flags) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n - S) return;
+  const uint32_t seg = segment_of(off, S, j, 1);
+  const uint64_t first = off[seg], count = off[seg + 1] - first, node_base = first - seg;
+  karras_node(keys + first, (int64_t)count, (int64_t)(j - node_base), nodes, leaf_parent, leaf_counts, flags,
+              nullptr, (uint32_t)node_base, (uint32_t)first);
 }
 
 int32_t ensure_capacity(bajo_bvh* h, uint64_t n) {
@@ -409,8 +554,47 @@ int32_t ensure_hploc_capacity(bajo_bvh* h, uint64_t n) {
   return BAJO_OK;
 }
 
+void free_segments(bajo_bvh* h) {
+  (void)hipFree(h->seg_off); (void)hipFree(h->seg_acc); (void)hipFree(h->seg_scene); (void)hipFree(h->seg_prim);
+  (void)hipFree(h->seg_sort_ws);
+  h->seg_off = nullptr; h->seg_acc = nullptr; h->seg_scene = nullptr; h->seg_prim = nullptr; h->seg_sort_ws = nullptr;
+  h->seg_capacity = 0;
+  h->seg_leaf_capacity = 0;
+  h->seg_sort_ws_bytes = 0;
+}
+
+// Segment arrays of a BLAS set: offsets, bounds accumulators and boxes per
+// segment; the input index per sorted leaf; the pair-sort workspace.
+int32_t ensure_segment_capacity(bajo_bvh* h, uint64_t n, uint32_t S, size_t sort_bytes) {
+  if (S > h->seg_capacity) {
+    (void)hipFree(h->seg_off); (void)hipFree(h->seg_acc); (void)hipFree(h->seg_scene);
+    h->seg_off = nullptr; h->seg_acc = nullptr; h->seg_scene = nullptr;
+    h->seg_capacity = 0;
+    BJ_HIP(hipMalloc(&h->seg_off, ((uint64_t)S + 1) * sizeof(uint64_t)));
+    BJ_HIP(hipMalloc(&h->seg_acc, 12ull * S * sizeof(uint32_t)));
+    BJ_HIP(hipMalloc(&h->seg_scene, 12ull * S * sizeof(float)));
+    h->seg_capacity = S;
+  }
+  if (n > h->seg_leaf_capacity) {
+    (void)hipFree(h->seg_prim);
+    h->seg_prim = nullptr;
+    h->seg_leaf_capacity = 0;
+    BJ_HIP(hipMalloc(&h->seg_prim, n * sizeof(uint32_t)));
+    h->seg_leaf_capacity = n;
+  }
+  if (sort_bytes > h->seg_sort_ws_bytes) {
+    (void)hipFree(h->seg_sort_ws);
+    h->seg_sort_ws = nullptr;
+    h->seg_sort_ws_bytes = 0;
+    BJ_HIP(hipMalloc(&h->seg_sort_ws, sort_bytes));
+    h->seg_sort_ws_bytes = sort_bytes;
+  }
+  return BAJO_OK;
+}
+
 void free_bvh(bajo_bvh* h) {
   if (!h) return;
+  free_segments(h);
   free_hploc(h);
   (void)hipFree(h->nodes); (void)hipFree(h->tris); (void)hipFree(h->keys); (void)hipFree(h->leaf_parent); (void)hipFree(h->leaf_counts);
   (void)hipFree(h->flags); (void)hipFree(h->acc); (void)hipFree(h->scene); (void)hipFree(h->sort_ws);
@@ -450,6 +634,9 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
   }
   h->n = n;
   h->root = n == 1 ? (kLeafFlag | 0u) : 0u;
+  h->trav_nodes = (uint32_t)(n - 1);
+  h->trav_prims = (uint32_t)n;
+  h->wide_root = 0;
   SortPlan plan = make_sort_plan(n, 8, 0, kSortBeginBit, kSortEndBit);
   char* ws = (char*)h->sort_ws;
   uint32_t* hist = (uint32_t*)(ws + plan.off_hist);
@@ -495,11 +682,11 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
     }
     ev.record(4, st);
     if (h->boxes)
-      refit_kernel<true><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent,
-                                                               h->leaf_counts, h->flags, h->tris);
+      refit_kernel<true><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, nullptr, n, n - 1, h->nodes,
+                                                               h->leaf_parent, h->leaf_counts, h->flags, h->tris);
     else
-      refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, n, h->nodes, h->leaf_parent,
-                                                                h->leaf_counts, h->flags, h->tris);
+      refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, nullptr, n, n - 1, h->nodes,
+                                                                h->leaf_parent, h->leaf_counts, h->flags, h->tris);
     BJ_HIP(hipGetLastError());
   }
   ev.record(5, st);
@@ -531,6 +718,90 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
 
 int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo_build_options* options,
                     bajo_build_timings* timings, bajo_bvh** out, bool boxes);
+
+// bajo's segmented BLAS build (bvh/gpu/triangle_bvh.mojo:366-491 over
+// builder/segmented_build.mojo:89-162 and builder/lbvh.mojo:30-62, 235-329):
+// one pass of every stage over all segments -- per-segment boxes, a stable
+// (segment, code) pair sort, Karras per segment, one shared refit and an
+// optional 8-wide collapse whose level 0 holds the S segment roots.
+int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const uint64_t* offsets, uint32_t S,
+                        bajo_build_timings* timings) {
+  const uint64_t n = offsets[S];
+  int32_t rc = ensure_capacity(h, n);
+  if (rc) return rc;
+  if (h->node_width == BAJO_WIDTH_WIDE8) {
+    rc = ensure_wide_capacity(h, n);
+    if (rc) return rc;
+  }
+  int seg_bits = 0;
+  while ((1ull << seg_bits) < S) ++seg_bits;
+  const SortPlan plan = make_sort_plan(n, 8, 1, 0, 30 + seg_bits);
+  rc = ensure_segment_capacity(h, n, S, plan.total);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  BJ_HIP(hipMemcpyAsync(h->seg_off, offsets, ((uint64_t)S + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  BJ_HIP(hipStreamSynchronize(st));  // the caller's offsets may be released on return
+  h->n = n;
+  h->seg_count = S;
+  h->root = kSentinel;  // the store itself is traced through its segment views
+  h->trav_nodes = (uint32_t)(n - S);
+  h->trav_prims = (uint32_t)n;
+  h->wide_root = 0;
+  StageEvents ev;
+  rc = ev.create(timings != nullptr);
+  if (rc) return rc;
+  unsigned stream_blocks = div_up(n, (uint64_t)kBlock * 8);
+  if (stream_blocks > 4096) stream_blocks = 4096;
+  if (stream_blocks < 1) stream_blocks = 1;
+  unsigned seg_blocks = div_up(12ull * S, (uint64_t)kBlock);
+  if (seg_blocks > 1024) seg_blocks = 1024;
+  ev.record(0, st);
+  seg_init_kernel<<<seg_blocks, kBlock, 0, st>>>(h->seg_acc, S);
+  seg_bounds_kernel<false><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->seg_off, S, h->seg_acc);
+  BJ_HIP(hipGetLastError());
+  ev.record(1, st);
+  seg_morton_kernel<false><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->seg_off, S, h->seg_acc, h->seg_scene,
+                                                             h->keys, h->seg_prim);
+  BJ_HIP(hipGetLastError());
+  ev.record(2, st);
+  rc = onesweep_run(ctx, plan, h->keys, h->seg_prim, h->seg_sort_ws, false);
+  if (rc) return rc;
+  seg_keys_kernel<<<div_up(n, kBlock), kBlock, 0, st>>>(h->keys, h->seg_prim, n, h->seg_off, h->leaf_parent);
+  BJ_HIP(hipGetLastError());
+  ev.record(3, st);
+  if (n > S) {
+    seg_karras_kernel<<<div_up(n - S, kBlock), kBlock, 0, st>>>(h->keys, n, h->seg_off, S, h->nodes, h->leaf_parent,
+                                                                h->leaf_counts, h->flags);
+    BJ_HIP(hipGetLastError());
+  }
+  ev.record(4, st);
+  refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, h->seg_prim, n, n - S, h->nodes,
+                                                            h->leaf_parent, h->leaf_counts, h->flags, h->tris);
+  BJ_HIP(hipGetLastError());
+  ev.record(5, st);
+  if (h->node_width == BAJO_WIDTH_WIDE8) {
+    rc = collapse_wide(ctx, h);
+    if (rc) return rc;
+  } else {
+    h->wide_count = 0;
+  }
+  ev.record(6, st);
+  if (timings) {
+    BJ_HIP(hipEventSynchronize(ev.ev[6]));
+    float ms[6];
+    for (int k = 0; k < 6; ++k) BJ_HIP(hipEventElapsedTime(&ms[k], ev.ev[k], ev.ev[k + 1]));
+    timings->bounds_ns = (uint64_t)(ms[0] * 1e6);
+    timings->morton_ns = (uint64_t)(ms[1] * 1e6);
+    timings->sort_ns = (uint64_t)(ms[2] * 1e6);
+    timings->topology_ns = (uint64_t)(ms[3] * 1e6);
+    timings->refit_ns = (uint64_t)(ms[4] * 1e6);
+    timings->collapse_ns = (uint64_t)(ms[5] * 1e6);
+    float total;
+    BJ_HIP(hipEventElapsedTime(&total, ev.ev[0], ev.ev[6]));
+    timings->total_ns = (uint64_t)(total * 1e6);
+  }
+  return BAJO_OK;
+}
 }  // namespace bj
 
 extern "C" {
@@ -586,6 +857,7 @@ int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo
     BJ_REQUIRE(h->device == ctx->device, "%s: handle belongs to device %d", fn, h->device);
   }
   BJ_REQUIRE(fresh || h->boxes == boxes, "bajo_build: a handle keeps its input kind (triangles or boxes)");
+  BJ_REQUIRE(fresh || (!h->owner && !h->seg_count), "%s: a BLAS set or its segment view cannot be rebuilt here", fn);
   h->node_width = opt.node_width;
   h->method = opt.method;
   h->wide_leaf = (uint32_t)opt.leaf_size;
@@ -604,21 +876,34 @@ int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo
 extern "C" {
 
 int32_t bajo_bvh_free(bajo_bvh* bvh) {
+  BJ_REQUIRE(bvh == nullptr || bvh->owner == nullptr,
+             "bajo_bvh_free: a segment view is released with its BLAS set (bajo_blas_set_free)");
+  BJ_REQUIRE(bvh == nullptr || bvh->seg_count == 0, "bajo_bvh_free: a BLAS set is released by bajo_blas_set_free");
   bj::free_bvh(bvh);
   return BAJO_OK;
 }
+
+namespace {
+// A view's node / leaf references as the single-hierarchy layout of its
+// segment would hold them (its own records numbered from 0).
+uint32_t rebase_ref(uint32_t r, uint64_t node_base, uint64_t leaf_base) {
+  if (r == bj::kSentinel) return r;
+  if (r & bj::kLeafFlag) return bj::kLeafFlag | (uint32_t)((r & bj::kIndexMask) - leaf_base);
+  return (uint32_t)(r - node_base);
+}
+}  // namespace
 
 int32_t bajo_bvh_get_info(const bajo_bvh* h, bajo_bvh_info* out) {
   BJ_REQUIRE(h != nullptr && out != nullptr, "bajo_bvh_get_info: NULL argument");
   out->prim_count = h->n;
   out->node_count = h->n > 0 ? h->n - 1 : 0;
-  out->root = h->root;
+  out->root = h->owner ? rebase_ref(h->root, h->node_base, h->leaf_base) : h->root;
   out->node_width = (uint32_t)h->node_width;
-  out->nodes = h->nodes;
-  out->triangles = h->tris;
-  out->sorted_keys = h->keys;
-  out->leaf_parent = h->leaf_parent;
-  out->leaf_counts = h->leaf_counts;
+  out->nodes = h->nodes ? h->nodes + h->node_base : nullptr;
+  out->triangles = h->tris ? h->tris + h->leaf_base : nullptr;
+  out->sorted_keys = h->keys ? h->keys + h->leaf_base : nullptr;
+  out->leaf_parent = h->leaf_parent ? h->leaf_parent + h->leaf_base : nullptr;
+  out->leaf_counts = h->leaf_counts ? h->leaf_counts + h->node_base : nullptr;
   out->scene_bounds = h->scene;
   out->wide_node_count = h->node_width == BAJO_WIDTH_WIDE8 ? h->wide_count : 0;
   out->wide_nodes = h->node_width == BAJO_WIDTH_WIDE8 ? h->wide : nullptr;
@@ -629,12 +914,50 @@ int32_t bajo_bvh_copy_wide(bajo_ctx* ctx, const bajo_bvh* h, void* nodes, uint64
   BJ_REQUIRE(ctx != nullptr && h != nullptr, "bajo_bvh_copy_wide: NULL argument");
   BJ_REQUIRE(h->node_width == BAJO_WIDTH_WIDE8, "bajo_bvh_copy_wide: the BVH was built with node width %d",
              h->node_width);
-  if (count) *count = h->wide_count;
-  if (!nodes) return BAJO_OK;
-  BJ_REQUIRE(capacity >= h->wide_count, "bajo_bvh_copy_wide: capacity %llu < %u nodes",
-             (unsigned long long)capacity, h->wide_count);
+  BJ_REQUIRE(h->seg_count == 0, "bajo_bvh_copy_wide: copy a BLAS set's segments through their views");
+  if (!h->owner) {
+    if (count) *count = h->wide_count;
+    if (!nodes) return BAJO_OK;
+    BJ_REQUIRE(capacity >= h->wide_count, "bajo_bvh_copy_wide: capacity %llu < %u nodes",
+               (unsigned long long)capacity, h->wide_count);
+    BJ_HIP(hipStreamSynchronize(ctx->stream));
+    BJ_HIP(hipMemcpy(nodes, h->wide, (uint64_t)h->wide_count * sizeof(BjWide), hipMemcpyDeviceToHost));
+    return BAJO_OK;
+  }
+  // a segment view: the nodes reachable from its root, depth-first (children
+  // in slot order), with inner references renumbered and leaf triangle
+  // ranges relative to the segment's first sorted leaf
+  std::vector<BjWide> all(h->wide_count);
   BJ_HIP(hipStreamSynchronize(ctx->stream));
-  BJ_HIP(hipMemcpy(nodes, h->wide, (uint64_t)h->wide_count * sizeof(BjWide), hipMemcpyDeviceToHost));
+  BJ_HIP(hipMemcpy(all.data(), h->wide, all.size() * sizeof(BjWide), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> order, stack{h->wide_root}, pos(all.size(), bj::kSentinel);
+  while (!stack.empty()) {
+    const uint32_t k = stack.back();
+    stack.pop_back();
+    BJ_REQUIRE(k < all.size() && pos[k] == bj::kSentinel, "bajo_bvh_copy_wide: malformed segment hierarchy");
+    pos[k] = (uint32_t)order.size();
+    order.push_back(k);
+    for (int c = (int)all[k].n - 1; c >= 0; --c)
+      if (!(all[k].child[c] & bj::kLeafFlag)) stack.push_back(all[k].child[c]);
+  }
+  if (count) *count = order.size();
+  if (!nodes) return BAJO_OK;
+  BJ_REQUIRE(capacity >= order.size(), "bajo_bvh_copy_wide: capacity %llu < %zu nodes",
+             (unsigned long long)capacity, order.size());
+  BjWide* dst = (BjWide*)nodes;
+  for (size_t i = 0; i < order.size(); ++i) {
+    BjWide w = all[order[i]];
+    for (int c = 0; c < (int)w.n; ++c) {
+      const uint32_t r = w.child[c];
+      if (r & bj::kLeafFlag) {
+        const uint32_t start = (r & kWideLeafStartMask) - (uint32_t)h->leaf_base;
+        w.child[c] = (r & ~kWideLeafStartMask) | start;
+      } else {
+        w.child[c] = pos[r];
+      }
+    }
+    dst[i] = w;
+  }
   return BAJO_OK;
 }
 
@@ -642,11 +965,13 @@ int32_t bajo_bvh_copy_binary(bajo_ctx* ctx, const bajo_bvh* h, uint64_t* sorted_
                              uint32_t* node_meta, float* node_bounds, uint32_t* leaf_parent, uint32_t* leaf_counts,
                              float* scene_bounds) {
   BJ_REQUIRE(ctx != nullptr && h != nullptr, "bajo_bvh_copy_binary: NULL argument");
+  BJ_REQUIRE(h->seg_count == 0, "bajo_bvh_copy_binary: copy a BLAS set's segments through their views");
   const uint64_t n = h->n, m = n > 0 ? n - 1 : 0;
+  const uint64_t nb = h->node_base, lb = h->leaf_base;  // 0 unless a segment view
   BJ_HIP(hipStreamSynchronize(ctx->stream));
   uint64_t* keys = (uint64_t*)malloc(n * sizeof(uint64_t));
   BJ_REQUIRE(keys != nullptr, "bajo_bvh_copy_binary: host allocation failed");
-  hipError_t e = hipMemcpy(keys, h->keys, n * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  hipError_t e = hipMemcpy(keys, h->keys + lb, n * sizeof(uint64_t), hipMemcpyDeviceToHost);
   if (e != hipSuccess) { free(keys); return bj::hip_fail(e, "hipMemcpy(keys)", __FILE__, __LINE__); }
   if (sorted_keys) memcpy(sorted_keys, keys, n * sizeof(uint64_t));
   if (leaf_ids)
@@ -655,22 +980,106 @@ int32_t bajo_bvh_copy_binary(bajo_ctx* ctx, const bajo_bvh* h, uint64_t* sorted_
   if ((node_meta || node_bounds) && m > 0) {
     BjNode* nodes = (BjNode*)malloc(m * sizeof(BjNode));
     BJ_REQUIRE(nodes != nullptr, "bajo_bvh_copy_binary: host allocation failed");
-    e = hipMemcpy(nodes, h->nodes, m * sizeof(BjNode), hipMemcpyDeviceToHost);
+    e = hipMemcpy(nodes, h->nodes + nb, m * sizeof(BjNode), hipMemcpyDeviceToHost);
     if (e != hipSuccess) { free(nodes); return bj::hip_fail(e, "hipMemcpy(nodes)", __FILE__, __LINE__); }
     for (uint64_t i = 0; i < m; ++i) {
       if (node_bounds) memcpy(node_bounds + 12 * i, nodes[i].b, 12 * sizeof(float));
       if (node_meta) {
-        node_meta[4 * i + 0] = nodes[i].parent;
-        node_meta[4 * i + 1] = nodes[i].left;
-        node_meta[4 * i + 2] = nodes[i].right;
-        node_meta[4 * i + 3] = nodes[i].fence;
+        node_meta[4 * i + 0] = rebase_ref(nodes[i].parent, nb, lb);
+        node_meta[4 * i + 1] = rebase_ref(nodes[i].left, nb, lb);
+        node_meta[4 * i + 2] = rebase_ref(nodes[i].right, nb, lb);
+        node_meta[4 * i + 3] = (uint32_t)(nodes[i].fence - lb);
       }
     }
     free(nodes);
   }
-  if (leaf_parent) BJ_HIP(hipMemcpy(leaf_parent, h->leaf_parent, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  if (leaf_counts && m > 0) BJ_HIP(hipMemcpy(leaf_counts, h->leaf_counts, m * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (leaf_parent) {
+    BJ_HIP(hipMemcpy(leaf_parent, h->leaf_parent + lb, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; ++i) leaf_parent[i] = rebase_ref(leaf_parent[i], nb, lb);
+  }
+  if (leaf_counts && m > 0)
+    BJ_HIP(hipMemcpy(leaf_counts, h->leaf_counts + nb, m * sizeof(uint32_t), hipMemcpyDeviceToHost));
   if (scene_bounds) BJ_HIP(hipMemcpy(scene_bounds, h->scene, 12 * sizeof(float), hipMemcpyDeviceToHost));
+  return BAJO_OK;
+}
+
+// ---- BLAS sets ---------------------------------------------------------------
+
+struct bajo_blas_set {
+  bajo_bvh* store = nullptr;
+  std::vector<bajo_bvh> views;
+};
+
+int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const uint64_t* segment_offsets,
+                                     uint32_t segment_count, const bajo_build_options* options,
+                                     bajo_build_timings* timings, bajo_blas_set** out) {
+  using namespace bj;
+  const char* fn = "bajo_build_triangle_blas_set";
+  bajo_build_options opt = options ? *options : bajo_build_options{BAJO_BUILD_LBVH, BAJO_WIDTH_BINARY, 0, 0};
+  BJ_REQUIRE(ctx && vertices && segment_offsets && out, "%s: NULL argument", fn);
+  BJ_REQUIRE(segment_count > 0, "%s: a BLAS set needs at least one segment", fn);
+  BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH, "%s: segmented builds use the LBVH (method 0), got %d", fn, opt.method);
+  BJ_REQUIRE(opt.node_width == BAJO_WIDTH_BINARY || opt.node_width == BAJO_WIDTH_WIDE8,
+             "%s: node_width must be 2 or 8, got %d", fn, opt.node_width);
+  if (opt.leaf_size == 0) opt.leaf_size = (int32_t)kWideDefaultLeaf;
+  BJ_REQUIRE(opt.leaf_size >= 1 && opt.leaf_size <= (int32_t)kWideMaxLeaf, "%s: leaf_size must be in [1, 8]", fn);
+  BJ_REQUIRE(opt.reserved == 0, "%s: reserved option field must be 0", fn);
+  BJ_REQUIRE(segment_offsets[0] == 0, "%s: segment_offsets[0] must be 0", fn);
+  for (uint32_t s = 0; s < segment_count; ++s)
+    BJ_REQUIRE(segment_offsets[s + 1] > segment_offsets[s], "%s: segment %u is empty (offsets must increase)", fn, s);
+  const uint64_t n = segment_offsets[segment_count];
+  BJ_REQUIRE(n <= kSortMaxKeys, "%s: %llu triangles exceed 2^30-1", fn, (unsigned long long)n);
+  BJ_REQUIRE(opt.node_width != BAJO_WIDTH_WIDE8 || n < kWideMaxPrims, "%s: %llu triangles exceed the 8-wide limit",
+             fn, (unsigned long long)n);
+  bajo_blas_set* set = new bajo_blas_set();
+  bajo_bvh* h = new bajo_bvh();
+  memset(h, 0, sizeof(*h));
+  h->device = ctx->device;
+  h->node_width = opt.node_width;
+  h->method = opt.method;
+  h->wide_leaf = (uint32_t)opt.leaf_size;
+  set->store = h;
+  int32_t rc = build_segmented(ctx, h, vertices, segment_offsets, segment_count, timings);
+  if (rc) {
+    free_bvh(h);
+    delete set;
+    return rc;
+  }
+  set->views.resize(segment_count);
+  for (uint32_t s = 0; s < segment_count; ++s) {
+    bajo_bvh& v = set->views[s];
+    v = *h;
+    const uint64_t first = segment_offsets[s], count = segment_offsets[s + 1] - first;
+    v.seg_count = 0;
+    v.owner = h;
+    v.n = count;
+    v.leaf_base = first;
+    v.node_base = first - s;
+    v.root = count == 1 ? (kLeafFlag | (uint32_t)first) : (uint32_t)(first - s);
+    v.scene = h->seg_scene + 12 * (uint64_t)s;
+    v.wide_root = s;
+  }
+  *out = set;
+  return BAJO_OK;
+}
+
+int32_t bajo_blas_set_count(const bajo_blas_set* set, uint32_t* count) {
+  BJ_REQUIRE(set && count, "bajo_blas_set_count: NULL argument");
+  *count = (uint32_t)set->views.size();
+  return BAJO_OK;
+}
+
+int32_t bajo_blas_set_get(const bajo_blas_set* set, uint32_t index, bajo_bvh** view) {
+  BJ_REQUIRE(set && view, "bajo_blas_set_get: NULL argument");
+  BJ_REQUIRE(index < set->views.size(), "bajo_blas_set_get: index %u of %zu segments", index, set->views.size());
+  *view = const_cast<bajo_bvh*>(&set->views[index]);
+  return BAJO_OK;
+}
+
+int32_t bajo_blas_set_free(bajo_blas_set* set) {
+  if (!set) return BAJO_OK;
+  bj::free_bvh(set->store);
+  delete set;
   return BAJO_OK;
 }
 

@@ -303,7 +303,8 @@ int32_t bajo_tlas_create(bajo_ctx* ctx, const bajo_bvh* const* blases, uint32_t 
   std::vector<float> roots(6 * (size_t)blas_count);
   for (uint32_t b = 0; b < blas_count; ++b) {
     const bajo_bvh* h = blases[b];
-    BJ_REQUIRE(h != nullptr && !h->boxes && h->n > 0, "bajo_tlas_create: BLAS %u is not a triangle BVH", b);
+    BJ_REQUIRE(h != nullptr && !h->boxes && h->n > 0 && !h->seg_count,
+               "bajo_tlas_create: BLAS %u is not a triangle BVH (a BLAS set enters through its segment views)", b);
     BJ_REQUIRE(h->device == ctx->device, "bajo_tlas_create: BLAS %u belongs to device %d", b, h->device);
     BJ_HIP(hipMemcpyAsync(roots.data() + 6 * (size_t)b, h->scene, 6 * sizeof(float), hipMemcpyDeviceToHost,
                           ctx->stream));

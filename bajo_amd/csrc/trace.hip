@@ -287,6 +287,7 @@ int32_t bajo_trace_rays(bajo_ctx* ctx, const bajo_bvh* bvh, int32_t mode, const 
   BJ_REQUIRE(rays != nullptr && hits != nullptr, "bajo_trace_rays: NULL rays/hits");
   BJ_REQUIRE(bvh->n > 0, "bajo_trace_rays: empty BVH");
   BJ_REQUIRE(!bvh->boxes, "bajo_trace_rays: the hierarchy was built over boxes, not triangles");
+  BJ_REQUIRE(!bvh->seg_count, "bajo_trace_rays: trace a BLAS set through its segment views (bajo_blas_set_get)");
   if (mode == BAJO_TRACE_ANY_HIT) {
     BJ_REQUIRE(stats == nullptr, "bajo_trace_rays: stats are collected for closest-hit queries only");
     return bj::launch<BAJO_TRACE_ANY_HIT, false, false>(ctx, bvh, rays, nullptr, 0, 0, hits, ray_count, nullptr);
@@ -303,6 +304,7 @@ int32_t bajo_trace_camera(bajo_ctx* ctx, const bajo_bvh* bvh, const float* camer
   BJ_REQUIRE(camera != nullptr && hits != nullptr, "bajo_trace_camera: NULL camera/hits");
   BJ_REQUIRE(bvh->n > 0, "bajo_trace_camera: empty BVH");
   BJ_REQUIRE(!bvh->boxes, "bajo_trace_camera: the hierarchy was built over boxes, not triangles");
+  BJ_REQUIRE(!bvh->seg_count, "bajo_trace_camera: trace a BLAS set through its segment views (bajo_blas_set_get)");
   if (stats)
     return bj::launch<BAJO_TRACE_CLOSEST_HIT, true, true>(ctx, bvh, nullptr, camera, width, height, hits, ray_count, stats);
   return bj::launch<BAJO_TRACE_CLOSEST_HIT, false, true>(ctx, bvh, nullptr, camera, width, height, hits, ray_count,

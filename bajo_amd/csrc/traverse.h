@@ -266,6 +266,7 @@ struct WideView {
   const BjTri* tris;
   uint32_t node_count, prim_count;
   uint32_t* status;
+  uint32_t root;
 };
 
 BJ_D float ubyte_f(uint32_t word, int k) { return (float)((word >> (8 * k)) & 0xFFu); }
@@ -310,7 +311,7 @@ struct Walk8 {
     pz = rcp.z >= 0.0f;
     occluded = false;
     st.sp = 0;
-    cur = wv.node_count == 0 ? kSentinel : 0u;
+    cur = wv.node_count == 0 ? kSentinel : wv.root;
     n_visit = n_inner = n_leaf = max_sp = 0;
   }
 
@@ -480,8 +481,9 @@ struct AccelView {
 // Host: the view of a built hierarchy; `status` receives traversal errors.
 inline AccelView accel_view(const bajo_bvh* h, uint32_t* status) {
   AccelView av;
-  av.bin = BvhView{h->nodes, h->tris, h->root, (uint32_t)(h->n - 1), (uint32_t)h->n, status};
-  av.wide = WideView{h->wide, h->tris, h->node_width == BAJO_WIDTH_WIDE8 ? h->wide_count : 0u, (uint32_t)h->n, status};
+  av.bin = BvhView{h->nodes, h->tris, h->root, h->trav_nodes, h->trav_prims, status};
+  av.wide = WideView{h->wide, h->tris, h->node_width == BAJO_WIDTH_WIDE8 ? h->wide_count : 0u, h->trav_prims, status,
+                     h->wide_root};
   return av;
 }
 

@@ -118,6 +118,47 @@ __global__ void collapse_root_kernel(const float* __restrict__ scene, uint32_t n
   ctl[kCtlEnd] = 1;
 }
 
+// Segmented builds: wide node s is the root of segment s (S roots form level
+// 0).  A one-triangle segment's root is written here, with a single leaf
+// child spanning the segment's box; its wide_src entry marks it done.
+__global__ void seg_collapse_root_kernel(const float* __restrict__ seg_scene, const uint64_t* __restrict__ off,
+                                         uint32_t S, BjWide* __restrict__ wide, uint32_t* __restrict__ wide_src,
+                                         uint32_t* __restrict__ ctl) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gtid == 0) {
+    ctl[kCtlStatus] = 0;
+    ctl[kCtlCount] = S;
+    ctl[kCtlBegin] = 0;
+    ctl[kCtlEnd] = S;
+  }
+  for (uint32_t s = gtid; s < S; s += stride) {
+    const uint64_t first = off[s], count = off[s + 1] - first;
+    if (count > 1) {
+      wide_src[s] = (uint32_t)(first - s);
+      continue;
+    }
+    const float* b = seg_scene + 12 * (uint64_t)s;
+    BjWide w;
+    w.ox = b[0]; w.oy = b[1]; w.oz = b[2];
+    w.ex = (uint8_t)grid_exponent(b[0], b[3]);
+    w.ey = (uint8_t)grid_exponent(b[1], b[4]);
+    w.ez = (uint8_t)grid_exponent(b[2], b[5]);
+    w.n = 1;
+    for (int i = 0; i < 8; ++i) {
+      w.lo_x[i] = w.lo_y[i] = w.lo_z[i] = 255;
+      w.hi_x[i] = w.hi_y[i] = w.hi_z[i] = 0;
+      w.child[i] = 0;
+    }
+    quantize(b[0], w.ex, b[0], b[3], w.lo_x[0], w.hi_x[0]);
+    quantize(b[1], w.ey, b[1], b[4], w.lo_y[0], w.hi_y[0]);
+    quantize(b[2], w.ez, b[2], b[5], w.lo_z[0], w.hi_z[0]);
+    w.child[0] = kLeafFlag | (uint32_t)first;
+    wide[s] = w;
+    wide_src[s] = kSentinel;
+  }
+}
+
 // The next level is the nodes reserved by the last one, clipped to the
 // capacity; after a capacity failure the level range is left empty.
 __global__ void level_advance_kernel(uint32_t* __restrict__ ctl, uint32_t capacity) {
@@ -142,7 +183,7 @@ __global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const Bj
   // whole waves iterate together so the slot reservation below can ballot
   for (uint32_t base = begin + (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); base < end; base += stride) {
     const uint32_t w = base + lane;
-    const bool active = w < end;
+    const bool active = w < end && wide_src[w] != kSentinel;
     // The children live in registers: every index into ch[] below is a
     // compile-time constant after unrolling (a data-dependent index would
     // place the array in scratch memory).
@@ -268,7 +309,13 @@ int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h) {
   int dev = 0, cus = 0;
   BJ_HIP(hipGetDevice(&dev));
   BJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  collapse_root_kernel<<<1, 64, 0, st>>>(h->scene, n, n > 1 ? h->root : 0u, h->wide, h->wide_src, h->wide_ctl);
+  if (h->seg_count) {
+    const uint32_t S = h->seg_count;
+    seg_collapse_root_kernel<<<div_up(S, 256u) < 1024u ? div_up(S, 256u) : 1024u, 256, 0, st>>>(
+        h->seg_scene, h->seg_off, S, h->wide, h->wide_src, h->wide_ctl);
+  } else {
+    collapse_root_kernel<<<1, 64, 0, st>>>(h->scene, n, n > 1 ? h->root : 0u, h->wide, h->wide_src, h->wide_ctl);
+  }
   BJ_HIP(hipGetLastError());
   if (n == 1) {
     h->wide_count = 1;

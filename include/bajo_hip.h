@@ -83,7 +83,11 @@ typedef struct {
   uint64_t total_ns;
 } bajo_build_timings;
 
-/* Shape of a built BVH (host copy of its sizes and device addresses). */
+/* Shape of a built BVH (host copy of its sizes and device addresses).  For a
+ * segment view of a BLAS set the addresses point at the segment's first
+ * records inside the set's shared arrays, whose references are numbered
+ * across the whole set; `root` is the segment's own (0 or leaf 0), as in the
+ * export of bajo_bvh_copy_binary. */
 typedef struct {
   uint64_t prim_count;
   uint64_t node_count;       /* internal nodes = prim_count - 1         */
@@ -174,14 +178,18 @@ int32_t bajo_bvh_get_info(const bajo_bvh* bvh, bajo_bvh_info* out);
  * (bvh/gpu/builder/binary_layout.mojo:303-330): any pointer may be NULL.
  * sorted_keys u64[n], leaf_ids u32[n], node_meta u32[4(n-1)] (parent, left,
  * right, fence), node_bounds f32[12(n-1)], leaf_parent u32[n],
- * leaf_counts u32[n-1], scene_bounds f32[12].  Host pointers; synchronizes. */
+ * leaf_counts u32[n-1], scene_bounds f32[12].  Host pointers; synchronizes.
+ * A segment view exports its segment renumbered from 0, equal to the
+ * standalone build of its mesh. */
 int32_t bajo_bvh_copy_binary(bajo_ctx* ctx, const bajo_bvh* bvh, uint64_t* sorted_keys, uint32_t* leaf_ids,
                              uint32_t* node_meta, float* node_bounds, uint32_t* leaf_parent,
                              uint32_t* leaf_counts, float* scene_bounds);
 /* Copies the 8-wide nodes (96 bytes each, layout in bajo_bvh_info) into a
  * host buffer of `capacity` nodes; *count receives the node count.  Node
  * numbering depends on the order in which waves reserve nodes, so only the
- * hierarchy it describes is reproducible (as for bajo's wide collapse). */
+ * hierarchy it describes is reproducible (as for bajo's wide collapse).  A
+ * segment view copies the nodes reachable from its root in depth-first order,
+ * leaf ranges relative to its first triangle. */
 int32_t bajo_bvh_copy_wide(bajo_ctx* ctx, const bajo_bvh* bvh, void* nodes, uint64_t capacity, uint64_t* count);
 
 /* ---- traversal ----------------------------------------------------------
@@ -212,6 +220,25 @@ int32_t bajo_ao_rays(bajo_ctx* ctx, const float* rays, const float* hits, uint64
  * flattened to 20 floats (bvh/camera.mojo:210-231).  Host-side helper. */
 int32_t bajo_camera_make(const float* origin3, const float* target3, const float* up3, float fov_scale,
                          float focus_dist, float* out20);
+
+/* ---- BLAS sets: bajo/bvh/gpu/triangle_bvh.mojo:366-491, blas_storage.mojo ----
+ * build_triangle_blas_set (triangle_bvh.mojo:469-491): the meshes of a set
+ * built in one segmented pass (every stage one launch over all segments).
+ * vertices f32[9N] hold the meshes back to back; segment_offsets (host)
+ * u64[S + 1] hold each mesh's first triangle (offsets[0] = 0, increasing,
+ * offsets[S] = N).  Each segment's hierarchy equals the standalone build of
+ * its mesh; hit prim ids are indices inside the mesh.  Method LBVH, node
+ * width 2 or 8.  bajo_blas_set_get returns segment k as a bajo_bvh view
+ * (bajo's BlasDesc, blas_desc.mojo) that traces, exports and enters a TLAS
+ * like any BVH; it shares the set's storage, cannot be rebuilt or freed on
+ * its own, and lives until bajo_blas_set_free. */
+typedef struct bajo_blas_set bajo_blas_set;
+int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const uint64_t* segment_offsets,
+                                     uint32_t segment_count, const bajo_build_options* options,
+                                     bajo_build_timings* timings, bajo_blas_set** out);
+int32_t bajo_blas_set_count(const bajo_blas_set* set, uint32_t* count);
+int32_t bajo_blas_set_get(const bajo_blas_set* set, uint32_t index, bajo_bvh** view);
+int32_t bajo_blas_set_free(bajo_blas_set* set);
 
 /* ---- instances: bajo/bvh/gpu/tlas.mojo -----------------------------------
  * build_triangle_tlas (bvh/gpu/tlas.mojo:618-645): a top-level hierarchy

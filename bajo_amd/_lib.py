@@ -174,6 +174,7 @@ SIGNATURES = {
     "bajo_blas_set_count": (c_int32, [c_void_p, POINTER(c_uint32)]),
     "bajo_blas_set_get": (c_int32, [c_void_p, c_uint32, POINTER(c_void_p)]),
     "bajo_blas_set_free": (c_int32, [c_void_p]),
+    "bajo_blas_set_trace_camera": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
     "bajo_tlas_create": (
         c_int32, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, POINTER(BuildOptions), POINTER(c_void_p)],
     ),

@@ -240,6 +240,27 @@ class GpuTriangleBlasSet:
     def blases(self) -> list:
         return [self.blas(k) for k in range(self.blas_count)]
 
+    def rebuild(self, vertices: torch.Tensor, offsets: np.ndarray, timings: bool = False):
+        """Rebuilds in place (same mesh count) over meshes stored back to back; views stay valid."""
+        if len(offsets) - 1 != self.blas_count:
+            raise ValueError("a rebuild keeps the mesh count")
+        t = {} if timings else None
+        _build_set(vertices, offsets, self.ctx, self.node_width, self.leaf_size, t, ctypes.c_void_p(self.handle.value))
+        self.counts = np.diff(np.asarray(offsets, np.int64)).tolist()
+        return t
+
+    def launch_camera(self, cameras, hits, width, height, stats=None) -> None:
+        """One camera launch over every mesh: view e (cameras f32[20 S]) traces mesh e; hits f32[8 S w h]."""
+        require_device(cameras, self.ctx, "cameras")
+        require_device(hits, self.ctx, "hits")
+        R = self.blas_count * int(width) * int(height)
+        if cameras.numel() < self.blas_count * CAMERA_STRIDE:
+            raise ValueError("one 20-float camera per mesh is needed")
+        if hits.numel() < R * HIT_STRIDE or (stats is not None and stats.numel() < R * STATS_STRIDE):
+            raise ValueError("hit / stats output buffer is too short")
+        _lib.check(_lib.lib().bajo_blas_set_trace_camera(self.ctx.handle, self.handle, ptr(cameras), ptr(hits),
+                                                         int(width), int(height), ptr(stats)))
+
     def free(self) -> None:
         if self._handle:
             _lib.lib().bajo_blas_set_free(self._handle)
@@ -252,6 +273,22 @@ class GpuTriangleBlasSet:
             pass
 
 
+def _build_set(flat: torch.Tensor, offsets: np.ndarray, ctx: DeviceContext, node_width: int, leaf_size: int,
+               timings: dict | None, handle: ctypes.c_void_p) -> ctypes.c_void_p:
+    _check_vertices(flat, ctx)
+    off = np.ascontiguousarray(offsets, np.uint64)
+    if off.size < 2 or int(off[-1]) * 9 != flat.numel():
+        raise ValueError("segment offsets must end at the triangle count of the vertex buffer")
+    t = _lib.BuildTimings() if timings is not None else None
+    opts = build_options(node_width, leaf_size, GpuBvhBuildMethod.LBVH)
+    _lib.check(_lib.lib().bajo_build_triangle_blas_set(
+        ctx.handle, ptr(flat), off.ctypes.data_as(ctypes.c_void_p), off.size - 1, ctypes.byref(opts),
+        ctypes.byref(t) if t else None, ctypes.byref(handle)))
+    if t is not None:
+        timings.update(t.as_dict())
+    return handle
+
+
 def build_triangle_blas_set(vertex_sets: list, ctx: DeviceContext | None = None, node_width: int = 2,
                             leaf_size: int = 0, timings: dict | None = None) -> GpuTriangleBlasSet:
     """build_triangle_blas_set[W, L, LBVH] (bvh/gpu/triangle_bvh.mojo:469-491): one segmented LBVH build over
@@ -262,14 +299,17 @@ def build_triangle_blas_set(vertex_sets: list, ctx: DeviceContext | None = None,
     counts = [_check_vertices(v, ctx) for v in vertex_sets]
     flat = torch.cat([v.reshape(-1) for v in vertex_sets]).contiguous()
     offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
-    t = _lib.BuildTimings() if timings is not None else None
-    handle = ctypes.c_void_p()
-    opts = build_options(node_width, leaf_size, GpuBvhBuildMethod.LBVH)
-    _lib.check(_lib.lib().bajo_build_triangle_blas_set(
-        ctx.handle, ptr(flat), offsets.ctypes.data_as(ctypes.c_void_p), len(counts), ctypes.byref(opts),
-        ctypes.byref(t) if t else None, ctypes.byref(handle)))
-    if t is not None:
-        timings.update(t.as_dict())
+    return build_triangle_blas_set_flat(flat, offsets, ctx, node_width, leaf_size, timings)
+
+
+def build_triangle_blas_set_flat(vertices: torch.Tensor, offsets: np.ndarray, ctx: DeviceContext | None = None,
+                                 node_width: int = 2, leaf_size: int = 0,
+                                 timings: dict | None = None) -> GpuTriangleBlasSet:
+    """The same build over meshes stored back to back in `vertices`, mesh k = triangles
+    [offsets[k], offsets[k + 1])."""
+    ctx = ctx or default_context(vertices.device.index or 0)
+    handle = _build_set(vertices, offsets, ctx, node_width, leaf_size, timings, ctypes.c_void_p())
+    counts = np.diff(np.asarray(offsets, np.int64)).tolist()
     return GpuTriangleBlasSet(handle, ctx, counts, node_width, leaf_size)
 
 

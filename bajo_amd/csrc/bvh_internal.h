@@ -78,7 +78,7 @@ struct bajo_bvh {
   // input kind: triangles f32[9N] (traceable) or leaf boxes f32[6N]
   bool boxes;
   // segmented build (a BLAS set): seg_count > 0 segments share this storage;
-  // segment s owns sorted leaves [off_s, off_{s+1}) and nodes from off_s - s
+  // segment s owns sorted leaves [off_s, off_{s+1}) and nodes [off_s, off_{s+1} - 1)
   uint32_t seg_count;
   uint64_t seg_capacity;
   uint64_t* seg_off;      // device u64[S + 1]
@@ -104,6 +104,8 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* src);
 // Builds the segments of a BLAS set into h (lbvh.hip); offsets are host u64[S + 1].
 int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const uint64_t* offsets, uint32_t S,
                         bajo_build_timings* timings);
+// The shared storage of a BLAS set (lbvh.hip).
+const bajo_bvh* blas_set_store(const bajo_blas_set* set);
 // Karras topology over sorted 64-bit keys (lbvh.hip); `range_first` may be NULL.
 __global__ void karras_kernel(const uint64_t* __restrict__ keys, uint64_t n_u, BjNode* __restrict__ nodes,
                               uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ leaf_counts,

@@ -360,17 +360,18 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
 
 // ---- segmented builds (bajo's BLAS sets) ------------------------------------
 // Segment s owns sorted leaves [off_s, off_{s+1}) (every segment nonempty)
-// and internal nodes [off_s - s, off_{s+1} - s - 1): its hierarchy is the
-// single-hierarchy layout of its own triangles, shifted by those bases.
+// and internal nodes [off_s, off_{s+1} - 1): its hierarchy is the
+// single-hierarchy layout of its own triangles, shifted by off_s.  Node slot
+// off_{s+1} - 1 stays unused, so a node's index is one end of its leaf range
+// as in a single hierarchy, which keeps the refit's block-local merges.
 
 // The segment holding position i: the last s with off[s] <= i
-// (off[0] = 0 <= i < off[S]); `shift` = 1 finds the segment of internal node i
-// through the node bases off[s] - s.
-BJ_D uint32_t segment_of(const uint64_t* __restrict__ off, uint32_t S, uint64_t i, uint64_t shift) {
+// (off[0] = 0 <= i < off[S]).
+BJ_D uint32_t segment_of(const uint64_t* __restrict__ off, uint32_t S, uint64_t i) {
   uint32_t lo = 0, hi = S;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (off[mid] - shift * mid <= i) lo = mid;
+    if (off[mid] <= i) lo = mid;
     else hi = mid;
   }
   return lo;
@@ -400,7 +401,7 @@ __global__ void __launch_bounds__(kBlock) seg_bounds_kernel(const float* __restr
     for (int k = 0; k < 12; ++k) r[k] = ((k % 6) < 3) ? kFltMax : -kFltMax;
     uint32_t seg = kSentinel;
     if (active) {
-      seg = segment_of(off, S, i, 0);
+      seg = segment_of(off, S, i);
       v3 lo, hi;
       leaf_src<kBoxes>(src, i, lo, hi);
       const v3 ce = scale(add(lo, hi), 0.5f);
@@ -441,7 +442,7 @@ __global__ void __launch_bounds__(kBlock) seg_morton_kernel(const float* __restr
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t w = gtid; w < 12ull * S; w += stride) scene[w] = ord2f(acc[w]);
   for (uint64_t i = gtid; i < n; i += stride) {
-    const uint32_t seg = segment_of(off, S, i, 0);
+    const uint32_t seg = segment_of(off, S, i);
     const uint32_t* a = acc + 12 * (uint64_t)seg;
     // bvh/gpu/builder/lbvh.mojo:46-52 with the segment's centroid box
     const v3 cmin = mk(ord2f(a[6]), ord2f(a[7]), ord2f(a[8]));
@@ -477,11 +478,12 @@ __global__ void __launch_bounds__(kBlock) seg_karras_kernel(const uint64_t* __re
                                                             uint32_t* __restrict__ leaf_counts,
                                                             uint32_t* __restrict__ flags) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n - S) return;
-  const uint32_t seg = segment_of(off, S, j, 1);
-  const uint64_t first = off[seg], count = off[seg + 1] - first, node_base = first - seg;
-  karras_node(keys + first, (int64_t)count, (int64_t)(j - node_base), nodes, leaf_parent, leaf_counts, flags,
-              nullptr, (uint32_t)node_base, (uint32_t)first);
+  if (j + 1 >= n) return;
+  const uint32_t seg = segment_of(off, S, j);
+  const uint64_t first = off[seg], count = off[seg + 1] - first;
+  if (j - first + 1 >= count) return;  // the segment's unused last slot
+  karras_node(keys + first, (int64_t)count, (int64_t)(j - first), nodes, leaf_parent, leaf_counts, flags, nullptr,
+              (uint32_t)first, (uint32_t)first);
 }
 
 int32_t ensure_capacity(bajo_bvh* h, uint64_t n) {
@@ -744,7 +746,7 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
   h->n = n;
   h->seg_count = S;
   h->root = kSentinel;  // the store itself is traced through its segment views
-  h->trav_nodes = (uint32_t)(n - S);
+  h->trav_nodes = (uint32_t)(n - 1);
   h->trav_prims = (uint32_t)n;
   h->wide_root = 0;
   StageEvents ev;
@@ -770,12 +772,12 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
   BJ_HIP(hipGetLastError());
   ev.record(3, st);
   if (n > S) {
-    seg_karras_kernel<<<div_up(n - S, kBlock), kBlock, 0, st>>>(h->keys, n, h->seg_off, S, h->nodes, h->leaf_parent,
+    seg_karras_kernel<<<div_up(n - 1, kBlock), kBlock, 0, st>>>(h->keys, n, h->seg_off, S, h->nodes, h->leaf_parent,
                                                                 h->leaf_counts, h->flags);
     BJ_HIP(hipGetLastError());
   }
   ev.record(4, st);
-  refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, h->seg_prim, n, n - S, h->nodes,
+  refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, h->seg_prim, n, n - 1, h->nodes,
                                                             h->leaf_parent, h->leaf_counts, h->flags, h->tris);
   BJ_HIP(hipGetLastError());
   ev.record(5, st);
@@ -1031,18 +1033,29 @@ int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const
   BJ_REQUIRE(n <= kSortMaxKeys, "%s: %llu triangles exceed 2^30-1", fn, (unsigned long long)n);
   BJ_REQUIRE(opt.node_width != BAJO_WIDTH_WIDE8 || n < kWideMaxPrims, "%s: %llu triangles exceed the 8-wide limit",
              fn, (unsigned long long)n);
-  bajo_blas_set* set = new bajo_blas_set();
-  bajo_bvh* h = new bajo_bvh();
-  memset(h, 0, sizeof(*h));
-  h->device = ctx->device;
+  bajo_blas_set* set = *out;
+  const bool fresh = set == nullptr;
+  if (!fresh) {
+    // rebuild in place: same segment count, so the views stay where they are
+    BJ_REQUIRE(set->store->device == ctx->device, "%s: the set belongs to device %d", fn, set->store->device);
+    BJ_REQUIRE(set->views.size() == segment_count, "%s: a rebuild keeps the segment count (%zu, got %u)", fn,
+               set->views.size(), segment_count);
+  } else {
+    set = new bajo_blas_set();
+    set->store = new bajo_bvh();
+    memset(set->store, 0, sizeof(*set->store));
+    set->store->device = ctx->device;
+  }
+  bajo_bvh* h = set->store;
   h->node_width = opt.node_width;
   h->method = opt.method;
   h->wide_leaf = (uint32_t)opt.leaf_size;
-  set->store = h;
   int32_t rc = build_segmented(ctx, h, vertices, segment_offsets, segment_count, timings);
   if (rc) {
-    free_bvh(h);
-    delete set;
+    if (fresh) {
+      free_bvh(h);
+      delete set;
+    }
     return rc;
   }
   set->views.resize(segment_count);
@@ -1054,14 +1067,20 @@ int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const
     v.owner = h;
     v.n = count;
     v.leaf_base = first;
-    v.node_base = first - s;
-    v.root = count == 1 ? (kLeafFlag | (uint32_t)first) : (uint32_t)(first - s);
+    v.node_base = first;
+    v.root = count == 1 ? (kLeafFlag | (uint32_t)first) : (uint32_t)first;
     v.scene = h->seg_scene + 12 * (uint64_t)s;
     v.wide_root = s;
   }
   *out = set;
   return BAJO_OK;
 }
+
+}  // extern "C"
+
+const bajo_bvh* bj::blas_set_store(const bajo_blas_set* set) { return set ? set->store : nullptr; }
+
+extern "C" {
 
 int32_t bajo_blas_set_count(const bajo_blas_set* set, uint32_t* count) {
   BJ_REQUIRE(set && count, "bajo_blas_set_count: NULL argument");

@@ -23,7 +23,17 @@
 namespace bj {
 namespace {
 
-template <int kMode, bool kStats, bool kCamera, bool kWide>
+// Batched camera launches over a BLAS set: camera view e traces segment e,
+// whose root follows from the segment offsets (one-triangle segments are a
+// single leaf; an 8-wide segment's root is wide node e).
+template <bool kWide>
+BJ_D uint32_t segment_root(const uint64_t* __restrict__ off, uint64_t e) {
+  if constexpr (kWide) return (uint32_t)e;
+  const uint64_t first = off[e];
+  return off[e + 1] - first == 1 ? (kLeafFlag | (uint32_t)first) : (uint32_t)first;
+}
+
+template <int kMode, bool kStats, bool kCamera, bool kWide, bool kSeg>
 __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const AccelView av,
                                                             const float* __restrict__ rays,
                                                             const float* __restrict__ cams, uint32_t cam_w,
@@ -31,7 +41,8 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const AccelView av,
                                                             float* __restrict__ hits, uint64_t R,
                                                             uint32_t* __restrict__ stats,
                                                             uint32_t* __restrict__ counter,
-                                                            uint32_t* __restrict__ spill) {
+                                                            uint32_t* __restrict__ spill,
+                                                            const uint64_t* __restrict__ seg_off) {
   __shared__ uint32_t s_stack[kLdsStack * kTraceBlock];
   const uint32_t lane = lane_id();
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -63,7 +74,13 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const AccelView av,
       ray.tmax = rays[7 * R + r];
     }
     TraceCounters tc;
-    Hit hit = trace_accel<kMode, kStats, kWide>(av, ray, st, &tc);
+    AccelView v = av;
+    if constexpr (kSeg) {
+      const uint32_t root = segment_root<kWide>(seg_off, r / ((uint64_t)cam_w * cam_h));
+      if constexpr (kWide) v.wide.root = root;
+      else v.bin.root = root;
+    }
+    Hit hit = trace_accel<kMode, kStats, kWide>(v, ray, st, &tc);
     store_hit(hits, r, hit);
     if constexpr (kStats) {
       uint32_t* s = stats + 5 * r;
@@ -81,14 +98,15 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const AccelView av,
 constexpr uint32_t kRefillLanes = 16;
 constexpr uint32_t kRefillChunk = 256;
 
-template <int kMode, bool kStats, bool kCamera>
+template <int kMode, bool kStats, bool kCamera, bool kSeg>
 __global__ void __launch_bounds__(kTraceBlock) trace_refill_kernel(const WideView wv, const float* __restrict__ rays,
                                                                    const float* __restrict__ cams, uint32_t cam_w,
                                                                    uint32_t cam_h, float inv_height,
                                                                    float* __restrict__ hits, uint64_t R,
                                                                    uint32_t* __restrict__ stats,
                                                                    uint32_t* __restrict__ counter,
-                                                                   uint32_t* __restrict__ spill) {
+                                                                   uint32_t* __restrict__ spill,
+                                                                   const uint64_t* __restrict__ seg_off) {
   __shared__ uint32_t s_stack[kLdsStack * kTraceBlock];
   const uint32_t lane = lane_id();
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -127,7 +145,13 @@ __global__ void __launch_bounds__(kTraceBlock) trace_refill_kernel(const WideVie
             ray.d = mk(rays[4 * R + r], rays[5 * R + r], rays[6 * R + r]);
             ray.tmax = rays[7 * R + r];
           }
-          w.begin(wv, ray, st);
+          if constexpr (kSeg) {
+            WideView sv = wv;
+            sv.root = segment_root<true>(seg_off, r / ((uint64_t)cam_w * cam_h));
+            w.begin(sv, ray, st);
+          } else {
+            w.begin(wv, ray, st);
+          }
           active = true;
         }
         next = next + n_idle < stop ? next + n_idle : stop;
@@ -163,7 +187,7 @@ struct TraceScratch {
   unsigned grid;
 };
 
-template <int kMode, bool kStats, bool kCamera, bool kWide>
+template <int kMode, bool kStats, bool kCamera, bool kWide, bool kSeg>
 int32_t occupancy_grid(unsigned* grid) {
   static unsigned cached = 0;
   if (!cached) {
@@ -171,10 +195,11 @@ int32_t occupancy_grid(unsigned* grid) {
     BJ_HIP(hipGetDevice(&dev));
     BJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     if constexpr (kWide)
-      BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_refill_kernel<kMode, kStats, kCamera>,
+      BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_refill_kernel<kMode, kStats, kCamera, kSeg>,
                                                           kTraceBlock, 0));
     else
-      BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<kMode, kStats, kCamera, kWide>,
+      BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                          trace_kernel<kMode, kStats, kCamera, kWide, kSeg>,
                                                           kTraceBlock, 0));
     if (per_cu < 1) per_cu = 1;
     cached = (unsigned)(cus * per_cu);
@@ -183,11 +208,11 @@ int32_t occupancy_grid(unsigned* grid) {
   return BAJO_OK;
 }
 
-template <int kMode, bool kStats, bool kCamera, bool kWide>
+template <int kMode, bool kStats, bool kCamera, bool kWide, bool kSeg = false>
 int32_t launch_width(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const float* cams, uint32_t w, uint32_t hh,
                      float* hits, uint64_t R, uint32_t* stats) {
   unsigned grid = 0;
-  int32_t rc = occupancy_grid<kMode, kStats, kCamera, kWide>(&grid);
+  int32_t rc = occupancy_grid<kMode, kStats, kCamera, kWide, kSeg>(&grid);
   if (rc) return rc;
   uint64_t need_blocks = (R + kTraceBlock - 1) / kTraceBlock;
   if ((uint64_t)grid > need_blocks) grid = (unsigned)need_blocks;
@@ -201,12 +226,13 @@ int32_t launch_width(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const 
   uint32_t* spill = (uint32_t*)((char*)scratch + 256);
   trace_init_kernel<<<1, 64, 0, ctx->stream>>>(counter, status);
   const AccelView av = accel_view(h, status);
+  const uint64_t* seg_off = kSeg ? h->seg_off : nullptr;
   if constexpr (kWide)
-    trace_refill_kernel<kMode, kStats, kCamera><<<grid, kTraceBlock, 0, ctx->stream>>>(
-        av.wide, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill);
+    trace_refill_kernel<kMode, kStats, kCamera, kSeg><<<grid, kTraceBlock, 0, ctx->stream>>>(
+        av.wide, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill, seg_off);
   else
-    trace_kernel<kMode, kStats, kCamera, kWide><<<grid, kTraceBlock, 0, ctx->stream>>>(
-        av, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill);
+    trace_kernel<kMode, kStats, kCamera, kWide, kSeg><<<grid, kTraceBlock, 0, ctx->stream>>>(
+        av, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill, seg_off);
   BJ_HIP(hipGetLastError());
   return BAJO_OK;
 }
@@ -309,6 +335,24 @@ int32_t bajo_trace_camera(bajo_ctx* ctx, const bajo_bvh* bvh, const float* camer
     return bj::launch<BAJO_TRACE_CLOSEST_HIT, true, true>(ctx, bvh, nullptr, camera, width, height, hits, ray_count, stats);
   return bj::launch<BAJO_TRACE_CLOSEST_HIT, false, true>(ctx, bvh, nullptr, camera, width, height, hits, ray_count,
                                                          nullptr);
+}
+
+int32_t bajo_blas_set_trace_camera(bajo_ctx* ctx, const bajo_blas_set* set, const float* cameras, float* hits,
+                                   uint32_t width, uint32_t height, uint32_t* stats) {
+  BJ_REQUIRE(ctx != nullptr && set != nullptr, "bajo_blas_set_trace_camera: NULL ctx/set");
+  BJ_REQUIRE(cameras != nullptr && hits != nullptr, "bajo_blas_set_trace_camera: NULL cameras/hits");
+  BJ_REQUIRE(width > 0 && height > 0, "bajo_blas_set_trace_camera: launch dimensions must be positive");
+  const bajo_bvh* h = bj::blas_set_store(set);
+  const uint64_t R = (uint64_t)h->seg_count * width * height;
+  BJ_REQUIRE(R < (1ull << 31), "bajo_blas_set_trace_camera: %u segments x %u x %u rays exceed 2^31", h->seg_count,
+             width, height);
+  constexpr int kC = BAJO_TRACE_CLOSEST_HIT;
+  if (h->node_width == BAJO_WIDTH_WIDE8) {
+    if (stats) return bj::launch_width<kC, true, true, true, true>(ctx, h, nullptr, cameras, width, height, hits, R, stats);
+    return bj::launch_width<kC, false, true, true, true>(ctx, h, nullptr, cameras, width, height, hits, R, nullptr);
+  }
+  if (stats) return bj::launch_width<kC, true, true, false, true>(ctx, h, nullptr, cameras, width, height, hits, R, stats);
+  return bj::launch_width<kC, false, true, false, true>(ctx, h, nullptr, cameras, width, height, hits, R, nullptr);
 }
 
 int32_t bajo_camera_make(const float* o3, const float* t3, const float* u3, float fov_scale, float focus_dist,

@@ -135,7 +135,7 @@ __global__ void seg_collapse_root_kernel(const float* __restrict__ seg_scene, co
   for (uint32_t s = gtid; s < S; s += stride) {
     const uint64_t first = off[s], count = off[s + 1] - first;
     if (count > 1) {
-      wide_src[s] = (uint32_t)(first - s);
+      wide_src[s] = (uint32_t)first;
       continue;
     }
     const float* b = seg_scene + 12 * (uint64_t)s;

@@ -50,6 +50,19 @@ def icosphere(n: int, frequency: int = 458, noise_seed: int = 2028, amplitude: f
     return out
 
 
+def batched_soups(count: int = 4096, seed: int = 2031, lo: int = 256, hi: int = 65536) -> tuple[np.ndarray, np.ndarray]:
+    """`count` independent soups (the batched-environment workload): triangle counts
+    round(exp(U[ln lo, ln hi])) from numpy's PCG64 stream `seed`, soup k drawn by the soup generator from
+    PCG32 stream (seed, k).  Returns the meshes back to back (f32[9 N]) and their offsets u64[count + 1]."""
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    out = np.empty(9 * int(offsets[-1]), dtype=np.float32)
+    for k, n in enumerate(counts):
+        out[9 * int(offsets[k]): 9 * int(offsets[k + 1])] = soup(int(n), initstate=seed, initseq=k)
+    return out, offsets
+
+
 def bounds(vertices: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     v = vertices.reshape(-1, 3)
     return v.min(axis=0), v.max(axis=0)

@@ -356,6 +356,69 @@ def tlas_workload(ctx) -> dict:
     return out
 
 
+def batched_envs_workload(ctx, count: int = 4096, width: int = 64, height: int = 64) -> dict:
+    """The batched-environment shape: `count` independent soups with triangle counts log-uniform in
+    [256, 65536] (47.5 M triangles at 4,096), each with its own LBVH, and 64 x 64 primary rays per
+    environment from the small-scene camera.  One segmented BLAS-set build plus one batched camera launch,
+    against a per-environment loop (one rebuild of a single BVH handle and one camera launch per
+    environment) over the same data.  Closest hit; binary and 8-wide."""
+    import torch
+
+    from bajo_amd import scenes
+    from bajo_amd.bvh import Camera, build_triangle_blas_set_flat, build_triangle_bvh
+
+    dev = ctx.torch_device
+    stream = torch.cuda.current_stream(dev)
+    vh, off = scenes.batched_soups(count)
+    v = torch.from_numpy(vh).to(dev)
+    del vh
+    n = int(off[-1])
+    ppe = width * height
+    R = count * ppe
+    cam = Camera((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 0.41421356).flatten()
+    cams = torch.from_numpy(np.tile(cam, count)).to(dev)
+    one = torch.from_numpy(cam).to(dev)
+    hits = torch.empty(8 * R, dtype=torch.float32, device=dev)
+    out = {"workload": f"{count} soups (triangle counts log-uniform in [256, 65536], {n} triangles), one LBVH "
+           f"each, {width}x{height} primary rays per environment, closest hit", "environments": count,
+           "triangles": n, "rays": R}
+    print(f"[bench] batched envs: {count} scenes, {n} triangles resident", file=sys.stderr, flush=True)
+    for width_nodes in (2, 8):
+        bs = build_triangle_blas_set_flat(v, off, ctx=ctx, node_width=width_nodes, leaf_size=4)
+        build_ms = _timed(lambda: bs.rebuild(v, off), stream)
+        stages = bs.rebuild(v, off, timings=True)
+        trace_ms = _timed(lambda: bs.launch_camera(cams, hits, width, height), stream)
+        hit_frac = float((hits.view(-1, 8)[:, 2].view(torch.int32) != -1).float().mean())
+        bs.free()
+        # the per-environment loop over the same meshes
+        first = v[: 9 * int(off[1])]
+        b = build_triangle_bvh(first, ctx=ctx, node_width=width_nodes, leaf_size=4)
+
+        def loop():
+            for k in range(count):
+                vk = v[9 * int(off[k]): 9 * int(off[k + 1])]
+                b.rebuild(vk)
+                b.launch_camera(one, hits[8 * k * ppe: 8 * (k + 1) * ppe], ppe, width, height)
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loop()
+        torch.cuda.synchronize()
+        loop_ms = (time.perf_counter() - t0) * 1e3
+        b.free()
+        key = "binary" if width_nodes == 2 else "wide8"
+        out[key] = {"set_build_ms": build_ms, "set_build_mtris_s": n / (build_ms / 1e3) / 1e6,
+                    "set_build_stages_ms": {k2: val / 1e6 for k2, val in stages.items()},
+                    "batched_trace_ms": trace_ms, "batched_trace_mrays_s": R / (trace_ms / 1e3) / 1e6,
+                    "hit_fraction": hit_frac, "per_env_loop_ms": loop_ms,
+                    "speedup_vs_loop": loop_ms / (build_ms + trace_ms)}
+        print(f"[bench] batched envs {key}: set build {build_ms:.2f} ms + batched trace {trace_ms:.2f} ms vs "
+              f"per-environment loop {loop_ms:.1f} ms", file=sys.stderr, flush=True)
+    out["note"] = ("the loop's time includes its host launch overhead (2 library calls per environment); the "
+                   "NEE render of each environment is not part of this workload")
+    return out
+
+
 def cpu_baseline(scene, cam, threads: int) -> dict:
     import oracle
 
@@ -385,11 +448,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-workloads", action="store_true",
-                    help="skip the 65k-soup camera, 16.7M soup-grid H-PLOC and TLAS workloads")
+                    help="skip the 65k-soup camera, 16.7M soup-grid H-PLOC, TLAS and batched-environment workloads")
     ap.add_argument("--node-width", type=int, default=8, choices=(2, 8), help="BVH layout: binary or 8-wide")
     ap.add_argument("--leaf-size", type=int, default=4, help="8-wide leaf size (triangles per leaf child)")
     ap.add_argument("--method", choices=("lbvh", "hploc"), default="lbvh", help="binary builder")
     ap.add_argument("--tlas-only", action="store_true", help="run only the TLAS workload and print its JSON")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched-environment workload and print its JSON")
     ap.add_argument(
         "--copy-calibration-mib",
         type=int,
@@ -417,6 +482,9 @@ def main() -> int:
     print(f"[bench] device {ctx.name()}", file=sys.stderr, flush=True)
     if args.tlas_only:
         print(json.dumps({"tlas": tlas_workload(ctx)}), flush=True)
+        return 0
+    if args.batched_only:
+        print(json.dumps({"batched_envs": batched_envs_workload(ctx)}), flush=True)
         return 0
     scene = scenes.icosphere_scene(N_SPHERE)
     method = {"lbvh": 0, "hploc": 1}[args.method]
@@ -539,6 +607,7 @@ def main() -> int:
         line["soup_camera_65k"] = soup_camera_workload(ctx, min(16, os.cpu_count() or 1), not args.no_cpu_baseline)
         line["soup_grid_16m"] = soup_grid_workload(ctx)
         line["tlas"] = tlas_workload(ctx)
+        line["batched_envs"] = batched_envs_workload(ctx)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, cam, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)

@@ -231,7 +231,8 @@ int32_t bajo_camera_make(const float* origin3, const float* target3, const float
  * width 2 or 8.  bajo_blas_set_get returns segment k as a bajo_bvh view
  * (bajo's BlasDesc, blas_desc.mojo) that traces, exports and enters a TLAS
  * like any BVH; it shares the set's storage, cannot be rebuilt or freed on
- * its own, and lives until bajo_blas_set_free. */
+ * its own, and lives until bajo_blas_set_free.  A non-NULL *out is rebuilt
+ * in place (same segment count; its views stay valid and see the new build). */
 typedef struct bajo_blas_set bajo_blas_set;
 int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const uint64_t* segment_offsets,
                                      uint32_t segment_count, const bajo_build_options* options,
@@ -239,6 +240,14 @@ int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const
 int32_t bajo_blas_set_count(const bajo_blas_set* set, uint32_t* count);
 int32_t bajo_blas_set_get(const bajo_blas_set* set, uint32_t index, bajo_bvh** view);
 int32_t bajo_blas_set_free(bajo_blas_set* set);
+/* Batched camera launch over a BLAS set (the batched many-environment trace:
+ * one launch for every segment instead of a per-segment launch loop; bajo
+ * has no counterpart, its closest code is launch_camera per BLAS,
+ * bvh/gpu/triangle_bvh.mojo:520-548): camera f32[20 S], view e traced against
+ * segment e, width x height raster rays per view, hits f32[8 S w h] view-major
+ * (prim = index inside the segment's mesh); `stats` NULL or u32[5 S w h]. */
+int32_t bajo_blas_set_trace_camera(bajo_ctx* ctx, const bajo_blas_set* set, const float* cameras, float* hits,
+                                   uint32_t width, uint32_t height, uint32_t* stats);
 
 /* ---- instances: bajo/bvh/gpu/tlas.mojo -----------------------------------
  * build_triangle_tlas (bvh/gpu/tlas.mojo:618-645): a top-level hierarchy

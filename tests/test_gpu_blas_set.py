@@ -78,6 +78,35 @@ def test_segments_trace_like_the_oracle(gpu_ctx, width):
     bs.free()
 
 
+@pytest.mark.parametrize("width", [2, 8])
+def test_batched_camera_matches_per_mesh_oracle(gpu_ctx, width):
+    from bajo_amd.bvh import Camera, build_triangle_blas_set
+
+    announce(f"blas set batched camera width={width}")
+    ms = meshes(SIZES, 300)
+    bs = build_triangle_blas_set([to_dev(m) for m in ms], node_width=width, leaf_size=4)
+    w, h = 48, 32
+    cams = np.concatenate([Camera((0.5 + 0.05 * k, 0.5, -1.5), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 0.41421356).flatten()
+                           for k in range(len(ms))])
+    R = len(ms) * w * h
+    hits = torch.empty(8 * R, dtype=torch.float32, device="cuda:0")
+    stats = torch.empty(5 * R, dtype=torch.int32, device="cuda:0")
+    bs.launch_camera(to_dev(cams), hits, w, h, stats=stats)
+    got = to_host(hits, np.float32).reshape(len(ms), w * h, 8)
+    st = to_host(stats, np.uint32).reshape(len(ms), w * h, 5)
+    for k, m in enumerate(ms):
+        lb = oracle.build_lbvh(m)
+        rays = oracle.camera_rays(cams[20 * k: 20 * k + 20], w * h, w, h)
+        if width == 8:
+            want, want_st, _ = oracle.trace_wide(m, lb["leaf_ids"], bs.blas(k).wide(), rays, with_stats=True)
+        else:
+            want, want_st = oracle.trace_bvh2(m, lb, rays, stats=True)
+            want, want_st = want.reshape(-1, 8), want_st.reshape(-1, 5)
+        np.testing.assert_array_equal(got[k].view(np.uint32), want.view(np.uint32), err_msg=f"view {k}")
+        np.testing.assert_array_equal(st[k][:, [0, 1, 3, 4]], want_st[:, [0, 1, 3, 4]], err_msg=f"view {k}")
+    bs.free()
+
+
 def test_tlas_over_blas_set(gpu_ctx):
     from bajo_amd import scenes
     from bajo_amd.bvh import Instance, build_triangle_blas_set, build_triangle_tlas

@@ -210,6 +210,14 @@ SIGNATURES = {
         c_int32,
         [c_void_p, c_void_p, POINTER(RenderSettings), POINTER(c_float), c_void_p, POINTER(RenderStats)],
     ),
+    "bajo_render_target_create": (
+        c_int32, [c_void_p, POINTER(RenderSettings), POINTER(c_float), c_void_p, POINTER(c_void_p)],
+    ),
+    "bajo_render_target_set_camera": (c_int32, [c_void_p, c_void_p, POINTER(c_float)]),
+    "bajo_render_enqueue": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "bajo_render_target_capture": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "bajo_render_target_replay": (c_int32, [c_void_p, c_void_p]),
+    "bajo_render_target_free": (c_int32, [c_void_p]),
     "bajo_rng_stage_floats": (c_int32, [c_uint64, c_uint32, c_uint32, c_uint32, c_void_p]),
     "bajo_scene_soup": (c_int32, [c_uint64, c_uint64, c_uint64, c_float, c_float, c_void_p]),
     "bajo_scene_icosphere": (c_int32, [c_uint32, c_uint64, c_uint64, c_float, c_float, c_void_p]),

@@ -105,14 +105,13 @@ BJ_D float power_heuristic(float a, float b) {
 
 // ---- primary (rt/gpu/common_kernels.mojo:22-94, bvh/camera.mojo:156-208) ----
 struct PrimaryArgs {
-  Cam cam;
+  const float* cam;  // 20 floats in device memory (the frame's camera)
   uint32_t width, height, spp;
 };
 
 // The camera path of `path_id`: origin, t_min, direction, t_max, throughput
 // 1 and pdf 0 in the path-plane order of the queues.
 BJ_D void primary_path(const PrimaryArgs& pa, uint32_t path_id, uint64_t seed, float (&f)[kPathPlanes]) {
-  const Cam& cam = pa.cam;
   const uint32_t width = pa.width, height = pa.height, spp = pa.spp;
   const uint32_t pixel = path_id / spp;
   const uint32_t px = pixel % width, py = pixel / width;
@@ -123,7 +122,7 @@ BJ_D void primary_path(const PrimaryArgs& pa, uint32_t path_id, uint64_t seed, f
   const float rr = sqrtf(rng.next());
   const float lens_u = rr * c, lens_v = rr * s;
   const float pu = rng.next(), pv = rng.next();
-  const float* k = cam.v;
+  const float* __restrict__ k = pa.cam;
   const float fw = (float)width, fh = (float)height;
   const float aspect = fw / fh;
   const float sx = (((float)px + pu) / fw) * 2.0f - 1.0f;
@@ -805,6 +804,24 @@ __global__ void __launch_bounds__(kRtBlock) resolve_kernel(const float* __restri
 }  // namespace
 
 // ---- scene + arena ---------------------------------------------------------
+// Path queues, sample planes, counters and the frame's camera of one render
+// (a scene keeps one for bajo_render; a render target owns its own).
+struct RenderArena {
+  uint64_t cap = 0;
+  uint32_t* ids[2] = {nullptr, nullptr};
+  float* fields[2] = {nullptr, nullptr};
+  uint32_t* shadow_ids = nullptr;
+  float* shadow_fields = nullptr;
+  float* sample = nullptr;
+  uint32_t* counters = nullptr;
+  unsigned long long* stats = nullptr;
+  uint32_t* spill = nullptr;
+  uint64_t spill_threads = 0;
+  unsigned long long* work_ext = nullptr;  // per-block traversal work partials
+  unsigned long long* work_sh = nullptr;
+  float* cam = nullptr;                    // 20 floats
+};
+
 struct bajo_scene {
   int device = 0;
   uint64_t n = 0;
@@ -820,41 +837,22 @@ struct bajo_scene {
   uint32_t* light_kinds = nullptr;
   uint32_t light_count = 0;
   float total_light_weight = 0.0f;
-  // reusable render arena
-  uint64_t cap = 0;
-  uint32_t* ids[2] = {nullptr, nullptr};
-  float* fields[2] = {nullptr, nullptr};
-  uint32_t* shadow_ids = nullptr;
-  float* shadow_fields = nullptr;
-  float* sample = nullptr;
-  uint32_t* counters = nullptr;
-  unsigned long long* stats = nullptr;
-  uint32_t* spill = nullptr;
-  uint64_t spill_threads = 0;
-  unsigned long long* work_ext = nullptr;  // per-block traversal work partials
-  unsigned long long* work_sh = nullptr;
+  RenderArena arena;  // reused by bajo_render
 };
 
 namespace {
 
-void free_arena(bajo_scene* s) {
-  (void)hipFree(s->ids[0]); (void)hipFree(s->ids[1]); (void)hipFree(s->fields[0]); (void)hipFree(s->fields[1]);
-  (void)hipFree(s->shadow_ids); (void)hipFree(s->shadow_fields); (void)hipFree(s->sample); (void)hipFree(s->counters); (void)hipFree(s->stats);
-  (void)hipFree(s->spill);
-  (void)hipFree(s->work_ext);
-  (void)hipFree(s->work_sh);
-  s->work_ext = s->work_sh = nullptr;
-  s->ids[0] = s->ids[1] = nullptr;
-  s->fields[0] = s->fields[1] = nullptr;
-  s->shadow_ids = nullptr; s->shadow_fields = nullptr; s->sample = nullptr; s->counters = nullptr;
-  s->stats = nullptr; s->spill = nullptr;
-  s->cap = 0;
-  s->spill_threads = 0;
+void free_arena(RenderArena& a) {
+  (void)hipFree(a.ids[0]); (void)hipFree(a.ids[1]); (void)hipFree(a.fields[0]); (void)hipFree(a.fields[1]);
+  (void)hipFree(a.shadow_ids); (void)hipFree(a.shadow_fields); (void)hipFree(a.sample); (void)hipFree(a.counters);
+  (void)hipFree(a.stats); (void)hipFree(a.spill); (void)hipFree(a.work_ext); (void)hipFree(a.work_sh);
+  (void)hipFree(a.cam);
+  a = RenderArena();
 }
 
 void free_scene(bajo_scene* s) {
   if (!s) return;
-  free_arena(s);
+  free_arena(s->arena);
   bajo_bvh_free(s->bvh);
   (void)hipFree(s->vertices); (void)hipFree(s->surfaces); (void)hipFree(s->lambertian); (void)hipFree(s->emissive);
   (void)hipFree(s->metals); (void)hipFree(s->dielectrics);
@@ -872,25 +870,38 @@ int32_t grid_of(K kernel, unsigned* grid) {
   return BAJO_OK;
 }
 
-int32_t ensure_arena(bajo_scene* s, uint64_t cap, uint64_t spill_threads) {
-  if (cap <= s->cap && spill_threads <= s->spill_threads) return BAJO_OK;
-  free_arena(s);
+int32_t ensure_arena(RenderArena& a, uint64_t cap, uint64_t spill_threads) {
+  if (cap <= a.cap && spill_threads <= a.spill_threads) return BAJO_OK;
+  // keep the frame's camera across a regrowth
+  float* cam = a.cam;
+  a.cam = nullptr;
+  free_arena(a);
+  a.cam = cam;
+  if (!a.cam) BJ_HIP(hipMalloc(&a.cam, 20 * sizeof(float)));
   const uint64_t tiles = (cap + 63) / 64 * 64;  // plane storage is whole 64-entry tiles
   for (int k = 0; k < 2; ++k) {
-    BJ_HIP(hipMalloc(&s->ids[k], cap * sizeof(uint32_t)));
-    BJ_HIP(hipMalloc(&s->fields[k], tiles * kPathPlanes * sizeof(float)));
+    BJ_HIP(hipMalloc(&a.ids[k], cap * sizeof(uint32_t)));
+    BJ_HIP(hipMalloc(&a.fields[k], tiles * kPathPlanes * sizeof(float)));
   }
-  BJ_HIP(hipMalloc(&s->shadow_ids, cap * sizeof(uint32_t)));
-  BJ_HIP(hipMalloc(&s->shadow_fields, tiles * kShadowPlanes * sizeof(float)));
-  BJ_HIP(hipMalloc(&s->sample, tiles * kSamplePlanes * sizeof(float)));
-  BJ_HIP(hipMalloc(&s->counters, kCounterWords * sizeof(uint32_t)));
-  BJ_HIP(hipMalloc(&s->stats, kStatsWords * sizeof(unsigned long long)));
-  BJ_HIP(hipMalloc(&s->spill, spill_threads * kSpill * sizeof(uint32_t)));
-  BJ_HIP(hipMalloc(&s->work_ext, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
-  BJ_HIP(hipMalloc(&s->work_sh, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
-  s->cap = cap;
-  s->spill_threads = spill_threads;
+  BJ_HIP(hipMalloc(&a.shadow_ids, cap * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&a.shadow_fields, tiles * kShadowPlanes * sizeof(float)));
+  BJ_HIP(hipMalloc(&a.sample, tiles * kSamplePlanes * sizeof(float)));
+  BJ_HIP(hipMalloc(&a.counters, kCounterWords * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&a.stats, kStatsWords * sizeof(unsigned long long)));
+  BJ_HIP(hipMalloc(&a.spill, spill_threads * kSpill * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&a.work_ext, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
+  BJ_HIP(hipMalloc(&a.work_sh, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
+  a.cap = cap;
+  a.spill_threads = spill_threads;
   return BAJO_OK;
+}
+
+// The frame's camera, written by a kernel on the stream (so a captured graph
+// replays with whatever camera the buffer holds).
+__global__ void set_camera_kernel(float* __restrict__ cam, Cam c) {
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int k = 0; k < 20; ++k) cam[k] = c.v[k];
 }
 
 // LightStore.build_alias_table (rt/types.mojo:207-247): Walker-Vose with
@@ -938,19 +949,25 @@ struct LaunchEvents {
   }
 };
 
+// Arena shape of a render: path capacity (whole pixels) and the threads of
+// the widest launch (their spill columns and work partials).
+struct RenderShape {
+  uint64_t cap = 0;
+  unsigned g_ext = 0, g_tail = 0, simds = 0;
+  uint64_t spill_threads = 0;
+};
+
 template <int kAlg, bool kWide>
-int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st, const float* camera, float* pixels,
-                   LaunchEvents* ev) {
+int32_t render_shape(const bajo_render_settings* st, RenderShape* out) {
   const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
   uint64_t cap = st->path_capacity ? st->path_capacity : total;
   if (cap > total) cap = total;
   cap = (cap / st->samples_per_pixel) * st->samples_per_pixel;
   if (cap == 0) cap = st->samples_per_pixel;
-  int32_t rc = BAJO_OK;
   constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
   unsigned g_tail = 0, simds = 0;
   if constexpr (kTail) {
-    rc = grid_of(tail_kernel<kAlg, kWide>, &g_tail);
+    int32_t rc = grid_of(tail_kernel<kAlg, kWide>, &g_tail);
     if (rc) return rc;
     int dev = 0, cus = 0;
     BJ_HIP(hipGetDevice(&dev));
@@ -958,22 +975,48 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
     simds = 4u * (unsigned)cus;
   }
   // extend and shadow launch one thread per arena entry
-  const unsigned g_ext = div_up(cap, kRtBlock), g_sh = g_ext;
+  const unsigned g_ext = div_up(cap, kRtBlock);
   if (g_tail < 1) g_tail = 1;
+  const unsigned g_max = g_ext > g_tail ? g_ext : g_tail;
+  out->cap = cap;
+  out->g_ext = g_ext;
+  out->g_tail = g_tail;
+  out->simds = simds;
+  out->spill_threads = (uint64_t)g_max * kRtBlock;
+  return BAJO_OK;
+}
+
+// Enqueues one render into arena `a` (no host synchronization without `ev`).
+// `camera` (host, 20 floats) is written into the arena's camera buffer first;
+// NULL renders with the camera the buffer already holds.
+template <int kAlg, bool kWide>
+int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st, const float* camera,
+                   float* pixels, LaunchEvents* ev) {
+  const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
+  RenderShape shape;
+  int32_t rc = render_shape<kAlg, kWide>(st, &shape);
+  if (rc) return rc;
+  constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
+  const uint64_t cap = shape.cap;
+  const unsigned g_ext = shape.g_ext, g_sh = g_ext, g_tail = shape.g_tail, simds = shape.simds;
   // the tail takes over when the remaining paths fit one pass of its resident threads
   const uint32_t tail_threshold = g_tail * kRtBlock;
-  const unsigned g_max = g_ext > g_tail ? g_ext : g_tail;
-  const uint64_t spill_threads = (uint64_t)g_max * kRtBlock;
-  rc = ensure_arena(s, cap, spill_threads);
+  rc = ensure_arena(a, cap, shape.spill_threads);
   if (rc) return rc;
   hipStream_t stream = ctx->stream;
+  if (camera) {
+    Cam c;
+    memcpy(c.v, camera, sizeof(c.v));
+    set_camera_kernel<<<1, 64, 0, stream>>>(a.cam, c);
+    BJ_HIP(hipGetLastError());
+  }
   PrimaryArgs pa;
-  memcpy(pa.cam.v, camera, sizeof(pa.cam.v));
+  pa.cam = a.cam;
   pa.width = st->image_width;
   pa.height = st->image_height;
   pa.spp = st->samples_per_pixel;
   SceneView sv;
-  sv.bvh = accel_view(s->bvh, s->counters + 8);
+  sv.bvh = accel_view(s->bvh, a.counters + 8);
   sv.surfaces = s->surfaces;
   sv.lambertian = s->lambertian;
   sv.metals = s->metals;
@@ -986,43 +1029,84 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st,
   const uint32_t depth = (kAlg == BAJO_RENDER_NORMALS || kAlg == BAJO_RENDER_AO) ? 1u : st->max_depth;
   for (uint64_t begin = 0; begin < total; begin += cap) {
     const uint32_t chunk = (uint32_t)((total - begin) < cap ? (total - begin) : cap);
-    init_counters_kernel<<<1, 64, 0, stream>>>(s->counters, chunk, s->stats, begin == 0);
+    init_counters_kernel<<<1, 64, 0, stream>>>(a.counters, chunk, a.stats, begin == 0);
     BJ_HIP(hipGetLastError());
     for (uint32_t b = 0; b < depth; ++b) {
-      Queues q{s->ids[b & 1], s->fields[b & 1], s->ids[(b + 1) & 1], s->fields[(b + 1) & 1], s->shadow_ids,
-               s->shadow_fields, s->sample, s->counters, cap, (uint32_t)begin};
+      Queues q{a.ids[b & 1], a.fields[b & 1], a.ids[(b + 1) & 1], a.fields[(b + 1) & 1], a.shadow_ids,
+               a.shadow_fields, a.sample, a.counters, cap, (uint32_t)begin};
       if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if constexpr (kTail) {
         if (b > 0) {
           tail_kernel<kAlg, kWide><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, tail_threshold,
-                                                                    simds, s->stats, s->spill);
+                                                                    simds, a.stats, a.spill);
           BJ_HIP(hipGetLastError());
         }
       }
       if (b == 0)
         extend_kernel<kAlg, kWide, true><<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(
-            sv, q, st->rng_seed, b, pa, s->spill, s->work_ext, s->stats);
+            sv, q, st->rng_seed, b, pa, a.spill, a.work_ext, a.stats);
       else
-        extend_kernel<kAlg, kWide, false><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, s->spill,
-                                                                          s->work_ext, s->stats);
+        extend_kernel<kAlg, kWide, false><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, a.spill,
+                                                                          a.work_ext, a.stats);
       BJ_HIP(hipGetLastError());
       if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
-        shadow_kernel<kAlg, kWide><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, s->spill, s->work_sh, s->stats);
+        shadow_kernel<kAlg, kWide><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats);
         BJ_HIP(hipGetLastError());
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
       }
       const bool traced_shadows = kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS;
-      advance_kernel<<<1, 256, 0, stream>>>(s->counters, s->stats, b, (uint32_t)cap, s->work_ext, g_ext, s->work_sh,
+      advance_kernel<<<1, 256, 0, stream>>>(a.counters, a.stats, b, (uint32_t)cap, a.work_ext, g_ext, a.work_sh,
                                             traced_shadows ? g_sh : 0u);
     }
     const uint32_t pixel_count = chunk / st->samples_per_pixel;
     resolve_kernel<<<div_up(pixel_count, kRtBlock), kRtBlock, 0, stream>>>(
-        s->sample, pixels, cap, (uint32_t)(begin / st->samples_per_pixel), pixel_count, st->samples_per_pixel);
+        a.sample, pixels, cap, (uint32_t)(begin / st->samples_per_pixel), pixel_count, st->samples_per_pixel);
     BJ_HIP(hipGetLastError());
   }
   return BAJO_OK;
+}
+
+// Sizes arena `a` for a render of `st` over scene `s` without enqueueing it.
+int32_t prepare_arena(const bajo_scene* s, RenderArena& a, const bajo_render_settings* st) {
+  const bool wide = s->bvh->node_width == BAJO_WIDTH_WIDE8;
+  RenderShape shape;
+  int32_t rc;
+  switch (st->algorithm) {
+    case BAJO_RENDER_PATH: rc = wide ? render_shape<BAJO_RENDER_PATH, true>(st, &shape) : render_shape<BAJO_RENDER_PATH, false>(st, &shape); break;
+    case BAJO_RENDER_NORMALS: rc = wide ? render_shape<BAJO_RENDER_NORMALS, true>(st, &shape) : render_shape<BAJO_RENDER_NORMALS, false>(st, &shape); break;
+    case BAJO_RENDER_AO: rc = wide ? render_shape<BAJO_RENDER_AO, true>(st, &shape) : render_shape<BAJO_RENDER_AO, false>(st, &shape); break;
+    case BAJO_RENDER_NEE: rc = wide ? render_shape<BAJO_RENDER_NEE, true>(st, &shape) : render_shape<BAJO_RENDER_NEE, false>(st, &shape); break;
+    case BAJO_RENDER_MIS: rc = wide ? render_shape<BAJO_RENDER_MIS, true>(st, &shape) : render_shape<BAJO_RENDER_MIS, false>(st, &shape); break;
+    default: BJ_REQUIRE(false, "bajo_render: unknown algorithm %d", st->algorithm);
+  }
+  if (rc) return rc;
+  return ensure_arena(a, shape.cap, shape.spill_threads);
+}
+
+int32_t dispatch_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st,
+                        const float* camera, float* pixels, LaunchEvents* ev) {
+  const bool wide = s->bvh->node_width == BAJO_WIDTH_WIDE8;
+  switch (st->algorithm) {
+    case BAJO_RENDER_PATH:
+      return wide ? run_render<BAJO_RENDER_PATH, true>(ctx, s, a, st, camera, pixels, ev)
+                  : run_render<BAJO_RENDER_PATH, false>(ctx, s, a, st, camera, pixels, ev);
+    case BAJO_RENDER_NORMALS:
+      return wide ? run_render<BAJO_RENDER_NORMALS, true>(ctx, s, a, st, camera, pixels, ev)
+                  : run_render<BAJO_RENDER_NORMALS, false>(ctx, s, a, st, camera, pixels, ev);
+    case BAJO_RENDER_AO:
+      return wide ? run_render<BAJO_RENDER_AO, true>(ctx, s, a, st, camera, pixels, ev)
+                  : run_render<BAJO_RENDER_AO, false>(ctx, s, a, st, camera, pixels, ev);
+    case BAJO_RENDER_NEE:
+      return wide ? run_render<BAJO_RENDER_NEE, true>(ctx, s, a, st, camera, pixels, ev)
+                  : run_render<BAJO_RENDER_NEE, false>(ctx, s, a, st, camera, pixels, ev);
+    case BAJO_RENDER_MIS:
+      return wide ? run_render<BAJO_RENDER_MIS, true>(ctx, s, a, st, camera, pixels, ev)
+                  : run_render<BAJO_RENDER_MIS, false>(ctx, s, a, st, camera, pixels, ev);
+    default:
+      BJ_REQUIRE(false, "bajo_render: unknown algorithm %d", st->algorithm);
+  }
 }
 
 }  // namespace
@@ -1186,23 +1270,15 @@ int32_t bajo_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st
     ev->end = ev->make();
     BJ_HIP(hipEventRecord(ev->begin, ctx->stream));
   }
-  const bool wide = s->bvh->node_width == BAJO_WIDTH_WIDE8;
-  switch (st->algorithm) {
-    case BAJO_RENDER_PATH: rc = wide ? run_render<BAJO_RENDER_PATH, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_PATH, false>(ctx, s, st, camera, pixels, ev); break;
-    case BAJO_RENDER_NORMALS: rc = wide ? run_render<BAJO_RENDER_NORMALS, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_NORMALS, false>(ctx, s, st, camera, pixels, ev); break;
-    case BAJO_RENDER_AO: rc = wide ? run_render<BAJO_RENDER_AO, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_AO, false>(ctx, s, st, camera, pixels, ev); break;
-    case BAJO_RENDER_NEE: rc = wide ? run_render<BAJO_RENDER_NEE, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_NEE, false>(ctx, s, st, camera, pixels, ev); break;
-    case BAJO_RENDER_MIS: rc = wide ? run_render<BAJO_RENDER_MIS, true>(ctx, s, st, camera, pixels, ev) : run_render<BAJO_RENDER_MIS, false>(ctx, s, st, camera, pixels, ev); break;
-    default: BJ_REQUIRE(false, "bajo_render: unknown algorithm %d", st->algorithm);
-  }
+  rc = dispatch_render(ctx, s, s->arena, st, camera, pixels, ev);
   if (rc) return rc;
   if (stats) {
     BJ_HIP(hipEventRecord(ev->end, ctx->stream));
     memset(stats, 0, sizeof(*stats));
     unsigned long long host[kStatsWords];
     uint32_t counters[kCounterWords];
-    BJ_HIP(hipMemcpyAsync(host, s->stats, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
-    BJ_HIP(hipMemcpyAsync(counters, s->counters, sizeof(counters), hipMemcpyDeviceToHost, ctx->stream));
+    BJ_HIP(hipMemcpyAsync(host, s->arena.stats, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+    BJ_HIP(hipMemcpyAsync(counters, s->arena.counters, sizeof(counters), hipMemcpyDeviceToHost, ctx->stream));
     BJ_HIP(hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     BJ_HIP(hipEventElapsedTime(&ms, ev->begin, ev->end));
@@ -1239,6 +1315,129 @@ int32_t bajo_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st
     if (stats->status) return (int32_t)(BAJO_STATUS_WAVE_BASE + stats->status);
     if (stats->traversal_status) return (int32_t)stats->traversal_status;
   }
+  return BAJO_OK;
+}
+
+// ---- persistent render target (rt/gpu/resources.mojo:38-155) ----------------
+
+struct bajo_render_target {
+  int device = 0;
+  bajo_render_settings settings{};
+  float* pixels = nullptr;  // the caller's f32[3 w h] device buffer
+  RenderArena arena;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+void drop_graph(bajo_render_target* t) {
+  if (t->exec) (void)hipGraphExecDestroy(t->exec);
+  if (t->graph) (void)hipGraphDestroy(t->graph);
+  t->exec = nullptr;
+  t->graph = nullptr;
+}
+
+int32_t check_settings(const bajo_render_settings* st, const char* fn) {
+  BJ_REQUIRE(st->image_width > 0 && st->image_height > 0 && st->samples_per_pixel > 0,
+             "%s: image size and samples per pixel must be positive", fn);
+  BJ_REQUIRE(st->max_depth <= kMaxBounces, "%s: max_depth above %d", fn, kMaxBounces);
+  BJ_REQUIRE(st->algorithm >= BAJO_RENDER_PATH && st->algorithm <= BAJO_RENDER_MIS, "%s: unknown algorithm %d", fn,
+             st->algorithm);
+  const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
+  BJ_REQUIRE(total < (1ull << 31), "%s: %llu paths exceed the 31-bit path id", fn, (unsigned long long)total);
+  return BAJO_OK;
+}
+}  // namespace
+
+int32_t bajo_render_target_create(bajo_ctx* ctx, const bajo_render_settings* settings, const float* camera,
+                                  float* pixels, bajo_render_target** out) {
+  BJ_REQUIRE(ctx && settings && pixels && out, "bajo_render_target_create: NULL argument");
+  int32_t rc = check_settings(settings, "bajo_render_target_create");
+  if (rc) return rc;
+  bajo_render_target* t = new bajo_render_target();
+  t->device = ctx->device;
+  t->settings = *settings;
+  t->pixels = pixels;
+  hipError_t e = hipMalloc(&t->arena.cam, 20 * sizeof(float));
+  if (e != hipSuccess) {
+    delete t;
+    return hip_fail(e, "hipMalloc(camera)", __FILE__, __LINE__);
+  }
+  if (camera) {
+    Cam c;
+    memcpy(c.v, camera, sizeof(c.v));
+    set_camera_kernel<<<1, 64, 0, ctx->stream>>>(t->arena.cam, c);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+      free_arena(t->arena);
+      delete t;
+      return hip_fail(e, "set_camera_kernel", __FILE__, __LINE__);
+    }
+  }
+  *out = t;
+  return BAJO_OK;
+}
+
+int32_t bajo_render_target_set_camera(bajo_ctx* ctx, bajo_render_target* t, const float* camera) {
+  BJ_REQUIRE(ctx && t && camera, "bajo_render_target_set_camera: NULL argument");
+  Cam c;
+  memcpy(c.v, camera, sizeof(c.v));
+  set_camera_kernel<<<1, 64, 0, ctx->stream>>>(t->arena.cam, c);
+  BJ_HIP(hipGetLastError());
+  return BAJO_OK;
+}
+
+int32_t bajo_render_enqueue(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s) {
+  BJ_REQUIRE(ctx && t && s, "bajo_render_enqueue: NULL argument");
+  BJ_REQUIRE(t->device == ctx->device && s->device == ctx->device, "bajo_render_enqueue: device mismatch");
+  return dispatch_render(ctx, s, t->arena, &t->settings, nullptr, t->pixels, nullptr);
+}
+
+int32_t bajo_render_target_capture(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s) {
+  BJ_REQUIRE(ctx && t && s, "bajo_render_target_capture: NULL argument");
+  BJ_REQUIRE(t->device == ctx->device && s->device == ctx->device, "bajo_render_target_capture: device mismatch");
+  drop_graph(t);
+  // every allocation happens before the capture
+  int32_t rc = prepare_arena(s, t->arena, &t->settings);
+  if (rc) return rc;
+  hipStream_t cs = nullptr;
+  BJ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  bajo_ctx cap = *ctx;
+  cap.stream = cs;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
+  }
+  rc = dispatch_render(&cap, s, t->arena, &t->settings, nullptr, t->pixels, nullptr);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  if (rc || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  }
+  t->graph = g;
+  e = hipGraphInstantiate(&t->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_graph(t);
+    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  return BAJO_OK;
+}
+
+int32_t bajo_render_target_replay(bajo_ctx* ctx, bajo_render_target* t) {
+  BJ_REQUIRE(ctx && t, "bajo_render_target_replay: NULL argument");
+  BJ_REQUIRE(t->exec != nullptr, "bajo_render_target_replay: nothing captured (bajo_render_target_capture)");
+  BJ_HIP(hipGraphLaunch(t->exec, ctx->stream));
+  return BAJO_OK;
+}
+
+int32_t bajo_render_target_free(bajo_render_target* t) {
+  if (!t) return BAJO_OK;
+  drop_graph(t);
+  free_arena(t->arena);
+  delete t;
   return BAJO_OK;
 }
 

@@ -196,6 +196,71 @@ def enqueue_render_gpu(
     return RenderResult(pixels, out)
 
 
+def _cam20(camera) -> ctypes.Array:
+    return (ctypes.c_float * 20)(*[float(x) for x in np.asarray(camera, np.float32).reshape(-1)[:20]])
+
+
+class GpuRtRenderTarget:
+    """GpuRtRenderTarget (rt/gpu/resources.mojo:38-100): path queues, the frame's camera and the pixel output
+    kept on the device across frames, for enqueue-only submission (`enqueue`) and HIP-graph replay
+    (`capture` once, then `replay` per frame; `update_camera` between replays takes effect)."""
+
+    def __init__(self, settings: RenderSettings, camera, algorithm: int = RENDER.PATH, ctx: DeviceContext | None = None,
+                 pixels: torch.Tensor | None = None):
+        self.ctx = ctx or default_context(0)
+        w, h = settings.image_width, settings.image_height
+        self.pixels = pixels if pixels is not None else torch.empty((h, w, 3), dtype=torch.float32,
+                                                                   device=self.ctx.torch_device)
+        require_device(self.pixels, self.ctx, "pixels")
+        if self.pixels.numel() < 3 * w * h:
+            raise ValueError("pixel buffer too small")
+        self.settings = settings
+        self.algorithm = algorithm
+        st = settings.as_c(algorithm)
+        self._handle = ctypes.c_void_p()
+        _lib.check(_lib.lib().bajo_render_target_create(self.ctx.handle, ctypes.byref(st), _cam20(camera),
+                                                        ptr(self.pixels), ctypes.byref(self._handle)))
+
+    @property
+    def handle(self):
+        if not self._handle:
+            raise RuntimeError("render target already freed")
+        return self._handle
+
+    def update_camera(self, camera) -> None:
+        """update_gpu_camera (rt/gpu/resources.mojo:103-115), ordered on the stream instead of synchronizing."""
+        _lib.check(_lib.lib().bajo_render_target_set_camera(self.ctx.handle, self.handle, _cam20(camera)))
+
+    def enqueue(self, scene: "GpuScene") -> torch.Tensor:
+        """Enqueues one render of `scene` into `pixels` and returns without synchronizing."""
+        _lib.check(_lib.lib().bajo_render_enqueue(self.ctx.handle, self.handle, scene.handle))
+        return self.pixels
+
+    def capture(self, scene: "GpuScene") -> None:
+        _lib.check(_lib.lib().bajo_render_target_capture(self.ctx.handle, self.handle, scene.handle))
+        self._scene = scene  # the graph holds the scene's device pointers
+
+    def replay(self) -> torch.Tensor:
+        _lib.check(_lib.lib().bajo_render_target_replay(self.ctx.handle, self.handle))
+        return self.pixels
+
+    def free(self) -> None:
+        if self._handle:
+            _lib.lib().bajo_render_target_free(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def update_gpu_camera(target: GpuRtRenderTarget, camera) -> None:
+    """update_gpu_camera (rt/gpu/resources.mojo:103-115)."""
+    target.update_camera(camera)
+
+
 def render_gpu(settings: RenderSettings, camera: np.ndarray, world, algorithm: int = RENDER.PATH, ctx=None) -> RenderResult:
     """render_gpu[ALGORITHM](settings, camera, world) (rt/gpu/render.mojo:322-420)."""
     scene = world if isinstance(world, GpuScene) else GpuScene(world, ctx)

@@ -379,6 +379,28 @@ int32_t bajo_scene_light_table(const bajo_scene* scene, float* fields, uint32_t*
  * rt/gpu/resources.mojo:268-285). */
 int32_t bajo_render(bajo_ctx* ctx, bajo_scene* scene, const bajo_render_settings* settings, const float* camera,
                     float* pixels, bajo_render_stats* stats);
+/* Persistent render target (GpuRtRenderTarget, update_gpu_camera and the
+ * enqueue-only submission of rt/gpu/resources.mojo:38-155 and
+ * rt/gpu/render.mojo:61-235): the target keeps its path queues, sample
+ * planes and the frame's camera in device memory across frames and renders
+ * into the caller's pixel buffer f32[3 w h] (which must outlive it).
+ * `settings` are fixed at creation.  bajo_render_target_set_camera writes the
+ * camera on the stream (a kernel, so it orders with renders and replays).
+ * bajo_render_enqueue enqueues one render of `scene` and returns without
+ * synchronizing.  bajo_render_target_capture records that enqueue into a
+ * HIP graph (allocations happen first; the graph holds kernel nodes only) and
+ * bajo_render_target_replay launches it on the context stream: a replay
+ * reads the camera buffer when it runs, so camera updates between replays
+ * take effect; a new scene, scene rebuild to a larger size or new settings
+ * need a new capture.  Targets render concurrently with each other. */
+typedef struct bajo_render_target bajo_render_target;
+int32_t bajo_render_target_create(bajo_ctx* ctx, const bajo_render_settings* settings, const float* camera,
+                                  float* pixels, bajo_render_target** out);
+int32_t bajo_render_target_set_camera(bajo_ctx* ctx, bajo_render_target* target, const float* camera);
+int32_t bajo_render_enqueue(bajo_ctx* ctx, bajo_render_target* target, bajo_scene* scene);
+int32_t bajo_render_target_capture(bajo_ctx* ctx, bajo_render_target* target, bajo_scene* scene);
+int32_t bajo_render_target_replay(bajo_ctx* ctx, bajo_render_target* target);
+int32_t bajo_render_target_free(bajo_render_target* target);
 /* Host view of one stage's random stream (tests and tools): `count` floats
  * of the Philox stream keyed by (seed, path_id, stage). */
 int32_t bajo_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float* out);

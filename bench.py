@@ -419,6 +419,49 @@ def batched_envs_workload(ctx, count: int = 4096, width: int = 64, height: int =
     return out
 
 
+def render_target_workload(ctx) -> dict:
+    """Frames of the box scene through a persistent render target (NEE, depth 5, 8-wide): the plain per-frame
+    call, enqueue-only submission to the target, and a captured HIP graph replayed per frame, each over 20
+    frames with one synchronization at the end; at 320 x 240 x 4 spp (GPU-bound) and 64 x 48 x 1 spp
+    (launch-bound)."""
+    import torch
+
+    from bajo_amd import scenes
+    from bajo_amd.bvh import Camera
+    from bajo_amd.rt import RENDER, GpuRtRenderTarget, GpuScene, RenderSettings, enqueue_render_gpu
+
+    gscene = GpuScene(scenes.box_scene(), ctx, node_width=8)
+    cam = Camera((0.0, 1.0, -3.0), (0.0, 0.9, 0.0), (0.0, 1.0, 0.0), 0.5).flatten()
+    frames = 20
+    out = {"workload": "box scene, NEE, depth 5, 8-wide LBVH, 20 frames per mode"}
+
+    def wall(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / frames
+
+    for w, h, spp in ((320, 240, 4), (64, 48, 1)):
+        settings = RenderSettings(w, h, spp, 7, 5)
+        target = GpuRtRenderTarget(settings, cam, RENDER.NEE, ctx)
+        pixels = torch.empty((h, w, 3), dtype=torch.float32, device=ctx.torch_device)
+        plain = wall(lambda: enqueue_render_gpu(settings, cam, gscene, RENDER.NEE, pixels=pixels))
+        enq = wall(lambda: target.enqueue(gscene))
+        target.capture(gscene)
+        graph = wall(target.replay)
+        same = bool(torch.equal(target.pixels, pixels))
+        target.free()
+        print(f"[bench] render target {w}x{h}x{spp}: per-frame call {plain:.3f} ms, enqueue {enq:.3f} ms, graph "
+              f"replay {graph:.3f} ms", file=sys.stderr, flush=True)
+        out[f"{w}x{h}x{spp}"] = {"plain_ms_per_frame": plain, "enqueue_ms_per_frame": enq,
+                                 "graph_replay_ms_per_frame": graph, "replay_equals_plain": same}
+    gscene.free()
+    return out
+
+
 def cpu_baseline(scene, cam, threads: int) -> dict:
     import oracle
 
@@ -448,13 +491,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-workloads", action="store_true",
-                    help="skip the 65k-soup camera, 16.7M soup-grid H-PLOC, TLAS and batched-environment workloads")
+                    help="skip the 65k-soup camera, 16.7M soup-grid H-PLOC, TLAS, batched-environment and render-target "
+                    "workloads")
     ap.add_argument("--node-width", type=int, default=8, choices=(2, 8), help="BVH layout: binary or 8-wide")
     ap.add_argument("--leaf-size", type=int, default=4, help="8-wide leaf size (triangles per leaf child)")
     ap.add_argument("--method", choices=("lbvh", "hploc"), default="lbvh", help="binary builder")
     ap.add_argument("--tlas-only", action="store_true", help="run only the TLAS workload and print its JSON")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-environment workload and print its JSON")
+    ap.add_argument("--target-only", action="store_true", help="run only the render-target workload")
     ap.add_argument(
         "--copy-calibration-mib",
         type=int,
@@ -485,6 +530,9 @@ def main() -> int:
         return 0
     if args.batched_only:
         print(json.dumps({"batched_envs": batched_envs_workload(ctx)}), flush=True)
+        return 0
+    if args.target_only:
+        print(json.dumps({"render_target": render_target_workload(ctx)}), flush=True)
         return 0
     scene = scenes.icosphere_scene(N_SPHERE)
     method = {"lbvh": 0, "hploc": 1}[args.method]
@@ -608,6 +656,7 @@ def main() -> int:
         line["soup_grid_16m"] = soup_grid_workload(ctx)
         line["tlas"] = tlas_workload(ctx)
         line["batched_envs"] = batched_envs_workload(ctx)
+        line["render_target"] = render_target_workload(ctx)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, cam, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)

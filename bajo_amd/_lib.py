@@ -154,6 +154,10 @@ SIGNATURES = {
         c_int32,
         [c_void_p, c_void_p, c_uint64, POINTER(BuildOptions), POINTER(BuildTimings), POINTER(c_void_p)],
     ),
+    "bajo_build_sphere_bvh": (
+        c_int32,
+        [c_void_p, c_void_p, c_uint64, POINTER(BuildOptions), POINTER(BuildTimings), POINTER(c_void_p)],
+    ),
     "bajo_bvh_free": (c_int32, [c_void_p]),
     "bajo_bvh_get_info": (c_int32, [c_void_p, POINTER(BvhInfo)]),
     "bajo_bvh_copy_binary": (

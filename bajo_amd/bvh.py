@@ -207,6 +207,47 @@ class GpuTriangleBvh:
             pass
 
 
+class GpuSphereBvh(GpuTriangleBvh):
+    """GpuSphereBvh (bvh/gpu/sphere_bvh.mojo:274-318): the box builder over Sphere.bounds with each sorted
+    leaf's sphere packed for traversal.  Launches trace bajo's ray-sphere test (hit u = v = 0,
+    normal = normalize(p - center)); `binary()` / `wide()` export the hierarchy like GpuBinaryBoundsBvh."""
+
+    def rebuild(self, spheres: torch.Tensor, timings: bool = False):
+        n = _check_spheres(spheres, self.ctx)
+        t = _lib.BuildTimings() if timings else None
+        handle = ctypes.c_void_p(self.handle.value)
+        opts = build_options(self.node_width, self.leaf_size, self.method)
+        _lib.check(_lib.lib().bajo_build_sphere_bvh(self.ctx.handle, ptr(spheres), n, ctypes.byref(opts),
+                                                    ctypes.byref(t) if t else None, ctypes.byref(handle)))
+        self._handle = handle
+        self.prim_count = n
+        return t.as_dict() if t else None
+
+
+def _check_spheres(spheres: torch.Tensor, ctx: DeviceContext) -> int:
+    require_device(spheres, ctx, "spheres")
+    if spheres.dtype != torch.float32:
+        raise ValueError("spheres must be float32")
+    if spheres.numel() % 4 != 0 or spheres.numel() == 0:
+        raise ValueError("spheres hold four floats (center.xyz, radius) per sphere")
+    return spheres.numel() // 4
+
+
+def build_sphere_bvh(spheres: torch.Tensor, ctx: DeviceContext | None = None, node_width: int = 2, leaf_size: int = 0,
+                     method: int = GpuBvhBuildMethod.LBVH, timings: dict | None = None) -> GpuSphereBvh:
+    """build_sphere_bvh[WORLD, W, L] (bvh/gpu/sphere_bvh.mojo:320-371) over spheres f32[4N] on the device."""
+    ctx = ctx or default_context(spheres.device.index or 0)
+    n = _check_spheres(spheres, ctx)
+    t = _lib.BuildTimings() if timings is not None else None
+    handle = ctypes.c_void_p()
+    opts = build_options(node_width, leaf_size, method)
+    _lib.check(_lib.lib().bajo_build_sphere_bvh(ctx.handle, ptr(spheres), n, ctypes.byref(opts),
+                                                ctypes.byref(t) if t else None, ctypes.byref(handle)))
+    if t is not None:
+        timings.update(t.as_dict())
+    return GpuSphereBvh(handle, ctx, n, node_width, leaf_size, method)
+
+
 class GpuTriangleBlasSet:
     """GpuBlasSet of triangles (bvh/gpu/blas_storage.mojo:21-120) made by build_triangle_blas_set
     (bvh/gpu/triangle_bvh.mojo:469-491): several meshes built in one segmented pass.  `blas(k)` is mesh k

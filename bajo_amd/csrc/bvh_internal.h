@@ -75,8 +75,10 @@ struct bajo_bvh {
   uint32_t* hp_ctl;       // node counter, status, root
   uint32_t* host_hp_ctl;  // pinned host mirror of hp_ctl
   uint64_t hp_capacity;
-  // input kind: triangles f32[9N] (traceable) or leaf boxes f32[6N]
+  // input kind: triangles f32[9N] (traceable) or leaf boxes f32[6N]; a
+  // sphere BVH is a boxes build whose sorted leaf records hold the spheres
   bool boxes;
+  bool spheres;
   // segmented build (a BLAS set): seg_count > 0 segments share this storage;
   // segment s owns sorted leaves [off_s, off_{s+1}) and nodes [off_s, off_{s+1} - 1)
   uint32_t seg_count;

@@ -486,6 +486,32 @@ __global__ void __launch_bounds__(kBlock) seg_karras_kernel(const uint64_t* __re
               (uint32_t)first, (uint32_t)first);
 }
 
+// ---- sphere BVHs (bvh/gpu/sphere_bvh.mojo:320-371, 480-530) -------------------
+// Sphere.bounds (bvh/types.mojo:122-124): center -/+ radius per axis.
+__global__ void __launch_bounds__(kBlock) sphere_boxes_kernel(const float* __restrict__ spheres, uint64_t n,
+                                                              float* __restrict__ boxes) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 sp = reinterpret_cast<const float4*>(spheres)[i];
+  float* b = boxes + 6 * i;
+  b[0] = sp.x - sp.w; b[1] = sp.y - sp.w; b[2] = sp.z - sp.w;
+  b[3] = sp.x + sp.w; b[4] = sp.y + sp.w; b[5] = sp.z + sp.w;
+}
+
+// The sphere of each sorted leaf into its 48-byte record: center + prim, radius.
+__global__ void __launch_bounds__(kBlock) pack_spheres_kernel(const float* __restrict__ spheres,
+                                                              const uint64_t* __restrict__ keys, uint64_t n,
+                                                              BjTri* __restrict__ recs) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t prim = (uint32_t)keys[k];
+  const float4 sp = reinterpret_cast<const float4*>(spheres)[prim];
+  float4* out = reinterpret_cast<float4*>(recs + k);
+  out[0] = make_float4(sp.x, sp.y, sp.z, __builtin_bit_cast(float, prim));
+  out[1] = make_float4(sp.w, 0.0f, 0.0f, 0.0f);
+  out[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 int32_t ensure_capacity(bajo_bvh* h, uint64_t n) {
   SortPlan plan = make_sort_plan(n, 8, 0, kSortBeginBit, kSortEndBit);
   if (n <= h->capacity && plan.total <= h->sort_ws_bytes) return BAJO_OK;
@@ -822,7 +848,33 @@ int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_
 
 int32_t bajo_build_binary_bounds(bajo_ctx* ctx, const float* leaf_bounds, uint64_t n,
                                  const bajo_build_options* options, bajo_build_timings* timings, bajo_bvh** out) {
+  BJ_REQUIRE(out == nullptr || *out == nullptr || !(*out)->spheres,
+             "bajo_build_binary_bounds: a sphere BVH is rebuilt by bajo_build_sphere_bvh");
   return bj::build_entry(ctx, leaf_bounds, n, options, timings, out, true);
+}
+
+int32_t bajo_build_sphere_bvh(bajo_ctx* ctx, const float* spheres, uint64_t n, const bajo_build_options* options,
+                              bajo_build_timings* timings, bajo_bvh** out) {
+  using namespace bj;
+  BJ_REQUIRE(ctx && spheres && out, "bajo_build_sphere_bvh: NULL argument");
+  BJ_REQUIRE(n > 0, "bajo_build_sphere_bvh: a BVH needs nonempty input");
+  BJ_REQUIRE(n <= kSortMaxKeys, "bajo_build_sphere_bvh: n=%llu exceeds 2^30-1 spheres", (unsigned long long)n);
+  BJ_REQUIRE(*out == nullptr || (*out)->spheres, "bajo_build_sphere_bvh: the handle is not a sphere BVH");
+  BJ_REQUIRE(((uintptr_t)spheres & 15u) == 0, "bajo_build_sphere_bvh: spheres must be 16-byte aligned");
+  // the leaf boxes live in the context scratch for the duration of the build
+  void* boxes = nullptr;
+  int32_t rc = ctx_scratch(ctx, 6 * n * sizeof(float), &boxes);
+  if (rc) return rc;
+  sphere_boxes_kernel<<<div_up(n, (uint64_t)kBlock), kBlock, 0, ctx->stream>>>(spheres, n, (float*)boxes);
+  BJ_HIP(hipGetLastError());
+  bajo_bvh* h = *out;
+  rc = build_entry(ctx, (const float*)boxes, n, options, timings, &h, true);
+  if (rc) return rc;
+  pack_spheres_kernel<<<div_up(n, (uint64_t)kBlock), kBlock, 0, ctx->stream>>>(spheres, h->keys, n, h->tris);
+  BJ_HIP(hipGetLastError());
+  h->spheres = true;
+  *out = h;
+  return BAJO_OK;
 }
 
 }  // extern "C"

@@ -1,7 +1,7 @@
-// Instances of triangle BVHs under a top-level hierarchy (gfx950).
+// Instances of triangle or sphere BVHs under a top-level hierarchy (gfx950).
 //
-// Replaces build_triangle_tlas / GpuTriangleTlas.launch_camera
-// (bvh/gpu/tlas.mojo:54-233, 482-645) with bajo's Instance (bvh/types.mojo:
+// Replaces build_triangle_tlas / build_sphere_tlas and their launch_camera
+// (bvh/gpu/tlas.mojo:54-233, 482-715) with bajo's Instance (bvh/types.mojo:
 // 153-190), Affine3 (core/transform.mojo: point / vector / inverse /
 // normal_from_inverse) and AABB.apply_transform (core/aabb.mojo:101-137).
 //
@@ -24,7 +24,7 @@
 struct TlasBlas {
   bj::BvhView bin;
   bj::WideView wide;
-  uint32_t is_wide;
+  uint32_t is_wide, is_sphere;
   uint32_t prim_count;
 };
 
@@ -96,8 +96,12 @@ BJ_D Hit tlas_trace(const TlasView& tv, const Ray& ray, Stack& st, uint32_t* ts,
     lr.d = xf_vector(m, ray.d);
     lr.tmin = ray.tmin;
     lr.tmax = kMode == BAJO_TRACE_ANY_HIT ? ray.tmax : hit.t;
-    const Hit lh = b.is_wide ? traverse8<kMode, false>(b.wide, lr, st, nullptr)
-                             : traverse<kMode, false>(b.bin, lr, st, nullptr);
+    Hit lh;
+    if (b.is_sphere)
+      lh = b.is_wide ? traverse8<kMode, false, true>(b.wide, lr, st, nullptr)
+                     : traverse<kMode, false, true>(b.bin, lr, st, nullptr);
+    else
+      lh = b.is_wide ? traverse8<kMode, false>(b.wide, lr, st, nullptr) : traverse<kMode, false>(b.bin, lr, st, nullptr);
     if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
       if (lh.prim == kEmpty && lh.t == 0.0f) {
         occ_inst = inst;
@@ -303,8 +307,9 @@ int32_t bajo_tlas_create(bajo_ctx* ctx, const bajo_bvh* const* blases, uint32_t 
   std::vector<float> roots(6 * (size_t)blas_count);
   for (uint32_t b = 0; b < blas_count; ++b) {
     const bajo_bvh* h = blases[b];
-    BJ_REQUIRE(h != nullptr && !h->boxes && h->n > 0 && !h->seg_count,
-               "bajo_tlas_create: BLAS %u is not a triangle BVH (a BLAS set enters through its segment views)", b);
+    BJ_REQUIRE(h != nullptr && (!h->boxes || h->spheres) && h->n > 0 && !h->seg_count,
+               "bajo_tlas_create: BLAS %u is not a triangle or sphere BVH (a BLAS set enters through its segment "
+               "views)", b);
     BJ_REQUIRE(h->device == ctx->device, "bajo_tlas_create: BLAS %u belongs to device %d", b, h->device);
     BJ_HIP(hipMemcpyAsync(roots.data() + 6 * (size_t)b, h->scene, 6 * sizeof(float), hipMemcpyDeviceToHost,
                           ctx->stream));
@@ -341,7 +346,7 @@ int32_t bajo_tlas_create(bajo_ctx* ctx, const bajo_bvh* const* blases, uint32_t 
   for (uint32_t b = 0; b < blas_count; ++b) {
     const AccelView av = accel_view(blases[b], t->status);
     views[b] = TlasBlas{av.bin, av.wide, blases[b]->node_width == BAJO_WIDTH_WIDE8 ? 1u : 0u,
-                        (uint32_t)blases[b]->n};
+                        blases[b]->spheres ? 1u : 0u, (uint32_t)blases[b]->n};
   }
   TL_HIP(hipMalloc(&t->boxes, t->host_boxes.size() * sizeof(float)));
   TL_HIP(hipMalloc(&t->inv, inv.size() * sizeof(float)));

@@ -33,7 +33,7 @@ BJ_D uint32_t segment_root(const uint64_t* __restrict__ off, uint64_t e) {
   return off[e + 1] - first == 1 ? (kLeafFlag | (uint32_t)first) : (uint32_t)first;
 }
 
-template <int kMode, bool kStats, bool kCamera, bool kWide, bool kSeg>
+template <int kMode, bool kStats, bool kCamera, bool kWide, bool kSeg, bool kSph>
 __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const AccelView av,
                                                             const float* __restrict__ rays,
                                                             const float* __restrict__ cams, uint32_t cam_w,
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const AccelView av,
       if constexpr (kWide) v.wide.root = root;
       else v.bin.root = root;
     }
-    Hit hit = trace_accel<kMode, kStats, kWide>(v, ray, st, &tc);
+    Hit hit = trace_accel<kMode, kStats, kWide, kSph>(v, ray, st, &tc);
     store_hit(hits, r, hit);
     if constexpr (kStats) {
       uint32_t* s = stats + 5 * r;
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(const AccelView av,
 constexpr uint32_t kRefillLanes = 16;
 constexpr uint32_t kRefillChunk = 256;
 
-template <int kMode, bool kStats, bool kCamera, bool kSeg>
+template <int kMode, bool kStats, bool kCamera, bool kSeg, bool kSph>
 __global__ void __launch_bounds__(kTraceBlock) trace_refill_kernel(const WideView wv, const float* __restrict__ rays,
                                                                    const float* __restrict__ cams, uint32_t cam_w,
                                                                    uint32_t cam_h, float inv_height,
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kTraceBlock) trace_refill_kernel(const WideVie
   const uint32_t lane = lane_id();
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
-  Walk8<kMode, kStats> w;
+  Walk8<kMode, kStats, kSph> w;
   w.cur = kSentinel;
   bool active = false, drained = false;
   uint64_t r = 0, next = 0, stop = 0;  // the wave's reserved, not yet assigned indices [next, stop)
@@ -187,7 +187,7 @@ struct TraceScratch {
   unsigned grid;
 };
 
-template <int kMode, bool kStats, bool kCamera, bool kWide, bool kSeg>
+template <int kMode, bool kStats, bool kCamera, bool kWide, bool kSeg, bool kSph>
 int32_t occupancy_grid(unsigned* grid) {
   static unsigned cached = 0;
   if (!cached) {
@@ -195,11 +195,11 @@ int32_t occupancy_grid(unsigned* grid) {
     BJ_HIP(hipGetDevice(&dev));
     BJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     if constexpr (kWide)
-      BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_refill_kernel<kMode, kStats, kCamera, kSeg>,
+      BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_refill_kernel<kMode, kStats, kCamera, kSeg, kSph>,
                                                           kTraceBlock, 0));
     else
       BJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
-                                                          trace_kernel<kMode, kStats, kCamera, kWide, kSeg>,
+                                                          trace_kernel<kMode, kStats, kCamera, kWide, kSeg, kSph>,
                                                           kTraceBlock, 0));
     if (per_cu < 1) per_cu = 1;
     cached = (unsigned)(cus * per_cu);
@@ -208,11 +208,11 @@ int32_t occupancy_grid(unsigned* grid) {
   return BAJO_OK;
 }
 
-template <int kMode, bool kStats, bool kCamera, bool kWide, bool kSeg = false>
+template <int kMode, bool kStats, bool kCamera, bool kWide, bool kSeg = false, bool kSph = false>
 int32_t launch_width(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const float* cams, uint32_t w, uint32_t hh,
                      float* hits, uint64_t R, uint32_t* stats) {
   unsigned grid = 0;
-  int32_t rc = occupancy_grid<kMode, kStats, kCamera, kWide, kSeg>(&grid);
+  int32_t rc = occupancy_grid<kMode, kStats, kCamera, kWide, kSeg, kSph>(&grid);
   if (rc) return rc;
   uint64_t need_blocks = (R + kTraceBlock - 1) / kTraceBlock;
   if ((uint64_t)grid > need_blocks) grid = (unsigned)need_blocks;
@@ -228,10 +228,10 @@ int32_t launch_width(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const 
   const AccelView av = accel_view(h, status);
   const uint64_t* seg_off = kSeg ? h->seg_off : nullptr;
   if constexpr (kWide)
-    trace_refill_kernel<kMode, kStats, kCamera, kSeg><<<grid, kTraceBlock, 0, ctx->stream>>>(
+    trace_refill_kernel<kMode, kStats, kCamera, kSeg, kSph><<<grid, kTraceBlock, 0, ctx->stream>>>(
         av.wide, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill, seg_off);
   else
-    trace_kernel<kMode, kStats, kCamera, kWide, kSeg><<<grid, kTraceBlock, 0, ctx->stream>>>(
+    trace_kernel<kMode, kStats, kCamera, kWide, kSeg, kSph><<<grid, kTraceBlock, 0, ctx->stream>>>(
         av, rays, cams, w, hh, hh ? 1.0f / (float)hh : 0.0f, hits, R, stats, counter, spill, seg_off);
   BJ_HIP(hipGetLastError());
   return BAJO_OK;
@@ -240,6 +240,11 @@ int32_t launch_width(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const 
 template <int kMode, bool kStats, bool kCamera>
 int32_t launch(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const float* cams, uint32_t w, uint32_t hh,
                float* hits, uint64_t R, uint32_t* stats) {
+  if (h->spheres) {
+    if (h->node_width == BAJO_WIDTH_WIDE8)
+      return launch_width<kMode, kStats, kCamera, true, false, true>(ctx, h, rays, cams, w, hh, hits, R, stats);
+    return launch_width<kMode, kStats, kCamera, false, false, true>(ctx, h, rays, cams, w, hh, hits, R, stats);
+  }
   if (h->node_width == BAJO_WIDTH_WIDE8)
     return launch_width<kMode, kStats, kCamera, true>(ctx, h, rays, cams, w, hh, hits, R, stats);
   return launch_width<kMode, kStats, kCamera, false>(ctx, h, rays, cams, w, hh, hits, R, stats);
@@ -312,7 +317,7 @@ int32_t bajo_trace_rays(bajo_ctx* ctx, const bajo_bvh* bvh, int32_t mode, const 
   BJ_REQUIRE(ray_count < (1ull << 31), "bajo_trace_rays: ray count must be below 2^31");
   BJ_REQUIRE(rays != nullptr && hits != nullptr, "bajo_trace_rays: NULL rays/hits");
   BJ_REQUIRE(bvh->n > 0, "bajo_trace_rays: empty BVH");
-  BJ_REQUIRE(!bvh->boxes, "bajo_trace_rays: the hierarchy was built over boxes, not triangles");
+  BJ_REQUIRE(!bvh->boxes || bvh->spheres, "bajo_trace_rays: the hierarchy was built over boxes, not triangles or spheres");
   BJ_REQUIRE(!bvh->seg_count, "bajo_trace_rays: trace a BLAS set through its segment views (bajo_blas_set_get)");
   if (mode == BAJO_TRACE_ANY_HIT) {
     BJ_REQUIRE(stats == nullptr, "bajo_trace_rays: stats are collected for closest-hit queries only");
@@ -329,7 +334,7 @@ int32_t bajo_trace_camera(bajo_ctx* ctx, const bajo_bvh* bvh, const float* camer
   BJ_REQUIRE(ray_count < (1ull << 31), "bajo_trace_camera: ray count must be below 2^31");
   BJ_REQUIRE(camera != nullptr && hits != nullptr, "bajo_trace_camera: NULL camera/hits");
   BJ_REQUIRE(bvh->n > 0, "bajo_trace_camera: empty BVH");
-  BJ_REQUIRE(!bvh->boxes, "bajo_trace_camera: the hierarchy was built over boxes, not triangles");
+  BJ_REQUIRE(!bvh->boxes || bvh->spheres, "bajo_trace_camera: the hierarchy was built over boxes, not triangles or spheres");
   BJ_REQUIRE(!bvh->seg_count, "bajo_trace_camera: trace a BLAS set through its segment views (bajo_blas_set_get)");
   if (stats)
     return bj::launch<BAJO_TRACE_CLOSEST_HIT, true, true>(ctx, bvh, nullptr, camera, width, height, hits, ray_count, stats);

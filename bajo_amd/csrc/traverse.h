@@ -119,6 +119,74 @@ BJ_D bool leaf_any(const BjTri* __restrict__ tris, uint32_t s, const Ray& r) {
   return tri_scaled(r, tr, r.tmax, ad, ts, us, vs);
 }
 
+// ---- sphere leaves (bajo's GpuSphereBvh, bvh/gpu/sphere_bvh.mojo:423-478) ----
+// A sphere BVH stores one 48-byte record per sorted leaf like a triangle BVH:
+// center.xyz + prim id, then the radius (the rest unused).
+struct SphData {
+  v3 c;
+  float r;
+  uint32_t prim;
+};
+
+BJ_D SphData load_sphere(const BjTri* __restrict__ recs, uint32_t s) {
+  const float4* p = reinterpret_cast<const float4*>(recs + s);
+  const float4 a = p[0], b = p[1];
+  return SphData{mk(a.x, a.y, a.z), b.x, __builtin_bit_cast(uint32_t, a.w)};
+}
+
+// intersect_ray_sphere (core/intersect.mojo:309-350): the nearer root inside
+// (t_min, t_max), else the farther one.
+BJ_D bool sphere_t(const Ray& r, const SphData& sp, float t_max, float& t) {
+  const v3 oc = sub(r.o, sp.c);
+  const float a = dot(r.d, r.d), half_b = dot(oc, r.d), c = dot(oc, oc) - sp.r * sp.r;
+  const float det = half_b * half_b - a * c;
+  const bool ok = det >= 0.0f;
+  const float sq = sqrtf(fmax_sel(det, 0.0f));
+  const float inv_a = 1.0f / a;
+  const float t0 = (-half_b - sq) * inv_a, t1 = (-half_b + sq) * inv_a;
+  const bool near = ok && t0 > r.tmin && t0 < t_max;
+  const bool far = ok && !near && t1 > r.tmin && t1 < t_max;
+  t = near ? t0 : t1;
+  return near || far;
+}
+
+// Leaf tests and the final hit record for either primitive kind.
+template <bool kSph>
+BJ_D bool leaf_closest_p(const BjTri* __restrict__ recs, uint32_t s, const Ray& r, Closest& c) {
+  if constexpr (kSph) {
+    float t;
+    if (!sphere_t(r, load_sphere(recs, s), c.t, t)) return false;
+    c.t = t;
+    c.s = s;
+    return true;
+  } else {
+    return leaf_closest(recs, s, r, c);
+  }
+}
+
+template <bool kSph>
+BJ_D bool leaf_any_p(const BjTri* __restrict__ recs, uint32_t s, const Ray& r) {
+  if constexpr (kSph) {
+    float t;
+    return sphere_t(r, load_sphere(recs, s), r.tmax, t);
+  } else {
+    return leaf_any(recs, s, r);
+  }
+}
+
+// Sphere hit: u = v = 0, normal = normalize(o + t d - center).
+template <bool kSph>
+BJ_D Hit closest_hit_p(const BjTri* __restrict__ recs, const Closest& c, const Ray& r) {
+  if constexpr (kSph) {
+    if (c.s == kEmpty) return miss(c.t);
+    const SphData sp = load_sphere(recs, c.s);
+    const v3 p = add(r.o, scale(r.d, c.t));
+    return Hit{0.0f, 0.0f, sp.prim, kEmpty, normalize(sub(p, sp.c)), c.t};
+  } else {
+    return closest_hit(recs, c);
+  }
+}
+
 struct Stack {
   uint32_t* lds;       // this lane's column: entry k at lds[k * kTraceBlock]
   uint32_t* spill;     // this thread's column: entry k at spill[k * spill_stride]
@@ -149,7 +217,7 @@ struct TraceCounters {
 
 // Closest hit (kMode == BAJO_TRACE_CLOSEST_HIT) or occlusion (ANY_HIT: the
 // returned record is bajo's Hit.shadow_hit() or Hit.miss(t_max)).
-template <int kMode, bool kStats>
+template <int kMode, bool kStats, bool kSph = false>
 BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* tc) {
   uint32_t n_visit = 0, n_inner = 0, n_leaf = 0, max_sp = 0;
   const BjNode* __restrict__ nodes = bv.nodes;
@@ -163,8 +231,8 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
 
   if ((root & kLeafFlag) && root != kSentinel) {
     n_leaf++;
-    if constexpr (kMode == BAJO_TRACE_ANY_HIT) occluded = leaf_any(tris, root & kIndexMask, ray);
-    else leaf_closest(tris, root & kIndexMask, ray, best);
+    if constexpr (kMode == BAJO_TRACE_ANY_HIT) occluded = leaf_any_p<kSph>(tris, root & kIndexMask, ray);
+    else leaf_closest_p<kSph>(tris, root & kIndexMask, ray, best);
   } else if (root != kSentinel) {
     uint32_t cur = root;
     for (;;) {
@@ -186,7 +254,7 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
         if (h0) {
           if (l0) {
             n_leaf++;
-            if (leaf_any(tris, c0 & kIndexMask, ray)) { occluded = true; break; }
+            if (leaf_any_p<kSph>(tris, c0 & kIndexMask, ray)) { occluded = true; break; }
           } else if (!(c0 & kLeafFlag)) {
             if (st.sp >= kMaxStack) { atomicOr(status, (uint32_t)BAJO_STATUS_STACK_OVERFLOW); break; }
             st.push(c0);
@@ -195,7 +263,7 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
         if (h1) {
           if (l1) {
             n_leaf++;
-            if (leaf_any(tris, c1 & kIndexMask, ray)) { occluded = true; break; }
+            if (leaf_any_p<kSph>(tris, c1 & kIndexMask, ray)) { occluded = true; break; }
           } else if (!(c1 & kLeafFlag)) {
             if (st.sp >= kMaxStack) { atomicOr(status, (uint32_t)BAJO_STATUS_STACK_OVERFLOW); break; }
             st.push(c1);
@@ -207,11 +275,11 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
       } else {
         bool v0 = false, v1 = false;
         if (h0) {
-          if (l0) { n_leaf++; leaf_closest(tris, c0 & kIndexMask, ray, best); }
+          if (l0) { n_leaf++; leaf_closest_p<kSph>(tris, c0 & kIndexMask, ray, best); }
           else if (!(c0 & kLeafFlag)) { v0 = true; n_inner++; }
         }
         if (h1) {
-          if (l1) { n_leaf++; leaf_closest(tris, c1 & kIndexMask, ray, best); }
+          if (l1) { n_leaf++; leaf_closest_p<kSph>(tris, c1 & kIndexMask, ray, best); }
           else if (!(c1 & kLeafFlag)) { v1 = true; n_inner++; }
         }
         // bvh/gpu/trace.mojo:395-426: nearest child first, push the other.
@@ -239,7 +307,7 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
     // bvh/gpu/trace.mojo:44-59: occlusion is reported as Hit.shadow_hit().
     hit = occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
   } else {
-    hit = closest_hit(tris, best);
+    hit = closest_hit_p<kSph>(tris, best, ray);
   }
   if constexpr (kStats) {
     tc->visits = n_visit;
@@ -294,7 +362,7 @@ BJ_D uint32_t sel8(uint32_t bit, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t
 // done() once the stack is empty, the ray is occluded (any hit) or an error
 // ended it.  Launches that refill finished lanes with new rays call the steps
 // themselves; traverse8 runs one ray to the end.
-template <int kMode, bool kStats>
+template <int kMode, bool kStats, bool kSph = false>
 struct Walk8 {
   Ray ray;
   v3 rcp;
@@ -380,12 +448,12 @@ struct Walk8 {
       }
       n_leaf++;
       if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
-        if (leaf_any(wv.tris, tri, ray)) {
+        if (leaf_any_p<kSph>(wv.tris, tri, ray)) {
           occluded = true;
           break;
         }
       } else {
-        leaf_closest(wv.tris, tri, ray, best);
+        leaf_closest_p<kSph>(wv.tris, tri, ray, best);
       }
       ++tri;
     }
@@ -439,7 +507,7 @@ struct Walk8 {
     if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
       hit = occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
     } else {
-      hit = closest_hit(wv.tris, best);
+      hit = closest_hit_p<kSph>(wv.tris, best, ray);
     }
     if constexpr (kStats) {
       tc->visits = n_visit;
@@ -451,9 +519,9 @@ struct Walk8 {
   }
 };
 
-template <int kMode, bool kStats>
+template <int kMode, bool kStats, bool kSph = false>
 BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters* tc) {
-  Walk8<kMode, kStats> w;
+  Walk8<kMode, kStats, kSph> w;
   w.begin(wv, ray, st);
   while (!w.done()) w.step(wv, st);
   return w.finish(wv, tc);
@@ -487,10 +555,10 @@ inline AccelView accel_view(const bajo_bvh* h, uint32_t* status) {
   return av;
 }
 
-template <int kMode, bool kStats, bool kWide>
+template <int kMode, bool kStats, bool kWide, bool kSph = false>
 BJ_D Hit trace_accel(const AccelView& av, const Ray& ray, Stack& st, TraceCounters* tc) {
-  if constexpr (kWide) return traverse8<kMode, kStats>(av.wide, ray, st, tc);
-  else return traverse<kMode, kStats>(av.bin, ray, st, tc);
+  if constexpr (kWide) return traverse8<kMode, kStats, kSph>(av.wide, ray, st, tc);
+  else return traverse<kMode, kStats, kSph>(av.bin, ray, st, tc);
 }
 
 }  // namespace bj

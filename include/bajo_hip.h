@@ -172,6 +172,16 @@ int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_
  * boxes hierarchy too).  Ray launches reject a boxes hierarchy. */
 int32_t bajo_build_binary_bounds(bajo_ctx* ctx, const float* leaf_bounds, uint64_t n,
                                  const bajo_build_options* options, bajo_build_timings* timings, bajo_bvh** out);
+/* build_sphere_bvh (bvh/gpu/sphere_bvh.mojo:320-371): spheres f32[4N]
+ * (center.xyz, radius; 16-byte aligned) -> the box builder over Sphere.bounds
+ * (LBVH or H-PLOC, node width 2 or 8), then each sorted leaf's sphere packed
+ * into its leaf record.  Ray launches trace it with bajo's ray-sphere test
+ * (core/intersect.mojo:309-350; hit u = v = 0, normal = normalize(p - c));
+ * it enters a TLAS like a triangle BVH.  A non-NULL *out (a sphere BVH) is
+ * rebuilt in place.  Uses the context scratch (24 B per sphere) while
+ * building. */
+int32_t bajo_build_sphere_bvh(bajo_ctx* ctx, const float* spheres, uint64_t n, const bajo_build_options* options,
+                              bajo_build_timings* timings, bajo_bvh** out);
 int32_t bajo_bvh_free(bajo_bvh* bvh);
 int32_t bajo_bvh_get_info(const bajo_bvh* bvh, bajo_bvh_info* out);
 /* Copies the binary LBVH into bajo's GpuBinaryBoundsBvh layout

@@ -97,6 +97,15 @@ _SIG = {
         c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                 c_int],
     ),
+    "orc_trace_bvh2_spheres": (
+        None,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_uint64, c_int, c_void_p, c_void_p],
+    ),
+    "orc_trace_brute_spheres": (None, [c_void_p, c_uint64, c_void_p, c_uint64, c_void_p]),
+    "orc_trace_wide_spheres": (
+        c_uint64,
+        [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_uint64, c_int, c_void_p, c_void_p],
+    ),
     "orc_affine_inverse": (c_int, [c_void_p, c_void_p]),
     "orc_transform_box": (None, [c_void_p, c_void_p, c_void_p]),
     "orc_tlas_trace": (
@@ -643,7 +652,8 @@ def ao_rays(rays: np.ndarray, hits: np.ndarray, samples: int, seed: int) -> np.n
 # ---- instances ------------------------------------------------------------------
 
 class OrcBvh(ctypes.Structure):
-    """orc_bvh: one BLAS for orc_tlas_trace (binary layout, or 8-wide when `wide` is set)."""
+    """orc_bvh: one BLAS for orc_tlas_trace (binary layout, or 8-wide when `wide` is set; sphere leaves when
+    `spheres` is set)."""
 
     _fields_ = [
         ("vertices", c_void_p),
@@ -654,6 +664,7 @@ class OrcBvh(ctypes.Structure):
         ("wide", c_void_p),
         ("wide_count", c_uint64),
         ("prim_count", c_uint64),
+        ("spheres", c_void_p),
     ]
 
 
@@ -683,22 +694,28 @@ def tlas_instance_bounds(transforms: np.ndarray, blas_boxes: np.ndarray, inst_bl
 
 def tlas_trace(blases: list, transforms: np.ndarray, inst_blas: np.ndarray, top: dict, rays: np.ndarray,
                mode: int = 0) -> np.ndarray:
-    """Rays through instances of BLASes.  blases: dicts with 'vertices' and the binary layout of
-    build_lbvh / build_hploc ('leaf_ids', 'node_meta', 'node_bounds', 'root'), plus 'wide' (8-wide
-    nodes over the same leaf_ids) to traverse that layout instead; transforms f32[12I] BLAS -> world;
+    """Rays through instances of BLASes.  blases: dicts with 'vertices' (or 'spheres' f32[4n]) and the
+    binary layout of build_lbvh / build_hploc ('leaf_ids', 'node_meta', 'node_bounds', 'root'), plus
+    'wide' (8-wide nodes over the same leaf_ids) to traverse that layout instead; transforms f32[12I] BLAS -> world;
     top: binary layout over the instance boxes whose leaf_ids are instance indices."""
     keep = []
     descs = (OrcBvh * len(blases))()
     for k, b in enumerate(blases):
-        v = f32(b["vertices"])
+        sph = b.get("spheres") is not None
+        v = f32(b["spheres"] if sph else b["vertices"])
         ids = np.ascontiguousarray(b["leaf_ids"], np.uint32)
         meta = np.ascontiguousarray(b["node_meta"] if b["node_meta"].size else np.zeros(4), np.uint32)
         nb = f32(b["node_bounds"] if b["node_bounds"].size else np.zeros(12))
         keep += [v, ids, meta, nb]
         d = descs[k]
-        d.vertices, d.leaf_ids, d.node_meta, d.node_bounds = (x.ctypes.data for x in (v, ids, meta, nb))
+        d.leaf_ids, d.node_meta, d.node_bounds = (x.ctypes.data for x in (ids, meta, nb))
+        if sph:
+            d.spheres = v.ctypes.data
+            d.prim_count = v.size // 4
+        else:
+            d.vertices = v.ctypes.data
+            d.prim_count = v.size // 9
         d.root = int(b["root"])
-        d.prim_count = v.size // 9
         if b.get("wide") is not None:
             w = np.ascontiguousarray(b["wide"], WIDE_NODE_DTYPE)
             keep.append(w)
@@ -722,3 +739,49 @@ def tlas_trace(blases: list, transforms: np.ndarray, inst_blas: np.ndarray, top:
                          _p(tnb), int(top["root"]), _p(r), R, int(mode), _p(hits))
     del keep
     return hits
+
+
+# ---- spheres ----------------------------------------------------------------------
+
+def sphere_bounds(spheres) -> np.ndarray:
+    """Sphere.bounds (bvh/types.mojo:122-124) of f32[4N] spheres: f32[6N] boxes."""
+    sp = f32(spheres).reshape(-1, 4)
+    c, r = sp[:, :3], sp[:, 3:4]
+    return np.ascontiguousarray(np.concatenate([c - r, c + r], axis=1).reshape(-1), np.float32)
+
+
+def trace_spheres(spheres, bvh: dict, rays: np.ndarray, mode: int = 0, stats: bool = False):
+    """Near-first / unordered queries over a sphere BVH (binary layout of the box build over sphere_bounds)."""
+    sp = f32(spheres)
+    r = f32(rays)
+    R = r.size // 8
+    hits = np.empty(8 * R, np.float32)
+    meta = bvh["node_meta"] if bvh["node_meta"].size else np.zeros(4, np.uint32)
+    nb = bvh["node_bounds"] if bvh["node_bounds"].size else np.zeros(12, np.float32)
+    st = np.empty(5 * R, np.uint32) if stats else None
+    lib().orc_trace_bvh2_spheres(_p(sp), _p(np.ascontiguousarray(bvh["leaf_ids"], np.uint32)),
+                                 _p(np.ascontiguousarray(meta, np.uint32)), _p(f32(nb)), int(bvh["root"]), _p(r), R,
+                                 mode, _p(hits), _p(st) if st is not None else None)
+    return (hits.reshape(R, 8), st.reshape(R, 5)) if stats else hits.reshape(R, 8)
+
+
+def trace_spheres_wide(spheres, leaf_ids, wide: np.ndarray, rays: np.ndarray, mode: int = 0, with_stats: bool = False):
+    sp = f32(spheres)
+    ids = np.ascontiguousarray(leaf_ids, np.uint32)
+    w = np.ascontiguousarray(wide, WIDE_NODE_DTYPE)
+    r = f32(rays)
+    R = r.size // 8
+    hits = np.empty(8 * R, np.float32)
+    stats = np.empty(5 * R, np.uint32) if with_stats else None
+    over = lib().orc_trace_wide_spheres(_p(sp), sp.size // 4, _p(ids), w.ctypes.data_as(ctypes.c_void_p), w.size,
+                                        _p(r), R, mode, _p(hits), _p(stats) if stats is not None else None)
+    return hits.reshape(R, 8), (stats.reshape(R, 5) if stats is not None else None), int(over)
+
+
+def trace_brute_spheres(spheres, rays: np.ndarray) -> np.ndarray:
+    sp = f32(spheres)
+    r = f32(rays)
+    R = r.size // 8
+    hits = np.empty(8 * R, np.float32)
+    lib().orc_trace_brute_spheres(_p(sp), sp.size // 4, _p(r), R, _p(hits))
+    return hits.reshape(R, 8)

@@ -85,6 +85,15 @@ uint32_t orc_hploc_binary(uint32_t root, const uint32_t *left, const uint32_t *r
 void orc_hploc_lowest_leaf_numbering(const uint32_t *left, const uint32_t *right, const uint32_t *sorted_ids,
                                      uint64_t n, uint32_t *out);
 
+/* sphere BVHs (orc_trace.c, orc_wide.c) */
+void orc_trace_bvh2_spheres(const float *spheres, const uint32_t *leaf_ids, const uint32_t *node_meta,
+                            const float *node_bounds, uint32_t root, const float *rays, uint64_t R, int mode,
+                            float *hits, uint32_t *stats);
+void orc_trace_brute_spheres(const float *spheres, uint64_t n, const float *rays, uint64_t R, float *hits);
+uint64_t orc_trace_wide_spheres(const float *spheres, uint64_t n, const uint32_t *leaf_ids, const void *wide,
+                                uint64_t wide_count, const float *rays, uint64_t R, int mode, float *hits,
+                                uint32_t *stats);
+
 /* instances (orc_tlas.c); orc_bvh is declared in orc_internal.h */
 int orc_affine_inverse(const float *a, float *o);
 void orc_transform_box(const float *m, const float *b, float *out);

@@ -72,6 +72,13 @@ static void store_hit(float *hits, uint64_t r, orc_hit h) {
 /* One scaled-MT leaf test that updates the closest hit. */
 static int leaf_closest(const orc_bvh *b, uint32_t sorted_leaf, orc_v3 o, orc_v3 d, float t_min, orc_hit *hit) {
     uint32_t prim = b->leaf_ids[sorted_leaf];
+    if (b->spheres) {
+        const float *sp = b->spheres + 4 * (uint64_t)prim;
+        float t;
+        if (!orc_sphere_t(sp, o, d, t_min, hit->t, &t)) return 0;
+        orc_sphere_record(sp, prim, o, d, t, hit);
+        return 1;
+    }
     const float *t = b->vertices + 9 * (uint64_t)prim;
     orc_v3 v0 = orc_v(t[0], t[1], t[2]);
     orc_v3 e1 = orc_sub(orc_v(t[3], t[4], t[5]), v0);
@@ -90,6 +97,10 @@ static int leaf_closest(const orc_bvh *b, uint32_t sorted_leaf, orc_v3 o, orc_v3
 
 static int leaf_any(const orc_bvh *b, uint32_t sorted_leaf, orc_v3 o, orc_v3 d, float t_min, float t_max) {
     uint32_t prim = b->leaf_ids[sorted_leaf];
+    if (b->spheres) {
+        float t;
+        return orc_sphere_t(b->spheres + 4 * (uint64_t)prim, o, d, t_min, t_max, &t);
+    }
     const float *t = b->vertices + 9 * (uint64_t)prim;
     orc_v3 v0 = orc_v(t[0], t[1], t[2]);
     orc_v3 e1 = orc_sub(orc_v(t[3], t[4], t[5]), v0);
@@ -188,13 +199,46 @@ orc_hit orc_trace_any(const orc_bvh *b, orc_v3 o, orc_v3 d, float t_min, float t
     return hit_miss(t_max);
 }
 
+static void trace_bvh2(const orc_bvh *b, const float *rays, uint64_t R, int mode, float *hits, uint32_t *stats);
+
 /* Trace R field-major rays against the Karras-layout BVH; mode 0 closest
  * hit, 1 any hit.  stats (optional) u32[5R]: node visits, internal child
  * hits, (unused), leaf tests, max stack depth. */
 void orc_trace_bvh2(const float *vertices, const uint32_t *leaf_ids, const uint32_t *node_meta,
                     const float *node_bounds, uint32_t root, const float *rays, uint64_t R, int mode,
                     float *hits, uint32_t *stats) {
-    orc_bvh b = {vertices, leaf_ids, node_meta, node_bounds, root, NULL, 0, 0};
+    orc_bvh b = {vertices, leaf_ids, node_meta, node_bounds, root, NULL, 0, 0, NULL};
+    trace_bvh2(&b, rays, R, mode, hits, stats);
+}
+
+/* The same near-first / unordered queries over a sphere BVH (bajo's
+ * GpuSphereBvh, bvh/gpu/sphere_bvh.mojo:372-478): spheres f32[4n] in input
+ * order, the binary layout of the box build over their bounds. */
+void orc_trace_bvh2_spheres(const float *spheres, const uint32_t *leaf_ids, const uint32_t *node_meta,
+                            const float *node_bounds, uint32_t root, const float *rays, uint64_t R, int mode,
+                            float *hits, uint32_t *stats) {
+    orc_bvh b = {NULL, leaf_ids, node_meta, node_bounds, root, NULL, 0, 0, spheres};
+    trace_bvh2(&b, rays, R, mode, hits, stats);
+}
+
+/* Closest hit over every sphere in index order (strict '<': the lowest
+ * index wins exact ties), t_min as the ray carries it. */
+void orc_trace_brute_spheres(const float *spheres, uint64_t n, const float *rays, uint64_t R, float *hits) {
+    for (uint64_t r = 0; r < R; ++r) {
+        orc_v3 o = orc_v(rays[0 * R + r], rays[1 * R + r], rays[2 * R + r]);
+        orc_v3 d = orc_v(rays[4 * R + r], rays[5 * R + r], rays[6 * R + r]);
+        orc_hit hit = hit_miss(rays[7 * R + r]);
+        for (uint64_t i = 0; i < n; ++i) {
+            float t;
+            if (orc_sphere_t(spheres + 4 * i, o, d, rays[3 * R + r], hit.t, &t))
+                orc_sphere_record(spheres + 4 * i, (uint32_t)i, o, d, t, &hit);
+        }
+        store_hit(hits, r, hit);
+    }
+}
+
+static void trace_bvh2(const orc_bvh *bp, const float *rays, uint64_t R, int mode, float *hits, uint32_t *stats) {
+    orc_bvh b = *bp;
     for (uint64_t r = 0; r < R; ++r) {
         orc_v3 o = orc_v(rays[0 * R + r], rays[1 * R + r], rays[2 * R + r]);
         orc_v3 d = orc_v(rays[4 * R + r], rays[5 * R + r], rays[6 * R + r]);
@@ -263,7 +307,7 @@ typedef struct {
 
 static void *trace_worker(void *p) {
     trace_job *j = (trace_job *)p;
-    orc_bvh b = {j->vertices, j->leaf_ids, j->node_meta, j->node_bounds, j->root, NULL, 0, 0};
+    orc_bvh b = {j->vertices, j->leaf_ids, j->node_meta, j->node_bounds, j->root, NULL, 0, 0, NULL};
     uint64_t R = j->R;
     for (uint64_t r = j->first; r < j->last; ++r) {
         orc_v3 o = orc_v(j->rays[0 * R + r], j->rays[1 * R + r], j->rays[2 * R + r]);

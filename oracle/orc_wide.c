@@ -183,6 +183,13 @@ uint64_t orc_collapse_wide(const uint32_t *node_meta, const float *node_bounds, 
 
 static int wide_leaf_closest(const orc_bvh *b, uint32_t s, orc_v3 o, orc_v3 d, float t_min, orc_hit *hit) {
     uint32_t prim = b->leaf_ids[s];
+    if (b->spheres) {
+        const float *sp = b->spheres + 4 * (uint64_t)prim;
+        float t;
+        if (!orc_sphere_t(sp, o, d, t_min, hit->t, &t)) return 0;
+        orc_sphere_record(sp, prim, o, d, t, hit);
+        return 1;
+    }
     const float *t = b->vertices + 9 * (uint64_t)prim;
     orc_v3 v0 = orc_v(t[0], t[1], t[2]);
     orc_v3 e1 = orc_sub(orc_v(t[3], t[4], t[5]), v0);
@@ -201,6 +208,10 @@ static int wide_leaf_closest(const orc_bvh *b, uint32_t s, orc_v3 o, orc_v3 d, f
 
 static int wide_leaf_any(const orc_bvh *b, uint32_t s, orc_v3 o, orc_v3 d, float t_min, float t_max) {
     uint32_t prim = b->leaf_ids[s];
+    if (b->spheres) {
+        float t;
+        return orc_sphere_t(b->spheres + 4 * (uint64_t)prim, o, d, t_min, t_max, &t);
+    }
     const float *t = b->vertices + 9 * (uint64_t)prim;
     orc_v3 v0 = orc_v(t[0], t[1], t[2]);
     orc_v3 e1 = orc_sub(orc_v(t[3], t[4], t[5]), v0);
@@ -300,6 +311,8 @@ int orc_trace_wide_query(const orc_bvh *b, int any, orc_v3 o, orc_v3 d, float t_
 
 /* Field-major rays f32[8R] -> AoS hits f32[8R]; stats u32[5R] or NULL.
  * Returns the number of rays whose traversal overflowed the stack. */
+static uint64_t trace_wide(const orc_bvh *b, const float *rays, uint64_t R, int mode, float *hits, uint32_t *stats);
+
 uint64_t orc_trace_wide(const float *vertices, uint64_t n, const uint32_t *leaf_ids, const void *wide,
                         uint64_t wide_count, const float *rays, uint64_t R, int mode, float *hits, uint32_t *stats) {
     orc_bvh b;
@@ -309,6 +322,25 @@ uint64_t orc_trace_wide(const float *vertices, uint64_t n, const uint32_t *leaf_
     b.wide = wide;
     b.wide_count = wide_count;
     b.prim_count = n;
+    return trace_wide(&b, rays, R, mode, hits, stats);
+}
+
+/* The 8-wide queries over a sphere BVH (spheres f32[4n], input order). */
+uint64_t orc_trace_wide_spheres(const float *spheres, uint64_t n, const uint32_t *leaf_ids, const void *wide,
+                                uint64_t wide_count, const float *rays, uint64_t R, int mode, float *hits,
+                                uint32_t *stats) {
+    orc_bvh b;
+    memset(&b, 0, sizeof(b));
+    b.spheres = spheres;
+    b.leaf_ids = leaf_ids;
+    b.wide = wide;
+    b.wide_count = wide_count;
+    b.prim_count = n;
+    return trace_wide(&b, rays, R, mode, hits, stats);
+}
+
+static uint64_t trace_wide(const orc_bvh *bp, const float *rays, uint64_t R, int mode, float *hits, uint32_t *stats) {
+    orc_bvh b = *bp;
     uint64_t overflows = 0;
     for (uint64_t r = 0; r < R; ++r) {
         orc_v3 o = orc_v(rays[0 * R + r], rays[1 * R + r], rays[2 * R + r]);

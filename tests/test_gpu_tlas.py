@@ -140,5 +140,5 @@ def test_tlas_rejects_bad_arguments(gpu_ctx):
     with pytest.raises(BajoError, match="not invertible"):
         build_triangle_tlas([b], [Instance(singular, 0)])
     boxes = build_binary_bvh(to_dev(np.array([0, 0, 0, 1, 1, 1], np.float32)))
-    with pytest.raises(BajoError, match="not a triangle BVH"):
+    with pytest.raises(BajoError, match="not a triangle or sphere BVH"):
         build_triangle_tlas([boxes], [Instance(translation(), 0)])

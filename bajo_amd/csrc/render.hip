@@ -659,13 +659,22 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
   // A wave is as slow as its slowest path over all remaining bounces, so the
   // paths are spread over about one wave per SIMD with only `group` lanes
   // each (a power of two, at most 64) instead of packing 64 to a wave.
-  const uint32_t want = (active + simds - 1) / (simds ? simds : 1u);
-  uint32_t group = 1;
+  // Lanes pair up: the even lane carries the path (its extension rays), the
+  // odd lane traces the path's shadow ray of bounce b while the path traces
+  // its extension ray of bounce b + 1, which halves the serial chain.  The
+  // shadow contribution of bounce b is added to the path's sum before the
+  // extension contribution of bounce b + 1, the order of the wavefront.
+  const uint32_t want = (2u * active + simds - 1) / (simds ? simds : 1u);
+  uint32_t group = 2;
   while (group < want && group < 64) group <<= 1;
+  const uint32_t paths = group >> 1;
+  const bool is_path = (lane & 1u) == 0;
+  const uint32_t mate = lane ^ 1u;
   const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t gw = (uint32_t)(gtid >> 6); (uint64_t)gw * group < active; gw += waves) {
-    const uint32_t idx = gw * group + lane;
-    bool alive = lane < group && idx < active;
+  for (uint32_t gw = (uint32_t)(gtid >> 6); (uint64_t)gw * paths < active; gw += waves) {
+    const uint32_t idx = gw * paths + (lane >> 1);
+    const bool mine = lane < group && idx < active;
+    bool alive = mine && is_path;
     uint32_t path_id = 0;
     bool delta = false;
     float f[kPathPlanes];
@@ -681,67 +690,95 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
 #pragma unroll
       for (int p = 0; p < kSamplePlanes; ++p) acc[p] = q.sample[plane_at<kSamplePlanes>(slot, p)];
     }
-    for (uint32_t k = 0; k < nb; ++k) {
+    bool sh_live = false;  // shadow lane: a shadow ray of the previous bounce is pending
+    float sf[kShadowPlanes];
+#pragma unroll
+    for (int p = 0; p < kShadowPlanes; ++p) sf[p] = 0.0f;
+    // iteration k traces the extension rays of bounce + k and the shadow
+    // rays of bounce + k - 1; the last iteration only the pending shadows
+    for (uint32_t k = 0; k <= nb; ++k) {
       const uint32_t b = bounce + k;
-      if (b < kMaxBounces) wave_add(stats + 2 * b, alive ? 1u : 0u);
-      bool shadow = false;
+      const bool ext_step = k < nb;
+      if (!ext_step) alive = false;  // a path still going after the last bounce ends here
+      if (ext_step && b < kMaxBounces) wave_add(stats + 2 * b, alive ? 1u : 0u);
       uint32_t ext_max = 0, sh_max = 0;
-      if (alive) {
-        const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
+      // one traversal call for both lane kinds, so the path's extension ray
+      // and its mate's shadow ray step through the same instructions
+      Hit h;
+      Ray ray;
+      if (alive) ray = Ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
+      if (sh_live) ray = Ray{mk(sf[0], sf[1], sf[2]), mk(sf[4], sf[5], sf[6]), sf[3], sf[7]};
+      if (alive || sh_live) {
         TraceCounters tc;
-        const Hit h = trace_accel<BAJO_TRACE_CLOSEST_HIT, true, kWide>(sc.bvh, ray, st, &tc);
-        ext_visits += tc.visits;
-        ext_leaves += tc.leaves;
-        ext_max = tc.visits;
+        h = trace_accel<kTraceEither, true, kWide>(sc.bvh, ray, st, &tc, sh_live);
+        if (alive) {
+          ext_visits += tc.visits;
+          ext_leaves += tc.leaves;
+          ext_max = tc.visits;
+        } else {
+          sh_visits += tc.visits;
+          sh_leaves += tc.leaves;
+          sh_max = tc.visits;
+        }
+      }
+      bool add_s = false;
+      v3 cs = mk(0, 0, 0);
+      if (sh_live) {
+        const bool occluded = h.prim == kEmpty && h.t == 0.0f;
+        if (kAlg == BAJO_RENDER_AO || !occluded) {
+          cs = shadow_radiance<kAlg>(occluded, sf);
+          add_s = true;
+        }
+      }
+      // the previous bounce's shadow contribution, from the mate lane
+      const bool mate_add = __shfl(add_s, (int)mate, 64);
+      const float cx = __shfl(cs.x, (int)mate, 64), cy = __shfl(cs.y, (int)mate, 64), cz = __shfl(cs.z, (int)mate, 64);
+      if (mine && is_path && mate_add) {
+        acc[0] += cx;
+        acc[1] += cy;
+        acc[2] += cz;
+      }
+      bool cast = false;
+      float nsf[kShadowPlanes];
+#pragma unroll
+      for (int p = 0; p < kShadowPlanes; ++p) nsf[p] = 0.0f;
+      if (alive) {
         const v3 T = mk(f[8], f[9], f[10]);
         const Surface su = surface_of(sc, ray, h);
         ShadowWork sw;
-        bool add_s = false;
-        v3 cs = mk(0, 0, 0);
         if (shade_shadow<kAlg>(sc, seed, b, path_id, T, ray.d, su, sw)) {
-          shadow = true;
-          const Ray sray{mk(sw.f[0], sw.f[1], sw.f[2]), mk(sw.f[4], sw.f[5], sw.f[6]), sw.f[3], sw.f[7]};
-          TraceCounters tcs;
-          const Hit sh = trace_accel<BAJO_TRACE_ANY_HIT, true, kWide>(sc.bvh, sray, st, &tcs);
-          sh_visits += tcs.visits;
-          sh_leaves += tcs.leaves;
-          sh_max = tcs.visits;
-          const bool occluded = sh.prim == kEmpty && sh.t == 0.0f;
-          if (kAlg == BAJO_RENDER_AO || !occluded) {
-            cs = shadow_radiance<kAlg>(occluded, sw.f);
-            add_s = true;
-          }
+          cast = true;
+#pragma unroll
+          for (int p = 0; p < kShadowPlanes; ++p) nsf[p] = sw.f[p];
         }
         v3 add_c;
         NextPath np;
-        const uint32_t sf = shade_bounce<kAlg>(sc, seed, b, path_id, delta, ray, T, f[11], h.t, su, add_c, np);
-        const bool want_next = (sf & kShadeNext) != 0;
-        // the same order as the wavefront: the extension's radiance, then the shadow ray's
-        if (sf & kShadeAdd) {
+        const uint32_t sfl = shade_bounce<kAlg>(sc, seed, b, path_id, delta, ray, T, f[11], h.t, su, add_c, np);
+        if (sfl & kShadeAdd) {
           acc[0] += add_c.x;
           acc[1] += add_c.y;
           acc[2] += add_c.z;
         }
-        if (add_s) {
-          acc[0] += cs.x;
-          acc[1] += cs.y;
-          acc[2] += cs.z;
-        }
-        alive = want_next;
+        alive = (sfl & kShadeNext) != 0;
         if (alive) {
 #pragma unroll
           for (int p = 0; p < kPathPlanes; ++p) f[p] = np.f[p];
           delta = (np.id & kPathDelta) != 0;
         }
       }
-      if (b < kMaxBounces) {
-        wave_add(stats + 2 * b + 1, shadow ? 1u : 0u);
+      // the new shadow ray moves to the mate lane
+      const bool mate_cast = __shfl(cast, (int)mate, 64);
+#pragma unroll
+      for (int p = 0; p < kShadowPlanes; ++p) sf[p] = __shfl(nsf[p], (int)mate, 64);
+      sh_live = mine && !is_path && mate_cast;
+      if (ext_step && b < kMaxBounces) {
+        wave_add(stats + 2 * b + 1, cast ? 1u : 0u);
         wave_max(stats + kMaxVisitBase + 2 * b, ext_max);
-        wave_max(stats + kMaxVisitBase + 2 * b + 1, sh_max);
       }
-      if (!ballot(alive)) break;
+      if (k > 0 && b - 1 < kMaxBounces) wave_max(stats + kMaxVisitBase + 2 * (b - 1) + 1, sh_max);
+      if (!ballot(alive || sh_live)) break;
     }
-    if (lane < group && idx < active) {
+    if (mine && is_path) {
 #pragma unroll
       for (int p = 0; p < kSamplePlanes; ++p) q.sample[plane_at<kSamplePlanes>(slot, p)] = acc[p];
     }

@@ -17,6 +17,10 @@
 namespace bj {
 
 constexpr int kTraceBlock = 256;
+// A third traversal mode chosen per lane at run time (closest or any hit), so
+// lanes tracing either kind of ray step through one instruction stream: the
+// near-first closest-hit walk, ended at the first hit for an any-hit lane.
+constexpr int kTraceEither = 2;
 constexpr int kLdsStack = 24;
 constexpr int kMaxStack = 64;
 constexpr int kSpill = kMaxStack - kLdsStack;
@@ -218,7 +222,7 @@ struct TraceCounters {
 // Closest hit (kMode == BAJO_TRACE_CLOSEST_HIT) or occlusion (ANY_HIT: the
 // returned record is bajo's Hit.shadow_hit() or Hit.miss(t_max)).
 template <int kMode, bool kStats, bool kSph = false>
-BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* tc) {
+BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* tc, bool any_rt = false) {
   uint32_t n_visit = 0, n_inner = 0, n_leaf = 0, max_sp = 0;
   const BjNode* __restrict__ nodes = bv.nodes;
   const BjTri* __restrict__ tris = bv.tris;
@@ -232,7 +236,7 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
   if ((root & kLeafFlag) && root != kSentinel) {
     n_leaf++;
     if constexpr (kMode == BAJO_TRACE_ANY_HIT) occluded = leaf_any_p<kSph>(tris, root & kIndexMask, ray);
-    else leaf_closest_p<kSph>(tris, root & kIndexMask, ray, best);
+    else if (leaf_closest_p<kSph>(tris, root & kIndexMask, ray, best) && kMode == kTraceEither && any_rt) occluded = true;
   } else if (root != kSentinel) {
     uint32_t cur = root;
     for (;;) {
@@ -275,12 +279,22 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
       } else {
         bool v0 = false, v1 = false;
         if (h0) {
-          if (l0) { n_leaf++; leaf_closest_p<kSph>(tris, c0 & kIndexMask, ray, best); }
-          else if (!(c0 & kLeafFlag)) { v0 = true; n_inner++; }
+          if (l0) {
+            n_leaf++;
+            if (leaf_closest_p<kSph>(tris, c0 & kIndexMask, ray, best) && kMode == kTraceEither && any_rt) {
+              occluded = true;
+              break;
+            }
+          } else if (!(c0 & kLeafFlag)) { v0 = true; n_inner++; }
         }
         if (h1) {
-          if (l1) { n_leaf++; leaf_closest_p<kSph>(tris, c1 & kIndexMask, ray, best); }
-          else if (!(c1 & kLeafFlag)) { v1 = true; n_inner++; }
+          if (l1) {
+            n_leaf++;
+            if (leaf_closest_p<kSph>(tris, c1 & kIndexMask, ray, best) && kMode == kTraceEither && any_rt) {
+              occluded = true;
+              break;
+            }
+          } else if (!(c1 & kLeafFlag)) { v1 = true; n_inner++; }
         }
         // bvh/gpu/trace.mojo:395-426: nearest child first, push the other.
         int near = -1;
@@ -303,7 +317,7 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
     }
   }
   Hit hit;
-  if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
+  if (kMode == BAJO_TRACE_ANY_HIT || (kMode == kTraceEither && any_rt)) {
     // bvh/gpu/trace.mojo:44-59: occlusion is reported as Hit.shadow_hit().
     hit = occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
   } else {
@@ -366,12 +380,13 @@ template <int kMode, bool kStats, bool kSph = false>
 struct Walk8 {
   Ray ray;
   v3 rcp;
-  bool px, py, pz, occluded;
+  bool px, py, pz, occluded, any_rt;
   Closest best;
   uint32_t cur, n_visit, n_inner, n_leaf, max_sp;
 
-  BJ_D void begin(const WideView& wv, const Ray& r, Stack& st) {
+  BJ_D void begin(const WideView& wv, const Ray& r, Stack& st, bool any = false) {
     ray = r;
+    any_rt = any;
     best = no_closest(r.tmax);
     rcp = rcp_dir(r.d);
     px = rcp.x >= 0.0f;
@@ -452,17 +467,24 @@ struct Walk8 {
           occluded = true;
           break;
         }
+      } else if constexpr (kMode == kTraceEither) {
+        // best.t stays at t_max on an any-hit lane, so the closest-hit test is its occlusion test
+        if (leaf_closest_p<kSph>(wv.tris, tri, ray, best) && any_rt) {
+          occluded = true;
+          break;
+        }
       } else {
         leaf_closest_p<kSph>(wv.tris, tri, ray, best);
       }
       ++tri;
     }
-    if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
+    if constexpr (kMode == BAJO_TRACE_ANY_HIT || kMode == kTraceEither) {
       if (occluded) {
         cur = kSentinel;
         return;
       }
-    } else {
+    }
+    if constexpr (kMode != BAJO_TRACE_ANY_HIT) {
       // children entered beyond the (possibly improved) closest hit are dropped
 #pragma unroll
       for (int i = 0; i < 8; ++i)
@@ -504,7 +526,7 @@ struct Walk8 {
 
   BJ_D Hit finish(const WideView& wv, TraceCounters* tc) const {
     Hit hit;
-    if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
+    if (kMode == BAJO_TRACE_ANY_HIT || (kMode == kTraceEither && any_rt)) {
       hit = occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
     } else {
       hit = closest_hit_p<kSph>(wv.tris, best, ray);
@@ -520,9 +542,9 @@ struct Walk8 {
 };
 
 template <int kMode, bool kStats, bool kSph = false>
-BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters* tc) {
+BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters* tc, bool any_rt = false) {
   Walk8<kMode, kStats, kSph> w;
-  w.begin(wv, ray, st);
+  w.begin(wv, ray, st, any_rt);
   while (!w.done()) w.step(wv, st);
   return w.finish(wv, tc);
 }
@@ -556,9 +578,9 @@ inline AccelView accel_view(const bajo_bvh* h, uint32_t* status) {
 }
 
 template <int kMode, bool kStats, bool kWide, bool kSph = false>
-BJ_D Hit trace_accel(const AccelView& av, const Ray& ray, Stack& st, TraceCounters* tc) {
-  if constexpr (kWide) return traverse8<kMode, kStats, kSph>(av.wide, ray, st, tc);
-  else return traverse<kMode, kStats, kSph>(av.bin, ray, st, tc);
+BJ_D Hit trace_accel(const AccelView& av, const Ray& ray, Stack& st, TraceCounters* tc, bool any_rt = false) {
+  if constexpr (kWide) return traverse8<kMode, kStats, kSph>(av.wide, ray, st, tc, any_rt);
+  else return traverse<kMode, kStats, kSph>(av.bin, ray, st, tc, any_rt);
 }
 
 }  // namespace bj

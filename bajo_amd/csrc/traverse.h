@@ -191,6 +191,62 @@ BJ_D Hit closest_hit_p(const BjTri* __restrict__ recs, const Closest& c, const R
   }
 }
 
+// A leaf record held in registers (triangle: v0 + prim, e1, e2; sphere:
+// center + prim, radius), so a walk can load the next record before it
+// tests the current one.
+struct LeafRec {
+  float4 a, b, c;
+};
+
+template <bool kSph>
+BJ_D LeafRec load_rec(const BjTri* __restrict__ recs, uint32_t s) {
+  const float4* p = reinterpret_cast<const float4*>(recs + s);
+  LeafRec r;
+  r.a = p[0];
+  r.b = p[1];
+  if constexpr (kSph) r.c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  else r.c = p[2];
+  return r;
+}
+
+template <bool kSph>
+BJ_D bool rec_closest(const LeafRec& rec, uint32_t s, const Ray& r, Closest& c) {
+  if constexpr (kSph) {
+    const SphData sp{mk(rec.a.x, rec.a.y, rec.a.z), rec.b.x, __builtin_bit_cast(uint32_t, rec.a.w)};
+    float t;
+    if (!sphere_t(r, sp, c.t, t)) return false;
+    c.t = t;
+    c.s = s;
+    return true;
+  } else {
+    const TriData tr{mk(rec.a.x, rec.a.y, rec.a.z), mk(rec.b.x, rec.b.y, rec.b.z), mk(rec.c.x, rec.c.y, rec.c.z),
+                     __builtin_bit_cast(uint32_t, rec.a.w)};
+    float ad, ts, us, vs;
+    if (!tri_scaled(r, tr, c.t, ad, ts, us, vs)) return false;
+    const float inv = 1.0f / ad;
+    c.t = ts * inv;
+    c.us = us;
+    c.vs = vs;
+    c.inv = inv;
+    c.s = s;
+    return true;
+  }
+}
+
+template <bool kSph>
+BJ_D bool rec_any(const LeafRec& rec, const Ray& r) {
+  if constexpr (kSph) {
+    const SphData sp{mk(rec.a.x, rec.a.y, rec.a.z), rec.b.x, __builtin_bit_cast(uint32_t, rec.a.w)};
+    float t;
+    return sphere_t(r, sp, r.tmax, t);
+  } else {
+    const TriData tr{mk(rec.a.x, rec.a.y, rec.a.z), mk(rec.b.x, rec.b.y, rec.b.z), mk(rec.c.x, rec.c.y, rec.c.z),
+                     __builtin_bit_cast(uint32_t, rec.a.w)};
+    float ad, ts, us, vs;
+    return tri_scaled(r, tr, r.tmax, ad, ts, us, vs);
+  }
+}
+
 struct Stack {
   uint32_t* lds;       // this lane's column: entry k at lds[k * kTraceBlock]
   uint32_t* spill;     // this thread's column: entry k at spill[k * spill_stride]
@@ -448,35 +504,49 @@ struct Walk8 {
     const uint32_t leaf_bits = (c0 >> 31) | ((c1 >> 30) & 2u) | ((c2 >> 29) & 4u) | ((c3 >> 28) & 8u) |
                                ((c4 >> 27) & 16u) | ((c5 >> 26) & 32u) | ((c6 >> 25) & 64u) | ((c7 >> 24) & 128u);
     uint32_t leaf_mask = hit_mask & leaf_bits, inner_mask = hit_mask & ~leaf_bits;
-    // leaf children: one triangle per iteration across all hit leaves
+    // leaf children: one record per iteration across all hit leaves, in
+    // slot order; the next record's load is issued before the current one
+    // is tested, so a leaf's records arrive back to back
     uint32_t tri = 0, tri_end = 0;
-    while (true) {
-      if (tri == tri_end) {
-        if (!leaf_mask) break;
+    auto advance = [&](uint32_t& t, uint32_t& e) -> bool {
+      while (t >= e) {
+        if (!leaf_mask) return false;
         const uint32_t bit = leaf_mask & (0u - leaf_mask);
         leaf_mask ^= bit;
         const uint32_t ref = sel8(bit, c0, c1, c2, c3, c4, c5, c6, c7);
-        tri = ref & kWideLeafStartMask;
-        tri_end = tri + ((ref >> 28) & 7u) + 1u;
-        if (tri_end > wv.prim_count) tri_end = wv.prim_count;
-        if (tri >= tri_end) continue;
+        t = ref & kWideLeafStartMask;
+        e = t + ((ref >> 28) & 7u) + 1u;
+        if (e > wv.prim_count) e = wv.prim_count;
       }
+      return true;
+    };
+    bool have = advance(tri, tri_end);
+    LeafRec cur_rec;
+    if (have) cur_rec = load_rec<kSph>(wv.tris, tri);
+    while (have) {
+      uint32_t nt = tri + 1, ne = tri_end;
+      const bool nhave = advance(nt, ne);
+      LeafRec next_rec;
+      if (nhave) next_rec = load_rec<kSph>(wv.tris, nt);
       n_leaf++;
       if constexpr (kMode == BAJO_TRACE_ANY_HIT) {
-        if (leaf_any_p<kSph>(wv.tris, tri, ray)) {
+        if (rec_any<kSph>(cur_rec, ray)) {
           occluded = true;
           break;
         }
       } else if constexpr (kMode == kTraceEither) {
         // best.t stays at t_max on an any-hit lane, so the closest-hit test is its occlusion test
-        if (leaf_closest_p<kSph>(wv.tris, tri, ray, best) && any_rt) {
+        if (rec_closest<kSph>(cur_rec, tri, ray, best) && any_rt) {
           occluded = true;
           break;
         }
       } else {
-        leaf_closest_p<kSph>(wv.tris, tri, ray, best);
+        rec_closest<kSph>(cur_rec, tri, ray, best);
       }
-      ++tri;
+      tri = nt;
+      tri_end = ne;
+      have = nhave;
+      cur_rec = next_rec;
     }
     if constexpr (kMode == BAJO_TRACE_ANY_HIT || kMode == kTraceEither) {
       if (occluded) {

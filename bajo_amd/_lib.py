@@ -199,6 +199,11 @@ SIGNATURES = {
     "bajo_scene_create_materials": (
         c_int32, [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, POINTER(BuildOptions), POINTER(c_void_p)],
     ),
+    "bajo_scene_create_mixed": (
+        c_int32,
+        [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, POINTER(BuildOptions),
+         POINTER(c_void_p)],
+    ),
     "bajo_scene_create_ex": (
         c_int32,
         [

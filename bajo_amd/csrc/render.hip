@@ -32,7 +32,8 @@ constexpr int kRtBlock = kTraceBlock;
 // MAT kinds (rt/types.mojo:24-29)
 constexpr uint32_t kMatLambertian = 0, kMatMetal = 1, kMatDielectric = 2, kMatEmissive = 3;
 constexpr uint32_t kSurfaceIndexMask = (1u << 28) - 1;
-constexpr uint32_t kPrimTriangle = 1;
+constexpr uint32_t kPrimSphere = 0, kPrimTriangle = 1;  // bajo's PRIM values
+constexpr uint32_t kSphereHit = 1u;  // Hit.inst of a sphere hit in a mixed scene
 constexpr uint32_t kPathDelta = 0x80000000u, kPathIdMask = 0x7FFFFFFFu;
 constexpr int kLightStride = 14;
 constexpr int kPathPlanes = 12, kShadowPlanes = 11, kSamplePlanes = 3;
@@ -54,6 +55,9 @@ constexpr int kStatsWords = kMaxVisitBase + 2 * kMaxBounces;
 
 struct SceneView {
   AccelView bvh;
+  AccelView spheres;              // sphere BVH of a mixed scene (radii |r|)
+  const uint32_t* sphere_surfaces;
+  const float* sphere_radii;      // signed: r < 0 turns the normal inward
   const uint32_t* surfaces;
   const float* lambertian;
   const float* metals;       // albedo.xyz, fuzz per metal
@@ -277,17 +281,29 @@ BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uin
   const uint32_t packed = sc.light_kinds[sel];
   if (frac > sc.light_fields[sel * kLightStride + 13]) sel = packed >> 4;
   const float* L = sc.light_fields + (uint64_t)sel * kLightStride;
-  const v3 p0 = mk(L[0], L[1], L[2]), p1 = mk(L[3], L[4], L[5]), p2 = mk(L[6], L[7], L[8]);
-  const v3 e1 = sub(p1, p0), e2 = sub(p2, p0);
-  const v3 av = cross(e1, e2);
-  const float tas = length2(av);
-  if (tas <= 0.0f) return false;
-  const float sq = sqrtf(tas);
-  const v3 ln = mk(av.x / sq, av.y / sq, av.z / sq);
-  const float root_u = sqrtf(rng.next());
-  const float b1 = root_u * (1.0f - rng.next());
-  const float b2 = root_u - b1;
-  const v3 lp = add(add(p0, scale(e1, b1)), scale(e2, b2));
+  const v3 p0 = mk(L[0], L[1], L[2]);
+  v3 ln, lp;
+  if ((sc.light_kinds[sel] & 0xFu) == kPrimSphere) {
+    // a sphere light: random_unit_vector (core/random.mojo:39-45) as the light normal
+    float s, c;
+    sincos_turns(rng.next(), s, c);
+    const float z = 1.0f - 2.0f * rng.next();
+    const float r = sqrtf(1.0f - z * z);
+    ln = mk(r * c, r * s, z);
+    lp = add(p0, scale(ln, L[9]));
+  } else {
+    const v3 p1 = mk(L[3], L[4], L[5]), p2 = mk(L[6], L[7], L[8]);
+    const v3 e1 = sub(p1, p0), e2 = sub(p2, p0);
+    const v3 av = cross(e1, e2);
+    const float tas = length2(av);
+    if (tas <= 0.0f) return false;
+    const float sq = sqrtf(tas);
+    ln = mk(av.x / sq, av.y / sq, av.z / sq);
+    const float root_u = sqrtf(rng.next());
+    const float b1 = root_u * (1.0f - rng.next());
+    const float b2 = root_u - b1;
+    lp = add(add(p0, scale(e1, b1)), scale(e2, b2));
+  }
   const v3 to_light = sub(lp, point);
   const float d2 = length2(to_light);
   if (d2 <= 1.0e-8f) return false;
@@ -362,10 +378,13 @@ BJ_D Surface surface_of(const SceneView& sc, const Ray& ray, const Hit& h) {
   su.point = mk(0, 0, 0);
   su.kind = su.mi = 0;
   if (su.found) {
-    su.front = dot(ray.d, h.n) < 0.0f;
-    su.n = su.front ? h.n : mk(-h.n.x, -h.n.y, -h.n.z);
+    // rt/gpu/scene_trace.mojo:137-145: a sphere's outward normal, inward for a negative radius
+    const bool sph = h.inst == kSphereHit;
+    const v3 hn = sph && sc.sphere_radii[h.prim] < 0.0f ? mk(-h.n.x, -h.n.y, -h.n.z) : h.n;
+    su.front = dot(ray.d, hn) < 0.0f;
+    su.n = su.front ? hn : mk(-hn.x, -hn.y, -hn.z);
     su.point = add(ray.o, scale(ray.d, h.t));
-    const uint32_t surf = sc.surfaces[h.prim];
+    const uint32_t surf = sph ? sc.sphere_surfaces[h.prim] : sc.surfaces[h.prim];
     su.kind = surf >> 28;
     su.mi = surf & kSurfaceIndexMask;
   }
@@ -536,7 +555,37 @@ BJ_D v3 shadow_radiance(bool occluded, const float* f) {
 // kFirst: bounce 0 of a chunk, whose paths are the camera paths of the
 // chunk's path ids (generated here instead of read from a queue); it also
 // clears the chunk's sample accumulators.
-template <int kAlg, bool kWide, bool kFirst>
+// The scene's closest / any hit (rt/gpu/scene_trace.mojo:113-193): a mixed
+// scene traces its spheres first, then its triangles with t_max = the sphere
+// hit (a triangle must be strictly closer); any hit: either occludes.
+template <int kMode, bool kWide, bool kMixed>
+BJ_D Hit scene_trace(const SceneView& sc, const Ray& ray, Stack& st, TraceCounters* tc, bool any_rt = false) {
+  if constexpr (!kMixed) {
+    return trace_accel<kMode, true, kWide>(sc.bvh, ray, st, tc, any_rt);
+  } else {
+    const bool any = kMode == BAJO_TRACE_ANY_HIT || (kMode == kTraceEither && any_rt);
+    TraceCounters ts, tt;
+    const Hit hs = trace_accel<kMode, true, kWide, true>(sc.spheres, ray, st, &ts, any_rt);
+    const bool sph = any ? (hs.prim == kEmpty && hs.t == 0.0f) : hs.prim != kEmpty;
+    Hit h = hs;
+    tt.visits = tt.inner = tt.leaves = tt.max_sp = 0;
+    if (!(any && sph)) {
+      Ray tr = ray;
+      if (sph) tr.tmax = hs.t;
+      const Hit ht = trace_accel<kMode, true, kWide>(sc.bvh, tr, st, &tt, any_rt);
+      if (any) h = ht;
+      else if (ht.prim != kEmpty && ht.t < tr.tmax) h = ht;
+      else if (sph) h.inst = kSphereHit;
+    }
+    tc->visits = ts.visits + tt.visits;
+    tc->inner = ts.inner + tt.inner;
+    tc->leaves = ts.leaves + tt.leaves;
+    tc->max_sp = ts.max_sp > tt.max_sp ? ts.max_sp : tt.max_sp;
+    return h;
+  }
+}
+
+template <int kAlg, bool kWide, bool kFirst, bool kMixed>
 __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                           PrimaryArgs pa, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
@@ -580,7 +629,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
       }
       ray = Ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
       TraceCounters tc;
-      const Hit h = trace_accel<BAJO_TRACE_CLOSEST_HIT, true, kWide>(sc.bvh, ray, st, &tc);
+      const Hit h = scene_trace<BAJO_TRACE_CLOSEST_HIT, kWide, kMixed>(sc, ray, st, &tc);
       work_visits += tc.visits;
       work_leaves += tc.leaves;
       // throughput, pdf and id are read after the traversal rather than
@@ -639,7 +688,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
 // blocks of a launch take the same decision from the queue length; a
 // launch that runs marks the chunk done (tag = bounce + 1) so later extend,
 // shadow, advance and tail launches of the chunk do nothing.
-template <int kAlg, bool kWide>
+template <int kAlg, bool kWide, bool kMixed>
 __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                         uint32_t depth, uint32_t threshold, uint32_t simds,
                                                         unsigned long long* __restrict__ stats,
@@ -710,7 +759,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
       if (sh_live) ray = Ray{mk(sf[0], sf[1], sf[2]), mk(sf[4], sf[5], sf[6]), sf[3], sf[7]};
       if (alive || sh_live) {
         TraceCounters tc;
-        h = trace_accel<kTraceEither, true, kWide>(sc.bvh, ray, st, &tc, sh_live);
+        h = scene_trace<kTraceEither, kWide, kMixed>(sc, ray, st, &tc, sh_live);
         if (alive) {
           ext_visits += tc.visits;
           ext_leaves += tc.leaves;
@@ -789,7 +838,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
   wave_add(stats + kWorkBase + 3, (uint32_t)sh_leaves);
 }
 
-template <int kAlg, bool kWide>
+template <int kAlg, bool kWide, bool kMixed>
 __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t bounce,
                                                           uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
@@ -810,7 +859,7 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
     const uint32_t slot = q.shadow_ids[idx] - q.sample_base;
     const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
     TraceCounters tc;
-    const Hit h = trace_accel<BAJO_TRACE_ANY_HIT, true, kWide>(sc.bvh, ray, st, &tc);
+    const Hit h = scene_trace<BAJO_TRACE_ANY_HIT, kWide, kMixed>(sc, ray, st, &tc);
     work_visits += tc.visits;
     work_leaves += tc.leaves;
     const bool occluded = h.prim == kEmpty && h.t == 0.0f;
@@ -875,6 +924,13 @@ struct bajo_scene {
   uint32_t light_count = 0;
   float total_light_weight = 0.0f;
   RenderArena arena;  // reused by bajo_render
+  // spheres of a mixed scene: center + |r| (the BVH's input), signed radii,
+  // surface ids, and their BVH
+  uint64_t n_spheres = 0;
+  float* spheres = nullptr;
+  float* sphere_radii = nullptr;
+  uint32_t* sphere_surfaces = nullptr;
+  bajo_bvh* sphere_bvh = nullptr;
 };
 
 namespace {
@@ -891,10 +947,20 @@ void free_scene(bajo_scene* s) {
   if (!s) return;
   free_arena(s->arena);
   bajo_bvh_free(s->bvh);
+  bajo_bvh_free(s->sphere_bvh);
+  (void)hipFree(s->spheres); (void)hipFree(s->sphere_radii); (void)hipFree(s->sphere_surfaces);
   (void)hipFree(s->vertices); (void)hipFree(s->surfaces); (void)hipFree(s->lambertian); (void)hipFree(s->emissive);
   (void)hipFree(s->metals); (void)hipFree(s->dielectrics);
   (void)hipFree(s->light_fields); (void)hipFree(s->light_kinds);
   delete s;
+}
+
+// A view with no hierarchy: every query misses at once.
+AccelView empty_view(uint32_t* status) {
+  AccelView av;
+  av.bin = BvhView{nullptr, nullptr, kSentinel, 0u, 0u, status};
+  av.wide = WideView{nullptr, nullptr, 0u, 0u, status, 0u};
+  return av;
 }
 
 template <typename K>
@@ -994,7 +1060,7 @@ struct RenderShape {
   uint64_t spill_threads = 0;
 };
 
-template <int kAlg, bool kWide>
+template <int kAlg, bool kWide, bool kMixed>
 int32_t render_shape(const bajo_render_settings* st, RenderShape* out) {
   const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
   uint64_t cap = st->path_capacity ? st->path_capacity : total;
@@ -1004,7 +1070,7 @@ int32_t render_shape(const bajo_render_settings* st, RenderShape* out) {
   constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
   unsigned g_tail = 0, simds = 0;
   if constexpr (kTail) {
-    int32_t rc = grid_of(tail_kernel<kAlg, kWide>, &g_tail);
+    int32_t rc = grid_of(tail_kernel<kAlg, kWide, kMixed>, &g_tail);
     if (rc) return rc;
     int dev = 0, cus = 0;
     BJ_HIP(hipGetDevice(&dev));
@@ -1026,12 +1092,12 @@ int32_t render_shape(const bajo_render_settings* st, RenderShape* out) {
 // Enqueues one render into arena `a` (no host synchronization without `ev`).
 // `camera` (host, 20 floats) is written into the arena's camera buffer first;
 // NULL renders with the camera the buffer already holds.
-template <int kAlg, bool kWide>
+template <int kAlg, bool kWide, bool kMixed>
 int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st, const float* camera,
                    float* pixels, LaunchEvents* ev) {
   const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
   RenderShape shape;
-  int32_t rc = render_shape<kAlg, kWide>(st, &shape);
+  int32_t rc = render_shape<kAlg, kWide, kMixed>(st, &shape);
   if (rc) return rc;
   constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
   const uint64_t cap = shape.cap;
@@ -1053,7 +1119,10 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   pa.height = st->image_height;
   pa.spp = st->samples_per_pixel;
   SceneView sv;
-  sv.bvh = accel_view(s->bvh, a.counters + 8);
+  sv.bvh = s->bvh ? accel_view(s->bvh, a.counters + 8) : empty_view(a.counters + 8);
+  sv.spheres = s->sphere_bvh ? accel_view(s->sphere_bvh, a.counters + 8) : empty_view(a.counters + 8);
+  sv.sphere_surfaces = s->sphere_surfaces;
+  sv.sphere_radii = s->sphere_radii;
   sv.surfaces = s->surfaces;
   sv.lambertian = s->lambertian;
   sv.metals = s->metals;
@@ -1074,22 +1143,22 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
       if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if constexpr (kTail) {
         if (b > 0) {
-          tail_kernel<kAlg, kWide><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, tail_threshold,
+          tail_kernel<kAlg, kWide, kMixed><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, tail_threshold,
                                                                     simds, a.stats, a.spill);
           BJ_HIP(hipGetLastError());
         }
       }
       if (b == 0)
-        extend_kernel<kAlg, kWide, true><<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(
+        extend_kernel<kAlg, kWide, true, kMixed><<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(
             sv, q, st->rng_seed, b, pa, a.spill, a.work_ext, a.stats);
       else
-        extend_kernel<kAlg, kWide, false><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, a.spill,
+        extend_kernel<kAlg, kWide, false, kMixed><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, a.spill,
                                                                           a.work_ext, a.stats);
       BJ_HIP(hipGetLastError());
       if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
-        shadow_kernel<kAlg, kWide><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats);
+        shadow_kernel<kAlg, kWide, kMixed><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats);
         BJ_HIP(hipGetLastError());
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
       }
@@ -1105,45 +1174,42 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   return BAJO_OK;
 }
 
+// Instantiates `fn` for the scene's node width and geometry (triangles only,
+// or spheres + triangles).
+template <int kAlg, typename F>
+int32_t by_scene(const bajo_scene* s, F&& fn) {
+  const bool wide = s->options.node_width == BAJO_WIDTH_WIDE8, mixed = s->n_spheres > 0;
+  if (mixed) return wide ? fn.template operator()<kAlg, true, true>() : fn.template operator()<kAlg, false, true>();
+  return wide ? fn.template operator()<kAlg, true, false>() : fn.template operator()<kAlg, false, false>();
+}
+
+template <typename F>
+int32_t by_algorithm(const bajo_scene* s, int32_t algorithm, F&& fn) {
+  switch (algorithm) {
+    case BAJO_RENDER_PATH: return by_scene<BAJO_RENDER_PATH>(s, fn);
+    case BAJO_RENDER_NORMALS: return by_scene<BAJO_RENDER_NORMALS>(s, fn);
+    case BAJO_RENDER_AO: return by_scene<BAJO_RENDER_AO>(s, fn);
+    case BAJO_RENDER_NEE: return by_scene<BAJO_RENDER_NEE>(s, fn);
+    case BAJO_RENDER_MIS: return by_scene<BAJO_RENDER_MIS>(s, fn);
+    default: BJ_REQUIRE(false, "bajo_render: unknown algorithm %d", algorithm);
+  }
+}
+
 // Sizes arena `a` for a render of `st` over scene `s` without enqueueing it.
 int32_t prepare_arena(const bajo_scene* s, RenderArena& a, const bajo_render_settings* st) {
-  const bool wide = s->bvh->node_width == BAJO_WIDTH_WIDE8;
   RenderShape shape;
-  int32_t rc;
-  switch (st->algorithm) {
-    case BAJO_RENDER_PATH: rc = wide ? render_shape<BAJO_RENDER_PATH, true>(st, &shape) : render_shape<BAJO_RENDER_PATH, false>(st, &shape); break;
-    case BAJO_RENDER_NORMALS: rc = wide ? render_shape<BAJO_RENDER_NORMALS, true>(st, &shape) : render_shape<BAJO_RENDER_NORMALS, false>(st, &shape); break;
-    case BAJO_RENDER_AO: rc = wide ? render_shape<BAJO_RENDER_AO, true>(st, &shape) : render_shape<BAJO_RENDER_AO, false>(st, &shape); break;
-    case BAJO_RENDER_NEE: rc = wide ? render_shape<BAJO_RENDER_NEE, true>(st, &shape) : render_shape<BAJO_RENDER_NEE, false>(st, &shape); break;
-    case BAJO_RENDER_MIS: rc = wide ? render_shape<BAJO_RENDER_MIS, true>(st, &shape) : render_shape<BAJO_RENDER_MIS, false>(st, &shape); break;
-    default: BJ_REQUIRE(false, "bajo_render: unknown algorithm %d", st->algorithm);
-  }
+  const int32_t rc = by_algorithm(s, st->algorithm, [&]<int kAlg, bool kWide, bool kMixed>() {
+    return render_shape<kAlg, kWide, kMixed>(st, &shape);
+  });
   if (rc) return rc;
   return ensure_arena(a, shape.cap, shape.spill_threads);
 }
 
 int32_t dispatch_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st,
                         const float* camera, float* pixels, LaunchEvents* ev) {
-  const bool wide = s->bvh->node_width == BAJO_WIDTH_WIDE8;
-  switch (st->algorithm) {
-    case BAJO_RENDER_PATH:
-      return wide ? run_render<BAJO_RENDER_PATH, true>(ctx, s, a, st, camera, pixels, ev)
-                  : run_render<BAJO_RENDER_PATH, false>(ctx, s, a, st, camera, pixels, ev);
-    case BAJO_RENDER_NORMALS:
-      return wide ? run_render<BAJO_RENDER_NORMALS, true>(ctx, s, a, st, camera, pixels, ev)
-                  : run_render<BAJO_RENDER_NORMALS, false>(ctx, s, a, st, camera, pixels, ev);
-    case BAJO_RENDER_AO:
-      return wide ? run_render<BAJO_RENDER_AO, true>(ctx, s, a, st, camera, pixels, ev)
-                  : run_render<BAJO_RENDER_AO, false>(ctx, s, a, st, camera, pixels, ev);
-    case BAJO_RENDER_NEE:
-      return wide ? run_render<BAJO_RENDER_NEE, true>(ctx, s, a, st, camera, pixels, ev)
-                  : run_render<BAJO_RENDER_NEE, false>(ctx, s, a, st, camera, pixels, ev);
-    case BAJO_RENDER_MIS:
-      return wide ? run_render<BAJO_RENDER_MIS, true>(ctx, s, a, st, camera, pixels, ev)
-                  : run_render<BAJO_RENDER_MIS, false>(ctx, s, a, st, camera, pixels, ev);
-    default:
-      BJ_REQUIRE(false, "bajo_render: unknown algorithm %d", st->algorithm);
-  }
+  return by_algorithm(s, st->algorithm, [&]<int kAlg, bool kWide, bool kMixed>() {
+    return run_render<kAlg, kWide, kMixed>(ctx, s, a, st, camera, pixels, ev);
+  });
 }
 
 }  // namespace
@@ -1167,9 +1233,19 @@ int32_t bajo_scene_create_ex(bajo_ctx* ctx, const float* vertices, uint64_t n, c
 int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
                                     const bajo_materials* materials, const bajo_build_options* options,
                                     bajo_scene** out) {
+  BJ_REQUIRE(n > 0, "bajo_scene_create: a scene needs triangles and their surfaces");
+  return bajo_scene_create_mixed(ctx, vertices, n, surfaces, nullptr, 0, nullptr, materials, options, out);
+}
+
+int32_t bajo_scene_create_mixed(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
+                                const float* spheres, uint64_t n_spheres, const uint32_t* sphere_surfaces,
+                                const bajo_materials* materials, const bajo_build_options* options,
+                                bajo_scene** out) {
   BJ_REQUIRE(ctx && out && materials, "bajo_scene_create: NULL ctx/materials/out");
-  BJ_REQUIRE(n > 0 && vertices && surfaces, "bajo_scene_create: a scene needs triangles and their surfaces");
-  BJ_REQUIRE(n < (1ull << 31), "bajo_scene_create: too many triangles");
+  BJ_REQUIRE(n + n_spheres > 0, "bajo_scene_create: a scene needs triangles or spheres");
+  BJ_REQUIRE(n == 0 || (vertices && surfaces), "bajo_scene_create: triangles need their vertices and surfaces");
+  BJ_REQUIRE(n_spheres == 0 || (spheres && sphere_surfaces), "bajo_scene_create: spheres need their surfaces");
+  BJ_REQUIRE(n < (1ull << 31) && n_spheres < (1ull << 31), "bajo_scene_create: too many primitives");
   const float *lambertian = materials->lambertian, *metal = materials->metal, *dielectric = materials->dielectric,
               *emissive = materials->emissive;
   const uint32_t n_lambertian = materials->n_lambertian, n_metal = materials->n_metal,
@@ -1177,18 +1253,27 @@ int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64
   BJ_REQUIRE((n_lambertian == 0 || lambertian) && (n_metal == 0 || metal) && (n_dielectric == 0 || dielectric) &&
                  (n_emissive == 0 || emissive),
              "bajo_scene_create: NULL material array with a nonzero count");
-  for (uint64_t i = 0; i < n; ++i) {
-    const uint32_t kind = surfaces[i] >> 28, mi = surfaces[i] & kSurfaceIndexMask;
-    BJ_REQUIRE(kind <= kMatEmissive, "bajo_scene_create: triangle %llu has material kind %u",
-               (unsigned long long)i, kind);
+  auto check_surface = [&](uint32_t sid, const char* what, uint64_t i) -> int32_t {
+    const uint32_t kind = sid >> 28, mi = sid & kSurfaceIndexMask;
+    BJ_REQUIRE(kind <= kMatEmissive, "bajo_scene_create: %s %llu has material kind %u", what, (unsigned long long)i,
+               kind);
     BJ_REQUIRE(kind != kMatLambertian || mi < n_lambertian, "bajo_scene_create: Lambertian index out of range");
     BJ_REQUIRE(kind != kMatMetal || mi < n_metal, "bajo_scene_create: metal index out of range");
     BJ_REQUIRE(kind != kMatDielectric || mi < n_dielectric, "bajo_scene_create: dielectric index out of range");
     BJ_REQUIRE(kind != kMatEmissive || mi < n_emissive, "bajo_scene_create: emissive index out of range");
+    return BAJO_OK;
+  };
+  for (uint64_t i = 0; i < n; ++i)
+    if (int32_t rc = check_surface(surfaces[i], "triangle", i)) return rc;
+  for (uint64_t i = 0; i < n_spheres; ++i) {
+    if (int32_t rc = check_surface(sphere_surfaces[i], "sphere", i)) return rc;
+    // SceneData (rt/types.mojo:448-452): a sphere's radius must be non-zero
+    BJ_REQUIRE(spheres[4 * i + 3] != 0.0f, "bajo_scene_create: sphere %llu has radius 0", (unsigned long long)i);
   }
   bajo_scene* s = new bajo_scene();
   s->device = ctx->device;
   s->n = n;
+  s->n_spheres = n_spheres;
   s->options = options ? *options : bajo_build_options{BAJO_BUILD_LBVH, BAJO_WIDTH_BINARY, 0, 0};
   auto fail = [&](int32_t rc) { free_scene(s); return rc; };
 #define SC_HIP(call)                                              \
@@ -1196,10 +1281,12 @@ int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64
     hipError_t e_ = (call);                                       \
     if (e_ != hipSuccess) return fail(hip_fail(e_, #call, __FILE__, __LINE__)); \
   } while (0)
-  SC_HIP(hipMalloc(&s->vertices, n * 9 * sizeof(float)));
-  SC_HIP(hipMemcpy(s->vertices, vertices, n * 9 * sizeof(float), hipMemcpyHostToDevice));
-  SC_HIP(hipMalloc(&s->surfaces, n * sizeof(uint32_t)));
-  SC_HIP(hipMemcpy(s->surfaces, surfaces, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (n) {
+    SC_HIP(hipMalloc(&s->vertices, n * 9 * sizeof(float)));
+    SC_HIP(hipMemcpy(s->vertices, vertices, n * 9 * sizeof(float), hipMemcpyHostToDevice));
+    SC_HIP(hipMalloc(&s->surfaces, n * sizeof(uint32_t)));
+    SC_HIP(hipMemcpy(s->surfaces, surfaces, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
   SC_HIP(hipMalloc(&s->lambertian, (n_lambertian ? n_lambertian : 1) * 3 * sizeof(float)));
   if (n_lambertian) SC_HIP(hipMemcpy(s->lambertian, lambertian, n_lambertian * 3 * sizeof(float), hipMemcpyHostToDevice));
   SC_HIP(hipMalloc(&s->metals, (n_metal ? n_metal : 1) * 4 * sizeof(float)));
@@ -1208,10 +1295,12 @@ int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64
   if (n_dielectric) SC_HIP(hipMemcpy(s->dielectrics, dielectric, n_dielectric * sizeof(float), hipMemcpyHostToDevice));
   SC_HIP(hipMalloc(&s->emissive, (n_emissive ? n_emissive : 1) * 3 * sizeof(float)));
   if (n_emissive) SC_HIP(hipMemcpy(s->emissive, emissive, n_emissive * 3 * sizeof(float), hipMemcpyHostToDevice));
-  // _build_light_store (rt/types.mojo:490-506): one light per emissive
-  // triangle, weight = area * mean radiance.
+  // _build_light_store (rt/types.mojo:490-530): one light per emissive
+  // triangle (weight = area * mean radiance), then per emissive sphere
+  // (weight = 4 pi r^2 * mean radiance, |r|); one alias table over both.
   std::vector<float> lf, weights;
-  std::vector<uint32_t> tri_of_light;
+  std::vector<uint32_t> prim_of_light;
+  std::vector<uint8_t> sphere_light;
   float total = 0.0f;
   for (uint64_t i = 0; i < n; ++i) {
     if ((surfaces[i] >> 28) != kMatEmissive) continue;
@@ -1223,7 +1312,21 @@ int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64
     const float w = area * imp;
     if (w > 0.0f) {
       weights.push_back(w);
-      tri_of_light.push_back((uint32_t)i);
+      prim_of_light.push_back((uint32_t)i);
+      sphere_light.push_back(0);
+      total += w;
+    }
+  }
+  for (uint64_t i = 0; i < n_spheres; ++i) {
+    if ((sphere_surfaces[i] >> 28) != kMatEmissive) continue;
+    const float r = fabsf(spheres[4 * i + 3]);
+    const float* e = emissive + 3 * (sphere_surfaces[i] & kSurfaceIndexMask);
+    const float imp = fmax_sel((e[0] + e[1] + e[2]) / 3.0f, 0.0f);
+    const float w = 4.0f * 3.14159265358979f * r * r * imp;
+    if (w > 0.0f) {
+      weights.push_back(w);
+      prim_of_light.push_back((uint32_t)i);
+      sphere_light.push_back(1);
       total += w;
     }
   }
@@ -1234,14 +1337,22 @@ int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64
   std::vector<uint32_t> kinds(nl ? nl : 1, 0u);
   lf.assign((nl ? nl : 1) * kLightStride, 0.0f);
   for (uint32_t l = 0; l < nl; ++l) {
-    const float* t = vertices + 9 * (uint64_t)tri_of_light[l];
-    const float* e = emissive + 3 * (surfaces[tri_of_light[l]] & kSurfaceIndexMask);
     float* o = lf.data() + (size_t)l * kLightStride;
-    for (int k = 0; k < 9; ++k) o[k] = t[k];
-    o[9] = 0.0f;
+    const float* e;
+    if (sphere_light[l]) {
+      const float* sp = spheres + 4 * (uint64_t)prim_of_light[l];
+      e = emissive + 3 * (sphere_surfaces[prim_of_light[l]] & kSurfaceIndexMask);
+      o[0] = sp[0]; o[1] = sp[1]; o[2] = sp[2];
+      o[9] = fabsf(sp[3]);
+    } else {
+      const float* t = vertices + 9 * (uint64_t)prim_of_light[l];
+      e = emissive + 3 * (surfaces[prim_of_light[l]] & kSurfaceIndexMask);
+      for (int k = 0; k < 9; ++k) o[k] = t[k];
+      o[9] = 0.0f;
+    }
     o[10] = e[0]; o[11] = e[1]; o[12] = e[2];
     o[13] = prob[l];
-    kinds[l] = (alias[l] << 4) | kPrimTriangle;
+    kinds[l] = (alias[l] << 4) | (sphere_light[l] ? kPrimSphere : kPrimTriangle);
   }
   s->light_count = nl;
   s->total_light_weight = total;
@@ -1249,8 +1360,23 @@ int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64
   SC_HIP(hipMemcpy(s->light_fields, lf.data(), lf.size() * sizeof(float), hipMemcpyHostToDevice));
   SC_HIP(hipMalloc(&s->light_kinds, kinds.size() * sizeof(uint32_t)));
   SC_HIP(hipMemcpy(s->light_kinds, kinds.data(), kinds.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (n_spheres) {
+    // the BVH is built over |r| (sphere_for_acceleration, rt/geometry.mojo:41-46)
+    std::vector<float> absr(4 * n_spheres), radii(n_spheres);
+    for (uint64_t i = 0; i < n_spheres; ++i) {
+      for (int k = 0; k < 3; ++k) absr[4 * i + k] = spheres[4 * i + k];
+      absr[4 * i + 3] = fabsf(spheres[4 * i + 3]);
+      radii[i] = spheres[4 * i + 3];
+    }
+    SC_HIP(hipMalloc(&s->spheres, absr.size() * sizeof(float)));
+    SC_HIP(hipMemcpy(s->spheres, absr.data(), absr.size() * sizeof(float), hipMemcpyHostToDevice));
+    SC_HIP(hipMalloc(&s->sphere_radii, radii.size() * sizeof(float)));
+    SC_HIP(hipMemcpy(s->sphere_radii, radii.data(), radii.size() * sizeof(float), hipMemcpyHostToDevice));
+    SC_HIP(hipMalloc(&s->sphere_surfaces, n_spheres * sizeof(uint32_t)));
+    SC_HIP(hipMemcpy(s->sphere_surfaces, sphere_surfaces, n_spheres * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
 #undef SC_HIP
-  int32_t rc = bajo_build_triangle_bvh_ex(ctx, s->vertices, n, &s->options, nullptr, &s->bvh);
+  int32_t rc = bajo_scene_rebuild(ctx, s, nullptr);
   if (rc) return fail(rc);
   *out = s;
   return BAJO_OK;
@@ -1258,7 +1384,16 @@ int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64
 
 int32_t bajo_scene_rebuild(bajo_ctx* ctx, bajo_scene* s, bajo_build_timings* timings) {
   BJ_REQUIRE(ctx && s, "bajo_scene_rebuild: NULL argument");
-  return bajo_build_triangle_bvh_ex(ctx, s->vertices, s->n, &s->options, timings, &s->bvh);
+  if (s->n) {
+    const int32_t rc = bajo_build_triangle_bvh_ex(ctx, s->vertices, s->n, &s->options, timings, &s->bvh);
+    if (rc) return rc;
+  }
+  if (s->n_spheres) {
+    // the sphere BVH: the LBVH over the spheres' boxes at the scene's node width and leaf size
+    const bajo_build_options so{BAJO_BUILD_LBVH, s->options.node_width, s->options.leaf_size, 0};
+    return bajo_build_sphere_bvh(ctx, s->spheres, s->n_spheres, &so, nullptr, &s->sphere_bvh);
+  }
+  return BAJO_OK;
 }
 
 int32_t bajo_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float* out) {

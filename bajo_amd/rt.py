@@ -3,8 +3,8 @@
 Mirrors ``render_gpu[ALGORITHM](settings, camera, world) -> RenderResult``
 (rt/gpu/render.mojo:322-420), ``RenderSettings`` (rt/types.mojo:324-357),
 the ``RENDER`` / ``MAT`` enums (rt/types.mojo:24-57) and ``SceneData`` for
-triangle scenes with Lambertian, metal, dielectric and emissive surfaces
-(rt/types.mojo:382-526).
+scenes of triangles and spheres with Lambertian, metal, dielectric and
+emissive surfaces (rt/types.mojo:382-526).
 The renderer runs entirely in libbajo_hip.so (bajo_amd/csrc/render.hip).
 """
 
@@ -66,7 +66,7 @@ class RenderSettings:
 
 @dataclass
 class SceneData:
-    """Host description of a triangle scene."""
+    """Host description of a scene: triangles and, optionally, spheres (SceneData, rt/types.mojo:382-452)."""
 
     vertices: np.ndarray  # f32[9N]
     surfaces: np.ndarray  # u32[N], surface_id(kind, index)
@@ -74,10 +74,16 @@ class SceneData:
     emissive: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[3E] radiance
     metal: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[4M] albedo, fuzz
     dielectric: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[D] refraction index
+    spheres: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[4S] center, signed radius
+    sphere_surfaces: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))  # u32[S]
 
     @property
     def triangle_count(self) -> int:
         return self.vertices.size // 9
+
+    @property
+    def sphere_count(self) -> int:
+        return np.asarray(self.spheres).size // 4
 
 
 class GpuScene:
@@ -94,8 +100,12 @@ class GpuScene:
         emit = np.ascontiguousarray(scene.emissive, dtype=np.float32).reshape(-1)
         met = np.ascontiguousarray(getattr(scene, "metal", np.zeros(0)), dtype=np.float32).reshape(-1)
         die = np.ascontiguousarray(getattr(scene, "dielectric", np.zeros(0)), dtype=np.float32).reshape(-1)
-        if s.size != v.size // 9:
+        sp = np.ascontiguousarray(getattr(scene, "spheres", np.zeros(0)), dtype=np.float32).reshape(-1)
+        ss = np.ascontiguousarray(getattr(scene, "sphere_surfaces", np.zeros(0)), dtype=np.uint32).reshape(-1)
+        if s.size != v.size // 9 or v.size % 9:
             raise ValueError("one surface id per triangle is required")
+        if sp.size % 4 or ss.size != sp.size // 4:
+            raise ValueError("spheres are f32[4] (center, signed radius) with one surface id each")
         if lamb.size % 3 or emit.size % 3 or met.size % 4:
             raise ValueError("materials are f32[3] per Lambertian / emissive and f32[4] per metal")
         handle = ctypes.c_void_p()
@@ -105,11 +115,14 @@ class GpuScene:
             die.ctypes.data_as(ctypes.c_void_p), die.size, emit.ctypes.data_as(ctypes.c_void_p), emit.size // 3,
         )
         _lib.check(
-            _lib.lib().bajo_scene_create_materials(
+            _lib.lib().bajo_scene_create_mixed(
                 self.ctx.handle,
                 v.ctypes.data_as(ctypes.c_void_p),
                 v.size // 9,
                 s.ctypes.data_as(ctypes.c_void_p),
+                sp.ctypes.data_as(ctypes.c_void_p),
+                sp.size // 4,
+                ss.ctypes.data_as(ctypes.c_void_p),
                 ctypes.byref(mats),
                 ctypes.byref(opts),
                 ctypes.byref(handle),
@@ -118,6 +131,7 @@ class GpuScene:
         self._keep = (lamb, emit, met, die)
         self._handle = handle
         self.triangle_count = v.size // 9
+        self.sphere_count = sp.size // 4
         self.node_width = node_width
 
     @property

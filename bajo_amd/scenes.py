@@ -171,3 +171,81 @@ def material_box_scene(seed: int = 7, n_soup: int = 60, emission: float = 12.0):
     assert surfaces.size * 9 == vertices.size
     return SceneData(vertices, surfaces, lamb.reshape(-1), np.array([emission] * 3, np.float32), metal.reshape(-1),
                      dielectric)
+
+
+def mixed_box_scene(seed: int = 11, n_small: int = 24, emission: float = 12.0, sphere_emission: float = 6.0):
+    """The box scene's walls and quad light with spheres of every material kind (bajo's mixed
+    triangle + sphere scenes, rt/gpu/mixed_path.mojo): a Lambertian ball, a rough metal ball, a glass
+    ball with a hollow inner sphere (negative radius: the normal points inward), an emissive ball
+    (a sphere light), and `n_small` small spheres cycling Lambertian / metal / glass; for mixed-scene
+    parity and radiance tests."""
+    from .rt import MAT, SceneData, surface_id
+
+    floor = quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))
+    back = quad((-1, 0, 1), (-1, 2, 1), (1, 2, 1), (1, 0, 1))
+    left = quad((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1))
+    right = quad((1, 0, 1), (1, 2, 1), (1, 2, -1), (1, 0, -1))
+    light = quad((-0.3, 1.9, -0.3), (0.3, 1.9, -0.3), (0.3, 1.9, 0.3), (-0.3, 1.9, 0.3))
+    walls = np.concatenate([floor, back, left, right])
+    vertices = np.concatenate([walls, light]).astype(np.float32)
+    surfaces = np.concatenate([
+        (np.uint32(MAT.LAMBERTIAN) << 28) | np.repeat(np.arange(4, dtype=np.uint32), 2),
+        np.full(2, surface_id(MAT.EMISSIVE, 0), np.uint32),
+    ]).astype(np.uint32)
+    lamb = np.array([[0.7, 0.7, 0.7], [0.7, 0.7, 0.7], [0.6, 0.2, 0.2], [0.2, 0.6, 0.2], [0.5, 0.4, 0.3]], np.float32)
+    metal = np.array([[0.8, 0.65, 0.3, 0.25], [0.7, 0.75, 0.8, 0.0]], np.float32)
+    dielectric = np.array([1.5], np.float32)
+    emit = np.array([[emission] * 3, [sphere_emission, 0.8 * sphere_emission, 0.5 * sphere_emission]], np.float32)
+    big = [
+        ((-0.5, 0.3, 0.2, 0.3), surface_id(MAT.LAMBERTIAN, 4)),
+        ((0.45, 0.25, 0.35, 0.25), surface_id(MAT.METAL, 0)),
+        ((0.1, 0.35, -0.35, 0.3), surface_id(MAT.DIELECTRIC, 0)),
+        ((0.1, 0.35, -0.35, -0.25), surface_id(MAT.DIELECTRIC, 0)),
+        ((-0.55, 1.2, 0.6, 0.12), surface_id(MAT.EMISSIVE, 1)),
+    ]
+    rng = np.random.default_rng(seed)
+    c = rng.uniform([-0.8, 0.6, -0.6], [0.8, 1.5, 0.6], (n_small, 3))
+    r = rng.uniform(0.04, 0.1, n_small)
+    cycle = np.array([surface_id(MAT.LAMBERTIAN, 4), surface_id(MAT.METAL, 1), surface_id(MAT.DIELECTRIC, 0)],
+                     np.uint32)
+    spheres = np.concatenate([np.array([b[0] for b in big], np.float32).reshape(-1),
+                              np.concatenate([c, r[:, None]], axis=1).astype(np.float32).reshape(-1)])
+    sphere_surfaces = np.concatenate([np.array([b[1] for b in big], np.uint32), cycle[np.arange(n_small) % 3]])
+    return SceneData(vertices, surfaces, lamb.reshape(-1), emit.reshape(-1), metal.reshape(-1), dielectric,
+                     spheres.astype(np.float32), sphere_surfaces.astype(np.uint32))
+
+
+def sphere_field_scene(n: int = 2000, seed: int = 13, emission: float = 4.0):
+    """A spheres-only scene (no triangles): a large Lambertian ground sphere, `n` small spheres of every
+    material kind scattered above it and a few emissive spheres as the lights."""
+    from .rt import MAT, SceneData, surface_id
+
+    rng = np.random.default_rng(seed)
+    xz = rng.uniform(-4.0, 4.0, (n, 2))
+    r = rng.uniform(0.05, 0.2, n)
+    c = np.stack([xz[:, 0], r, xz[:, 1]], axis=1)
+    kinds = rng.integers(0, 3, n)
+    lamb_n = int((kinds == 0).sum())
+    lamb = np.concatenate([[[0.5, 0.5, 0.5]], rng.uniform(0.1, 0.9, (lamb_n, 3))]).astype(np.float32)
+    metal = np.array([[0.7, 0.6, 0.5, 0.0], [0.8, 0.8, 0.9, 0.3]], np.float32)
+    surf = np.empty(n, np.uint32)
+    li = 1
+    for i, k in enumerate(kinds):
+        if k == 0:
+            surf[i] = surface_id(MAT.LAMBERTIAN, li)
+            li += 1
+        elif k == 1:
+            surf[i] = surface_id(MAT.METAL, i % 2)
+        else:
+            surf[i] = surface_id(MAT.DIELECTRIC, 0)
+    lights = np.array([[-2.0, 3.0, -1.0, 0.6], [2.5, 2.5, 1.5, 0.4], [0.0, 4.0, 3.0, 0.8]], np.float32)
+    spheres = np.concatenate([
+        np.array([0.0, -1000.0, 0.0, 1000.0], np.float32),
+        np.concatenate([c, r[:, None]], axis=1).astype(np.float32).reshape(-1),
+        lights.reshape(-1),
+    ])
+    sphere_surfaces = np.concatenate([[surface_id(MAT.LAMBERTIAN, 0)], surf,
+                                      np.full(3, surface_id(MAT.EMISSIVE, 0), np.uint32)]).astype(np.uint32)
+    return SceneData(np.zeros(0, np.float32), np.zeros(0, np.uint32), lamb.reshape(-1),
+                     np.array([emission] * 3, np.float32), metal.reshape(-1), np.array([1.5], np.float32),
+                     spheres.astype(np.float32), sphere_surfaces)

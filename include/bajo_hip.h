@@ -375,6 +375,17 @@ int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64
                                     const bajo_materials* materials, const bajo_build_options* options,
                                     bajo_scene** out);
 /* Rebuilds the scene's BVH from its device-resident vertices. */
+/* A mixed scene (GpuRtMixedScene / SceneData with spheres, rt/gpu/mixed_path.mojo,
+ * rt/types.mojo:382-530): triangles as above (n may be 0) plus spheres
+ * f32[4S] (center, signed radius: a negative radius turns the normal
+ * inward) with one surface id each.  Emissive spheres are lights (weight
+ * 4 pi r^2 * mean radiance, after the triangle lights).  Closest hits test
+ * the spheres first, then the triangles (bajo's scene trace order).  The
+ * sphere BVH is the LBVH at the scene's node width and leaf size. */
+int32_t bajo_scene_create_mixed(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
+                                const float* spheres, uint64_t n_spheres, const uint32_t* sphere_surfaces,
+                                const bajo_materials* materials, const bajo_build_options* options,
+                                bajo_scene** out);
 int32_t bajo_scene_rebuild(bajo_ctx* ctx, bajo_scene* scene, bajo_build_timings* timings);
 int32_t bajo_scene_free(bajo_scene* scene);
 int32_t bajo_scene_get_bvh(const bajo_scene* scene, bajo_bvh** out);

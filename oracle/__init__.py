@@ -113,6 +113,18 @@ _SIG = {
         [c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p,
          c_uint64, c_int, c_void_p],
     ),
+    "orc_render_mixed": (
+        c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                c_void_p, c_void_p, c_void_p, c_void_p, c_int],
+    ),
+    "orc_render_fp64_mixed": (
+        c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                c_void_p, c_void_p, c_void_p, c_int],
+    ),
+    "orc_light_table_mixed": (
+        None, [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+               c_void_p],
+    ),
     "orc_pow_unit": (c_float, [c_float, c_float]),
     "orc_reflectance": (c_float, [c_float, c_float]),
     "orc_reflect3": (None, [c_void_p, c_void_p, c_void_p]),
@@ -358,18 +370,27 @@ def alias_table(weights) -> tuple[np.ndarray, np.ndarray]:
     return prob[:n], alias[:n]
 
 
-def light_table(vertices, surfaces, emissive):
+def light_table(vertices, surfaces, emissive, spheres=None, sphere_surfaces=None):
     v = f32(vertices)
     n = v.size // 9
     surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
     e = f32(emissive) if len(emissive) else np.zeros(3, np.float32)
-    fields = np.empty(14 * max(n, 1), np.float32)
-    kinds = np.empty(max(n, 1), np.uint32)
+    sp, ss, ns = _spheres(spheres, sphere_surfaces)
+    fields = np.empty(14 * max(n + ns, 1), np.float32)
+    kinds = np.empty(max(n + ns, 1), np.uint32)
     count = np.zeros(1, np.uint32)
     total = np.zeros(1, np.float32)
-    lib().orc_light_table(_p(v), n, _p(surf), _p(e), _p(fields), _p(kinds), _p(count), _p(total))
+    lib().orc_light_table_mixed(_p(v), n, _p(surf), _p(sp), ns, _p(ss), _p(e), _p(fields), _p(kinds), _p(count),
+                                _p(total))
     k = int(count[0])
     return fields[: 14 * k], kinds[:k], float(total[0])
+
+
+def _spheres(spheres, sphere_surfaces):
+    if spheres is None or len(spheres) == 0:
+        return np.zeros(4, np.float32), np.zeros(1, np.uint32), 0
+    sp = f32(spheres)
+    return sp, np.ascontiguousarray(sphere_surfaces, np.uint32), sp.size // 4
 
 
 def _materials(lambertian, emissive, metal, dielectric):
@@ -380,18 +401,21 @@ def _materials(lambertian, emissive, metal, dielectric):
 
 
 def render(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1,
-           node_width=2, leaf_size=0, method=0, metal=None, dielectric=None):
+           node_width=2, leaf_size=0, method=0, metal=None, dielectric=None, spheres=None, sphere_surfaces=None):
     """The FP32 wavefront renderer restated (Lambertian, metal f32[4 per material], dielectric f32[1],
-    emissive): (pixels, (extension rays, shadow rays))."""
-    v = f32(vertices)
-    surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
+    emissive; optional spheres f32[4S] with signed radii and their surface ids):
+    (pixels, (extension rays, shadow rays))."""
+    v = f32(vertices) if len(vertices) else np.zeros(9, np.float32)
+    n = len(vertices) // 9
+    surf = np.ascontiguousarray(surfaces, dtype=np.uint32) if n else np.zeros(1, np.uint32)
     lamb, emit, met, die = _materials(lambertian, emissive, metal, dielectric)
+    sp, ss, ns = _spheres(spheres, sphere_surfaces)
     st = RenderSettings(width, height, spp, max_depth, seed, algorithm, node_width, leaf_size, method)
     pixels = np.empty(3 * width * height, np.float32)
     rays = np.zeros(2, np.uint64)
-    rc = lib().orc_render(
-        _p(v), v.size // 9, _p(surf), _p(lamb), _p(met), _p(die), _p(emit), ctypes.byref(st), _p(f32(cam20)),
-        _p(pixels), _p(rays), threads,
+    rc = lib().orc_render_mixed(
+        _p(v), n, _p(surf), _p(sp), ns, _p(ss), _p(lamb), _p(met), _p(die), _p(emit), ctypes.byref(st),
+        _p(f32(cam20)), _p(pixels), _p(rays), threads,
     )
     if rc != 0:
         raise RuntimeError(f"orc_render failed ({rc})")
@@ -479,14 +503,16 @@ def canonical_wide(nodes: np.ndarray) -> np.ndarray:
 
 
 def render_fp64(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1,
-                metal=None, dielectric=None):
-    v = f32(vertices)
-    surf = np.ascontiguousarray(surfaces, dtype=np.uint32)
+                metal=None, dielectric=None, spheres=None, sphere_surfaces=None):
+    v = f32(vertices) if len(vertices) else np.zeros(9, np.float32)
+    n = len(vertices) // 9
+    surf = np.ascontiguousarray(surfaces, dtype=np.uint32) if n else np.zeros(1, np.uint32)
     lamb, emit, met, die = _materials(lambertian, emissive, metal, dielectric)
+    sp, ss, ns = _spheres(spheres, sphere_surfaces)
     st = RenderSettings(width, height, spp, max_depth, seed, algorithm, 2, 0, 0)
     pixels = np.empty(3 * width * height, np.float64)
-    lib().orc_render_fp64(_p(v), v.size // 9, _p(surf), _p(lamb), _p(met), _p(die), _p(emit), ctypes.byref(st),
-                          _p(f32(cam20)), _p(pixels), threads)
+    lib().orc_render_fp64_mixed(_p(v), n, _p(surf), _p(sp), ns, _p(ss), _p(lamb), _p(met), _p(die), _p(emit),
+                                ctypes.byref(st), _p(f32(cam20)), _p(pixels), threads)
     return pixels
 
 

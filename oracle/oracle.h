@@ -109,6 +109,20 @@ typedef struct {
     int leaf_size;  /* width 8: triangles per leaf child, 1..8; 0 = 4 */
     int method;     /* 0 LBVH, 1 H-PLOC (orc_hploc.c, depth-first leaf order) */
 } orc_render_settings;
+
+/* mixed triangle + sphere scenes (orc_render.c); spheres f32[4S] with signed radii */
+int orc_render_mixed(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *spheres, uint64_t ns,
+                     const uint32_t *sphere_surfaces, const float *lambertian, const float *metal,
+                     const float *dielectric, const float *emissive, const orc_render_settings *st,
+                     const float *cam20, float *pixels, uint64_t *ray_counts, int threads);
+int orc_render_fp64_mixed(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *spheres,
+                          uint64_t ns, const uint32_t *sphere_surfaces, const float *lambertian, const float *metal,
+                          const float *dielectric, const float *emissive, const orc_render_settings *st,
+                          const float *cam20, double *pixels, int threads);
+void orc_light_table_mixed(const float *v, uint64_t n, const uint32_t *surf, const float *sph, uint64_t ns,
+                           const uint32_t *ssurf, const float *emissive, float *fields, uint32_t *kinds,
+                           uint32_t *count, float *total);
+
 void orc_philox4x32_10(const uint32_t *ctr, uint32_t k0, uint32_t k1, uint32_t *out);
 void orc_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float *out);
 void orc_sincos_turns(float u, float *s, float *c);

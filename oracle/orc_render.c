@@ -296,11 +296,17 @@ typedef struct {
     float total;
 } lights_t;
 
-static void build_lights(const float *v, uint64_t n, const uint32_t *surf, const float *emissive, lights_t *L) {
-    uint32_t cnt = 0;
+/* SceneData._build_light_store (rt/types.mojo:490-530): emissive triangles
+ * (weight area * mean radiance), then emissive spheres (weight
+ * 4 pi r^2 * mean radiance, |r|); one alias table over both. */
+static void build_lights_mixed(const float *v, uint64_t n, const uint32_t *surf, const float *sph, uint64_t ns,
+                               const uint32_t *ssurf, const float *emissive, lights_t *L) {
+    uint64_t cnt = 0;
     for (uint64_t i = 0; i < n; ++i) if ((surf[i] >> 28) == MAT_EMIT) cnt++;
+    for (uint64_t i = 0; i < ns; ++i) if ((ssurf[i] >> 28) == MAT_EMIT) cnt++;
     float *w = (float *)malloc((cnt ? cnt : 1) * sizeof(float));
     uint32_t *tri = (uint32_t *)malloc((cnt ? cnt : 1) * sizeof(uint32_t));
+    uint8_t *is_s = (uint8_t *)malloc(cnt ? cnt : 1);
     uint32_t k = 0;
     float total = 0.0f;
     for (uint64_t i = 0; i < n; ++i) {
@@ -312,7 +318,15 @@ static void build_lights(const float *v, uint64_t n, const uint32_t *surf, const
         const float *e = emissive + 3 * (surf[i] & IDX_MASK);
         float imp = orc_fmax((e[0] + e[1] + e[2]) / 3.0f, 0.0f);
         float wt = area * imp;
-        if (wt > 0.0f) { w[k] = wt; tri[k] = (uint32_t)i; total += wt; k++; }
+        if (wt > 0.0f) { w[k] = wt; tri[k] = (uint32_t)i; is_s[k] = 0; total += wt; k++; }
+    }
+    for (uint64_t i = 0; i < ns; ++i) {
+        if ((ssurf[i] >> 28) != MAT_EMIT) continue;
+        float r = fabsf(sph[4 * i + 3]);
+        const float *e = emissive + 3 * (ssurf[i] & IDX_MASK);
+        float imp = orc_fmax((e[0] + e[1] + e[2]) / 3.0f, 0.0f);
+        float wt = 4.0f * 3.14159265358979f * r * r * imp;
+        if (wt > 0.0f) { w[k] = wt; tri[k] = (uint32_t)i; is_s[k] = 1; total += wt; k++; }
     }
     float *prob = (float *)malloc((k ? k : 1) * sizeof(float));
     uint32_t *alias = (uint32_t *)malloc((k ? k : 1) * sizeof(uint32_t));
@@ -320,23 +334,38 @@ static void build_lights(const float *v, uint64_t n, const uint32_t *surf, const
     L->fields = (float *)calloc((k ? k : 1) * L_STRIDE, sizeof(float));
     L->kinds = (uint32_t *)calloc(k ? k : 1, sizeof(uint32_t));
     for (uint32_t l = 0; l < k; ++l) {
-        const float *t = v + 9 * (uint64_t)tri[l];
-        const float *e = emissive + 3 * (surf[tri[l]] & IDX_MASK);
         float *o = L->fields + (size_t)l * L_STRIDE;
-        for (int j = 0; j < 9; ++j) o[j] = t[j];
+        const float *e;
+        if (is_s[l]) {
+            const float *sp = sph + 4 * (uint64_t)tri[l];
+            e = emissive + 3 * (ssurf[tri[l]] & IDX_MASK);
+            o[0] = sp[0]; o[1] = sp[1]; o[2] = sp[2];
+            o[9] = fabsf(sp[3]);
+        } else {
+            const float *t = v + 9 * (uint64_t)tri[l];
+            e = emissive + 3 * (surf[tri[l]] & IDX_MASK);
+            for (int j = 0; j < 9; ++j) o[j] = t[j];
+        }
         o[10] = e[0]; o[11] = e[1]; o[12] = e[2];
         o[13] = prob[l];
-        L->kinds[l] = (alias[l] << 4) | 1u;
+        L->kinds[l] = (alias[l] << 4) | (is_s[l] ? 0u : 1u);
     }
     L->count = k;
     L->total = total;
-    free(w); free(tri); free(prob); free(alias);
+    free(w); free(tri); free(is_s); free(prob); free(alias);
 }
+
 
 void orc_light_table(const float *v, uint64_t n, const uint32_t *surf, const float *emissive, float *fields,
                      uint32_t *kinds, uint32_t *count, float *total) {
+    orc_light_table_mixed(v, n, surf, NULL, 0, NULL, emissive, fields, kinds, count, total);
+}
+
+void orc_light_table_mixed(const float *v, uint64_t n, const uint32_t *surf, const float *sph, uint64_t ns,
+                           const uint32_t *ssurf, const float *emissive, float *fields, uint32_t *kinds,
+                           uint32_t *count, float *total) {
     lights_t L;
-    build_lights(v, n, surf, emissive, &L);
+    build_lights_mixed(v, n, surf, sph, ns, ssurf, emissive, &L);
     if (fields) memcpy(fields, L.fields, (size_t)L.count * L_STRIDE * sizeof(float));
     if (kinds) memcpy(kinds, L.kinds, (size_t)L.count * sizeof(uint32_t));
     *count = L.count;
@@ -352,7 +381,47 @@ typedef struct {
     lights_t lights;
     orc_render_settings st;
     float cam[20];
+    /* spheres (rt/gpu/scene_trace.mojo:113-193): their BVH over |r|, the
+     * signed radii (r < 0 turns the normal inward) and their surfaces */
+    uint64_t ns;
+    orc_bvh sbvh;
+    const float *srad;
+    const uint32_t *ssurf;
 } scene_t;
+
+/* Closest hit of a mixed scene: spheres first, then triangles with t_max =
+ * the sphere hit (a triangle must be strictly closer); *surf receives the
+ * hit's surface id and the normal is the outward one of bajo's geometry. */
+static orc_hit scene_closest(const scene_t *sc, orc_v3 o, orc_v3 d, float tmin, uint32_t *surf) {
+    if (!sc->ns) {
+        orc_hit h = orc_trace_closest(&sc->bvh, o, d, tmin, ORC_FLT_MAX, NULL);
+        if (h.prim != ORC_EMPTY) *surf = sc->surf[h.prim];
+        return h;
+    }
+    orc_hit hs = orc_trace_closest(&sc->sbvh, o, d, tmin, ORC_FLT_MAX, NULL);
+    float tmax = hs.prim != ORC_EMPTY ? hs.t : ORC_FLT_MAX;
+    orc_hit ht = sc->bvh.prim_count ? orc_trace_closest(&sc->bvh, o, d, tmin, tmax, NULL) : hs;
+    if (sc->bvh.prim_count && ht.prim != ORC_EMPTY && ht.t < tmax) {
+        *surf = sc->surf[ht.prim];
+        return ht;
+    }
+    if (hs.prim != ORC_EMPTY) {
+        if (sc->srad[hs.prim] < 0.0f) hs.n = orc_v(-hs.n.x, -hs.n.y, -hs.n.z);
+        *surf = sc->ssurf[hs.prim];
+    }
+    return hs;
+}
+
+/* Occlusion of a mixed scene: spheres, then triangles. */
+static int scene_occluded(const scene_t *sc, orc_v3 o, orc_v3 d, float tmin, float tmax) {
+    if (sc->ns) {
+        orc_hit hs = orc_trace_any(&sc->sbvh, o, d, tmin, tmax);
+        if (hs.prim == ORC_EMPTY && hs.t == 0.0f) return 1;
+        if (!sc->bvh.prim_count) return 0;
+    }
+    orc_hit h = orc_trace_any(&sc->bvh, o, d, tmin, tmax);
+    return h.prim == ORC_EMPTY && h.t == 0.0f;
+}
 
 static float power_heuristic(float a, float b) {
     float a2 = a * a, b2 = b * b, d = a2 + b2;
@@ -382,17 +451,28 @@ static int direct_light(const scene_t *sc, uint32_t path_id, uint32_t bounce, or
     uint32_t packed = L->kinds[sel];
     if (frac > L->fields[sel * L_STRIDE + 13]) sel = packed >> 4;
     const float *f = L->fields + (size_t)sel * L_STRIDE;
-    orc_v3 p0 = orc_v(f[0], f[1], f[2]), p1 = orc_v(f[3], f[4], f[5]), p2 = orc_v(f[6], f[7], f[8]);
-    orc_v3 e1 = orc_sub(p1, p0), e2 = orc_sub(p2, p0);
-    orc_v3 av = orc_cross(e1, e2);
-    float tas = orc_dot(av, av);
-    if (tas <= 0.0f) return 0;
-    float sq = sqrtf(tas);
-    orc_v3 ln = orc_v(av.x / sq, av.y / sq, av.z / sq);
-    float root_u = sqrtf(rng_next(&rng));
-    float b1 = root_u * (1.0f - rng_next(&rng));
-    float b2 = root_u - b1;
-    orc_v3 lp = orc_add(orc_add(p0, orc_scale(e1, b1)), orc_scale(e2, b2));
+    orc_v3 p0 = orc_v(f[0], f[1], f[2]), ln, lp;
+    if ((L->kinds[sel] & 0xFu) == 0u) {
+        /* sphere light: random_unit_vector (core/random.mojo:39-45) as the light normal */
+        float ss, cs;
+        orc_sincos_turns(rng_next(&rng), &ss, &cs);
+        float z = 1.0f - 2.0f * rng_next(&rng);
+        float r = sqrtf(1.0f - z * z);
+        ln = orc_v(r * cs, r * ss, z);
+        lp = orc_add(p0, orc_scale(ln, f[9]));
+    } else {
+        orc_v3 p1 = orc_v(f[3], f[4], f[5]), p2 = orc_v(f[6], f[7], f[8]);
+        orc_v3 e1 = orc_sub(p1, p0), e2 = orc_sub(p2, p0);
+        orc_v3 av = orc_cross(e1, e2);
+        float tas = orc_dot(av, av);
+        if (tas <= 0.0f) return 0;
+        float sq = sqrtf(tas);
+        ln = orc_v(av.x / sq, av.y / sq, av.z / sq);
+        float root_u = sqrtf(rng_next(&rng));
+        float b1 = root_u * (1.0f - rng_next(&rng));
+        float b2 = root_u - b1;
+        lp = orc_add(orc_add(p0, orc_scale(e1, b1)), orc_scale(e2, b2));
+    }
     orc_v3 tl = orc_sub(lp, point);
     float d2 = orc_dot(tl, tl);
     if (d2 <= 1.0e-8f) return 0;
@@ -457,7 +537,8 @@ static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint
     sample[0] = sample[1] = sample[2] = 0.0f;
     for (uint32_t b = 0; b < depth; ++b) {
         if (rays) rays[0]++;
-        orc_hit h = orc_trace_closest(&sc->bvh, o, d, tmin, ORC_FLT_MAX, NULL);
+        uint32_t surf = 0;
+        orc_hit h = scene_closest(sc, o, d, tmin, &surf);
         int found = h.prim != ORC_EMPTY && h.t < ORC_FLT_MAX;
         int shadow = 0, next = 0;
         orc_v3 sh_o = orc_v(0, 0, 0), sh_d = orc_v(0, 0, 0), sh_c = orc_v(0, 0, 0);
@@ -471,7 +552,7 @@ static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint
             int front = orc_dot(d, h.n) < 0.0f;
             orc_v3 n = front ? h.n : orc_v(-h.n.x, -h.n.y, -h.n.z);
             orc_v3 point = orc_add(o, orc_scale(d, h.t));
-            uint32_t surf = sc->surf[h.prim], kind = surf >> 28, mi = surf & IDX_MASK;
+            uint32_t kind = surf >> 28, mi = surf & IDX_MASK;
             if (alg == 1) {
                 acc(sample, orc_scale(orc_add(n, orc_v(1.0f, 1.0f, 1.0f)), 0.5f));
             } else if (alg == 2) {
@@ -581,8 +662,7 @@ static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint
         }
         if (shadow) {
             if (rays) rays[1]++;
-            orc_hit sh = orc_trace_any(&sc->bvh, sh_o, sh_d, 0.001f, sh_tmax);
-            int occluded = sh.prim == ORC_EMPTY && sh.t == 0.0f;
+            int occluded = scene_occluded(sc, sh_o, sh_d, 0.001f, sh_tmax);
             if (alg == 2) { float v = occluded ? 0.08f : 1.0f; acc(sample, orc_v(v, v, v)); }
             else if (!occluded) acc(sample, sh_c);
         }
@@ -602,6 +682,60 @@ static void *render_worker(void *p) {
 int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
                const float *metal, const float *dielectric, const float *emissive, const orc_render_settings *st,
                const float *cam20, float *pixels, uint64_t *ray_counts, int threads) {
+    return orc_render_mixed(vertices, n, surfaces, NULL, 0, NULL, lambertian, metal, dielectric, emissive, st, cam20,
+                            pixels, ray_counts, threads);
+}
+
+/* The sphere half of a mixed scene: the LBVH over the spheres' boxes
+ * (|r|), collapsed to the render's node width; returns 0 on failure. */
+typedef struct { float *abs4, *boxes, *nb; uint64_t *keys; uint32_t *ids, *meta, *parent, *counts; void *wide; } sph_bvh_t;
+
+static int build_sphere_bvh(const float *sph, uint64_t ns, const orc_render_settings *st, orc_bvh *b, sph_bvh_t *t) {
+    uint64_t m = ns > 1 ? ns - 1 : 1;
+    memset(t, 0, sizeof(*t));
+    t->abs4 = (float *)malloc(4 * ns * sizeof(float));
+    t->boxes = (float *)malloc(6 * ns * sizeof(float));
+    for (uint64_t i = 0; i < ns; ++i) {
+        float r = fabsf(sph[4 * i + 3]);
+        for (int k = 0; k < 3; ++k) {
+            t->abs4[4 * i + k] = sph[4 * i + k];
+            t->boxes[6 * i + k] = sph[4 * i + k] - r;
+            t->boxes[6 * i + 3 + k] = sph[4 * i + k] + r;
+        }
+        t->abs4[4 * i + 3] = r;
+    }
+    t->keys = (uint64_t *)malloc(ns * sizeof(uint64_t));
+    t->ids = (uint32_t *)malloc(ns * sizeof(uint32_t));
+    t->meta = (uint32_t *)malloc(4 * m * sizeof(uint32_t));
+    t->parent = (uint32_t *)malloc(ns * sizeof(uint32_t));
+    t->counts = (uint32_t *)malloc(m * sizeof(uint32_t));
+    t->nb = (float *)malloc(12 * m * sizeof(float));
+    float scene12[12];
+    uint32_t root = orc_build_lbvh_bounds(t->boxes, ns, t->keys, t->ids, t->meta, t->parent, t->counts, t->nb, scene12);
+    memset(b, 0, sizeof(*b));
+    b->leaf_ids = t->ids; b->node_meta = t->meta; b->node_bounds = t->nb; b->root = root;
+    b->prim_count = ns; b->spheres = t->abs4;
+    if (st->node_width == 8) {
+        t->wide = malloc(ns * 96);
+        uint64_t wc = orc_collapse_wide(t->meta, t->nb, t->counts, ns, root, scene12,
+                                        st->leaf_size ? (uint32_t)st->leaf_size : 4u, t->wide, ns);
+        if (wc == 0) return 0;
+        b->wide = t->wide;
+        b->wide_count = wc;
+    }
+    return 1;
+}
+
+static void free_sphere_bvh(sph_bvh_t *t) {
+    free(t->abs4); free(t->boxes); free(t->nb); free(t->keys); free(t->ids); free(t->meta); free(t->parent);
+    free(t->counts); free(t->wide);
+}
+
+int orc_render_mixed(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *spheres, uint64_t ns,
+                     const uint32_t *sphere_surfaces, const float *lambertian, const float *metal,
+                     const float *dielectric, const float *emissive, const orc_render_settings *st,
+                     const float *cam20, float *pixels, uint64_t *ray_counts, int threads) {
+    if (n == 0 && ns == 0) return -3;
     uint64_t m = n > 1 ? n - 1 : 1;
     uint64_t *keys = (uint64_t *)malloc(n * sizeof(uint64_t));
     uint32_t *ids = (uint32_t *)malloc(n * sizeof(uint32_t));
@@ -610,7 +744,7 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
     uint32_t *counts = (uint32_t *)malloc(m * sizeof(uint32_t));
     float *nb = (float *)malloc(12 * m * sizeof(float));
     float scene12[12];
-    uint32_t root = orc_build_lbvh(vertices, n, keys, ids, meta, parent, counts, nb, scene12);
+    uint32_t root = n ? orc_build_lbvh(vertices, n, keys, ids, meta, parent, counts, nb, scene12) : ORC_SENTINEL;
     if (st->method == 1 && n > 1) {
         /* H-PLOC topology over the same sorted keys, as a depth-first binary layout */
         uint64_t m2 = 2 * n - 1;
@@ -637,7 +771,7 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
     sc.bvh.vertices = vertices; sc.bvh.leaf_ids = ids; sc.bvh.node_meta = meta; sc.bvh.node_bounds = nb; sc.bvh.root = root;
     sc.bvh.prim_count = n;
     void *wide = NULL;
-    if (st->node_width == 8) {
+    if (st->node_width == 8 && n > 0) {
         wide = malloc(n * 96);
         uint64_t wc = orc_collapse_wide(meta, nb, counts, n, root, scene12,
                                         st->leaf_size ? (uint32_t)st->leaf_size : 4u, wide, n);
@@ -647,7 +781,24 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
     }
     sc.surf = surfaces; sc.lamb = lambertian; sc.metal = metal; sc.diel = dielectric; sc.emit = emissive; sc.st = *st;
     memcpy(sc.cam, cam20, sizeof(sc.cam));
-    build_lights(vertices, n, surfaces, emissive, &sc.lights);
+    sc.ns = ns;
+    sc.srad = spheres;
+    sc.ssurf = sphere_surfaces;
+    sph_bvh_t sph_store;
+    memset(&sph_store, 0, sizeof(sph_store));
+    memset(&sc.sbvh, 0, sizeof(sc.sbvh));
+    if (ns && !build_sphere_bvh(spheres, ns, st, &sc.sbvh, &sph_store)) {
+        free_sphere_bvh(&sph_store);
+        free(wide); free(keys); free(ids); free(meta); free(parent); free(counts); free(nb);
+        return -1;
+    }
+    if (ns) {
+        /* signed radii are read per sphere from the caller's array: stride 4, radius at [3] */
+        float *sr = (float *)malloc(ns * sizeof(float));
+        for (uint64_t i = 0; i < ns; ++i) sr[i] = spheres[4 * i + 3];
+        sc.srad = sr;
+    }
+    build_lights_mixed(vertices, n, surfaces, spheres, ns, sphere_surfaces, emissive, &sc.lights);
     uint64_t total = (uint64_t)st->width * st->height * st->spp;
     float *samples = (float *)malloc(3 * total * sizeof(float));
     if (threads < 1) threads = 1;
@@ -670,6 +821,8 @@ int orc_render(const float *vertices, uint64_t n, const uint32_t *surfaces, cons
     if (ray_counts) { ray_counts[0] = rays[0]; ray_counts[1] = rays[1]; }
     free(samples); free(keys); free(ids); free(meta); free(parent); free(counts); free(nb); free(wide);
     free(sc.lights.fields); free(sc.lights.kinds);
+    if (ns) free((void *)sc.srad);
+    free_sphere_bvh(&sph_store);
     return 0;
 }
 
@@ -697,7 +850,10 @@ typedef struct {
     const float *v; uint64_t n; const uint32_t *surf; const float *lamb, *metal, *diel, *emit;
     double *lw; uint32_t *ltri; uint32_t nl; double total;
     orc_render_settings st; const float *cam;
+    const float *sph; uint64_t ns; const uint32_t *ssurf; uint8_t *lsph;  /* spheres (signed radii), light kind */
 } scene64;
+
+#define SPH_BIT (1ULL << 63)
 
 /* Russian roulette from depth 5 in double (survival max(T) in [0.05, 0.95]). */
 static int roulette64(pcg_t *rng, uint32_t b, d3 *T) {
@@ -779,6 +935,24 @@ static int hit64(const scene64 *s, d3 o, d3 d, double tmin, double tmax, double 
         double tt = ddot(e2, q) * inv;
         if (tt > tmin && tt < best) { best = tt; found = 1; *prim = i; *nrm = dnorm(dcross(e1, e2)); }
     }
+    /* spheres: the nearer root in (tmin, best), else the farther one; the
+     * outward normal (p - c) / r turns inward for a negative radius */
+    for (uint64_t i = 0; i < s->ns; ++i) {
+        const float *sp = s->sph + 4 * i;
+        d3 c = D(sp[0], sp[1], sp[2]);
+        double r = sp[3], ar = fabs(r);
+        d3 oc = dsub(o, c);
+        double a = ddot(d, d), hb = ddot(oc, d), cc = ddot(oc, oc) - ar * ar;
+        double disc = hb * hb - a * cc;
+        if (disc < 0) continue;
+        double sq = sqrt(disc), t0 = (-hb - sq) / a, t1 = (-hb + sq) / a, tt;
+        if (t0 > tmin && t0 < best) tt = t0;
+        else if (t1 > tmin && t1 < best) tt = t1;
+        else continue;
+        best = tt; found = 1; *prim = SPH_BIT | i;
+        d3 nn = dnorm(dsub(dadd(o, dsc(d, tt)), c));
+        *nrm = r < 0 ? dsc(nn, -1) : nn;
+    }
     *t_out = best;
     return found;
 }
@@ -815,7 +989,7 @@ static void render_path64(const scene64 *s, uint64_t path_id, pcg_t *rng, double
         int front = ddot(d, nrm) < 0;
         d3 n = front ? nrm : dsc(nrm, -1);
         d3 p = dadd(o, dsc(d, t));
-        uint32_t surf = s->surf[prim], kind = surf >> 28, mi = surf & IDX_MASK;
+        uint32_t surf = (prim & SPH_BIT) ? s->ssurf[prim & ~SPH_BIT] : s->surf[prim], kind = surf >> 28, mi = surf & IDX_MASK;
         if (alg == 1) { out[0] += 0.5 * (n.x + 1); out[1] += 0.5 * (n.y + 1); out[2] += 0.5 * (n.z + 1); return; }
         if (alg == 2) {
             double u1 = pcg_d(rng), u2 = pcg_d(rng);
@@ -845,17 +1019,32 @@ static void render_path64(const scene64 *s, uint64_t path_id, pcg_t *rng, double
         }
         d3 alb = D(s->lamb[3 * mi], s->lamb[3 * mi + 1], s->lamb[3 * mi + 2]);
         if (alg == 3 && s->nl > 0) {
-            /* light chosen by power, point uniform on the triangle */
+            /* light chosen by power, point uniform on the triangle or the sphere */
             double xi = pcg_d(rng) * s->total, accw = 0;
             uint32_t l = 0;
             while (l + 1 < s->nl && accw + s->lw[l] < xi) accw += s->lw[l++];
-            const float *lt = s->v + 9 * (uint64_t)s->ltri[l];
-            d3 p0 = D(lt[0], lt[1], lt[2]), e1 = dsub(D(lt[3], lt[4], lt[5]), p0), e2 = dsub(D(lt[6], lt[7], lt[8]), p0);
-            d3 av = dcross(e1, e2);
-            double area2 = sqrt(ddot(av, av));
-            d3 ln = dsc(av, 1.0 / area2);
-            double su = sqrt(pcg_d(rng)), b1 = su * (1 - pcg_d(rng)), b2 = su - b1;
-            d3 lp = dadd(dadd(p0, dsc(e1, b1)), dsc(e2, b2));
+            d3 ln, lp;
+            double area;
+            uint32_t lsurf;
+            if (s->lsph[l]) {
+                const float *sp = s->sph + 4 * (uint64_t)s->ltri[l];
+                double r = fabs(sp[3]), z = 1 - 2 * pcg_d(rng), th = 2 * M_PI * pcg_d(rng);
+                double rr = sqrt(fmax(0.0, 1 - z * z));
+                ln = D(rr * cos(th), rr * sin(th), z);
+                lp = dadd(D(sp[0], sp[1], sp[2]), dsc(ln, r));
+                area = 4 * M_PI * r * r;
+                lsurf = s->ssurf[s->ltri[l]];
+            } else {
+                const float *lt = s->v + 9 * (uint64_t)s->ltri[l];
+                d3 p0 = D(lt[0], lt[1], lt[2]), e1 = dsub(D(lt[3], lt[4], lt[5]), p0), e2 = dsub(D(lt[6], lt[7], lt[8]), p0);
+                d3 av = dcross(e1, e2);
+                double area2 = sqrt(ddot(av, av));
+                ln = dsc(av, 1.0 / area2);
+                double su = sqrt(pcg_d(rng)), b1 = su * (1 - pcg_d(rng)), b2 = su - b1;
+                lp = dadd(dadd(p0, dsc(e1, b1)), dsc(e2, b2));
+                area = 0.5 * area2;
+                lsurf = s->surf[s->ltri[l]];
+            }
             d3 tl = dsub(lp, p);
             double d2 = ddot(tl, tl), dist = sqrt(d2);
             d3 dir = dsc(tl, 1.0 / dist);
@@ -863,8 +1052,8 @@ static void render_path64(const scene64 *s, uint64_t path_id, pcg_t *rng, double
             if (sc > 0 && lc > 0 && dist - 0.002 > 0.001) {
                 double tt; uint64_t pp; d3 nn;
                 if (!hit64(s, p, dir, 0.001, dist - 0.002, &tt, &pp, &nn)) {
-                    const float *e = s->emit + 3 * (s->surf[s->ltri[l]] & IDX_MASK);
-                    double pdf_l = (s->lw[l] / s->total) * d2 / (lc * 0.5 * area2);
+                    const float *e = s->emit + 3 * (lsurf & IDX_MASK);
+                    double pdf_l = (s->lw[l] / s->total) * d2 / (lc * area);
                     double f = sc / (M_PI * pdf_l);
                     out[0] += T.x * alb.x * e[0] * f; out[1] += T.y * alb.y * e[1] * f; out[2] += T.z * alb.z * e[2] * f;
                 }
@@ -942,9 +1131,19 @@ void orc_ao_rays(const float *rays, const float *hits, uint64_t n, uint32_t samp
 int orc_render_fp64(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *lambertian,
                     const float *metal, const float *dielectric, const float *emissive,
                     const orc_render_settings *st, const float *cam20, double *pixels, int threads) {
-    scene64 s = {vertices, n, surfaces, lambertian, metal, dielectric, emissive, NULL, NULL, 0, 0.0, *st, cam20};
-    s.lw = (double *)malloc((n ? n : 1) * sizeof(double));
-    s.ltri = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    return orc_render_fp64_mixed(vertices, n, surfaces, NULL, 0, NULL, lambertian, metal, dielectric, emissive, st,
+                                 cam20, pixels, threads);
+}
+
+int orc_render_fp64_mixed(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *spheres,
+                          uint64_t ns, const uint32_t *sphere_surfaces, const float *lambertian, const float *metal,
+                          const float *dielectric, const float *emissive, const orc_render_settings *st,
+                          const float *cam20, double *pixels, int threads) {
+    scene64 s = {vertices, n, surfaces, lambertian, metal, dielectric, emissive, NULL, NULL, 0, 0.0, *st, cam20,
+                 spheres, ns, sphere_surfaces, NULL};
+    s.lw = (double *)malloc((n + ns ? n + ns : 1) * sizeof(double));
+    s.ltri = (uint32_t *)malloc((n + ns ? n + ns : 1) * sizeof(uint32_t));
+    s.lsph = (uint8_t *)calloc(n + ns ? n + ns : 1, 1);
     for (uint64_t i = 0; i < n; ++i) {
         if ((surfaces[i] >> 28) != MAT_EMIT) continue;
         const float *t = vertices + 9 * i;
@@ -953,6 +1152,13 @@ int orc_render_fp64(const float *vertices, uint64_t n, const uint32_t *surfaces,
         const float *e = emissive + 3 * (surfaces[i] & IDX_MASK);
         double w = 0.5 * sqrt(ddot(cr, cr)) * fmax((e[0] + (double)e[1] + e[2]) / 3.0, 0.0);
         if (w > 0) { s.lw[s.nl] = w; s.ltri[s.nl] = (uint32_t)i; s.total += w; s.nl++; }
+    }
+    for (uint64_t i = 0; i < ns; ++i) {
+        if ((sphere_surfaces[i] >> 28) != MAT_EMIT) continue;
+        double r = fabs(spheres[4 * i + 3]);
+        const float *e = emissive + 3 * (sphere_surfaces[i] & IDX_MASK);
+        double w = 4 * M_PI * r * r * fmax((e[0] + (double)e[1] + e[2]) / 3.0, 0.0);
+        if (w > 0) { s.lw[s.nl] = w; s.ltri[s.nl] = (uint32_t)i; s.lsph[s.nl] = 1; s.total += w; s.nl++; }
     }
     uint64_t pixels_n = (uint64_t)st->width * st->height;
     if (threads < 1) threads = 1;
@@ -965,6 +1171,6 @@ int orc_render_fp64(const float *vertices, uint64_t n, const uint32_t *surfaces,
         pthread_create(&tid[t], NULL, worker64, &jobs[t]);
     }
     for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
-    free(s.lw); free(s.ltri);
+    free(s.lw); free(s.ltri); free(s.lsph);
     return 0;
 }

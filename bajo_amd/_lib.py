@@ -199,6 +199,10 @@ SIGNATURES = {
     "bajo_scene_create_materials": (
         c_int32, [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, POINTER(BuildOptions), POINTER(c_void_p)],
     ),
+    "bajo_scene_create_batch": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, POINTER(BuildOptions), POINTER(c_void_p)],
+    ),
     "bajo_scene_create_mixed": (
         c_int32,
         [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, POINTER(BuildOptions),
@@ -221,6 +225,9 @@ SIGNATURES = {
     ),
     "bajo_render_target_create": (
         c_int32, [c_void_p, POINTER(RenderSettings), POINTER(c_float), c_void_p, POINTER(c_void_p)],
+    ),
+    "bajo_render_target_create_envs": (
+        c_int32, [c_void_p, POINTER(RenderSettings), c_uint32, POINTER(c_float), c_void_p, POINTER(c_void_p)],
     ),
     "bajo_render_target_set_camera": (c_int32, [c_void_p, c_void_p, POINTER(c_float)]),
     "bajo_render_enqueue": (c_int32, [c_void_p, c_void_p, c_void_p]),

@@ -34,6 +34,9 @@ constexpr uint32_t kMatLambertian = 0, kMatMetal = 1, kMatDielectric = 2, kMatEm
 constexpr uint32_t kSurfaceIndexMask = (1u << 28) - 1;
 constexpr uint32_t kPrimSphere = 0, kPrimTriangle = 1;  // bajo's PRIM values
 constexpr uint32_t kSphereHit = 1u;  // Hit.inst of a sphere hit in a mixed scene
+// scene geometry of the render kernels: triangles, spheres + triangles, or a
+// batch of triangle environments (one BVH segment per environment)
+constexpr int kGeoTriangles = 0, kGeoMixed = 1, kGeoBatch = 2;
 constexpr uint32_t kPathDelta = 0x80000000u, kPathIdMask = 0x7FFFFFFFu;
 constexpr int kLightStride = 14;
 constexpr int kPathPlanes = 12, kShadowPlanes = 11, kSamplePlanes = 3;
@@ -67,7 +70,25 @@ struct SceneView {
   const uint32_t* light_kinds;
   uint32_t light_count;
   float total_light_weight;
+  // A batch of environments (env_paths = 0 otherwise): path ids are
+  // environment-major, env_paths per environment; environment e traces BVH
+  // segment e (triangles [seg_off[e], seg_off[e+1])), samples the lights
+  // [env_light_base[e], env_light_base[e+1]) of total weight env_light_total[e]
+  // and draws its random numbers with key seed + e and its local path id, so
+  // each environment renders exactly as its scene would alone with that seed.
+  const uint64_t* seg_off;
+  const uint32_t* env_light_base;
+  const float* env_light_total;
+  uint32_t env_paths;
 };
+
+BJ_D StageRng path_rng(const SceneView& sc, uint64_t seed, uint32_t path_id, uint32_t stage) {
+  if (sc.env_paths) {
+    const uint32_t e = path_id / sc.env_paths;
+    return StageRng(seed + e, path_id - e * sc.env_paths, stage);
+  }
+  return StageRng(seed, path_id, stage);
+}
 
 struct Cam {
   float v[20];
@@ -111,15 +132,23 @@ BJ_D float power_heuristic(float a, float b) {
 struct PrimaryArgs {
   const float* cam;  // 20 floats in device memory (the frame's camera)
   uint32_t width, height, spp;
+  uint32_t env_paths;  // paths per environment of a batch (0: one scene)
 };
 
 // The camera path of `path_id`: origin, t_min, direction, t_max, throughput
 // 1 and pdf 0 in the path-plane order of the queues.
 BJ_D void primary_path(const PrimaryArgs& pa, uint32_t path_id, uint64_t seed, float (&f)[kPathPlanes]) {
   const uint32_t width = pa.width, height = pa.height, spp = pa.spp;
-  const uint32_t pixel = path_id / spp;
+  uint32_t local = path_id;
+  uint64_t key = seed;
+  if (pa.env_paths) {
+    const uint32_t e = path_id / pa.env_paths;
+    local = path_id - e * pa.env_paths;
+    key = seed + e;
+  }
+  const uint32_t pixel = local / spp;
   const uint32_t px = pixel % width, py = pixel / width;
-  StageRng rng(seed, path_id, 0u);
+  StageRng rng(key, local, 0u);
   // random_in_unit_disk (core/random.mojo:56-61), then the pixel jitter
   float s, c;
   sincos_turns(rng.next(), s, c);
@@ -272,18 +301,26 @@ BJ_D uint32_t wave_reserve(uint32_t* counter, bool want) {
 template <int kAlg>
 BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uint32_t bounce, v3 point, v3 normal,
                        uint32_t kind, uint32_t mi, v3 ray_d, v3 throughput, ShadowWork& sw) {
-  if (sc.light_count == 0 || !(sc.total_light_weight > 0.0f)) return false;
-  StageRng rng(seed, path_id, rng_light_stage(bounce));
-  const float alias_sample = rng.next() * (float)sc.light_count;
+  uint32_t lbase = 0, lcount = sc.light_count;
+  float ltotal = sc.total_light_weight;
+  if (sc.env_paths) {
+    const uint32_t e = path_id / sc.env_paths;
+    lbase = sc.env_light_base[e];
+    lcount = sc.env_light_base[e + 1] - lbase;
+    ltotal = sc.env_light_total[e];
+  }
+  if (lcount == 0 || !(ltotal > 0.0f)) return false;
+  StageRng rng = path_rng(sc, seed, path_id, rng_light_stage(bounce));
+  const float alias_sample = rng.next() * (float)lcount;
   uint32_t sel = (uint32_t)alias_sample;
-  if (sel >= sc.light_count) sel = sc.light_count - 1;
+  if (sel >= lcount) sel = lcount - 1;
   const float frac = alias_sample - (float)sel;
-  const uint32_t packed = sc.light_kinds[sel];
-  if (frac > sc.light_fields[sel * kLightStride + 13]) sel = packed >> 4;
-  const float* L = sc.light_fields + (uint64_t)sel * kLightStride;
+  const uint32_t packed = sc.light_kinds[lbase + sel];
+  if (frac > sc.light_fields[(uint64_t)(lbase + sel) * kLightStride + 13]) sel = packed >> 4;
+  const float* L = sc.light_fields + (uint64_t)(lbase + sel) * kLightStride;
   const v3 p0 = mk(L[0], L[1], L[2]);
   v3 ln, lp;
-  if ((sc.light_kinds[sel] & 0xFu) == kPrimSphere) {
+  if ((sc.light_kinds[lbase + sel] & 0xFu) == kPrimSphere) {
     // a sphere light: random_unit_vector (core/random.mojo:39-45) as the light normal
     float s, c;
     sincos_turns(rng.next(), s, c);
@@ -315,7 +352,7 @@ BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uin
   const float tmax = dist - 0.002f;
   if (tmax <= 0.001f) return false;
   const v3 E = mk(L[10], L[11], L[12]);
-  const float light_pdf = d2 * (E.x + E.y + E.z) / (3.0f * lcos * sc.total_light_weight);
+  const float light_pdf = d2 * (E.x + E.y + E.z) / (3.0f * lcos * ltotal);
   v3 value;
   float bsdf_pdf;
   if (kind == kMatMetal) {
@@ -404,7 +441,7 @@ BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint
                        const Surface& su, ShadowWork& sw) {
   if (!su.found) return false;
   if constexpr (kAlg == BAJO_RENDER_AO) {
-    StageRng rng(seed, path_id, 1u);
+    StageRng rng = path_rng(sc, seed, path_id, 1u);
     float s, c;
     sincos_turns(rng.next(), s, c);
     const float z = 1.0f - 2.0f * rng.next();
@@ -430,12 +467,12 @@ BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint
 // extend kernel and the tail kernel so both follow one definition.
 // russian_roulette (rt/common.mojo:43-67) at depth bounce + 1: from depth 5
 // a path survives with probability max(T) clamped to [0.05, 0.95].
-BJ_D bool survives_roulette(uint64_t seed, uint32_t path_id, uint32_t bounce, v3& Tn) {
+BJ_D bool survives_roulette(const SceneView& sc, uint64_t seed, uint32_t path_id, uint32_t bounce, v3& Tn) {
   if (bounce + 1u < 5u) return true;
   const float mx = fmax_sel(Tn.x, fmax_sel(Tn.y, Tn.z));
   if (mx <= 0.0f) return false;
   const float survival = fmin_sel(fmax_sel(mx, 0.05f), 0.95f);
-  StageRng rr(seed, path_id, rng_roulette_stage(bounce));
+  StageRng rr = path_rng(sc, seed, path_id, rng_roulette_stage(bounce));
   if (rr.next() >= survival) return false;
   Tn = mk(Tn.x / survival, Tn.y / survival, Tn.z / survival);
   return true;
@@ -479,9 +516,10 @@ BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, 
           if (bounce > 0 && !delta) {
             const float lc = fmax_sel(dot(su.n, normalize(mk(-ray.d.x, -ray.d.y, -ray.d.z))), 0.0f);
             float lpdf = 0.0f;
-            if (lc > 0.0f && sc.total_light_weight > 0.0f) {
+            const float ltotal = sc.env_paths ? sc.env_light_total[path_id / sc.env_paths] : sc.total_light_weight;
+            if (lc > 0.0f && ltotal > 0.0f) {
               const float d2 = t_hit * t_hit * length2(ray.d);
-              lpdf = d2 * (E.x + E.y + E.z) / (3.0f * lc * sc.total_light_weight);
+              lpdf = d2 * (E.x + E.y + E.z) / (3.0f * lc * ltotal);
             }
             w = power_heuristic(path_pdf, lpdf);
           }
@@ -492,7 +530,7 @@ BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, 
     } else if (su.kind == kMatLambertian) {
       // _sample_lambertian (rt/shading.mojo:92-118)
       const v3 n = su.n;
-      StageRng rng(seed, path_id, rng_bsdf_stage(bounce));
+      StageRng rng = path_rng(sc, seed, path_id, rng_bsdf_stage(bounce));
       const float ru = rng.next(), rv = rng.next();
       float s, c;
       sincos_turns(ru, s, c);
@@ -503,14 +541,14 @@ BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, 
       const v3 dir = normalize(sd);
       const float pdf = fmax_sel(dot(n, normalize(dir)), 0.0f) * kInvPi;
       v3 Tn = mul(T, albedo_of(sc, su));
-      if (survives_roulette(seed, path_id, bounce, Tn)) {
+      if (survives_roulette(sc, seed, path_id, bounce, Tn)) {
         next_path(np, path_id, false, ray, t_hit, dir, Tn, pdf);
         out |= kShadeNext;
       }
     } else if (su.kind == kMatMetal || su.kind == kMatDielectric) {
       // _gpu_rt_shade_one (rt/gpu/path_shading.mojo:669-783): random numbers
       // are drawn only where the sample uses them
-      StageRng rng(seed, path_id, rng_bsdf_stage(bounce));
+      StageRng rng = path_rng(sc, seed, path_id, rng_bsdf_stage(bounce));
       BsdfSample smp;
       if (su.kind == kMatMetal) {
         const float* m = sc.metals + 4 * (uint64_t)su.mi;
@@ -531,7 +569,7 @@ BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, 
       }
       if (smp.ok) {
         v3 Tn = mul(T, smp.weight);
-        if (survives_roulette(seed, path_id, bounce, Tn)) {
+        if (survives_roulette(sc, seed, path_id, bounce, Tn)) {
           next_path(np, path_id, smp.delta, ray, t_hit, smp.direction, Tn, smp.pdf);
           out |= kShadeNext;
         }
@@ -557,11 +595,22 @@ BJ_D v3 shadow_radiance(bool occluded, const float* f) {
 // clears the chunk's sample accumulators.
 // The scene's closest / any hit (rt/gpu/scene_trace.mojo:113-193): a mixed
 // scene traces its spheres first, then its triangles with t_max = the sphere
-// hit (a triangle must be strictly closer); any hit: either occludes.
-template <int kMode, bool kWide, bool kMixed>
-BJ_D Hit scene_trace(const SceneView& sc, const Ray& ray, Stack& st, TraceCounters* tc, bool any_rt = false) {
-  if constexpr (!kMixed) {
+// hit (a triangle must be strictly closer); any hit: either occludes.  In a
+// batch, environment `env` traces its own segment and the hit's triangle
+// index (local to the segment) becomes the batch's global index.
+template <int kMode, bool kWide, int kGeo>
+BJ_D Hit scene_trace(const SceneView& sc, const Ray& ray, Stack& st, TraceCounters* tc, bool any_rt = false,
+                     uint32_t env = 0) {
+  if constexpr (kGeo == kGeoTriangles) {
     return trace_accel<kMode, true, kWide>(sc.bvh, ray, st, tc, any_rt);
+  } else if constexpr (kGeo == kGeoBatch) {
+    AccelView av = sc.bvh;
+    const uint64_t first = sc.seg_off[env];
+    if constexpr (kWide) av.wide.root = env;  // segment roots lead the 8-wide node array
+    else av.bin.root = sc.seg_off[env + 1] - first == 1 ? (kLeafFlag | (uint32_t)first) : (uint32_t)first;
+    Hit h = trace_accel<kMode, true, kWide>(av, ray, st, tc, any_rt);
+    if (h.prim != kEmpty) h.prim += (uint32_t)first;
+    return h;
   } else {
     const bool any = kMode == BAJO_TRACE_ANY_HIT || (kMode == kTraceEither && any_rt);
     TraceCounters ts, tt;
@@ -585,7 +634,7 @@ BJ_D Hit scene_trace(const SceneView& sc, const Ray& ray, Stack& st, TraceCounte
   }
 }
 
-template <int kAlg, bool kWide, bool kFirst, bool kMixed>
+template <int kAlg, bool kWide, bool kFirst, int kGeo>
 __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                           PrimaryArgs pa, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
@@ -628,8 +677,11 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
         for (int p = 0; p < 8; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(idx, p)];
       }
       ray = Ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
+      uint32_t env = 0;
+      if constexpr (kGeo == kGeoBatch)
+        env = (kFirst ? q.sample_base + idx : (q.src_ids[idx] & kPathIdMask)) / sc.env_paths;
       TraceCounters tc;
-      const Hit h = scene_trace<BAJO_TRACE_CLOSEST_HIT, kWide, kMixed>(sc, ray, st, &tc);
+      const Hit h = scene_trace<BAJO_TRACE_CLOSEST_HIT, kWide, kGeo>(sc, ray, st, &tc, false, env);
       work_visits += tc.visits;
       work_leaves += tc.leaves;
       // throughput, pdf and id are read after the traversal rather than
@@ -688,7 +740,7 @@ __global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q
 // blocks of a launch take the same decision from the queue length; a
 // launch that runs marks the chunk done (tag = bounce + 1) so later extend,
 // shadow, advance and tail launches of the chunk do nothing.
-template <int kAlg, bool kWide, bool kMixed>
+template <int kAlg, bool kWide, int kGeo>
 __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                         uint32_t depth, uint32_t threshold, uint32_t simds,
                                                         unsigned long long* __restrict__ stats,
@@ -739,6 +791,9 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
 #pragma unroll
       for (int p = 0; p < kSamplePlanes; ++p) acc[p] = q.sample[plane_at<kSamplePlanes>(slot, p)];
     }
+    // both lanes of a pair work for the path's environment
+    uint32_t env = 0;
+    if constexpr (kGeo == kGeoBatch) env = __shfl(path_id, (int)(lane & ~1u), 64) / sc.env_paths;
     bool sh_live = false;  // shadow lane: a shadow ray of the previous bounce is pending
     float sf[kShadowPlanes];
 #pragma unroll
@@ -759,7 +814,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
       if (sh_live) ray = Ray{mk(sf[0], sf[1], sf[2]), mk(sf[4], sf[5], sf[6]), sf[3], sf[7]};
       if (alive || sh_live) {
         TraceCounters tc;
-        h = scene_trace<kTraceEither, kWide, kMixed>(sc, ray, st, &tc, sh_live);
+        h = scene_trace<kTraceEither, kWide, kGeo>(sc, ray, st, &tc, sh_live, env);
         if (alive) {
           ext_visits += tc.visits;
           ext_leaves += tc.leaves;
@@ -838,7 +893,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
   wave_add(stats + kWorkBase + 3, (uint32_t)sh_leaves);
 }
 
-template <int kAlg, bool kWide, bool kMixed>
+template <int kAlg, bool kWide, int kGeo>
 __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t bounce,
                                                           uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
@@ -856,10 +911,12 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
     float f[kShadowPlanes];
 #pragma unroll
     for (int p = 0; p < kShadowPlanes; ++p) f[p] = q.shadow_fields[plane_at<kShadowPlanes>(idx, p)];
-    const uint32_t slot = q.shadow_ids[idx] - q.sample_base;
+    const uint32_t sid = q.shadow_ids[idx];
+    const uint32_t slot = sid - q.sample_base;
     const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
     TraceCounters tc;
-    const Hit h = scene_trace<BAJO_TRACE_ANY_HIT, kWide, kMixed>(sc, ray, st, &tc);
+    const Hit h = scene_trace<BAJO_TRACE_ANY_HIT, kWide, kGeo>(sc, ray, st, &tc, false,
+                                                             kGeo == kGeoBatch ? sid / sc.env_paths : 0u);
     work_visits += tc.visits;
     work_leaves += tc.leaves;
     const bool occluded = h.prim == kEmpty && h.t == 0.0f;
@@ -931,6 +988,14 @@ struct bajo_scene {
   float* sphere_radii = nullptr;
   uint32_t* sphere_surfaces = nullptr;
   bajo_bvh* sphere_bvh = nullptr;
+  // a batch of environments: the BLAS set (segment e = environment e), the
+  // host offsets u64[S + 1], the environments' light ranges u32[S + 1] and
+  // total light weights f32[S]
+  uint32_t env_count = 0;
+  bajo_blas_set* set = nullptr;
+  std::vector<uint64_t> env_offsets;
+  uint32_t* env_light_base = nullptr;
+  float* env_light_total = nullptr;
 };
 
 namespace {
@@ -947,6 +1012,8 @@ void free_scene(bajo_scene* s) {
   if (!s) return;
   free_arena(s->arena);
   bajo_bvh_free(s->bvh);
+  if (s->set) bajo_blas_set_free(s->set);
+  (void)hipFree(s->env_light_base); (void)hipFree(s->env_light_total);
   bajo_bvh_free(s->sphere_bvh);
   (void)hipFree(s->spheres); (void)hipFree(s->sphere_radii); (void)hipFree(s->sphere_surfaces);
   (void)hipFree(s->vertices); (void)hipFree(s->surfaces); (void)hipFree(s->lambertian); (void)hipFree(s->emissive);
@@ -1060,9 +1127,15 @@ struct RenderShape {
   uint64_t spill_threads = 0;
 };
 
-template <int kAlg, bool kWide, bool kMixed>
-int32_t render_shape(const bajo_render_settings* st, RenderShape* out) {
-  const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
+// Paths of one render: every environment of a batch renders the full frame.
+uint64_t render_paths(const bajo_scene* s, const bajo_render_settings* st) {
+  return (uint64_t)st->image_width * st->image_height * st->samples_per_pixel * (s->env_count ? s->env_count : 1u);
+}
+
+template <int kAlg, bool kWide, int kGeo>
+int32_t render_shape(const bajo_scene* s, const bajo_render_settings* st, RenderShape* out) {
+  const uint64_t total = render_paths(s, st);
+  BJ_REQUIRE(total < (1ull << 31), "bajo_render: %llu paths exceed the 31-bit path id", (unsigned long long)total);
   uint64_t cap = st->path_capacity ? st->path_capacity : total;
   if (cap > total) cap = total;
   cap = (cap / st->samples_per_pixel) * st->samples_per_pixel;
@@ -1070,7 +1143,7 @@ int32_t render_shape(const bajo_render_settings* st, RenderShape* out) {
   constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
   unsigned g_tail = 0, simds = 0;
   if constexpr (kTail) {
-    int32_t rc = grid_of(tail_kernel<kAlg, kWide, kMixed>, &g_tail);
+    int32_t rc = grid_of(tail_kernel<kAlg, kWide, kGeo>, &g_tail);
     if (rc) return rc;
     int dev = 0, cus = 0;
     BJ_HIP(hipGetDevice(&dev));
@@ -1092,12 +1165,12 @@ int32_t render_shape(const bajo_render_settings* st, RenderShape* out) {
 // Enqueues one render into arena `a` (no host synchronization without `ev`).
 // `camera` (host, 20 floats) is written into the arena's camera buffer first;
 // NULL renders with the camera the buffer already holds.
-template <int kAlg, bool kWide, bool kMixed>
+template <int kAlg, bool kWide, int kGeo>
 int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st, const float* camera,
                    float* pixels, LaunchEvents* ev) {
-  const uint64_t total = (uint64_t)st->image_width * st->image_height * st->samples_per_pixel;
+  const uint64_t total = render_paths(s, st);
   RenderShape shape;
-  int32_t rc = render_shape<kAlg, kWide, kMixed>(st, &shape);
+  int32_t rc = render_shape<kAlg, kWide, kGeo>(s, st, &shape);
   if (rc) return rc;
   constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
   const uint64_t cap = shape.cap;
@@ -1118,8 +1191,10 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   pa.width = st->image_width;
   pa.height = st->image_height;
   pa.spp = st->samples_per_pixel;
+  pa.env_paths = s->env_count ? st->image_width * st->image_height * st->samples_per_pixel : 0u;
   SceneView sv;
-  sv.bvh = s->bvh ? accel_view(s->bvh, a.counters + 8) : empty_view(a.counters + 8);
+  const bajo_bvh* tri = s->set ? blas_set_store(s->set) : s->bvh;
+  sv.bvh = tri ? accel_view(tri, a.counters + 8) : empty_view(a.counters + 8);
   sv.spheres = s->sphere_bvh ? accel_view(s->sphere_bvh, a.counters + 8) : empty_view(a.counters + 8);
   sv.sphere_surfaces = s->sphere_surfaces;
   sv.sphere_radii = s->sphere_radii;
@@ -1132,6 +1207,10 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   sv.light_kinds = s->light_kinds;
   sv.light_count = s->light_count;
   sv.total_light_weight = s->total_light_weight;
+  sv.seg_off = s->set ? tri->seg_off : nullptr;
+  sv.env_light_base = s->env_light_base;
+  sv.env_light_total = s->env_light_total;
+  sv.env_paths = pa.env_paths;
   const uint32_t depth = (kAlg == BAJO_RENDER_NORMALS || kAlg == BAJO_RENDER_AO) ? 1u : st->max_depth;
   for (uint64_t begin = 0; begin < total; begin += cap) {
     const uint32_t chunk = (uint32_t)((total - begin) < cap ? (total - begin) : cap);
@@ -1143,22 +1222,22 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
       if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if constexpr (kTail) {
         if (b > 0) {
-          tail_kernel<kAlg, kWide, kMixed><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, tail_threshold,
+          tail_kernel<kAlg, kWide, kGeo><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, tail_threshold,
                                                                     simds, a.stats, a.spill);
           BJ_HIP(hipGetLastError());
         }
       }
       if (b == 0)
-        extend_kernel<kAlg, kWide, true, kMixed><<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(
+        extend_kernel<kAlg, kWide, true, kGeo><<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(
             sv, q, st->rng_seed, b, pa, a.spill, a.work_ext, a.stats);
       else
-        extend_kernel<kAlg, kWide, false, kMixed><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, a.spill,
+        extend_kernel<kAlg, kWide, false, kGeo><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, a.spill,
                                                                           a.work_ext, a.stats);
       BJ_HIP(hipGetLastError());
       if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
-        shadow_kernel<kAlg, kWide, kMixed><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats);
+        shadow_kernel<kAlg, kWide, kGeo><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats);
         BJ_HIP(hipGetLastError());
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
       }
@@ -1178,9 +1257,13 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
 // or spheres + triangles).
 template <int kAlg, typename F>
 int32_t by_scene(const bajo_scene* s, F&& fn) {
-  const bool wide = s->options.node_width == BAJO_WIDTH_WIDE8, mixed = s->n_spheres > 0;
-  if (mixed) return wide ? fn.template operator()<kAlg, true, true>() : fn.template operator()<kAlg, false, true>();
-  return wide ? fn.template operator()<kAlg, true, false>() : fn.template operator()<kAlg, false, false>();
+  const bool wide = s->options.node_width == BAJO_WIDTH_WIDE8;
+  if (s->env_count)
+    return wide ? fn.template operator()<kAlg, true, kGeoBatch>() : fn.template operator()<kAlg, false, kGeoBatch>();
+  if (s->n_spheres)
+    return wide ? fn.template operator()<kAlg, true, kGeoMixed>() : fn.template operator()<kAlg, false, kGeoMixed>();
+  return wide ? fn.template operator()<kAlg, true, kGeoTriangles>()
+              : fn.template operator()<kAlg, false, kGeoTriangles>();
 }
 
 template <typename F>
@@ -1198,8 +1281,8 @@ int32_t by_algorithm(const bajo_scene* s, int32_t algorithm, F&& fn) {
 // Sizes arena `a` for a render of `st` over scene `s` without enqueueing it.
 int32_t prepare_arena(const bajo_scene* s, RenderArena& a, const bajo_render_settings* st) {
   RenderShape shape;
-  const int32_t rc = by_algorithm(s, st->algorithm, [&]<int kAlg, bool kWide, bool kMixed>() {
-    return render_shape<kAlg, kWide, kMixed>(st, &shape);
+  const int32_t rc = by_algorithm(s, st->algorithm, [&]<int kAlg, bool kWide, int kGeo>() {
+    return render_shape<kAlg, kWide, kGeo>(s, st, &shape);
   });
   if (rc) return rc;
   return ensure_arena(a, shape.cap, shape.spill_threads);
@@ -1207,8 +1290,8 @@ int32_t prepare_arena(const bajo_scene* s, RenderArena& a, const bajo_render_set
 
 int32_t dispatch_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st,
                         const float* camera, float* pixels, LaunchEvents* ev) {
-  return by_algorithm(s, st->algorithm, [&]<int kAlg, bool kWide, bool kMixed>() {
-    return run_render<kAlg, kWide, kMixed>(ctx, s, a, st, camera, pixels, ev);
+  return by_algorithm(s, st->algorithm, [&]<int kAlg, bool kWide, int kGeo>() {
+    return run_render<kAlg, kWide, kGeo>(ctx, s, a, st, camera, pixels, ev);
   });
 }
 
@@ -1237,10 +1320,24 @@ int32_t bajo_scene_create_materials(bajo_ctx* ctx, const float* vertices, uint64
   return bajo_scene_create_mixed(ctx, vertices, n, surfaces, nullptr, 0, nullptr, materials, options, out);
 }
 
+namespace {
+int32_t scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces, const float* spheres,
+                     uint64_t n_spheres, const uint32_t* sphere_surfaces, const bajo_materials* materials,
+                     const bajo_build_options* options, bool build, bajo_scene** out);
+}  // namespace
+
 int32_t bajo_scene_create_mixed(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces,
                                 const float* spheres, uint64_t n_spheres, const uint32_t* sphere_surfaces,
                                 const bajo_materials* materials, const bajo_build_options* options,
                                 bajo_scene** out) {
+  return scene_create(ctx, vertices, n, surfaces, spheres, n_spheres, sphere_surfaces, materials, options, true, out);
+}
+
+namespace {
+// Validates and uploads a scene and its light table; builds its BVHs if `build`.
+int32_t scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* surfaces, const float* spheres,
+                     uint64_t n_spheres, const uint32_t* sphere_surfaces, const bajo_materials* materials,
+                     const bajo_build_options* options, bool build, bajo_scene** out) {
   BJ_REQUIRE(ctx && out && materials, "bajo_scene_create: NULL ctx/materials/out");
   BJ_REQUIRE(n + n_spheres > 0, "bajo_scene_create: a scene needs triangles or spheres");
   BJ_REQUIRE(n == 0 || (vertices && surfaces), "bajo_scene_create: triangles need their vertices and surfaces");
@@ -1376,7 +1473,99 @@ int32_t bajo_scene_create_mixed(bajo_ctx* ctx, const float* vertices, uint64_t n
     SC_HIP(hipMemcpy(s->sphere_surfaces, sphere_surfaces, n_spheres * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
 #undef SC_HIP
-  int32_t rc = bajo_scene_rebuild(ctx, s, nullptr);
+  if (build) {
+    const int32_t rc = bajo_scene_rebuild(ctx, s, nullptr);
+    if (rc) return fail(rc);
+  }
+  *out = s;
+  return BAJO_OK;
+}
+}  // namespace
+
+int32_t bajo_scene_create_batch(bajo_ctx* ctx, const float* vertices, const uint64_t* offsets, uint32_t env_count,
+                                const uint32_t* surfaces, const bajo_materials* materials,
+                                const bajo_build_options* options, bajo_scene** out) {
+  BJ_REQUIRE(ctx && vertices && offsets && surfaces && materials && out, "bajo_scene_create_batch: NULL argument");
+  BJ_REQUIRE(env_count >= 1, "bajo_scene_create_batch: a batch needs at least one environment");
+  BJ_REQUIRE(offsets[0] == 0, "bajo_scene_create_batch: offsets[0] must be 0");
+  for (uint32_t e = 0; e < env_count; ++e)
+    BJ_REQUIRE(offsets[e + 1] > offsets[e], "bajo_scene_create_batch: environment %u is empty", e);
+  const uint64_t n = offsets[env_count];
+  BJ_REQUIRE(n < (1ull << 31), "bajo_scene_create_batch: too many triangles");
+  const bajo_build_options opt = options ? *options : bajo_build_options{BAJO_BUILD_LBVH, BAJO_WIDTH_BINARY, 0, 0};
+  BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH, "bajo_scene_create_batch: segmented builds use the LBVH (method 0)");
+  // materials, surfaces and uploads as for one scene; the lights are then
+  // rebuilt per environment below
+  bajo_scene* s = nullptr;
+  int32_t rc = scene_create(ctx, vertices, n, surfaces, nullptr, 0, nullptr, materials, &opt, false, &s);
+  if (rc) return rc;
+  auto fail = [&](int32_t r) { free_scene(s); return r; };
+  (void)hipFree(s->light_fields); (void)hipFree(s->light_kinds);
+  s->light_fields = nullptr;
+  s->light_kinds = nullptr;
+  // _build_light_store (rt/types.mojo:490-530) per environment: its emissive
+  // triangles (weight = area * mean radiance) and its own alias table; the
+  // alias indices are local to the environment's light range
+  const float* emissive = materials->emissive;
+  std::vector<float> lf;
+  std::vector<uint32_t> kinds, base(env_count + 1, 0u);
+  std::vector<float> totals(env_count, 0.0f);
+  for (uint32_t e = 0; e < env_count; ++e) {
+    std::vector<float> weights;
+    std::vector<uint32_t> prims;
+    float total = 0.0f;
+    for (uint64_t i = offsets[e]; i < offsets[e + 1]; ++i) {
+      if ((surfaces[i] >> 28) != kMatEmissive) continue;
+      const float* t = vertices + 9 * i;
+      const v3 p0 = mk(t[0], t[1], t[2]), p1 = mk(t[3], t[4], t[5]), p2 = mk(t[6], t[7], t[8]);
+      const float area = 0.5f * sqrtf(length2(cross(sub(p1, p0), sub(p2, p0))));
+      const float* em = emissive + 3 * (surfaces[i] & kSurfaceIndexMask);
+      const float w = area * fmax_sel((em[0] + em[1] + em[2]) / 3.0f, 0.0f);
+      if (w > 0.0f) {
+        weights.push_back(w);
+        prims.push_back((uint32_t)i);
+        total += w;
+      }
+    }
+    std::vector<float> prob;
+    std::vector<uint32_t> alias;
+    build_alias(weights, total, prob, alias);
+    for (size_t l = 0; l < weights.size(); ++l) {
+      const float* t = vertices + 9 * (uint64_t)prims[l];
+      const float* em = emissive + 3 * (surfaces[prims[l]] & kSurfaceIndexMask);
+      float o[kLightStride] = {};
+      for (int k = 0; k < 9; ++k) o[k] = t[k];
+      o[10] = em[0]; o[11] = em[1]; o[12] = em[2];
+      o[13] = prob[l];
+      lf.insert(lf.end(), o, o + kLightStride);
+      kinds.push_back((alias[l] << 4) | kPrimTriangle);
+    }
+    base[e + 1] = base[e] + (uint32_t)weights.size();
+    totals[e] = total;
+  }
+  s->light_count = base[env_count];
+  s->total_light_weight = 0.0f;
+  if (lf.empty()) {
+    lf.assign(kLightStride, 0.0f);
+    kinds.assign(1, 0u);
+  }
+#define SB_HIP(call)                                              \
+  do {                                                            \
+    hipError_t e_ = (call);                                       \
+    if (e_ != hipSuccess) return fail(hip_fail(e_, #call, __FILE__, __LINE__)); \
+  } while (0)
+  SB_HIP(hipMalloc(&s->light_fields, lf.size() * sizeof(float)));
+  SB_HIP(hipMemcpy(s->light_fields, lf.data(), lf.size() * sizeof(float), hipMemcpyHostToDevice));
+  SB_HIP(hipMalloc(&s->light_kinds, kinds.size() * sizeof(uint32_t)));
+  SB_HIP(hipMemcpy(s->light_kinds, kinds.data(), kinds.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  SB_HIP(hipMalloc(&s->env_light_base, base.size() * sizeof(uint32_t)));
+  SB_HIP(hipMemcpy(s->env_light_base, base.data(), base.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  SB_HIP(hipMalloc(&s->env_light_total, totals.size() * sizeof(float)));
+  SB_HIP(hipMemcpy(s->env_light_total, totals.data(), totals.size() * sizeof(float), hipMemcpyHostToDevice));
+#undef SB_HIP
+  s->env_count = env_count;
+  s->env_offsets.assign(offsets, offsets + env_count + 1);
+  rc = bajo_scene_rebuild(ctx, s, nullptr);
   if (rc) return fail(rc);
   *out = s;
   return BAJO_OK;
@@ -1384,6 +1573,11 @@ int32_t bajo_scene_create_mixed(bajo_ctx* ctx, const float* vertices, uint64_t n
 
 int32_t bajo_scene_rebuild(bajo_ctx* ctx, bajo_scene* s, bajo_build_timings* timings) {
   BJ_REQUIRE(ctx && s, "bajo_scene_rebuild: NULL argument");
+  if (s->env_count) {
+    // every environment's BVH in one segmented build (rebuilt in place)
+    return bajo_build_triangle_blas_set(ctx, s->vertices, s->env_offsets.data(), s->env_count, &s->options, timings,
+                                        &s->set);
+  }
   if (s->n) {
     const int32_t rc = bajo_build_triangle_bvh_ex(ctx, s->vertices, s->n, &s->options, timings, &s->bvh);
     if (rc) return rc;
@@ -1495,7 +1689,8 @@ int32_t bajo_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st
 struct bajo_render_target {
   int device = 0;
   bajo_render_settings settings{};
-  float* pixels = nullptr;  // the caller's f32[3 w h] device buffer
+  float* pixels = nullptr;  // the caller's f32[3 w h E] device buffer
+  uint32_t environments = 1;  // scenes rendered through the target have this many environments
   RenderArena arena;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1523,13 +1718,23 @@ int32_t check_settings(const bajo_render_settings* st, const char* fn) {
 
 int32_t bajo_render_target_create(bajo_ctx* ctx, const bajo_render_settings* settings, const float* camera,
                                   float* pixels, bajo_render_target** out) {
+  return bajo_render_target_create_envs(ctx, settings, 1, camera, pixels, out);
+}
+
+int32_t bajo_render_target_create_envs(bajo_ctx* ctx, const bajo_render_settings* settings, uint32_t environments,
+                                       const float* camera, float* pixels, bajo_render_target** out) {
   BJ_REQUIRE(ctx && settings && pixels && out, "bajo_render_target_create: NULL argument");
+  BJ_REQUIRE(environments >= 1, "bajo_render_target_create: environments must be at least 1");
   int32_t rc = check_settings(settings, "bajo_render_target_create");
   if (rc) return rc;
+  const uint64_t paths = (uint64_t)settings->image_width * settings->image_height * settings->samples_per_pixel;
+  BJ_REQUIRE(paths * environments < (1ull << 31), "bajo_render_target_create: %llu paths exceed the 31-bit path id",
+             (unsigned long long)(paths * environments));
   bajo_render_target* t = new bajo_render_target();
   t->device = ctx->device;
   t->settings = *settings;
   t->pixels = pixels;
+  t->environments = environments;
   hipError_t e = hipMalloc(&t->arena.cam, 20 * sizeof(float));
   if (e != hipSuccess) {
     delete t;
@@ -1562,12 +1767,18 @@ int32_t bajo_render_target_set_camera(bajo_ctx* ctx, bajo_render_target* t, cons
 int32_t bajo_render_enqueue(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s) {
   BJ_REQUIRE(ctx && t && s, "bajo_render_enqueue: NULL argument");
   BJ_REQUIRE(t->device == ctx->device && s->device == ctx->device, "bajo_render_enqueue: device mismatch");
+  BJ_REQUIRE((s->env_count ? s->env_count : 1u) == t->environments,
+             "bajo_render_enqueue: the scene has %u environments, the target's pixels hold %u",
+             s->env_count ? s->env_count : 1u, t->environments);
   return dispatch_render(ctx, s, t->arena, &t->settings, nullptr, t->pixels, nullptr);
 }
 
 int32_t bajo_render_target_capture(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s) {
   BJ_REQUIRE(ctx && t && s, "bajo_render_target_capture: NULL argument");
   BJ_REQUIRE(t->device == ctx->device && s->device == ctx->device, "bajo_render_target_capture: device mismatch");
+  BJ_REQUIRE((s->env_count ? s->env_count : 1u) == t->environments,
+             "bajo_render_target_capture: the scene has %u environments, the target's pixels hold %u",
+             s->env_count ? s->env_count : 1u, t->environments);
   drop_graph(t);
   // every allocation happens before the capture
   int32_t rc = prepare_arena(s, t->arena, &t->settings);

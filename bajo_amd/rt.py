@@ -171,6 +171,56 @@ class GpuScene:
             pass
 
 
+class GpuSceneBatch(GpuScene):
+    """A batch of triangle environments rendered in one wavefront (the batched many-environment render;
+    bajo renders one scene per ``render_gpu`` call, rt/gpu/render.mojo:322-420).  Environment e is the
+    triangles [offsets[e], offsets[e+1]) of ``scene`` with its own lights; all share ``scene``'s
+    materials; their BVHs are one segmented LBVH build (bvh/gpu/builder/segmented_build.mojo:89-162).
+    A render writes f32[E, H, W, 3]; environment e's image equals a render of its scene alone with
+    seed ``rng_seed + e``."""
+
+    def __init__(self, scene: SceneData, offsets, ctx: DeviceContext | None = None, node_width: int = 2,
+                 leaf_size: int = 0):
+        from .bvh import build_options
+
+        self.ctx = ctx or default_context(0)
+        v = np.ascontiguousarray(scene.vertices, dtype=np.float32).reshape(-1)
+        s = np.ascontiguousarray(scene.surfaces, dtype=np.uint32).reshape(-1)
+        off = np.ascontiguousarray(offsets, dtype=np.uint64).reshape(-1)
+        lamb = np.ascontiguousarray(scene.lambertian, dtype=np.float32).reshape(-1)
+        emit = np.ascontiguousarray(scene.emissive, dtype=np.float32).reshape(-1)
+        met = np.ascontiguousarray(scene.metal, dtype=np.float32).reshape(-1)
+        die = np.ascontiguousarray(scene.dielectric, dtype=np.float32).reshape(-1)
+        if v.size % 9 or s.size != v.size // 9:
+            raise ValueError("one surface id per triangle is required")
+        if off.size < 2 or int(off[-1]) != s.size:
+            raise ValueError("offsets must be u64[E + 1] ending at the triangle count")
+        if np.asarray(scene.spheres).size:
+            raise ValueError("environment batches hold triangles only")
+        if lamb.size % 3 or emit.size % 3 or met.size % 4:
+            raise ValueError("materials are f32[3] per Lambertian / emissive and f32[4] per metal")
+        handle = ctypes.c_void_p()
+        opts = build_options(node_width, leaf_size, 0)
+        mats = _lib.Materials(
+            lamb.ctypes.data_as(ctypes.c_void_p), lamb.size // 3, met.ctypes.data_as(ctypes.c_void_p), met.size // 4,
+            die.ctypes.data_as(ctypes.c_void_p), die.size, emit.ctypes.data_as(ctypes.c_void_p), emit.size // 3,
+        )
+        _lib.check(_lib.lib().bajo_scene_create_batch(
+            self.ctx.handle, v.ctypes.data_as(ctypes.c_void_p), off.ctypes.data_as(ctypes.c_void_p), off.size - 1,
+            s.ctypes.data_as(ctypes.c_void_p), ctypes.byref(mats), ctypes.byref(opts), ctypes.byref(handle)))
+        self._keep = (lamb, emit, met, die)
+        self._handle = handle
+        self.triangle_count = s.size
+        self.sphere_count = 0
+        self.env_count = off.size - 1
+        self.offsets = off.copy()
+        self.node_width = node_width
+
+
+def _environments(scene) -> int:
+    return getattr(scene, "env_count", 1)
+
+
 @dataclass
 class RenderResult:
     pixels: torch.Tensor  # f32[H, W, 3] on the device
@@ -188,10 +238,12 @@ def enqueue_render_gpu(
     """Asynchronous render (rt/gpu/render.mojo:61-235): enqueues and returns."""
     ctx = scene.ctx
     w, h = settings.image_width, settings.image_height
+    envs = _environments(scene)
     if pixels is None:
-        pixels = torch.empty((h, w, 3), dtype=torch.float32, device=ctx.torch_device)
+        shape = (envs, h, w, 3) if isinstance(scene, GpuSceneBatch) else (h, w, 3)
+        pixels = torch.empty(shape, dtype=torch.float32, device=ctx.torch_device)
     require_device(pixels, ctx, "pixels")
-    if pixels.numel() < 3 * w * h:
+    if pixels.numel() < 3 * w * h * envs:
         raise ValueError("pixel buffer too small")
     cam = (ctypes.c_float * 20)(*[float(x) for x in np.asarray(camera, np.float32).reshape(-1)[:20]])
     st = settings.as_c(algorithm)
@@ -220,20 +272,23 @@ class GpuRtRenderTarget:
     (`capture` once, then `replay` per frame; `update_camera` between replays takes effect)."""
 
     def __init__(self, settings: RenderSettings, camera, algorithm: int = RENDER.PATH, ctx: DeviceContext | None = None,
-                 pixels: torch.Tensor | None = None):
+                 pixels: torch.Tensor | None = None, environments: int = 1):
         self.ctx = ctx or default_context(0)
         w, h = settings.image_width, settings.image_height
-        self.pixels = pixels if pixels is not None else torch.empty((h, w, 3), dtype=torch.float32,
+        shape = (h, w, 3) if environments == 1 else (environments, h, w, 3)
+        self.pixels = pixels if pixels is not None else torch.empty(shape, dtype=torch.float32,
                                                                    device=self.ctx.torch_device)
         require_device(self.pixels, self.ctx, "pixels")
-        if self.pixels.numel() < 3 * w * h:
+        if self.pixels.numel() < 3 * w * h * environments:
             raise ValueError("pixel buffer too small")
         self.settings = settings
         self.algorithm = algorithm
+        self.environments = environments
         st = settings.as_c(algorithm)
         self._handle = ctypes.c_void_p()
-        _lib.check(_lib.lib().bajo_render_target_create(self.ctx.handle, ctypes.byref(st), _cam20(camera),
-                                                        ptr(self.pixels), ctypes.byref(self._handle)))
+        _lib.check(_lib.lib().bajo_render_target_create_envs(self.ctx.handle, ctypes.byref(st), environments,
+                                                             _cam20(camera), ptr(self.pixels),
+                                                             ctypes.byref(self._handle)))
 
     @property
     def handle(self):

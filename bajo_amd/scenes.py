@@ -249,3 +249,37 @@ def sphere_field_scene(n: int = 2000, seed: int = 13, emission: float = 4.0):
     return SceneData(np.zeros(0, np.float32), np.zeros(0, np.uint32), lamb.reshape(-1),
                      np.array([emission] * 3, np.float32), metal.reshape(-1), np.array([1.5], np.float32),
                      spheres.astype(np.float32), sphere_surfaces)
+
+
+def batched_environments(count: int = 4096, seed: int = 2031, lo: int = 256, hi: int = 65536, emission: float = 6.0,
+                         soups=None):
+    """The batched-environment render scenes: environment k is soup k of `batched_soups(count, seed, lo, hi)`
+    (Lambertian, albedo U[0.2, 0.8]^3 per environment from numpy's PCG64 stream seed + 1) under one
+    downward-facing quad light over the unit cube (y = 1.2, x and z in [0.2, 0.8], the given emission).
+    Returns (SceneData of every environment back to back, offsets u64[count + 1]).  `soups` may pass the
+    result of `batched_soups(count, seed, lo, hi)` computed before."""
+    from .rt import MAT, SceneData, surface_id
+
+    soups, soup_off = soups if soups is not None else batched_soups(count, seed, lo, hi)
+    light = quad((0.2, 1.2, 0.2), (0.8, 1.2, 0.2), (0.8, 1.2, 0.8), (0.2, 1.2, 0.8))
+    n = np.diff(soup_off.astype(np.int64))
+    offsets = np.concatenate([[0], np.cumsum(n + 2)]).astype(np.uint64)
+    tri = np.empty((int(offsets[-1]), 9), np.float32)
+    surf = np.empty(int(offsets[-1]), np.uint32)
+    src = soups.reshape(-1, 9)
+    for k in range(count):
+        a, b = int(offsets[k]), int(offsets[k + 1])
+        tri[a:b - 2] = src[int(soup_off[k]):int(soup_off[k + 1])]
+        tri[b - 2:b] = light.reshape(2, 9)
+        surf[a:b - 2] = surface_id(MAT.LAMBERTIAN, k)
+        surf[b - 2:b] = surface_id(MAT.EMISSIVE, 0)
+    albedo = np.random.default_rng(seed + 1).uniform(0.2, 0.8, (count, 3)).astype(np.float32)
+    return SceneData(tri.reshape(-1), surf, albedo.reshape(-1), np.array([emission] * 3, np.float32)), offsets
+
+
+def environment(batch, offsets, k: int):
+    """Environment k of a batch as a scene of its own (materials shared)."""
+    from dataclasses import replace
+
+    a, b = int(offsets[k]), int(offsets[k + 1])
+    return replace(batch, vertices=batch.vertices[9 * a:9 * b].copy(), surfaces=batch.surfaces[a:b].copy())

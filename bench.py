@@ -415,8 +415,76 @@ def batched_envs_workload(ctx, count: int = 4096, width: int = 64, height: int =
         print(f"[bench] batched envs {key}: set build {build_ms:.2f} ms + batched trace {trace_ms:.2f} ms vs "
               f"per-environment loop {loop_ms:.1f} ms", file=sys.stderr, flush=True)
     out["note"] = ("the loop's time includes its host launch overhead (2 library calls per environment); the "
-                   "NEE render of each environment is not part of this workload")
+                   "NEE render of every environment is the batched_render workload")
     return out
+
+
+def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int = 64, spp: int = 4,
+                            depth: int = 3, soups=None) -> dict:
+    """The batched-environment render: the `count` soups of the batched-environment shape, each under its own
+    quad light (emission 6) with its own Lambertian albedo, rendered with NEE at width x height x spp, depth 3,
+    through one wavefront over one segmented 8-wide LBVH build (environment e with seed 2026 + e).  Timed three
+    ways over the same environments: one batched render call, the batched render captured in a HIP graph and
+    replayed (render target), and a per-environment loop of render calls over the environments built as scenes
+    of their own (scene creation untimed)."""
+    import torch
+
+    from bajo_amd import scenes
+    from bajo_amd.bvh import Camera
+    from bajo_amd.rt import RENDER, GpuRtRenderTarget, GpuScene, GpuSceneBatch, RenderSettings, enqueue_render_gpu
+
+    dev = ctx.torch_device
+    stream = torch.cuda.current_stream(dev)
+    sc, off = scenes.batched_environments(count, soups=soups)
+    n = int(off[-1])
+    cam = Camera((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 0.41421356).flatten()
+    st = RenderSettings(width, height, spp, 2026, depth)
+    print(f"[bench] batched render: {count} environments, {n} triangles", file=sys.stderr, flush=True)
+    g = GpuSceneBatch(sc, off, ctx, node_width=8, leaf_size=4)
+    first = enqueue_render_gpu(st, cam, g, RENDER.NEE, with_stats=True)
+    rays = first.stats["extension_rays"] + first.stats["shadow_rays"]
+    pixels = first.pixels
+    render_ms = _timed(lambda: enqueue_render_gpu(st, cam, g, RENDER.NEE, pixels=pixels), stream)
+    t = GpuRtRenderTarget(st, cam, RENDER.NEE, ctx, environments=count)
+    t.capture(g)
+    replay_ms = _timed(t.replay, stream)
+    same = bool(torch.equal(t.pixels, pixels))
+    t.free()
+    g.free()
+    print(f"[bench] batched render {render_ms:.2f} ms, graph replay {replay_ms:.2f} ms", file=sys.stderr, flush=True)
+    # the per-environment loop: every environment a scene of its own, one render call each; the scenes are
+    # made (untimed, with one untimed render that sizes each scene's path arena) in groups of 256 to bound
+    # device memory, and the loop's wall time is summed over the groups
+    outs = torch.empty((count, height, width, 3), dtype=torch.float32, device=dev)
+    loop_ms = 0.0
+    for g0 in range(0, count, 256):
+        ks = range(g0, min(count, g0 + 256))
+        envs = {k: GpuScene(scenes.environment(sc, off, k), ctx, node_width=8, leaf_size=4) for k in ks}
+
+        def loop():
+            for k in ks:
+                enqueue_render_gpu(RenderSettings(width, height, spp, 2026 + k, depth), cam, envs[k], RENDER.NEE,
+                                   pixels=outs[k])
+
+        loop()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loop()
+        torch.cuda.synchronize()
+        loop_ms += (time.perf_counter() - t0) * 1e3
+        for e in envs.values():
+            e.free()
+    loop_equal = bool(torch.equal(outs, pixels))
+    print(f"[bench] per-environment render loop {loop_ms:.1f} ms (images equal: {loop_equal})", file=sys.stderr,
+          flush=True)
+    return {"workload": f"{count} environments ({n} triangles: soups log-uniform in [256, 65536] plus a 2-triangle "
+            f"quad light each), NEE {width}x{height}x{spp} spp, depth {depth}, 8-wide segmented LBVH",
+            "paths": count * width * height * spp, "rays": rays, "extension_rays": first.stats["extension_rays"],
+            "shadow_rays": first.stats["shadow_rays"], "batched_render_ms": render_ms,
+            "batched_render_mrays_s": rays / (render_ms / 1e3) / 1e6, "graph_replay_ms": replay_ms,
+            "graph_replay_mrays_s": rays / (replay_ms / 1e3) / 1e6, "graph_equals_render": same,
+            "per_env_loop_ms": loop_ms, "per_env_loop_mrays_s": rays / (loop_ms / 1e3) / 1e6,
+            "loop_images_equal_batch": loop_equal, "speedup_vs_loop": loop_ms / render_ms}
 
 
 def render_target_workload(ctx) -> dict:
@@ -529,7 +597,8 @@ def main() -> int:
         print(json.dumps({"tlas": tlas_workload(ctx)}), flush=True)
         return 0
     if args.batched_only:
-        print(json.dumps({"batched_envs": batched_envs_workload(ctx)}), flush=True)
+        print(json.dumps({"batched_envs": batched_envs_workload(ctx), "batched_render": batched_render_workload(ctx)}),
+              flush=True)
         return 0
     if args.target_only:
         print(json.dumps({"render_target": render_target_workload(ctx)}), flush=True)
@@ -656,6 +725,7 @@ def main() -> int:
         line["soup_grid_16m"] = soup_grid_workload(ctx)
         line["tlas"] = tlas_workload(ctx)
         line["batched_envs"] = batched_envs_workload(ctx)
+        line["batched_render"] = batched_render_workload(ctx)
         line["render_target"] = render_target_workload(ctx)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, cam, min(16, os.cpu_count() or 1))

@@ -386,6 +386,22 @@ int32_t bajo_scene_create_mixed(bajo_ctx* ctx, const float* vertices, uint64_t n
                                 const float* spheres, uint64_t n_spheres, const uint32_t* sphere_surfaces,
                                 const bajo_materials* materials, const bajo_build_options* options,
                                 bajo_scene** out);
+/* A batch of environments (the batched many-environment render; bajo has no
+ * counterpart, its closest code is the segmented BLAS build,
+ * bvh/gpu/builder/segmented_build.mojo:89-162, and render_gpu per scene,
+ * rt/gpu/render.mojo:322-420): environment e is the triangles
+ * [offsets[e], offsets[e+1]) of `vertices` (host f32[9N], surfaces u32[N]
+ * sharing `materials`), with its own lights (its emissive triangles, its own
+ * alias table).  All environments are built in one segmented LBVH pass
+ * (options: method LBVH, width 2 or 8).  bajo_render of a batch renders
+ * every environment through the same camera and settings into pixels
+ * f32[E h w 3] (environment-major) in one wavefront; environment e's image
+ * equals a render of its scene alone with seed rng_seed + e.  A render
+ * target renders a batch only if made by bajo_render_target_create_envs with
+ * the same environment count. */
+int32_t bajo_scene_create_batch(bajo_ctx* ctx, const float* vertices, const uint64_t* offsets, uint32_t env_count,
+                                const uint32_t* surfaces, const bajo_materials* materials,
+                                const bajo_build_options* options, bajo_scene** out);
 int32_t bajo_scene_rebuild(bajo_ctx* ctx, bajo_scene* scene, bajo_build_timings* timings);
 int32_t bajo_scene_free(bajo_scene* scene);
 int32_t bajo_scene_get_bvh(const bajo_scene* scene, bajo_bvh** out);
@@ -418,6 +434,10 @@ typedef struct bajo_render_target bajo_render_target;
 int32_t bajo_render_target_create(bajo_ctx* ctx, const bajo_render_settings* settings, const float* camera,
                                   float* pixels, bajo_render_target** out);
 int32_t bajo_render_target_set_camera(bajo_ctx* ctx, bajo_render_target* target, const float* camera);
+/* A render target whose pixels (f32[E h w 3]) hold `environments` images:
+ * it renders batches of that many environments (bajo_scene_create_batch). */
+int32_t bajo_render_target_create_envs(bajo_ctx* ctx, const bajo_render_settings* settings, uint32_t environments,
+                                       const float* camera, float* pixels, bajo_render_target** out);
 int32_t bajo_render_enqueue(bajo_ctx* ctx, bajo_render_target* target, bajo_scene* scene);
 int32_t bajo_render_target_capture(bajo_ctx* ctx, bajo_render_target* target, bajo_scene* scene);
 int32_t bajo_render_target_replay(bajo_ctx* ctx, bajo_render_target* target);

@@ -1,0 +1,123 @@
+"""Batched environments: many triangle scenes rendered in one wavefront over one
+segmented BVH build (bajo has no counterpart; its closest code is the segmented
+BLAS build, bvh/gpu/builder/segmented_build.mojo:89-162, and render_gpu per scene,
+rt/gpu/render.mojo:322-420).  Environment e of a batch render is bit-identical
+to a render of its scene alone with seed + e (the per-scene launch loop), at
+both node widths; it matches the oracle renderer (image RMSE <= 1e-3, > 99 %
+bit-identical pixels); ray counts add up; a render target replays a batch
+(HIP graph) bit-identically and refuses a batch of another size."""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+ALGS = {"PATH": 0, "NEE": 3, "MIS": 4}
+W, H, SPP, DEPTH, SEED = 24, 16, 4, 3, 4242
+
+
+def camera():
+    from bajo_amd.bvh import Camera
+
+    return Camera((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 0.41421356).flatten()
+
+
+@pytest.fixture(scope="module")
+def batch():
+    from bajo_amd import scenes
+
+    return scenes.batched_environments(count=9, seed=77, lo=40, hi=900)
+
+
+def settings():
+    from bajo_amd.rt import RenderSettings
+
+    return RenderSettings(W, H, SPP, SEED, DEPTH)
+
+
+@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("name", ["PATH", "NEE", "MIS"])
+def test_batch_equals_per_scene_loop(gpu_ctx, batch, width, name):
+    from bajo_amd import scenes
+    from bajo_amd.rt import GpuScene, GpuSceneBatch, RenderSettings, enqueue_render_gpu
+
+    announce(f"batch {name} width {width}")
+    sc, off = batch
+    g = GpuSceneBatch(sc, off, gpu_ctx, node_width=width)
+    res = enqueue_render_gpu(settings(), camera(), g, ALGS[name], with_stats=True)
+    got = res.numpy()
+    assert got.shape == (9, H, W, 3)
+    ext = 0
+    for e in range(9):
+        one = GpuScene(scenes.environment(sc, off, e), gpu_ctx, node_width=width)
+        r = enqueue_render_gpu(RenderSettings(W, H, SPP, SEED + e, DEPTH), camera(), one, ALGS[name],
+                               with_stats=True)
+        np.testing.assert_array_equal(got[e], r.numpy(), err_msg=f"environment {e}")
+        ext += r.stats["extension_rays"]
+        one.free()
+    assert res.stats["extension_rays"] == ext
+    assert res.stats["status"] == 0 and res.stats["traversal_status"] == 0
+    g.free()
+
+
+@pytest.mark.parametrize("name", ["NEE", "MIS"])
+def test_batch_environments_match_oracle(gpu_ctx, batch, name):
+    from bajo_amd import scenes
+    from bajo_amd.rt import GpuSceneBatch, enqueue_render_gpu
+
+    announce(f"batch vs oracle {name}")
+    sc, off = batch
+    g = GpuSceneBatch(sc, off, gpu_ctx, node_width=8)
+    got = enqueue_render_gpu(settings(), camera(), g, ALGS[name]).numpy()
+    for e in (0, 4, 8):
+        env = scenes.environment(sc, off, e)
+        want, _ = oracle.render(env.vertices, env.surfaces, env.lambertian, env.emissive, W, H, SPP, DEPTH, SEED + e,
+                                ALGS[name], camera(), 8, node_width=8, leaf_size=4)
+        img = got[e].reshape(-1)
+        rmse = float(np.sqrt(np.mean((img.astype(np.float64) - want) ** 2)))
+        exact = float(np.mean(img.view(np.uint32) == want.view(np.uint32)))
+        print(f"environment {e}: rmse {rmse:.3e}, bit-identical {exact:.4f}")
+        assert rmse <= 1e-3 and exact > 0.99
+    g.free()
+
+
+def test_batch_light_table_is_per_environment(gpu_ctx, batch):
+    from bajo_amd import scenes
+    from bajo_amd.rt import GpuSceneBatch
+
+    announce("batch light table")
+    sc, off = batch
+    g = GpuSceneBatch(sc, off, gpu_ctx)
+    f, k, _ = g.light_table()
+    assert k.size == 2 * 9
+    for e in range(9):
+        env = scenes.environment(sc, off, e)
+        wf, wk, _ = oracle.light_table(env.vertices, env.surfaces, env.emissive)
+        np.testing.assert_array_equal(k[2 * e:2 * e + 2], wk)
+        np.testing.assert_array_equal(f[28 * e:28 * e + 28], wf)
+    g.free()
+
+
+def test_batch_render_target_replay(gpu_ctx, batch):
+    import torch
+
+    from bajo_amd.rt import GpuRtRenderTarget, GpuSceneBatch, enqueue_render_gpu
+
+    announce("batch render target")
+    sc, off = batch
+    g = GpuSceneBatch(sc, off, gpu_ctx, node_width=8)
+    want = enqueue_render_gpu(settings(), camera(), g, ALGS["NEE"]).numpy()
+    t = GpuRtRenderTarget(settings(), camera(), ALGS["NEE"], gpu_ctx, environments=9)
+    t.capture(g)
+    for _ in range(2):
+        got = t.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(got.cpu().numpy(), want)
+    t.free()
+    single = GpuRtRenderTarget(settings(), camera(), ALGS["NEE"], gpu_ctx)
+    with pytest.raises(RuntimeError, match="environments"):
+        single.enqueue(g)
+    single.free()
+    g.free()

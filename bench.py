@@ -420,7 +420,7 @@ def batched_envs_workload(ctx, count: int = 4096, width: int = 64, height: int =
 
 
 def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int = 64, spp: int = 4,
-                            depth: int = 3, soups=None) -> dict:
+                            depth: int = 3, soups=None, per_env_loop: bool = True) -> dict:
     """The batched-environment render: the `count` soups of the batched-environment shape, each under its own
     quad light (emission 6) with its own Lambertian albedo, rendered with NEE at width x height x spp, depth 3,
     through one wavefront over one segmented 8-wide LBVH build (environment e with seed 2026 + e).  Timed three
@@ -452,6 +452,14 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
     t.free()
     g.free()
     print(f"[bench] batched render {render_ms:.2f} ms, graph replay {replay_ms:.2f} ms", file=sys.stderr, flush=True)
+    out = {"workload": f"{count} environments ({n} triangles: soups log-uniform in [256, 65536] plus a 2-triangle "
+           f"quad light each), NEE {width}x{height}x{spp} spp, depth {depth}, 8-wide segmented LBVH",
+           "paths": count * width * height * spp, "rays": rays, "extension_rays": first.stats["extension_rays"],
+           "shadow_rays": first.stats["shadow_rays"], "batched_render_ms": render_ms,
+           "batched_render_mrays_s": rays / (render_ms / 1e3) / 1e6, "graph_replay_ms": replay_ms,
+           "graph_replay_mrays_s": rays / (replay_ms / 1e3) / 1e6, "graph_equals_render": same}
+    if not per_env_loop:
+        return out
     # the per-environment loop: every environment a scene of its own, one render call each; the scenes are
     # made (untimed, with one untimed render that sizes each scene's path arena) in groups of 256 to bound
     # device memory, and the loop's wall time is summed over the groups
@@ -477,14 +485,9 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
     loop_equal = bool(torch.equal(outs, pixels))
     print(f"[bench] per-environment render loop {loop_ms:.1f} ms (images equal: {loop_equal})", file=sys.stderr,
           flush=True)
-    return {"workload": f"{count} environments ({n} triangles: soups log-uniform in [256, 65536] plus a 2-triangle "
-            f"quad light each), NEE {width}x{height}x{spp} spp, depth {depth}, 8-wide segmented LBVH",
-            "paths": count * width * height * spp, "rays": rays, "extension_rays": first.stats["extension_rays"],
-            "shadow_rays": first.stats["shadow_rays"], "batched_render_ms": render_ms,
-            "batched_render_mrays_s": rays / (render_ms / 1e3) / 1e6, "graph_replay_ms": replay_ms,
-            "graph_replay_mrays_s": rays / (replay_ms / 1e3) / 1e6, "graph_equals_render": same,
-            "per_env_loop_ms": loop_ms, "per_env_loop_mrays_s": rays / (loop_ms / 1e3) / 1e6,
-            "loop_images_equal_batch": loop_equal, "speedup_vs_loop": loop_ms / render_ms}
+    out.update({"per_env_loop_ms": loop_ms, "per_env_loop_mrays_s": rays / (loop_ms / 1e3) / 1e6,
+                "loop_images_equal_batch": loop_equal, "speedup_vs_loop": loop_ms / render_ms})
+    return out
 
 
 def render_target_workload(ctx) -> dict:
@@ -568,6 +571,8 @@ def main() -> int:
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-environment workload and print its JSON")
     ap.add_argument("--target-only", action="store_true", help="run only the render-target workload")
+    ap.add_argument("--batched-render-only", action="store_true",
+                    help="run only the batched render (without its per-environment loop, for profiling it)")
     ap.add_argument(
         "--copy-calibration-mib",
         type=int,
@@ -599,6 +604,9 @@ def main() -> int:
     if args.batched_only:
         print(json.dumps({"batched_envs": batched_envs_workload(ctx), "batched_render": batched_render_workload(ctx)}),
               flush=True)
+        return 0
+    if args.batched_render_only:
+        print(json.dumps({"batched_render": batched_render_workload(ctx, per_env_loop=False)}), flush=True)
         return 0
     if args.target_only:
         print(json.dumps({"render_target": render_target_workload(ctx)}), flush=True)

@@ -254,12 +254,14 @@ class GpuTriangleBlasSet:
     as a ready-to-trace BVH that shares the set's storage (bajo's BlasDesc); it stays valid until the set
     is freed."""
 
-    def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, counts: list, node_width: int, leaf_size: int):
+    def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, counts: list, node_width: int, leaf_size: int,
+                 method: int = 0):
         self._handle = handle
         self.ctx = ctx
         self.counts = list(counts)
         self.node_width = node_width
         self.leaf_size = leaf_size
+        self.method = method
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -274,7 +276,7 @@ class GpuTriangleBlasSet:
     def blas(self, k: int) -> GpuTriangleBvh:
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().bajo_blas_set_get(self.handle, int(k), ctypes.byref(h)))
-        view = GpuTriangleBvh(h, self.ctx, self.counts[k], self.node_width, self.leaf_size)
+        view = GpuTriangleBvh(h, self.ctx, self.counts[k], self.node_width, self.leaf_size, self.method)
         view._owner = self
         return view
 
@@ -286,7 +288,8 @@ class GpuTriangleBlasSet:
         if len(offsets) - 1 != self.blas_count:
             raise ValueError("a rebuild keeps the mesh count")
         t = {} if timings else None
-        _build_set(vertices, offsets, self.ctx, self.node_width, self.leaf_size, t, ctypes.c_void_p(self.handle.value))
+        _build_set(vertices, offsets, self.ctx, self.node_width, self.leaf_size, t, ctypes.c_void_p(self.handle.value),
+                   self.method)
         self.counts = np.diff(np.asarray(offsets, np.int64)).tolist()
         return t
 
@@ -315,13 +318,13 @@ class GpuTriangleBlasSet:
 
 
 def _build_set(flat: torch.Tensor, offsets: np.ndarray, ctx: DeviceContext, node_width: int, leaf_size: int,
-               timings: dict | None, handle: ctypes.c_void_p) -> ctypes.c_void_p:
+               timings: dict | None, handle: ctypes.c_void_p, method: int = 0) -> ctypes.c_void_p:
     _check_vertices(flat, ctx)
     off = np.ascontiguousarray(offsets, np.uint64)
     if off.size < 2 or int(off[-1]) * 9 != flat.numel():
         raise ValueError("segment offsets must end at the triangle count of the vertex buffer")
     t = _lib.BuildTimings() if timings is not None else None
-    opts = build_options(node_width, leaf_size, GpuBvhBuildMethod.LBVH)
+    opts = build_options(node_width, leaf_size, method)
     _lib.check(_lib.lib().bajo_build_triangle_blas_set(
         ctx.handle, ptr(flat), off.ctypes.data_as(ctypes.c_void_p), off.size - 1, ctypes.byref(opts),
         ctypes.byref(t) if t else None, ctypes.byref(handle)))
@@ -331,27 +334,30 @@ def _build_set(flat: torch.Tensor, offsets: np.ndarray, ctx: DeviceContext, node
 
 
 def build_triangle_blas_set(vertex_sets: list, ctx: DeviceContext | None = None, node_width: int = 2,
-                            leaf_size: int = 0, timings: dict | None = None) -> GpuTriangleBlasSet:
-    """build_triangle_blas_set[W, L, LBVH] (bvh/gpu/triangle_bvh.mojo:469-491): one segmented LBVH build over
-    the meshes `vertex_sets` (float32 tensors of 9 floats per triangle on one device); node_width 2 or 8."""
+                            leaf_size: int = 0, timings: dict | None = None,
+                            method: int = GpuBvhBuildMethod.LBVH) -> GpuTriangleBlasSet:
+    """build_triangle_blas_set[W, L, METHOD] (bvh/gpu/triangle_bvh.mojo:469-491): one segmented build (LBVH or
+    H-PLOC) over the meshes `vertex_sets` (float32 tensors of 9 floats per triangle on one device); node_width
+    2 or 8."""
     if not vertex_sets:
         raise ValueError("a BLAS set needs at least one mesh")
     ctx = ctx or default_context(vertex_sets[0].device.index or 0)
     counts = [_check_vertices(v, ctx) for v in vertex_sets]
     flat = torch.cat([v.reshape(-1) for v in vertex_sets]).contiguous()
     offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
-    return build_triangle_blas_set_flat(flat, offsets, ctx, node_width, leaf_size, timings)
+    return build_triangle_blas_set_flat(flat, offsets, ctx, node_width, leaf_size, timings, method)
 
 
 def build_triangle_blas_set_flat(vertices: torch.Tensor, offsets: np.ndarray, ctx: DeviceContext | None = None,
                                  node_width: int = 2, leaf_size: int = 0,
-                                 timings: dict | None = None) -> GpuTriangleBlasSet:
+                                 timings: dict | None = None,
+                                 method: int = GpuBvhBuildMethod.LBVH) -> GpuTriangleBlasSet:
     """The same build over meshes stored back to back in `vertices`, mesh k = triangles
     [offsets[k], offsets[k + 1])."""
     ctx = ctx or default_context(vertices.device.index or 0)
-    handle = _build_set(vertices, offsets, ctx, node_width, leaf_size, timings, ctypes.c_void_p())
+    handle = _build_set(vertices, offsets, ctx, node_width, leaf_size, timings, ctypes.c_void_p(), method)
     counts = np.diff(np.asarray(offsets, np.int64)).tolist()
-    return GpuTriangleBlasSet(handle, ctx, counts, node_width, leaf_size)
+    return GpuTriangleBlasSet(handle, ctx, counts, node_width, leaf_size, method)
 
 
 @dataclass

@@ -103,6 +103,7 @@ namespace bj {
 int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h);
 // Builds h's binary hierarchy with H-PLOC over the sorted keys; hploc.hip.
 int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* src);
+int32_t build_hploc_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices);
 // Builds the segments of a BLAS set into h (lbvh.hip); offsets are host u64[S + 1].
 int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const uint64_t* offsets, uint32_t S,
                         bajo_build_timings* timings);

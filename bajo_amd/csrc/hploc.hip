@@ -22,6 +22,13 @@
 //     mutual pairs are found with one ballot, node indices are reserved with
 //     one atomic per round, and the list is compacted with a forward lane
 //     permute (ds_permute);
+//   - a segmented build (a BLAS set) runs every segment's H-PLOC in the same
+//     launch: the guide is the per-segment Karras forest, a guide node whose
+//     parent is the sentinel is its segment's root (merged down to one
+//     cluster), node numbering by retired leaf positions keeps each
+//     segment's nodes in its own slots [first, last), and a fix-up moves each
+//     segment's root to slot `first`, where the segmented layout expects it
+//     (bajo's segmented H-PLOC, hploc_multi_wave.mojo:126-140, 187-545);
 //   - output nodes use the 64-byte binary record of the LBVH; a last pass
 //     renumbers the leaves in depth-first order (so every subtree covers a
 //     contiguous range of sorted triangles, which the 8-wide collapse and the
@@ -105,6 +112,17 @@ BJ_D float merged_area(const HpEntry& a, const float* o) {
 }
 
 BJ_D uint32_t shfl_u(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, 64); }
+
+// Segment of sorted position i (off: u64[S + 1]), by binary search.
+BJ_D uint32_t seg_of(const uint64_t* __restrict__ off, uint32_t S, uint64_t i) {
+  uint32_t lo = 0, hi = S;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 BJ_D float shfl_f(float v, int src) { return __shfl(v, src, 64); }
 BJ_D uint32_t permute_u(uint32_t dst_lane, uint32_t v) {
   return (uint32_t)__builtin_amdgcn_ds_permute((int)(dst_lane << 2), (int)v);
@@ -138,7 +156,8 @@ BJ_D void write_node(BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_cou
 BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
                        const uint32_t* __restrict__ range_first, HpEntry* __restrict__ clusters,
                        uint32_t* __restrict__ ids, uint32_t* __restrict__ counts, BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_counts,
-                       uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ ctl) {
+                       uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ ctl,
+                       const uint64_t* __restrict__ seg_off, uint32_t S, uint32_t* __restrict__ seg_roots) {
   const uint32_t lane = lane_id();
   const uint32_t first = range_first[g], last = guide[g].fence;
   const uint32_t gl = guide[g].left, gr = guide[g].right;
@@ -151,7 +170,8 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
     if (lane == 0) atomicOr(ctl + kHpStatus, kHpStatusBadList);
     return;
   }
-  const bool root = first == 0 && (uint64_t)last == n - 1;
+  // the hierarchy's (or the segment's) root: no parent in the guide
+  const bool root = guide[g].parent == kSentinel;
   const uint32_t thr = root ? 1u : kHpThreshold;
   if (c <= thr) {
     // no merging: the list is the left list (already at `first`) followed by
@@ -271,7 +291,10 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
   }
   if (lane == 0) {
     store_wt(counts + g, c);
-    if (root) store_wt(ctl + kHpRoot, e.ref);
+    if (root) {
+      if (seg_roots) store_wt(seg_roots + seg_of(seg_off, S, first), e.ref);
+      else store_wt(ctl + kHpRoot, e.ref);
+    }
   }
   publish_fence();
 }
@@ -286,13 +309,17 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
                                                          uint32_t* __restrict__ ids,
                                                          uint32_t* __restrict__ counts, BjNode* __restrict__ nodes,
                                                          uint32_t* __restrict__ leaf_counts,
-                                                         uint32_t* __restrict__ ctl) {
+                                                         uint32_t* __restrict__ ctl,
+                                                         const uint32_t* __restrict__ gprim,
+                                                         const uint64_t* __restrict__ seg_off, uint32_t S,
+                                                         uint32_t* __restrict__ seg_roots) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool active = i < n;
   bool pending = false;
   uint32_t cur = kLeafFlag | (uint32_t)i;
   if (active) {
-    const uint32_t prim = (uint32_t)keys[i];
+    // a segmented build's keys hold the index inside the segment; gprim the input index
+    const uint32_t prim = gprim ? gprim[i] : (uint32_t)keys[i];
     v3 lo, hi;
     leaf_box<kBoxes>(vertices, prim, lo, hi);
     HpEntry e{kLeafFlag | (uint32_t)i, 1u, {lo.x, lo.y, lo.z}, {hi.x, hi.y, hi.z}};
@@ -326,7 +353,7 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
       const uint32_t split = gl & kIndexMask;
       const uint32_t cl = (gl & kLeafFlag) ? 1u : load_co(counts + split);
       const uint32_t cr = (gr & kLeafFlag) ? 1u : load_co(counts + (gr & kIndexMask));
-      const bool root = first == 0 && (uint64_t)last == n - 1;
+      const bool root = guide[g].parent == kSentinel;
       const bool sane = cl >= 1 && cr >= 1 && cl <= kHpThreshold && cr <= kHpThreshold && first <= split &&
                         split < last && (uint64_t)last < n && cl <= split - first + 1 && cr <= last - split;
       if (sane && !root && cl + cr <= kHpThreshold) {
@@ -345,10 +372,11 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
       const int leader = __builtin_ctzll(todo);
       todo &= todo - 1;
       const uint32_t g = shfl_u(cur, leader);
-      process_node(g, n, guide, range_first, clusters, ids, counts, nodes, leaf_counts, leaf_parent, ctl);
+      process_node(g, n, guide, range_first, clusters, ids, counts, nodes, leaf_counts, leaf_parent, ctl, seg_off, S,
+                   seg_roots);
       if ((int)lane_id() == leader) {
         pending = false;
-        if (range_first[g] == 0 && (uint64_t)guide[g].fence == n - 1) active = false;
+        if (guide[g].parent == kSentinel) active = false;
       }
     }
     if (!ballot(active)) return;
@@ -372,6 +400,71 @@ __global__ void hploc_root_kernel(BjNode* __restrict__ nodes, uint64_t n, const 
   }
 }
 
+__global__ void seg_roots_init_kernel(uint32_t* __restrict__ seg_roots, uint32_t S) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < S) seg_roots[s] = kSentinel;
+}
+
+// Segmented builds: moves segment s's root (any slot of the segment) to slot
+// off[s] by swapping it with the node there, relinking both nodes'
+// children and the moved node's parent.  One thread per segment; segments
+// own disjoint slots.  Checks each root's leaf count.
+__global__ void seg_root_fix_kernel(BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_counts,
+                                    uint32_t* __restrict__ leaf_parent, const uint64_t* __restrict__ off, uint32_t S,
+                                    const uint32_t* __restrict__ seg_roots, uint32_t* __restrict__ ctl) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const uint64_t first = off[s], cnt = off[s + 1] - first;
+  if (cnt < 2) return;  // a one-triangle segment is its own root
+  const uint32_t r = seg_roots[s], x = (uint32_t)first;
+  if ((r & kLeafFlag) || r < x || (uint64_t)r >= first + cnt - 1 || leaf_counts[r] != cnt) {
+    atomicOr(ctl + kHpStatus, kHpStatusBadList);
+    return;
+  }
+  // every reference the fix-up follows must stay inside the segment
+  const uint32_t end_node = (uint32_t)(first + cnt - 1), end_leaf = (uint32_t)(first + cnt);
+  auto ok = [&](uint32_t ref) {
+    return (ref & kLeafFlag) ? ((ref & kIndexMask) >= x && (ref & kIndexMask) < end_leaf) : (ref >= x && ref < end_node);
+  };
+  if (!ok(nodes[r].left) || !ok(nodes[r].right) || !ok(nodes[x].left) || !ok(nodes[x].right) ||
+      (r != x && !ok(nodes[x].parent))) {
+    atomicOr(ctl + kHpStatus, kHpStatusBadList);
+    return;
+  }
+  if (r != x) {
+    BjNode R = nodes[r], X = nodes[x];
+    const uint32_t cR = leaf_counts[r], cX = leaf_counts[x];
+    auto remap = [&](uint32_t ref) { return (ref & kLeafFlag) ? ref : ref == r ? x : ref == x ? r : ref; };
+    R.left = remap(R.left);
+    R.right = remap(R.right);
+    X.left = remap(X.left);
+    X.right = remap(X.right);
+    X.parent = remap(X.parent);
+    nodes[x] = R;
+    nodes[r] = X;
+    leaf_counts[x] = cR;
+    leaf_counts[r] = cX;
+    const uint32_t moved[2] = {x, r};
+    for (int k = 0; k < 2; ++k) {
+      const uint32_t y = moved[k];
+      const uint32_t kids[2] = {nodes[y].left, nodes[y].right};
+      for (int c = 0; c < 2; ++c) {
+        const uint32_t ref = kids[c];
+        if (ref & kLeafFlag) leaf_parent[ref & kIndexMask] = y;
+        else nodes[ref].parent = y;
+      }
+    }
+    // the parent of the node moved to r (unless that parent is the root, whose
+    // references were remapped above) still points at x
+    const uint32_t p = nodes[r].parent;
+    if (p != x) {
+      if (nodes[p].left == x) nodes[p].left = r;
+      else if (nodes[p].right == x) nodes[p].right = r;
+    }
+  }
+  nodes[x].parent = kSentinel;
+}
+
 // Depth-first position of each sorted leaf: the leaves of the left subtrees
 // of every ancestor reached from the right come first.  Also writes the
 // fence (last depth-first position) of every node whose rightmost leaf this
@@ -379,12 +472,14 @@ __global__ void hploc_root_kernel(BjNode* __restrict__ nodes, uint64_t n, const 
 __global__ void __launch_bounds__(kHpBlock) dfs_position_kernel(BjNode* nodes, uint64_t n,
                                                                 const uint32_t* __restrict__ leaf_parent,
                                                                 const uint32_t* __restrict__ leaf_counts,
-                                                                uint32_t* __restrict__ pos_out) {
+                                                                uint32_t* __restrict__ pos_out,
+                                                                const uint64_t* __restrict__ seg_off, uint32_t S) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   uint32_t ref = kLeafFlag | (uint32_t)j;
   uint32_t parent = leaf_parent[j];
-  uint32_t pos = 0;
+  // a segment's leaves start at its first sorted position
+  uint32_t pos = seg_off ? (uint32_t)seg_off[seg_of(seg_off, S, j)] : 0u;
   bool leaf_is_right = false;
   for (int depth = 0; parent != kSentinel && (uint64_t)parent < n - 1 && depth < 4096; ++depth) {
     const uint32_t l = nodes[parent].left;
@@ -415,7 +510,8 @@ __global__ void __launch_bounds__(kHpBlock) dfs_emit_kernel(const float* __restr
                                                             const uint32_t* __restrict__ leaf_parent,
                                                             BjNode* __restrict__ nodes, BjTri* __restrict__ tris,
                                                             uint64_t* __restrict__ keys_out,
-                                                            uint32_t* __restrict__ leaf_parent_out) {
+                                                            uint32_t* __restrict__ leaf_parent_out,
+                                                            const uint32_t* __restrict__ gprim) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t pw = pos_in[j];
@@ -425,7 +521,7 @@ __global__ void __launch_bounds__(kHpBlock) dfs_emit_kernel(const float* __restr
   const uint32_t prim = (uint32_t)key;
   if constexpr (!kBoxes) {
     v3 a, b, c;
-    load_tri_vertices(vertices, prim, a, b, c);
+    load_tri_vertices(vertices, gprim ? gprim[j] : prim, a, b, c);
     const v3 e1 = sub(b, a), e2 = sub(c, a);
     float4* out = reinterpret_cast<float4*>(tris + pos);
     out[0] = make_float4(a.x, a.y, a.z, __builtin_bit_cast(float, prim));
@@ -455,25 +551,27 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
   if (h->boxes)
     hploc_kernel<true><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
                                                                  h->leaf_parent, h->flags, clusters, h->hp_pos,
-                                                                 h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl);
+                                                                 h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl,
+                                                                 nullptr, nullptr, 0u, nullptr);
   else
     hploc_kernel<false><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
                                                                   h->leaf_parent, h->flags, clusters, h->hp_pos,
-                                                                  h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl);
+                                                                  h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl,
+                                                                  nullptr, nullptr, 0u, nullptr);
   hploc_root_kernel<<<1, 64, 0, st>>>(h->nodes, n, h->leaf_counts, h->hp_ctl);
   BJ_HIP(hipGetLastError());
   // the cluster area is free now: it holds the reordered keys and leaf parents
   uint64_t* keys_dfs = reinterpret_cast<uint64_t*>(h->hp_clusters);
   uint32_t* parent_dfs = reinterpret_cast<uint32_t*>(keys_dfs + n);
   dfs_position_kernel<<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(h->nodes, n, h->leaf_parent,
-                                                                h->leaf_counts, h->hp_pos);
+                                                                h->leaf_counts, h->hp_pos, nullptr, 0u);
   if (h->boxes)
     dfs_emit_kernel<true><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_pos, h->leaf_parent,
-                                                                    h->nodes, h->tris, keys_dfs, parent_dfs);
+                                                                    h->nodes, h->tris, keys_dfs, parent_dfs, nullptr);
   else
     dfs_emit_kernel<false><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_pos,
                                                                      h->leaf_parent, h->nodes, h->tris, keys_dfs,
-                                                                     parent_dfs);
+                                                                     parent_dfs, nullptr);
   BJ_HIP(hipGetLastError());
   BJ_HIP(hipMemcpyAsync(h->keys, keys_dfs, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
   BJ_HIP(hipMemcpyAsync(h->leaf_parent, parent_dfs, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
@@ -486,6 +584,44 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
              (unsigned long long)n);
   BJ_REQUIRE(!(c[kHpRoot] & kLeafFlag) && (uint64_t)c[kHpRoot] < n - 1, "H-PLOC root %u is invalid", c[kHpRoot]);
   h->root = c[kHpRoot];
+  return BAJO_OK;
+}
+
+// H-PLOC over the segments of a segmented build: `h->hp_guide` / `h->hp_first`
+// already hold the per-segment Karras forest and `h->keys` the sorted keys
+// with in-segment indices (`h->seg_prim`: input indices).  Writes every
+// segment's nodes (root at its first slot), fences and packed triangles.
+int32_t build_hploc_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
+  hipStream_t st = ctx->stream;
+  const uint64_t n = h->n;
+  const uint32_t S = h->seg_count;
+  if (n < 2 || n == S) return BAJO_OK;  // only one-triangle segments: packed by the caller
+  HpEntry* clusters = reinterpret_cast<HpEntry*>(h->hp_clusters);
+  uint32_t* seg_roots = h->seg_acc;  // free after the Morton codes: u32[S] of the 12 S words
+  hploc_init_kernel<<<1, 64, 0, st>>>(h->hp_ctl);
+  seg_roots_init_kernel<<<div_up(S, kHpBlock), kHpBlock, 0, st>>>(seg_roots, S);
+  hploc_kernel<false><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
+                                                                h->leaf_parent, h->flags, clusters, h->hp_pos,
+                                                                h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl,
+                                                                h->seg_prim, h->seg_off, S, seg_roots);
+  seg_root_fix_kernel<<<div_up(S, kHpBlock), kHpBlock, 0, st>>>(h->nodes, h->leaf_counts, h->leaf_parent, h->seg_off,
+                                                               S, seg_roots, h->hp_ctl);
+  BJ_HIP(hipGetLastError());
+  uint64_t* keys_dfs = reinterpret_cast<uint64_t*>(h->hp_clusters);
+  uint32_t* parent_dfs = reinterpret_cast<uint32_t*>(keys_dfs + n);
+  dfs_position_kernel<<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(h->nodes, n, h->leaf_parent, h->leaf_counts,
+                                                                h->hp_pos, h->seg_off, S);
+  dfs_emit_kernel<false><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_pos, h->leaf_parent,
+                                                                   h->nodes, h->tris, keys_dfs, parent_dfs,
+                                                                   h->seg_prim);
+  BJ_HIP(hipGetLastError());
+  BJ_HIP(hipMemcpyAsync(h->keys, keys_dfs, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+  BJ_HIP(hipMemcpyAsync(h->leaf_parent, parent_dfs, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+  BJ_HIP(hipMemcpyAsync(h->host_hp_ctl, h->hp_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  BJ_HIP(hipStreamSynchronize(st));
+  BJ_REQUIRE(h->host_hp_ctl[kHpStatus] == 0,
+             "segmented H-PLOC build failed with status %u (1 = no progress, 2 = corrupt cluster list or root)",
+             h->host_hp_ctl[kHpStatus]);
   return BAJO_OK;
 }
 

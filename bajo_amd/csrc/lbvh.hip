@@ -476,13 +476,14 @@ __global__ void __launch_bounds__(kBlock) seg_karras_kernel(const uint64_t* __re
                                                             BjNode* __restrict__ nodes,
                                                             uint32_t* __restrict__ leaf_parent,
                                                             uint32_t* __restrict__ leaf_counts,
-                                                            uint32_t* __restrict__ flags) {
+                                                            uint32_t* __restrict__ flags,
+                                                            uint32_t* __restrict__ range_first) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j + 1 >= n) return;
   const uint32_t seg = segment_of(off, S, j);
   const uint64_t first = off[seg], count = off[seg + 1] - first;
   if (j - first + 1 >= count) return;  // the segment's unused last slot
-  karras_node(keys + first, (int64_t)count, (int64_t)(j - first), nodes, leaf_parent, leaf_counts, flags, nullptr,
+  karras_node(keys + first, (int64_t)count, (int64_t)(j - first), nodes, leaf_parent, leaf_counts, flags, range_first,
               (uint32_t)first, (uint32_t)first);
 }
 
@@ -761,6 +762,10 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
     rc = ensure_wide_capacity(h, n);
     if (rc) return rc;
   }
+  if (h->method == BAJO_BUILD_HPLOC) {
+    rc = ensure_hploc_capacity(h, n);
+    if (rc) return rc;
+  }
   int seg_bits = 0;
   while ((1ull << seg_bits) < S) ++seg_bits;
   const SortPlan plan = make_sort_plan(n, 8, 1, 0, 30 + seg_bits);
@@ -797,15 +802,26 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
   seg_keys_kernel<<<div_up(n, kBlock), kBlock, 0, st>>>(h->keys, h->seg_prim, n, h->seg_off, h->leaf_parent);
   BJ_HIP(hipGetLastError());
   ev.record(3, st);
+  const bool hploc = h->method == BAJO_BUILD_HPLOC && n > S;
   if (n > S) {
-    seg_karras_kernel<<<div_up(n - 1, kBlock), kBlock, 0, st>>>(h->keys, n, h->seg_off, S, h->nodes, h->leaf_parent,
-                                                                h->leaf_counts, h->flags);
+    // the LBVH writes its nodes directly; H-PLOC's guide is the same forest
+    seg_karras_kernel<<<div_up(n - 1, kBlock), kBlock, 0, st>>>(h->keys, n, h->seg_off, S,
+                                                                hploc ? h->hp_guide : h->nodes, h->leaf_parent,
+                                                                h->leaf_counts, h->flags,
+                                                                hploc ? h->hp_first : nullptr);
     BJ_HIP(hipGetLastError());
   }
   ev.record(4, st);
-  refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, h->seg_prim, n, n - 1, h->nodes,
-                                                            h->leaf_parent, h->leaf_counts, h->flags, h->tris);
-  BJ_HIP(hipGetLastError());
+  if (hploc) {
+    // bvh/gpu/builder/segmented_build.mojo:117-130 with build_method HPLOC;
+    // its depth-first pass also packs the one-triangle segments
+    rc = build_hploc_segmented(ctx, h, vertices);
+    if (rc) return rc;
+  } else {
+    refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, h->seg_prim, n, n - 1, h->nodes,
+                                                              h->leaf_parent, h->leaf_counts, h->flags, h->tris);
+    BJ_HIP(hipGetLastError());
+  }
   ev.record(5, st);
   if (h->node_width == BAJO_WIDTH_WIDE8) {
     rc = collapse_wide(ctx, h);
@@ -1072,7 +1088,8 @@ int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const
   bajo_build_options opt = options ? *options : bajo_build_options{BAJO_BUILD_LBVH, BAJO_WIDTH_BINARY, 0, 0};
   BJ_REQUIRE(ctx && vertices && segment_offsets && out, "%s: NULL argument", fn);
   BJ_REQUIRE(segment_count > 0, "%s: a BLAS set needs at least one segment", fn);
-  BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH, "%s: segmented builds use the LBVH (method 0), got %d", fn, opt.method);
+  BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH || opt.method == BAJO_BUILD_HPLOC,
+             "%s: method must be LBVH (0) or H-PLOC (1), got %d", fn, opt.method);
   BJ_REQUIRE(opt.node_width == BAJO_WIDTH_BINARY || opt.node_width == BAJO_WIDTH_WIDE8,
              "%s: node_width must be 2 or 8, got %d", fn, opt.node_width);
   if (opt.leaf_size == 0) opt.leaf_size = (int32_t)kWideDefaultLeaf;

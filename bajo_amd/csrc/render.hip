@@ -1493,7 +1493,8 @@ int32_t bajo_scene_create_batch(bajo_ctx* ctx, const float* vertices, const uint
   const uint64_t n = offsets[env_count];
   BJ_REQUIRE(n < (1ull << 31), "bajo_scene_create_batch: too many triangles");
   const bajo_build_options opt = options ? *options : bajo_build_options{BAJO_BUILD_LBVH, BAJO_WIDTH_BINARY, 0, 0};
-  BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH, "bajo_scene_create_batch: segmented builds use the LBVH (method 0)");
+  BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH || opt.method == BAJO_BUILD_HPLOC,
+             "bajo_scene_create_batch: method must be LBVH (0) or H-PLOC (1)");
   // materials, surfaces and uploads as for one scene; the lights are then
   // rebuilt per environment below
   bajo_scene* s = nullptr;

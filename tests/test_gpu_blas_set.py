@@ -158,7 +158,7 @@ def test_blas_set_rejects_bad_arguments(gpu_ctx):
 
     assert call([0, 4, 4, 10]) != 0 and b"empty" in _lib.lib().bajo_last_error()
     assert call([1, 10]) != 0
-    assert call([0, 10], method=1) != 0 and b"LBVH" in _lib.lib().bajo_last_error()
+    assert call([0, 10], method=5) != 0 and b"H-PLOC" in _lib.lib().bajo_last_error()
     bs = build_triangle_blas_set([v])
     view = bs.blas(0)
     assert _lib.lib().bajo_bvh_free(view.handle) != 0  # a view is released with its set
@@ -166,4 +166,60 @@ def test_blas_set_rejects_bad_arguments(gpu_ctx):
         view.rebuild(v)
     with pytest.raises(BajoError):
         bs.blas(1)
+    bs.free()
+
+
+def _preorder(b):
+    """Preorder walk of an exported binary layout: (refs with leaf positions, inner boxes, inner fences)."""
+    meta = b["node_meta"].reshape(-1, 4)
+    nb = b["node_bounds"].reshape(-1, 12)
+    refs, boxes, fences = [], [], []
+    stack = [int(b["root"])]
+    while stack:
+        r = stack.pop()
+        refs.append(r if r & 0x80000000 else -1)
+        if r & 0x80000000:
+            continue
+        boxes.append(nb[r].view(np.uint32))
+        fences.append(int(meta[r, 3]))
+        stack.append(int(meta[r, 2]))  # meta rows: parent, left, right, fence
+        stack.append(int(meta[r, 1]))
+    return np.array(refs), np.array(boxes), np.array(fences)
+
+
+@pytest.mark.parametrize("width", [2, 8])
+def test_hploc_segments_equal_standalone_hploc(gpu_ctx, width):
+    """bajo's default set build is H-PLOC (triangle_bvh.mojo:469-491, segmented_build.mojo:117-130): every
+    segment equals the standalone H-PLOC build of its mesh (same tree in preorder: leaf order, node boxes,
+    fences; same 8-wide collapse; identical hits) and the oracle's H-PLOC topology."""
+    from bajo_amd.bvh import build_triangle_blas_set, build_triangle_bvh
+
+    announce(f"hploc blas set width={width}")
+    ms = meshes(SIZES, 300)
+    bs = build_triangle_blas_set([to_dev(m) for m in ms], node_width=width, leaf_size=4, method=1)
+    for k, m in enumerate(ms):
+        view = bs.blas(k)
+        one = build_triangle_bvh(to_dev(m), node_width=width, leaf_size=4, method=1)
+        got, want = view.binary(), one.binary()
+        np.testing.assert_array_equal(got["leaf_ids"], want["leaf_ids"], err_msg=f"segment {k}")
+        if m.size // 9 > 1:
+            ga, gb, gf = _preorder(got)
+            wa, wb, wf = _preorder(want)
+            np.testing.assert_array_equal(ga, wa, err_msg=f"segment {k}")
+            np.testing.assert_array_equal(gb, wb, err_msg=f"segment {k}")
+            np.testing.assert_array_equal(gf, wf, err_msg=f"segment {k}")
+            assert got["root"] == 0
+            ref = oracle.build_hploc(m)
+            np.testing.assert_array_equal(oracle.preorder_binary(got["root"], got["node_meta"], got["leaf_ids"]),
+                                          oracle.preorder_topology(ref["topology"]), err_msg=f"segment {k}")
+        if width == 8:
+            assert oracle.canonical_wide(view.wide()).tobytes() == oracle.canonical_wide(one.wide()).tobytes()
+        R = 4096
+        rays = to_dev(random_rays(R, -0.1, 1.1, 40 + k, t_min=1e-4))
+        ha = torch.empty(8 * R, dtype=torch.float32, device="cuda:0")
+        hb = torch.empty_like(ha)
+        view.launch_rays(rays, ha, R)
+        one.launch_rays(rays, hb, R)
+        np.testing.assert_array_equal(to_host(ha, np.uint32), to_host(hb, np.uint32), err_msg=f"segment {k}")
+        one.free()
     bs.free()

@@ -361,7 +361,7 @@ def batched_envs_workload(ctx, count: int = 4096, width: int = 64, height: int =
     [256, 65536] (47.5 M triangles at 4,096), each with its own LBVH, and 64 x 64 primary rays per
     environment from the small-scene camera.  One segmented BLAS-set build plus one batched camera launch,
     against a per-environment loop (one rebuild of a single BVH handle and one camera launch per
-    environment) over the same data.  Closest hit; binary and 8-wide."""
+    environment) over the same data.  Closest hit; binary and 8-wide LBVH sets, and the 8-wide H-PLOC set."""
     import torch
 
     from bajo_amd import scenes
@@ -407,6 +407,17 @@ def batched_envs_workload(ctx, count: int = 4096, width: int = 64, height: int =
         loop_ms = (time.perf_counter() - t0) * 1e3
         b.free()
         key = "binary" if width_nodes == 2 else "wide8"
+        if width_nodes == 8:
+            # bajo's default set build: H-PLOC (then the same collapse and batched trace)
+            hs = build_triangle_blas_set_flat(v, off, ctx=ctx, node_width=8, leaf_size=4, method=1)
+            hp_build_ms = _timed(lambda: hs.rebuild(v, off), stream)
+            hp_trace_ms = _timed(lambda: hs.launch_camera(cams, hits, width, height), stream)
+            hs.free()
+            out["wide8_hploc"] = {"set_build_ms": hp_build_ms, "set_build_mtris_s": n / (hp_build_ms / 1e3) / 1e6,
+                                  "batched_trace_ms": hp_trace_ms,
+                                  "batched_trace_mrays_s": R / (hp_trace_ms / 1e3) / 1e6}
+            print(f"[bench] batched envs wide8 H-PLOC: set build {hp_build_ms:.2f} ms + batched trace "
+                  f"{hp_trace_ms:.2f} ms", file=sys.stderr, flush=True)
         out[key] = {"set_build_ms": build_ms, "set_build_mtris_s": n / (build_ms / 1e3) / 1e6,
                     "set_build_stages_ms": {k2: val / 1e6 for k2, val in stages.items()},
                     "batched_trace_ms": trace_ms, "batched_trace_mrays_s": R / (trace_ms / 1e3) / 1e6,

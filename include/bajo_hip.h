@@ -237,8 +237,8 @@ int32_t bajo_camera_make(const float* origin3, const float* target3, const float
  * vertices f32[9N] hold the meshes back to back; segment_offsets (host)
  * u64[S + 1] hold each mesh's first triangle (offsets[0] = 0, increasing,
  * offsets[S] = N).  Each segment's hierarchy equals the standalone build of
- * its mesh; hit prim ids are indices inside the mesh.  Method LBVH, node
- * width 2 or 8.  bajo_blas_set_get returns segment k as a bajo_bvh view
+ * its mesh; hit prim ids are indices inside the mesh.  Method LBVH or
+ * H-PLOC (bajo's default for sets), node width 2 or 8.  bajo_blas_set_get returns segment k as a bajo_bvh view
  * (bajo's BlasDesc, blas_desc.mojo) that traces, exports and enters a TLAS
  * like any BVH; it shares the set's storage, cannot be rebuilt or freed on
  * its own, and lives until bajo_blas_set_free.  A non-NULL *out is rebuilt
@@ -392,8 +392,8 @@ int32_t bajo_scene_create_mixed(bajo_ctx* ctx, const float* vertices, uint64_t n
  * rt/gpu/render.mojo:322-420): environment e is the triangles
  * [offsets[e], offsets[e+1]) of `vertices` (host f32[9N], surfaces u32[N]
  * sharing `materials`), with its own lights (its emissive triangles, its own
- * alias table).  All environments are built in one segmented LBVH pass
- * (options: method LBVH, width 2 or 8).  bajo_render of a batch renders
+ * alias table).  All environments are built in one segmented pass
+ * (options: method LBVH or H-PLOC, width 2 or 8).  bajo_render of a batch renders
  * every environment through the same camera and settings into pixels
  * f32[E h w 3] (environment-major) in one wavefront; environment e's image
  * equals a render of its scene alone with seed rng_seed + e.  A render

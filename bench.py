@@ -456,6 +456,15 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
     rays = first.stats["extension_rays"] + first.stats["shadow_rays"]
     pixels = first.pixels
     render_ms = _timed(lambda: enqueue_render_gpu(st, cam, g, RENDER.NEE, pixels=pixels), stream)
+    # the autotuner's schedule: paths in flight per chunk, timed on the device over whole-frame and 1M..64M
+    # chunks (untimed here); the tuned render is then timed like the default one
+    from bajo_amd.autotune import tune_render
+
+    tuned, table = tune_render(st, cam, g, RENDER.NEE, pixels=torch.empty_like(pixels))
+    tuned_px = torch.empty_like(pixels)
+    tuned_ms = _timed(lambda: enqueue_render_gpu(tuned, cam, g, RENDER.NEE, pixels=tuned_px), stream)
+    tuned_equal = bool(torch.equal(tuned_px, pixels))
+    print(f"[bench] autotuned path capacity {tuned.path_capacity}: {tuned_ms:.2f} ms", file=sys.stderr, flush=True)
     t = GpuRtRenderTarget(st, cam, RENDER.NEE, ctx, environments=count)
     t.capture(g)
     replay_ms = _timed(t.replay, stream)
@@ -468,7 +477,10 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
            "paths": count * width * height * spp, "rays": rays, "extension_rays": first.stats["extension_rays"],
            "shadow_rays": first.stats["shadow_rays"], "batched_render_ms": render_ms,
            "batched_render_mrays_s": rays / (render_ms / 1e3) / 1e6, "graph_replay_ms": replay_ms,
-           "graph_replay_mrays_s": rays / (replay_ms / 1e3) / 1e6, "graph_equals_render": same}
+           "graph_replay_mrays_s": rays / (replay_ms / 1e3) / 1e6, "graph_equals_render": same,
+           "autotuned": {"path_capacity": tuned.path_capacity, "candidates_ms": {str(k): v for k, v in table.items()},
+                         "render_ms": tuned_ms, "mrays_s": rays / (tuned_ms / 1e3) / 1e6,
+                         "images_equal_default": tuned_equal}}
     if not per_env_loop:
         return out
     # the per-environment loop: every environment a scene of its own, one render call each; the scenes are

@@ -121,3 +121,22 @@ def test_batch_render_target_replay(gpu_ctx, batch):
         single.enqueue(g)
     single.free()
     g.free()
+
+
+def test_autotuned_schedule_renders_the_same_image(gpu_ctx, batch):
+    from bajo_amd.autotune import tune_render
+    from bajo_amd.rt import GpuSceneBatch, enqueue_render_gpu
+
+    announce("batch autotune")
+    sc, off = batch
+    g = GpuSceneBatch(sc, off, gpu_ctx, node_width=8)
+    want = enqueue_render_gpu(settings(), camera(), g, ALGS["MIS"]).numpy()
+    tuned, table = tune_render(settings(), camera(), g, ALGS["MIS"], candidates=[0, 1536, 6144], reps=1)
+    assert set(table) == {0, 1536, 6144} and all(v > 0 for v in table.values())
+    assert tuned.path_capacity == min(table, key=table.get)
+    for cap in table:
+        from dataclasses import replace
+
+        got = enqueue_render_gpu(replace(settings(), path_capacity=cap), camera(), g, ALGS["MIS"]).numpy()
+        np.testing.assert_array_equal(got, want, err_msg=f"path capacity {cap}")
+    g.free()

@@ -581,8 +581,9 @@ def cpu_baseline(scene, cam, threads: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    # a cold GPU needs some 0.3 s of work to reach its steady clocks: 20 warm-up steps (about 0.3 s) come first
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-workloads", action="store_true",
                     help="skip the 65k-soup camera, 16.7M soup-grid H-PLOC, TLAS, batched-environment and render-target "

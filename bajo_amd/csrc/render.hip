@@ -635,7 +635,7 @@ BJ_D Hit scene_trace(const SceneView& sc, const Ray& ray, Stack& st, TraceCounte
 }
 
 template <int kAlg, bool kWide, bool kFirst, int kGeo>
-__global__ void __launch_bounds__(kRtBlock) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
+__global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(6))) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                           PrimaryArgs pa, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
                                                           unsigned long long* __restrict__ stats) {

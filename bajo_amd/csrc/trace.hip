@@ -99,7 +99,7 @@ constexpr uint32_t kRefillLanes = 16;
 constexpr uint32_t kRefillChunk = 256;
 
 template <int kMode, bool kStats, bool kCamera, bool kSeg, bool kSph>
-__global__ void __launch_bounds__(kTraceBlock) trace_refill_kernel(const WideView wv, const float* __restrict__ rays,
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))) trace_refill_kernel(const WideView wv, const float* __restrict__ rays,
                                                                    const float* __restrict__ cams, uint32_t cam_w,
                                                                    uint32_t cam_h, float inv_height,
                                                                    float* __restrict__ hits, uint64_t R,

@@ -1278,21 +1278,37 @@ int32_t by_algorithm(const bajo_scene* s, int32_t algorithm, F&& fn) {
   }
 }
 
+// The instantiation-dispatched calls (by_algorithm picks the template arguments).
+struct ShapeCall {
+  const bajo_scene* s;
+  const bajo_render_settings* st;
+  RenderShape* out;
+  template <int kAlg, bool kWide, int kGeo>
+  int32_t operator()() const { return render_shape<kAlg, kWide, kGeo>(s, st, out); }
+};
+struct RenderCall {
+  bajo_ctx* ctx;
+  bajo_scene* s;
+  RenderArena* a;
+  const bajo_render_settings* st;
+  const float* camera;
+  float* pixels;
+  LaunchEvents* ev;
+  template <int kAlg, bool kWide, int kGeo>
+  int32_t operator()() const { return run_render<kAlg, kWide, kGeo>(ctx, s, *a, st, camera, pixels, ev); }
+};
+
 // Sizes arena `a` for a render of `st` over scene `s` without enqueueing it.
 int32_t prepare_arena(const bajo_scene* s, RenderArena& a, const bajo_render_settings* st) {
   RenderShape shape;
-  const int32_t rc = by_algorithm(s, st->algorithm, [&]<int kAlg, bool kWide, int kGeo>() {
-    return render_shape<kAlg, kWide, kGeo>(s, st, &shape);
-  });
+  const int32_t rc = by_algorithm(s, st->algorithm, ShapeCall{s, st, &shape});
   if (rc) return rc;
   return ensure_arena(a, shape.cap, shape.spill_threads);
 }
 
 int32_t dispatch_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st,
                         const float* camera, float* pixels, LaunchEvents* ev) {
-  return by_algorithm(s, st->algorithm, [&]<int kAlg, bool kWide, int kGeo>() {
-    return run_render<kAlg, kWide, kGeo>(ctx, s, a, st, camera, pixels, ev);
-  });
+  return by_algorithm(s, st->algorithm, RenderCall{ctx, s, &a, st, camera, pixels, ev});
 }
 
 }  // namespace

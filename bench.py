@@ -431,7 +431,7 @@ def batched_envs_workload(ctx, count: int = 4096, width: int = 64, height: int =
 
 
 def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int = 64, spp: int = 4,
-                            depth: int = 3, soups=None, per_env_loop: bool = True) -> dict:
+                            depth: int = 3, soups=None, per_env_loop: bool = True, autotune: bool = True) -> dict:
     """The batched-environment render: the `count` soups of the batched-environment shape, each under its own
     quad light (emission 6) with its own Lambertian albedo, rendered with NEE at width x height x spp, depth 3,
     through one wavefront over one segmented 8-wide LBVH build (environment e with seed 2026 + e).  Timed three
@@ -460,11 +460,14 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
     # chunks (untimed here); the tuned render is then timed like the default one
     from bajo_amd.autotune import tune_render
 
-    tuned, table = tune_render(st, cam, g, RENDER.NEE, pixels=torch.empty_like(pixels))
-    tuned_px = torch.empty_like(pixels)
-    tuned_ms = _timed(lambda: enqueue_render_gpu(tuned, cam, g, RENDER.NEE, pixels=tuned_px), stream)
-    tuned_equal = bool(torch.equal(tuned_px, pixels))
-    print(f"[bench] autotuned path capacity {tuned.path_capacity}: {tuned_ms:.2f} ms", file=sys.stderr, flush=True)
+    tuned = None
+    if autotune:
+        tuned, table = tune_render(st, cam, g, RENDER.NEE, pixels=torch.empty_like(pixels))
+        tuned_px = torch.empty_like(pixels)
+        tuned_ms = _timed(lambda: enqueue_render_gpu(tuned, cam, g, RENDER.NEE, pixels=tuned_px), stream)
+        tuned_equal = bool(torch.equal(tuned_px, pixels))
+        print(f"[bench] autotuned path capacity {tuned.path_capacity}: {tuned_ms:.2f} ms", file=sys.stderr,
+              flush=True)
     t = GpuRtRenderTarget(st, cam, RENDER.NEE, ctx, environments=count)
     t.capture(g)
     replay_ms = _timed(t.replay, stream)
@@ -477,10 +480,11 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
            "paths": count * width * height * spp, "rays": rays, "extension_rays": first.stats["extension_rays"],
            "shadow_rays": first.stats["shadow_rays"], "batched_render_ms": render_ms,
            "batched_render_mrays_s": rays / (render_ms / 1e3) / 1e6, "graph_replay_ms": replay_ms,
-           "graph_replay_mrays_s": rays / (replay_ms / 1e3) / 1e6, "graph_equals_render": same,
-           "autotuned": {"path_capacity": tuned.path_capacity, "candidates_ms": {str(k): v for k, v in table.items()},
-                         "render_ms": tuned_ms, "mrays_s": rays / (tuned_ms / 1e3) / 1e6,
-                         "images_equal_default": tuned_equal}}
+           "graph_replay_mrays_s": rays / (replay_ms / 1e3) / 1e6, "graph_equals_render": same}
+    if tuned is not None:
+        out["autotuned"] = {"path_capacity": tuned.path_capacity,
+                            "candidates_ms": {str(k): v for k, v in table.items()}, "render_ms": tuned_ms,
+                            "mrays_s": rays / (tuned_ms / 1e3) / 1e6, "images_equal_default": tuned_equal}
     if not per_env_loop:
         return out
     # the per-environment loop: every environment a scene of its own, one render call each; the scenes are
@@ -596,7 +600,8 @@ def main() -> int:
                     help="run only the batched-environment workload and print its JSON")
     ap.add_argument("--target-only", action="store_true", help="run only the render-target workload")
     ap.add_argument("--batched-render-only", action="store_true",
-                    help="run only the batched render (without its per-environment loop, for profiling it)")
+                    help="run only the batched render (without its autotuner and per-environment loop, for "
+                         "profiling it)")
     ap.add_argument(
         "--copy-calibration-mib",
         type=int,
@@ -630,7 +635,8 @@ def main() -> int:
               flush=True)
         return 0
     if args.batched_render_only:
-        print(json.dumps({"batched_render": batched_render_workload(ctx, per_env_loop=False)}), flush=True)
+        print(json.dumps({"batched_render": batched_render_workload(ctx, per_env_loop=False, autotune=False)}),
+              flush=True)
         return 0
     if args.target_only:
         print(json.dumps({"render_target": render_target_workload(ctx)}), flush=True)

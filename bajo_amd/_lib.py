@@ -226,6 +226,9 @@ SIGNATURES = {
     "bajo_render_target_create": (
         c_int32, [c_void_p, POINTER(RenderSettings), POINTER(c_float), c_void_p, POINTER(c_void_p)],
     ),
+    "bajo_render_cameras": (
+        c_int32, [c_void_p, c_void_p, POINTER(RenderSettings), c_void_p, c_uint32, c_void_p, POINTER(RenderStats)],
+    ),
     "bajo_render_target_create_envs": (
         c_int32, [c_void_p, POINTER(RenderSettings), c_uint32, POINTER(c_float), c_void_p, POINTER(c_void_p)],
     ),

@@ -133,6 +133,7 @@ struct PrimaryArgs {
   const float* cam;  // 20 floats in device memory (the frame's camera)
   uint32_t width, height, spp;
   uint32_t env_paths;  // paths per environment of a batch (0: one scene)
+  uint32_t cam_stride; // 20: one camera per environment at cam + 20 e; 0: one camera
 };
 
 // The camera path of `path_id`: origin, t_min, direction, t_max, throughput
@@ -141,10 +142,12 @@ BJ_D void primary_path(const PrimaryArgs& pa, uint32_t path_id, uint64_t seed, f
   const uint32_t width = pa.width, height = pa.height, spp = pa.spp;
   uint32_t local = path_id;
   uint64_t key = seed;
+  const float* __restrict__ k = pa.cam;
   if (pa.env_paths) {
     const uint32_t e = path_id / pa.env_paths;
     local = path_id - e * pa.env_paths;
     key = seed + e;
+    k += (uint64_t)e * pa.cam_stride;
   }
   const uint32_t pixel = local / spp;
   const uint32_t px = pixel % width, py = pixel / width;
@@ -155,7 +158,6 @@ BJ_D void primary_path(const PrimaryArgs& pa, uint32_t path_id, uint64_t seed, f
   const float rr = sqrtf(rng.next());
   const float lens_u = rr * c, lens_v = rr * s;
   const float pu = rng.next(), pv = rng.next();
-  const float* __restrict__ k = pa.cam;
   const float fw = (float)width, fh = (float)height;
   const float aspect = fw / fh;
   const float sx = (((float)px + pu) / fw) * 2.0f - 1.0f;
@@ -963,6 +965,8 @@ struct RenderArena {
   unsigned long long* work_ext = nullptr;  // per-block traversal work partials
   unsigned long long* work_sh = nullptr;
   float* cam = nullptr;                    // 20 floats
+  float* cams = nullptr;                   // 20 floats per environment (per-environment cameras)
+  uint32_t cams_cap = 0;
 };
 
 struct bajo_scene {
@@ -1004,7 +1008,7 @@ void free_arena(RenderArena& a) {
   (void)hipFree(a.ids[0]); (void)hipFree(a.ids[1]); (void)hipFree(a.fields[0]); (void)hipFree(a.fields[1]);
   (void)hipFree(a.shadow_ids); (void)hipFree(a.shadow_fields); (void)hipFree(a.sample); (void)hipFree(a.counters);
   (void)hipFree(a.stats); (void)hipFree(a.spill); (void)hipFree(a.work_ext); (void)hipFree(a.work_sh);
-  (void)hipFree(a.cam);
+  (void)hipFree(a.cam); (void)hipFree(a.cams);
   a = RenderArena();
 }
 
@@ -1167,7 +1171,7 @@ int32_t render_shape(const bajo_scene* s, const bajo_render_settings* st, Render
 // NULL renders with the camera the buffer already holds.
 template <int kAlg, bool kWide, int kGeo>
 int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st, const float* camera,
-                   float* pixels, LaunchEvents* ev) {
+                   uint32_t camera_count, float* pixels, LaunchEvents* ev) {
   const uint64_t total = render_paths(s, st);
   RenderShape shape;
   int32_t rc = render_shape<kAlg, kWide, kGeo>(s, st, &shape);
@@ -1180,14 +1184,29 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   rc = ensure_arena(a, cap, shape.spill_threads);
   if (rc) return rc;
   hipStream_t stream = ctx->stream;
-  if (camera) {
+  PrimaryArgs pa;
+  pa.cam = a.cam;
+  pa.cam_stride = 0;
+  if (camera && camera_count > 1) {
+    // one camera per environment of a batch
+    BJ_REQUIRE(camera_count == s->env_count, "bajo_render: %u cameras for %u environments", camera_count,
+               s->env_count);
+    if (camera_count > a.cams_cap) {
+      (void)hipFree(a.cams);
+      a.cams = nullptr;
+      a.cams_cap = 0;
+      BJ_HIP(hipMalloc(&a.cams, (size_t)camera_count * 20 * sizeof(float)));
+      a.cams_cap = camera_count;
+    }
+    BJ_HIP(hipMemcpyAsync(a.cams, camera, (size_t)camera_count * 20 * sizeof(float), hipMemcpyHostToDevice, stream));
+    pa.cam = a.cams;
+    pa.cam_stride = 20;
+  } else if (camera) {
     Cam c;
     memcpy(c.v, camera, sizeof(c.v));
     set_camera_kernel<<<1, 64, 0, stream>>>(a.cam, c);
     BJ_HIP(hipGetLastError());
   }
-  PrimaryArgs pa;
-  pa.cam = a.cam;
   pa.width = st->image_width;
   pa.height = st->image_height;
   pa.spp = st->samples_per_pixel;
@@ -1292,10 +1311,13 @@ struct RenderCall {
   RenderArena* a;
   const bajo_render_settings* st;
   const float* camera;
+  uint32_t camera_count;
   float* pixels;
   LaunchEvents* ev;
   template <int kAlg, bool kWide, int kGeo>
-  int32_t operator()() const { return run_render<kAlg, kWide, kGeo>(ctx, s, *a, st, camera, pixels, ev); }
+  int32_t operator()() const {
+    return run_render<kAlg, kWide, kGeo>(ctx, s, *a, st, camera, camera_count, pixels, ev);
+  }
 };
 
 // Sizes arena `a` for a render of `st` over scene `s` without enqueueing it.
@@ -1307,8 +1329,8 @@ int32_t prepare_arena(const bajo_scene* s, RenderArena& a, const bajo_render_set
 }
 
 int32_t dispatch_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st,
-                        const float* camera, float* pixels, LaunchEvents* ev) {
-  return by_algorithm(s, st->algorithm, RenderCall{ctx, s, &a, st, camera, pixels, ev});
+                        const float* camera, uint32_t camera_count, float* pixels, LaunchEvents* ev) {
+  return by_algorithm(s, st->algorithm, RenderCall{ctx, s, &a, st, camera, camera_count, pixels, ev});
 }
 
 }  // namespace
@@ -1639,7 +1661,14 @@ int32_t bajo_scene_light_table(const bajo_scene* s, float* fields, uint32_t* kin
 
 int32_t bajo_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st, const float* camera, float* pixels,
                     bajo_render_stats* stats) {
+  return bajo_render_cameras(ctx, s, st, camera, 1, pixels, stats);
+}
+
+int32_t bajo_render_cameras(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st, const float* camera,
+                            uint32_t camera_count, float* pixels, bajo_render_stats* stats) {
   BJ_REQUIRE(ctx && s && st && camera && pixels, "bajo_render: NULL argument");
+  BJ_REQUIRE(camera_count == 1 || camera_count == s->env_count,
+             "bajo_render: %u cameras; one, or one per environment (%u)", camera_count, s->env_count);
   BJ_REQUIRE(st->image_width > 0 && st->image_height > 0 && st->samples_per_pixel > 0,
              "bajo_render: image size and samples per pixel must be positive");
   BJ_REQUIRE(st->max_depth <= kMaxBounces, "bajo_render: max_depth above %d", kMaxBounces);
@@ -1653,7 +1682,7 @@ int32_t bajo_render(bajo_ctx* ctx, bajo_scene* s, const bajo_render_settings* st
     ev->end = ev->make();
     BJ_HIP(hipEventRecord(ev->begin, ctx->stream));
   }
-  rc = dispatch_render(ctx, s, s->arena, st, camera, pixels, ev);
+  rc = dispatch_render(ctx, s, s->arena, st, camera, camera_count, pixels, ev);
   if (rc) return rc;
   if (stats) {
     BJ_HIP(hipEventRecord(ev->end, ctx->stream));
@@ -1787,7 +1816,7 @@ int32_t bajo_render_enqueue(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s)
   BJ_REQUIRE((s->env_count ? s->env_count : 1u) == t->environments,
              "bajo_render_enqueue: the scene has %u environments, the target's pixels hold %u",
              s->env_count ? s->env_count : 1u, t->environments);
-  return dispatch_render(ctx, s, t->arena, &t->settings, nullptr, t->pixels, nullptr);
+  return dispatch_render(ctx, s, t->arena, &t->settings, nullptr, 0, t->pixels, nullptr);
 }
 
 int32_t bajo_render_target_capture(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s) {
@@ -1809,7 +1838,7 @@ int32_t bajo_render_target_capture(bajo_ctx* ctx, bajo_render_target* t, bajo_sc
     (void)hipStreamDestroy(cs);
     return hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
   }
-  rc = dispatch_render(&cap, s, t->arena, &t->settings, nullptr, t->pixels, nullptr);
+  rc = dispatch_render(&cap, s, t->arena, &t->settings, nullptr, 0, t->pixels, nullptr);
   hipGraph_t g = nullptr;
   e = hipStreamEndCapture(cs, &g);
   (void)hipStreamDestroy(cs);

@@ -245,12 +245,20 @@ def enqueue_render_gpu(
     require_device(pixels, ctx, "pixels")
     if pixels.numel() < 3 * w * h * envs:
         raise ValueError("pixel buffer too small")
-    cam = (ctypes.c_float * 20)(*[float(x) for x in np.asarray(camera, np.float32).reshape(-1)[:20]])
+    cams = np.ascontiguousarray(np.asarray(camera, np.float32).reshape(-1))
     st = settings.as_c(algorithm)
     stats = _lib.RenderStats() if with_stats else None
-    _lib.check(
-        _lib.lib().bajo_render(ctx.handle, scene.handle, ctypes.byref(st), cam, ptr(pixels), ctypes.byref(stats) if stats else None)
-    )
+    if cams.size > 20:
+        # one camera per environment of a batch: f32[E, 20]
+        if cams.size != 20 * envs:
+            raise ValueError(f"{cams.size // 20} cameras for {envs} environments")
+        _lib.check(_lib.lib().bajo_render_cameras(
+            ctx.handle, scene.handle, ctypes.byref(st), cams.ctypes.data_as(ctypes.c_void_p), envs, ptr(pixels),
+            ctypes.byref(stats) if stats else None))
+    else:
+        cam = (ctypes.c_float * 20)(*[float(x) for x in cams[:20]])
+        _lib.check(_lib.lib().bajo_render(ctx.handle, scene.handle, ctypes.byref(st), cam, ptr(pixels),
+                                          ctypes.byref(stats) if stats else None))
     out = None
     if stats is not None:
         out = {name: getattr(stats, name) for name, _ in _lib.RenderStats._fields_}

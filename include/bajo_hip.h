@@ -434,6 +434,12 @@ typedef struct bajo_render_target bajo_render_target;
 int32_t bajo_render_target_create(bajo_ctx* ctx, const bajo_render_settings* settings, const float* camera,
                                   float* pixels, bajo_render_target** out);
 int32_t bajo_render_target_set_camera(bajo_ctx* ctx, bajo_render_target* target, const float* camera);
+/* bajo_render with one camera per environment of a batch: cameras f32[20 E]
+ * (camera_count = E; 1 = one camera for all, as bajo_render).  Environment
+ * e's image equals a render of its scene alone through camera e with seed
+ * rng_seed + e. */
+int32_t bajo_render_cameras(bajo_ctx* ctx, bajo_scene* scene, const bajo_render_settings* settings,
+                            const float* cameras, uint32_t camera_count, float* pixels, bajo_render_stats* stats);
 /* A render target whose pixels (f32[E h w 3]) hold `environments` images:
  * it renders batches of that many environments (bajo_scene_create_batch). */
 int32_t bajo_render_target_create_envs(bajo_ctx* ctx, const bajo_render_settings* settings, uint32_t environments,

@@ -19,6 +19,8 @@ BAJO_BUILD_LBVH = 0
 BAJO_BUILD_HPLOC = 1
 BAJO_WIDTH_BINARY = 2
 BAJO_WIDTH_WIDE8 = 8
+BAJO_STATUS_SORT_LOOKBACK = 4
+BAJO_STATUS_REFIT_CORRUPT = 5
 
 
 class BajoError(RuntimeError):
@@ -126,6 +128,8 @@ SIGNATURES = {
     "bajo_ctx_create": (c_int32, [c_int32, c_void_p, POINTER(c_void_p)]),
     "bajo_ctx_destroy": (c_int32, [c_void_p]),
     "bajo_ctx_synchronize": (c_int32, [c_void_p]),
+    "bajo_ctx_status": (c_int32, [c_void_p, POINTER(c_uint32)]),
+    "bajo_ctx_set_spin_limit": (c_int32, [c_void_p, c_uint32]),
     "bajo_last_error": (c_char_p, []),
     "bajo_status_string": (c_char_p, [c_int32]),
     "bajo_device_name": (c_int32, [c_int32, c_char_p, c_int32]),

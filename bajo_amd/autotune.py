@@ -11,17 +11,19 @@ streams and owns its sample slot, so the tuner only measures time.
 
 `tune_render` times each candidate with HIP events on the context's stream
 (one untimed render first, which also sizes the path arena) and returns the
-fastest settings. Results are cached per (scene, image size, spp, depth,
-algorithm).
+fastest settings. Results are cached per scene object (a weak reference, so
+the entry goes with the scene) and, within it, per (native handle, build
+count, image size, spp, depth, algorithm): a rebuilt scene is timed again.
 """
 
 from __future__ import annotations
 
+import weakref
 from dataclasses import replace
 
 import torch
 
-_CACHE: dict = {}
+_CACHE: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
 
 
 def default_candidates(total_paths: int, spp: int) -> list[int]:
@@ -53,10 +55,11 @@ def tune_render(settings, camera, scene, algorithm: int, candidates: list[int] |
 
     envs = getattr(scene, "env_count", 1)
     total = settings.image_width * settings.image_height * settings.samples_per_pixel * envs
-    key = (id(scene), scene.triangle_count, envs, settings.image_width, settings.image_height,
-           settings.samples_per_pixel, settings.max_depth, algorithm)
-    if key in _CACHE and candidates is None:
-        best, table = _CACHE[key]
+    key = (scene.handle.value, getattr(scene, "builds", 0), scene.triangle_count, envs, settings.image_width,
+           settings.image_height, settings.samples_per_pixel, settings.max_depth, algorithm)
+    per_scene = _CACHE.setdefault(scene, {})
+    if key in per_scene and candidates is None:
+        best, table = per_scene[key]
         return replace(settings, path_capacity=best), dict(table)
     cands = candidates if candidates is not None else default_candidates(total, settings.samples_per_pixel)
     stream = scene.ctx.stream
@@ -69,5 +72,5 @@ def tune_render(settings, camera, scene, algorithm: int, candidates: list[int] |
         table[int(cap)] = _time(lambda: enqueue_render_gpu(st, camera, scene, algorithm, pixels=out), stream, reps)
     best = min(table, key=table.get)
     if candidates is None:
-        _CACHE[key] = (best, dict(table))
+        per_scene[key] = (best, dict(table))
     return replace(settings, path_capacity=best), table

@@ -31,7 +31,18 @@ class DeviceContext:
         return self._handle
 
     def synchronize(self) -> None:
+        """Waits for the stream; raises BajoError if a kernel raised a device status since the last check."""
         _lib.check(_lib.lib().bajo_ctx_synchronize(self._handle))
+
+    def status(self) -> int:
+        """Synchronizes, returns the device status word (0 = none raised) and clears it."""
+        word = ctypes.c_uint32()
+        _lib.check(_lib.lib().bajo_ctx_status(self._handle, ctypes.byref(word)))
+        return word.value
+
+    def set_spin_limit(self, spin_limit: int) -> None:
+        """Spin cap of the sort's look-back wait (0 = default); a tile past it raises BAJO_STATUS_SORT_LOOKBACK."""
+        _lib.check(_lib.lib().bajo_ctx_set_spin_limit(self._handle, int(spin_limit)))
 
     def name(self) -> str:
         buf = ctypes.create_string_buffer(256)

@@ -116,12 +116,16 @@ int32_t hip_fail(hipError_t e, const char* what, const char* file, int line);
 
 // Library context: one device, one stream, plus a growable scratch arena
 // that callers without their own workspace borrow.
+// `status` is a device word kernels raise BAJO_STATUS_* codes into
+// (atomicMax); it is read and cleared at the context's sync points.
 struct bajo_ctx {
   int device;
   hipStream_t stream;
   void* scratch;
   size_t scratch_bytes;
   int owns_stream;
+  uint32_t* status;
+  uint32_t spin_limit;
 };
 
 namespace bj {

@@ -56,6 +56,11 @@ const char* bajo_status_string(int32_t status) {
     case BAJO_ERR_OUT_OF_MEMORY: return "out of device memory";
     case BAJO_ERR_UNSUPPORTED: return "unsupported";
     case BAJO_STATUS_STACK_OVERFLOW: return "traversal stack overflow";
+    case BAJO_STATUS_BAD_NODE: return "traversal met a node outside the hierarchy";
+    case BAJO_STATUS_SORT_LOOKBACK: return "sort look-back gave up waiting (corrupted look-back table)";
+    case BAJO_STATUS_REFIT_CORRUPT: return "refit met an inconsistent hierarchy";
+    case BAJO_STATUS_PATH_OVERFLOW: return "path queue overflow";
+    case BAJO_STATUS_SHADOW_OVERFLOW: return "shadow queue overflow";
     default: return "unknown status";
   }
 }
@@ -73,23 +78,56 @@ int32_t bajo_ctx_create(int32_t device, void* stream, bajo_ctx** out) {
   ctx->scratch = nullptr;
   ctx->scratch_bytes = 0;
   ctx->owns_stream = 0;
+  ctx->status = nullptr;
+  ctx->spin_limit = 0;
+  hipError_t e = hipMalloc(&ctx->status, 4 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(ctx->status, 0, 4 * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    (void)hipFree(ctx->status);
+    delete ctx;
+    return bj::hip_fail(e, "hipMalloc(context status)", __FILE__, __LINE__);
+  }
   *out = ctx;
   return BAJO_OK;
 }
 
 int32_t bajo_ctx_destroy(bajo_ctx* ctx) {
   if (!ctx) return BAJO_OK;
-  if (ctx->scratch) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(ctx->scratch);
-  }
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  (void)hipFree(ctx->status);
   delete ctx;
+  return BAJO_OK;
+}
+
+int32_t bajo_ctx_status(bajo_ctx* ctx, uint32_t* status) {
+  BJ_REQUIRE(ctx != nullptr && status != nullptr, "bajo_ctx_status: NULL argument");
+  uint32_t word = 0;
+  BJ_HIP(hipMemcpyAsync(&word, ctx->status, sizeof(word), hipMemcpyDeviceToHost, ctx->stream));
+  BJ_HIP(hipStreamSynchronize(ctx->stream));
+  if (word) {
+    BJ_HIP(hipMemsetAsync(ctx->status, 0, sizeof(word), ctx->stream));
+    BJ_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  *status = word;
   return BAJO_OK;
 }
 
 int32_t bajo_ctx_synchronize(bajo_ctx* ctx) {
   BJ_REQUIRE(ctx != nullptr, "bajo_ctx_synchronize: ctx is NULL");
-  BJ_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t word = 0;
+  const int32_t rc = bajo_ctx_status(ctx, &word);
+  if (rc) return rc;
+  if (word) {
+    bj::set_error("device status %u: %s", word, bajo_status_string((int32_t)word));
+    return (int32_t)word;
+  }
+  return BAJO_OK;
+}
+
+int32_t bajo_ctx_set_spin_limit(bajo_ctx* ctx, uint32_t spin_limit) {
+  BJ_REQUIRE(ctx != nullptr, "bajo_ctx_set_spin_limit: ctx is NULL");
+  ctx->spin_limit = spin_limit;
   return BAJO_OK;
 }
 

@@ -256,7 +256,8 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
                                                        uint64_t node_total, BjNode* __restrict__ nodes,
                                                        uint32_t* __restrict__ leaf_parent,
                                                        const uint32_t* __restrict__ leaf_counts,
-                                                       uint32_t* __restrict__ flags, BjTri* __restrict__ tris) {
+                                                       uint32_t* __restrict__ flags, BjTri* __restrict__ tris,
+                                                       uint32_t* __restrict__ status) {
   __shared__ uint32_t s_flag[kBlock];
   __shared__ float s_box[kBlock][2][6];
   s_flag[threadIdx.x] = 0;
@@ -289,12 +290,21 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
   uint32_t cur = kLeafFlag | (uint32_t)s;
   uint32_t parent = leaf_parent[s];
   // A Karras hierarchy over 64-bit keys is at most 64 levels deep; the index
-  // and depth checks only keep a corrupted hierarchy from leaving the arrays.
-  for (int depth = 0; parent != kSentinel && parent < node_total && depth < 128; ++depth) {
+  // and depth checks only keep a corrupted hierarchy from leaving the arrays,
+  // and a climb they stop raises BAJO_STATUS_REFIT_CORRUPT.
+  int depth = 0;
+  for (; parent != kSentinel && depth < 128; ++depth) {
+    if (parent >= node_total) {
+      atomicMax(status, (uint32_t)BAJO_STATUS_REFIT_CORRUPT);
+      break;
+    }
     BjNode* node = nodes + parent;
     const uint32_t left = node->left, right = node->right;  // written by the topology launch
     const bool is_left = cur == left;
-    if (!is_left && cur != right) break;
+    if (!is_left && cur != right) {
+      atomicMax(status, (uint32_t)BAJO_STATUS_REFIT_CORRUPT);
+      break;
+    }
     // Karras node i is one end of its leaf range [fence + 1 - count, fence]
     const uint64_t last = node->fence, first = last + 1 - leaf_counts[parent];
     v3 olo, ohi;
@@ -356,6 +366,7 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
     cur = parent;
     parent = node->parent;
   }
+  if (depth == 128) atomicMax(status, (uint32_t)BAJO_STATUS_REFIT_CORRUPT);
 }
 
 // ---- segmented builds (bajo's BLAS sets) ------------------------------------
@@ -712,10 +723,12 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
     ev.record(4, st);
     if (h->boxes)
       refit_kernel<true><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, nullptr, n, n - 1, h->nodes,
-                                                               h->leaf_parent, h->leaf_counts, h->flags, h->tris);
+                                                               h->leaf_parent, h->leaf_counts, h->flags, h->tris,
+                                                               ctx->status);
     else
       refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, nullptr, n, n - 1, h->nodes,
-                                                                h->leaf_parent, h->leaf_counts, h->flags, h->tris);
+                                                                h->leaf_parent, h->leaf_counts, h->flags, h->tris,
+                                                                ctx->status);
     BJ_HIP(hipGetLastError());
   }
   ev.record(5, st);
@@ -819,7 +832,8 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
     if (rc) return rc;
   } else {
     refit_kernel<false><<<div_up(n, kBlock), kBlock, 0, st>>>(vertices, h->keys, h->seg_prim, n, n - 1, h->nodes,
-                                                              h->leaf_parent, h->leaf_counts, h->flags, h->tris);
+                                                              h->leaf_parent, h->leaf_counts, h->flags, h->tris,
+                                                              ctx->status);
     BJ_HIP(hipGetLastError());
   }
   ev.record(5, st);

@@ -1328,6 +1328,16 @@ int32_t prepare_arena(const bajo_scene* s, RenderArena& a, const bajo_render_set
   return ensure_arena(a, shape.cap, shape.spill_threads);
 }
 
+// Whether a render of `s` with `st` would reallocate arena `a` (its path
+// capacity or the spill columns of its widest launch grow).
+int32_t arena_grows(const bajo_scene* s, const RenderArena& a, const bajo_render_settings* st, bool* grows) {
+  RenderShape shape;
+  const int32_t rc = by_algorithm(s, st->algorithm, ShapeCall{s, st, &shape});
+  if (rc) return rc;
+  *grows = shape.cap > a.cap || shape.spill_threads > a.spill_threads;
+  return BAJO_OK;
+}
+
 int32_t dispatch_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st,
                         const float* camera, uint32_t camera_count, float* pixels, LaunchEvents* ev) {
   return by_algorithm(s, st->algorithm, RenderCall{ctx, s, &a, st, camera, camera_count, pixels, ev});
@@ -1816,6 +1826,17 @@ int32_t bajo_render_enqueue(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s)
   BJ_REQUIRE((s->env_count ? s->env_count : 1u) == t->environments,
              "bajo_render_enqueue: the scene has %u environments, the target's pixels hold %u",
              s->env_count ? s->env_count : 1u, t->environments);
+  if (t->exec) {
+    // a captured graph holds the arena's buffers: growing the arena frees
+    // them, so the graph goes first (a replay then reports nothing captured)
+    bool grows = false;
+    const int32_t rc = arena_grows(s, t->arena, &t->settings, &grows);
+    if (rc) return rc;
+    if (grows) {
+      BJ_HIP(hipStreamSynchronize(ctx->stream));
+      drop_graph(t);
+    }
+  }
   return dispatch_render(ctx, s, t->arena, &t->settings, nullptr, 0, t->pixels, nullptr);
 }
 

@@ -88,7 +88,8 @@ __global__ void __launch_bounds__(kSortBlock) sort_bin_kernel(const K* __restric
                                                               uint32_t* __restrict__ out_vals, uint64_t n,
                                                               uint32_t shift, const uint32_t* __restrict__ pass_hist,
                                                               uint32_t* __restrict__ lookback,
-                                                              uint32_t* __restrict__ ticket) {
+                                                              uint32_t* __restrict__ ticket,
+                                                              uint32_t* __restrict__ status, uint32_t spin_limit) {
   __shared__ K s_keys[kSortTile];
   __shared__ uint32_t s_vals[kValues ? kSortTile : 1];
   __shared__ uint32_t s_wave_hist[kSortWaves][256];
@@ -167,8 +168,12 @@ __global__ void __launch_bounds__(kSortBlock) sort_bin_kernel(const K* __restric
       uint32_t flag = v & ~kLookbackMask;
       if (flag == 0) {
         // Every predecessor holds an earlier ticket and is resident, so it
-        // publishes; the cap only bounds the wait if the table was corrupted.
-        if (++spins > kLookbackSpinLimit) break;
+        // publishes; the cap only bounds the wait if the table was corrupted,
+        // and giving up is reported through the status word.
+        if (++spins > spin_limit) {
+          atomicMax(status, (uint32_t)BAJO_STATUS_SORT_LOOKBACK);
+          break;
+        }
         __builtin_amdgcn_s_sleep(1);
         continue;
       }
@@ -236,14 +241,17 @@ int32_t run_typed(bajo_ctx* ctx, const SortPlan& p, K* keys, uint32_t* values, c
   K* dst = alt_keys;
   uint32_t* vsrc = values;
   uint32_t* vdst = alt_vals;
+  const uint32_t spin_limit = ctx->spin_limit ? ctx->spin_limit : kLookbackSpinLimit;
   for (int pass = 0; pass < p.passes; ++pass) {
     uint32_t shift = (uint32_t)(p.begin_bit + 8 * pass);
     if (p.has_values)
       sort_bin_kernel<K, true><<<p.tiles, kSortBlock, 0, ctx->stream>>>(
-          src, dst, vsrc, vdst, p.n, shift, hist + pass * 256, lookback + lb_pass_words * pass, tickets + pass);
+          src, dst, vsrc, vdst, p.n, shift, hist + pass * 256, lookback + lb_pass_words * pass, tickets + pass,
+          ctx->status, spin_limit);
     else
       sort_bin_kernel<K, false><<<p.tiles, kSortBlock, 0, ctx->stream>>>(
-          src, dst, nullptr, nullptr, p.n, shift, hist + pass * 256, lookback + lb_pass_words * pass, tickets + pass);
+          src, dst, nullptr, nullptr, p.n, shift, hist + pass * 256, lookback + lb_pass_words * pass, tickets + pass,
+          ctx->status, spin_limit);
     BJ_HIP(hipGetLastError());
     K* tk = src; src = dst; dst = tk;
     uint32_t* tv = vsrc; vsrc = vdst; vdst = tv;
@@ -356,5 +364,4 @@ int32_t bajo_onesweep_radix_sort_pairs_u64_u32(bajo_ctx* ctx, uint64_t* keys, ui
   return sort_entry(ctx, keys, values, n, 8, begin_bit, end_bit, workspace, workspace_bytes,
                     "bajo_onesweep_radix_sort_pairs_u64_u32");
 }
-
 }  // extern "C"

@@ -140,9 +140,12 @@ class GpuScene:
             raise RuntimeError("scene already freed")
         return self._handle
 
+    builds = 1  # BVH builds so far (the autotuner's cache keys on it)
+
     def rebuild(self, timings: bool = False):
         t = _lib.BuildTimings() if timings else None
         _lib.check(_lib.lib().bajo_scene_rebuild(self.ctx.handle, self.handle, ctypes.byref(t) if t else None))
+        self.builds += 1
         return t.as_dict() if t else None
 
     def light_table(self):

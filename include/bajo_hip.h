@@ -30,6 +30,12 @@ extern "C" {
 #define BAJO_ERR_UNSUPPORTED (-4)
 #define BAJO_STATUS_STACK_OVERFLOW 1 /* a traversal needed more than 64 stack entries */
 #define BAJO_STATUS_BAD_NODE 2       /* a traversal met a node index outside the hierarchy */
+/* Context status words (raised by kernels into the context's device status
+ * word, reported at its sync points, like bajo's HPLOC_STATUS / collapse
+ * TIMEOUT words, bvh/gpu/builder/hploc_layout.mojo:9-11,
+ * wide_collapse.mojo:34-38,236-254): */
+#define BAJO_STATUS_SORT_LOOKBACK 4  /* a sort tile gave up waiting for its predecessor (spin cap) */
+#define BAJO_STATUS_REFIT_CORRUPT 5  /* the refit climb met an inconsistent hierarchy or passed the depth bound */
 /* WAVE_STATUS words of the renderer, rt/wavefront_contract.mojo:25-29, offset by 10 */
 #define BAJO_STATUS_WAVE_BASE 10
 #define BAJO_STATUS_PATH_OVERFLOW 11
@@ -110,7 +116,20 @@ typedef struct {
  * NULL for the device's null stream. */
 int32_t bajo_ctx_create(int32_t device, void* stream, bajo_ctx** out);
 int32_t bajo_ctx_destroy(bajo_ctx* ctx);
+/* Synchronizes the context stream; returns the context status word (a
+ * positive BAJO_STATUS_*) if a kernel raised one since the last check, and
+ * clears it. */
 int32_t bajo_ctx_synchronize(bajo_ctx* ctx);
+/* Synchronizes and reads the context status word into *status (0 = none
+ * raised), then clears it.  Returns BAJO_OK unless the read itself fails. */
+int32_t bajo_ctx_status(bajo_ctx* ctx, uint32_t* status);
+/* Spin cap of the sort's decoupled look-back wait (0 = the default 2^24):
+ * a tile that waits longer gives up and raises BAJO_STATUS_SORT_LOOKBACK
+ * instead of waiting forever on a corrupted look-back table.  bajo's wide
+ * collapse has the same kind of fail-safe (a 10M-spin cap,
+ * bvh/gpu/builder/wide_collapse.mojo:236-254); a tiny cap lets tests force
+ * the status path from the host. */
+int32_t bajo_ctx_set_spin_limit(bajo_ctx* ctx, uint32_t spin_limit);
 const char* bajo_last_error(void);
 const char* bajo_status_string(int32_t status);
 int32_t bajo_device_name(int32_t device, char* out, int32_t cap);

@@ -32,6 +32,7 @@ _SIG = {
     "orc_lbvh_find_split": (c_int64, [c_void_p, c_int64, c_int64, c_int64]),
     "orc_karras_topology": (c_uint32, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p]),
     "orc_refit": (None, [c_uint32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "orc_refit_climb": (c_uint32, [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p]),
     "orc_build_lbvh": (
         c_uint32,
         [c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
@@ -244,6 +245,17 @@ def _build_lbvh(v, n, fn) -> dict:
     out["node_bounds"] = out["node_bounds"][: 12 * m]
     out["root"] = int(root)
     return out
+
+
+def refit_climb(leaf_bounds_sorted: np.ndarray, node_meta: np.ndarray, leaf_parent: np.ndarray):
+    """The GPU refit's bottom-up climb with its corruption checks: (status, node_bounds f32[12(n-1)])."""
+    b = f32(leaf_bounds_sorted)
+    n = b.size // 6
+    meta = np.ascontiguousarray(node_meta, dtype=np.uint32)
+    par = np.ascontiguousarray(leaf_parent, dtype=np.uint32)
+    out = np.zeros(12 * max(n - 1, 1), np.float32)
+    status = lib().orc_refit_climb(_p(b), n, _p(meta), _p(par), _p(out))
+    return int(status), out[: 12 * max(n - 1, 0)]
 
 
 def karras_topology(sorted_codes: np.ndarray):

@@ -31,6 +31,8 @@ uint32_t orc_karras_topology(const uint32_t *sorted_codes, uint64_t n, uint32_t 
                              uint32_t *leaf_counts);
 void orc_refit(uint32_t root, const float *leaf_bounds, const uint32_t *leaf_ids, const uint32_t *node_meta,
                float *node_bounds);
+uint32_t orc_refit_climb(const float *leaf_bounds, uint64_t n, const uint32_t *node_meta,
+                         const uint32_t *leaf_parent, float *node_bounds);
 uint32_t orc_build_lbvh(const float *vertices, uint64_t n, uint64_t *sorted_keys, uint32_t *leaf_ids,
                         uint32_t *node_meta, uint32_t *leaf_parent, uint32_t *leaf_counts, float *node_bounds,
                         float *scene12);

@@ -192,6 +192,44 @@ void orc_refit(uint32_t root, const float *leaf_bounds, const uint32_t *leaf_ids
     refit_rec(root, leaf_bounds, leaf_ids, node_meta, node_bounds, 0);
 }
 
+/* The GPU refit's bottom-up climb (refit_lbvh_bounds_from_leaves_kernel,
+ * bvh/gpu/builder/lbvh.mojo:116-161), run leaf by leaf: a leaf's box climbs
+ * through leaf_parent / node parents; the first child to arrive at a node
+ * stops, the second merges (left then right) and climbs on.  It restates the
+ * library's corruption checks: a parent index outside the node array, a node
+ * that does not list the arriving child, or a climb past 128 levels stops
+ * that climb and makes the result 5 (BAJO_STATUS_REFIT_CORRUPT); 0 otherwise.
+ * leaf_bounds are indexed by sorted position; node_bounds receive each
+ * node's left box at +0 and right box at +6. */
+uint32_t orc_refit_climb(const float *leaf_bounds, uint64_t n, const uint32_t *node_meta,
+                         const uint32_t *leaf_parent, float *node_bounds) {
+    if (n < 2) return 0;
+    const uint64_t m = n - 1;
+    uint32_t status = 0;
+    uint8_t *arrived = (uint8_t *)calloc(m, 1);
+    for (uint64_t s = 0; s < n; ++s) {
+        orc_aabb box = orc_aabb_load6(leaf_bounds + 6 * s);
+        uint32_t cur = ORC_LEAF_FLAG | (uint32_t)s;
+        uint32_t parent = leaf_parent[s];
+        int depth = 0;
+        for (; parent != ORC_SENTINEL && depth < 128; ++depth) {
+            if (parent >= m) { status = 5; break; }
+            const uint32_t *meta = node_meta + 4 * (uint64_t)parent;
+            const int is_left = cur == meta[1];
+            if (!is_left && cur != meta[2]) { status = 5; break; }
+            orc_aabb_store6(node_bounds + 12 * (uint64_t)parent + (is_left ? 0 : 6), box);
+            if (arrived[parent]++ == 0) break;
+            orc_aabb other = orc_aabb_load6(node_bounds + 12 * (uint64_t)parent + (is_left ? 6 : 0));
+            box = is_left ? orc_aabb_merge(box, other) : orc_aabb_merge(other, box);
+            cur = parent;
+            parent = meta[0];
+        }
+        if (depth == 128) status = 5;
+    }
+    free(arrived);
+    return status;
+}
+
 /* Full single-segment LBVH in the GPU builder's layout.  Outputs (caller
  * allocated): sorted_keys u64[n] = (code<<32)|prim, leaf_ids u32[n],
  * node_meta u32[4(n-1)], leaf_parent u32[n], leaf_counts u32[n-1],

@@ -105,3 +105,38 @@ def test_target_rejects_bad_use(gpu_ctx):
     with pytest.raises(BajoError, match="nothing captured"):
         t.replay()
     t.free()
+
+
+def test_enqueue_of_another_scene_never_leaves_a_stale_graph(gpu_ctx):
+    """A captured graph holds the target's arena buffers.  Enqueueing a scene whose
+    launches need a larger arena (another node width or geometry can change the
+    tail kernel's resident grid) must drop the graph before the arena is reallocated:
+    afterwards a replay either reports that nothing is captured or still renders the
+    captured scene exactly."""
+    from bajo_amd import scenes
+    from bajo_amd._lib import BajoError
+    from bajo_amd.rt import RENDER, GpuRtRenderTarget, GpuScene, RenderSettings
+
+    announce("render target: capture, then enqueue scenes of other widths and geometry")
+    settings = RenderSettings(32, 24, 2, 9, 5)
+    ca, _ = cams()
+    binary = GpuScene(scenes.box_scene(), node_width=2)
+    want = eager(settings, ca, binary, RENDER.NEE)
+    others = [GpuScene(scenes.box_scene(), node_width=8), GpuScene(scenes.mixed_box_scene(), node_width=8),
+              GpuScene(scenes.mixed_box_scene(), node_width=2)]
+    t = GpuRtRenderTarget(settings, ca, RENDER.NEE)
+    t.capture(binary)
+    np.testing.assert_array_equal(host(t.replay()), want)
+    for k, other in enumerate(others):
+        np.testing.assert_array_equal(host(t.enqueue(other)), eager(settings, ca, other, RENDER.NEE),
+                                      err_msg=f"enqueue of scene {k}")
+        try:
+            got = host(t.replay())
+        except BajoError as e:
+            assert "nothing captured" in str(e)
+            t.capture(binary)  # capture again for the next scene
+            got = host(t.replay())
+        np.testing.assert_array_equal(got, want, err_msg=f"replay after scene {k}")
+    t.free()
+    for g in (binary, *others):
+        g.free()

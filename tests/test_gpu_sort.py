@@ -112,3 +112,39 @@ def test_sort_with_caller_workspace(gpu_ctx):
     small = torch.empty(16, dtype=torch.uint8, device="cuda:0")
     with pytest.raises(Exception, match="workspace"):
         sort.device_radix_sort_pairs(k, v, workspace=small)
+
+
+def test_lookback_give_up_is_reported_never_silent(gpu_ctx):
+    """The look-back wait is capped; a tile past the cap raises BAJO_STATUS_SORT_LOOKBACK.
+    With the cap forced to 1 spin from the host (every kernel still exits normally), a
+    multi-tile sort either comes out right or reports the status: never silently wrong.
+    With the default cap the same sort is right and reports nothing."""
+    from bajo_amd import sort
+    from bajo_amd._lib import BAJO_STATUS_SORT_LOOKBACK
+    from bajo_amd.context import DeviceContext
+
+    announce("sort look-back spin cap forced to 1")
+    ctx = DeviceContext(0)
+    n = 1 << 21
+    rng = np.random.default_rng(77)
+    keys = rng.integers(0, 2**64 - 1, n, dtype=np.uint64, endpoint=True)
+    want = np.sort(keys)
+    ctx.set_spin_limit(1)
+    silent_wrong = 0
+    raised = 0
+    for _ in range(3):
+        k = to_dev(keys)
+        sort.device_radix_sort_keys(k, ctx=ctx)
+        status = ctx.status()
+        right = np.array_equal(to_host(k, np.uint64), want)
+        assert status in (0, BAJO_STATUS_SORT_LOOKBACK)
+        raised += status != 0
+        silent_wrong += (not right) and status == 0
+    assert silent_wrong == 0
+    ctx.set_spin_limit(0)
+    k = to_dev(keys)
+    sort.device_radix_sort_keys(k, ctx=ctx)
+    assert ctx.status() == 0
+    np.testing.assert_array_equal(to_host(k, np.uint64), want)
+    print(f"forced give-ups reported in {raised} of 3 sorts")
+    ctx.close()

@@ -84,6 +84,8 @@ class Materials(ctypes.Structure):
         ("n_dielectric", c_uint32),
         ("emissive", c_void_p),
         ("n_emissive", c_uint32),
+        ("ggx", c_void_p),
+        ("n_ggx", c_uint32),
     ]
 
 

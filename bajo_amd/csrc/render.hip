@@ -31,6 +31,8 @@ using namespace bj;
 constexpr int kRtBlock = kTraceBlock;
 // MAT kinds (rt/types.mojo:24-29)
 constexpr uint32_t kMatLambertian = 0, kMatMetal = 1, kMatDielectric = 2, kMatEmissive = 3;
+// a GGX microfacet conductor (F0.rgb, roughness per material); bajo has no counterpart
+constexpr uint32_t kMatGgx = 4;
 constexpr uint32_t kSurfaceIndexMask = (1u << 28) - 1;
 constexpr uint32_t kPrimSphere = 0, kPrimTriangle = 1;  // bajo's PRIM values
 constexpr uint32_t kSphereHit = 1u;  // Hit.inst of a sphere hit in a mixed scene
@@ -65,6 +67,7 @@ struct SceneView {
   const float* lambertian;
   const float* metals;       // albedo.xyz, fuzz per metal
   const float* dielectrics;  // refraction index per dielectric
+  const float* ggx;          // specular color F0.xyz, roughness per GGX material
   const float* emissive;
   const float* light_fields;
   const uint32_t* light_kinds;
@@ -363,6 +366,11 @@ BJ_D bool direct_light(const SceneView& sc, uint64_t seed, uint32_t path_id, uin
     const BsdfEval ev = eval_metal(ray_d, normal, mk(m[0], m[1], m[2]), m[3], dir);
     value = ev.value;
     bsdf_pdf = ev.pdf;
+  } else if (kind == kMatGgx) {
+    const float* g = sc.ggx + 4 * (uint64_t)mi;
+    const BsdfEval ev = eval_ggx(ray_d, normal, mk(g[0], g[1], g[2]), g[3], dir);
+    value = ev.value;
+    bsdf_pdf = ev.pdf;
   } else {
     // _evaluate_lambertian (rt/shading.mojo:13-27)
     const float* a = sc.lambertian + 3 * (uint64_t)mi;
@@ -455,8 +463,9 @@ BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint
     sw.id = path_id;
     return true;
   } else if constexpr (kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
-    // _sample_direct_light_candidate (rt/gpu/path_shading.mojo:261-403): Lambertian and metal surfaces
-    if (su.kind != kMatLambertian && su.kind != kMatMetal) return false;
+    // _sample_direct_light_candidate (rt/gpu/path_shading.mojo:261-403): Lambertian and metal
+    // surfaces, and GGX surfaces (their BSDF has a density everywhere above the surface)
+    if (su.kind != kMatLambertian && su.kind != kMatMetal && su.kind != kMatGgx) return false;
     return direct_light<kAlg>(sc, seed, path_id, bounce, su.point, su.n, su.kind, su.mi, ray_d, T, sw);
   } else {
     return false;
@@ -547,12 +556,16 @@ BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, 
         next_path(np, path_id, false, ray, t_hit, dir, Tn, pdf);
         out |= kShadeNext;
       }
-    } else if (su.kind == kMatMetal || su.kind == kMatDielectric) {
+    } else if (su.kind == kMatMetal || su.kind == kMatDielectric || su.kind == kMatGgx) {
       // _gpu_rt_shade_one (rt/gpu/path_shading.mojo:669-783): random numbers
       // are drawn only where the sample uses them
       StageRng rng = path_rng(sc, seed, path_id, rng_bsdf_stage(bounce));
       BsdfSample smp;
-      if (su.kind == kMatMetal) {
+      if (su.kind == kMatGgx) {
+        const float* g = sc.ggx + 4 * (uint64_t)su.mi;
+        const float ru = rng.next(), rv = rng.next();
+        smp = sample_ggx(ray.d, su.n, mk(g[0], g[1], g[2]), g[3], ru, rv);
+      } else if (su.kind == kMatMetal) {
         const float* m = sc.metals + 4 * (uint64_t)su.mi;
         float ru = 0.0f, rv = 0.0f;
         if (m[3] > 1.0e-4f) {
@@ -979,6 +992,7 @@ struct bajo_scene {
   float* lambertian = nullptr;
   float* metals = nullptr;
   float* dielectrics = nullptr;
+  float* ggx = nullptr;
   float* emissive = nullptr;
   float* light_fields = nullptr;
   uint32_t* light_kinds = nullptr;
@@ -1021,7 +1035,7 @@ void free_scene(bajo_scene* s) {
   bajo_bvh_free(s->sphere_bvh);
   (void)hipFree(s->spheres); (void)hipFree(s->sphere_radii); (void)hipFree(s->sphere_surfaces);
   (void)hipFree(s->vertices); (void)hipFree(s->surfaces); (void)hipFree(s->lambertian); (void)hipFree(s->emissive);
-  (void)hipFree(s->metals); (void)hipFree(s->dielectrics);
+  (void)hipFree(s->metals); (void)hipFree(s->dielectrics); (void)hipFree(s->ggx);
   (void)hipFree(s->light_fields); (void)hipFree(s->light_kinds);
   delete s;
 }
@@ -1221,6 +1235,7 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   sv.lambertian = s->lambertian;
   sv.metals = s->metals;
   sv.dielectrics = s->dielectrics;
+  sv.ggx = s->ggx;
   sv.emissive = s->emissive;
   sv.light_fields = s->light_fields;
   sv.light_kinds = s->light_kinds;
@@ -1392,16 +1407,21 @@ int32_t scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, const uin
   BJ_REQUIRE(n_spheres == 0 || (spheres && sphere_surfaces), "bajo_scene_create: spheres need their surfaces");
   BJ_REQUIRE(n < (1ull << 31) && n_spheres < (1ull << 31), "bajo_scene_create: too many primitives");
   const float *lambertian = materials->lambertian, *metal = materials->metal, *dielectric = materials->dielectric,
-              *emissive = materials->emissive;
+              *emissive = materials->emissive, *ggx = materials->ggx;
   const uint32_t n_lambertian = materials->n_lambertian, n_metal = materials->n_metal,
-                 n_dielectric = materials->n_dielectric, n_emissive = materials->n_emissive;
+                 n_dielectric = materials->n_dielectric, n_emissive = materials->n_emissive,
+                 n_ggx = materials->n_ggx;
   BJ_REQUIRE((n_lambertian == 0 || lambertian) && (n_metal == 0 || metal) && (n_dielectric == 0 || dielectric) &&
-                 (n_emissive == 0 || emissive),
+                 (n_emissive == 0 || emissive) && (n_ggx == 0 || ggx),
              "bajo_scene_create: NULL material array with a nonzero count");
+  for (uint32_t i = 0; i < n_ggx; ++i)
+    BJ_REQUIRE(ggx[4 * i + 3] > 0.0f && ggx[4 * i + 3] <= 1.0f,
+               "bajo_scene_create: GGX material %u has roughness %g outside (0, 1]", i, (double)ggx[4 * i + 3]);
   auto check_surface = [&](uint32_t sid, const char* what, uint64_t i) -> int32_t {
     const uint32_t kind = sid >> 28, mi = sid & kSurfaceIndexMask;
-    BJ_REQUIRE(kind <= kMatEmissive, "bajo_scene_create: %s %llu has material kind %u", what, (unsigned long long)i,
+    BJ_REQUIRE(kind <= kMatGgx, "bajo_scene_create: %s %llu has material kind %u", what, (unsigned long long)i,
                kind);
+    BJ_REQUIRE(kind != kMatGgx || mi < n_ggx, "bajo_scene_create: GGX index out of range");
     BJ_REQUIRE(kind != kMatLambertian || mi < n_lambertian, "bajo_scene_create: Lambertian index out of range");
     BJ_REQUIRE(kind != kMatMetal || mi < n_metal, "bajo_scene_create: metal index out of range");
     BJ_REQUIRE(kind != kMatDielectric || mi < n_dielectric, "bajo_scene_create: dielectric index out of range");
@@ -1438,6 +1458,8 @@ int32_t scene_create(bajo_ctx* ctx, const float* vertices, uint64_t n, const uin
   if (n_metal) SC_HIP(hipMemcpy(s->metals, metal, n_metal * 4 * sizeof(float), hipMemcpyHostToDevice));
   SC_HIP(hipMalloc(&s->dielectrics, (n_dielectric ? n_dielectric : 1) * sizeof(float)));
   if (n_dielectric) SC_HIP(hipMemcpy(s->dielectrics, dielectric, n_dielectric * sizeof(float), hipMemcpyHostToDevice));
+  SC_HIP(hipMalloc(&s->ggx, (n_ggx ? n_ggx : 1) * 4 * sizeof(float)));
+  if (n_ggx) SC_HIP(hipMemcpy(s->ggx, ggx, n_ggx * 4 * sizeof(float), hipMemcpyHostToDevice));
   SC_HIP(hipMalloc(&s->emissive, (n_emissive ? n_emissive : 1) * 3 * sizeof(float)));
   if (n_emissive) SC_HIP(hipMemcpy(s->emissive, emissive, n_emissive * 3 * sizeof(float), hipMemcpyHostToDevice));
   // _build_light_store (rt/types.mojo:490-530): one light per emissive

@@ -267,4 +267,120 @@ BJ_HD BsdfSample sample_dielectric(v3 ray_d, v3 n, float ior, bool front, float 
   return out;
 }
 
+// ---- GGX microfacet conductor (no bajo counterpart) ----------------------------
+// Trowbridge-Reitz (GGX) normal distribution of width alpha = the material's
+// roughness, Smith height-correlated masking-shadowing
+// G2 = 1 / (1 + Lambda(wo) + Lambda(wi)), Schlick Fresnel with F0 = the
+// material's specular color.  Directions are drawn from the distribution of
+// visible normals (Heitz 2018, "Sampling the GGX Distribution of Visible
+// Normals", JCGT 7(4)): pdf(wi) = G1(wo) D(h) / (4 cos_o), weight
+// f cos_i / pdf = F G2 / G1(wo).  The local frame around the shading normal
+// is Duff et al. 2017 ("Building an Orthonormal Basis, Revisited", JCGT
+// 6(1)).  Plain FP32 operations in a fixed order, so the CPU restatement
+// gives the same bits.
+constexpr float kPi = 3.14159265358979323846f;
+
+struct Frame {
+  v3 t, b, n;
+};
+BJ_HD Frame frame_of(v3 n) {
+  const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  Frame f;
+  f.t = mk(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  f.b = mk(b, sign + n.y * n.y * a, -n.y);
+  f.n = n;
+  return f;
+}
+BJ_HD v3 to_local(const Frame& f, v3 v) { return mk(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
+BJ_HD v3 to_world(const Frame& f, v3 v) { return add(add(scale(f.t, v.x), scale(f.b, v.y)), scale(f.n, v.z)); }
+
+// D(h) for cos_h = n.h > 0.
+BJ_HD float ggx_d(float a2, float cos_h) {
+  const float c2 = cos_h * cos_h;
+  const float d = c2 * (a2 - 1.0f) + 1.0f;
+  return a2 / (kPi * d * d);
+}
+// Smith Lambda(w) for cos_w = n.w > 0.
+BJ_HD float ggx_lambda(float a2, float cos_w) {
+  const float c2 = cos_w * cos_w;
+  const float t2 = fmax_sel(1.0f - c2, 0.0f) / c2;
+  return 0.5f * (sqrtf(1.0f + a2 * t2) - 1.0f);
+}
+BJ_HD v3 schlick_f0(v3 f0, float c) {
+  const float x = 1.0f - fmin_sel(fmax_sel(c, 0.0f), 1.0f);
+  const float x2 = x * x;
+  const float x5 = x2 * x2 * x;
+  return mk(fmaf(1.0f - f0.x, x5, f0.x), fmaf(1.0f - f0.y, x5, f0.y), fmaf(1.0f - f0.z, x5, f0.z));
+}
+
+// f(wo, wi) and pdf(wi) in the local frame (z = shading normal).
+BJ_HD BsdfEval eval_ggx_local(v3 wo, v3 wi, v3 f0, float alpha) {
+  BsdfEval out;
+  out.value = mk(0.0f, 0.0f, 0.0f);
+  out.pdf = 0.0f;
+  out.delta = false;
+  if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) return out;
+  const float a2 = alpha * alpha;
+  const v3 h = normalize(add(wo, wi));
+  const float d = ggx_d(a2, h.z);
+  const float lo = ggx_lambda(a2, wo.z), li = ggx_lambda(a2, wi.z);
+  const float g2 = 1.0f / (1.0f + lo + li);
+  const float g1o = 1.0f / (1.0f + lo);
+  const v3 f = schlick_f0(f0, dot(wo, h));
+  out.value = scale(f, d * g2 / (4.0f * wo.z * wi.z));
+  out.pdf = g1o * d / (4.0f * wo.z);
+  return out;
+}
+
+// World-space evaluation toward `dir` for a ray arriving along ray_d at a
+// surface with oriented normal n (n . ray_d < 0).
+BJ_HD BsdfEval eval_ggx(v3 ray_d, v3 n, v3 f0, float alpha, v3 dir) {
+  const Frame fr = frame_of(n);
+  const v3 wo = to_local(fr, mk(-ray_d.x, -ray_d.y, -ray_d.z));
+  const v3 wi = to_local(fr, dir);
+  return eval_ggx_local(normalize(wo), normalize(wi), f0, alpha);
+}
+
+// Visible-normal sample (u1, u2 in [0, 1)); ok = false when the reflected
+// direction falls below the surface (the sample is absorbed).
+BJ_HD BsdfSample sample_ggx(v3 ray_d, v3 n, v3 f0, float alpha, float u1, float u2) {
+  BsdfSample out;
+  out.direction = n;
+  out.weight = mk(0.0f, 0.0f, 0.0f);
+  out.pdf = 0.0f;
+  out.delta = false;
+  out.ok = false;
+  const Frame fr = frame_of(n);
+  const v3 wo = normalize(to_local(fr, mk(-ray_d.x, -ray_d.y, -ray_d.z)));
+  if (!(wo.z > 0.0f)) return out;
+  const v3 vh = normalize(mk(alpha * wo.x, alpha * wo.y, wo.z));
+  const float lensq = vh.x * vh.x + vh.y * vh.y;
+  const float inv_len = lensq > 0.0f ? 1.0f / sqrtf(lensq) : 0.0f;
+  const v3 t1v = lensq > 0.0f ? mk(-vh.y * inv_len, vh.x * inv_len, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+  const v3 t2v = cross(vh, t1v);
+  const float r = sqrtf(u1);
+  float sp, cp;
+  sincos_turns(u2, sp, cp);
+  const float t1 = r * cp;
+  float t2 = r * sp;
+  const float sh = 0.5f * (1.0f + vh.z);
+  t2 = (1.0f - sh) * sqrtf(fmax_sel(1.0f - t1 * t1, 0.0f)) + sh * t2;
+  const float tz = sqrtf(fmax_sel(1.0f - t1 * t1 - t2 * t2, 0.0f));
+  const v3 nh = add(add(scale(t1v, t1), scale(t2v, t2)), scale(vh, tz));
+  const v3 h = normalize(mk(alpha * nh.x, alpha * nh.y, fmax_sel(nh.z, 0.0f)));
+  const float oh = dot(wo, h);
+  const v3 wi = sub(scale(h, 2.0f * oh), wo);
+  if (!(wi.z > 0.0f) || !(oh > 0.0f)) return out;
+  const float a2 = alpha * alpha;
+  const float lo = ggx_lambda(a2, wo.z), li = ggx_lambda(a2, wi.z);
+  const v3 f = schlick_f0(f0, oh);
+  out.weight = scale(f, (1.0f + lo) / (1.0f + lo + li));
+  out.pdf = ggx_d(a2, h.z) / ((1.0f + lo) * 4.0f * wo.z);
+  out.direction = normalize(to_world(fr, wi));
+  out.ok = out.pdf > 0.0f;
+  return out;
+}
+
 }  // namespace bj

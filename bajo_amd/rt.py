@@ -33,6 +33,7 @@ class MAT:
     METAL = 1
     DIELECTRIC = 2
     EMISSIVE = 3
+    GGX = 4  # this library's GGX conductor (F0.rgb, roughness); bajo's MAT stops at EMISSIVE
 
 
 SURFACE_KIND_SHIFT = 28
@@ -76,6 +77,7 @@ class SceneData:
     dielectric: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[D] refraction index
     spheres: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[4S] center, signed radius
     sphere_surfaces: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))  # u32[S]
+    ggx: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # f32[4G] F0.rgb, roughness
 
     @property
     def triangle_count(self) -> int:
@@ -100,19 +102,21 @@ class GpuScene:
         emit = np.ascontiguousarray(scene.emissive, dtype=np.float32).reshape(-1)
         met = np.ascontiguousarray(getattr(scene, "metal", np.zeros(0)), dtype=np.float32).reshape(-1)
         die = np.ascontiguousarray(getattr(scene, "dielectric", np.zeros(0)), dtype=np.float32).reshape(-1)
+        ggx = np.ascontiguousarray(getattr(scene, "ggx", np.zeros(0)), dtype=np.float32).reshape(-1)
         sp = np.ascontiguousarray(getattr(scene, "spheres", np.zeros(0)), dtype=np.float32).reshape(-1)
         ss = np.ascontiguousarray(getattr(scene, "sphere_surfaces", np.zeros(0)), dtype=np.uint32).reshape(-1)
         if s.size != v.size // 9 or v.size % 9:
             raise ValueError("one surface id per triangle is required")
         if sp.size % 4 or ss.size != sp.size // 4:
             raise ValueError("spheres are f32[4] (center, signed radius) with one surface id each")
-        if lamb.size % 3 or emit.size % 3 or met.size % 4:
-            raise ValueError("materials are f32[3] per Lambertian / emissive and f32[4] per metal")
+        if lamb.size % 3 or emit.size % 3 or met.size % 4 or ggx.size % 4:
+            raise ValueError("materials are f32[3] per Lambertian / emissive and f32[4] per metal / GGX")
         handle = ctypes.c_void_p()
         opts = build_options(node_width, leaf_size, method)
         mats = _lib.Materials(
             lamb.ctypes.data_as(ctypes.c_void_p), lamb.size // 3, met.ctypes.data_as(ctypes.c_void_p), met.size // 4,
             die.ctypes.data_as(ctypes.c_void_p), die.size, emit.ctypes.data_as(ctypes.c_void_p), emit.size // 3,
+            ggx.ctypes.data_as(ctypes.c_void_p), ggx.size // 4,
         )
         _lib.check(
             _lib.lib().bajo_scene_create_mixed(
@@ -128,7 +132,7 @@ class GpuScene:
                 ctypes.byref(handle),
             )
         )
-        self._keep = (lamb, emit, met, die)
+        self._keep = (lamb, emit, met, die, ggx)
         self._handle = handle
         self.triangle_count = v.size // 9
         self.sphere_count = sp.size // 4
@@ -194,6 +198,7 @@ class GpuSceneBatch(GpuScene):
         emit = np.ascontiguousarray(scene.emissive, dtype=np.float32).reshape(-1)
         met = np.ascontiguousarray(scene.metal, dtype=np.float32).reshape(-1)
         die = np.ascontiguousarray(scene.dielectric, dtype=np.float32).reshape(-1)
+        ggx = np.ascontiguousarray(getattr(scene, "ggx", np.zeros(0)), dtype=np.float32).reshape(-1)
         if v.size % 9 or s.size != v.size // 9:
             raise ValueError("one surface id per triangle is required")
         if off.size < 2 or int(off[-1]) != s.size:
@@ -207,11 +212,12 @@ class GpuSceneBatch(GpuScene):
         mats = _lib.Materials(
             lamb.ctypes.data_as(ctypes.c_void_p), lamb.size // 3, met.ctypes.data_as(ctypes.c_void_p), met.size // 4,
             die.ctypes.data_as(ctypes.c_void_p), die.size, emit.ctypes.data_as(ctypes.c_void_p), emit.size // 3,
+            ggx.ctypes.data_as(ctypes.c_void_p), ggx.size // 4,
         )
         _lib.check(_lib.lib().bajo_scene_create_batch(
             self.ctx.handle, v.ctypes.data_as(ctypes.c_void_p), off.ctypes.data_as(ctypes.c_void_p), off.size - 1,
             s.ctypes.data_as(ctypes.c_void_p), ctypes.byref(mats), ctypes.byref(opts), ctypes.byref(handle)))
-        self._keep = (lamb, emit, met, die)
+        self._keep = (lamb, emit, met, die, ggx)
         self._handle = handle
         self.triangle_count = s.size
         self.sphere_count = 0

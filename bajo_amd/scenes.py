@@ -173,6 +173,45 @@ def material_box_scene(seed: int = 7, n_soup: int = 60, emission: float = 12.0):
                      dielectric)
 
 
+def ggx_box_scene(seed: int = 17, n_soup: int = 60, emission: float = 12.0):
+    """The box scene with GGX conductors beside Lambertian surfaces: a glossy
+    GGX floor (roughness 0.2), a GGX ball (roughness 0.08), a Lambertian
+    ball, and a soup cycling Lambertian / GGX (roughness 0.05-0.5); for GGX
+    parity and radiance tests."""
+    from .rt import MAT, SceneData, surface_id
+
+    floor = quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))
+    back = quad((-1, 0, 1), (-1, 2, 1), (1, 2, 1), (1, 0, 1))
+    left = quad((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1))
+    right = quad((1, 0, 1), (1, 2, 1), (1, 2, -1), (1, 0, -1))
+    light = quad((-0.3, 1.9, -0.3), (0.3, 1.9, -0.3), (0.3, 1.9, 0.3), (-0.3, 1.9, 0.3))
+    ball = icosphere(320, frequency=4, amplitude=0.0).reshape(-1, 3)
+    gball = (ball * 0.3 + np.array([0.4, 0.3, 0.1], np.float32)).astype(np.float32).reshape(-1)
+    lball = (ball * 0.25 + np.array([-0.45, 0.25, 0.3], np.float32)).astype(np.float32).reshape(-1)
+    rng = np.random.default_rng(seed)
+    v0 = rng.uniform([-0.8, 0.6, -0.6], [0.8, 1.5, 0.6], (n_soup, 3))
+    e = rng.uniform(-0.2, 0.2, (n_soup, 2, 3))
+    soup_t = np.concatenate([v0, v0 + e[:, 0], v0 + e[:, 1]], axis=1).astype(np.float32).reshape(-1)
+    lamb = np.array([[0.7, 0.7, 0.7], [0.6, 0.2, 0.2], [0.2, 0.6, 0.2], [0.5, 0.4, 0.3]], np.float32)
+    ggx = np.array([[0.8, 0.8, 0.8, 0.2], [0.95, 0.7, 0.35, 0.08], [0.6, 0.7, 0.9, 0.05], [0.9, 0.9, 0.9, 0.5]],
+                   np.float32)
+    soup_kinds = np.array([surface_id(MAT.LAMBERTIAN, 3), surface_id(MAT.GGX, 2), surface_id(MAT.GGX, 3)], np.uint32)
+    kinds = [
+        np.full(2, surface_id(MAT.GGX, 0), np.uint32),
+        np.full(2, surface_id(MAT.LAMBERTIAN, 0), np.uint32),
+        np.full(2, surface_id(MAT.LAMBERTIAN, 1), np.uint32),
+        np.full(2, surface_id(MAT.LAMBERTIAN, 2), np.uint32),
+        np.full(gball.size // 9, surface_id(MAT.GGX, 1), np.uint32),
+        np.full(lball.size // 9, surface_id(MAT.LAMBERTIAN, 0), np.uint32),
+        soup_kinds[np.arange(n_soup) % 3],
+        np.full(2, surface_id(MAT.EMISSIVE, 0), np.uint32),
+    ]
+    vertices = np.concatenate([floor, back, left, right, gball, lball, soup_t, light]).astype(np.float32)
+    surfaces = np.concatenate(kinds).astype(np.uint32)
+    assert surfaces.size * 9 == vertices.size
+    return SceneData(vertices, surfaces, lamb.reshape(-1), np.array([emission] * 3, np.float32), ggx=ggx.reshape(-1))
+
+
 def mixed_box_scene(seed: int = 11, n_small: int = 24, emission: float = 12.0, sphere_emission: float = 6.0):
     """The box scene's walls and quad light with spheres of every material kind (bajo's mixed
     triangle + sphere scenes, rt/gpu/mixed_path.mojo): a Lambertian ball, a rough metal ball, a glass

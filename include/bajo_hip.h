@@ -325,6 +325,12 @@ int32_t bajo_tlas_free(bajo_tlas* tlas);
 #define BAJO_MAT_METAL 1
 #define BAJO_MAT_DIELECTRIC 2
 #define BAJO_MAT_EMISSIVE 3
+/* A GGX microfacet conductor (no bajo counterpart; bajo's MAT stops at
+ * EMISSIVE): Trowbridge-Reitz distribution of width alpha = roughness in
+ * (0, 1], Smith height-correlated masking-shadowing, Schlick Fresnel with
+ * F0 = the specular color; sampled from the visible normals (Heitz 2018).
+ * It takes direct light (NEE / MIS) like Lambertian surfaces. */
+#define BAJO_MAT_GGX 4
 
 typedef struct bajo_scene bajo_scene;
 
@@ -373,8 +379,9 @@ int32_t bajo_scene_create_ex(bajo_ctx* ctx, const float* vertices, uint64_t n, c
 /* The material arrays of SceneData (rt/types.mojo:382-526), flattened as
  * bajo's GPU shading view does (rt/gpu/path_shading.mojo:87-120): Lambertian
  * albedo f32[3], metal albedo + fuzz f32[4], dielectric refraction index
- * f32[1], emissive radiance f32[3] per material.  Surface ids select them by
- * kind (MAT: 0 Lambertian, 1 metal, 2 dielectric, 3 emissive). */
+ * f32[1], emissive radiance f32[3] per material, and this library's GGX
+ * conductor f32[4] (BAJO_MAT_GGX).  Surface ids select them by kind (MAT: 0
+ * Lambertian, 1 metal, 2 dielectric, 3 emissive, 4 GGX). */
 typedef struct bajo_materials {
   const float* lambertian;
   uint32_t n_lambertian;
@@ -384,6 +391,8 @@ typedef struct bajo_materials {
   uint32_t n_dielectric;
   const float* emissive;
   uint32_t n_emissive;
+  const float* ggx; /* f32[4] per GGX material: F0.rgb, roughness; may be NULL with n_ggx 0 */
+  uint32_t n_ggx;
 } bajo_materials;
 /* bajo_scene_create_ex with every material kind: metal (bajo's Phong-lobe
  * metal, mirror at fuzz <= 1e-4) and dielectric (Schlick reflection,

@@ -118,6 +118,8 @@ _SIG = {
         c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                 c_void_p, c_void_p, c_void_p, c_void_p, c_int],
     ),
+    "orc_eval_ggx": (None, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
+    "orc_sample_ggx": (None, [c_void_p, c_void_p, c_void_p, c_float, c_float, c_float, c_void_p]),
     "orc_render_fp64_mixed": (
         c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                 c_void_p, c_void_p, c_void_p, c_int],
@@ -146,6 +148,7 @@ class RenderSettings(ctypes.Structure):
         ("node_width", c_int),
         ("leaf_size", c_int),
         ("method", c_int),
+        ("ggx", c_void_p),
     ]
 
 _lib = None
@@ -413,16 +416,18 @@ def _materials(lambertian, emissive, metal, dielectric):
 
 
 def render(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1,
-           node_width=2, leaf_size=0, method=0, metal=None, dielectric=None, spheres=None, sphere_surfaces=None):
+           node_width=2, leaf_size=0, method=0, metal=None, dielectric=None, spheres=None, sphere_surfaces=None,
+           ggx=None):
     """The FP32 wavefront renderer restated (Lambertian, metal f32[4 per material], dielectric f32[1],
-    emissive; optional spheres f32[4S] with signed radii and their surface ids):
+    emissive, GGX f32[4 per material]; optional spheres f32[4S] with signed radii and their surface ids):
     (pixels, (extension rays, shadow rays))."""
     v = f32(vertices) if len(vertices) else np.zeros(9, np.float32)
     n = len(vertices) // 9
     surf = np.ascontiguousarray(surfaces, dtype=np.uint32) if n else np.zeros(1, np.uint32)
     lamb, emit, met, die = _materials(lambertian, emissive, metal, dielectric)
     sp, ss, ns = _spheres(spheres, sphere_surfaces)
-    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, node_width, leaf_size, method)
+    g = f32(ggx) if ggx is not None and len(ggx) else np.zeros(4, np.float32)
+    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, node_width, leaf_size, method, _p(g))
     pixels = np.empty(3 * width * height, np.float32)
     rays = np.zeros(2, np.uint64)
     rc = lib().orc_render_mixed(
@@ -515,17 +520,33 @@ def canonical_wide(nodes: np.ndarray) -> np.ndarray:
 
 
 def render_fp64(vertices, surfaces, lambertian, emissive, width, height, spp, max_depth, seed, algorithm, cam20, threads=1,
-                metal=None, dielectric=None, spheres=None, sphere_surfaces=None):
+                metal=None, dielectric=None, spheres=None, sphere_surfaces=None, ggx=None):
     v = f32(vertices) if len(vertices) else np.zeros(9, np.float32)
     n = len(vertices) // 9
     surf = np.ascontiguousarray(surfaces, dtype=np.uint32) if n else np.zeros(1, np.uint32)
     lamb, emit, met, die = _materials(lambertian, emissive, metal, dielectric)
     sp, ss, ns = _spheres(spheres, sphere_surfaces)
-    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, 2, 0, 0)
+    g = f32(ggx) if ggx is not None and len(ggx) else np.zeros(4, np.float32)
+    st = RenderSettings(width, height, spp, max_depth, seed, algorithm, 2, 0, 0, _p(g))
     pixels = np.empty(3 * width * height, np.float64)
     lib().orc_render_fp64_mixed(_p(v), n, _p(surf), _p(sp), ns, _p(ss), _p(lamb), _p(met), _p(die), _p(emit),
                                 ctypes.byref(st), _p(f32(cam20)), _p(pixels), threads)
     return pixels
+
+
+# ---- GGX conductor (the library's BAJO_MAT_GGX) -----------------------------------
+def eval_ggx(ray_d, n, f0, alpha, direction):
+    """FP32 GGX f and pdf toward `direction` (value.xyz, pdf)."""
+    out = np.empty(4, np.float32)
+    lib().orc_eval_ggx(_p(f32(ray_d)), _p(f32(n)), _p(f32(f0)), float(alpha), _p(f32(direction)), _p(out))
+    return out[:3].copy(), float(out[3])
+
+
+def sample_ggx(ray_d, n, f0, alpha, u1, u2):
+    """FP32 visible-normal GGX sample: (direction, weight, pdf, ok)."""
+    out = np.empty(8, np.float32)
+    lib().orc_sample_ggx(_p(f32(ray_d)), _p(f32(n)), _p(f32(f0)), float(alpha), float(u1), float(u2), _p(out))
+    return out[:3].copy(), out[3:6].copy(), float(out[6]), bool(out[7])
 
 
 # ---- materials (rt/shading.mojo) ----------------------------------------------

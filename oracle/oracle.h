@@ -110,7 +110,10 @@ typedef struct {
     int node_width; /* 2 (binary LBVH) or 8 (collapsed 8-wide, orc_wide.c); 0 = 2 */
     int leaf_size;  /* width 8: triangles per leaf child, 1..8; 0 = 4 */
     int method;     /* 0 LBVH, 1 H-PLOC (orc_hploc.c, depth-first leaf order) */
+    const float *ggx; /* f32[4] per GGX material (F0.rgb, roughness) or NULL */
 } orc_render_settings;
+void orc_eval_ggx(const float *ray_d, const float *n, const float *f0, float alpha, const float *dir, float *out);
+void orc_sample_ggx(const float *ray_d, const float *n, const float *f0, float alpha, float u1, float u2, float *out);
 
 /* mixed triangle + sphere scenes (orc_render.c); spheres f32[4S] with signed radii */
 int orc_render_mixed(const float *vertices, uint64_t n, const uint32_t *surfaces, const float *spheres, uint64_t ns,

@@ -260,10 +260,115 @@ void orc_reflect3(const float *v, const float *n, float *out) {
 
 /* ---- light table ------------------------------------------------------------- */
 #define L_STRIDE 14
+/* ---- GGX conductor (the library's BAJO_MAT_GGX; bajo has no counterpart) ------
+ * Trowbridge-Reitz D of width alpha = roughness, Smith height-correlated G2,
+ * Schlick F with F0 = the specular color, visible-normal sampling (Heitz
+ * 2018, JCGT 7(4)), frame of Duff et al. 2017 (JCGT 6(1)).  Restated
+ * operation for operation in FP32 so it yields the device's bits. */
+#define PI_F 3.14159265358979323846f
+typedef struct { orc_v3 t, b, n; } orc_frame;
+static orc_frame frame_of(orc_v3 n) {
+    float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    orc_frame f;
+    f.t = orc_v(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    f.b = orc_v(b, sign + n.y * n.y * a, -n.y);
+    f.n = n;
+    return f;
+}
+static orc_v3 to_local(orc_frame f, orc_v3 v) { return orc_v(orc_dot(v, f.t), orc_dot(v, f.b), orc_dot(v, f.n)); }
+static orc_v3 to_world(orc_frame f, orc_v3 v) {
+    return orc_add(orc_add(orc_scale(f.t, v.x), orc_scale(f.b, v.y)), orc_scale(f.n, v.z));
+}
+static float ggx_d(float a2, float cos_h) {
+    float c2 = cos_h * cos_h;
+    float d = c2 * (a2 - 1.0f) + 1.0f;
+    return a2 / (PI_F * d * d);
+}
+static float ggx_lambda(float a2, float cos_w) {
+    float c2 = cos_w * cos_w;
+    float t2 = orc_fmax(1.0f - c2, 0.0f) / c2;
+    return 0.5f * (sqrtf(1.0f + a2 * t2) - 1.0f);
+}
+static orc_v3 schlick_f0(orc_v3 f0, float c) {
+    float x = 1.0f - orc_fmin(orc_fmax(c, 0.0f), 1.0f);
+    float x2 = x * x;
+    float x5 = x2 * x2 * x;
+    return orc_v(fmaf(1.0f - f0.x, x5, f0.x), fmaf(1.0f - f0.y, x5, f0.y), fmaf(1.0f - f0.z, x5, f0.z));
+}
+static orc_eval eval_ggx_local(orc_v3 wo, orc_v3 wi, orc_v3 f0, float alpha) {
+    orc_eval out = {orc_v(0, 0, 0), 0.0f, 0};
+    if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) return out;
+    float a2 = alpha * alpha;
+    orc_v3 h = orc_normalize(orc_add(wo, wi));
+    float d = ggx_d(a2, h.z);
+    float lo = ggx_lambda(a2, wo.z), li = ggx_lambda(a2, wi.z);
+    float g2 = 1.0f / (1.0f + lo + li);
+    float g1o = 1.0f / (1.0f + lo);
+    orc_v3 f = schlick_f0(f0, orc_dot(wo, h));
+    out.value = orc_scale(f, d * g2 / (4.0f * wo.z * wi.z));
+    out.pdf = g1o * d / (4.0f * wo.z);
+    return out;
+}
+static orc_eval eval_ggx(orc_v3 ray_d, orc_v3 n, orc_v3 f0, float alpha, orc_v3 dir) {
+    orc_frame fr = frame_of(n);
+    orc_v3 wo = to_local(fr, orc_v(-ray_d.x, -ray_d.y, -ray_d.z));
+    orc_v3 wi = to_local(fr, dir);
+    return eval_ggx_local(orc_normalize(wo), orc_normalize(wi), f0, alpha);
+}
+static orc_sample sample_ggx(orc_v3 ray_d, orc_v3 n, orc_v3 f0, float alpha, float u1, float u2) {
+    orc_sample out = {n, orc_v(0, 0, 0), 0.0f, 0, 0};
+    orc_frame fr = frame_of(n);
+    orc_v3 wo = orc_normalize(to_local(fr, orc_v(-ray_d.x, -ray_d.y, -ray_d.z)));
+    if (!(wo.z > 0.0f)) return out;
+    orc_v3 vh = orc_normalize(orc_v(alpha * wo.x, alpha * wo.y, wo.z));
+    float lensq = vh.x * vh.x + vh.y * vh.y;
+    float inv_len = lensq > 0.0f ? 1.0f / sqrtf(lensq) : 0.0f;
+    orc_v3 t1v = lensq > 0.0f ? orc_v(-vh.y * inv_len, vh.x * inv_len, 0.0f) : orc_v(1.0f, 0.0f, 0.0f);
+    orc_v3 t2v = orc_cross(vh, t1v);
+    float r = sqrtf(u1);
+    float sp, cp;
+    orc_sincos_turns(u2, &sp, &cp);
+    float t1 = r * cp;
+    float t2 = r * sp;
+    float sh = 0.5f * (1.0f + vh.z);
+    t2 = (1.0f - sh) * sqrtf(orc_fmax(1.0f - t1 * t1, 0.0f)) + sh * t2;
+    float tz = sqrtf(orc_fmax(1.0f - t1 * t1 - t2 * t2, 0.0f));
+    orc_v3 nh = orc_add(orc_add(orc_scale(t1v, t1), orc_scale(t2v, t2)), orc_scale(vh, tz));
+    orc_v3 h = orc_normalize(orc_v(alpha * nh.x, alpha * nh.y, orc_fmax(nh.z, 0.0f)));
+    float oh = orc_dot(wo, h);
+    orc_v3 wi = orc_sub(orc_scale(h, 2.0f * oh), wo);
+    if (!(wi.z > 0.0f) || !(oh > 0.0f)) return out;
+    float a2 = alpha * alpha;
+    float lo = ggx_lambda(a2, wo.z), li = ggx_lambda(a2, wi.z);
+    orc_v3 f = schlick_f0(f0, oh);
+    out.weight = orc_scale(f, (1.0f + lo) / (1.0f + lo + li));
+    out.pdf = ggx_d(a2, h.z) / ((1.0f + lo) * 4.0f * wo.z);
+    out.dir = orc_normalize(to_world(fr, wi));
+    out.ok = out.pdf > 0.0f;
+    return out;
+}
+/* out: value.xyz, pdf */
+void orc_eval_ggx(const float *ray_d, const float *n, const float *f0, float alpha, const float *dir, float *out) {
+    orc_eval r = eval_ggx(orc_v(ray_d[0], ray_d[1], ray_d[2]), orc_v(n[0], n[1], n[2]), orc_v(f0[0], f0[1], f0[2]),
+                          alpha, orc_v(dir[0], dir[1], dir[2]));
+    out[0] = r.value.x; out[1] = r.value.y; out[2] = r.value.z; out[3] = r.pdf;
+}
+/* out: dir.xyz, weight.xyz, pdf, ok */
+void orc_sample_ggx(const float *ray_d, const float *n, const float *f0, float alpha, float u1, float u2, float *out) {
+    orc_sample r = sample_ggx(orc_v(ray_d[0], ray_d[1], ray_d[2]), orc_v(n[0], n[1], n[2]), orc_v(f0[0], f0[1], f0[2]),
+                              alpha, u1, u2);
+    out[0] = r.dir.x; out[1] = r.dir.y; out[2] = r.dir.z;
+    out[3] = r.weight.x; out[4] = r.weight.y; out[5] = r.weight.z;
+    out[6] = r.pdf; out[7] = (float)r.ok;
+}
+
 #define MAT_LAMB 0u
 #define MAT_METAL 1u
 #define MAT_DIEL 2u
 #define MAT_EMIT 3u
+#define MAT_GGX 4u
 #define IDX_MASK ((1u << 28) - 1u)
 
 void orc_alias_table(const float *w, uint32_t n, float total, float *prob, uint32_t *alias) {
@@ -492,6 +597,11 @@ static int direct_light(const scene_t *sc, uint32_t path_id, uint32_t bounce, or
         orc_eval ev = eval_metal(ray_d, normal, orc_v(m[0], m[1], m[2]), m[3], dir);
         value = ev.value;
         bsdf_pdf = ev.pdf;
+    } else if (kind == MAT_GGX) {
+        const float *g = sc->st.ggx + 4 * (size_t)mi;
+        orc_eval ev = eval_ggx(ray_d, normal, orc_v(g[0], g[1], g[2]), g[3], dir);
+        value = ev.value;
+        bsdf_pdf = ev.pdf;
     } else {
         const float *a = sc->lamb + 3 * (size_t)mi;
         value = orc_scale(orc_v(a[0], a[1], a[2]), 0.3183098861837907f);
@@ -568,7 +678,7 @@ static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint
             } else {
                 orc_v3 albedo = orc_v(0, 0, 0);
                 if (kind == MAT_LAMB) albedo = orc_v(sc->lamb[3 * mi], sc->lamb[3 * mi + 1], sc->lamb[3 * mi + 2]);
-                if ((alg == 3 || alg == 4) && (kind == MAT_LAMB || kind == MAT_METAL)) {
+                if ((alg == 3 || alg == 4) && (kind == MAT_LAMB || kind == MAT_METAL || kind == MAT_GGX)) {
                     shadow = direct_light(sc, path_id, b, point, n, kind, mi, d, T, &sh_d, &sh_tmax, &sh_c);
                     sh_o = point;
                 }
@@ -618,12 +728,16 @@ static void render_path(const scene_t *sc, uint32_t path_id, float *sample, uint
                         no = orc_v(o.x + h.t * d.x, o.y + h.t * d.y, o.z + h.t * d.z);
                         nd = dir; nT = Tn; npdf = pdf; ndelta = 0;
                     }
-                } else if (kind == MAT_METAL || kind == MAT_DIEL) {
+                } else if (kind == MAT_METAL || kind == MAT_DIEL || kind == MAT_GGX) {
                     /* rt/gpu/path_shading.mojo:669-783: numbers drawn only where used */
                     orc_rng br;
                     rng_init(&br, st->seed, path_id, stage_bsdf(b));
                     orc_sample smp;
-                    if (kind == MAT_METAL) {
+                    if (kind == MAT_GGX) {
+                        const float *g = st->ggx + 4 * (size_t)mi;
+                        float ru = rng_next(&br), rv = rng_next(&br);
+                        smp = sample_ggx(d, n, orc_v(g[0], g[1], g[2]), g[3], ru, rv);
+                    } else if (kind == MAT_METAL) {
                         const float *m = sc->metal + 4 * (size_t)mi;
                         float ru = 0.0f, rv = 0.0f;
                         if (m[3] > 1.0e-4f) { ru = rng_next(&br); rv = rng_next(&br); }
@@ -957,6 +1071,55 @@ static int hit64(const scene64 *s, d3 o, d3 d, double tmin, double tmax, double 
     return found;
 }
 
+/* GGX in double, from the definitions: D(h) = a^2 / (pi ((n.h)^2 (a^2 - 1) + 1)^2),
+ * Lambda(w) = (sqrt(1 + a^2 tan^2) - 1) / 2, G2 = 1 / (1 + L(o) + L(i)),
+ * F = F0 + (1 - F0)(1 - o.h)^5, f = F D G2 / (4 cos_o cos_i); directions from
+ * the visible normals (Heitz 2018) with weight F G2 / G1(o). */
+static double ggx64_lambda(double a2, double c) {
+    double t2 = fmax(1 - c * c, 0.0) / (c * c);
+    return 0.5 * (sqrt(1 + a2 * t2) - 1);
+}
+static void basis64(d3 n, d3 *t, d3 *b) {
+    d3 h = fabs(n.x) > 0.5 ? D(0, 1, 0) : D(1, 0, 0);
+    *t = dnorm(dcross(h, n));
+    *b = dcross(n, *t);
+}
+static d3 ggx64_f(const float *g, d3 n, d3 wo, d3 wi) {
+    double co = ddot(n, wo), ci = ddot(n, wi);
+    if (co <= 0 || ci <= 0) return D(0, 0, 0);
+    double a2 = (double)g[3] * g[3];
+    d3 h = dnorm(dadd(wo, wi));
+    double ch = ddot(n, h), dd = ch * ch * (a2 - 1) + 1;
+    double Dh = a2 / (M_PI * dd * dd);
+    double G2 = 1.0 / (1 + ggx64_lambda(a2, co) + ggx64_lambda(a2, ci));
+    double x = 1 - fmin(fmax(ddot(wo, h), 0.0), 1.0), x5 = x * x * x * x * x;
+    double k = Dh * G2 / (4 * co * ci);
+    return D((g[0] + (1 - g[0]) * x5) * k, (g[1] + (1 - g[1]) * x5) * k, (g[2] + (1 - g[2]) * x5) * k);
+}
+static int ggx64_sample(const float *g, d3 n, d3 wo_w, double u1, double u2, d3 *dir, d3 *weight) {
+    d3 t, b;
+    basis64(n, &t, &b);
+    d3 wo = D(ddot(wo_w, t), ddot(wo_w, b), ddot(wo_w, n));
+    if (wo.z <= 0) return 0;
+    double a = g[3], a2 = a * a;
+    d3 vh = dnorm(D(a * wo.x, a * wo.y, wo.z));
+    double l2 = vh.x * vh.x + vh.y * vh.y;
+    d3 T1 = l2 > 0 ? dsc(D(-vh.y, vh.x, 0), 1 / sqrt(l2)) : D(1, 0, 0);
+    d3 T2 = dcross(vh, T1);
+    double r = sqrt(u1), ph = 2 * M_PI * u2, t1 = r * cos(ph), t2 = r * sin(ph), sh = 0.5 * (1 + vh.z);
+    t2 = (1 - sh) * sqrt(fmax(1 - t1 * t1, 0.0)) + sh * t2;
+    d3 nh = dadd(dadd(dsc(T1, t1), dsc(T2, t2)), dsc(vh, sqrt(fmax(0.0, 1 - t1 * t1 - t2 * t2))));
+    d3 h = dnorm(D(a * nh.x, a * nh.y, fmax(nh.z, 0.0)));
+    double oh = ddot(wo, h);
+    d3 wi = dsub(dsc(h, 2 * oh), wo);
+    if (wi.z <= 0 || oh <= 0) return 0;
+    double lo = ggx64_lambda(a2, wo.z), li = ggx64_lambda(a2, wi.z);
+    double x = 1 - fmin(oh, 1.0), x5 = x * x * x * x * x, k = (1 + lo) / (1 + lo + li);
+    *weight = D((g[0] + (1 - g[0]) * x5) * k, (g[1] + (1 - g[1]) * x5) * k, (g[2] + (1 - g[2]) * x5) * k);
+    *dir = dnorm(dadd(dadd(dsc(t, wi.x), dsc(b, wi.y)), dsc(n, wi.z)));
+    return 1;
+}
+
 static void render_path64(const scene64 *s, uint64_t path_id, pcg_t *rng, double *out) {
     const orc_render_settings *st = &s->st;
     const float *k = s->cam;
@@ -1017,7 +1180,10 @@ static void render_path64(const scene64 *s, uint64_t path_id, pcg_t *rng, double
             o = p; d = nd;
             continue;
         }
-        d3 alb = D(s->lamb[3 * mi], s->lamb[3 * mi + 1], s->lamb[3 * mi + 2]);
+        const int ggx = kind == MAT_GGX;
+        const float *gm = ggx ? s->st.ggx + 4 * (size_t)mi : NULL;
+        d3 alb = ggx ? D(0, 0, 0) : D(s->lamb[3 * mi], s->lamb[3 * mi + 1], s->lamb[3 * mi + 2]);
+        d3 wo = dnorm(dsc(d, -1));
         if (alg == 3 && s->nl > 0) {
             /* light chosen by power, point uniform on the triangle or the sphere */
             double xi = pcg_d(rng) * s->total, accw = 0;
@@ -1054,10 +1220,20 @@ static void render_path64(const scene64 *s, uint64_t path_id, pcg_t *rng, double
                 if (!hit64(s, p, dir, 0.001, dist - 0.002, &tt, &pp, &nn)) {
                     const float *e = s->emit + 3 * (lsurf & IDX_MASK);
                     double pdf_l = (s->lw[l] / s->total) * d2 / (lc * area);
-                    double f = sc / (M_PI * pdf_l);
-                    out[0] += T.x * alb.x * e[0] * f; out[1] += T.y * alb.y * e[1] * f; out[2] += T.z * alb.z * e[2] * f;
+                    d3 fv = ggx ? ggx64_f(gm, n, wo, dir) : dsc(alb, 1.0 / M_PI);
+                    double f = sc / pdf_l;
+                    out[0] += T.x * fv.x * e[0] * f; out[1] += T.y * fv.y * e[1] * f; out[2] += T.z * fv.z * e[2] * f;
                 }
             }
+        }
+        if (ggx) {
+            double u1 = pcg_d(rng), u2 = pcg_d(rng);
+            d3 nd, wgt;
+            if (!ggx64_sample(gm, n, wo, u1, u2, &nd, &wgt)) return;
+            T = dmul(T, wgt);
+            if (!roulette64(rng, b, &T)) return;
+            o = p; d = nd;
+            continue;
         }
         /* cosine-weighted bounce, throughput *= albedo */
         double u1 = pcg_d(rng), u2 = pcg_d(rng);

@@ -148,6 +148,10 @@ SIGNATURES = {
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_uint64, c_int32, c_int32, c_void_p, c_uint64],
     ),
+    "bajo_onesweep_radix_sort_pairs_u64_u32_noalloc": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_uint64, c_int32, c_int32],
+    ),
     "bajo_build_triangle_bvh": (
         c_int32,
         [c_void_p, c_void_p, c_uint64, c_int32, POINTER(BuildTimings), POINTER(c_void_p)],

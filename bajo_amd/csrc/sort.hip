@@ -14,6 +14,8 @@
 //      runs.
 // The global exclusive digit offsets are recomputed by every tile from the
 // 256-bin histogram, so no separate scan launch is needed.
+#include <mutex>
+
 #include "sort_internal.h"
 
 namespace bj {
@@ -320,6 +322,42 @@ int32_t sort_entry(bajo_ctx* ctx, void* keys, uint32_t* values, uint64_t n, int 
   return bj::onesweep_run(ctx, plan, keys, values, ws, false);
 }
 
+// ---- fixed-scratch sort (no allocation) --------------------------------------
+// The workspace of bajo_onesweep_radix_sort_pairs_u64_u32_noalloc lives in a
+// file-scope device array: alternate keys and values, digit histograms,
+// look-back table and tile tickets of one sort of at most kNoAllocMaxKeys
+// pairs.  Each device holds its own copy for the life of the process; host
+// code uses the address hipGetSymbolAddress returns for the current device,
+// never the array's name.
+constexpr uint64_t kNoAllocMaxKeys = 1ull << 20;
+constexpr size_t kNoAllocScratchBytes = 16ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on `device`, looked up once per device
+// with that device current.
+int32_t noalloc_scratch(int device, void** out) {
+  BJ_REQUIRE(device >= 0 && device < kNoAllocMaxDevices, "noalloc sort: device %d outside [0, %d)", device,
+             kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    int prev = 0;
+    BJ_HIP(hipGetDevice(&prev));
+    BJ_HIP(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) return bj::hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    BJ_REQUIRE(ptr != nullptr, "noalloc sort: hipGetSymbolAddress returned NULL");
+    g_noalloc_ptr[device] = ptr;
+  }
+  *out = g_noalloc_ptr[device];
+  return BAJO_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -364,4 +402,23 @@ int32_t bajo_onesweep_radix_sort_pairs_u64_u32(bajo_ctx* ctx, uint64_t* keys, ui
   return sort_entry(ctx, keys, values, n, 8, begin_bit, end_bit, workspace, workspace_bytes,
                     "bajo_onesweep_radix_sort_pairs_u64_u32");
 }
+int32_t bajo_onesweep_radix_sort_pairs_u64_u32_noalloc(bajo_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n,
+                                                       int32_t begin_bit, int32_t end_bit) {
+  const char* name = "bajo_onesweep_radix_sort_pairs_u64_u32_noalloc";
+  BJ_REQUIRE(ctx != nullptr, "%s: ctx is NULL", name);
+  BJ_REQUIRE(n == 0 || (keys != nullptr && values != nullptr), "%s: keys or values is NULL", name);
+  BJ_REQUIRE(n <= kNoAllocMaxKeys, "%s: n=%llu exceeds the fixed scratch's limit of %llu pairs", name,
+             (unsigned long long)n, (unsigned long long)kNoAllocMaxKeys);
+  BJ_REQUIRE(begin_bit >= 0 && end_bit <= 64 && begin_bit <= end_bit && begin_bit % 8 == 0 && end_bit % 8 == 0,
+             "%s: bit range [%d, %d) must be byte aligned inside the key", name, begin_bit, end_bit);
+  if (n <= 1 || begin_bit == end_bit) return BAJO_OK;
+  const bj::SortPlan plan = bj::make_sort_plan(n, 8, 1, begin_bit, end_bit);
+  BJ_REQUIRE(plan.total <= kNoAllocScratchBytes, "%s: workspace of %llu bytes exceeds the fixed scratch", name,
+             (unsigned long long)plan.total);
+  void* ws = nullptr;
+  const int32_t rc = noalloc_scratch(ctx->device, &ws);
+  if (rc) return rc;
+  return bj::onesweep_run(ctx, plan, keys, values, ws, false);
+}
+
 }  // extern "C"

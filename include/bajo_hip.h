@@ -161,6 +161,16 @@ int32_t bajo_onesweep_radix_sort_keys_u64(bajo_ctx* ctx, uint64_t* keys, uint64_
 int32_t bajo_onesweep_radix_sort_pairs_u64_u32(bajo_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n,
                                                int32_t begin_bit, int32_t end_bit, void* workspace,
                                                uint64_t workspace_bytes);
+/* No counterpart in bajo.  The same sort as
+ * bajo_onesweep_radix_sort_pairs_u64_u32 for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight submission loops,
+ * harnesses that forbid hipMalloc): its workspace is a fixed 16 MiB device
+ * array inside the library (one per GPU, held until the process ends), so
+ * the call allocates nothing.  n is at most 2^20 pairs (larger n returns
+ * BAJO_ERR_INVALID_ARGUMENT).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other streams, other threads) race. */
+int32_t bajo_onesweep_radix_sort_pairs_u64_u32_noalloc(bajo_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n,
+                                                       int32_t begin_bit, int32_t end_bit);
 
 /* ---- BVH build: bajo/bvh/gpu -------------------------------------------
  * build_triangle_bvh[frame, W, L, LBVH](ctx, vertices f32[9N]),

@@ -125,6 +125,34 @@ class RenderStats(ctypes.Structure):
     ]
 
 
+class GuideOptions(ctypes.Structure):
+    """bajo_guide_options."""
+
+    _fields_ = [
+        ("learning_rate", c_float),
+        ("beta1", c_float),
+        ("beta2", c_float),
+        ("epsilon", c_float),
+        ("batch_size", c_uint32),
+        ("steps_per_frame", c_uint32),
+        ("seed", c_uint64),
+    ]
+
+
+class GuideFrameStats(ctypes.Structure):
+    """bajo_guide_frame_stats."""
+
+    _fields_ = [
+        ("guided_vertices", c_uint64),
+        ("training_samples", c_uint64),
+        ("adam_steps", c_uint32),
+        ("loss", c_float),
+        ("infer_ns", c_uint64),
+        ("train_ns", c_uint64),
+        ("render_ns", c_uint64),
+    ]
+
+
 # name -> (restype, argtypes); the exported surface of include/bajo_hip.h
 SIGNATURES = {
     "bajo_ctx_create": (c_int32, [c_int32, c_void_p, POINTER(c_void_p)]),
@@ -151,6 +179,19 @@ SIGNATURES = {
     "bajo_onesweep_radix_sort_pairs_u64_u32_noalloc": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_uint64, c_int32, c_int32],
+    ),
+    "bajo_guide_create": (c_int32, [c_void_p, POINTER(GuideOptions), POINTER(c_void_p)]),
+    "bajo_guide_free": (c_int32, [c_void_p]),
+    "bajo_guide_get_params": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "bajo_guide_set_params": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "bajo_guide_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
+    "bajo_guide_gradient": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, POINTER(c_float)]),
+    "bajo_guide_train": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64]),
+    "bajo_guide_mixture_sample": (c_int32, [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p]),
+    "bajo_render_guided": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, POINTER(RenderSettings), POINTER(c_float), c_void_p, c_int32,
+         POINTER(GuideFrameStats)],
     ),
     "bajo_build_triangle_bvh": (
         c_int32,

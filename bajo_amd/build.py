@@ -41,7 +41,7 @@ HIP_FLAGS = [
     "-packed-fp32-ops",
 ]
 
-SOURCES = ["context.hip", "sort.hip", "lbvh.hip", "hploc.hip", "wide.hip", "trace.hip", "tlas.hip", "render.hip",
+SOURCES = ["context.hip", "sort.hip", "lbvh.hip", "hploc.hip", "wide.hip", "trace.hip", "tlas.hip", "render.hip", "guide.hip",
            "scenes.cpp"]
 
 

@@ -322,3 +322,109 @@ def environment(batch, offsets, k: int):
 
     a, b = int(offsets[k]), int(offsets[k + 1])
     return replace(batch, vertices=batch.vertices[9 * a:9 * b].copy(), surfaces=batch.surfaces[a:b].copy())
+
+
+# ---- the guided-rendering room -------------------------------------------------------
+
+# Room layout (metres): the main room is [0, 8] x [0, 3] x [0, 6] (x, y, z); a small light room
+# [8, 10] x [0, 3] x [2, 4] sits behind the main room's x = 8 wall, which has a doorway
+# z in [2.5, 3.5], y in [0, 2]; the only emitter is a small downward-facing quad under the light
+# room's ceiling, so the main room is lit through the doorway.
+ROOM_TILE = 0.5
+ROOM_LIGHT = ((9.2, 2.95, 2.8), (9.6, 2.95, 2.8), (9.6, 2.95, 3.2), (9.2, 2.95, 3.2))
+ROOM_CAMERA = ((0.8, 1.7, 0.5), (8.0, 1.1, 3.3), (0.0, 1.0, 0.0), 0.7)
+ROOM_SPHERES = 8
+
+
+def _room_walls():
+    """Wall pieces as (origin, u axis, v axis, tiles along u, tiles along v, skipped tiles); every piece
+    faces the inside of its room (u x v is the inward normal)."""
+    T = ROOM_TILE
+    pieces = [
+        # main room
+        ((0, 0, 0), (0, 0, T), (T, 0, 0), 12, 16, ()),          # floor y = 0
+        ((0, 3, 0), (T, 0, 0), (0, 0, T), 16, 12, ()),          # ceiling y = 3
+        ((0, 0, 0), (T, 0, 0), (0, T, 0), 16, 6, ()),           # wall z = 0
+        ((0, 0, 6), (0, T, 0), (T, 0, 0), 6, 16, ()),           # wall z = 6
+        ((0, 0, 0), (0, T, 0), (0, 0, T), 6, 12, ()),           # wall x = 0
+        # wall x = 8 (both rooms see it): doorway tiles z in [2.5, 3.5] (iv 5, 6), y in [0, 2] (iu 0..3)
+        ((8, 0, 0), (0, 0, T), (0, T, 0), 12, 6, tuple((iu, iv) for iu in (5, 6) for iv in range(4))),
+        # light room
+        ((8, 0, 2), (T, 0, 0), (0, 0, T), 4, 4, ()),            # floor y = 0
+        ((8, 3, 2), (0, 0, T), (T, 0, 0), 4, 4, ()),            # ceiling y = 3
+        ((8, 0, 2), (0, T, 0), (T, 0, 0), 6, 4, ()),            # wall z = 2
+        ((8, 0, 4), (T, 0, 0), (0, T, 0), 4, 6, ()),            # wall z = 4
+        ((10, 0, 2), (0, 0, T), (0, T, 0), 4, 6, ()),           # wall x = 10
+    ]
+    return pieces
+
+
+def room_scene(seed: int = 2030, cells_per_tile: int = 16, sphere_frequency: int = 70, emission: float = 100.0):
+    """The guided-rendering room: tessellated walls of the two rooms above (tiles of 0.5 m, each
+    `cells_per_tile`^2 cells of two triangles), `ROOM_SPHERES` displaced icospheres of frequency
+    `sphere_frequency` on the main room's floor, and one emissive quad in the light room.  Wall tiles
+    are checkered per wall piece (even tiles Lambertian, odd tiles GGX) and spheres alternate, so
+    exactly half of the non-emissive triangles are Lambertian and half GGX.  Lambertian albedo
+    U[0.3, 0.8]^3, GGX F0 U[0.5, 0.95]^3 and roughness U[0.05, 0.5], sphere radius U[0.35, 0.6] and
+    positions (rejection-sampled apart) come from numpy's PCG64 stream `seed`.  At the defaults:
+    417,792 wall + 784,000 sphere + 2 light = 1,201,794 triangles."""
+    from .rt import MAT, SceneData, surface_id
+
+    rng = np.random.default_rng(seed)
+    c = int(cells_per_tile)
+    tris, surf, lamb, ggx = [], [], [], []
+    g = np.arange(c, dtype=np.float64)
+    cu, cv = np.meshgrid(g, g, indexing="ij")
+    cu, cv = cu.reshape(-1), cv.reshape(-1)
+    for origin, u, v, nu, nv, skip in _room_walls():
+        o, u, v = (np.asarray(x, np.float64) for x in (origin, u, v))
+        for iu in range(nu):
+            for iv in range(nv):
+                if (iu, iv) in skip:
+                    continue
+                p00 = o + np.outer(iu + cu / c, u) + np.outer(iv + cv / c, v)
+                p10 = p00 + u / c
+                p11 = p10 + v / c
+                p01 = p00 + v / c
+                t = np.concatenate([np.stack([p00, p10, p11], 1), np.stack([p00, p11, p01], 1)])
+                tris.append(t.reshape(-1, 9))
+                if (iu + iv) % 2 == 0:
+                    sid = surface_id(MAT.LAMBERTIAN, len(lamb))
+                    lamb.append(rng.uniform(0.3, 0.8, 3))
+                else:
+                    sid = surface_id(MAT.GGX, len(ggx))
+                    ggx.append(np.concatenate([rng.uniform(0.5, 0.95, 3), rng.uniform(0.05, 0.5, 1)]))
+                surf.append(np.full(2 * c * c, sid, np.uint32))
+    unit = icosphere(20 * sphere_frequency * sphere_frequency, frequency=sphere_frequency).reshape(-1, 3)
+    placed = []
+    while len(placed) < ROOM_SPHERES:
+        r = rng.uniform(0.35, 0.6)
+        x, z = rng.uniform(1.0, 7.0), rng.uniform(1.0, 5.0)
+        if all((x - px) ** 2 + (z - pz) ** 2 > (r + pr + 0.1) ** 2 for px, pz, pr in placed):
+            placed.append((x, z, r))
+    for k, (x, z, r) in enumerate(placed):
+        ball = unit.astype(np.float64) * (r / 1.05) + np.array([x, r, z])
+        tris.append(ball.reshape(-1, 9))
+        if k % 2 == 0:
+            sid = surface_id(MAT.LAMBERTIAN, len(lamb))
+            lamb.append(rng.uniform(0.3, 0.8, 3))
+        else:
+            sid = surface_id(MAT.GGX, len(ggx))
+            ggx.append(np.concatenate([rng.uniform(0.5, 0.95, 3), rng.uniform(0.05, 0.5, 1)]))
+        surf.append(np.full(ball.shape[0] // 3, sid, np.uint32))
+    light = quad(*ROOM_LIGHT).reshape(-1, 9)
+    if np.cross(light[0, 3:6] - light[0, 0:3], light[0, 6:9] - light[0, 0:3])[1] > 0:
+        light = quad(*ROOM_LIGHT[::-1]).reshape(-1, 9)  # face down, into the light room
+    tris.append(light.astype(np.float64))
+    surf.append(np.full(2, surface_id(MAT.EMISSIVE, 0), np.uint32))
+    vertices = np.concatenate(tris).astype(np.float32).reshape(-1)
+    return SceneData(vertices, np.concatenate(surf).astype(np.uint32), np.asarray(lamb, np.float32).reshape(-1),
+                     np.array([emission] * 3, np.float32), ggx=np.asarray(ggx, np.float32).reshape(-1))
+
+
+def room_camera():
+    """The room's camera: in the main room's corner, looking at the doorway."""
+    from .bvh import Camera
+
+    o, t, up, fov = ROOM_CAMERA
+    return Camera(o, t, up, fov).flatten()

@@ -490,6 +490,72 @@ int32_t bajo_render_target_free(bajo_render_target* target);
  * of the Philox stream keyed by (seed, path_id, stage). */
 int32_t bajo_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, uint32_t count, float* out);
 
+/* ---- learned path guiding (no bajo counterpart) ---------------------------
+ * A small network maps a path vertex (position normalized to the scene box,
+ * shading normal, outgoing direction; 12-frequency sin/cos encoding -> 216
+ * features) through three hidden layers of 64 ReLU units to an 8-lobe von
+ * Mises-Fisher mixture over directions (40 outputs: per lobe a mean
+ * direction 3, log concentration 1, logit 1).  Weights are BF16 on the
+ * matrix cores with FP32 accumulation; the FP32 master weights are trained
+ * with Adam on the KL divergence to the incident radiance the path tracer
+ * measures: per sample -w log q(direction), w = radiance luminance / density
+ * it was drawn with, normalized by the batch's sum of w.  The guided renderer
+ * (BAJO_RENDER_GUIDED) picks the guide or the BSDF with probability 1/2 at
+ * every Lambertian and GGX vertex and weights by the combined density
+ * (one-sample MIS), on top of NEE; bajo's BSDF sampling
+ * (rt/gpu/path_shading.mojo:463-535, rt/shading.mojo:92-118) is the other
+ * half and, without a guide, the whole of it (plain NEE). */
+typedef struct bajo_guide bajo_guide;
+#define BAJO_GUIDE_PARAMS 24808
+#define BAJO_GUIDE_OUTPUTS 40
+#define BAJO_GUIDE_SAMPLE_FLOATS 13 /* inputs 9, direction 3, weight 1 */
+#define BAJO_RENDER_GUIDED 5
+typedef struct {
+  float learning_rate;      /* Adam step size; 0 = 1e-2                          */
+  float beta1, beta2;       /* Adam moment decays; 0 = 0.9 / 0.999               */
+  float epsilon;            /* Adam epsilon; 0 = 1e-8                            */
+  uint32_t batch_size;      /* samples per Adam step (multiple of 32); 0 = 65536  */
+  uint32_t steps_per_frame; /* Adam steps after a training render; 0 = 4         */
+  uint64_t seed;            /* weight initialization (He-uniform, output x 0.1)  */
+} bajo_guide_options;
+int32_t bajo_guide_create(bajo_ctx* ctx, const bajo_guide_options* options, bajo_guide** out);
+int32_t bajo_guide_free(bajo_guide* guide);
+/* Host copies of the FP32 master weights f32[BAJO_GUIDE_PARAMS] (W1[64][216],
+ * b1[64], W2[64][64], b2, W3[64][64], b3, W4[40][64], b4[40]; W[j][k] maps
+ * input k to output j); set_params also resets the Adam state. */
+int32_t bajo_guide_get_params(bajo_ctx* ctx, const bajo_guide* guide, float* params);
+int32_t bajo_guide_set_params(bajo_ctx* ctx, bajo_guide* guide, const float* params);
+/* Network outputs f32[40 n] for inputs f32[9 n] (device buffers). */
+int32_t bajo_guide_forward(bajo_ctx* ctx, const bajo_guide* guide, const float* inputs, uint64_t n, float* outputs);
+/* Loss gradient f32[BAJO_GUIDE_PARAMS] (device) of one batch of samples
+ * f32[13 n] (device; n <= batch_size), without an update; `loss` (host,
+ * may be NULL) receives the normalized loss (synchronizes). */
+int32_t bajo_guide_gradient(bajo_ctx* ctx, bajo_guide* guide, const float* samples, uint64_t n, float* gradient,
+                            float* loss);
+/* Adam steps over samples f32[13 n] (device), one per batch_size samples. */
+int32_t bajo_guide_train(bajo_ctx* ctx, bajo_guide* guide, const float* samples, uint64_t n);
+/* Decodes outputs f32[40 n] into mixtures and, per row, draws a direction
+ * from uniforms f32[3 n] (lobe, then the lobe's two) into dirs f32[3 n] and
+ * its mixture density into pdfs f32[n] (device buffers). */
+int32_t bajo_guide_mixture_sample(bajo_ctx* ctx, const float* outputs, uint64_t n, const float* uniforms, float* dirs,
+                                  float* pdfs);
+/* Per-frame figures of a guided render (bajo_render_guided). */
+typedef struct {
+  uint64_t guided_vertices;  /* vertices that sampled through the guide's MIS  */
+  uint64_t training_samples; /* samples collected for training (train != 0)   */
+  uint32_t adam_steps;       /* Adam steps taken after the frame              */
+  float loss;                /* mean normalized loss of the last step          */
+  uint64_t infer_ns, train_ns, render_ns; /* device time of network inference, training, whole frame */
+} bajo_guide_frame_stats;
+/* One frame of BAJO_RENDER_GUIDED: NEE with guided continuation (settings'
+ * algorithm is ignored).  With `train` the frame records its vertices and
+ * the radiance that reached them, and Adam steps over those samples follow
+ * (they change the guide for the next frame).  `guide` NULL renders plain
+ * NEE.  The frame must fit one path chunk (path_capacity 0 or >= paths).
+ * With `stats` it synchronizes. */
+int32_t bajo_render_guided(bajo_ctx* ctx, bajo_scene* scene, bajo_guide* guide, const bajo_render_settings* settings,
+                           const float* camera, float* pixels, int32_t train, bajo_guide_frame_stats* stats);
+
 /* ---- synthetic scenes (host) -------------------------------------------
  * Seeded PCG32 (O'Neill 2014) generators for the benchmark workloads.  All
  * write n triangles as f32[9n] into a host buffer. */

@@ -119,6 +119,18 @@ _SIG = {
                 c_void_p, c_void_p, c_void_p, c_void_p, c_int],
     ),
     "orc_eval_ggx": (None, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
+    "orc_bf16_round": (c_float, [c_float]),
+    "orc_guide_encode": (None, [c_void_p, c_void_p]),
+    "orc_guide_forward": (None, [c_void_p, c_void_p, c_uint64, c_int, c_void_p]),
+    "orc_guide_loss_grad": (ctypes.c_double, [c_void_p, c_void_p, c_uint64, c_int, c_void_p]),
+    "orc_guide_adam": (
+        None,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+         ctypes.c_double],
+    ),
+    "orc_guide_mixture": (c_float, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "orc_vmf_pdf64": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+    "orc_vmf_sample_cos64": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
     "orc_sample_ggx": (None, [c_void_p, c_void_p, c_void_p, c_float, c_float, c_float, c_void_p]),
     "orc_render_fp64_mixed": (
         c_int, [c_void_p, c_uint64, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -844,3 +856,78 @@ def trace_brute_spheres(spheres, rays: np.ndarray) -> np.ndarray:
     hits = np.empty(8 * R, np.float32)
     lib().orc_trace_brute_spheres(_p(sp), sp.size // 4, _p(r), R, _p(hits))
     return hits.reshape(R, 8)
+
+
+# ---- learned path guiding ------------------------------------------------------------
+GUIDE_PARAMS = 24808
+GUIDE_OUTPUTS = 40
+GUIDE_FEATURES = 216
+# offsets of W1, b1, W2, b2, W3, b3, W4, b4 in the parameter vector
+GUIDE_OFFSETS = {"W1": 0, "b1": 13824, "W2": 13888, "b2": 17984, "W3": 18048, "b3": 22144, "W4": 22208, "b4": 24768}
+
+
+def bf16_round(x) -> np.ndarray:
+    """FP32 -> BF16 (round to nearest even) -> FP32, elementwise."""
+    a = f32(x)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).reshape(a.shape)
+
+
+def guide_encode(x9) -> np.ndarray:
+    """The 216 features of one sample's 9 inputs (FP32 series, device order)."""
+    x = f32(x9)
+    out = np.empty(GUIDE_FEATURES, np.float32)
+    lib().orc_guide_encode(_p(x), _p(out))
+    return out
+
+
+def guide_forward(params, inputs, emulate_bf16=True) -> np.ndarray:
+    """Network outputs f64[n, 40] for inputs f32[n, 9] (FP64 evaluation; emulate_bf16 rounds operands as the
+    device does)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    x = f32(inputs).reshape(-1, 9)
+    out = np.empty((x.shape[0], GUIDE_OUTPUTS), np.float64)
+    lib().orc_guide_forward(_p(p), _p(x), x.shape[0], int(emulate_bf16), _p(out))
+    return out
+
+
+def guide_loss_grad(params, samples, emulate_bf16=False, with_grad=True):
+    """(normalized loss, gradient f64[24808] or None) of samples f32[n, 13] (inputs, direction, weight)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    s = f32(samples).reshape(-1, 13)
+    g = np.empty(GUIDE_PARAMS, np.float64) if with_grad else None
+    loss = lib().orc_guide_loss_grad(_p(p), _p(s), s.shape[0], int(emulate_bf16), _p(g) if with_grad else None)
+    return loss, g
+
+
+def guide_adam(params, m, v, grad, step, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One FP64 Adam step in place (step is 1-based)."""
+    for a in (params, m, v):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    g = np.ascontiguousarray(grad, dtype=np.float64)
+    lib().orc_guide_adam(_p(params), _p(m), _p(v), _p(g), int(step), lr, beta1, beta2, eps)
+
+
+def guide_mixture_sample(outputs40, u3):
+    """(direction f32[3], density) drawn from the mixture of one output row (FP32, device operations)."""
+    o = f32(outputs40)
+    u = f32(u3)
+    d = np.empty(3, np.float32)
+    pdf = lib().orc_guide_mixture(_p(o), _p(u), _p(d), None)
+    return d, pdf
+
+
+def guide_mixture_pdf(outputs40, direction) -> float:
+    o = f32(outputs40)
+    d = f32(direction)
+    return float(lib().orc_guide_mixture(_p(o), None, None, _p(d)))
+
+
+def vmf_pdf64(kappa: float, cos_t: float) -> float:
+    return float(lib().orc_vmf_pdf64(kappa, cos_t))
+
+
+def vmf_sample_cos64(kappa: float, u: float) -> float:
+    return float(lib().orc_vmf_sample_cos64(kappa, u))
+

@@ -98,7 +98,7 @@ void orc_sincos_turns(float u, float *s, float *c) {
  * its odd series to s^11; 2^t via t = n + f, n = rint(t), e^(f ln 2) by its
  * Taylor series to degree 8; results below 2^-125 are 0.  (The renderer's
  * own definition; FP32 operations and fmaf only.) */
-static float orc_log2_pos(float x) {
+float orc_log2_pos(float x) {
     uint32_t b;
     memcpy(&b, &x, 4);
     int k = 0;

@@ -187,6 +187,7 @@ SIGNATURES = {
     "bajo_guide_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
     "bajo_guide_gradient": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, POINTER(c_float)]),
     "bajo_guide_train": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64]),
+    "bajo_guide_training_samples": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, POINTER(c_uint64)]),
     "bajo_guide_mixture_sample": (c_int32, [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p]),
     "bajo_render_guided": (
         c_int32,

@@ -130,20 +130,17 @@ __global__ void __launch_bounds__(256) guide_pack_kernel(const float* __restrict
 // Chunk q (0..27) of a sample's features: input d = q / 3, frequencies
 // 4 (q % 3) .. + 3, (sin, cos) pairs; chunk 27 is padding (zeros).
 BJ_D bf16x8 encode_chunk(const float (&x)[kGuideInputs], int q) {
-  uint32_t w[4] = {0u, 0u, 0u, 0u};
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f}, c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   if (q < 27) {
     const int d = q / 3, fb = q - 3 * d;
     float xd = x[0];
 #pragma unroll
     for (int a = 1; a < kGuideInputs; ++a) xd = d == a ? x[a] : xd;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      float s, c;
-      guide_encode_pair(xd, 4 * fb + t, s, c);
-      w[t] = (uint32_t)f2bf(s) | ((uint32_t)f2bf(c) << 16);
-    }
+    guide_encode_group(xd, fb, s, c);
   }
-  return __builtin_bit_cast(bf16x8, uint4{w[0], w[1], w[2], w[3]});
+  // round to nearest even (the hardware conversion equals f2bf for finite values)
+  return bf16x8{(__bf16)s[0], (__bf16)c[0], (__bf16)s[1], (__bf16)c[1],
+                (__bf16)s[2], (__bf16)c[2], (__bf16)s[3], (__bf16)c[3]};
 }
 
 // Bias + ReLU of a 16 x 64 layer result into a row-major BF16 tile.
@@ -218,11 +215,6 @@ struct InputRows {
     for (int a = 0; a < kGuideInputs; ++a) v[a] = x[r * kGuideInputs + a];
   }
 };
-
-// Vertex planes of the renderer's guide queue (tiles of 64 entries, planes
-// point 0-2, normal 3-5, ray direction 6-8).
-constexpr int kVertexPlanes = 14;
-BJ_HD uint64_t vplane(uint64_t i, int p) { return ((i >> 6) * kVertexPlanes + (uint64_t)p) * 64u + (i & 63u); }
 
 struct VertexRows {
   const float* f;
@@ -378,7 +370,9 @@ BJ_D void store_bias_rows(uint16_t* dst, uint64_t ldb, uint64_t col, uint32_t la
 }
 
 struct TrainArgs {
-  const float* samples;  // f32[13 n]
+  const float* samples;  // f32[13 * src_count]
+  // batch row r is sample (src_offset + r * src_stride) % src_count
+  uint64_t src_count, src_offset, src_stride;
   uint64_t n;            // samples
   uint64_t n_pad;        // n rounded up to a whole chunk (padded rows carry weight 0)
   uint64_t ldb;          // row stride of the transposed buffers
@@ -408,8 +402,9 @@ __global__ void __launch_bounds__(256) guide_train_kernel(const uint8_t* __restr
     const uint64_t col = tile * 16;
     const uint64_t r = col + i;
     float x[kGuideInputs];
+    const float* sp = ta.samples + ((ta.src_offset + r * ta.src_stride) % ta.src_count) * kGuideSampleFloats;
 #pragma unroll
-    for (int a = 0; a < kGuideInputs; ++a) x[a] = r < ta.n ? ta.samples[r * kGuideSampleFloats + a] : 0.0f;
+    for (int a = 0; a < kGuideInputs; ++a) x[a] = r < ta.n ? sp[a] : 0.0f;
     f32x4 acc[4];
     layer1(x, s_w + kPackW1, acc, lane, ta.x0t, ta.ldb, col);
     store_hidden(acc, s_b, h1, lane);
@@ -440,7 +435,6 @@ __global__ void __launch_bounds__(256) guide_train_kernel(const uint8_t* __restr
       float w = 0.0f;
       v3 dir = mk(0.0f, 0.0f, 1.0f);
       if (r < ta.n) {
-        const float* sp = ta.samples + r * kGuideSampleFloats;
         dir = mk(sp[9], sp[10], sp[11]);
         w = sp[12];
       }
@@ -571,7 +565,7 @@ __global__ void __launch_bounds__(256) guide_mixture_kernel(const float* __restr
   if (r >= n) return;
   GuideMixture m;
   guide_decode(outputs + r * kGuideOut, m);
-  const v3 d = guide_sample(m, u[3 * r], u[3 * r + 1], u[3 * r + 2]);
+  const v3 d = guide_sample(m, outputs + r * kGuideOut, u[3 * r], u[3 * r + 1], u[3 * r + 2]);
   dirs[3 * r] = d.x;
   dirs[3 * r + 1] = d.y;
   dirs[3 * r + 2] = d.z;
@@ -629,6 +623,7 @@ void free_guide(bajo_guide* g) {
   (void)hipFree(g->d1t); (void)hipFree(g->d2t); (void)hipFree(g->d3t); (void)hipFree(g->dot);
   (void)hipFree(g->tile_part); (void)hipFree(g->grad_part);
   (void)hipFree(g->records); (void)hipFree(g->contrib); (void)hipFree(g->samples); (void)hipFree(g->sample_count);
+  (void)hipFree(g->vertices); (void)hipFree(g->outputs);
   delete g;
 }
 
@@ -643,12 +638,12 @@ int32_t blocks_per_cu(K kernel, int* out) {
 }
 
 // Forward + backward + weight-gradient partials of one batch (n <= batch_cap).
-int32_t run_backward(hipStream_t stream, bajo_guide* g, const float* samples, uint64_t n, uint64_t* ldb_out,
-                     uint32_t* chunks_out) {
+int32_t run_backward(hipStream_t stream, bajo_guide* g, const float* samples, uint64_t src_count, uint64_t src_offset,
+                     uint64_t src_stride, uint64_t n, uint64_t* ldb_out, uint32_t* chunks_out) {
   const uint64_t ldb = (n + kChunk - 1) / kChunk * kChunk;  // padded batch
   const uint32_t chunks = (uint32_t)(ldb / kChunk);
-  TrainArgs ta{samples, n, ldb, g->batch_cap, g->x0t, g->h1t, g->h2t, g->h3t, g->d1t, g->d2t, g->d3t, g->dot,
-               g->tile_part};
+  TrainArgs ta{samples, src_count, src_offset, src_stride, n, ldb, g->batch_cap, g->x0t, g->h1t, g->h2t, g->h3t,
+               g->d1t, g->d2t, g->d3t, g->dot, g->tile_part};
   const uint64_t tiles = ldb / 16;
   int grid = 0;
   int32_t rc = blocks_per_cu(guide_train_kernel, &grid);
@@ -699,11 +694,167 @@ int32_t guide_train_samples(hipStream_t stream, bajo_guide* g, const float* samp
     const uint64_t m = n - b < g->batch_cap ? n - b : g->batch_cap;
     uint64_t ldb = 0;
     uint32_t chunks = 0;
-    int32_t rc = run_backward(stream, g, samples + b * kGuideSampleFloats, m, &ldb, &chunks);
+    int32_t rc = run_backward(stream, g, samples, n, b, 1, m, &ldb, &chunks);
     if (rc) return rc;
     rc = run_adam(stream, g, ldb, chunks, nullptr);
     if (rc) return rc;
   }
+  return BAJO_OK;
+}
+
+namespace {
+
+// Training samples of one frame: thread p walks path p's records; vertex k
+// receives the radiance added at bounces after it, over the throughput the
+// path carried out of it (per channel), as luminance over the density its
+// direction was drawn with.
+__global__ void __launch_bounds__(256) guide_collect_kernel(const float* __restrict__ records,
+                                                            const float* __restrict__ contrib, uint64_t paths,
+                                                            uint32_t depth, const float* __restrict__ bounds,
+                                                            float* __restrict__ samples, uint32_t* __restrict__ count,
+                                                            uint64_t cap) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const v3 lo = mk(bounds[0], bounds[1], bounds[2]);
+  const v3 ext = sub(mk(bounds[3], bounds[4], bounds[5]), lo);
+  const v3 inv = mk(ext.x > 0.0f ? 1.0f / ext.x : 0.0f, ext.y > 0.0f ? 1.0f / ext.y : 0.0f,
+                    ext.z > 0.0f ? 1.0f / ext.z : 0.0f);
+  for (uint32_t k = 0; k < depth; ++k) {
+    bool want = false;
+    float out[kGuideSampleFloats];
+    if (p < paths) {
+      const float* r = records + (p * depth + k) * kRecordPlanes;
+      const float pdf = r[12];
+      if (pdf > 0.0f) {
+        v3 L = mk(0.0f, 0.0f, 0.0f);
+        for (uint32_t j = k + 1; j < depth; ++j) {
+          const float* c = contrib + 3 * (p * depth + j);
+          L = add(L, mk(c[0], c[1], c[2]));
+        }
+        const v3 T = mk(r[13], r[14], r[15]);
+        L = mk(T.x > 0.0f ? L.x / T.x : 0.0f, T.y > 0.0f ? L.y / T.y : 0.0f, T.z > 0.0f ? L.z / T.z : 0.0f);
+        const float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
+        if (lum > 0.0f) {
+          const GuideInput gi = guide_input(mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), mk(r[6], r[7], r[8]), lo, inv);
+#pragma unroll
+          for (int a = 0; a < kGuideInputs; ++a) out[a] = gi.v[a];
+          out[9] = r[9];
+          out[10] = r[10];
+          out[11] = r[11];
+          out[12] = lum / pdf;
+          want = true;
+        }
+      }
+    }
+    const uint64_t m = ballot(want);
+    if (m == 0) continue;
+    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(count, (uint32_t)__builtin_popcountll(m));
+    base = __shfl(base, (int)leader, 64);
+    const uint64_t slot = (uint64_t)base + mask_rank(m);
+    if (want && slot < cap)
+#pragma unroll
+      for (int a = 0; a < kGuideSampleFloats; ++a) samples[slot * kGuideSampleFloats + a] = out[a];
+  }
+}
+
+uint64_t gcd_u64(uint64_t a, uint64_t b) {
+  while (b) {
+    const uint64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+}  // namespace
+
+int32_t guide_infer_vertices(hipStream_t stream, const bajo_guide* g, const float* vertex_fields,
+                             const uint32_t* count_dev, uint64_t cap, const float* scene_bounds, float* outputs) {
+  int grid = 0;
+  int32_t rc = blocks_per_cu(guide_forward_kernel<VertexRows>, &grid);
+  if (rc) return rc;
+  const uint64_t need = (cap + 63) / 64;
+  if ((uint64_t)grid > need) grid = (int)need;
+  guide_forward_kernel<VertexRows><<<grid, 256, 0, stream>>>(g->pack, VertexRows{vertex_fields, scene_bounds},
+                                                             count_dev, cap, outputs);
+  BJ_HIP(hipGetLastError());
+  return BAJO_OK;
+}
+
+int32_t guide_prepare_frame(hipStream_t stream, bajo_guide* g, uint64_t paths, uint32_t depth, bool train) {
+  if (paths > g->vertex_cap) {
+    BJ_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(g->vertices);
+    (void)hipFree(g->outputs);
+    g->vertices = g->outputs = nullptr;
+    g->vertex_cap = 0;
+    const uint64_t tiles = (paths + 63) / 64;
+    BJ_HIP(hipMalloc(&g->vertices, tiles * 64 * kVertexPlanes * sizeof(float)));
+    BJ_HIP(hipMalloc(&g->outputs, paths * kGuideOut * sizeof(float)));
+    g->vertex_cap = paths;
+  }
+  if (!train) return BAJO_OK;
+  const uint64_t slots = paths * depth;
+  if (slots > g->record_cap) {
+    BJ_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(g->records);
+    (void)hipFree(g->contrib);
+    (void)hipFree(g->samples);
+    g->records = g->contrib = g->samples = nullptr;
+    g->record_cap = g->sample_cap = 0;
+    BJ_HIP(hipMalloc(&g->records, slots * kRecordPlanes * sizeof(float)));
+    BJ_HIP(hipMalloc(&g->contrib, slots * 3 * sizeof(float)));
+    BJ_HIP(hipMalloc(&g->samples, slots * kGuideSampleFloats * sizeof(float)));
+    g->record_cap = g->sample_cap = slots;
+  }
+  BJ_HIP(hipMemsetAsync(g->records, 0, slots * kRecordPlanes * sizeof(float), stream));
+  BJ_HIP(hipMemsetAsync(g->contrib, 0, slots * 3 * sizeof(float), stream));
+  BJ_HIP(hipMemsetAsync(g->sample_count, 0, sizeof(uint32_t), stream));
+  return BAJO_OK;
+}
+
+int32_t guide_collect(hipStream_t stream, bajo_guide* g, uint64_t paths, uint32_t depth, const float* scene_bounds) {
+  guide_collect_kernel<<<div_up(paths, 256), 256, 0, stream>>>(g->records, g->contrib, paths, depth, scene_bounds,
+                                                              g->samples, g->sample_count, g->sample_cap);
+  BJ_HIP(hipGetLastError());
+  return BAJO_OK;
+}
+
+int32_t guide_train_collected(hipStream_t stream, bajo_guide* g, uint64_t* samples_out, uint32_t* steps_out,
+                              float* loss_out) {
+  uint32_t count = 0;
+  BJ_HIP(hipMemcpyAsync(&count, g->sample_count, sizeof(count), hipMemcpyDeviceToHost, stream));
+  BJ_HIP(hipStreamSynchronize(stream));
+  const uint64_t total = count < g->sample_cap ? count : g->sample_cap;
+  *samples_out = total;
+  *steps_out = 0;
+  *loss_out = 0.0f;
+  if (total == 0) return BAJO_OK;
+  const uint64_t n = total < g->batch_cap ? total : g->batch_cap;
+  // a stride coprime with the set's size spreads each batch over the whole frame
+  uint64_t stride = total / n;
+  if (stride < 1) stride = 1;
+  while (gcd_u64(stride, total) != 1) ++stride;
+  uint64_t ldb = 0;
+  for (uint32_t k = 0; k < g->opt.steps_per_frame; ++k) {
+    uint32_t chunks = 0;
+    const uint64_t offset = g->sample_cursor++ % total;
+    int32_t rc = run_backward(stream, g, g->samples, total, offset, stride, n, &ldb, &chunks);
+    if (rc) return rc;
+    rc = run_adam(stream, g, ldb, chunks, nullptr);
+    if (rc) return rc;
+    *steps_out += 1;
+  }
+  std::vector<float> part(ldb / 16 * 2);
+  BJ_HIP(hipMemcpyAsync(part.data(), g->tile_part, part.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+  BJ_HIP(hipStreamSynchronize(stream));
+  double ws = 0.0, ls = 0.0;
+  for (size_t i = 0; i < part.size(); i += 2) {
+    ws += part[i];
+    ls += part[i + 1];
+  }
+  *loss_out = ws > 0.0 ? (float)(ls / ws) : 0.0f;
   return BAJO_OK;
 }
 
@@ -816,7 +967,7 @@ int32_t bajo_guide_gradient(bajo_ctx* ctx, bajo_guide* guide, const float* sampl
              (unsigned long long)n, guide->batch_cap);
   uint64_t ldb = 0;
   uint32_t chunks = 0;
-  int32_t rc = bj::run_backward(ctx->stream, guide, samples, n, &ldb, &chunks);
+  int32_t rc = bj::run_backward(ctx->stream, guide, samples, n, 0, 1, n, &ldb, &chunks);
   if (rc) return rc;
   rc = bj::run_adam(ctx->stream, guide, ldb, chunks, gradient);
   if (rc) return rc;
@@ -841,6 +992,26 @@ int32_t bajo_guide_train(bajo_ctx* ctx, bajo_guide* guide, const float* samples,
   return bj::guide_train_samples(ctx->stream, guide, samples, n);
 }
 
+int32_t bajo_guide_training_samples(bajo_ctx* ctx, const bajo_guide* guide, float* samples, uint64_t cap,
+                                    uint64_t* count) {
+  BJ_REQUIRE(ctx && guide && count, "bajo_guide_training_samples: NULL argument");
+  BJ_REQUIRE(cap == 0 || samples, "bajo_guide_training_samples: samples is NULL");
+  uint32_t c = 0;
+  if (guide->sample_count) {
+    BJ_HIP(hipMemcpyAsync(&c, guide->sample_count, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    BJ_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  uint64_t have = c < guide->sample_cap ? c : guide->sample_cap;
+  *count = have;
+  if (have > cap) have = cap;
+  if (have) {
+    BJ_HIP(hipMemcpyAsync(samples, guide->samples, have * bj::kGuideSampleFloats * sizeof(float),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    BJ_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return BAJO_OK;
+}
+
 int32_t bajo_guide_mixture_sample(bajo_ctx* ctx, const float* outputs, uint64_t n, const float* uniforms, float* dirs,
                                   float* pdfs) {
   BJ_REQUIRE(ctx != nullptr, "bajo_guide_mixture_sample: ctx is NULL");
@@ -852,11 +1023,3 @@ int32_t bajo_guide_mixture_sample(bajo_ctx* ctx, const float* outputs, uint64_t 
 }
 
 }  // extern "C"
-
-extern "C" int32_t bajo_render_guided(bajo_ctx* ctx, bajo_scene* scene, bajo_guide* guide,
-                                      const bajo_render_settings* settings, const float* camera, float* pixels,
-                                      int32_t train, bajo_guide_frame_stats* stats) {
-  (void)ctx; (void)scene; (void)guide; (void)settings; (void)camera; (void)pixels; (void)train; (void)stats;
-  bj::set_error("bajo_render_guided: not built yet");
-  return BAJO_ERR_UNSUPPORTED;
-}

@@ -31,6 +31,11 @@ struct bajo_guide {
   uint64_t sample_cap = 0;
   float* samples = nullptr;      // kGuideSampleFloats per training sample
   uint32_t* sample_count = nullptr;
+  // rendering: the guide queue of one bounce and the network's outputs for it
+  uint64_t vertex_cap = 0;
+  float* vertices = nullptr;     // kVertexPlanes per entry, 64-entry tiles
+  float* outputs = nullptr;      // kGuideOut per entry
+  uint64_t sample_cursor = 0;    // next training sample of the collected set
 };
 
 namespace bj {
@@ -40,18 +45,30 @@ namespace bj {
 // drawn with).
 constexpr int kGuideSampleFloats = 13;
 // A training record of one path vertex: position, normal, outgoing direction,
-// direction taken, its density and the path throughput after the vertex.
+// direction taken, its density (0: no continuation) and the path throughput
+// after the vertex.
 constexpr int kRecordPlanes = 16;
+
+// Vertex planes of the renderer's guide queue, in tiles of 64 entries:
+// point 0-2, oriented normal 3-5, ray direction 6-8, throughput 9-11,
+// surface id 12 and path id 13 (bit patterns).
+constexpr int kVertexPlanes = 14;
+BJ_HD uint64_t vplane(uint64_t i, int p) { return ((i >> 6) * kVertexPlanes + (uint64_t)p) * 64u + (i & 63u); }
 
 // Network forward over `count_dev[0]` (capped at `cap`) render vertices read
 // from the extension kernel's vertex planes; writes 40 outputs per vertex.
 int32_t guide_infer_vertices(hipStream_t stream, const bajo_guide* g, const float* vertex_fields,
                              const uint32_t* count_dev, uint64_t cap, const float* scene_bounds, float* outputs);
-// Turns a frame's records into training samples (appended to g->samples).
+// Makes room for a frame of `paths` paths of `depth` bounces: the guide
+// queue and outputs, and with `train` the training records, per-bounce
+// radiance and samples (records and radiance cleared on the stream).
+int32_t guide_prepare_frame(hipStream_t stream, bajo_guide* g, uint64_t paths, uint32_t depth, bool train);
+// Turns a frame's records into training samples (g->samples, count in
+// g->sample_count[0]).
 int32_t guide_collect(hipStream_t stream, bajo_guide* g, uint64_t paths, uint32_t depth, const float* scene_bounds);
-// Adam steps over the collected samples (at most g->opt.steps_per_frame).
-int32_t guide_train_collected(hipStream_t stream, bajo_guide* g);
-// Makes room for the records of `paths` paths of `depth` bounces.
-int32_t guide_ensure_records(bajo_guide* g, uint64_t paths, uint32_t depth);
+// Synchronizes, reads the collected sample count and takes up to
+// steps_per_frame Adam steps of batch_size samples, strided over the set.
+int32_t guide_train_collected(hipStream_t stream, bajo_guide* g, uint64_t* samples_out, uint32_t* steps_out,
+                              float* loss_out);
 
 }  // namespace bj

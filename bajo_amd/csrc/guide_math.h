@@ -5,7 +5,8 @@
 //
 // Network: 9 inputs (position normalized to the scene box, shading normal,
 // outgoing direction), each encoded at 12 frequencies as
-// sin / cos(pi 2^f x) -> 216 features; three hidden layers of 64 ReLU units;
+// sin / cos(pi 2^f x) -> 216 features (frequencies 0, 4, 8 from the series,
+// the others by doubling); three hidden layers of 64 ReLU units;
 // 40 outputs = 8 lobes x (mean direction 3, log concentration 1, logit 1).
 // Feature k = 24 d + 2 f + c (input d, frequency f, c = 0 sin / 1 cos).
 //
@@ -78,6 +79,20 @@ BJ_HD void guide_encode_pair(float x, int f, float& s, float& c) {
   sincos_turns(u, s, c);
 }
 
+// The four pairs of frequencies 4 b .. 4 b + 3: the first from the series,
+// the next three by the double-angle formulas sin 2a = 2 sin a cos a,
+// cos 2a = (cos a - sin a)(cos a + sin a), in this order of FP32 operations
+// (each doubling at most doubles the first pair's error, well below the
+// BF16 features' precision).
+BJ_HD void guide_encode_group(float x, int b, float (&s)[4], float (&c)[4]) {
+  guide_encode_pair(x, 4 * b, s[0], c[0]);
+#pragma unroll
+  for (int t = 1; t < 4; ++t) {
+    s[t] = 2.0f * s[t - 1] * c[t - 1];
+    c[t] = (c[t - 1] - s[t - 1]) * (c[t - 1] + s[t - 1]);
+  }
+}
+
 // The 9 network inputs of a path vertex.
 struct GuideInput {
   float v[kGuideInputs];
@@ -122,14 +137,17 @@ BJ_HD GuideLobe guide_lobe(const float* o) {
 // Decodes the 40 outputs of one vertex.
 BJ_HD void guide_decode(const float* o, GuideMixture& m) {
   float amax = o[4];
+#pragma unroll
   for (int j = 1; j < kGuideLobes; ++j) amax = fmax_sel(amax, o[5 * j + 4]);
   float sum = 0.0f;
+#pragma unroll
   for (int j = 0; j < kGuideLobes; ++j) {
     m.lobe[j] = guide_lobe(o + 5 * j);
     m.weight[j] = exp_f(o[5 * j + 4] - amax);
     sum += m.weight[j];
   }
   const float inv = 1.0f / sum;
+#pragma unroll
   for (int j = 0; j < kGuideLobes; ++j) m.weight[j] *= inv;
 }
 
@@ -137,6 +155,7 @@ BJ_HD float vmf_pdf(const GuideLobe& l, v3 w) { return exp_f(fmaf(l.kappa, dot(l
 
 BJ_HD float guide_pdf(const GuideMixture& m, v3 w) {
   float p = 0.0f;
+#pragma unroll
   for (int j = 0; j < kGuideLobes; ++j) p = fmaf(m.weight[j], vmf_pdf(m.lobe[j], w), p);
   return p;
 }
@@ -154,19 +173,22 @@ BJ_HD v3 vmf_sample(const GuideLobe& l, float u1, float u2) {
   return normalize(d);
 }
 
-// A direction of the mixture: lobe by u_lobe against the cumulative weights.
-BJ_HD v3 guide_sample(const GuideMixture& m, float u_lobe, float u1, float u2) {
-  GuideLobe l = m.lobe[kGuideLobes - 1];
+// A direction of the mixture decoded from outputs `o`: lobe by u_lobe
+// against the cumulative weights, that lobe decoded again from its outputs
+// (same bits as m.lobe[sel]; picking a lobe out of `m` by a run-time index
+// would put the mixture in scratch memory).
+BJ_HD v3 guide_sample(const GuideMixture& m, const float* o, float u_lobe, float u1, float u2) {
+  int sel = kGuideLobes - 1;
   float c = 0.0f;
   bool found = false;
 #pragma unroll
   for (int j = 0; j < kGuideLobes - 1; ++j) {
     c += m.weight[j];
     const bool take = !found && u_lobe < c;
-    if (take) l = m.lobe[j];
+    sel = take ? j : sel;
     found = found || take;
   }
-  return vmf_sample(l, u1, u2);
+  return vmf_sample(guide_lobe(o + 5 * sel), u1, u2);
 }
 
 // Stage of the guide's random numbers (domain 3 beside BSDF 0, roulette 1,

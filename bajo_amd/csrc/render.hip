@@ -22,6 +22,7 @@
 
 #include <vector>
 
+#include "guide_internal.h"
 #include "rt_math.h"
 #include "traverse.h"
 
@@ -45,6 +46,8 @@ constexpr int kPathPlanes = 12, kShadowPlanes = 11, kSamplePlanes = 3;
 // counter words (bajo's WAVE_COUNTER plus two fetch cursors)
 constexpr int kActive = 0, kNext = 1, kShade = 2, kShadow = 3, kStatus = 4, kFetchExt = 5, kFetchShadow = 6,
               kTailDone = 7;
+// guided renders: vertices queued for the guide this bounce (word 8 is the traversal status)
+constexpr int kVertex = 9;
 constexpr int kCounterWords = 16;
 constexpr uint32_t kStatusPathOverflow = 1, kStatusShadowOverflow = 3;
 constexpr int kMaxBounces = 64;
@@ -56,7 +59,15 @@ constexpr int kWorkBase = 2 * kMaxBounces;
 // ray, per bounce (kMaxBounces each)
 constexpr int kMaxVisitBase = kWorkBase + 4;
 constexpr int kWorkStride = 3;  // per-block partials: node visits, leaf tests, max visits of one ray
-constexpr int kStatsWords = kMaxVisitBase + 2 * kMaxBounces;
+// then the vertices that sampled through the guide (guided renders)
+constexpr int kGuidedWord = kMaxVisitBase + 2 * kMaxBounces;
+constexpr int kStatsWords = kGuidedWord + 1;
+// Algorithms that compute radiance (the tail and the wavefront's emission
+// and sky terms apply), and those whose emission hits take NEE's weights.
+constexpr bool radiance_alg(int a) {
+  return a == BAJO_RENDER_PATH || a == BAJO_RENDER_NEE || a == BAJO_RENDER_MIS || a == BAJO_RENDER_GUIDED;
+}
+constexpr bool nee_alg(int a) { return a == BAJO_RENDER_NEE || a == BAJO_RENDER_GUIDED; }
 
 struct SceneView {
   AccelView bvh;
@@ -108,7 +119,21 @@ struct Queues {
   uint32_t* counters;
   uint64_t capacity;
   uint32_t sample_base;
+  // guided renders: the vertex planes of this bounce (kVertexPlanes per
+  // entry, 64-entry tiles), and when the frame trains, the radiance added per
+  // path and bounce (rgb, max_depth slots per path)
+  float* vertices;
+  float* contrib;
+  uint32_t depth;
 };
+
+// Radiance added to a path at `bounce`, kept per bounce for the guide's training.
+BJ_D void contrib_add(float* contrib, uint32_t slot, uint32_t depth, uint32_t bounce, v3 c) {
+  float* p = contrib + 3ull * ((uint64_t)slot * depth + bounce);
+  p[0] += c.x;
+  p[1] += c.y;
+  p[2] += c.z;
+}
 
 // Queue planes live in tiles of 64 entries (one wave): plane p of entry i is
 // at ((i / 64) * planes + p) * 64 + i % 64.  A wave touches 256 contiguous
@@ -230,6 +255,7 @@ __global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsign
     stats[2 * bounce + 0] += counters[kActive];
     stats[2 * bounce + 1] += counters[kShadow];
   }
+  stats[kGuidedWord] += counters[kVertex];
   // appends past the arena were dropped (and flagged); never read past it
   const uint32_t next = counters[kNext];
   counters[kActive] = next < cap ? next : cap;
@@ -238,6 +264,7 @@ __global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsign
   counters[kShadow] = 0;
   counters[kFetchExt] = 0;
   counters[kFetchShadow] = 0;
+  counters[kVertex] = 0;
 }
 
 struct NextPath {
@@ -462,7 +489,7 @@ BJ_D bool shade_shadow(const SceneView& sc, uint64_t seed, uint32_t bounce, uint
     for (int p = 0; p < kShadowPlanes; ++p) sw.f[p] = fs[p];
     sw.id = path_id;
     return true;
-  } else if constexpr (kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
+  } else if constexpr (nee_alg(kAlg) || kAlg == BAJO_RENDER_MIS) {
     // _sample_direct_light_candidate (rt/gpu/path_shading.mojo:261-403): Lambertian and metal
     // surfaces, and GGX surfaces (their BSDF has a density everywhere above the surface)
     if (su.kind != kMatLambertian && su.kind != kMatMetal && su.kind != kMatGgx) return false;
@@ -501,13 +528,16 @@ BJ_D void next_path(NextPath& np, uint32_t path_id, bool delta, const Ray& ray, 
 // and kShadeNext when `np` holds the next path segment (flags in one value:
 // bools returned through references were kept in scratch memory).
 constexpr uint32_t kShadeAdd = 1u, kShadeNext = 2u;
+// guided renders: the hit is a Lambertian / GGX vertex whose continuation the
+// guide's sample kernel draws
+constexpr uint32_t kShadeVertex = 4u;
 template <int kAlg>
 BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, uint32_t path_id, bool delta,
                            const Ray& ray, v3 T, float path_pdf, float t_hit, const Surface& su, v3& add_c,
-                           NextPath& np) {
+                           NextPath& np, uint32_t depth = kMaxBounces) {
   uint32_t out = 0;
   if (!su.found) {
-    if constexpr (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
+    if constexpr (radiance_alg(kAlg)) {
       add_c = mul(T, sky_color(ray.d));
       out = kShadeAdd;
     }
@@ -516,12 +546,12 @@ BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, 
   if constexpr (kAlg == BAJO_RENDER_NORMALS) {
     add_c = scale(add(su.n, mk(1.0f, 1.0f, 1.0f)), 0.5f);
     out |= kShadeAdd;
-  } else if constexpr (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS) {
+  } else if constexpr (radiance_alg(kAlg)) {
     if (su.kind == kMatEmissive) {
       if (su.front) {
         const v3 E = mk(sc.emissive[3 * su.mi + 0], sc.emissive[3 * su.mi + 1], sc.emissive[3 * su.mi + 2]);
         float w = 1.0f;
-        if constexpr (kAlg == BAJO_RENDER_NEE) {
+        if constexpr (nee_alg(kAlg)) {
           if (bounce > 0 && !delta) w = 0.0f;
         } else if constexpr (kAlg == BAJO_RENDER_MIS) {
           if (bounce > 0 && !delta) {
@@ -538,6 +568,10 @@ BJ_D uint32_t shade_bounce(const SceneView& sc, uint64_t seed, uint32_t bounce, 
         add_c = scale(mul(T, E), w);
         out |= kShadeAdd;
       }
+    } else if (kAlg == BAJO_RENDER_GUIDED && (su.kind == kMatLambertian || su.kind == kMatGgx)) {
+      // the guide's sample kernel draws the continuation; after the last
+      // bounce there is none to draw
+      if (bounce + 1 < depth) out |= kShadeVertex;
     } else if (su.kind == kMatLambertian) {
       // _sample_lambertian (rt/shading.mojo:92-118)
       const v3 n = su.n;
@@ -725,13 +759,35 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(6
         atomicMax(q.counters + kStatus, kStatusShadowOverflow);
       }
     }
-    bool want_next = false;
+    bool want_next = false, want_vertex = false;
     NextPath np;
     if (live) {
       v3 add_c;
-      const uint32_t sf = shade_bounce<kAlg>(sc, seed, bounce, path_id, delta, ray, T, path_pdf, t_hit, su, add_c, np);
+      const uint32_t sf =
+          shade_bounce<kAlg>(sc, seed, bounce, path_id, delta, ray, T, path_pdf, t_hit, su, add_c, np, q.depth);
       want_next = (sf & kShadeNext) != 0;
-      if (sf & kShadeAdd) accumulate(q.sample, path_id - q.sample_base, add_c);
+      want_vertex = (sf & kShadeVertex) != 0;
+      if (sf & kShadeAdd) {
+        accumulate(q.sample, path_id - q.sample_base, add_c);
+        if constexpr (kAlg == BAJO_RENDER_GUIDED)
+          if (q.contrib) contrib_add(q.contrib, path_id - q.sample_base, q.depth, bounce, add_c);
+      }
+    }
+    if constexpr (kAlg == BAJO_RENDER_GUIDED) {
+      // the guide's queue: the hit, its oriented normal, the ray, throughput and ids
+      const uint32_t vs = wave_reserve(q.counters + kVertex, want_vertex);
+      if (want_vertex) {
+        if (vs < cap) {
+          const float vf[kVertexPlanes] = {su.point.x, su.point.y, su.point.z, su.n.x, su.n.y, su.n.z,
+                                           ray.d.x, ray.d.y, ray.d.z, T.x, T.y, T.z,
+                                           __builtin_bit_cast(float, (su.kind << 28) | su.mi),
+                                           __builtin_bit_cast(float, path_id)};
+#pragma unroll
+          for (int p = 0; p < kVertexPlanes; ++p) q.vertices[vplane(vs, p)] = vf[p];
+        } else {
+          atomicMax(q.counters + kStatus, kStatusPathOverflow);
+        }
+      }
     }
     const uint32_t ns = wave_reserve(q.counters + kNext, want_next);
     if (want_next) {
@@ -935,9 +991,119 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
     work_visits += tc.visits;
     work_leaves += tc.leaves;
     const bool occluded = h.prim == kEmpty && h.t == 0.0f;
-    if (kAlg == BAJO_RENDER_AO || !occluded) accumulate(q.sample, slot, shadow_radiance<kAlg>(occluded, f));
+    if (kAlg == BAJO_RENDER_AO || !occluded) {
+      const v3 c = shadow_radiance<kAlg>(occluded, f);
+      accumulate(q.sample, slot, c);
+      if constexpr (kAlg == BAJO_RENDER_GUIDED)
+        if (q.contrib) contrib_add(q.contrib, slot, q.depth, bounce, c);
+    }
   }
   store_block_work(work, work_visits, work_leaves, (uint32_t)work_visits);
+}
+
+// The continuation of the guided vertices of one bounce: one-sample MIS of
+// the BSDF and the guide's mixture, each chosen with probability 1/2 (the
+// guide stage's first number), weighted by f cos / (pdf_bsdf + pdf_guide) / 2.
+// The BSDF half draws exactly what the plain renderer draws (same stream,
+// same arithmetic); the guide half draws lobe, then direction.  With
+// `records` the vertex and the throughput it passes on are kept for the
+// frame's training samples.
+__global__ void __launch_bounds__(kRtBlock) guide_sample_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
+                                                                const float* __restrict__ outputs,
+                                                                float* __restrict__ records) {
+  const uint64_t cap = q.capacity;
+  const uint32_t count = q.counters[kVertex] < cap ? q.counters[kVertex] : (uint32_t)cap;
+  if ((uint64_t)blockIdx.x * kRtBlock >= count) return;
+  const uint32_t idx = blockIdx.x * kRtBlock + threadIdx.x;
+  bool want_next = false;
+  NextPath np;
+  if (idx < count) {
+    const float* V = q.vertices;
+    const v3 p = mk(V[vplane(idx, 0)], V[vplane(idx, 1)], V[vplane(idx, 2)]);
+    const v3 n = mk(V[vplane(idx, 3)], V[vplane(idx, 4)], V[vplane(idx, 5)]);
+    const v3 d = mk(V[vplane(idx, 6)], V[vplane(idx, 7)], V[vplane(idx, 8)]);
+    const v3 T = mk(V[vplane(idx, 9)], V[vplane(idx, 10)], V[vplane(idx, 11)]);
+    const uint32_t surf = __builtin_bit_cast(uint32_t, V[vplane(idx, 12)]);
+    const uint32_t path_id = __builtin_bit_cast(uint32_t, V[vplane(idx, 13)]);
+    const uint32_t kind = surf >> 28, mi = surf & kSurfaceIndexMask;
+    GuideMixture m;
+    guide_decode(outputs + (uint64_t)idx * kGuideOut, m);
+    StageRng gr = path_rng(sc, seed, path_id, rng_guide_stage(bounce));
+    const float u_choose = gr.next(), u_lobe = gr.next(), u1 = gr.next(), u2 = gr.next();
+    v3 dir = n, fcos = mk(0.0f, 0.0f, 0.0f);
+    float pdf_b = 0.0f;
+    bool ok = false;
+    const bool ggx = kind == kMatGgx;
+    const float* g4 = sc.ggx + 4 * (uint64_t)mi;
+    const float* alb = sc.lambertian + 3 * (uint64_t)mi;
+    if (u_choose < 1.0f - kGuideMix) {
+      StageRng rng = path_rng(sc, seed, path_id, rng_bsdf_stage(bounce));
+      const float ru = rng.next(), rv = rng.next();
+      if (ggx) {
+        const BsdfSample smp = sample_ggx(d, n, mk(g4[0], g4[1], g4[2]), g4[3], ru, rv);
+        ok = smp.ok;
+        dir = smp.direction;
+        pdf_b = smp.pdf;
+        fcos = scale(smp.weight, smp.pdf);
+      } else {
+        // _sample_lambertian (rt/shading.mojo:92-118), as the plain renderer draws it
+        float s, c;
+        sincos_turns(ru, s, c);
+        const float z = 1.0f - 2.0f * rv;
+        const float radius = sqrtf(fmax_sel(1.0f - z * z, 0.0f));
+        v3 sd = add(n, mk(radius * c, radius * s, z));
+        if (fabsf(sd.x) < 1e-8f && fabsf(sd.y) < 1e-8f && fabsf(sd.z) < 1e-8f) sd = n;
+        dir = normalize(sd);
+        pdf_b = fmax_sel(dot(n, normalize(dir)), 0.0f) * kInvPi;
+        fcos = scale(mk(alb[0], alb[1], alb[2]), pdf_b);
+        ok = pdf_b > 0.0f;
+      }
+    } else {
+      dir = guide_sample(m, outputs + (uint64_t)idx * kGuideOut, u_lobe, u1, u2);
+      const float cos_i = dot(n, dir);
+      if (ggx) {
+        const BsdfEval ev = eval_ggx(d, n, mk(g4[0], g4[1], g4[2]), g4[3], dir);
+        pdf_b = ev.pdf;
+        fcos = scale(ev.value, fmax_sel(cos_i, 0.0f));
+        ok = ev.pdf > 0.0f && cos_i > 0.0f;
+      } else {
+        pdf_b = fmax_sel(cos_i, 0.0f) * kInvPi;
+        fcos = scale(mk(alb[0], alb[1], alb[2]), pdf_b);
+        ok = cos_i > 0.0f;
+      }
+    }
+    const float pmix = guide_mix_pdf(pdf_b, guide_pdf(m, dir));
+    v3 Tn = mk(0.0f, 0.0f, 0.0f);
+    if (ok && pmix > 0.0f) {
+      Tn = mul(T, scale(fcos, 1.0f / pmix));
+      want_next = survives_roulette(sc, seed, path_id, bounce, Tn);
+    }
+    if (want_next) {
+      np.id = path_id;
+      const float nf[kPathPlanes] = {p.x, p.y, p.z, kT_MinSecondary, dir.x, dir.y, dir.z, kFltMax,
+                                     Tn.x, Tn.y, Tn.z, pmix};
+#pragma unroll
+      for (int k = 0; k < kPathPlanes; ++k) np.f[k] = nf[k];
+    }
+    if (records) {
+      const v3 wo = normalize(mk(-d.x, -d.y, -d.z));
+      const float rf[kRecordPlanes] = {p.x, p.y, p.z, n.x, n.y, n.z, wo.x, wo.y, wo.z, dir.x, dir.y, dir.z,
+                                       want_next ? pmix : 0.0f, Tn.x, Tn.y, Tn.z};
+      float* r = records + ((uint64_t)(path_id - q.sample_base) * q.depth + bounce) * kRecordPlanes;
+#pragma unroll
+      for (int k = 0; k < kRecordPlanes; ++k) r[k] = rf[k];
+    }
+  }
+  const uint32_t ns = wave_reserve(q.counters + kNext, want_next);
+  if (want_next) {
+    if (ns < cap) {
+      q.dst_ids[ns] = np.id;
+#pragma unroll
+      for (int k = 0; k < kPathPlanes; ++k) q.dst_fields[plane_at<kPathPlanes>(ns, k)] = np.f[k];
+    } else {
+      atomicMax(q.counters + kStatus, kStatusPathOverflow);
+    }
+  }
 }
 
 __global__ void __launch_bounds__(kRtBlock) resolve_kernel(const float* __restrict__ sample, float* __restrict__ pixels,
@@ -1122,6 +1288,7 @@ void build_alias(const std::vector<float>& w, float total, std::vector<float>& p
 
 struct LaunchEvents {
   std::vector<hipEvent_t> ext, sh;  // start/end pairs
+  std::vector<hipEvent_t> inf;      // guided renders: network inference start/end pairs
   hipEvent_t begin = nullptr, end = nullptr;
   bool on = false;
   hipEvent_t make() {
@@ -1132,6 +1299,7 @@ struct LaunchEvents {
   ~LaunchEvents() {
     for (auto e : ext) (void)hipEventDestroy(e);
     for (auto e : sh) (void)hipEventDestroy(e);
+    for (auto e : inf) (void)hipEventDestroy(e);
     if (begin) (void)hipEventDestroy(begin);
     if (end) (void)hipEventDestroy(end);
   }
@@ -1180,12 +1348,20 @@ int32_t render_shape(const bajo_scene* s, const bajo_render_settings* st, Render
   return BAJO_OK;
 }
 
+// A guided render's network and whether the frame records training data.
+struct GuideRun {
+  bajo_guide* g = nullptr;
+  bool train = false;
+  const float* bounds = nullptr;  // scene box lo.xyz, hi.xyz (device)
+};
+
 // Enqueues one render into arena `a` (no host synchronization without `ev`).
 // `camera` (host, 20 floats) is written into the arena's camera buffer first;
-// NULL renders with the camera the buffer already holds.
+// NULL renders with the camera the buffer already holds.  Guided renders
+// (kAlg = BAJO_RENDER_GUIDED) take the guide in `gr`.
 template <int kAlg, bool kWide, int kGeo>
 int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_render_settings* st, const float* camera,
-                   uint32_t camera_count, float* pixels, LaunchEvents* ev) {
+                   uint32_t camera_count, float* pixels, LaunchEvents* ev, const GuideRun* gr = nullptr) {
   const uint64_t total = render_paths(s, st);
   RenderShape shape;
   int32_t rc = render_shape<kAlg, kWide, kGeo>(s, st, &shape);
@@ -1252,7 +1428,12 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
     BJ_HIP(hipGetLastError());
     for (uint32_t b = 0; b < depth; ++b) {
       Queues q{a.ids[b & 1], a.fields[b & 1], a.ids[(b + 1) & 1], a.fields[(b + 1) & 1], a.shadow_ids,
-               a.shadow_fields, a.sample, a.counters, cap, (uint32_t)begin};
+               a.shadow_fields, a.sample, a.counters, cap, (uint32_t)begin, nullptr, nullptr, 0u};
+      if constexpr (kAlg == BAJO_RENDER_GUIDED) {
+        q.vertices = gr->g->vertices;
+        q.contrib = gr->train ? gr->g->contrib : nullptr;
+        q.depth = depth;
+      }
       if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if constexpr (kTail) {
         if (b > 0) {
@@ -1269,6 +1450,20 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
                                                                           a.work_ext, a.stats);
       BJ_HIP(hipGetLastError());
       if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
+      if constexpr (kAlg == BAJO_RENDER_GUIDED) {
+        // the network over this bounce's guided vertices, then their
+        // continuations (none after the last bounce)
+        if (b + 1 < depth && ev) { ev->inf.push_back(ev->make()); (void)hipEventRecord(ev->inf.back(), stream); }
+        if (b + 1 < depth) {
+          rc = guide_infer_vertices(stream, gr->g, gr->g->vertices, a.counters + kVertex, cap, gr->bounds,
+                                    gr->g->outputs);
+          if (rc) return rc;
+          if (ev) { ev->inf.push_back(ev->make()); (void)hipEventRecord(ev->inf.back(), stream); }
+          guide_sample_kernel<<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, gr->g->outputs,
+                                                             gr->train ? gr->g->records : nullptr);
+          BJ_HIP(hipGetLastError());
+        }
+      }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
         shadow_kernel<kAlg, kWide, kGeo><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats);
@@ -1758,6 +1953,91 @@ int32_t bajo_render_cameras(bajo_ctx* ctx, bajo_scene* s, const bajo_render_sett
     stats->traversal_status = counters[8];
     if (stats->status) return (int32_t)(BAJO_STATUS_WAVE_BASE + stats->status);
     if (stats->traversal_status) return (int32_t)stats->traversal_status;
+  }
+  return BAJO_OK;
+}
+
+// ---- guided rendering (no bajo counterpart) ------------------------------------
+
+int32_t bajo_render_guided(bajo_ctx* ctx, bajo_scene* s, bajo_guide* guide, const bajo_render_settings* settings,
+                           const float* camera, float* pixels, int32_t train, bajo_guide_frame_stats* stats) {
+  BJ_REQUIRE(ctx && s && settings && camera && pixels, "bajo_render_guided: NULL argument");
+  if (stats) memset(stats, 0, sizeof(*stats));
+  bajo_render_settings st = *settings;
+  if (!guide) {
+    st.algorithm = BAJO_RENDER_NEE;
+    return bajo_render(ctx, s, &st, camera, pixels, nullptr);
+  }
+  BJ_REQUIRE(st.image_width > 0 && st.image_height > 0 && st.samples_per_pixel > 0 && st.max_depth > 0,
+             "bajo_render_guided: image size, samples per pixel and depth must be positive");
+  BJ_REQUIRE(st.max_depth <= kMaxBounces, "bajo_render_guided: max_depth above %d", kMaxBounces);
+  BJ_REQUIRE(s->n_spheres == 0 && s->env_count == 0 && s->bvh != nullptr,
+             "bajo_render_guided: guided renders take triangle scenes (no spheres, no batches)");
+  BJ_REQUIRE(guide->device == ctx->device, "bajo_render_guided: the guide lives on device %d, the context on %d",
+             guide->device, ctx->device);
+  const uint64_t total = (uint64_t)st.image_width * st.image_height * st.samples_per_pixel;
+  BJ_REQUIRE(total < (1ull << 31), "bajo_render_guided: %llu paths exceed the 31-bit path id",
+             (unsigned long long)total);
+  BJ_REQUIRE(st.path_capacity == 0 || st.path_capacity >= total,
+             "bajo_render_guided: the frame must fit one path chunk (path_capacity 0 or >= %llu)",
+             (unsigned long long)total);
+  st.algorithm = BAJO_RENDER_GUIDED;
+  st.path_capacity = 0;
+  hipStream_t stream = ctx->stream;
+  int32_t rc = guide_prepare_frame(stream, guide, total, st.max_depth, train != 0);
+  if (rc) return rc;
+  GuideRun gr;
+  gr.g = guide;
+  gr.train = train != 0;
+  gr.bounds = s->bvh->scene;
+  LaunchEvents events;
+  LaunchEvents* ev = stats ? &events : nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (ev) {
+    ev->begin = ev->make();
+    ev->end = ev->make();
+    t0 = ev->make();
+    t1 = ev->make();
+    BJ_HIP(hipEventRecord(ev->begin, stream));
+  }
+  if (s->options.node_width == BAJO_WIDTH_WIDE8)
+    rc = run_render<BAJO_RENDER_GUIDED, true, kGeoTriangles>(ctx, s, s->arena, &st, camera, 1, pixels, ev, &gr);
+  else
+    rc = run_render<BAJO_RENDER_GUIDED, false, kGeoTriangles>(ctx, s, s->arena, &st, camera, 1, pixels, ev, &gr);
+  if (rc) return rc;
+  uint64_t samples = 0;
+  uint32_t steps = 0;
+  float loss = 0.0f;
+  if (ev) BJ_HIP(hipEventRecord(t0, stream));
+  if (train) {
+    rc = guide_collect(stream, guide, total, st.max_depth, gr.bounds);
+    if (rc) return rc;
+    rc = guide_train_collected(stream, guide, &samples, &steps, &loss);
+    if (rc) return rc;
+  }
+  if (ev) {
+    BJ_HIP(hipEventRecord(t1, stream));
+    BJ_HIP(hipEventRecord(ev->end, stream));
+    unsigned long long host[kStatsWords];
+    uint32_t counters[kCounterWords];
+    BJ_HIP(hipMemcpyAsync(host, s->arena.stats, sizeof(host), hipMemcpyDeviceToHost, stream));
+    BJ_HIP(hipMemcpyAsync(counters, s->arena.counters, sizeof(counters), hipMemcpyDeviceToHost, stream));
+    BJ_HIP(hipStreamSynchronize(stream));
+    float ms = 0.0f;
+    BJ_HIP(hipEventElapsedTime(&ms, ev->begin, ev->end));
+    stats->render_ns = (uint64_t)(ms * 1.0e6f);
+    BJ_HIP(hipEventElapsedTime(&ms, t0, t1));
+    stats->train_ns = (uint64_t)(ms * 1.0e6f);
+    for (size_t i = 0; i + 1 < ev->inf.size(); i += 2) {
+      BJ_HIP(hipEventElapsedTime(&ms, ev->inf[i], ev->inf[i + 1]));
+      stats->infer_ns += (uint64_t)(ms * 1.0e6f);
+    }
+    stats->guided_vertices = host[kGuidedWord];
+    stats->training_samples = samples;
+    stats->adam_steps = steps;
+    stats->loss = loss;
+    if (counters[kStatus]) return (int32_t)(BAJO_STATUS_WAVE_BASE + counters[kStatus]);
+    if (counters[8]) return (int32_t)counters[8];
   }
   return BAJO_OK;
 }

@@ -83,6 +83,15 @@ class PathGuide:
         n = s.numel() // GUIDE_SAMPLE_FLOATS
         _lib.check(_lib.lib().bajo_guide_train(self.ctx.handle, self._handle, ptr(s), n))
 
+    def training_samples(self, cap: int = 1 << 22) -> np.ndarray:
+        """The samples the last training frame collected, f32[n, 13] (inputs 9, direction 3, weight 1)."""
+        buf = np.empty((cap, GUIDE_SAMPLE_FLOATS), np.float32)
+        count = ctypes.c_uint64()
+        _lib.check(_lib.lib().bajo_guide_training_samples(self.ctx.handle, self._handle,
+                                                          buf.ctypes.data_as(ctypes.c_void_p), cap,
+                                                          ctypes.byref(count)))
+        return buf[: min(count.value, cap)].copy()
+
     def free(self) -> None:
         if getattr(self, "_handle", None):
             _lib.lib().bajo_guide_free(self._handle)

@@ -560,6 +560,150 @@ def render_target_workload(ctx) -> dict:
     return out
 
 
+GUIDE_FLOP_PER_SAMPLE = 2 * (216 * 64 + 2 * 64 * 64 + 64 * 40)  # forward, algorithmic
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def guided_workload(ctx, width: int = WIDTH, height: int = HEIGHT, plain_spps=(16, 64, 256), ref_spp: int = 2048,
+                    pretrain_frames: int = 20, pretrain_size=(960, 540), frame_spp: int = 4, max_frames: int = 64,
+                    seed: int = 2030) -> dict:
+    """Learned path guiding on the 1.2M-triangle room (half Lambertian, half GGX, lit through a doorway),
+    plain NEE against guided NEE, both measured against a `ref_spp`-spp plain NEE reference of the frame.
+
+    Plain: one render of S spp for each S in `plain_spps` (its wall time and MSE).  Guided: the guide starts
+    untrained; `pretrain_frames` frames of 1 spp at `pretrain_size` each train it (16 Adam steps of 65,536
+    samples per frame, lr 1e-2; their images are not kept), then full-size frames of `frame_spp` spp add to
+    the running image.  Every call is timed with a synchronize on both sides, pretraining included.  For each
+    S: the guided wall time to reach plain's MSE (its ratio to plain's time is the metric), and the MSE
+    ratio at equal time.  The reference's own variance (estimated from its two halves) is reported; it adds
+    equally to every measured MSE, so it does not move the match points."""
+    import torch
+
+    from bajo_amd import scenes
+    from bajo_amd.guide import PathGuide, render_guided
+    from bajo_amd.rt import RENDER, GpuScene, RenderSettings, enqueue_render_gpu
+
+    room = scenes.room_scene()
+    g = GpuScene(room, ctx, node_width=8)
+    cam = scenes.room_camera()
+    dev = ctx.torch_device
+    out = torch.empty((height, width, 3), dtype=torch.float32, device=dev)
+    chunk = 64
+    halves = [torch.zeros((height, width, 3), dtype=torch.float64, device=dev) for _ in range(2)]
+    t0 = time.perf_counter()
+    for k in range(ref_spp // chunk):
+        enqueue_render_gpu(RenderSettings(width, height, chunk, 10_000 + k, DEPTH), cam, g, RENDER.NEE, pixels=out)
+        halves[k % 2] += out.double()
+    ref = (halves[0] + halves[1]) / (ref_spp // chunk)
+    ref_var = float((((halves[0] - halves[1]) / (ref_spp // chunk // 2)) ** 2).mean()) / 4.0
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t0
+    del halves
+    print(f"[bench] guided: reference {ref_spp} spp in {ref_s:.1f} s", file=sys.stderr, flush=True)
+
+    def mse(img):
+        return float(((img.double() - ref) ** 2).mean())
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, r
+
+    plain = {}
+    enqueue_render_gpu(RenderSettings(width, height, 4, seed, DEPTH), cam, g, RENDER.NEE, pixels=out)  # warm-up
+    for S in plain_spps:
+        dt, r = timed(lambda: enqueue_render_gpu(RenderSettings(width, height, S, seed + S, DEPTH), cam, g,
+                                                 RENDER.NEE, pixels=out, with_stats=True))
+        plain[S] = {"ms": dt * 1e3, "mse": mse(out),
+                    "mrays_s": (r.stats["extension_rays"] + r.stats["shadow_rays"]) / dt / 1e6}
+    guide = PathGuide(ctx, learning_rate=1e-2, batch_size=65536, steps_per_frame=16, seed=seed)
+    pw, ph = pretrain_size
+    pre = torch.empty((ph, pw, 3), dtype=torch.float32, device=dev)
+    wall, train_ms, samples = 0.0, 0.0, 0
+    for f in range(pretrain_frames):
+        dt, (_, fs) = timed(lambda: render_guided(g, guide, RenderSettings(pw, ph, 1, seed + 100 + f, DEPTH), cam,
+                                                  pixels=pre, train=True, with_stats=True))
+        wall += dt
+        train_ms += fs.train_ns / 1e6
+        samples += fs.training_samples
+    pretrain_s = wall
+    acc = torch.zeros((height, width, 3), dtype=torch.float64, device=dev)
+    curve, infer_ns, vertices, frames, frame_s = [], 0, 0, 0, 0.0
+    reach = {}
+    for f in range(max_frames):
+        dt, (_, fs) = timed(lambda: render_guided(g, guide, RenderSettings(width, height, frame_spp, seed + 1000 + f,
+                                                                           DEPTH), cam, pixels=out, with_stats=True))
+        wall += dt
+        frame_s += dt
+        frames += 1
+        acc += out.double()
+        e = float(((acc / frames - ref) ** 2).mean())
+        curve.append((frames * frame_spp, round(wall * 1e3, 3), e))
+        infer_ns += fs.infer_ns
+        vertices += fs.guided_vertices
+        for S in plain_spps:
+            if S not in reach and e <= plain[S]["mse"]:
+                reach[S] = (wall, frames * frame_spp)
+        if len(reach) == len(plain_spps):
+            break
+    per_spp_ms = frame_s * 1e3 / (frames * frame_spp)
+    by_budget = {}
+    for S in plain_spps:
+        tg = reach.get(S)
+        eq = next((c[2] for c in curve if c[1] >= plain[S]["ms"]), None)
+        by_budget[str(S)] = {
+            "plain_ms": plain[S]["ms"],
+            "plain_mse": plain[S]["mse"],
+            "guided_ms_to_plain_mse": tg[0] * 1e3 if tg else None,
+            "guided_spp_to_plain_mse": tg[1] if tg else None,
+            "wall_time_ratio": tg[0] * 1e3 / plain[S]["ms"] if tg else None,
+            "guided_mse_at_equal_time": eq,
+            "equal_time_mse_ratio": eq / plain[S]["mse"] if eq else None,
+        }
+    s16 = next((c for c in curve if c[0] >= 16), None)
+    infer_s = infer_ns / 1e9
+    achieved = vertices * GUIDE_FLOP_PER_SAMPLE / infer_s / 1e12 if infer_s > 0 else None
+    print(f"[bench] guided: " + "; ".join(
+        f"S={S}: plain {b['plain_ms']:.0f} ms, guided {b['guided_ms_to_plain_mse']} ms (ratio {b['wall_time_ratio']})"
+        for S, b in by_budget.items()), file=sys.stderr, flush=True)
+    res = {
+        "workload": f"1.2M-triangle room ({g.triangle_count} triangles, 50% Lambertian / 50% GGX roughness "
+        f"U[0.05,0.5], one emitter seen through a doorway), {width}x{height}, depth {DEPTH}, 8-wide LBVH; guide: "
+        "9 inputs x 12 frequencies sin/cos -> 3 x 64 ReLU (BF16 MFMA, FP32 accumulate) -> 8 vMF lobes, Adam lr 1e-2, "
+        "MIS 50/50 with BSDF sampling, NEE",
+        "reference": f"plain NEE, {ref_spp} spp ({ref_s:.1f} s on the GPU); MSE over all pixels and channels; "
+        f"the reference's own variance {ref_var:.3e} is inside every MSE",
+        "reference_variance": ref_var,
+        "protocol": f"guide untrained at start; {pretrain_frames} training frames of {pw}x{ph}x1 spp (16 Adam steps "
+        f"each), then {width}x{height}x{frame_spp}-spp guided frames; wall time includes training",
+        "pretrain_ms": pretrain_s * 1e3,
+        "pretrain_train_ms": train_ms,
+        "pretrain_samples": samples,
+        "by_plain_budget": by_budget,
+        "plain_ms_per_spp": plain[plain_spps[-1]]["ms"] / plain_spps[-1],
+        "plain_mrays_s": plain[plain_spps[-1]]["mrays_s"],
+        "guided_ms_per_spp": per_spp_ms,
+        "equal_spp_mse_ratio_16": (s16[2] / plain[16]["mse"]) if (s16 and 16 in plain) else None,
+        "mse_curve": curve[:: max(1, len(curve) // 16)],
+        "guide_infer_ms_per_spp": infer_s * 1e3 / (frames * frame_spp),
+        "guided_vertices_per_spp": vertices / (frames * frame_spp),
+        "mlp_roofline": {
+            "kernel": "guide_forward_kernel<VertexRows> (encoding + 4 BF16 MFMA layers)",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": MFMA_BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / MFMA_BF16_PEAK_TFLOPS if achieved else None,
+            "flop_per_sample": GUIDE_FLOP_PER_SAMPLE,
+        },
+    }
+    guide.free()
+    g.free()
+    return res
+
+
 def cpu_baseline(scene, cam, threads: int) -> dict:
     import oracle
 
@@ -599,6 +743,7 @@ def main() -> int:
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-environment workload and print its JSON")
     ap.add_argument("--target-only", action="store_true", help="run only the render-target workload")
+    ap.add_argument("--guided-only", action="store_true", help="run only the learned-guiding room workload")
     ap.add_argument("--batched-render-only", action="store_true",
                     help="run only the batched render (without its autotuner and per-environment loop, for "
                          "profiling it)")
@@ -640,6 +785,9 @@ def main() -> int:
         return 0
     if args.target_only:
         print(json.dumps({"render_target": render_target_workload(ctx)}), flush=True)
+        return 0
+    if args.guided_only:
+        print(json.dumps({"guided": guided_workload(ctx)}), flush=True)
         return 0
     scene = scenes.icosphere_scene(N_SPHERE)
     method = {"lbvh": 0, "hploc": 1}[args.method]

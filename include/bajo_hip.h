@@ -539,6 +539,10 @@ int32_t bajo_guide_train(bajo_ctx* ctx, bajo_guide* guide, const float* samples,
  * its mixture density into pdfs f32[n] (device buffers). */
 int32_t bajo_guide_mixture_sample(bajo_ctx* ctx, const float* outputs, uint64_t n, const float* uniforms, float* dirs,
                                   float* pdfs);
+/* Host copy of the training samples the last training frame collected
+ * (f32[13] each, at most `cap`); *count receives how many it collected. */
+int32_t bajo_guide_training_samples(bajo_ctx* ctx, const bajo_guide* guide, float* samples, uint64_t cap,
+                                    uint64_t* count);
 /* Per-frame figures of a guided render (bajo_render_guided). */
 typedef struct {
   uint64_t guided_vertices;  /* vertices that sampled through the guide's MIS  */

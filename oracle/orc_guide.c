@@ -4,8 +4,9 @@
  * bajo has no learned guiding (its renderer samples the BSDF only,
  * rt/gpu/path_shading.mojo:463-535, rt/shading.mojo:92-118), so this file
  * restates the library's own definition from its documented formulas:
- *  - encoding: 9 inputs x 12 frequencies, (sin, cos)(pi 2^f x) computed as
- *    sincos_turns(frac(x 2^(f-1))) in FP32 (the renderer's sin/cos series);
+ *  - encoding: 9 inputs x 12 frequencies, (sin, cos)(pi 2^f x): frequencies
+ *    0, 4, 8 as sincos_turns(frac(x 2^(f-1))) in FP32 (the renderer's sin/cos
+ *    series), the others by FP32 double-angle steps;
  *  - network: 216 -> 64 -> 64 -> 64 -> 40, ReLU, evaluated in FP64; with
  *    `emulate` the operands are rounded to BF16 (round to nearest even)
  *    where the device rounds them (features, weights, hidden activations,
@@ -57,17 +58,24 @@ float orc_bf16_round(float x) {
 
 static double rnd(double x, int emulate) { return emulate ? (double)orc_bf16_round((float)x) : x; }
 
-/* Features of one sample (FP32 series, as the device computes them). */
+/* Features of one sample as the device computes them: frequencies 0, 4 and 8
+ * from the FP32 series of sin/cos(2 pi frac(x 2^(f-1))), the three after each
+ * by the double-angle formulas in FP32. */
 void orc_guide_encode(const float *x, float *feat) {
     for (int d = 0; d < G_IN; ++d)
-        for (int f = 0; f < G_FREQ; ++f) {
-            float scale = ldexpf(1.0f, f - 1);
-            float y = x[d] * scale;
+        for (int b = 0; b < 3; ++b) {
+            float s[4], c[4];
+            float y = x[d] * ldexpf(1.0f, 4 * b - 1);
             float u = y - floorf(y);
-            float s, c;
-            orc_sincos_turns(u, &s, &c);
-            feat[24 * d + 2 * f] = s;
-            feat[24 * d + 2 * f + 1] = c;
+            orc_sincos_turns(u, &s[0], &c[0]);
+            for (int t = 1; t < 4; ++t) {
+                s[t] = 2.0f * s[t - 1] * c[t - 1];
+                c[t] = (c[t - 1] - s[t - 1]) * (c[t - 1] + s[t - 1]);
+            }
+            for (int t = 0; t < 4; ++t) {
+                feat[24 * d + 2 * (4 * b + t)] = s[t];
+                feat[24 * d + 2 * (4 * b + t) + 1] = c[t];
+            }
         }
 }
 

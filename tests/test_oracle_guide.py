@@ -68,7 +68,7 @@ def test_encoding_matches_numpy():
             a = math.pi * 2.0**f * float(x[d])
             want[24 * d + 2 * f] = math.sin(a)
             want[24 * d + 2 * f + 1] = math.cos(a)
-    np.testing.assert_allclose(feat, want, atol=3e-6)
+    np.testing.assert_allclose(feat, want, atol=2e-5)
 
 
 def test_gradient_matches_finite_differences():

@@ -63,6 +63,45 @@ def batched_soups(count: int = 4096, seed: int = 2031, lo: int = 256, hi: int = 
     return out, offsets
 
 
+# bajo's regular-grid BVH fixture (bajo/benchmark/bvh_fixtures.mojo:11-58): 256 x 256 triangles on the
+# plane z = 2 and 4 rays per triangle, one in four a miss; its published closest-hit checksum
+# (sum over hits of t + u + v + normal + prim) is 6443188224.0 (bench/results/bvh_cpu/bvh_cpu.md).
+GRID_SIDE = 256
+GRID_CHECKSUM = 6443188224.0
+
+
+def grid_fixture() -> tuple[np.ndarray, np.ndarray]:
+    """(vertices f32[9 * 65536], rays f32[8 * 262144] field-major: origin, t_min, direction, t_max)."""
+    i = np.arange(GRID_SIDE * GRID_SIDE)
+    cx = ((i % GRID_SIDE).astype(np.float32) - np.float32(GRID_SIDE) * np.float32(0.5)) * np.float32(3.0)
+    cy = ((i // GRID_SIDE).astype(np.float32) - np.float32(GRID_SIDE) * np.float32(0.5)) * np.float32(3.0)
+    z = np.full_like(cx, 2.0)
+    v = np.stack([cx - 0.75, cy - 0.75, z, cx + 0.75, cy - 0.75, z, cx, cy + 0.75, z], axis=1).astype(np.float32)
+    r = np.arange(4 * GRID_SIDE * GRID_SIDE)
+    prim = r % (GRID_SIDE * GRID_SIDE)
+    miss = r % 4 == 0
+    rays = np.zeros((8, r.size), np.float32)
+    rays[0] = np.where(miss, np.float32(10000.0) + r.astype(np.float32), cx[prim])
+    rays[1] = np.where(miss, np.float32(10000.0), cy[prim])
+    rays[2] = 0.0
+    rays[3] = 0.0
+    rays[6] = 1.0
+    rays[7] = np.finfo(np.float32).max
+    return v.reshape(-1), rays.reshape(-1)
+
+
+def grid_checksum(hits: np.ndarray) -> float:
+    """bajo's checksum of closest hits f32[8 R] (u, v, prim bits, inst, normal.xyz, t per hit;
+    bench/bvh/bench_cpu_bvh_grid.mojo:35-52): sum over hits of t + u + v + normal.x + normal.y +
+    normal.z + prim, in float64."""
+    h = np.ascontiguousarray(hits, np.float32).reshape(-1, 8)
+    prim = h[:, 2].view(np.uint32)
+    hit = prim != 0xFFFFFFFF
+    hh = h[hit].astype(np.float64)
+    return float((hh[:, 7] + hh[:, 0] + hh[:, 1] + hh[:, 4] + hh[:, 5] + hh[:, 6]).sum()
+                 + prim[hit].astype(np.float64).sum())
+
+
 def bounds(vertices: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     v = vertices.reshape(-1, 3)
     return v.min(axis=0), v.max(axis=0)

@@ -119,6 +119,8 @@ _SIG = {
                 c_void_p, c_void_p, c_void_p, c_void_p, c_int],
     ),
     "orc_eval_ggx": (None, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
+    "orc_render_set_timing": (None, [c_int]),
+    "orc_render_best_seconds": (ctypes.c_double, []),
     "orc_bf16_round": (c_float, [c_float]),
     "orc_guide_encode": (None, [c_void_p, c_void_p]),
     "orc_guide_forward": (None, [c_void_p, c_void_p, c_uint64, c_int, c_void_p]),
@@ -172,18 +174,49 @@ def build() -> Path:
     return LIB_PATH
 
 
+def _load(path: Path) -> ctypes.CDLL:
+    h = ctypes.CDLL(str(path))
+    for name, (res, args) in _SIG.items():
+        fn = getattr(h, name)
+        fn.restype = res
+        fn.argtypes = args
+    return h
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        h = ctypes.CDLL(str(LIB_PATH))
-        for name, (res, args) in _SIG.items():
-            fn = getattr(h, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = h
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+NATIVE_PATH = HERE / "liboracle_native.so"
+
+
+def native_lib() -> ctypes.CDLL:
+    """The same sources built -O3 for the host's own instruction set (the CPU baseline's build): compiled here,
+    on the machine that runs it."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "native"], check=True)
+    return _load(NATIVE_PATH)
+
+
+class use_lib:
+    """Context manager: routes the module's functions through another build of the oracle."""
+
+    def __init__(self, handle: ctypes.CDLL):
+        self.handle = handle
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = self.handle
+        return self.handle
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
 
 
 def _p(a: np.ndarray) -> c_void_p:

@@ -108,3 +108,23 @@ def test_icosphere_4m_incoherent(gpu_ctx):
     got = to_host(hits, np.float32).reshape(-1, 8)[sample].reshape(-1)
     frac = compare_hits(got, want)
     assert frac > 0.3  # about 44% of box-origin rays reach the surface
+
+
+@pytest.mark.parametrize("width", [2, 8])
+def test_grid_fixture_checksum_equals_bajo_published(gpu_ctx, width):
+    """bajo's 65,536-triangle grid fixture and its 262,144 rays (bajo/benchmark/bvh_fixtures.mojo:11-58):
+    the GPU hits give bajo's published checksum (bench/results/bvh_cpu/bvh_cpu.md) and equal the oracle's."""
+    from bajo_amd import scenes
+    from bajo_amd.bvh import build_triangle_bvh
+
+    announce(f"grid fixture width {width}")
+    v, rays = scenes.grid_fixture()
+    bvh = build_triangle_bvh(torch.from_numpy(v).cuda(), node_width=width)
+    n = rays.size // 8
+    hits = torch.empty(8 * n, dtype=torch.float32, device="cuda")
+    bvh.launch_rays(torch.from_numpy(rays).cuda(), hits, n)
+    got = to_host(hits, np.float32)
+    assert scenes.grid_checksum(got) == scenes.GRID_CHECKSUM
+    want = oracle.trace_bvh2(v, oracle.build_lbvh(v), rays)
+    compare_hits(got, want)
+    bvh.free()

@@ -365,3 +365,15 @@ def test_camera_rays_follow_make_ray_raster():
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     np.testing.assert_allclose(rays[4:7].T, d, atol=2e-7)
     assert np.all(rays[3] == 0) and np.all(rays[7] == np.finfo(np.float32).max)
+
+
+def test_grid_fixture_checksum_equals_bajo_published():
+    """bajo's 65,536-triangle grid fixture (bajo/benchmark/bvh_fixtures.mojo:11-58) traced closest-hit
+    through the oracle's LBVH: the published checksum of bench/results/bvh_cpu/bvh_cpu.md (hits 196,608,
+    checksum 6443188224.000)."""
+    from bajo_amd import scenes
+
+    v, rays = scenes.grid_fixture()
+    hits = oracle.trace_bvh2(v, oracle.build_lbvh(v), rays)
+    assert int((oracle.hit_prims(hits) != oracle.SENTINEL).sum()) == 196608
+    assert scenes.grid_checksum(hits) == scenes.GRID_CHECKSUM

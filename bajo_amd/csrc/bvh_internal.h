@@ -37,6 +37,8 @@ struct __attribute__((aligned(16))) BjWide {
 static_assert(sizeof(BjWide) == 96, "BjWide must be 96 bytes");
 
 constexpr uint32_t kWideLeafStartMask = 0x0FFFFFFFu;
+// Node widths with the quantized wide layout (4 or 8 live child slots).
+BJ_HD bool wide_layout(int node_width) { return node_width == BAJO_WIDTH_WIDE4 || node_width == BAJO_WIDTH_WIDE8; }
 constexpr uint32_t kWideMaxLeaf = 8;      // largest leaf child the encoding holds
 constexpr uint32_t kWideDefaultLeaf = 4;  // triangles per leaf child ("fat leaf")
 constexpr uint64_t kWideMaxPrims = 1ull << 28;

@@ -663,7 +663,7 @@ struct StageEvents {
 int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n, bajo_build_timings* timings) {
   int32_t rc = ensure_capacity(h, n);
   if (rc) return rc;
-  if (h->node_width == BAJO_WIDTH_WIDE8) {
+  if (wide_layout(h->node_width)) {
     rc = ensure_wide_capacity(h, n);
     if (rc) return rc;
   }
@@ -732,7 +732,7 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
     BJ_HIP(hipGetLastError());
   }
   ev.record(5, st);
-  if (h->node_width == BAJO_WIDTH_WIDE8) {
+  if (wide_layout(h->node_width)) {
     rc = collapse_wide(ctx, h);
     if (rc) return rc;
   } else {
@@ -771,7 +771,7 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
   const uint64_t n = offsets[S];
   int32_t rc = ensure_capacity(h, n);
   if (rc) return rc;
-  if (h->node_width == BAJO_WIDTH_WIDE8) {
+  if (wide_layout(h->node_width)) {
     rc = ensure_wide_capacity(h, n);
     if (rc) return rc;
   }
@@ -837,7 +837,7 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
     BJ_HIP(hipGetLastError());
   }
   ev.record(5, st);
-  if (h->node_width == BAJO_WIDTH_WIDE8) {
+  if (wide_layout(h->node_width)) {
     rc = collapse_wide(ctx, h);
     if (rc) return rc;
   } else {
@@ -919,8 +919,8 @@ int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo
   BJ_REQUIRE(ctx != nullptr && out != nullptr, "%s: ctx/out is NULL", fn);
   BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH || opt.method == BAJO_BUILD_HPLOC,
              "%s: unsupported build method %d", fn, opt.method);
-  BJ_REQUIRE(opt.node_width == BAJO_WIDTH_BINARY || opt.node_width == BAJO_WIDTH_WIDE8,
-             "%s: node_width must be 2 or 8, got %d", fn, opt.node_width);
+  BJ_REQUIRE(opt.node_width == BAJO_WIDTH_BINARY || wide_layout(opt.node_width),
+             "%s: node_width must be 2, 4 or 8, got %d", fn, opt.node_width);
   if (opt.leaf_size == 0) opt.leaf_size = (int32_t)kWideDefaultLeaf;
   BJ_REQUIRE(opt.leaf_size >= 1 && opt.leaf_size <= (int32_t)kWideMaxLeaf,
              "%s: leaf_size must be in [1, 8], got %d", fn, opt.leaf_size);
@@ -929,7 +929,7 @@ int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo
   BJ_REQUIRE(vertices != nullptr, "%s: input array is NULL", fn);
   BJ_REQUIRE(n <= bj::kSortMaxKeys, "%s: n=%llu exceeds 2^30-1 %s", fn,
              (unsigned long long)n, items);
-  BJ_REQUIRE(opt.node_width != BAJO_WIDTH_WIDE8 || n < kWideMaxPrims,
+  BJ_REQUIRE(!wide_layout(opt.node_width) || n < kWideMaxPrims,
              "%s: n=%llu exceeds the 8-wide limit of 2^28-1 %s", fn, (unsigned long long)n, items);
   bajo_bvh* h = *out;
   bool fresh = h == nullptr;
@@ -989,14 +989,14 @@ int32_t bajo_bvh_get_info(const bajo_bvh* h, bajo_bvh_info* out) {
   out->leaf_parent = h->leaf_parent ? h->leaf_parent + h->leaf_base : nullptr;
   out->leaf_counts = h->leaf_counts ? h->leaf_counts + h->node_base : nullptr;
   out->scene_bounds = h->scene;
-  out->wide_node_count = h->node_width == BAJO_WIDTH_WIDE8 ? h->wide_count : 0;
-  out->wide_nodes = h->node_width == BAJO_WIDTH_WIDE8 ? h->wide : nullptr;
+  out->wide_node_count = wide_layout(h->node_width) ? h->wide_count : 0;
+  out->wide_nodes = wide_layout(h->node_width) ? h->wide : nullptr;
   return BAJO_OK;
 }
 
 int32_t bajo_bvh_copy_wide(bajo_ctx* ctx, const bajo_bvh* h, void* nodes, uint64_t capacity, uint64_t* count) {
   BJ_REQUIRE(ctx != nullptr && h != nullptr, "bajo_bvh_copy_wide: NULL argument");
-  BJ_REQUIRE(h->node_width == BAJO_WIDTH_WIDE8, "bajo_bvh_copy_wide: the BVH was built with node width %d",
+  BJ_REQUIRE(wide_layout(h->node_width), "bajo_bvh_copy_wide: the BVH was built with node width %d",
              h->node_width);
   BJ_REQUIRE(h->seg_count == 0, "bajo_bvh_copy_wide: copy a BLAS set's segments through their views");
   if (!h->owner) {
@@ -1104,8 +1104,8 @@ int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const
   BJ_REQUIRE(segment_count > 0, "%s: a BLAS set needs at least one segment", fn);
   BJ_REQUIRE(opt.method == BAJO_BUILD_LBVH || opt.method == BAJO_BUILD_HPLOC,
              "%s: method must be LBVH (0) or H-PLOC (1), got %d", fn, opt.method);
-  BJ_REQUIRE(opt.node_width == BAJO_WIDTH_BINARY || opt.node_width == BAJO_WIDTH_WIDE8,
-             "%s: node_width must be 2 or 8, got %d", fn, opt.node_width);
+  BJ_REQUIRE(opt.node_width == BAJO_WIDTH_BINARY || wide_layout(opt.node_width),
+             "%s: node_width must be 2, 4 or 8, got %d", fn, opt.node_width);
   if (opt.leaf_size == 0) opt.leaf_size = (int32_t)kWideDefaultLeaf;
   BJ_REQUIRE(opt.leaf_size >= 1 && opt.leaf_size <= (int32_t)kWideMaxLeaf, "%s: leaf_size must be in [1, 8]", fn);
   BJ_REQUIRE(opt.reserved == 0, "%s: reserved option field must be 0", fn);
@@ -1114,7 +1114,7 @@ int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const
     BJ_REQUIRE(segment_offsets[s + 1] > segment_offsets[s], "%s: segment %u is empty (offsets must increase)", fn, s);
   const uint64_t n = segment_offsets[segment_count];
   BJ_REQUIRE(n <= kSortMaxKeys, "%s: %llu triangles exceed 2^30-1", fn, (unsigned long long)n);
-  BJ_REQUIRE(opt.node_width != BAJO_WIDTH_WIDE8 || n < kWideMaxPrims, "%s: %llu triangles exceed the 8-wide limit",
+  BJ_REQUIRE(!wide_layout(opt.node_width) || n < kWideMaxPrims, "%s: %llu triangles exceed the 8-wide limit",
              fn, (unsigned long long)n);
   bajo_blas_set* set = *out;
   const bool fresh = set == nullptr;

@@ -1486,7 +1486,7 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
 // or spheres + triangles).
 template <int kAlg, typename F>
 int32_t by_scene(const bajo_scene* s, F&& fn) {
-  const bool wide = s->options.node_width == BAJO_WIDTH_WIDE8;
+  const bool wide = wide_layout(s->options.node_width);
   if (s->env_count)
     return wide ? fn.template operator()<kAlg, true, kGeoBatch>() : fn.template operator()<kAlg, false, kGeoBatch>();
   if (s->n_spheres)
@@ -2000,7 +2000,7 @@ int32_t bajo_render_guided(bajo_ctx* ctx, bajo_scene* s, bajo_guide* guide, cons
     t1 = ev->make();
     BJ_HIP(hipEventRecord(ev->begin, stream));
   }
-  if (s->options.node_width == BAJO_WIDTH_WIDE8)
+  if (wide_layout(s->options.node_width))
     rc = run_render<BAJO_RENDER_GUIDED, true, kGeoTriangles>(ctx, s, s->arena, &st, camera, 1, pixels, ev, &gr);
   else
     rc = run_render<BAJO_RENDER_GUIDED, false, kGeoTriangles>(ctx, s, s->arena, &st, camera, 1, pixels, ev, &gr);

@@ -345,7 +345,7 @@ int32_t bajo_tlas_create(bajo_ctx* ctx, const bajo_bvh* const* blases, uint32_t 
   TL_HIP(hipMalloc(&t->status, sizeof(uint32_t)));
   for (uint32_t b = 0; b < blas_count; ++b) {
     const AccelView av = accel_view(blases[b], t->status);
-    views[b] = TlasBlas{av.bin, av.wide, blases[b]->node_width == BAJO_WIDTH_WIDE8 ? 1u : 0u,
+    views[b] = TlasBlas{av.bin, av.wide, wide_layout(blases[b]->node_width) ? 1u : 0u,
                         blases[b]->spheres ? 1u : 0u, (uint32_t)blases[b]->n};
   }
   TL_HIP(hipMalloc(&t->boxes, t->host_boxes.size() * sizeof(float)));

@@ -241,11 +241,11 @@ template <int kMode, bool kStats, bool kCamera>
 int32_t launch(bajo_ctx* ctx, const bajo_bvh* h, const float* rays, const float* cams, uint32_t w, uint32_t hh,
                float* hits, uint64_t R, uint32_t* stats) {
   if (h->spheres) {
-    if (h->node_width == BAJO_WIDTH_WIDE8)
+    if (wide_layout(h->node_width))
       return launch_width<kMode, kStats, kCamera, true, false, true>(ctx, h, rays, cams, w, hh, hits, R, stats);
     return launch_width<kMode, kStats, kCamera, false, false, true>(ctx, h, rays, cams, w, hh, hits, R, stats);
   }
-  if (h->node_width == BAJO_WIDTH_WIDE8)
+  if (wide_layout(h->node_width))
     return launch_width<kMode, kStats, kCamera, true>(ctx, h, rays, cams, w, hh, hits, R, stats);
   return launch_width<kMode, kStats, kCamera, false>(ctx, h, rays, cams, w, hh, hits, R, stats);
 }
@@ -352,7 +352,7 @@ int32_t bajo_blas_set_trace_camera(bajo_ctx* ctx, const bajo_blas_set* set, cons
   BJ_REQUIRE(R < (1ull << 31), "bajo_blas_set_trace_camera: %u segments x %u x %u rays exceed 2^31", h->seg_count,
              width, height);
   constexpr int kC = BAJO_TRACE_CLOSEST_HIT;
-  if (h->node_width == BAJO_WIDTH_WIDE8) {
+  if (wide_layout(h->node_width)) {
     if (stats) return bj::launch_width<kC, true, true, true, true>(ctx, h, nullptr, cameras, width, height, hits, R, stats);
     return bj::launch_width<kC, false, true, true, true>(ctx, h, nullptr, cameras, width, height, hits, R, nullptr);
   }

@@ -642,7 +642,7 @@ struct AccelView {
 inline AccelView accel_view(const bajo_bvh* h, uint32_t* status) {
   AccelView av;
   av.bin = BvhView{h->nodes, h->tris, h->root, h->trav_nodes, h->trav_prims, status};
-  av.wide = WideView{h->wide, h->tris, h->node_width == BAJO_WIDTH_WIDE8 ? h->wide_count : 0u, h->trav_prims, status,
+  av.wide = WideView{h->wide, h->tris, wide_layout(h->node_width) ? h->wide_count : 0u, h->trav_prims, status,
                      h->wide_root};
   return av;
 }

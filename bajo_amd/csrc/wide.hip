@@ -176,7 +176,8 @@ __global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const Bj
                                                                         BjWide* __restrict__ wide,
                                                                         uint32_t* __restrict__ wide_src,
                                                                         uint32_t* __restrict__ ctl,
-                                                                        uint32_t capacity, uint32_t max_leaf) {
+                                                                        uint32_t capacity, uint32_t max_leaf,
+                                                                        int max_children) {
   const uint32_t begin = ctl[kCtlBegin], end = ctl[kCtlEnd];
   const uint32_t lane = lane_id();
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -194,7 +195,7 @@ __global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const Bj
       n = 2;
       // open largest-area children: first those too big for a leaf, then any inner child
       for (int phase = 0; phase < 2; ++phase) {
-        for (int step = 0; step < 6 && n < 8; ++step) {
+        for (int step = 0; step < 6 && n < max_children; ++step) {
           int pick = -1;
           float best = -1.0f;
           uint32_t pick_ref = 0;
@@ -331,7 +332,7 @@ int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h) {
     for (int l = 0; l < kLevelBatch; ++l) {
       collapse_level_kernel<<<grid, kCollapseBlock, 0, st>>>(h->nodes, h->leaf_counts, h->wide, h->wide_src,
                                                              h->wide_ctl, (uint32_t)h->wide_capacity,
-                                                             (uint32_t)h->wide_leaf);
+                                                             (uint32_t)h->wide_leaf, h->node_width);
       level_advance_kernel<<<1, 64, 0, st>>>(h->wide_ctl, (uint32_t)h->wide_capacity);
     }
     BJ_HIP(hipGetLastError());

@@ -63,11 +63,15 @@ extern "C" {
  * bvh/gpu/compressed_bounds_bvh.mojo:84-369). */
 #define BAJO_WIDTH_BINARY 2
 #define BAJO_WIDTH_WIDE8 8
+/* 4-wide nodes (bajo's render_gpu default W = 4, bvh/gpu/builder/
+ * wide_collapse.mojo:148-643): the same collapse opening up to 4 children,
+ * stored in the same quantized node record with 4 live slots. */
+#define BAJO_WIDTH_WIDE4 4
 
 /* Build options (bajo's compile-time W, L and method parameters). */
 typedef struct {
   int32_t method;     /* BAJO_BUILD_LBVH or BAJO_BUILD_HPLOC               */
-  int32_t node_width; /* BAJO_WIDTH_BINARY or BAJO_WIDTH_WIDE8             */
+  int32_t node_width; /* BAJO_WIDTH_BINARY, BAJO_WIDTH_WIDE4 or _WIDE8      */
   int32_t leaf_size;  /* width 8: most triangles per leaf child, 1..8; 0 = 4 */
   int32_t reserved;   /* must be 0                                         */
 } bajo_build_options;

@@ -119,6 +119,10 @@ _SIG = {
                 c_void_p, c_void_p, c_void_p, c_void_p, c_int],
     ),
     "orc_eval_ggx": (None, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
+    "orc_collapse_wide_n": (
+        c_uint64,
+        [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_void_p, c_uint32, c_uint32, c_void_p, c_uint64],
+    ),
     "orc_render_set_timing": (None, [c_int]),
     "orc_render_best_seconds": (ctypes.c_double, []),
     "orc_bf16_round": (c_float, [c_float]),
@@ -509,17 +513,17 @@ def quantize(o: float, e: int, lo: float, hi: float):
     return a.value, b.value
 
 
-def collapse_wide(bvh: dict, leaf_size: int = 4) -> np.ndarray:
-    """8-wide nodes collapsed from an oracle LBVH (build_lbvh's dict), breadth-first numbering."""
+def collapse_wide(bvh: dict, leaf_size: int = 4, width: int = 8) -> np.ndarray:
+    """W-wide nodes (W = 8 or 4) collapsed from an oracle LBVH (build_lbvh's dict), breadth-first numbering."""
     n = bvh["leaf_ids"].size
     cap = max(n, 1)
     out = np.zeros(cap, WIDE_NODE_DTYPE)
     meta = np.ascontiguousarray(bvh["node_meta"], np.uint32)
     nb = f32(bvh["node_bounds"])
     counts = np.ascontiguousarray(bvh["leaf_counts"], np.uint32)
-    k = lib().orc_collapse_wide(
+    k = lib().orc_collapse_wide_n(
         _p(meta) if meta.size else None, _p(nb) if nb.size else None, _p(counts) if counts.size else None, n,
-        int(bvh["root"]) if n > 1 else 0, _p(f32(bvh["scene_bounds"])), leaf_size,
+        int(bvh["root"]) if n > 1 else 0, _p(f32(bvh["scene_bounds"])), leaf_size, width,
         out.ctypes.data_as(ctypes.c_void_p), cap,
     )
     if k == 0:

@@ -68,6 +68,9 @@ void orc_quantize(float o, uint32_t e, float lo, float hi, uint8_t *qlo, uint8_t
 uint64_t orc_collapse_wide(const uint32_t *node_meta, const float *node_bounds, const uint32_t *leaf_counts,
                            uint64_t n, uint32_t root, const float *scene_bounds, uint32_t leaf_size, void *out_nodes,
                            uint64_t capacity);
+uint64_t orc_collapse_wide_n(const uint32_t *node_meta, const float *node_bounds, const uint32_t *leaf_counts,
+                             uint64_t n, uint32_t root, const float *scene_bounds, uint32_t leaf_size, uint32_t width,
+                             void *out_nodes, uint64_t capacity);
 uint64_t orc_trace_wide(const float *vertices, uint64_t n, const uint32_t *leaf_ids, const void *wide,
                         uint64_t wide_count, const float *rays, uint64_t R, int mode, float *hits, uint32_t *stats);
 

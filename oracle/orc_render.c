@@ -830,10 +830,11 @@ static int build_sphere_bvh(const float *sph, uint64_t ns, const orc_render_sett
     memset(b, 0, sizeof(*b));
     b->leaf_ids = t->ids; b->node_meta = t->meta; b->node_bounds = t->nb; b->root = root;
     b->prim_count = ns; b->spheres = t->abs4;
-    if (st->node_width == 8) {
+    if (st->node_width >= 4) {
         t->wide = malloc(ns * 96);
-        uint64_t wc = orc_collapse_wide(t->meta, t->nb, t->counts, ns, root, scene12,
-                                        st->leaf_size ? (uint32_t)st->leaf_size : 4u, t->wide, ns);
+        uint64_t wc = orc_collapse_wide_n(t->meta, t->nb, t->counts, ns, root, scene12,
+                                          st->leaf_size ? (uint32_t)st->leaf_size : 4u, (uint32_t)st->node_width,
+                                          t->wide, ns);
         if (wc == 0) return 0;
         b->wide = t->wide;
         b->wide_count = wc;
@@ -893,10 +894,11 @@ int orc_render_mixed(const float *vertices, uint64_t n, const uint32_t *surfaces
     sc.bvh.vertices = vertices; sc.bvh.leaf_ids = ids; sc.bvh.node_meta = meta; sc.bvh.node_bounds = nb; sc.bvh.root = root;
     sc.bvh.prim_count = n;
     void *wide = NULL;
-    if (st->node_width == 8 && n > 0) {
+    if (st->node_width >= 4 && n > 0) {
         wide = malloc(n * 96);
-        uint64_t wc = orc_collapse_wide(meta, nb, counts, n, root, scene12,
-                                        st->leaf_size ? (uint32_t)st->leaf_size : 4u, wide, n);
+        uint64_t wc = orc_collapse_wide_n(meta, nb, counts, n, root, scene12,
+                                          st->leaf_size ? (uint32_t)st->leaf_size : 4u, (uint32_t)st->node_width, wide,
+                                          n);
         if (wc == 0) { free(wide); free(keys); free(ids); free(meta); free(parent); free(counts); free(nb); return -1; }
         sc.bvh.wide = wide;
         sc.bvh.wide_count = wc;

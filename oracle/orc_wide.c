@@ -102,8 +102,18 @@ static void empty_slots(orc_wide_node *w, int from) {
 uint64_t orc_collapse_wide(const uint32_t *node_meta, const float *node_bounds, const uint32_t *leaf_counts,
                            uint64_t n, uint32_t root, const float *scene_bounds, uint32_t leaf_size, void *out_nodes,
                            uint64_t capacity) {
+    return orc_collapse_wide_n(node_meta, node_bounds, leaf_counts, n, root, scene_bounds, leaf_size, 8, out_nodes,
+                               capacity);
+}
+
+/* The collapse at width W (4 or 8): nodes open children until they hold W
+ * (bajo's W parameter, bvh/gpu/builder/wide_collapse.mojo:148-643), in the
+ * same node record with W live slots. */
+uint64_t orc_collapse_wide_n(const uint32_t *node_meta, const float *node_bounds, const uint32_t *leaf_counts,
+                             uint64_t n, uint32_t root, const float *scene_bounds, uint32_t leaf_size, uint32_t width,
+                             void *out_nodes, uint64_t capacity) {
     orc_wide_node *out = (orc_wide_node *)out_nodes;
-    if (n == 0 || capacity == 0 || leaf_size < 1 || leaf_size > 8) return 0;
+    if (n == 0 || capacity == 0 || leaf_size < 1 || leaf_size > 8 || (width != 4 && width != 8)) return 0;
     if (n == 1) {
         orc_wide_node w;
         w.ox = scene_bounds[0]; w.oy = scene_bounds[1]; w.oz = scene_bounds[2];
@@ -127,7 +137,7 @@ uint64_t orc_collapse_wide(const uint32_t *node_meta, const float *node_bounds, 
         int nc = 2;
         load_children(node_meta, node_bounds, leaf_counts, src[w], &ch[0], &ch[1]);
         for (int phase = 0; phase < 2; ++phase) {
-            while (nc < 8) {
+            while (nc < (int)width) {
                 int pick = -1;
                 float best = -1.0f;
                 for (int i = 0; i < nc; ++i) {

@@ -143,7 +143,7 @@ def test_wide_rejects_bad_options(gpu_ctx):
     from bajo_amd.bvh import build_triangle_bvh
 
     dv = to_dev(soup(10, 1))
-    for kw in ({"node_width": 4}, {"node_width": 8, "leaf_size": 9}, {"node_width": 8, "leaf_size": -1}):
+    for kw in ({"node_width": 3}, {"node_width": 16}, {"node_width": 8, "leaf_size": 9}, {"node_width": 8, "leaf_size": -1}):
         with pytest.raises(BajoError):
             build_triangle_bvh(dv, **kw)
     binary = build_triangle_bvh(dv)
@@ -201,6 +201,54 @@ def test_wide_render_icosphere_matches_oracle(gpu_ctx):
     exact = float(np.mean(got.view(np.uint32) == want.view(np.uint32)))
     rmse = float(np.sqrt(np.mean((got.astype(np.float64) - want) ** 2)))
     print(f"icosphere wide NEE: rmse {rmse:.3e}, identical {exact:.4f}")
+    assert rmse <= 1e-3 and exact > 0.99
+    assert res.stats["extension_rays"] == rays[0] and res.stats["shadow_rays"] == rays[1]
+    g.free()
+
+
+@pytest.mark.parametrize("n,leaf", [(1, 4), (2, 4), (3, 1), (100, 4), (4096, 4), (50_000, 4), (50_000, 1)])
+def test_wide4_structure_and_hits_equal_oracle(gpu_ctx, n, leaf):
+    """Width 4 (bajo's render_gpu default W): the device collapse equals the oracle's 4-wide collapse in
+    canonical order, no node holds more than 4 children, and closest / any hits equal the oracle's."""
+    from bajo_amd.bvh import TRACE, build_triangle_bvh
+
+    announce(f"4-wide build n={n} leaf={leaf}")
+    v = soup(n, seed=7 * n + leaf)
+    bvh = build_triangle_bvh(to_dev(v), node_width=4, leaf_size=leaf)
+    lb = oracle.build_lbvh(v)
+    want = oracle.collapse_wide(lb, leaf, width=4)
+    got = bvh.wide()
+    assert bvh.info().node_width == 4 and len(got) == len(want)
+    assert got["count"].max() <= 4
+    validate_wide(got, v, lb["leaf_ids"], leaf)
+    assert oracle.canonical_wide(got).tobytes() == oracle.canonical_wide(want).tobytes()
+    R = 20_000
+    rays = random_rays(R, -0.1, 1.1, n + 3)
+    hits = torch.empty(8 * R, dtype=torch.float32, device="cuda:0")
+    for mode in (TRACE.CLOSEST_HIT, TRACE.ANY_HIT):
+        bvh.launch_rays(to_dev(rays), hits, R, mode)
+        w, _, over = oracle.trace_wide(v, lb["leaf_ids"], got, rays, mode=mode)
+        assert over == 0
+        np.testing.assert_array_equal(to_host(hits, np.uint32).reshape(-1, 8), w.view(np.uint32))
+    bvh.free()
+
+
+@pytest.mark.parametrize("name", ["NEE", "MIS"])
+def test_wide4_render_matches_oracle(gpu_ctx, name):
+    from bajo_amd import scenes
+    from bajo_amd.bvh import Camera
+    from bajo_amd.rt import GpuScene, RenderSettings, enqueue_render_gpu
+
+    announce(f"4-wide render {name}")
+    sc = scenes.material_box_scene()
+    cam = Camera((0.0, 1.0, -3.0), (0.0, 0.9, 0.0), (0.0, 1.0, 0.0), 0.5).flatten()
+    g = GpuScene(sc, gpu_ctx, node_width=4)
+    res = enqueue_render_gpu(RenderSettings(40, 30, 8, 2027, 6), cam, g, ALGS[name], with_stats=True)
+    got = res.numpy().reshape(-1)
+    want, rays = oracle.render(sc.vertices, sc.surfaces, sc.lambertian, sc.emissive, 40, 30, 8, 6, 2027, ALGS[name],
+                               cam, 8, node_width=4, leaf_size=4, metal=sc.metal, dielectric=sc.dielectric)
+    rmse = float(np.sqrt(np.mean((got.astype(np.float64) - want) ** 2)))
+    exact = float(np.mean(got.view(np.uint32) == want.view(np.uint32)))
     assert rmse <= 1e-3 and exact > 0.99
     assert res.stats["extension_rays"] == rays[0] and res.stats["shadow_rays"] == rays[1]
     g.free()

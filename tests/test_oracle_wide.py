@@ -186,3 +186,22 @@ def test_oracle_render_wide_matches_binary():
     same = np.mean(a.view(np.uint32) == b.view(np.uint32))
     assert same > 0.97, same
     assert np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)) < 0.05
+
+
+@pytest.mark.parametrize("n,leaf,seed", [(2, 4, 1), (64, 4, 2), (2000, 1, 3), (2000, 4, 4), (5000, 8, 5)])
+def test_collapse_width4(n, leaf, seed):
+    """Width 4: every node holds at most 4 children and opens its largest children first (nodes are fuller
+    than binary), and traversal of the 4-wide hierarchy returns the binary hierarchy's hits."""
+    v = soup(n, seed=seed)
+    lb = oracle.build_lbvh(v)
+    w4 = oracle.collapse_wide(lb, leaf, width=4)
+    info = validate_wide(w4, v, lb["leaf_ids"], leaf)
+    assert w4["count"].max() <= 4
+    if n >= 64:
+        fan_out = (info["leaf_children"] + info["inner_children"]) / info["nodes"]
+        assert fan_out > (2.5 if leaf == 1 else 3.0), fan_out
+    rays = random_rays(2000, -0.1, 1.1, seed + 11)
+    got, _, over = oracle.trace_wide(v, lb["leaf_ids"], w4, rays)
+    assert over == 0
+    want = oracle.trace_bvh2(v, lb, rays)
+    compare_hits(got.reshape(-1), want)

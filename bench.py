@@ -39,7 +39,7 @@ N_SPHERE = 4_194_304
 WIDTH, HEIGHT, SPP, DEPTH, SEED = 1920, 1080, 8, 5, 2026
 FOV_SCALE = 0.41421356  # tan(22.5 deg)
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = {2: 64, 8: 96}  # binary node (both child boxes + children); 8-wide quantized node
+NODE_BYTES = {2: 64, 4: 96, 8: 96}  # binary node (both child boxes + children); 4/8-wide quantized node
 TRI_BYTES = 48  # v0, e1, e2, prim
 PATH_BYTES = 52  # 12 float planes + id
 SHADOW_BYTES = 48  # 11 float planes + id
@@ -118,6 +118,28 @@ def incoherent_trace(scene, ctx, node_width: int, leaf_size: int, method: int, r
         "hbm_bytes_measured": (hbm := measured_traffic("trace_refill_kernel HBM bytes per 16.7M-ray launch")),
         "hbm_gbs": hbm / (ms / 1e3) / 1e9 if hbm and rays == 1 << 24 else None,
         "rays_def": "origins uniform in the scene box, directions uniform on the sphere, closest hit",
+    }
+
+
+def width4_workload(scene, ctx, cam) -> dict:
+    """The same scene at node width 4 (bajo's render_gpu default): build, one NEE render of the benchmark
+    frame and the 16M random-ray launch."""
+    import torch
+
+    from bajo_amd.rt import RENDER, GpuScene, RenderSettings, enqueue_render_gpu
+
+    g = GpuScene(scene, ctx, node_width=4, leaf_size=4)
+    st = RenderSettings(WIDTH, HEIGHT, SPP, SEED, DEPTH)
+    enqueue_render_gpu(st, cam, g, RENDER.NEE)
+    r = enqueue_render_gpu(st, cam, g, RENDER.NEE, with_stats=True).stats
+    t = g.rebuild(timings=True)
+    g.free()
+    return {
+        "build_ms": t["total_ns"] / 1e6,
+        "build_mtris_s": scene.triangle_count / (t["total_ns"] / 1e9) / 1e6,
+        "nee_render_ms": r["total_ms"],
+        "nee_mrays_s": (r["extension_rays"] + r["shadow_rays"]) / (r["total_ms"] / 1e3) / 1e6,
+        "random_rays": incoherent_trace(scene, ctx, 4, 4, 0),
     }
 
 
@@ -1027,6 +1049,7 @@ def main() -> int:
         line["guided_equal_spp_mse_ratio"] = gw["equal_spp_mse_ratio_16"]
         line["guided"] = gw
     line["grid_65k"] = grid_workload(ctx)
+    line["width4"] = width4_workload(scene, ctx, cam)
     if not args.no_extra_workloads:
         line["soup_camera_65k"] = soup_camera_workload(ctx, min(16, os.cpu_count() or 1), not args.no_cpu_baseline)
         line["soup_grid_16m"] = soup_grid_workload(ctx)

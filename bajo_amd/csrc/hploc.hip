@@ -19,9 +19,10 @@
 //     writeback or invalidate is needed across the 8 XCDs;
 //   - a guide node that needs merging is processed by its whole wave: up to
 //     32 clusters sit one per lane, neighbour boxes move by lane shuffles,
-//     mutual pairs are found with one ballot, node indices are reserved with
-//     one atomic per round, and the list is compacted with a forward lane
-//     permute (ds_permute);
+//     mutual pairs are found with one ballot, a new node takes the index
+//     (retired leaf position - 1) of the larger lowest leaf position of its
+//     two clusters (no shared counter), and the list is compacted with a
+//     forward lane permute (ds_permute);
 //   - a segmented build (a BLAS set) runs every segment's H-PLOC in the same
 //     launch: the guide is the per-segment Karras forest, a guide node whose
 //     parent is the sentinel is its segment's root (merged down to one

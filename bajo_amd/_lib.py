@@ -21,6 +21,9 @@ BAJO_WIDTH_BINARY = 2
 BAJO_WIDTH_WIDE8 = 8
 BAJO_STATUS_SORT_LOOKBACK = 4
 BAJO_STATUS_REFIT_CORRUPT = 5
+BAJO_STATUS_HPLOC_FAILED = 6
+BAJO_STATUS_COLLAPSE_FAILED = 7
+BAJO_CAPTURE_REBUILD = 1
 
 
 class BajoError(RuntimeError):
@@ -287,6 +290,9 @@ SIGNATURES = {
     "bajo_render_target_set_camera": (c_int32, [c_void_p, c_void_p, POINTER(c_float)]),
     "bajo_render_enqueue": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "bajo_render_target_capture": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "bajo_render_target_capture_ex": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    "bajo_scene_vertices": (c_int32, [c_void_p, POINTER(c_void_p), POINTER(c_uint64)]),
+    "bajo_scene_set_vertices": (c_int32, [c_void_p, c_void_p, c_void_p, c_uint64]),
     "bajo_render_target_replay": (c_int32, [c_void_p, c_void_p]),
     "bajo_render_target_free": (c_int32, [c_void_p]),
     "bajo_rng_stage_floats": (c_int32, [c_uint64, c_uint32, c_uint32, c_uint32, c_void_p]),

@@ -67,6 +67,11 @@ struct bajo_bvh {
   uint64_t wide_capacity;
   uint32_t wide_count;
   uint32_t wide_leaf;  // largest leaf child, 1..kWideMaxLeaf
+  // Builds enqueue without reading anything back: the wide node count stays
+  // in wide_ctl and H-PLOC's root in hp_ctl until a host query reads them.
+  bool wide_pending;    // wide_count not read back yet (traversals bound by wide_capacity)
+  bool root_on_device;  // H-PLOC: traversals read the root from hp_ctl[kHpCtlRoot]
+  bool root_pending;    // root not read back yet
   // H-PLOC (method BAJO_BUILD_HPLOC): guide LBVH and cluster workspace
   int method;
   BjNode* hp_guide;       // n-1 guide nodes (Karras over the sorted keys)
@@ -86,6 +91,9 @@ struct bajo_bvh {
   uint32_t seg_count;
   uint64_t seg_capacity;
   uint64_t* seg_off;      // device u64[S + 1]
+  uint64_t* seg_off_host; // pinned u64[S + 1]: the offsets of the last upload (its staging buffer)
+  uint32_t seg_uploaded;  // segment count of that upload (0 = none)
+  hipEvent_t seg_upload;  // recorded after that upload
   uint32_t* seg_acc;      // 12 S ordered-int bounds accumulators
   float* seg_scene;       // 12 S: root box and centroid box of each segment
   uint32_t* seg_prim;     // n: input index of each sorted leaf
@@ -101,8 +109,16 @@ struct bajo_bvh {
 };
 
 namespace bj {
-// Collapses h's binary hierarchy into h->wide (8-wide quantized nodes); wide.hip.
-int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h);
+// hp_ctl word holding H-PLOC's root reference.
+constexpr int kHpCtlRoot = 2;
+// Collapses h's binary hierarchy into h->wide (W-wide quantized nodes);
+// `largest_tree` bounds the leaves of one tree (a segment of a set); wide.hip.
+int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h, uint64_t largest_tree);
+// The node count in a host copy of wide_ctl, clipped to the capacity.
+uint32_t collapse_read_count(const uint32_t* host_ctl, uint64_t capacity);
+// Reads back what a build left on the device (wide node count, H-PLOC root)
+// of h or its owning set; synchronizes the device if anything is pending.
+int32_t resolve_build(const bajo_bvh* h);
 // Builds h's binary hierarchy with H-PLOC over the sorted keys; hploc.hip.
 int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* src);
 int32_t build_hploc_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices);

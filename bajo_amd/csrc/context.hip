@@ -59,6 +59,8 @@ const char* bajo_status_string(int32_t status) {
     case BAJO_STATUS_BAD_NODE: return "traversal met a node outside the hierarchy";
     case BAJO_STATUS_SORT_LOOKBACK: return "sort look-back gave up waiting (corrupted look-back table)";
     case BAJO_STATUS_REFIT_CORRUPT: return "refit met an inconsistent hierarchy";
+    case BAJO_STATUS_HPLOC_FAILED: return "H-PLOC clustering failed (no progress, corrupt cluster list or no single root)";
+    case BAJO_STATUS_COLLAPSE_FAILED: return "wide collapse failed (out of nodes, bad root or too deep)";
     case BAJO_STATUS_PATH_OVERFLOW: return "path queue overflow";
     case BAJO_STATUS_SHADOW_OVERFLOW: return "shadow queue overflow";
     default: return "unknown status";

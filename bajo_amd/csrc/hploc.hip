@@ -46,7 +46,7 @@ constexpr int kHpRadius = 8;
 constexpr uint32_t kHpThreshold = 16;
 constexpr int kHpMaxLoop = 512;  // climb iterations per wave (depth is far smaller)
 // ctl words
-constexpr int kHpLeaves = 0, kHpStatus = 1, kHpRoot = 2;
+constexpr int kHpLeaves = 0, kHpStatus = 1, kHpRoot = kHpCtlRoot;
 constexpr uint32_t kHpStatusNoProgress = 1, kHpStatusBadList = 2;
 
 // A cluster: subtree reference (bit 31 = leaf at a sorted position), its
@@ -392,13 +392,24 @@ __global__ void hploc_init_kernel(uint32_t* __restrict__ ctl) {
 // The root gets no parent from a merge; its leaf count goes to ctl word
 // kHpLeaves for the host's completeness check.
 __global__ void hploc_root_kernel(BjNode* __restrict__ nodes, uint64_t n, const uint32_t* __restrict__ leaf_counts,
-                                  uint32_t* __restrict__ ctl) {
+                                  uint32_t* __restrict__ ctl, uint32_t* __restrict__ status) {
   if (threadIdx.x != 0) return;
   const uint32_t root = ctl[kHpRoot];
-  if (!(root & kLeafFlag) && (uint64_t)root < n - 1) {
+  const bool valid = !(root & kLeafFlag) && (uint64_t)root < n - 1;
+  if (valid) {
     nodes[root].parent = kSentinel;
     ctl[kHpLeaves] = leaf_counts[root];
   }
+  // the build's own checks, raised into the context status word
+  if (ctl[kHpStatus] != 0 || !valid || (uint64_t)ctl[kHpLeaves] != n) {
+    if (!valid) ctl[kHpRoot] = kSentinel;  // traversals then see an empty hierarchy
+    atomicMax(status, (uint32_t)BAJO_STATUS_HPLOC_FAILED);
+  }
+}
+
+// Raises a segmented build's clustering status into the context status word.
+__global__ void hploc_status_kernel(const uint32_t* __restrict__ ctl, uint32_t* __restrict__ status) {
+  if (threadIdx.x == 0 && ctl[kHpStatus] != 0) atomicMax(status, (uint32_t)BAJO_STATUS_HPLOC_FAILED);
 }
 
 __global__ void seg_roots_init_kernel(uint32_t* __restrict__ seg_roots, uint32_t S) {
@@ -559,7 +570,7 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
                                                                   h->leaf_parent, h->flags, clusters, h->hp_pos,
                                                                   h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl,
                                                                   nullptr, nullptr, 0u, nullptr);
-  hploc_root_kernel<<<1, 64, 0, st>>>(h->nodes, n, h->leaf_counts, h->hp_ctl);
+  hploc_root_kernel<<<1, 64, 0, st>>>(h->nodes, n, h->leaf_counts, h->hp_ctl, ctx->status);
   BJ_HIP(hipGetLastError());
   // the cluster area is free now: it holds the reordered keys and leaf parents
   uint64_t* keys_dfs = reinterpret_cast<uint64_t*>(h->hp_clusters);
@@ -576,15 +587,12 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
   BJ_HIP(hipGetLastError());
   BJ_HIP(hipMemcpyAsync(h->keys, keys_dfs, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
   BJ_HIP(hipMemcpyAsync(h->leaf_parent, parent_dfs, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-  BJ_HIP(hipMemcpyAsync(h->host_hp_ctl, h->hp_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  BJ_HIP(hipStreamSynchronize(st));
-  const uint32_t* c = h->host_hp_ctl;
-  BJ_REQUIRE(c[kHpStatus] == 0, "H-PLOC build failed with status %u (1 = no progress, 2 = corrupt cluster list)",
-             c[kHpStatus]);
-  BJ_REQUIRE((uint64_t)c[kHpLeaves] == n, "H-PLOC root holds %u of %llu leaves", c[kHpLeaves],
-             (unsigned long long)n);
-  BJ_REQUIRE(!(c[kHpRoot] & kLeafFlag) && (uint64_t)c[kHpRoot] < n - 1, "H-PLOC root %u is invalid", c[kHpRoot]);
-  h->root = c[kHpRoot];
+  // no read-back: traversals and the collapse take the root from hp_ctl, a
+  // host query reads it (bajo_bvh_get_info), and a failed build has raised
+  // BAJO_STATUS_HPLOC_FAILED for the next sync point
+  h->root = kSentinel;
+  h->root_on_device = true;
+  h->root_pending = true;
   return BAJO_OK;
 }
 
@@ -618,11 +626,8 @@ int32_t build_hploc_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices)
   BJ_HIP(hipGetLastError());
   BJ_HIP(hipMemcpyAsync(h->keys, keys_dfs, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
   BJ_HIP(hipMemcpyAsync(h->leaf_parent, parent_dfs, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-  BJ_HIP(hipMemcpyAsync(h->host_hp_ctl, h->hp_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  BJ_HIP(hipStreamSynchronize(st));
-  BJ_REQUIRE(h->host_hp_ctl[kHpStatus] == 0,
-             "segmented H-PLOC build failed with status %u (1 = no progress, 2 = corrupt cluster list or root)",
-             h->host_hp_ctl[kHpStatus]);
+  hploc_status_kernel<<<1, 64, 0, st>>>(h->hp_ctl, ctx->status);
+  BJ_HIP(hipGetLastError());
   return BAJO_OK;
 }
 

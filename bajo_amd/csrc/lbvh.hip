@@ -594,7 +594,20 @@ int32_t ensure_hploc_capacity(bajo_bvh* h, uint64_t n) {
   return BAJO_OK;
 }
 
+// The staging buffer may still be read by an enqueued upload: wait for it.
+void free_segment_staging(bajo_bvh* h) {
+  if (h->seg_upload) {
+    (void)hipEventSynchronize(h->seg_upload);
+    (void)hipEventDestroy(h->seg_upload);
+  }
+  (void)hipHostFree(h->seg_off_host);
+  h->seg_upload = nullptr;
+  h->seg_off_host = nullptr;
+  h->seg_uploaded = 0;
+}
+
 void free_segments(bajo_bvh* h) {
+  free_segment_staging(h);
   (void)hipFree(h->seg_off); (void)hipFree(h->seg_acc); (void)hipFree(h->seg_scene); (void)hipFree(h->seg_prim);
   (void)hipFree(h->seg_sort_ws);
   h->seg_off = nullptr; h->seg_acc = nullptr; h->seg_scene = nullptr; h->seg_prim = nullptr; h->seg_sort_ws = nullptr;
@@ -607,9 +620,12 @@ void free_segments(bajo_bvh* h) {
 // segment; the input index per sorted leaf; the pair-sort workspace.
 int32_t ensure_segment_capacity(bajo_bvh* h, uint64_t n, uint32_t S, size_t sort_bytes) {
   if (S > h->seg_capacity) {
+    free_segment_staging(h);
     (void)hipFree(h->seg_off); (void)hipFree(h->seg_acc); (void)hipFree(h->seg_scene);
     h->seg_off = nullptr; h->seg_acc = nullptr; h->seg_scene = nullptr;
     h->seg_capacity = 0;
+    BJ_HIP(hipHostMalloc(&h->seg_off_host, ((uint64_t)S + 1) * sizeof(uint64_t), hipHostMallocDefault));
+    BJ_HIP(hipEventCreateWithFlags(&h->seg_upload, hipEventDisableTiming));
     BJ_HIP(hipMalloc(&h->seg_off, ((uint64_t)S + 1) * sizeof(uint64_t)));
     BJ_HIP(hipMalloc(&h->seg_acc, 12ull * S * sizeof(uint32_t)));
     BJ_HIP(hipMalloc(&h->seg_scene, 12ull * S * sizeof(float)));
@@ -674,6 +690,8 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
   }
   h->n = n;
   h->root = n == 1 ? (kLeafFlag | 0u) : 0u;
+  h->root_on_device = false;
+  h->root_pending = false;
   h->trav_nodes = (uint32_t)(n - 1);
   h->trav_prims = (uint32_t)n;
   h->wide_root = 0;
@@ -733,10 +751,11 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
   }
   ev.record(5, st);
   if (wide_layout(h->node_width)) {
-    rc = collapse_wide(ctx, h);
+    rc = collapse_wide(ctx, h, n);
     if (rc) return rc;
   } else {
     h->wide_count = 0;
+    h->wide_pending = false;
   }
   ev.record(6, st);
   if (timings) {
@@ -756,10 +775,49 @@ int32_t build_into(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, uint64_t n
   return BAJO_OK;
 }
 
+// Puts a set's segment offsets on the device without a host wait: offsets
+// equal to the last upload's are already there; new ones go through the
+// pinned staging buffer, after any earlier upload from it has run.  While the
+// stream is being captured into a graph, the offsets must be unchanged.
+int32_t upload_segment_offsets(hipStream_t st, bajo_bvh* h, const uint64_t* offsets, uint32_t S) {
+  const size_t bytes = ((size_t)S + 1) * sizeof(uint64_t);
+  if (h->seg_uploaded == S && memcmp(h->seg_off_host, offsets, bytes) == 0) return BAJO_OK;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  BJ_HIP(hipStreamIsCapturing(st, &cap));
+  BJ_REQUIRE(cap == hipStreamCaptureStatusNone,
+             "set build under graph capture: the segment offsets differ from the last build's");
+  if (h->seg_uploaded) BJ_HIP(hipEventSynchronize(h->seg_upload));
+  memcpy(h->seg_off_host, offsets, bytes);
+  BJ_HIP(hipMemcpyAsync(h->seg_off, h->seg_off_host, bytes, hipMemcpyHostToDevice, st));
+  BJ_HIP(hipEventRecord(h->seg_upload, st));
+  h->seg_uploaded = S;
+  return BAJO_OK;
+}
+
 }  // namespace
 
 int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo_build_options* options,
                     bajo_build_timings* timings, bajo_bvh** out, bool boxes);
+
+int32_t resolve_build(const bajo_bvh* view) {
+  bajo_bvh* h = const_cast<bajo_bvh*>(view->owner ? view->owner : view);
+  if (!h->wide_pending && !h->root_pending) return BAJO_OK;
+  int prev = 0;
+  BJ_HIP(hipGetDevice(&prev));
+  hipError_t e = hipSetDevice(h->device);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess && h->wide_pending)
+    e = hipMemcpy(h->host_ctl, h->wide_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && h->root_pending)
+    e = hipMemcpy(h->host_hp_ctl, h->hp_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess) return hip_fail(e, "reading back the build's sizes", __FILE__, __LINE__);
+  if (h->wide_pending) h->wide_count = collapse_read_count(h->host_ctl, h->wide_capacity);
+  if (h->root_pending) h->root = h->host_hp_ctl[kHpCtlRoot];
+  h->wide_pending = false;
+  h->root_pending = false;
+  return BAJO_OK;
+}
 
 // bajo's segmented BLAS build (bvh/gpu/triangle_bvh.mojo:366-491 over
 // builder/segmented_build.mojo:89-162 and builder/lbvh.mojo:30-62, 235-329):
@@ -785,9 +843,14 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
   rc = ensure_segment_capacity(h, n, S, plan.total);
   if (rc) return rc;
   hipStream_t st = ctx->stream;
-  BJ_HIP(hipMemcpyAsync(h->seg_off, offsets, ((uint64_t)S + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
-  BJ_HIP(hipStreamSynchronize(st));  // the caller's offsets may be released on return
+  rc = upload_segment_offsets(st, h, offsets, S);
+  if (rc) return rc;
+  uint64_t largest = 0;
+  for (uint32_t s = 0; s < S; ++s)
+    if (offsets[s + 1] - offsets[s] > largest) largest = offsets[s + 1] - offsets[s];
   h->n = n;
+  h->root_on_device = false;
+  h->root_pending = false;
   h->seg_count = S;
   h->root = kSentinel;  // the store itself is traced through its segment views
   h->trav_nodes = (uint32_t)(n - 1);
@@ -838,10 +901,11 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
   }
   ev.record(5, st);
   if (wide_layout(h->node_width)) {
-    rc = collapse_wide(ctx, h);
+    rc = collapse_wide(ctx, h, largest);
     if (rc) return rc;
   } else {
     h->wide_count = 0;
+    h->wide_pending = false;
   }
   ev.record(6, st);
   if (timings) {
@@ -979,6 +1043,9 @@ uint32_t rebase_ref(uint32_t r, uint64_t node_base, uint64_t leaf_base) {
 
 int32_t bajo_bvh_get_info(const bajo_bvh* h, bajo_bvh_info* out) {
   BJ_REQUIRE(h != nullptr && out != nullptr, "bajo_bvh_get_info: NULL argument");
+  const int32_t rc = bj::resolve_build(h);
+  if (rc) return rc;
+  const bajo_bvh* store = h->owner ? h->owner : h;
   out->prim_count = h->n;
   out->node_count = h->n > 0 ? h->n - 1 : 0;
   out->root = h->owner ? rebase_ref(h->root, h->node_base, h->leaf_base) : h->root;
@@ -989,7 +1056,7 @@ int32_t bajo_bvh_get_info(const bajo_bvh* h, bajo_bvh_info* out) {
   out->leaf_parent = h->leaf_parent ? h->leaf_parent + h->leaf_base : nullptr;
   out->leaf_counts = h->leaf_counts ? h->leaf_counts + h->node_base : nullptr;
   out->scene_bounds = h->scene;
-  out->wide_node_count = wide_layout(h->node_width) ? h->wide_count : 0;
+  out->wide_node_count = wide_layout(h->node_width) ? store->wide_count : 0;
   out->wide_nodes = wide_layout(h->node_width) ? h->wide : nullptr;
   return BAJO_OK;
 }
@@ -999,6 +1066,9 @@ int32_t bajo_bvh_copy_wide(bajo_ctx* ctx, const bajo_bvh* h, void* nodes, uint64
   BJ_REQUIRE(wide_layout(h->node_width), "bajo_bvh_copy_wide: the BVH was built with node width %d",
              h->node_width);
   BJ_REQUIRE(h->seg_count == 0, "bajo_bvh_copy_wide: copy a BLAS set's segments through their views");
+  BJ_HIP(hipStreamSynchronize(ctx->stream));
+  const int32_t rc = bj::resolve_build(h);
+  if (rc) return rc;
   if (!h->owner) {
     if (count) *count = h->wide_count;
     if (!nodes) return BAJO_OK;
@@ -1011,8 +1081,7 @@ int32_t bajo_bvh_copy_wide(bajo_ctx* ctx, const bajo_bvh* h, void* nodes, uint64
   // a segment view: the nodes reachable from its root, depth-first (children
   // in slot order), with inner references renumbered and leaf triangle
   // ranges relative to the segment's first sorted leaf
-  std::vector<BjWide> all(h->wide_count);
-  BJ_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<BjWide> all(h->owner->wide_count);
   BJ_HIP(hipMemcpy(all.data(), h->wide, all.size() * sizeof(BjWide), hipMemcpyDeviceToHost));
   std::vector<uint32_t> order, stack{h->wide_root}, pos(all.size(), bj::kSentinel);
   while (!stack.empty()) {

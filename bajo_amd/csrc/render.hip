@@ -1863,6 +1863,21 @@ int32_t bajo_rng_stage_floats(uint64_t seed, uint32_t path_id, uint32_t stage, u
   return BAJO_OK;
 }
 
+int32_t bajo_scene_vertices(const bajo_scene* s, float** vertices, uint64_t* count) {
+  BJ_REQUIRE(s && vertices && count, "bajo_scene_vertices: NULL argument");
+  *vertices = s->vertices;
+  *count = s->n;
+  return BAJO_OK;
+}
+
+int32_t bajo_scene_set_vertices(bajo_ctx* ctx, bajo_scene* s, const float* vertices, uint64_t n) {
+  BJ_REQUIRE(ctx && s && vertices, "bajo_scene_set_vertices: NULL argument");
+  BJ_REQUIRE(n == s->n, "bajo_scene_set_vertices: %llu triangles, the scene holds %llu", (unsigned long long)n,
+             (unsigned long long)s->n);
+  BJ_HIP(hipMemcpyAsync(s->vertices, vertices, n * 9 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+  return BAJO_OK;
+}
+
 int32_t bajo_scene_free(bajo_scene* s) {
   free_scene(s);
   return BAJO_OK;
@@ -2143,15 +2158,27 @@ int32_t bajo_render_enqueue(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s)
 }
 
 int32_t bajo_render_target_capture(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s) {
+  return bajo_render_target_capture_ex(ctx, t, s, 0);
+}
+
+int32_t bajo_render_target_capture_ex(bajo_ctx* ctx, bajo_render_target* t, bajo_scene* s, uint32_t flags) {
   BJ_REQUIRE(ctx && t && s, "bajo_render_target_capture: NULL argument");
   BJ_REQUIRE(t->device == ctx->device && s->device == ctx->device, "bajo_render_target_capture: device mismatch");
   BJ_REQUIRE((s->env_count ? s->env_count : 1u) == t->environments,
              "bajo_render_target_capture: the scene has %u environments, the target's pixels hold %u",
              s->env_count ? s->env_count : 1u, t->environments);
+  BJ_REQUIRE((flags & ~(uint32_t)BAJO_CAPTURE_REBUILD) == 0, "bajo_render_target_capture: unknown flags %#x", flags);
+  const bool rebuild = (flags & BAJO_CAPTURE_REBUILD) != 0;
   drop_graph(t);
-  // every allocation happens before the capture
+  // every allocation happens before the capture: the arena, and with a
+  // rebuild one eager rebuild (it sizes the build buffers and the context
+  // scratch, and uploads the segment offsets the captured build reuses)
   int32_t rc = prepare_arena(s, t->arena, &t->settings);
   if (rc) return rc;
+  if (rebuild) {
+    rc = bajo_scene_rebuild(ctx, s, nullptr);
+    if (rc) return rc;
+  }
   hipStream_t cs = nullptr;
   BJ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   bajo_ctx cap = *ctx;
@@ -2161,7 +2188,8 @@ int32_t bajo_render_target_capture(bajo_ctx* ctx, bajo_render_target* t, bajo_sc
     (void)hipStreamDestroy(cs);
     return hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
   }
-  rc = dispatch_render(&cap, s, t->arena, &t->settings, nullptr, 0, t->pixels, nullptr);
+  rc = rebuild ? bajo_scene_rebuild(&cap, s, nullptr) : BAJO_OK;
+  if (!rc) rc = dispatch_render(&cap, s, t->arena, &t->settings, nullptr, 0, t->pixels, nullptr);
   hipGraph_t g = nullptr;
   e = hipStreamEndCapture(cs, &g);
   (void)hipStreamDestroy(cs);

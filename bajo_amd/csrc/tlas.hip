@@ -357,7 +357,9 @@ int32_t bajo_tlas_create(bajo_ctx* ctx, const bajo_bvh* const* blases, uint32_t 
   TL_HIP(hipMemcpy(t->inst_blas, blas_indices, (size_t)instance_count * sizeof(uint32_t), hipMemcpyHostToDevice));
   TL_HIP(hipMemcpy(t->blas, views.data(), views.size() * sizeof(TlasBlas), hipMemcpyHostToDevice));
 #undef TL_HIP
-  const int32_t rc = bajo_build_binary_bounds(ctx, t->boxes, instance_count, &opt, nullptr, &t->top);
+  int32_t rc = bajo_build_binary_bounds(ctx, t->boxes, instance_count, &opt, nullptr, &t->top);
+  if (rc) return fail(rc);
+  rc = resolve_build(t->top);  // the top-level root is passed by value to traversals
   if (rc) return fail(rc);
   *out = t;
   return BAJO_OK;

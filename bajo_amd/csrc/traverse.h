@@ -269,6 +269,7 @@ struct BvhView {
   const BjTri* tris;
   uint32_t root, node_count, prim_count;
   uint32_t* status;
+  const uint32_t* root_dev;  // when set, the root is read here (H-PLOC leaves it on the device)
 };
 
 struct TraceCounters {
@@ -282,7 +283,8 @@ BJ_D Hit traverse(const BvhView& bv, const Ray& ray, Stack& st, TraceCounters* t
   uint32_t n_visit = 0, n_inner = 0, n_leaf = 0, max_sp = 0;
   const BjNode* __restrict__ nodes = bv.nodes;
   const BjTri* __restrict__ tris = bv.tris;
-  const uint32_t root = bv.root, node_count = bv.node_count, prim_count = bv.prim_count;
+  const uint32_t root = bv.root_dev ? *bv.root_dev : bv.root;
+  const uint32_t node_count = bv.node_count, prim_count = bv.prim_count;
   uint32_t* status = bv.status;
   Closest best = no_closest(ray.tmax);
   const v3 rcp = rcp_dir(ray.d);
@@ -641,8 +643,11 @@ struct AccelView {
 // Host: the view of a built hierarchy; `status` receives traversal errors.
 inline AccelView accel_view(const bajo_bvh* h, uint32_t* status) {
   AccelView av;
-  av.bin = BvhView{h->nodes, h->tris, h->root, h->trav_nodes, h->trav_prims, status};
-  av.wide = WideView{h->wide, h->tris, wide_layout(h->node_width) ? h->wide_count : 0u, h->trav_prims, status,
+  av.bin = BvhView{h->nodes, h->tris, h->root, h->trav_nodes, h->trav_prims, status,
+                   h->root_on_device ? h->hp_ctl + kHpCtlRoot : nullptr};
+  // a collapse whose node count has not been read back is bounded by its capacity
+  const uint32_t wide_nodes = h->wide_pending ? (uint32_t)h->wide_capacity : h->wide_count;
+  av.wide = WideView{h->wide, h->tris, wide_layout(h->node_width) ? wide_nodes : 0u, h->trav_prims, status,
                      h->wide_root};
   return av;
 }

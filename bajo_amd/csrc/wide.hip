@@ -12,21 +12,27 @@
 // Unlike bajo's persistent work queue with a spin fail-safe, the collapse here
 // is level-synchronous: level k processes the wide nodes allocated while
 // processing level k-1 (a contiguous index range), so no kernel ever waits on
-// another wave.  Levels are launched in batches; the host reads the level
-// size after each batch and stops at the first empty level.  Leaf children
-// reference ranges of the sorted triangle records that the refit already
-// wrote (an LBVH subtree covers a contiguous sorted range), so no triangle
-// data is copied.
+// another wave.  The host never reads the level sizes back: it enqueues a
+// depth-bounded number of full-grid level launches (each one's last block
+// opens the next level; a launch past the last level does nothing), then one
+// single-block kernel that finishes any deeper levels with block barriers.
+// The whole collapse is therefore enqueue-only and can be captured in a HIP
+// graph.  Leaf children reference ranges of the sorted triangle records that
+// the refit already wrote (an LBVH subtree covers a contiguous sorted range),
+// so no triangle data is copied.
 #include "bvh_internal.h"
 
 namespace bj {
 namespace {
 
 constexpr int kCollapseBlock = 256;
-constexpr int kLevelBatch = 6;
-// wide_ctl words
-constexpr int kCtlCount = 0, kCtlBegin = 1, kCtlEnd = 2, kCtlStatus = 3;
-constexpr uint32_t kStatusOutOfNodes = 1;
+constexpr int kTailBlock = 1024;
+// Levels the single-block finishing kernel takes at most (a tree deeper than
+// the full-grid launches plus this raises BAJO_STATUS_COLLAPSE_FAILED).
+constexpr uint32_t kTailLevelCap = 4096;
+// wide_ctl words: node counter, level range, status, finished-block ticket
+constexpr int kCtlCount = 0, kCtlBegin = 1, kCtlEnd = 2, kCtlStatus = 3, kCtlTicket = 4;
+constexpr uint32_t kStatusOutOfNodes = 1, kStatusTooDeep = 2, kStatusBadRoot = 4;
 
 struct Child {
   uint32_t ref, count;
@@ -84,11 +90,16 @@ BJ_HD void quantize(float o, uint32_t e, float lo, float hi, uint8_t& qlo, uint8
   qhi = (uint8_t)fh;
 }
 
+// `root_dev` (H-PLOC: the root its clustering wrote) overrides `root_binary`;
+// a root outside the hierarchy leaves the level range empty and raises
+// BAJO_STATUS_COLLAPSE_FAILED.
 __global__ void collapse_root_kernel(const float* __restrict__ scene, uint32_t n, uint32_t root_binary,
-                                     BjWide* __restrict__ wide, uint32_t* __restrict__ wide_src,
-                                     uint32_t* __restrict__ ctl) {
+                                     const uint32_t* __restrict__ root_dev, BjWide* __restrict__ wide,
+                                     uint32_t* __restrict__ wide_src, uint32_t* __restrict__ ctl,
+                                     uint32_t* __restrict__ status) {
   if (threadIdx.x != 0) return;
   ctl[kCtlStatus] = 0;
+  ctl[kCtlTicket] = 0;
   if (n == 1) {
     // single triangle: one wide node with one leaf child spanning the scene box
     BjWide w;
@@ -112,9 +123,16 @@ __global__ void collapse_root_kernel(const float* __restrict__ scene, uint32_t n
     ctl[kCtlEnd] = 0;
     return;
   }
-  wide_src[0] = root_binary;
+  const uint32_t root = root_dev ? *root_dev : root_binary;
   ctl[kCtlCount] = 1;
   ctl[kCtlBegin] = 0;
+  if ((root & kLeafFlag) || root >= n - 1) {
+    ctl[kCtlStatus] = kStatusBadRoot;
+    ctl[kCtlEnd] = 0;
+    atomicMax(status, (uint32_t)BAJO_STATUS_COLLAPSE_FAILED);
+    return;
+  }
+  wide_src[0] = root;
   ctl[kCtlEnd] = 1;
 }
 
@@ -128,6 +146,7 @@ __global__ void seg_collapse_root_kernel(const float* __restrict__ seg_scene, co
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid == 0) {
     ctl[kCtlStatus] = 0;
+    ctl[kCtlTicket] = 0;
     ctl[kCtlCount] = S;
     ctl[kCtlBegin] = 0;
     ctl[kCtlEnd] = S;
@@ -160,29 +179,29 @@ __global__ void seg_collapse_root_kernel(const float* __restrict__ seg_scene, co
 }
 
 // The next level is the nodes reserved by the last one, clipped to the
-// capacity; after a capacity failure the level range is left empty.
-__global__ void level_advance_kernel(uint32_t* __restrict__ ctl, uint32_t capacity) {
-  if (threadIdx.x != 0) return;
+// capacity; after a capacity failure the level range is left empty.  Run by
+// one thread once every wave of the level has reserved its nodes.
+BJ_D void level_advance(uint32_t* __restrict__ ctl, uint32_t capacity, uint32_t* __restrict__ status) {
   const uint32_t end = ctl[kCtlEnd];
-  uint32_t next = ctl[kCtlCount];
+  uint32_t next = __hip_atomic_load(ctl + kCtlCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (next > capacity) next = capacity;
-  if (ctl[kCtlStatus]) next = end;
+  if (__hip_atomic_load(ctl + kCtlStatus, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    next = end;
+    atomicMax(status, (uint32_t)BAJO_STATUS_COLLAPSE_FAILED);
+  }
   ctl[kCtlBegin] = end;
   ctl[kCtlEnd] = next;
 }
 
-__global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const BjNode* __restrict__ bnodes,
-                                                                        const uint32_t* __restrict__ leaf_counts,
-                                                                        BjWide* __restrict__ wide,
-                                                                        uint32_t* __restrict__ wide_src,
-                                                                        uint32_t* __restrict__ ctl,
-                                                                        uint32_t capacity, uint32_t max_leaf,
-                                                                        int max_children) {
-  const uint32_t begin = ctl[kCtlBegin], end = ctl[kCtlEnd];
+// Collapses the wide nodes [begin, end) of one level: whole waves starting at
+// `wave_base` (a multiple of 64) step by `stride` threads.
+BJ_D void collapse_level(const BjNode* __restrict__ bnodes, const uint32_t* __restrict__ leaf_counts,
+                         BjWide* __restrict__ wide, uint32_t* __restrict__ wide_src, uint32_t* __restrict__ ctl,
+                         uint32_t capacity, uint32_t max_leaf, int max_children, uint32_t begin, uint32_t end,
+                         uint32_t wave_base, uint32_t stride) {
   const uint32_t lane = lane_id();
-  const uint32_t stride = gridDim.x * blockDim.x;
   // whole waves iterate together so the slot reservation below can ballot
-  for (uint32_t base = begin + (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); base < end; base += stride) {
+  for (uint32_t base = begin + wave_base; base < end; base += stride) {
     const uint32_t w = base + lane;
     const bool active = w < end && wide_src[w] != kSentinel;
     // The children live in registers: every index into ch[] below is a
@@ -300,11 +319,82 @@ __global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const Bj
   }
 }
 
+// One level over the whole grid.  The last block to finish opens the next
+// level; a launch whose level range is empty returns at once.
+__global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const BjNode* __restrict__ bnodes,
+                                                                        const uint32_t* __restrict__ leaf_counts,
+                                                                        BjWide* __restrict__ wide,
+                                                                        uint32_t* __restrict__ wide_src,
+                                                                        uint32_t* __restrict__ ctl,
+                                                                        uint32_t capacity, uint32_t max_leaf,
+                                                                        int max_children, uint32_t* __restrict__ status) {
+  const uint32_t begin = ctl[kCtlBegin], end = ctl[kCtlEnd];
+  if (begin == end) return;  // the same for every block: nothing is left to collapse
+  collapse_level(bnodes, leaf_counts, wide, wide_src, ctl, capacity, max_leaf, max_children, begin, end,
+                 blockIdx.x * blockDim.x + (threadIdx.x & ~63u), gridDim.x * blockDim.x);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();  // this block's nodes and reservations before its ticket
+    const uint32_t ticket = atomicAdd(ctl + kCtlTicket, 1u);
+    if (ticket == gridDim.x - 1) {
+      __threadfence();
+      level_advance(ctl, capacity, status);
+      ctl[kCtlTicket] = 0;
+    }
+  }
+}
+
+// Every level left after the full-grid launches, in one block: level by level
+// with block barriers (no wait on another block), up to kTailLevelCap levels.
+__global__ void __launch_bounds__(kTailBlock) collapse_tail_kernel(const BjNode* __restrict__ bnodes,
+                                                                   const uint32_t* __restrict__ leaf_counts,
+                                                                   BjWide* __restrict__ wide,
+                                                                   uint32_t* __restrict__ wide_src,
+                                                                   uint32_t* __restrict__ ctl, uint32_t capacity,
+                                                                   uint32_t max_leaf, int max_children,
+                                                                   uint32_t* __restrict__ status) {
+  __shared__ uint32_t range[2];
+  for (uint32_t level = 0;; ++level) {
+    if (threadIdx.x == 0) {
+      range[0] = ctl[kCtlBegin];
+      range[1] = ctl[kCtlEnd];
+    }
+    __syncthreads();
+    const uint32_t begin = range[0], end = range[1];
+    if (begin == end) break;
+    if (level == kTailLevelCap) {
+      if (threadIdx.x == 0) {
+        atomicOr(ctl + kCtlStatus, kStatusTooDeep);
+        atomicMax(status, (uint32_t)BAJO_STATUS_COLLAPSE_FAILED);
+      }
+      break;
+    }
+    collapse_level(bnodes, leaf_counts, wide, wide_src, ctl, capacity, max_leaf, max_children, begin, end,
+                   threadIdx.x & ~63u, blockDim.x);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) level_advance(ctl, capacity, status);
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
-// Builds the 8-wide layout of h from its binary LBVH.  Enqueued on the
-// context stream; synchronizes once per batch of kLevelBatch levels.
-int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h) {
+// Full-grid level launches enqueued before the finishing kernel: the depth of
+// a W-wide tree over `leaves` leaf groups of max_leaf triangles plus the few
+// deeper levels an LBVH or H-PLOC tree adds (measured: 2-3 at W = 8, 4-5 at
+// W = 4 for 1M-16M triangles).  Launches past the last level return at once.
+uint32_t collapse_grid_levels(uint64_t leaves, uint32_t max_leaf, int width) {
+  const uint64_t groups = leaves / (max_leaf ? max_leaf : 1) + 1;
+  uint32_t depth = 0;
+  for (uint64_t span = 1; span < groups; span *= (uint64_t)width) ++depth;
+  return depth + (width == 4 ? 5u : 4u);
+}
+
+// Builds the W-wide layout of h from its binary hierarchy.  Enqueue-only: the
+// host reads nothing back (the node count is read at the first host query,
+// H-PLOC's root comes from the device).
+int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h, uint64_t largest_tree) {
   hipStream_t st = ctx->stream;
   const uint32_t n = (uint32_t)h->n;
   int dev = 0, cus = 0;
@@ -315,9 +405,12 @@ int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h) {
     seg_collapse_root_kernel<<<div_up(S, 256u) < 1024u ? div_up(S, 256u) : 1024u, 256, 0, st>>>(
         h->seg_scene, h->seg_off, S, h->wide, h->wide_src, h->wide_ctl);
   } else {
-    collapse_root_kernel<<<1, 64, 0, st>>>(h->scene, n, n > 1 ? h->root : 0u, h->wide, h->wide_src, h->wide_ctl);
+    const uint32_t* root_dev = (n > 1 && h->root_on_device) ? h->hp_ctl + kHpCtlRoot : nullptr;
+    collapse_root_kernel<<<1, 64, 0, st>>>(h->scene, n, n > 1 ? h->root : 0u, root_dev, h->wide, h->wide_src,
+                                           h->wide_ctl, ctx->status);
   }
   BJ_HIP(hipGetLastError());
+  h->wide_pending = false;
   if (n == 1) {
     h->wide_count = 1;
     return BAJO_OK;
@@ -327,27 +420,23 @@ int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h) {
   const unsigned cap_grid = (unsigned)cus * 8;
   if (grid > cap_grid) grid = cap_grid;
   if (grid < 1) grid = 1;
-  bool done = false;
-  for (int batch = 0; batch < 64 && !done; ++batch) {
-    for (int l = 0; l < kLevelBatch; ++l) {
-      collapse_level_kernel<<<grid, kCollapseBlock, 0, st>>>(h->nodes, h->leaf_counts, h->wide, h->wide_src,
-                                                             h->wide_ctl, (uint32_t)h->wide_capacity,
-                                                             (uint32_t)h->wide_leaf, h->node_width);
-      level_advance_kernel<<<1, 64, 0, st>>>(h->wide_ctl, (uint32_t)h->wide_capacity);
-    }
-    BJ_HIP(hipGetLastError());
-    BJ_HIP(hipMemcpyAsync(h->host_ctl, h->wide_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    BJ_HIP(hipStreamSynchronize(st));
-    if (h->host_ctl[kCtlStatus]) {
-      set_error("8-wide collapse ran out of node capacity");
-      return BAJO_ERR_INVALID_ARGUMENT;
-    }
-    done = h->host_ctl[kCtlBegin] == h->host_ctl[kCtlEnd];
-  }
-  // a binary tree over n <= 2^28 leaves is far shallower than 64 batches
-  BJ_REQUIRE(done, "8-wide collapse did not finish within %d levels", 64 * kLevelBatch);
-  h->wide_count = h->host_ctl[kCtlCount];
+  const uint32_t levels = collapse_grid_levels(largest_tree, h->wide_leaf, h->node_width);
+  for (uint32_t l = 0; l < levels; ++l)
+    collapse_level_kernel<<<grid, kCollapseBlock, 0, st>>>(h->nodes, h->leaf_counts, h->wide, h->wide_src,
+                                                           h->wide_ctl, (uint32_t)h->wide_capacity,
+                                                           (uint32_t)h->wide_leaf, h->node_width, ctx->status);
+  collapse_tail_kernel<<<1, kTailBlock, 0, st>>>(h->nodes, h->leaf_counts, h->wide, h->wide_src, h->wide_ctl,
+                                                 (uint32_t)h->wide_capacity, (uint32_t)h->wide_leaf, h->node_width,
+                                                 ctx->status);
+  BJ_HIP(hipGetLastError());
+  h->wide_pending = true;
   return BAJO_OK;
+}
+
+// The node count the last collapse left in wide_ctl (clipped to the capacity).
+uint32_t collapse_read_count(const uint32_t* host_ctl, uint64_t capacity) {
+  const uint32_t c = host_ctl[kCtlCount];
+  return (uint64_t)c > capacity ? (uint32_t)capacity : c;
 }
 
 }  // namespace bj

@@ -152,6 +152,13 @@ class GpuScene:
         self.builds += 1
         return t.as_dict() if t else None
 
+    def set_vertices(self, vertices: torch.Tensor) -> None:
+        """Enqueues a copy of device triangles f32[n, 9] into the scene (the next rebuild's input)."""
+        require_device(vertices, self.ctx, "vertices")
+        if vertices.dtype != torch.float32 or not vertices.is_contiguous() or vertices.numel() != 9 * self.triangle_count:
+            raise ValueError("vertices must be contiguous f32 with 9 floats per scene triangle")
+        _lib.check(_lib.lib().bajo_scene_set_vertices(self.ctx.handle, self.handle, ptr(vertices), self.triangle_count))
+
     def light_table(self):
         count = ctypes.c_uint32()
         total = ctypes.c_float()
@@ -182,12 +189,13 @@ class GpuSceneBatch(GpuScene):
     """A batch of triangle environments rendered in one wavefront (the batched many-environment render;
     bajo renders one scene per ``render_gpu`` call, rt/gpu/render.mojo:322-420).  Environment e is the
     triangles [offsets[e], offsets[e+1]) of ``scene`` with its own lights; all share ``scene``'s
-    materials; their BVHs are one segmented LBVH build (bvh/gpu/builder/segmented_build.mojo:89-162).
+    materials; their BVHs are one segmented build, LBVH or H-PLOC (``method``,
+    bvh/gpu/builder/segmented_build.mojo:89-162).
     A render writes f32[E, H, W, 3]; environment e's image equals a render of its scene alone with
     seed ``rng_seed + e``."""
 
     def __init__(self, scene: SceneData, offsets, ctx: DeviceContext | None = None, node_width: int = 2,
-                 leaf_size: int = 0):
+                 leaf_size: int = 0, method: int = 0):
         from .bvh import build_options
 
         self.ctx = ctx or default_context(0)
@@ -208,7 +216,7 @@ class GpuSceneBatch(GpuScene):
         if lamb.size % 3 or emit.size % 3 or met.size % 4:
             raise ValueError("materials are f32[3] per Lambertian / emissive and f32[4] per metal")
         handle = ctypes.c_void_p()
-        opts = build_options(node_width, leaf_size, 0)
+        opts = build_options(node_width, leaf_size, method)
         mats = _lib.Materials(
             lamb.ctypes.data_as(ctypes.c_void_p), lamb.size // 3, met.ctypes.data_as(ctypes.c_void_p), met.size // 4,
             die.ctypes.data_as(ctypes.c_void_p), die.size, emit.ctypes.data_as(ctypes.c_void_p), emit.size // 3,
@@ -322,8 +330,10 @@ class GpuRtRenderTarget:
         _lib.check(_lib.lib().bajo_render_enqueue(self.ctx.handle, self.handle, scene.handle))
         return self.pixels
 
-    def capture(self, scene: "GpuScene") -> None:
-        _lib.check(_lib.lib().bajo_render_target_capture(self.ctx.handle, self.handle, scene.handle))
+    def capture(self, scene: "GpuScene", rebuild: bool = False) -> None:
+        """Records the render of `scene` (with `rebuild`, its BVH rebuild first) into one HIP graph."""
+        flags = _lib.BAJO_CAPTURE_REBUILD if rebuild else 0
+        _lib.check(_lib.lib().bajo_render_target_capture_ex(self.ctx.handle, self.handle, scene.handle, flags))
         self._scene = scene  # the graph holds the scene's device pointers
 
     def replay(self) -> torch.Tensor:

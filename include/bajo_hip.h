@@ -36,6 +36,8 @@ extern "C" {
  * wide_collapse.mojo:34-38,236-254): */
 #define BAJO_STATUS_SORT_LOOKBACK 4  /* a sort tile gave up waiting for its predecessor (spin cap) */
 #define BAJO_STATUS_REFIT_CORRUPT 5  /* the refit climb met an inconsistent hierarchy or passed the depth bound */
+#define BAJO_STATUS_HPLOC_FAILED 6   /* H-PLOC made no progress, met a corrupt cluster list or ended without one root */
+#define BAJO_STATUS_COLLAPSE_FAILED 7 /* the wide collapse ran out of nodes, met a bad root or passed its level bound */
 /* WAVE_STATUS words of the renderer, rt/wavefront_contract.mojo:25-29, offset by 10 */
 #define BAJO_STATUS_WAVE_BASE 10
 #define BAJO_STATUS_PATH_OVERFLOW 11
@@ -52,8 +54,9 @@ extern "C" {
  * topology of bajo's CPU definition (bvh/cpu/builder/hploc.mojo:855-917).
  * Leaves are then renumbered depth-first, so in the exported binary layout
  * every subtree covers a contiguous range of positions and `fence` is the
- * last one (bajo leaves FENCE unset for H-PLOC); the build synchronizes the
- * context stream once to read back its root and status. */
+ * last one (bajo leaves FENCE unset for H-PLOC).  The root stays on the
+ * device (traversals read it there; bajo_bvh_get_info reads it back) and a
+ * failed clustering raises BAJO_STATUS_HPLOC_FAILED at the next sync point. */
 #define BAJO_BUILD_HPLOC 1
 
 /* Node widths (bajo's W parameter, bvh/gpu/triangle_bvh.mojo:617): 2 keeps
@@ -189,9 +192,14 @@ int32_t bajo_build_triangle_bvh(bajo_ctx* ctx, const float* vertices, uint64_t n
  * opens the child of largest surface area until it has 8 children, subtrees
  * of at most leaf_size triangles become leaf children (ranges of the sorted
  * triangle records), child boxes are stored on a per-axis power-of-two grid
- * and padded by one grid step.  The collapse runs level by level and reads
- * the level size back every few levels, so a width-8 build synchronizes the
- * context stream.  `options` NULL = binary LBVH.  n < 2^28 for width 8. */
+ * and padded by one grid step.  Every build is enqueue-only: nothing is read
+ * back and the host never waits (without `timings`), so a rebuild can be
+ * captured into a HIP graph once its buffers are allocated.  The collapse
+ * runs a depth-bounded number of level launches (a launch past the last
+ * level does nothing) and a single-block kernel for any deeper levels; the
+ * wide node count is read back at the first bajo_bvh_get_info.  Failures
+ * raise BAJO_STATUS_* into the context status word.  `options` NULL =
+ * binary LBVH.  n < 2^28 for widths 4 and 8. */
 int32_t bajo_build_triangle_bvh_ex(bajo_ctx* ctx, const float* vertices, uint64_t n,
                                    const bajo_build_options* options, bajo_build_timings* timings,
                                    bajo_bvh** out);
@@ -488,6 +496,26 @@ int32_t bajo_render_target_create_envs(bajo_ctx* ctx, const bajo_render_settings
                                        const float* camera, float* pixels, bajo_render_target** out);
 int32_t bajo_render_enqueue(bajo_ctx* ctx, bajo_render_target* target, bajo_scene* scene);
 int32_t bajo_render_target_capture(bajo_ctx* ctx, bajo_render_target* target, bajo_scene* scene);
+/* Capture flags. */
+#define BAJO_CAPTURE_REBUILD 1 /* record the scene's rebuild before its render */
+/* bajo_render_target_capture with flags (bajo has no counterpart; its closest
+ * code is the segmented build, bvh/gpu/builder/segmented_build.mojo:89-162,
+ * and the enqueue-only render, rt/gpu/render.mojo:61-235).  With
+ * BAJO_CAPTURE_REBUILD the graph holds bajo_scene_rebuild from the scene's
+ * device vertices (bajo_scene_vertices) and then the render: for a batch,
+ * the segmented build of every environment and the batched render, so one
+ * bajo_render_target_replay rebuilds and renders the whole batch with no
+ * host work.  One eager rebuild runs first (it sizes every buffer).  The
+ * light tables are not re-derived: between replays move only non-emissive
+ * triangles. */
+int32_t bajo_render_target_capture_ex(bajo_ctx* ctx, bajo_render_target* target, bajo_scene* scene, uint32_t flags);
+/* Device address and count of the scene's triangles f32[9 n] (every
+ * environment of a batch, in order): the input of bajo_scene_rebuild, which
+ * callers may rewrite on the context stream between rebuilds. */
+int32_t bajo_scene_vertices(const bajo_scene* scene, float** vertices, uint64_t* count);
+/* Enqueues a copy of device f32[9 n] into the scene's triangles (n = the
+ * scene's count); the next bajo_scene_rebuild or captured rebuild uses them. */
+int32_t bajo_scene_set_vertices(bajo_ctx* ctx, bajo_scene* scene, const float* vertices, uint64_t n);
 int32_t bajo_render_target_replay(bajo_ctx* ctx, bajo_render_target* target);
 int32_t bajo_render_target_free(bajo_render_target* target);
 /* Host view of one stage's random stream (tests and tools): `count` floats

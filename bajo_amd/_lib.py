@@ -101,6 +101,8 @@ class RenderSettings(ctypes.Structure):
         ("rng_seed", c_uint64),
         ("algorithm", c_int32),
         ("path_capacity", c_uint32),
+        ("extend_waves", c_uint32),
+        ("tail_paths", c_uint32),
     ]
 
 

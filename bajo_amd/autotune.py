@@ -1,18 +1,32 @@
 """Launch-schedule autotuner for wavefront renders.
 
 bajo has no counterpart: its renders use one fixed chunk of paths in flight
-(GPU_RT_DEFAULT_CAPACITY, rt/gpu/resources.mojo:24).  Here the number of paths
-in flight per chunk (`RenderSettings.path_capacity`) is the schedule knob.
-Whole-frame chunks give every launch the most parallel work. Smaller chunks
-keep each bounce's working set (the environments and BVH nodes that the
-chunk's paths touch) closer to the 256 MB Infinity Cache and shrink the path
-arena.  Images do not depend on the chunking: every path draws its own random
-streams and owns its sample slot, so the tuner only measures time.
+(GPU_RT_DEFAULT_CAPACITY, rt/gpu/resources.mojo:24) and fixed launch shapes.
+Here three launch knobs of `RenderSettings` form the schedule:
 
-`tune_render` times each candidate with HIP events on the context's stream
-(one untimed render first, which also sizes the path arena) and returns the
-fastest settings. Results are cached per scene object (a weak reference, so
-the entry goes with the scene) and, within it, per (native handle, build
+- ``extend_waves``: the occupancy the bounce >= 1 extension kernel is compiled
+  for (4, 6 or 8 waves per SIMD).  Fewer waves give each lane more VGPRs (no
+  spills, deeper register-held traversal state); more waves hide more memory
+  latency.  The best point depends on how incoherent the scene's rays are.
+- ``tail_paths``: the live-path count at which the tail kernel (one launch
+  that finishes a chunk's remaining bounces with paired path / shadow lanes)
+  takes over from the per-bounce launches.  Earlier saves launches on long
+  thin tails; later keeps the wide per-bounce kernels busy.
+- ``path_capacity``: paths in flight per chunk.  Whole-frame chunks give every
+  launch the most parallel work; smaller chunks keep each bounce's working set
+  (the environments and BVH nodes its paths touch) closer to the 256 MB
+  Infinity Cache and shrink the path arena.
+
+Images do not depend on the schedule: every path draws its own random streams
+and owns its sample slot, and the tail keeps the wavefront's order, so the
+tuner only measures time (the tests check the images stay bit-identical).
+
+`tune_render` is a coordinate search: starting from the defaults it times
+every candidate of one knob with the others at their best so far, keeps the
+fastest, and moves to the next knob.  Each candidate is timed with HIP events
+on the context's stream (one untimed render first, which also sizes the path
+arena).  Results are cached per scene object (a weak reference, so the entry
+goes with the scene) and, within it, per device and per (native handle, build
 count, image size, spp, depth, algorithm): a rebuilt scene is timed again.
 """
 
@@ -25,6 +39,8 @@ import torch
 
 _CACHE: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
 
+KNOBS = ("extend_waves", "tail_paths", "path_capacity")
+
 
 def default_candidates(total_paths: int, spp: int) -> list[int]:
     """Whole pixels per chunk: the whole frame, then halvings down to 1 M paths (at least 2 chunks apart)."""
@@ -34,6 +50,15 @@ def default_candidates(total_paths: int, spp: int) -> list[int]:
         out.append((cap // spp) * spp)
         cap <<= 1
     return out
+
+
+def default_space(total_paths: int, spp: int) -> dict[str, list[int]]:
+    """Candidate values per knob (0 = the library's default for that knob)."""
+    return {
+        "extend_waves": [0, 4, 8],
+        "tail_paths": [0, 1 << 15, 1 << 17, 1 << 19, 1 << 21],
+        "path_capacity": default_candidates(total_paths, spp),
+    }
 
 
 def _time(fn, stream, reps: int) -> float:
@@ -47,30 +72,48 @@ def _time(fn, stream, reps: int) -> float:
     return ev[0].elapsed_time(ev[1]) / reps
 
 
+def _device_key(ctx) -> tuple:
+    return (int(ctx.device), torch.cuda.get_device_name(ctx.torch_device))
+
+
 def tune_render(settings, camera, scene, algorithm: int, candidates: list[int] | None = None, reps: int = 2,
-                pixels: torch.Tensor | None = None) -> tuple:
-    """Times `enqueue_render_gpu` over `candidates` path capacities (0 = the whole frame) and returns
-    (best settings, {capacity: milliseconds}).  The render output goes to `pixels` (allocated if None)."""
+                pixels: torch.Tensor | None = None, space: dict[str, list[int]] | None = None) -> tuple:
+    """Tunes the render schedule of `enqueue_render_gpu(settings, camera, scene, algorithm)`.
+
+    Returns (best settings, {(knob, value): milliseconds}).  `space` maps knobs to candidate values
+    (default: `default_space`); `candidates` alone restricts the search to those path capacities.  The
+    render output goes to `pixels` (allocated if None).  Only full default searches are cached."""
     from .rt import enqueue_render_gpu
 
     envs = getattr(scene, "env_count", 1)
     total = settings.image_width * settings.image_height * settings.samples_per_pixel * envs
-    key = (scene.handle.value, getattr(scene, "builds", 0), scene.triangle_count, envs, settings.image_width,
-           settings.image_height, settings.samples_per_pixel, settings.max_depth, algorithm)
+    cacheable = candidates is None and space is None
+    if candidates is not None:
+        space = {"path_capacity": list(candidates)}
+    elif space is None:
+        space = default_space(total, settings.samples_per_pixel)
+    key = (_device_key(scene.ctx), scene.handle.value, getattr(scene, "builds", 0), scene.triangle_count, envs,
+           settings.image_width, settings.image_height, settings.samples_per_pixel, settings.max_depth, algorithm)
     per_scene = _CACHE.setdefault(scene, {})
-    if key in per_scene and candidates is None:
+    if cacheable and key in per_scene:
         best, table = per_scene[key]
-        return replace(settings, path_capacity=best), dict(table)
-    cands = candidates if candidates is not None else default_candidates(total, settings.samples_per_pixel)
+        return replace(settings, **best), dict(table)
     stream = scene.ctx.stream
     out = pixels
-    table = {}
-    for cap in cands:
-        st = replace(settings, path_capacity=int(cap))
-        if out is None:
-            out = enqueue_render_gpu(st, camera, scene, algorithm).pixels
-        table[int(cap)] = _time(lambda: enqueue_render_gpu(st, camera, scene, algorithm, pixels=out), stream, reps)
-    best = min(table, key=table.get)
-    if candidates is None:
-        per_scene[key] = (best, dict(table))
-    return replace(settings, path_capacity=best), table
+    best = {k: getattr(settings, k) for k in KNOBS if k in space}
+    table: dict[tuple[str, int], float] = {}
+    for knob in KNOBS:
+        if knob not in space:
+            continue
+        times = {}
+        for value in space[knob]:
+            st = replace(settings, **{**best, knob: int(value)})
+            if out is None:
+                out = enqueue_render_gpu(st, camera, scene, algorithm).pixels
+            times[int(value)] = _time(lambda: enqueue_render_gpu(st, camera, scene, algorithm, pixels=out), stream,
+                                      reps)
+            table[(knob, int(value))] = times[int(value)]
+        best[knob] = min(times, key=times.get)
+    if cacheable:
+        per_scene[key] = (dict(best), dict(table))
+    return replace(settings, **best), table

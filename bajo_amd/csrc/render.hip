@@ -683,8 +683,8 @@ BJ_D Hit scene_trace(const SceneView& sc, const Ray& ray, Stack& st, TraceCounte
   }
 }
 
-template <int kAlg, bool kWide, bool kFirst, int kGeo>
-__global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(6))) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
+template <int kAlg, bool kWide, bool kFirst, int kGeo, int kWaves = 6>
+__global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                           PrimaryArgs pa, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
                                                           unsigned long long* __restrict__ stats) {
@@ -1369,8 +1369,15 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   constexpr bool kTail = kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS;
   const uint64_t cap = shape.cap;
   const unsigned g_ext = shape.g_ext, g_sh = g_ext, g_tail = shape.g_tail, simds = shape.simds;
-  // the tail takes over when the remaining paths fit one pass of its resident threads
-  const uint32_t tail_threshold = g_tail * kRtBlock;
+  // the tail takes over when the remaining paths fit one pass of its resident
+  // threads, or at the schedule's own threshold
+  const uint32_t tail_threshold = st->tail_paths ? st->tail_paths : g_tail * kRtBlock;
+  BJ_REQUIRE(st->extend_waves == 0 || st->extend_waves == 4 || st->extend_waves == 6 || st->extend_waves == 8,
+             "bajo_render: extend_waves must be 0, 4, 6 or 8 (got %u)", st->extend_waves);
+  // extension kernels compiled for other occupancies (the hot renders only)
+  constexpr bool kTunable = kWide && kGeo != kGeoMixed &&
+                            (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS);
+  const uint32_t ext_waves = kTunable ? st->extend_waves : 0u;
   rc = ensure_arena(a, cap, shape.spill_threads);
   if (rc) return rc;
   hipStream_t stream = ctx->stream;
@@ -1444,6 +1451,12 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
       }
       if (b == 0)
         extend_kernel<kAlg, kWide, true, kGeo><<<div_up(chunk, kRtBlock), kRtBlock, 0, stream>>>(
+            sv, q, st->rng_seed, b, pa, a.spill, a.work_ext, a.stats);
+      else if (kTunable && ext_waves == 4)
+        extend_kernel<kAlg, kWide, false, kGeo, kTunable ? 4 : 6><<<g_ext, kRtBlock, 0, stream>>>(
+            sv, q, st->rng_seed, b, pa, a.spill, a.work_ext, a.stats);
+      else if (kTunable && ext_waves == 8)
+        extend_kernel<kAlg, kWide, false, kGeo, kTunable ? 8 : 6><<<g_ext, kRtBlock, 0, stream>>>(
             sv, q, st->rng_seed, b, pa, a.spill, a.work_ext, a.stats);
       else
         extend_kernel<kAlg, kWide, false, kGeo><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, a.spill,

@@ -13,9 +13,10 @@
 // is level-synchronous: level k processes the wide nodes allocated while
 // processing level k-1 (a contiguous index range), so no kernel ever waits on
 // another wave.  The host never reads the level sizes back: it enqueues a
-// depth-bounded number of full-grid level launches (each one's last block
-// opens the next level; a launch past the last level does nothing), then one
-// single-block kernel that finishes any deeper levels with block barriers.
+// depth-bounded number of full-grid level launches (each followed by a
+// one-thread launch that opens the next level; a launch past the last level
+// does nothing), then one single-block kernel that finishes any deeper
+// levels with block barriers.
 // The whole collapse is therefore enqueue-only and can be captured in a HIP
 // graph.  Leaf children reference ranges of the sorted triangle records that
 // the refit already wrote (an LBVH subtree covers a contiguous sorted range),
@@ -30,8 +31,8 @@ constexpr int kTailBlock = 1024;
 // Levels the single-block finishing kernel takes at most (a tree deeper than
 // the full-grid launches plus this raises BAJO_STATUS_COLLAPSE_FAILED).
 constexpr uint32_t kTailLevelCap = 4096;
-// wide_ctl words: node counter, level range, status, finished-block ticket
-constexpr int kCtlCount = 0, kCtlBegin = 1, kCtlEnd = 2, kCtlStatus = 3, kCtlTicket = 4;
+// wide_ctl words: node counter, level range, status
+constexpr int kCtlCount = 0, kCtlBegin = 1, kCtlEnd = 2, kCtlStatus = 3;
 constexpr uint32_t kStatusOutOfNodes = 1, kStatusTooDeep = 2, kStatusBadRoot = 4;
 
 struct Child {
@@ -99,7 +100,6 @@ __global__ void collapse_root_kernel(const float* __restrict__ scene, uint32_t n
                                      uint32_t* __restrict__ status) {
   if (threadIdx.x != 0) return;
   ctl[kCtlStatus] = 0;
-  ctl[kCtlTicket] = 0;
   if (n == 1) {
     // single triangle: one wide node with one leaf child spanning the scene box
     BjWide w;
@@ -146,7 +146,6 @@ __global__ void seg_collapse_root_kernel(const float* __restrict__ seg_scene, co
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid == 0) {
     ctl[kCtlStatus] = 0;
-    ctl[kCtlTicket] = 0;
     ctl[kCtlCount] = S;
     ctl[kCtlBegin] = 0;
     ctl[kCtlEnd] = S;
@@ -193,16 +192,24 @@ BJ_D void level_advance(uint32_t* __restrict__ ctl, uint32_t capacity, uint32_t*
   ctl[kCtlEnd] = next;
 }
 
-// Collapses the wide nodes [begin, end) of one level: whole waves starting at
-// `wave_base` (a multiple of 64) step by `stride` threads.
+// LDS of one block's slot reservation: per-wave totals, then offsets.
+struct Reserve {
+  uint32_t wave_total[kTailBlock / 64];
+  uint32_t first;
+};
+
+// Collapses the wide nodes [begin, end) of one level.  The whole block steps
+// together (block `block` of `blocks`, `stride` threads apart) so that one
+// atomic per block and step reserves the nodes its children need: a
+// per-node atomic on the shared counter serializes (about 25 ns each).
 BJ_D void collapse_level(const BjNode* __restrict__ bnodes, const uint32_t* __restrict__ leaf_counts,
                          BjWide* __restrict__ wide, uint32_t* __restrict__ wide_src, uint32_t* __restrict__ ctl,
                          uint32_t capacity, uint32_t max_leaf, int max_children, uint32_t begin, uint32_t end,
-                         uint32_t wave_base, uint32_t stride) {
+                         uint32_t block_base, uint32_t stride, Reserve* __restrict__ rs) {
   const uint32_t lane = lane_id();
-  // whole waves iterate together so the slot reservation below can ballot
-  for (uint32_t base = begin + wave_base; base < end; base += stride) {
-    const uint32_t w = base + lane;
+  const uint32_t wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  for (uint32_t bbase = begin + block_base; bbase < end; bbase += stride) {
+    const uint32_t w = bbase + threadIdx.x;
     const bool active = w < end && wide_src[w] != kSentinel;
     // The children live in registers: every index into ch[] below is a
     // compile-time constant after unrolling (a data-dependent index would
@@ -254,10 +261,19 @@ BJ_D void collapse_level(const BjNode* __restrict__ bnodes, const uint32_t* __re
       const uint32_t y = __shfl_up(incl, off, 64);
       if (lane >= (uint32_t)off) incl += y;
     }
-    const uint32_t total = __shfl(incl, 63, 64);
-    uint32_t first = 0;
-    if (lane == 0 && total) first = atomicAdd(ctl + kCtlCount, total);
-    first = __shfl(first, 0, 64) + (incl - k);
+    if (lane == 63) rs->wave_total[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t sum = 0;
+      for (uint32_t i = 0; i < waves; ++i) {
+        const uint32_t t = rs->wave_total[i];
+        rs->wave_total[i] = sum;
+        sum += t;
+      }
+      rs->first = sum ? atomicAdd(ctl + kCtlCount, sum) : 0u;
+    }
+    __syncthreads();
+    const uint32_t first = rs->first + rs->wave_total[wave] + (incl - k);
     if (!active) continue;
     if ((uint64_t)first + k > capacity) {
       atomicOr(ctl + kCtlStatus, kStatusOutOfNodes);
@@ -319,29 +335,25 @@ BJ_D void collapse_level(const BjNode* __restrict__ bnodes, const uint32_t* __re
   }
 }
 
-// One level over the whole grid.  The last block to finish opens the next
-// level; a launch whose level range is empty returns at once.
+// One level over the whole grid; a launch whose level range is empty returns
+// at once, and blocks past the level's nodes return before any barrier.
 __global__ void __launch_bounds__(kCollapseBlock) collapse_level_kernel(const BjNode* __restrict__ bnodes,
                                                                         const uint32_t* __restrict__ leaf_counts,
                                                                         BjWide* __restrict__ wide,
                                                                         uint32_t* __restrict__ wide_src,
                                                                         uint32_t* __restrict__ ctl,
                                                                         uint32_t capacity, uint32_t max_leaf,
-                                                                        int max_children, uint32_t* __restrict__ status) {
+                                                                        int max_children) {
+  __shared__ Reserve rs;
   const uint32_t begin = ctl[kCtlBegin], end = ctl[kCtlEnd];
-  if (begin == end) return;  // the same for every block: nothing is left to collapse
+  if ((uint64_t)begin + (uint64_t)blockIdx.x * blockDim.x >= end) return;
   collapse_level(bnodes, leaf_counts, wide, wide_src, ctl, capacity, max_leaf, max_children, begin, end,
-                 blockIdx.x * blockDim.x + (threadIdx.x & ~63u), gridDim.x * blockDim.x);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();  // this block's nodes and reservations before its ticket
-    const uint32_t ticket = atomicAdd(ctl + kCtlTicket, 1u);
-    if (ticket == gridDim.x - 1) {
-      __threadfence();
-      level_advance(ctl, capacity, status);
-      ctl[kCtlTicket] = 0;
-    }
-  }
+                 blockIdx.x * blockDim.x, gridDim.x * blockDim.x, &rs);
+}
+
+// Opens the next level once every block of the last one has finished.
+__global__ void level_advance_kernel(uint32_t* __restrict__ ctl, uint32_t capacity, uint32_t* __restrict__ status) {
+  if (threadIdx.x == 0 && ctl[kCtlBegin] != ctl[kCtlEnd]) level_advance(ctl, capacity, status);
 }
 
 // Every level left after the full-grid launches, in one block: level by level
@@ -354,6 +366,7 @@ __global__ void __launch_bounds__(kTailBlock) collapse_tail_kernel(const BjNode*
                                                                    uint32_t max_leaf, int max_children,
                                                                    uint32_t* __restrict__ status) {
   __shared__ uint32_t range[2];
+  __shared__ Reserve rs;
   for (uint32_t level = 0;; ++level) {
     if (threadIdx.x == 0) {
       range[0] = ctl[kCtlBegin];
@@ -369,8 +382,8 @@ __global__ void __launch_bounds__(kTailBlock) collapse_tail_kernel(const BjNode*
       }
       break;
     }
-    collapse_level(bnodes, leaf_counts, wide, wide_src, ctl, capacity, max_leaf, max_children, begin, end,
-                   threadIdx.x & ~63u, blockDim.x);
+    collapse_level(bnodes, leaf_counts, wide, wide_src, ctl, capacity, max_leaf, max_children, begin, end, 0u,
+                   blockDim.x, &rs);
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) level_advance(ctl, capacity, status);
@@ -421,10 +434,12 @@ int32_t collapse_wide(bajo_ctx* ctx, bajo_bvh* h, uint64_t largest_tree) {
   if (grid > cap_grid) grid = cap_grid;
   if (grid < 1) grid = 1;
   const uint32_t levels = collapse_grid_levels(largest_tree, h->wide_leaf, h->node_width);
-  for (uint32_t l = 0; l < levels; ++l)
+  for (uint32_t l = 0; l < levels; ++l) {
     collapse_level_kernel<<<grid, kCollapseBlock, 0, st>>>(h->nodes, h->leaf_counts, h->wide, h->wide_src,
                                                            h->wide_ctl, (uint32_t)h->wide_capacity,
-                                                           (uint32_t)h->wide_leaf, h->node_width, ctx->status);
+                                                           (uint32_t)h->wide_leaf, h->node_width);
+    level_advance_kernel<<<1, 64, 0, st>>>(h->wide_ctl, (uint32_t)h->wide_capacity, ctx->status);
+  }
   collapse_tail_kernel<<<1, kTailBlock, 0, st>>>(h->nodes, h->leaf_counts, h->wide, h->wide_src, h->wide_ctl,
                                                  (uint32_t)h->wide_capacity, (uint32_t)h->wide_leaf, h->node_width,
                                                  ctx->status);

@@ -52,6 +52,8 @@ class RenderSettings:
     rng_seed: int
     max_depth: int = 8
     path_capacity: int = 0  # paths in flight per chunk; 0 = the whole frame
+    extend_waves: int = 0  # occupancy of the bounce >= 1 extension kernel: 4, 6 or 8 waves per SIMD; 0 = 6
+    tail_paths: int = 0  # live paths at which the tail kernel takes over; 0 = its resident threads
 
     def as_c(self, algorithm: int) -> _lib.RenderSettings:
         return _lib.RenderSettings(
@@ -62,6 +64,8 @@ class RenderSettings:
             self.rng_seed,
             algorithm,
             self.path_capacity,
+            self.extend_waves,
+            self.tail_paths,
         )
 
 

@@ -481,8 +481,9 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
     rays = first.stats["extension_rays"] + first.stats["shadow_rays"]
     pixels = first.pixels
     render_ms = _timed(lambda: enqueue_render_gpu(st, cam, g, RENDER.NEE, pixels=pixels), stream)
-    # the autotuner's schedule: paths in flight per chunk, timed on the device over whole-frame and 1M..64M
-    # chunks (untimed here); the tuned render is then timed like the default one
+    # the autotuner's schedule: a coordinate search over the extension kernel's occupancy, the tail
+    # threshold and the paths in flight per chunk (untimed here); the tuned render is then timed like the
+    # default one
     from bajo_amd.autotune import tune_render
 
     tuned = None
@@ -491,24 +492,40 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
         tuned_px = torch.empty_like(pixels)
         tuned_ms = _timed(lambda: enqueue_render_gpu(tuned, cam, g, RENDER.NEE, pixels=tuned_px), stream)
         tuned_equal = bool(torch.equal(tuned_px, pixels))
-        print(f"[bench] autotuned path capacity {tuned.path_capacity}: {tuned_ms:.2f} ms", file=sys.stderr,
-              flush=True)
+        print(f"[bench] autotuned schedule extend_waves {tuned.extend_waves}, tail_paths {tuned.tail_paths}, "
+              f"path capacity {tuned.path_capacity}: {tuned_ms:.2f} ms", file=sys.stderr, flush=True)
     t = GpuRtRenderTarget(st, cam, RENDER.NEE, ctx, environments=count)
     t.capture(g)
     replay_ms = _timed(t.replay, stream)
     same = bool(torch.equal(t.pixels, pixels))
     t.free()
+    # the segmented build of every environment and the batched render, eagerly and as one HIP graph
+    def eager():
+        g.rebuild()
+        enqueue_render_gpu(st, cam, g, RENDER.NEE, pixels=pixels)
+
+    eager_ms = _timed(eager, stream)
+    tb = GpuRtRenderTarget(st, cam, RENDER.NEE, ctx, environments=count)
+    tb.capture(g, rebuild=True)
+    build_replay_ms = _timed(tb.replay, stream)
+    build_same = bool(torch.equal(tb.pixels, pixels))
+    tb.free()
     g.free()
-    print(f"[bench] batched render {render_ms:.2f} ms, graph replay {replay_ms:.2f} ms", file=sys.stderr, flush=True)
+    print(f"[bench] batched render {render_ms:.2f} ms, graph replay {replay_ms:.2f} ms; rebuild + render eager "
+          f"{eager_ms:.2f} ms, one graph {build_replay_ms:.2f} ms", file=sys.stderr, flush=True)
     out = {"workload": f"{count} environments ({n} triangles: soups log-uniform in [256, 65536] plus a 2-triangle "
            f"quad light each), NEE {width}x{height}x{spp} spp, depth {depth}, 8-wide segmented LBVH",
            "paths": count * width * height * spp, "rays": rays, "extension_rays": first.stats["extension_rays"],
            "shadow_rays": first.stats["shadow_rays"], "batched_render_ms": render_ms,
            "batched_render_mrays_s": rays / (render_ms / 1e3) / 1e6, "graph_replay_ms": replay_ms,
-           "graph_replay_mrays_s": rays / (replay_ms / 1e3) / 1e6, "graph_equals_render": same}
+           "graph_replay_mrays_s": rays / (replay_ms / 1e3) / 1e6, "graph_equals_render": same,
+           "rebuild_render_eager_ms": eager_ms, "rebuild_render_graph_ms": build_replay_ms,
+           "rebuild_render_graph_equal": build_same}
     if tuned is not None:
-        out["autotuned"] = {"path_capacity": tuned.path_capacity,
-                            "candidates_ms": {str(k): v for k, v in table.items()}, "render_ms": tuned_ms,
+        out["autotuned"] = {"extend_waves": tuned.extend_waves, "tail_paths": tuned.tail_paths,
+                            "path_capacity": tuned.path_capacity,
+                            "candidates_ms": {f"{k}={v}": ms for (k, v), ms in table.items()},
+                            "render_ms": tuned_ms, "default_render_ms": render_ms,
                             "mrays_s": rays / (tuned_ms / 1e3) / 1e6, "images_equal_default": tuned_equal}
     if not per_env_loop:
         return out

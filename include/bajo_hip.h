@@ -356,13 +356,25 @@ int32_t bajo_tlas_free(bajo_tlas* tlas);
 
 typedef struct bajo_scene bajo_scene;
 
-/* RenderSettings (rt/types.mojo:324-357) plus the path arena capacity of
- * GpuRtRenderTarget (rt/gpu/resources.mojo:56-99; 0 = the whole frame). */
+/* RenderSettings (rt/types.mojo:324-357) plus the launch schedule.  None of
+ * the schedule knobs changes an image (every path draws its own random
+ * streams and owns its sample slot; the tail keeps the wavefront's order):
+ *   path_capacity: paths in flight per chunk, the path arena of
+ *     GpuRtRenderTarget (rt/gpu/resources.mojo:56-99); 0 = the whole frame;
+ *   extend_waves: the occupancy the extension kernel of bounces >= 1 is
+ *     compiled for (waves per SIMD: 4, 6 or 8; 0 = 6), which trades VGPRs per
+ *     lane against resident waves; used by PATH, NEE and MIS over 4- and
+ *     8-wide triangle and batch scenes (other renders ignore it);
+ *   tail_paths: live paths at or below which the tail kernel finishes the
+ *     chunk's remaining bounces in one launch (0 = the tail grid's resident
+ *     threads). */
 typedef struct {
   uint32_t image_width, image_height, samples_per_pixel, max_depth;
   uint64_t rng_seed;
   int32_t algorithm;
   uint32_t path_capacity;
+  uint32_t extend_waves;
+  uint32_t tail_paths;
 } bajo_render_settings;
 
 /* Rays traced by one render (read back from device counters). */

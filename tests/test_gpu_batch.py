@@ -124,6 +124,8 @@ def test_batch_render_target_replay(gpu_ctx, batch):
 
 
 def test_autotuned_schedule_renders_the_same_image(gpu_ctx, batch):
+    from dataclasses import replace
+
     from bajo_amd.autotune import tune_render
     from bajo_amd.rt import GpuSceneBatch, enqueue_render_gpu
 
@@ -132,13 +134,22 @@ def test_autotuned_schedule_renders_the_same_image(gpu_ctx, batch):
     g = GpuSceneBatch(sc, off, gpu_ctx, node_width=8)
     want = enqueue_render_gpu(settings(), camera(), g, ALGS["MIS"]).numpy()
     tuned, table = tune_render(settings(), camera(), g, ALGS["MIS"], candidates=[0, 1536, 6144], reps=1)
-    assert set(table) == {0, 1536, 6144} and all(v > 0 for v in table.values())
-    assert tuned.path_capacity == min(table, key=table.get)
-    for cap in table:
-        from dataclasses import replace
-
-        got = enqueue_render_gpu(replace(settings(), path_capacity=cap), camera(), g, ALGS["MIS"]).numpy()
-        np.testing.assert_array_equal(got, want, err_msg=f"path capacity {cap}")
+    assert set(table) == {("path_capacity", c) for c in (0, 1536, 6144)} and all(v > 0 for v in table.values())
+    assert tuned.path_capacity == min((c for _, c in table), key=lambda c: table[("path_capacity", c)])
+    # every knob of the schedule, each value on its own: images stay bit-identical
+    space = {"extend_waves": [0, 4, 6, 8], "tail_paths": [0, 1, 64, 1 << 20], "path_capacity": [0, 1536]}
+    for knob, values in space.items():
+        for value in values:
+            got = enqueue_render_gpu(replace(settings(), **{knob: value}), camera(), g, ALGS["MIS"]).numpy()
+            np.testing.assert_array_equal(got, want, err_msg=f"{knob} = {value}")
+    # the full search: a cached result, an image equal to the default schedule's
+    best, full = tune_render(settings(), camera(), g, ALGS["MIS"], reps=1)
+    assert {k for k, _ in full} == {"extend_waves", "tail_paths", "path_capacity"}
+    again, cached = tune_render(settings(), camera(), g, ALGS["MIS"], reps=1)
+    assert again == best and cached == full
+    np.testing.assert_array_equal(enqueue_render_gpu(best, camera(), g, ALGS["MIS"]).numpy(), want)
+    with pytest.raises(RuntimeError, match="extend_waves"):
+        enqueue_render_gpu(replace(settings(), extend_waves=5), camera(), g, ALGS["MIS"])
     g.free()
 
 

@@ -4,7 +4,7 @@ autotuner's candidate schedules (no GPU)."""
 import numpy as np
 
 from bajo_amd import scenes
-from bajo_amd.autotune import default_candidates
+from bajo_amd.autotune import KNOBS, default_candidates, default_space
 from bajo_amd.rt import MAT, surface_id
 
 
@@ -34,3 +34,10 @@ def test_autotune_candidates():
     c = default_candidates(67_108_864, 4)
     assert c[0] == 0 and c[1:] == [1 << k for k in range(20, 26)]
     assert all(x % 3 == 0 for x in default_candidates(10_000_000, 3)[1:])
+
+
+def test_autotune_space_covers_every_knob():
+    sp = default_space(67_108_864, 4)
+    assert tuple(sp) == KNOBS
+    assert sp["extend_waves"] == [0, 4, 8] and sp["tail_paths"][0] == 0
+    assert sp["path_capacity"] == default_candidates(67_108_864, 4)

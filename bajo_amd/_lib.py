@@ -103,6 +103,8 @@ class RenderSettings(ctypes.Structure):
         ("path_capacity", c_uint32),
         ("extend_waves", c_uint32),
         ("tail_paths", c_uint32),
+        ("shadow_walk", c_uint32),
+        ("reserved", c_uint32),
     ]
 
 

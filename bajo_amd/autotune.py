@@ -2,12 +2,19 @@
 
 bajo has no counterpart: its renders use one fixed chunk of paths in flight
 (GPU_RT_DEFAULT_CAPACITY, rt/gpu/resources.mojo:24) and fixed launch shapes.
-Here three launch knobs of `RenderSettings` form the schedule:
+Here four launch knobs of `RenderSettings` form the schedule:
 
 - ``extend_waves``: the occupancy the bounce >= 1 extension kernel is compiled
-  for (4, 6 or 8 waves per SIMD).  Fewer waves give each lane more VGPRs (no
-  spills, deeper register-held traversal state); more waves hide more memory
-  latency.  The best point depends on how incoherent the scene's rays are.
+  for (4, 6 or 8 waves per SIMD), with its LDS traversal stack sized to match
+  (32, 24 or 16 entries per lane; deeper entries spill to device memory).
+  Fewer waves give each lane more VGPRs and a deeper on-chip stack; more
+  waves hide more memory latency.  The best point depends on how incoherent
+  the scene's rays are.
+- ``shadow_walk``: the any-hit walk of the shadow-ray kernel: one node per
+  step or two (the stack top loaded with the current node), leaf records one
+  ahead or four at a time.  Shadow rays leave a surface, so their walks are
+  long serial chains; the forms trade registers (occupancy) for memory-level
+  parallelism within a ray.
 - ``tail_paths``: the live-path count at which the tail kernel (one launch
   that finishes a chunk's remaining bounces with paired path / shadow lanes)
   takes over from the per-bounce launches.  Earlier saves launches on long
@@ -39,7 +46,7 @@ import torch
 
 _CACHE: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
 
-KNOBS = ("extend_waves", "tail_paths", "path_capacity")
+KNOBS = ("extend_waves", "shadow_walk", "tail_paths", "path_capacity")
 
 
 def default_candidates(total_paths: int, spp: int) -> list[int]:
@@ -56,6 +63,7 @@ def default_space(total_paths: int, spp: int) -> dict[str, list[int]]:
     """Candidate values per knob (0 = the library's default for that knob)."""
     return {
         "extend_waves": [0, 4, 8],
+        "shadow_walk": [0, 1, 2, 4],
         "tail_paths": [0, 1 << 15, 1 << 17, 1 << 19, 1 << 21],
         "path_capacity": default_candidates(total_paths, spp),
     }

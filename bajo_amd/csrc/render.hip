@@ -40,6 +40,12 @@ constexpr uint32_t kSphereHit = 1u;  // Hit.inst of a sphere hit in a mixed scen
 // scene geometry of the render kernels: triangles, spheres + triangles, or a
 // batch of triangle environments (one BVH segment per environment)
 constexpr int kGeoTriangles = 0, kGeoMixed = 1, kGeoBatch = 2;
+// Shadow walk form used when the settings leave it at 0 (1-based, as in
+// bajo_render_settings.shadow_walk): one node per step with leaf records
+// loaded four at a time measured fastest on the 4M-triangle NEE frame
+// (1.49 ms of shadow launches per render against 1.63 for the one-record
+// walk; two nodes per step lost occupancy and time).
+constexpr uint32_t kDefaultShadowWalk = 3;
 constexpr uint32_t kPathDelta = 0x80000000u, kPathIdMask = 0x7FFFFFFFu;
 constexpr int kLightStride = 14;
 constexpr int kPathPlanes = 12, kShadowPlanes = 11, kSamplePlanes = 3;
@@ -661,7 +667,7 @@ BJ_D Hit scene_trace(const SceneView& sc, const Ray& ray, Stack& st, TraceCounte
     if (h.prim != kEmpty) h.prim += (uint32_t)first;
     return h;
   } else {
-    const bool any = kMode == BAJO_TRACE_ANY_HIT || (kMode == kTraceEither && any_rt);
+    const bool any = kMode == BAJO_TRACE_ANY_HIT || kMode >= kTraceShadow || (kMode == kTraceEither && any_rt);
     TraceCounters ts, tt;
     const Hit hs = trace_accel<kMode, true, kWide, true>(sc.spheres, ray, st, &ts, any_rt);
     const bool sph = any ? (hs.prim == kEmpty && hs.t == 0.0f) : hs.prim != kEmpty;
@@ -688,9 +694,12 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(k
                                                           PrimaryArgs pa, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
                                                           unsigned long long* __restrict__ stats) {
-  __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
+  // the LDS stack sized to the occupancy: eight waves per SIMD need eight
+  // blocks' stacks in a CU's LDS; at four a deeper stack spills less
+  constexpr int kDepth = kWaves >= 8 ? kLdsStackMin : (kWaves <= 4 ? kLdsStackMax : kLdsStack);
+  __shared__ uint32_t s_stack[kDepth * kRtBlock];
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
+  Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0, kDepth};
   const uint64_t cap = q.capacity;
   const bool tail_done = q.counters[kTailDone] != 0;
   const uint32_t active = tail_done ? 0u : (q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap);
@@ -964,7 +973,9 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
   wave_add(stats + kWorkBase + 3, (uint32_t)sh_leaves);
 }
 
-template <int kAlg, bool kWide, int kGeo>
+// kWalk: the shadow walk form (traverse8_any: bit 0 two nodes per step, bit 1
+// leaf records four at a time).
+template <int kAlg, bool kWide, int kGeo, int kWalk = 0>
 __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t bounce,
                                                           uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
@@ -986,8 +997,8 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
     const uint32_t slot = sid - q.sample_base;
     const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
     TraceCounters tc;
-    const Hit h = scene_trace<BAJO_TRACE_ANY_HIT, kWide, kGeo>(sc, ray, st, &tc, false,
-                                                             kGeo == kGeoBatch ? sid / sc.env_paths : 0u);
+    const Hit h = scene_trace<kTraceShadow + kWalk, kWide, kGeo>(sc, ray, st, &tc, false,
+                                                               kGeo == kGeoBatch ? sid / sc.env_paths : 0u);
     work_visits += tc.visits;
     work_leaves += tc.leaves;
     const bool occluded = h.prim == kEmpty && h.t == 0.0f;
@@ -1242,7 +1253,7 @@ int32_t ensure_arena(RenderArena& a, uint64_t cap, uint64_t spill_threads) {
   BJ_HIP(hipMalloc(&a.sample, tiles * kSamplePlanes * sizeof(float)));
   BJ_HIP(hipMalloc(&a.counters, kCounterWords * sizeof(uint32_t)));
   BJ_HIP(hipMalloc(&a.stats, kStatsWords * sizeof(unsigned long long)));
-  BJ_HIP(hipMalloc(&a.spill, spill_threads * kSpill * sizeof(uint32_t)));
+  BJ_HIP(hipMalloc(&a.spill, spill_threads * kSpillMax * sizeof(uint32_t)));
   BJ_HIP(hipMalloc(&a.work_ext, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
   BJ_HIP(hipMalloc(&a.work_sh, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
   a.cap = cap;
@@ -1378,6 +1389,9 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   constexpr bool kTunable = kWide && kGeo != kGeoMixed &&
                             (kAlg == BAJO_RENDER_PATH || kAlg == BAJO_RENDER_NEE || kAlg == BAJO_RENDER_MIS);
   const uint32_t ext_waves = kTunable ? st->extend_waves : 0u;
+  BJ_REQUIRE(st->shadow_walk <= 4, "bajo_render: shadow_walk must be 0..4 (got %u)", st->shadow_walk);
+  // shadow walk form: 1..4 as given, 0 = the library's default
+  const uint32_t shadow_walk = (st->shadow_walk ? st->shadow_walk : kDefaultShadowWalk) - 1u;
   rc = ensure_arena(a, cap, shape.spill_threads);
   if (rc) return rc;
   hipStream_t stream = ctx->stream;
@@ -1479,7 +1493,17 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
       }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
-        shadow_kernel<kAlg, kWide, kGeo><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats);
+        if (kWide && shadow_walk == 1)
+          shadow_kernel<kAlg, kWide, kGeo, kWide ? 1 : 0><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
+                                                                                     a.work_sh, a.stats);
+        else if (kWide && shadow_walk == 2)
+          shadow_kernel<kAlg, kWide, kGeo, kWide ? 2 : 0><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
+                                                                                     a.work_sh, a.stats);
+        else if (kWide && shadow_walk == 3)
+          shadow_kernel<kAlg, kWide, kGeo, kWide ? 3 : 0><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
+                                                                                     a.work_sh, a.stats);
+        else
+          shadow_kernel<kAlg, kWide, kGeo><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats);
         BJ_HIP(hipGetLastError());
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
       }

@@ -21,9 +21,15 @@ constexpr int kTraceBlock = 256;
 // lanes tracing either kind of ray step through one instruction stream: the
 // near-first closest-hit walk, ended at the first hit for an any-hit lane.
 constexpr int kTraceEither = 2;
+constexpr int kTraceShadow = 3;  // 3..6: any hit through one of the shadow walks (8-wide)
 constexpr int kLdsStack = 24;
 constexpr int kMaxStack = 64;
 constexpr int kSpill = kMaxStack - kLdsStack;
+// The shallowest LDS stack a kernel uses (the 8-waves-per-SIMD extension
+// kernel, whose LDS must fit eight blocks per CU) and the spill it needs.
+constexpr int kLdsStackMin = 16;
+constexpr int kLdsStackMax = 32;  // the 4-waves-per-SIMD extension kernel
+constexpr int kSpillMax = kMaxStack - kLdsStackMin;
 
 struct Ray {
   v3 o, d;
@@ -252,14 +258,15 @@ struct Stack {
   uint32_t* spill;     // this thread's column: entry k at spill[k * spill_stride]
   uint64_t spill_stride;
   int sp;
+  int lds_depth = kLdsStack;  // entries held in LDS; deeper ones spill
   BJ_D void push(uint32_t v) {
-    if (sp < kLdsStack) lds[sp * kTraceBlock] = v;
-    else spill[(uint64_t)(sp - kLdsStack) * spill_stride] = v;
+    if (sp < lds_depth) lds[sp * kTraceBlock] = v;
+    else spill[(uint64_t)(sp - lds_depth) * spill_stride] = v;
     ++sp;
   }
   BJ_D uint32_t pop() {
     --sp;
-    return sp < kLdsStack ? lds[sp * kTraceBlock] : spill[(uint64_t)(sp - kLdsStack) * spill_stride];
+    return sp < lds_depth ? lds[sp * kTraceBlock] : spill[(uint64_t)(sp - lds_depth) * spill_stride];
   }
 };
 
@@ -621,6 +628,223 @@ BJ_D Hit traverse8(const WideView& wv, const Ray& ray, Stack& st, TraceCounters*
   return w.finish(wv, tc);
 }
 
+// Child slab tests of one 8-wide node: which slots the ray enters, which of
+// those are leaves, their entry distances and references.
+struct WideHits {
+  uint32_t c[8];
+  float th[8];
+  uint32_t leaf, inner;
+};
+
+BJ_D void wide_node_hits(const uint4 (&w)[6], const Ray& ray, const v3& rcp, bool px, bool py, bool pz,
+                         float node_tmax, WideHits& h) {
+  const uint32_t meta = w[0].w;
+  const int n = (int)(meta >> 24);
+  const float sx = __builtin_bit_cast(float, (meta & 0xFFu) << 23);
+  const float sy = __builtin_bit_cast(float, ((meta >> 8) & 0xFFu) << 23);
+  const float sz = __builtin_bit_cast(float, ((meta >> 16) & 0xFFu) << 23);
+  const float ax = (__builtin_bit_cast(float, w[0].x) - ray.o.x) * rcp.x;
+  const float ay = (__builtin_bit_cast(float, w[0].y) - ray.o.y) * rcp.y;
+  const float az = (__builtin_bit_cast(float, w[0].z) - ray.o.z) * rcp.z;
+  const float bx = sx * rcp.x, by = sy * rcp.y, bz = sz * rcp.z;
+  const uint32_t nx0 = px ? w[1].x : w[2].z, nx1 = px ? w[1].y : w[2].w, fx0 = px ? w[2].z : w[1].x,
+                 fx1 = px ? w[2].w : w[1].y;
+  const uint32_t ny0 = py ? w[1].z : w[3].x, ny1 = py ? w[1].w : w[3].y, fy0 = py ? w[3].x : w[1].z,
+                 fy1 = py ? w[3].y : w[1].w;
+  const uint32_t nz0 = pz ? w[2].x : w[3].z, nz1 = pz ? w[2].y : w[3].w, fz0 = pz ? w[3].z : w[2].x,
+                 fz1 = pz ? w[3].w : w[2].y;
+  h.c[0] = w[4].x; h.c[1] = w[4].y; h.c[2] = w[4].z; h.c[3] = w[4].w;
+  h.c[4] = w[5].x; h.c[5] = w[5].y; h.c[6] = w[5].z; h.c[7] = w[5].w;
+  uint32_t hit_mask = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int bt = i & 3;
+    const uint32_t qnx = i < 4 ? nx0 : nx1, qfx = i < 4 ? fx0 : fx1;
+    const uint32_t qny = i < 4 ? ny0 : ny1, qfy = i < 4 ? fy0 : fy1;
+    const uint32_t qnz = i < 4 ? nz0 : nz1, qfz = i < 4 ? fz0 : fz1;
+    const float tnx = fmaf(ubyte_f(qnx, bt), bx, ax), tfx = fmaf(ubyte_f(qfx, bt), bx, ax);
+    const float tny = fmaf(ubyte_f(qny, bt), by, ay), tfy = fmaf(ubyte_f(qfy, bt), by, ay);
+    const float tnz = fmaf(ubyte_f(qnz, bt), bz, az), tfz = fmaf(ubyte_f(qfz, bt), bz, az);
+    const float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+    const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, node_tmax));
+    h.th[i] = tmin;
+    hit_mask |= tmin <= tmax ? (1u << i) : 0u;
+  }
+  hit_mask &= n >= 8 ? 0xFFu : (1u << n) - 1u;
+  uint32_t leaf_bits = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) leaf_bits |= (h.c[i] >> 31) << i;
+  h.leaf = hit_mask & leaf_bits;
+  h.inner = hit_mask & ~leaf_bits;
+}
+
+// Moves (t, e) to the next leaf record of the leaf slots left in lm_a, then
+// lm_b (slot order); false once both are empty.
+BJ_D bool next_leaf_range2(uint32_t& lm_a, uint32_t& lm_b, const WideHits& ha, const WideHits& hb,
+                           uint32_t prim_count, uint32_t& t, uint32_t& e) {
+  while (t >= e) {
+    uint32_t ref;
+    if (lm_a) {
+      const uint32_t bit = lm_a & (0u - lm_a);
+      lm_a ^= bit;
+      ref = sel8(bit, ha.c[0], ha.c[1], ha.c[2], ha.c[3], ha.c[4], ha.c[5], ha.c[6], ha.c[7]);
+    } else if (lm_b) {
+      const uint32_t bit = lm_b & (0u - lm_b);
+      lm_b ^= bit;
+      ref = sel8(bit, hb.c[0], hb.c[1], hb.c[2], hb.c[3], hb.c[4], hb.c[5], hb.c[6], hb.c[7]);
+    } else {
+      return false;
+    }
+    t = ref & kWideLeafStartMask;
+    e = t + ((ref >> 28) & 7u) + 1u;
+    if (e > prim_count) e = prim_count;
+  }
+  return true;
+}
+
+// Any hit over the 8-wide layout for shadow rays, whose walks are long serial
+// chains (they leave a surface and pass many boxes and leaves near it), in
+// four forms picked per launch:
+//   kTwo: two nodes per step -- the node on top of the stack is loaded and
+//     tested together with the current one, so a step waits for one round of
+//     node loads for both;
+//   kBatch = 4: the leaf records of the entered leaves are loaded four at a
+//     time (a 4-triangle leaf costs one load round instead of four);
+//     kBatch = 1 keeps one record in flight ahead of the one being tested.
+// Whether a ray is occluded does not depend on the order its nodes and
+// records are visited in, so every form returns traverse8's result (an
+// occluded ray returns the same constant record); only the visit counters
+// can differ.
+template <bool kStats, bool kSph, bool kTwo, int kBatch>
+BJ_D Hit traverse8_any(const WideView& wv, const Ray& ray, Stack& st, TraceCounters* tc) {
+  const v3 rcp = rcp_dir(ray.d);
+  const bool px = rcp.x >= 0.0f, py = rcp.y >= 0.0f, pz = rcp.z >= 0.0f;
+  uint32_t n_visit = 0, n_inner = 0, n_leaf = 0, max_sp = 0;
+  bool occluded = false;
+  st.sp = 0;
+  uint32_t a = wv.node_count == 0 ? kSentinel : wv.root;
+  while (a != kSentinel) {
+    const uint32_t b = (kTwo && st.sp > 0) ? st.pop() : kSentinel;
+    if (a >= wv.node_count || (b != kSentinel && b >= wv.node_count) || n_visit > wv.node_count) {
+      atomicOr(wv.status, (uint32_t)BAJO_STATUS_BAD_NODE);
+      break;
+    }
+    uint4 wa[6], wb[6];
+    const uint4* pa = reinterpret_cast<const uint4*>(wv.nodes + a);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) wa[k] = pa[k];
+    if (kTwo && b != kSentinel) {
+      const uint4* pb = reinterpret_cast<const uint4*>(wv.nodes + b);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) wb[k] = pb[k];
+    }
+    WideHits ha, hb;
+    // both nodes' loads are in flight; test a, then b (kept apart so the
+    // scheduler does not interleave the two sets of slab temporaries)
+    __builtin_amdgcn_sched_barrier(0);
+    wide_node_hits(wa, ray, rcp, px, py, pz, ray.tmax, ha);
+    n_visit++;
+    __builtin_amdgcn_sched_barrier(0);
+    if (kTwo && b != kSentinel) {
+      wide_node_hits(wb, ray, rcp, px, py, pz, ray.tmax, hb);
+      n_visit++;
+    } else {
+      hb.leaf = hb.inner = 0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        hb.c[i] = 0;
+        hb.th[i] = 0.0f;
+      }
+    }
+    // the leaf records of both nodes (a's leaves first, slot order)
+    uint32_t lm_a = ha.leaf, lm_b = hb.leaf;
+    uint32_t tri = 0, tri_end = 0;
+    if constexpr (kBatch == 4) {
+      for (;;) {
+        uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        const bool h0 = next_leaf_range2(lm_a, lm_b, ha, hb, wv.prim_count, tri, tri_end);
+        if (h0) r0 = tri++;
+        const bool h1 = h0 && next_leaf_range2(lm_a, lm_b, ha, hb, wv.prim_count, tri, tri_end);
+        if (h1) r1 = tri++;
+        const bool h2 = h1 && next_leaf_range2(lm_a, lm_b, ha, hb, wv.prim_count, tri, tri_end);
+        if (h2) r2 = tri++;
+        const bool h3 = h2 && next_leaf_range2(lm_a, lm_b, ha, hb, wv.prim_count, tri, tri_end);
+        if (h3) r3 = tri++;
+        if (!h0) break;
+        LeafRec q0 = load_rec<kSph>(wv.tris, r0), q1, q2, q3;
+        if (h1) q1 = load_rec<kSph>(wv.tris, r1);
+        if (h2) q2 = load_rec<kSph>(wv.tris, r2);
+        if (h3) q3 = load_rec<kSph>(wv.tris, r3);
+        n_leaf += 1u + (h1 ? 1u : 0u) + (h2 ? 1u : 0u) + (h3 ? 1u : 0u);
+        if (rec_any<kSph>(q0, ray) || (h1 && rec_any<kSph>(q1, ray)) || (h2 && rec_any<kSph>(q2, ray)) ||
+            (h3 && rec_any<kSph>(q3, ray))) {
+          occluded = true;
+          break;
+        }
+        if (!h3) break;
+      }
+    } else {
+      bool have = next_leaf_range2(lm_a, lm_b, ha, hb, wv.prim_count, tri, tri_end);
+      LeafRec cur_rec;
+      if (have) cur_rec = load_rec<kSph>(wv.tris, tri);
+      while (have) {
+        uint32_t nt = tri + 1, ne = tri_end;
+        const bool nhave = next_leaf_range2(lm_a, lm_b, ha, hb, wv.prim_count, nt, ne);
+        LeafRec next_rec;
+        if (nhave) next_rec = load_rec<kSph>(wv.tris, nt);
+        n_leaf++;
+        if (rec_any<kSph>(cur_rec, ray)) {
+          occluded = true;
+          break;
+        }
+        tri = nt;
+        tri_end = ne;
+        have = nhave;
+        cur_rec = next_rec;
+      }
+    }
+    if (occluded) break;
+    uint32_t ia = ha.inner, ib = hb.inner;
+    if (kStats) n_inner += __builtin_popcount(ia) + __builtin_popcount(ib);
+    // push b's inner children, then a's farthest first; continue at a's
+    // nearest (or the top of the stack)
+    bool overflow = false;
+    while (ib) {
+      const uint32_t bit = ib & (0u - ib);
+      ib ^= bit;
+      if (st.sp >= kMaxStack) { overflow = true; break; }
+      st.push(sel8(bit, hb.c[0], hb.c[1], hb.c[2], hb.c[3], hb.c[4], hb.c[5], hb.c[6], hb.c[7]));
+    }
+    while (!overflow && (ia & (ia - 1u))) {
+      uint32_t far_bit = 0;
+      float far_t = -1.0f;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (((ia >> i) & 1u) && ha.th[i] >= far_t) {
+          far_t = ha.th[i];
+          far_bit = 1u << i;
+        }
+      ia ^= far_bit;
+      if (st.sp >= kMaxStack) { overflow = true; break; }
+      st.push(sel8(far_bit, ha.c[0], ha.c[1], ha.c[2], ha.c[3], ha.c[4], ha.c[5], ha.c[6], ha.c[7]));
+    }
+    if (overflow) {
+      atomicOr(wv.status, (uint32_t)BAJO_STATUS_STACK_OVERFLOW);
+      break;
+    }
+    if (kStats && (uint32_t)st.sp > max_sp) max_sp = (uint32_t)st.sp;
+    if (ia) a = sel8(ia, ha.c[0], ha.c[1], ha.c[2], ha.c[3], ha.c[4], ha.c[5], ha.c[6], ha.c[7]);
+    else a = st.sp == 0 ? kSentinel : st.pop();
+  }
+  if constexpr (kStats) {
+    tc->visits = n_visit;
+    tc->inner = n_inner;
+    tc->leaves = n_leaf;
+    tc->max_sp = max_sp;
+  }
+  return occluded ? Hit{0.0f, 0.0f, kEmpty, kEmpty, mk(0, 0, 0), 0.0f} : miss(ray.tmax);
+}
+
 // bvh/camera.mojo:137-154, camera record bvh/camera.mojo:19-30.
 BJ_D Ray camera_ray(const float* __restrict__ cams, uint64_t r, uint32_t w, uint32_t h, float inv_height) {
   uint64_t ppv = (uint64_t)w * h;
@@ -652,10 +876,22 @@ inline AccelView accel_view(const bajo_bvh* h, uint32_t* status) {
   return av;
 }
 
+// kTraceShadow + w (w = 0..3): any hit through a shadow walk -- w = 0 the
+// one-node Walk8, else traverse8_any with bit 0 = two nodes per step and
+// bit 1 = leaf records four at a time -- the same result as
+// BAJO_TRACE_ANY_HIT; the binary layout keeps its one-node walk.
 template <int kMode, bool kStats, bool kWide, bool kSph = false>
 BJ_D Hit trace_accel(const AccelView& av, const Ray& ray, Stack& st, TraceCounters* tc, bool any_rt = false) {
-  if constexpr (kWide) return traverse8<kMode, kStats, kSph>(av.wide, ray, st, tc, any_rt);
-  else return traverse<kMode, kStats, kSph>(av.bin, ray, st, tc, any_rt);
+  if constexpr (kMode >= kTraceShadow) {
+    constexpr int w = kMode - kTraceShadow;
+    if constexpr (kWide && w == 0) return traverse8<BAJO_TRACE_ANY_HIT, kStats, kSph>(av.wide, ray, st, tc, false);
+    else if constexpr (kWide) return traverse8_any<kStats, kSph, (w & 1) != 0, (w & 2) ? 4 : 1>(av.wide, ray, st, tc);
+    else return traverse<BAJO_TRACE_ANY_HIT, kStats, kSph>(av.bin, ray, st, tc, false);
+  } else if constexpr (kWide) {
+    return traverse8<kMode, kStats, kSph>(av.wide, ray, st, tc, any_rt);
+  } else {
+    return traverse<kMode, kStats, kSph>(av.bin, ray, st, tc, any_rt);
+  }
 }
 
 }  // namespace bj

@@ -54,6 +54,7 @@ class RenderSettings:
     path_capacity: int = 0  # paths in flight per chunk; 0 = the whole frame
     extend_waves: int = 0  # occupancy of the bounce >= 1 extension kernel: 4, 6 or 8 waves per SIMD; 0 = 6
     tail_paths: int = 0  # live paths at which the tail kernel takes over; 0 = its resident threads
+    shadow_walk: int = 0  # shadow any-hit walk: 1 one node, 2 two nodes, 3 / 4 with 4-record leaf loads; 0 = default
 
     def as_c(self, algorithm: int) -> _lib.RenderSettings:
         return _lib.RenderSettings(
@@ -66,6 +67,8 @@ class RenderSettings:
             self.path_capacity,
             self.extend_waves,
             self.tail_paths,
+            self.shadow_walk,
+            0,
         )
 
 

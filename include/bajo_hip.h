@@ -362,12 +362,17 @@ typedef struct bajo_scene bajo_scene;
  *   path_capacity: paths in flight per chunk, the path arena of
  *     GpuRtRenderTarget (rt/gpu/resources.mojo:56-99); 0 = the whole frame;
  *   extend_waves: the occupancy the extension kernel of bounces >= 1 is
- *     compiled for (waves per SIMD: 4, 6 or 8; 0 = 6), which trades VGPRs per
- *     lane against resident waves; used by PATH, NEE and MIS over 4- and
- *     8-wide triangle and batch scenes (other renders ignore it);
+ *     compiled for (waves per SIMD: 4, 6 or 8; 0 = 6) with its LDS traversal
+ *     stack sized to match (32, 24 or 16 entries), which trades VGPRs and
+ *     on-chip stack per lane against resident waves; used by PATH, NEE and
+ *     MIS over 4- and 8-wide triangle and batch scenes (others ignore it);
  *   tail_paths: live paths at or below which the tail kernel finishes the
  *     chunk's remaining bounces in one launch (0 = the tail grid's resident
- *     threads). */
+ *     threads);
+ *   shadow_walk: the 4- and 8-wide any-hit walk of the shadow-ray kernel:
+ *     1 = one node per step with one leaf record loaded ahead, 2 = two nodes
+ *     per step (the stack top with the current node), 3 = one node with leaf
+ *     records loaded four at a time, 4 = both; 0 = the library's default. */
 typedef struct {
   uint32_t image_width, image_height, samples_per_pixel, max_depth;
   uint64_t rng_seed;
@@ -375,6 +380,8 @@ typedef struct {
   uint32_t path_capacity;
   uint32_t extend_waves;
   uint32_t tail_paths;
+  uint32_t shadow_walk;
+  uint32_t reserved; /* must be 0 */
 } bajo_render_settings;
 
 /* Rays traced by one render (read back from device counters). */

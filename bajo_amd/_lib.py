@@ -104,6 +104,8 @@ class RenderSettings(ctypes.Structure):
         ("extend_waves", c_uint32),
         ("tail_paths", c_uint32),
         ("shadow_walk", c_uint32),
+        ("sparse_shadow_rays", c_uint32),
+        ("shadow_group", c_uint32),
         ("reserved", c_uint32),
     ]
 

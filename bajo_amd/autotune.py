@@ -2,7 +2,7 @@
 
 bajo has no counterpart: its renders use one fixed chunk of paths in flight
 (GPU_RT_DEFAULT_CAPACITY, rt/gpu/resources.mojo:24) and fixed launch shapes.
-Here four launch knobs of `RenderSettings` form the schedule:
+Here five launch knobs of `RenderSettings` form the schedule:
 
 - ``extend_waves``: the occupancy the bounce >= 1 extension kernel is compiled
   for (4, 6 or 8 waves per SIMD), with its LDS traversal stack sized to match
@@ -10,7 +10,11 @@ Here four launch knobs of `RenderSettings` form the schedule:
   Fewer waves give each lane more VGPRs and a deeper on-chip stack; more
   waves hide more memory latency.  The best point depends on how incoherent
   the scene's rays are.
-- ``shadow_walk``: the any-hit walk of the shadow-ray kernel: one node per
+- ``shadow_group``: lanes per shadow ray on dense bounces (1, 8, 16 or 32).
+  A group of G lanes takes G/8 nodes off its stack per step and tests all the
+  entered leaves' records at once, so a ray's serial chain shortens while
+  fewer rays are in flight; sparse bounces always get a wave per ray.
+- ``shadow_walk``: with one lane per ray, the any-hit walk: one node per
   step or two (the stack top loaded with the current node), leaf records one
   ahead or four at a time.  Shadow rays leave a surface, so their walks are
   long serial chains; the forms trade registers (occupancy) for memory-level
@@ -46,7 +50,7 @@ import torch
 
 _CACHE: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
 
-KNOBS = ("extend_waves", "shadow_walk", "tail_paths", "path_capacity")
+KNOBS = ("extend_waves", "shadow_group", "shadow_walk", "tail_paths", "path_capacity")
 
 
 def default_candidates(total_paths: int, spp: int) -> list[int]:
@@ -63,6 +67,7 @@ def default_space(total_paths: int, spp: int) -> dict[str, list[int]]:
     """Candidate values per knob (0 = the library's default for that knob)."""
     return {
         "extend_waves": [0, 4, 8],
+        "shadow_group": [0, 1, 8, 32],
         "shadow_walk": [0, 1, 2, 4],
         "tail_paths": [0, 1 << 15, 1 << 17, 1 << 19, 1 << 21],
         "path_capacity": default_candidates(total_paths, spp),

@@ -54,6 +54,18 @@ constexpr int kActive = 0, kNext = 1, kShade = 2, kShadow = 3, kStatus = 4, kFet
               kTailDone = 7;
 // guided renders: vertices queued for the guide this bounce (word 8 is the traversal status)
 constexpr int kVertex = 9;
+// blocks of work partials the one-ray-per-wave shadow kernel wrote this bounce (0: the per-lane kernel ran)
+constexpr int kSparseShadow = 10;
+// shadow rays per bounce at or below which one wave traces each ray
+constexpr uint32_t kSparseShadowRays = 8192;
+// Lanes per shadow ray above that threshold when the settings leave it at 0:
+// 16 up to kGroupShadowRays rays, one lane per ray beyond.  On the
+// 4M-triangle NEE frame's first bounce (412 K shadow rays) 16 lanes take
+// 0.76 ms against 0.95 with one lane per ray (0.80 with 8, 0.85 with 32);
+// on the 4096-environment batch (about 3 M shadow rays of small scenes) one
+// lane per ray is fastest.
+constexpr uint32_t kDefaultShadowGroup = 16;
+constexpr uint32_t kGroupShadowRays = 1u << 20;
 constexpr int kCounterWords = 16;
 constexpr uint32_t kStatusPathOverflow = 1, kStatusShadowOverflow = 3;
 constexpr int kMaxBounces = 64;
@@ -233,7 +245,8 @@ __global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsign
   const uint32_t active = counters[kActive] < cap ? counters[kActive] : cap;
   const uint32_t shadows = counters[kShadow] < cap ? counters[kShadow] : cap;
   const uint32_t ext_live = (active + 255u) / 256u < ext_blocks ? (active + 255u) / 256u : ext_blocks;
-  const uint32_t sh_live = (shadows + 255u) / 256u < sh_blocks ? (shadows + 255u) / 256u : sh_blocks;
+  uint32_t sh_live = (shadows + 255u) / 256u < sh_blocks ? (shadows + 255u) / 256u : sh_blocks;
+  if (counters[kSparseShadow]) sh_live = counters[kSparseShadow];  // the one-ray-per-wave kernel's partials
   for (uint32_t i = t; i < ext_live; i += blockDim.x) {
     a += ext_work[kWorkStride * i];
     b += ext_work[kWorkStride * i + 1];
@@ -268,6 +281,7 @@ __global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsign
   counters[kNext] = 0;
   counters[kShade] = 0;
   counters[kShadow] = 0;
+  counters[kSparseShadow] = 0;
   counters[kFetchExt] = 0;
   counters[kFetchShadow] = 0;
   counters[kVertex] = 0;
@@ -979,7 +993,8 @@ template <int kAlg, bool kWide, int kGeo, int kWalk = 0>
 __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t bounce,
                                                           uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
-                                                          unsigned long long* __restrict__ stats) {
+                                                          unsigned long long* __restrict__ stats,
+                                                          uint32_t sparse_rays) {
   __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Stack st{s_stack + threadIdx.x, spill + gtid, (uint64_t)gridDim.x * blockDim.x, 0};
@@ -987,7 +1002,7 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
   const uint32_t count =
       q.counters[kTailDone] ? 0u : (q.counters[kShadow] < cap ? q.counters[kShadow] : (uint32_t)cap);
   uint64_t work_visits = 0, work_leaves = 0;
-  if ((uint64_t)blockIdx.x * kRtBlock >= count) return;
+  if ((uint64_t)blockIdx.x * kRtBlock >= count || count <= sparse_rays) return;
   const uint32_t idx = (uint32_t)gtid;
   if (idx < count) {
     float f[kShadowPlanes];
@@ -1010,6 +1025,66 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
     }
   }
   store_block_work(work, work_visits, work_leaves, (uint32_t)work_visits);
+}
+
+// Shadow rays with G lanes per ray (GroupWalk) when the bounce's shadow count
+// lies in (lo, hi] -- decided on the device from the queue length, so exactly
+// one shadow launch of a bounce traces its rays.  After the first bounce only
+// a few thousand shadow rays remain, and the per-lane kernel's time is the
+// serial chain of its slowest lane (over 100 node visits): there a whole
+// wave (G = 64) carries each ray.  Each group takes the next ray as soon as
+// it finishes one.  Triangle and batch scenes at widths 4 and 8.
+template <int kAlg, int kGeo, int G>
+__global__ void __launch_bounds__(kRtBlock) shadow_group_kernel(SceneView sc, Queues q, uint32_t bounce, uint32_t lo,
+                                                                uint32_t hi, unsigned long long* __restrict__ work) {
+  constexpr int kGroups = kRtBlock / G;
+  __shared__ uint32_t s_stack[kGroups][GroupWalk<G>::kStack];
+  __shared__ uint32_t s_recs[kGroups][GroupWalk<G>::kRecs];
+  const uint64_t cap = q.capacity;
+  const uint32_t count =
+      q.counters[kTailDone] ? 0u : (q.counters[kShadow] < cap ? q.counters[kShadow] : (uint32_t)cap);
+  if (count <= lo || count > hi) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) q.counters[kSparseShadow] = gridDim.x;
+  const uint32_t grp = threadIdx.x / G, gl = lane_id() % G;
+  uint32_t* stack = s_stack[grp];
+  uint32_t* recs = s_recs[grp];
+  const uint32_t groups = gridDim.x * kGroups;
+  uint32_t idx = blockIdx.x * kGroups + grp;
+  uint32_t visits = 0, tests = 0, ray_max = 0, rv = 0;
+  GroupWalk<G> walk;
+  WideView wv = sc.bvh.wide;
+  float f[kShadowPlanes];
+  uint32_t slot = 0;
+  bool have = false;
+  for (;;) {
+    if (!have && idx < count) {
+#pragma unroll
+      for (int p = 0; p < kShadowPlanes; ++p) f[p] = q.shadow_fields[plane_at<kShadowPlanes>(idx, p)];
+      const uint32_t sid = q.shadow_ids[idx];
+      slot = sid - q.sample_base;
+      if constexpr (kGeo == kGeoBatch) wv.root = sid / sc.env_paths;  // segment roots lead the node array
+      walk.begin(wv, Ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]}, stack);
+      rv = 0;
+      have = true;
+    }
+    if (__ballot(have) == 0) break;  // every group of the wave is out of rays
+    if (have) {
+      bool occluded = false;
+      if (walk.step(wv, stack, recs, rv, tests, occluded)) {
+        if (gl == 0 && (kAlg == BAJO_RENDER_AO || !occluded)) {
+          const v3 c = shadow_radiance<kAlg>(occluded, f);
+          accumulate(q.sample, slot, c);
+          if constexpr (kAlg == BAJO_RENDER_GUIDED)
+            if (q.contrib) contrib_add(q.contrib, slot, q.depth, bounce, c);
+        }
+        visits += rv;
+        ray_max = rv > ray_max ? rv : ray_max;
+        have = false;
+        idx += groups;
+      }
+    }
+  }
+  store_block_work(work, gl == 0 ? visits : 0u, gl == 0 ? tests : 0u, ray_max);
 }
 
 // The continuation of the guided vertices of one bounce: one-sample MIS of
@@ -1392,6 +1467,25 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   BJ_REQUIRE(st->shadow_walk <= 4, "bajo_render: shadow_walk must be 0..4 (got %u)", st->shadow_walk);
   // shadow walk form: 1..4 as given, 0 = the library's default
   const uint32_t shadow_walk = (st->shadow_walk ? st->shadow_walk : kDefaultShadowWalk) - 1u;
+  // one ray per wave for bounces with few shadow rays: sparse_rays (0 = the
+  // default, UINT32_MAX never), a grid of at most that many waves
+  const uint32_t sparse_rays = st->sparse_shadow_rays == 0 ? kSparseShadowRays
+                               : (st->sparse_shadow_rays == 0xFFFFFFFFu ? 0u : st->sparse_shadow_rays);
+  const unsigned g_wave_cap = div_up(sparse_rays ? sparse_rays : 1u, kRtBlock / 64);
+  const unsigned g_wave = g_wave_cap < g_sh ? g_wave_cap : g_sh;
+  // lanes per shadow ray above the sparse threshold (1 = the per-lane walk)
+  BJ_REQUIRE(st->shadow_group == 0 || st->shadow_group == 1 || st->shadow_group == 8 || st->shadow_group == 16 ||
+                 st->shadow_group == 32,
+             "bajo_render: shadow_group must be 0, 1, 8, 16 or 32 (got %u)", st->shadow_group);
+  const bool adaptive_group = st->shadow_group == 0;
+  const uint32_t shadow_group = adaptive_group ? kDefaultShadowGroup : st->shadow_group;
+  // a resident grid for the lane-group kernel (six 256-thread blocks per CU)
+  int dev_id = 0, cu_count = 0;
+  BJ_HIP(hipGetDevice(&dev_id));
+  BJ_HIP(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev_id));
+  unsigned g_group = 6u * (unsigned)cu_count;
+  if (g_group > g_sh) g_group = g_sh;
+  if (g_group < 1) g_group = 1;
   rc = ensure_arena(a, cap, shape.spill_threads);
   if (rc) return rc;
   hipStream_t stream = ctx->stream;
@@ -1493,17 +1587,44 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
       }
       if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
-        if (kWide && shadow_walk == 1)
+        // sparse bounces: one wave per ray; dense ones: lane groups or one
+        // lane per ray (triangle and batch scenes, 4- and 8-wide)
+        constexpr bool kSparse = kWide && kGeo != kGeoMixed;
+        const uint32_t sparse = kSparse ? sparse_rays : 0u;
+        const uint32_t dense_group = kSparse ? shadow_group : 1u;
+        // counts the lane-group kernel takes: (sparse, group_hi]; the
+        // per-lane kernel takes the counts above `per_lane_lo`
+        const uint32_t group_hi = dense_group > 1 ? (adaptive_group ? kGroupShadowRays : 0xFFFFFFFFu) : sparse;
+        const uint32_t per_lane_lo = group_hi > sparse ? group_hi : sparse;
+        if (dense_group > 1 && !adaptive_group) {
+          // the lane-group kernel takes every count above the sparse threshold
+        } else if (kWide && shadow_walk == 1)
           shadow_kernel<kAlg, kWide, kGeo, kWide ? 1 : 0><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
-                                                                                     a.work_sh, a.stats);
+                                                                                     a.work_sh, a.stats, per_lane_lo);
         else if (kWide && shadow_walk == 2)
           shadow_kernel<kAlg, kWide, kGeo, kWide ? 2 : 0><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
-                                                                                     a.work_sh, a.stats);
+                                                                                     a.work_sh, a.stats, per_lane_lo);
         else if (kWide && shadow_walk == 3)
           shadow_kernel<kAlg, kWide, kGeo, kWide ? 3 : 0><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
-                                                                                     a.work_sh, a.stats);
+                                                                                     a.work_sh, a.stats, per_lane_lo);
         else
-          shadow_kernel<kAlg, kWide, kGeo><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats);
+          shadow_kernel<kAlg, kWide, kGeo><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats,
+                                                                          per_lane_lo);
+        if constexpr (kSparse) {
+          constexpr int kG = kSparse ? kGeo : kGeoTriangles;
+          // dense bounces with lane groups (the per-lane kernel then stands aside)
+          if (dense_group == 8)
+            shadow_group_kernel<kAlg, kG, 8><<<g_group, kRtBlock, 0, stream>>>(sv, q, b, sparse, group_hi,
+                                                                             a.work_sh);
+          else if (dense_group == 16)
+            shadow_group_kernel<kAlg, kG, 16><<<g_group, kRtBlock, 0, stream>>>(sv, q, b, sparse, group_hi,
+                                                                              a.work_sh);
+          else if (dense_group == 32)
+            shadow_group_kernel<kAlg, kG, 32><<<g_group, kRtBlock, 0, stream>>>(sv, q, b, sparse, group_hi,
+                                                                              a.work_sh);
+          if (sparse)
+            shadow_group_kernel<kAlg, kG, 64><<<g_wave, kRtBlock, 0, stream>>>(sv, q, b, 0u, sparse, a.work_sh);
+        }
         BJ_HIP(hipGetLastError());
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
       }

@@ -876,6 +876,147 @@ inline AccelView accel_view(const bajo_bvh* h, uint32_t* status) {
   return av;
 }
 
+// Orders one wave's LDS writes before its later LDS reads by other lanes.
+BJ_D void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Any hit with G lanes per ray (G = 8, 16, 32 or 64): the walk of rays whose
+// chains are long and serial when one lane carries a ray, such as shadow
+// rays (they leave a surface and pass many boxes and leaves near it).  A
+// step takes up to G/8 nodes off the group's LDS stack, lane 8j + i tests
+// child i of node j (the same quantized slab test as Walk8), entered inner
+// children are pushed, and the records of all entered leaves are tested one
+// per lane.  A step therefore waits for one round of node loads and one of
+// record loads whatever the width of the ray's frontier.  The records tested
+// are every record whose leaf box the ray enters, as in every other any-hit
+// walk, so occlusion comes out the same.
+//
+// GroupWalk holds one group's state; every lane of the group holds the same
+// copy.  step() is called by the group's lanes together (other groups of the
+// wave may be stepping their own rays at the same time).
+template <int G>
+struct GroupWalk {
+  static constexpr int kStack = 16 * G;  // a level pushes at most 7G/8 net; 18 levels fit
+  static constexpr int kRecs = 8 * G;    // G entered leaves of up to 8 records
+  Ray ray;
+  v3 rcp;
+  bool px, py, pz;
+  int sp;
+  uint32_t steps;
+
+  BJ_D uint64_t group_mask() const {
+    const uint32_t lane = lane_id();
+    return (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << ((lane / G) * G);
+  }
+
+  BJ_D void begin(const WideView& wv, const Ray& r, uint32_t* stack) {
+    ray = r;
+    rcp = rcp_dir(r.d);
+    px = rcp.x >= 0.0f;
+    py = rcp.y >= 0.0f;
+    pz = rcp.z >= 0.0f;
+    steps = 0;
+    sp = wv.node_count == 0 ? 0 : 1;
+    if (lane_id() % G == 0) stack[0] = wv.root;
+    wave_lds_sync();
+  }
+
+  // One step; returns true once the ray is finished (then *occluded says how).
+  BJ_D bool step(const WideView& wv, uint32_t* stack, uint32_t* recs, uint32_t& visits, uint32_t& tests,
+                 bool& occluded) {
+    occluded = false;
+    if (sp == 0) return true;
+    const uint32_t gl = lane_id() % G, j = gl >> 3, i = gl & 7;
+    const uint64_t gm = group_mask();
+    const int nf = sp < G / 8 ? sp : G / 8;
+    const bool mine = (int)j < nf;
+    const uint32_t node = mine ? stack[sp - 1 - (int)j] : 0u;
+    wave_lds_sync();
+    sp -= nf;
+    visits += (uint32_t)nf;
+    if ((__ballot(mine && node >= wv.node_count) & gm) != 0 || ++steps > wv.node_count) {
+      if (gl == 0) atomicOr(wv.status, (uint32_t)BAJO_STATUS_BAD_NODE);
+      return true;
+    }
+    bool hit = false;
+    uint32_t ref = 0;
+    if (mine) {
+      const uint4* np = reinterpret_cast<const uint4*>(wv.nodes + node);
+      const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+      ref = reinterpret_cast<const uint32_t*>(wv.nodes + node)[16 + i];
+      const uint32_t meta = w0.w;
+      const uint32_t n = meta >> 24;
+      const float sx = __builtin_bit_cast(float, (meta & 0xFFu) << 23);
+      const float sy = __builtin_bit_cast(float, ((meta >> 8) & 0xFFu) << 23);
+      const float sz = __builtin_bit_cast(float, ((meta >> 16) & 0xFFu) << 23);
+      const float ax = (__builtin_bit_cast(float, w0.x) - ray.o.x) * rcp.x;
+      const float ay = (__builtin_bit_cast(float, w0.y) - ray.o.y) * rcp.y;
+      const float az = (__builtin_bit_cast(float, w0.z) - ray.o.z) * rcp.z;
+      const float bx = sx * rcp.x, by = sy * rcp.y, bz = sz * rcp.z;
+      // byte i of lo_x (w1.xy), lo_y (w1.zw), lo_z (w2.xy), hi_x (w2.zw), hi_y (w3.xy), hi_z (w3.zw)
+      const bool hiw = i >= 4;
+      const int bt = (int)(i & 3u);
+      const uint32_t lox = hiw ? w1.y : w1.x, loy = hiw ? w1.w : w1.z, loz = hiw ? w2.y : w2.x;
+      const uint32_t hix = hiw ? w2.w : w2.z, hiy = hiw ? w3.y : w3.x, hiz = hiw ? w3.w : w3.z;
+      const uint32_t qnx = px ? lox : hix, qfx = px ? hix : lox;
+      const uint32_t qny = py ? loy : hiy, qfy = py ? hiy : loy;
+      const uint32_t qnz = pz ? loz : hiz, qfz = pz ? hiz : loz;
+      const float tnx = fmaf(ubyte_f(qnx, bt), bx, ax), tfx = fmaf(ubyte_f(qfx, bt), bx, ax);
+      const float tny = fmaf(ubyte_f(qny, bt), by, ay), tfy = fmaf(ubyte_f(qfy, bt), by, ay);
+      const float tnz = fmaf(ubyte_f(qnz, bt), bz, az), tfz = fmaf(ubyte_f(qfz, bt), bz, az);
+      const float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+      const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, ray.tmax));
+      hit = i < n && tmin <= tmax;
+    }
+    const bool is_leaf = hit && (ref & kLeafFlag) != 0;
+    const bool is_inner = hit && (ref & kLeafFlag) == 0;
+    // entered inner children onto the group's stack
+    const uint64_t inner = __ballot(is_inner) & gm;
+    const int n_inner = __builtin_popcountll(inner);
+    if (sp + n_inner > kStack) {
+      if (gl == 0) atomicOr(wv.status, (uint32_t)BAJO_STATUS_STACK_OVERFLOW);
+      sp = 0;
+      return true;
+    }
+    if (is_inner) stack[sp + (int)mask_rank(inner)] = ref;
+    sp += n_inner;
+    // the entered leaves' records, one per lane
+    uint32_t start = 0, k = 0;
+    if (is_leaf) {
+      start = ref & kWideLeafStartMask;
+      const uint32_t e = start + ((ref >> 28) & 7u) + 1u;
+      k = (e > wv.prim_count ? wv.prim_count : e) - (start < wv.prim_count ? start : wv.prim_count);
+    }
+    uint32_t incl = k;
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off, G);
+      if (gl >= (uint32_t)off) incl += y;
+    }
+    const uint32_t total = __shfl(incl, G - 1, G);
+    for (uint32_t m = 0; m < k; ++m) recs[incl - k + m] = start + m;
+    wave_lds_sync();
+    tests += total;
+    for (uint32_t base = 0; base < total; base += G) {
+      bool occ = false;
+      if (base + gl < total) {
+        const LeafRec rec = load_rec<false>(wv.tris, recs[base + gl]);
+        occ = rec_any<false>(rec, ray);
+      }
+      if ((__ballot(occ) & gm) != 0) {
+        occluded = true;
+        wave_lds_sync();
+        return true;
+      }
+    }
+    wave_lds_sync();
+    return sp == 0;
+  }
+};
+
 // kTraceShadow + w (w = 0..3): any hit through a shadow walk -- w = 0 the
 // one-node Walk8, else traverse8_any with bit 0 = two nodes per step and
 // bit 1 = leaf records four at a time -- the same result as

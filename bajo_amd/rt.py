@@ -55,6 +55,8 @@ class RenderSettings:
     extend_waves: int = 0  # occupancy of the bounce >= 1 extension kernel: 4, 6 or 8 waves per SIMD; 0 = 6
     tail_paths: int = 0  # live paths at which the tail kernel takes over; 0 = its resident threads
     shadow_walk: int = 0  # shadow any-hit walk: 1 one node, 2 two nodes, 3 / 4 with 4-record leaf loads; 0 = default
+    sparse_shadow_rays: int = 0  # shadow rays per bounce traced one per wave at or below; 0 = 8192, 2**32-1 never
+    shadow_group: int = 0  # lanes per shadow ray above that: 1, 8, 16 or 32; 0 = default
 
     def as_c(self, algorithm: int) -> _lib.RenderSettings:
         return _lib.RenderSettings(
@@ -68,6 +70,8 @@ class RenderSettings:
             self.extend_waves,
             self.tail_paths,
             self.shadow_walk,
+            self.sparse_shadow_rays,
+            self.shadow_group,
             0,
         )
 

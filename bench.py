@@ -492,8 +492,9 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
         tuned_px = torch.empty_like(pixels)
         tuned_ms = _timed(lambda: enqueue_render_gpu(tuned, cam, g, RENDER.NEE, pixels=tuned_px), stream)
         tuned_equal = bool(torch.equal(tuned_px, pixels))
-        print(f"[bench] autotuned schedule extend_waves {tuned.extend_waves}, tail_paths {tuned.tail_paths}, "
-              f"path capacity {tuned.path_capacity}: {tuned_ms:.2f} ms", file=sys.stderr, flush=True)
+        print(f"[bench] autotuned schedule extend_waves {tuned.extend_waves}, shadow_group {tuned.shadow_group}, "
+              f"shadow_walk {tuned.shadow_walk}, tail_paths {tuned.tail_paths}, path capacity {tuned.path_capacity}: "
+              f"{tuned_ms:.2f} ms", file=sys.stderr, flush=True)
     t = GpuRtRenderTarget(st, cam, RENDER.NEE, ctx, environments=count)
     t.capture(g)
     replay_ms = _timed(t.replay, stream)
@@ -522,7 +523,8 @@ def batched_render_workload(ctx, count: int = 4096, width: int = 64, height: int
            "rebuild_render_eager_ms": eager_ms, "rebuild_render_graph_ms": build_replay_ms,
            "rebuild_render_graph_equal": build_same}
     if tuned is not None:
-        out["autotuned"] = {"extend_waves": tuned.extend_waves, "tail_paths": tuned.tail_paths,
+        out["autotuned"] = {"extend_waves": tuned.extend_waves, "shadow_group": tuned.shadow_group,
+                            "shadow_walk": tuned.shadow_walk, "tail_paths": tuned.tail_paths,
                             "path_capacity": tuned.path_capacity,
                             "candidates_ms": {f"{k}={v}": ms for (k, v), ms in table.items()},
                             "render_ms": tuned_ms, "default_render_ms": render_ms,

@@ -372,7 +372,15 @@ typedef struct bajo_scene bajo_scene;
  *   shadow_walk: the 4- and 8-wide any-hit walk of the shadow-ray kernel:
  *     1 = one node per step with one leaf record loaded ahead, 2 = two nodes
  *     per step (the stack top with the current node), 3 = one node with leaf
- *     records loaded four at a time, 4 = both; 0 = the library's default. */
+ *     records loaded four at a time, 4 = both; 0 = the library's default;
+ *   sparse_shadow_rays: shadow rays per bounce at or below which each ray
+ *     gets a whole wave (up to eight nodes and all their leaf records per
+ *     step; 4- and 8-wide triangle and batch scenes); 0 = 8192, UINT32_MAX =
+ *     never;
+ *   shadow_group: lanes per shadow ray above that threshold: 1 = one lane
+ *     per ray (shadow_walk applies), 8, 16 or 32 = that many lanes per ray
+ *     (one node per 8 lanes per step); 0 = 16 lanes up to 2^20 rays and one
+ *     lane per ray beyond. */
 typedef struct {
   uint32_t image_width, image_height, samples_per_pixel, max_depth;
   uint64_t rng_seed;
@@ -381,6 +389,8 @@ typedef struct {
   uint32_t extend_waves;
   uint32_t tail_paths;
   uint32_t shadow_walk;
+  uint32_t sparse_shadow_rays;
+  uint32_t shadow_group;
   uint32_t reserved; /* must be 0 */
 } bajo_render_settings;
 

@@ -71,7 +71,7 @@ class BuildOptions(ctypes.Structure):
         ("method", c_int32),
         ("node_width", c_int32),
         ("leaf_size", c_int32),
-        ("reserved", c_int32),
+        ("merging_threshold", c_int32),
     ]
 
 

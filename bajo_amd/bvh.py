@@ -58,8 +58,10 @@ WIDE_NODE_DTYPE = np.dtype(
 assert WIDE_NODE_DTYPE.itemsize == 96
 
 
-def build_options(node_width: int = 2, leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH):
-    return _lib.BuildOptions(int(method), int(node_width), int(leaf_size), 0)
+def build_options(node_width: int = 2, leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH,
+                  merging_threshold: int = 0):
+    """H-PLOC's merging threshold: 0 / 16 = bajo's CPU definition, 32 = its GPU build on wave64."""
+    return _lib.BuildOptions(int(method), int(node_width), int(leaf_size), int(merging_threshold))
 
 
 @dataclass
@@ -86,6 +88,8 @@ class Camera:
 class GpuTriangleBvh:
     """Ready-to-trace triangle BVH (bvh/gpu/triangle_bvh.mojo:494-614)."""
 
+    merging_threshold = 0  # H-PLOC merging threshold of its builds (0 = 16)
+
     def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, prim_count: int, node_width: int = 2,
                  leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH):
         self._handle = handle
@@ -111,7 +115,7 @@ class GpuTriangleBvh:
         n = _check_vertices(vertices, self.ctx)
         t = _lib.BuildTimings() if timings else None
         handle = ctypes.c_void_p(self.handle.value)
-        opts = build_options(self.node_width, self.leaf_size, self.method)
+        opts = build_options(self.node_width, self.leaf_size, self.method, self.merging_threshold)
         _lib.check(
             _lib.lib().bajo_build_triangle_bvh_ex(
                 self.ctx.handle, ptr(vertices), n, ctypes.byref(opts), ctypes.byref(t) if t else None,
@@ -216,7 +220,7 @@ class GpuSphereBvh(GpuTriangleBvh):
         n = _check_spheres(spheres, self.ctx)
         t = _lib.BuildTimings() if timings else None
         handle = ctypes.c_void_p(self.handle.value)
-        opts = build_options(self.node_width, self.leaf_size, self.method)
+        opts = build_options(self.node_width, self.leaf_size, self.method, self.merging_threshold)
         _lib.check(_lib.lib().bajo_build_sphere_bvh(self.ctx.handle, ptr(spheres), n, ctypes.byref(opts),
                                                     ctypes.byref(t) if t else None, ctypes.byref(handle)))
         self._handle = handle
@@ -254,6 +258,8 @@ class GpuTriangleBlasSet:
     as a ready-to-trace BVH that shares the set's storage (bajo's BlasDesc); it stays valid until the set
     is freed."""
 
+    merging_threshold = 0  # H-PLOC merging threshold of its builds (0 = 16)
+
     def __init__(self, handle: ctypes.c_void_p, ctx: DeviceContext, counts: list, node_width: int, leaf_size: int,
                  method: int = 0):
         self._handle = handle
@@ -277,6 +283,7 @@ class GpuTriangleBlasSet:
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().bajo_blas_set_get(self.handle, int(k), ctypes.byref(h)))
         view = GpuTriangleBvh(h, self.ctx, self.counts[k], self.node_width, self.leaf_size, self.method)
+        view.merging_threshold = self.merging_threshold
         view._owner = self
         return view
 
@@ -289,7 +296,7 @@ class GpuTriangleBlasSet:
             raise ValueError("a rebuild keeps the mesh count")
         t = {} if timings else None
         _build_set(vertices, offsets, self.ctx, self.node_width, self.leaf_size, t, ctypes.c_void_p(self.handle.value),
-                   self.method)
+                   self.method, self.merging_threshold)
         self.counts = np.diff(np.asarray(offsets, np.int64)).tolist()
         return t
 
@@ -318,13 +325,14 @@ class GpuTriangleBlasSet:
 
 
 def _build_set(flat: torch.Tensor, offsets: np.ndarray, ctx: DeviceContext, node_width: int, leaf_size: int,
-               timings: dict | None, handle: ctypes.c_void_p, method: int = 0) -> ctypes.c_void_p:
+               timings: dict | None, handle: ctypes.c_void_p, method: int = 0,
+               merging_threshold: int = 0) -> ctypes.c_void_p:
     _check_vertices(flat, ctx)
     off = np.ascontiguousarray(offsets, np.uint64)
     if off.size < 2 or int(off[-1]) * 9 != flat.numel():
         raise ValueError("segment offsets must end at the triangle count of the vertex buffer")
     t = _lib.BuildTimings() if timings is not None else None
-    opts = build_options(node_width, leaf_size, method)
+    opts = build_options(node_width, leaf_size, method, merging_threshold)
     _lib.check(_lib.lib().bajo_build_triangle_blas_set(
         ctx.handle, ptr(flat), off.ctypes.data_as(ctypes.c_void_p), off.size - 1, ctypes.byref(opts),
         ctypes.byref(t) if t else None, ctypes.byref(handle)))
@@ -351,13 +359,17 @@ def build_triangle_blas_set(vertex_sets: list, ctx: DeviceContext | None = None,
 def build_triangle_blas_set_flat(vertices: torch.Tensor, offsets: np.ndarray, ctx: DeviceContext | None = None,
                                  node_width: int = 2, leaf_size: int = 0,
                                  timings: dict | None = None,
-                                 method: int = GpuBvhBuildMethod.LBVH) -> GpuTriangleBlasSet:
+                                 method: int = GpuBvhBuildMethod.LBVH,
+                                 merging_threshold: int = 0) -> GpuTriangleBlasSet:
     """The same build over meshes stored back to back in `vertices`, mesh k = triangles
     [offsets[k], offsets[k + 1])."""
     ctx = ctx or default_context(vertices.device.index or 0)
-    handle = _build_set(vertices, offsets, ctx, node_width, leaf_size, timings, ctypes.c_void_p(), method)
+    handle = _build_set(vertices, offsets, ctx, node_width, leaf_size, timings, ctypes.c_void_p(), method,
+                        merging_threshold)
     counts = np.diff(np.asarray(offsets, np.int64)).tolist()
-    return GpuTriangleBlasSet(handle, ctx, counts, node_width, leaf_size, method)
+    out = GpuTriangleBlasSet(handle, ctx, counts, node_width, leaf_size, method)
+    out.merging_threshold = merging_threshold
+    return out
 
 
 @dataclass
@@ -467,18 +479,19 @@ def _check_vertices(vertices: torch.Tensor, ctx: DeviceContext) -> int:
 
 def build_triangle_bvh(
     vertices: torch.Tensor, ctx: DeviceContext | None = None, timings: dict | None = None, node_width: int = 2,
-    leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH,
+    leaf_size: int = 0, method: int = GpuBvhBuildMethod.LBVH, merging_threshold: int = 0,
 ):
     """build_triangle_bvh[WORLD, W, L, method] (bvh/gpu/triangle_bvh.mojo:617-630).
 
     method LBVH or HPLOC builds the binary hierarchy; node_width 2 keeps it,
-    8 collapses it into quantized 8-wide nodes whose leaf children hold up to
-    `leaf_size` triangles (0 = 4)."""
+    4 or 8 collapses it into quantized wide nodes whose leaf children hold up
+    to `leaf_size` triangles (0 = 4).  H-PLOC merges with `merging_threshold`
+    16 (0) or 32."""
     ctx = ctx or default_context(vertices.device.index or 0)
     n = _check_vertices(vertices, ctx)
     t = _lib.BuildTimings() if timings is not None else None
     handle = ctypes.c_void_p()
-    opts = build_options(node_width, leaf_size, method)
+    opts = build_options(node_width, leaf_size, method, merging_threshold)
     _lib.check(
         _lib.lib().bajo_build_triangle_bvh_ex(
             ctx.handle, ptr(vertices), n, ctypes.byref(opts), ctypes.byref(t) if t else None, ctypes.byref(handle)
@@ -486,7 +499,9 @@ def build_triangle_bvh(
     )
     if t is not None:
         timings.update(t.as_dict())
-    return GpuTriangleBvh(handle, ctx, n, node_width, leaf_size, method)
+    out = GpuTriangleBvh(handle, ctx, n, node_width, leaf_size, method)
+    out.merging_threshold = merging_threshold
+    return out
 
 
 class GpuBinaryBoundsBvh(GpuTriangleBvh):
@@ -502,7 +517,7 @@ class GpuBinaryBoundsBvh(GpuTriangleBvh):
         n = _check_bounds(leaf_bounds, self.ctx)
         t = _lib.BuildTimings() if timings else None
         handle = ctypes.c_void_p(self.handle.value)
-        opts = build_options(self.node_width, self.leaf_size, self.method)
+        opts = build_options(self.node_width, self.leaf_size, self.method, self.merging_threshold)
         _lib.check(_lib.lib().bajo_build_binary_bounds(self.ctx.handle, ptr(leaf_bounds), n, ctypes.byref(opts),
                                                        ctypes.byref(t) if t else None, ctypes.byref(handle)))
         self._handle = handle

@@ -74,6 +74,7 @@ struct bajo_bvh {
   bool root_pending;    // root not read back yet
   // H-PLOC (method BAJO_BUILD_HPLOC): guide LBVH and cluster workspace
   int method;
+  uint32_t hp_threshold;  // merging threshold (16 or 32)
   BjNode* hp_guide;       // n-1 guide nodes (Karras over the sorted keys)
   uint32_t* hp_first;     // n-1 first sorted position of each guide node
   void* hp_clusters;      // n cluster entries of 32 bytes

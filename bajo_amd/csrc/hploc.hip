@@ -5,8 +5,9 @@
 // CPU definition (bvh/cpu/builder/hploc.mojo:450-663): the LBVH over the
 // sorted Morton keys guides a bottom-up pass; the second thread to arrive at
 // a guide node concatenates the cluster lists of its two halves (left then
-// right) and, while the list holds more than the merging threshold (16; 1 at
-// the root), runs merge rounds: each cluster picks the neighbour within
+// right) and, while the list holds more than the merging threshold (16 as in
+// bajo's CPU definition, or 32 as in its GPU build on wave64; 1 at the root),
+// runs merge rounds: each cluster picks the neighbour within
 // radius 8 whose merged box has the least surface area (right before left,
 // nearer first, strict '<'), mutual pairs merge into new nodes in list
 // order, and the list is compacted in order.
@@ -18,7 +19,7 @@
 //     with write-through stores and read with coherent loads, so no L2
 //     writeback or invalidate is needed across the 8 XCDs;
 //   - a guide node that needs merging is processed by its whole wave: up to
-//     32 clusters sit one per lane, neighbour boxes move by lane shuffles,
+//     2 x threshold (64) clusters sit one per lane, neighbour boxes move by lane shuffles,
 //     mutual pairs are found with one ballot, a new node takes the index
 //     (retired leaf position - 1) of the larger lowest leaf position of its
 //     two clusters (no shared counter), and the list is compacted with a
@@ -43,7 +44,10 @@ namespace {
 
 constexpr int kHpBlock = 256;
 constexpr int kHpRadius = 8;
-constexpr uint32_t kHpThreshold = 16;
+// merging thresholds: bajo's CPU definition (16) and its GPU build on wave64
+// (WARP_SIZE / 2 = 32, bvh/gpu/builder/hploc_layout.mojo:6); two lists of at
+// most 32 clusters fill the 64 lanes of a wave
+constexpr uint32_t kHpThreshold = 16, kHpThresholdMax = 32;
 constexpr int kHpMaxLoop = 512;  // climb iterations per wave (depth is far smaller)
 // ctl words
 constexpr int kHpLeaves = 0, kHpStatus = 1, kHpRoot = kHpCtlRoot;
@@ -158,7 +162,8 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
                        const uint32_t* __restrict__ range_first, HpEntry* __restrict__ clusters,
                        uint32_t* __restrict__ ids, uint32_t* __restrict__ counts, BjNode* __restrict__ nodes, uint32_t* __restrict__ leaf_counts,
                        uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ ctl,
-                       const uint64_t* __restrict__ seg_off, uint32_t S, uint32_t* __restrict__ seg_roots) {
+                       const uint64_t* __restrict__ seg_off, uint32_t S, uint32_t* __restrict__ seg_roots,
+                       uint32_t threshold) {
   const uint32_t lane = lane_id();
   const uint32_t first = range_first[g], last = guide[g].fence;
   const uint32_t gl = guide[g].left, gr = guide[g].right;
@@ -166,14 +171,14 @@ BJ_D void process_node(uint32_t g, uint64_t n, const BjNode* __restrict__ guide,
   const uint32_t cl = (gl & kLeafFlag) ? 1u : load_co(counts + split);
   const uint32_t cr = (gr & kLeafFlag) ? 1u : load_co(counts + (gr & kIndexMask));
   uint32_t c = cl + cr;
-  if (cl > kHpThreshold || cr > kHpThreshold || cl == 0 || cr == 0 || first > split || split >= last ||
+  if (cl > threshold || cr > threshold || cl == 0 || cr == 0 || first > split || split >= last ||
       (uint64_t)last >= n || cl > split - first + 1 || cr > last - split) {
     if (lane == 0) atomicOr(ctl + kHpStatus, kHpStatusBadList);
     return;
   }
   // the hierarchy's (or the segment's) root: no parent in the guide
   const bool root = guide[g].parent == kSentinel;
-  const uint32_t thr = root ? 1u : kHpThreshold;
+  const uint32_t thr = root ? 1u : threshold;
   if (c <= thr) {
     // no merging: the list is the left list (already at `first`) followed by
     // the right list, which moves only if the left one is shorter than its range
@@ -313,7 +318,7 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
                                                          uint32_t* __restrict__ ctl,
                                                          const uint32_t* __restrict__ gprim,
                                                          const uint64_t* __restrict__ seg_off, uint32_t S,
-                                                         uint32_t* __restrict__ seg_roots) {
+                                                         uint32_t* __restrict__ seg_roots, uint32_t threshold) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool active = i < n;
   bool pending = false;
@@ -355,9 +360,9 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
       const uint32_t cl = (gl & kLeafFlag) ? 1u : load_co(counts + split);
       const uint32_t cr = (gr & kLeafFlag) ? 1u : load_co(counts + (gr & kIndexMask));
       const bool root = guide[g].parent == kSentinel;
-      const bool sane = cl >= 1 && cr >= 1 && cl <= kHpThreshold && cr <= kHpThreshold && first <= split &&
+      const bool sane = cl >= 1 && cr >= 1 && cl <= threshold && cr <= threshold && first <= split &&
                         split < last && (uint64_t)last < n && cl <= split - first + 1 && cr <= last - split;
-      if (sane && !root && cl + cr <= kHpThreshold) {
+      if (sane && !root && cl + cr <= threshold) {
         if (split + 1 != first + cl)
           for (uint32_t k = 0; k < cr; ++k) {
             store_entry(clusters + first + cl + k, load_entry(clusters + split + 1 + k));
@@ -374,7 +379,7 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
       todo &= todo - 1;
       const uint32_t g = shfl_u(cur, leader);
       process_node(g, n, guide, range_first, clusters, ids, counts, nodes, leaf_counts, leaf_parent, ctl, seg_off, S,
-                   seg_roots);
+                   seg_roots, threshold);
       if ((int)lane_id() == leader) {
         pending = false;
         if (guide[g].parent == kSentinel) active = false;
@@ -555,6 +560,7 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
   hipStream_t st = ctx->stream;
   const uint64_t n = h->n;
   if (n < 2) return BAJO_OK;  // a single leaf is its own root; the LBVH path packed it
+  const uint32_t threshold = h->hp_threshold ? h->hp_threshold : kHpThreshold;
   HpEntry* clusters = reinterpret_cast<HpEntry*>(h->hp_clusters);
   hploc_init_kernel<<<1, 64, 0, st>>>(h->hp_ctl);
   karras_kernel<<<div_up(n - 1, kHpBlock), kHpBlock, 0, st>>>(h->keys, n, h->hp_guide, h->leaf_parent,
@@ -564,12 +570,12 @@ int32_t build_hploc(bajo_ctx* ctx, bajo_bvh* h, const float* vertices) {
     hploc_kernel<true><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
                                                                  h->leaf_parent, h->flags, clusters, h->hp_pos,
                                                                  h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl,
-                                                                 nullptr, nullptr, 0u, nullptr);
+                                                                 nullptr, nullptr, 0u, nullptr, threshold);
   else
     hploc_kernel<false><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
                                                                   h->leaf_parent, h->flags, clusters, h->hp_pos,
                                                                   h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl,
-                                                                  nullptr, nullptr, 0u, nullptr);
+                                                                  nullptr, nullptr, 0u, nullptr, threshold);
   hploc_root_kernel<<<1, 64, 0, st>>>(h->nodes, n, h->leaf_counts, h->hp_ctl, ctx->status);
   BJ_HIP(hipGetLastError());
   // the cluster area is free now: it holds the reordered keys and leaf parents
@@ -605,6 +611,7 @@ int32_t build_hploc_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices)
   const uint64_t n = h->n;
   const uint32_t S = h->seg_count;
   if (n < 2 || n == S) return BAJO_OK;  // only one-triangle segments: packed by the caller
+  const uint32_t threshold = h->hp_threshold ? h->hp_threshold : kHpThreshold;
   HpEntry* clusters = reinterpret_cast<HpEntry*>(h->hp_clusters);
   uint32_t* seg_roots = h->seg_acc;  // free after the Morton codes: u32[S] of the 12 S words
   hploc_init_kernel<<<1, 64, 0, st>>>(h->hp_ctl);
@@ -612,7 +619,7 @@ int32_t build_hploc_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices)
   hploc_kernel<false><<<div_up(n, kHpBlock), kHpBlock, 0, st>>>(vertices, h->keys, n, h->hp_guide, h->hp_first,
                                                                 h->leaf_parent, h->flags, clusters, h->hp_pos,
                                                                 h->hp_counts, h->nodes, h->leaf_counts, h->hp_ctl,
-                                                                h->seg_prim, h->seg_off, S, seg_roots);
+                                                                h->seg_prim, h->seg_off, S, seg_roots, threshold);
   seg_root_fix_kernel<<<div_up(S, kHpBlock), kHpBlock, 0, st>>>(h->nodes, h->leaf_counts, h->leaf_parent, h->seg_off,
                                                                S, seg_roots, h->hp_ctl);
   BJ_HIP(hipGetLastError());

@@ -988,7 +988,8 @@ int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo
   if (opt.leaf_size == 0) opt.leaf_size = (int32_t)kWideDefaultLeaf;
   BJ_REQUIRE(opt.leaf_size >= 1 && opt.leaf_size <= (int32_t)kWideMaxLeaf,
              "%s: leaf_size must be in [1, 8], got %d", fn, opt.leaf_size);
-  BJ_REQUIRE(opt.reserved == 0, "%s: reserved option field must be 0", fn);
+  BJ_REQUIRE(opt.merging_threshold == 0 || opt.merging_threshold == 16 || opt.merging_threshold == 32,
+             "%s: merging_threshold must be 0, 16 or 32, got %d", fn, opt.merging_threshold);
   BJ_REQUIRE(n > 0, "%s: a BVH needs nonempty input", fn);
   BJ_REQUIRE(vertices != nullptr, "%s: input array is NULL", fn);
   BJ_REQUIRE(n <= bj::kSortMaxKeys, "%s: n=%llu exceeds 2^30-1 %s", fn,
@@ -1008,6 +1009,7 @@ int32_t build_entry(bajo_ctx* ctx, const float* vertices, uint64_t n, const bajo
   BJ_REQUIRE(fresh || (!h->owner && !h->seg_count), "%s: a BLAS set or its segment view cannot be rebuilt here", fn);
   h->node_width = opt.node_width;
   h->method = opt.method;
+  h->hp_threshold = opt.merging_threshold ? (uint32_t)opt.merging_threshold : 16u;
   h->wide_leaf = (uint32_t)opt.leaf_size;
   h->boxes = boxes;
   int32_t rc = bj::build_into(ctx, h, vertices, n, timings);
@@ -1177,7 +1179,8 @@ int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const
              "%s: node_width must be 2, 4 or 8, got %d", fn, opt.node_width);
   if (opt.leaf_size == 0) opt.leaf_size = (int32_t)kWideDefaultLeaf;
   BJ_REQUIRE(opt.leaf_size >= 1 && opt.leaf_size <= (int32_t)kWideMaxLeaf, "%s: leaf_size must be in [1, 8]", fn);
-  BJ_REQUIRE(opt.reserved == 0, "%s: reserved option field must be 0", fn);
+  BJ_REQUIRE(opt.merging_threshold == 0 || opt.merging_threshold == 16 || opt.merging_threshold == 32,
+             "%s: merging_threshold must be 0, 16 or 32, got %d", fn, opt.merging_threshold);
   BJ_REQUIRE(segment_offsets[0] == 0, "%s: segment_offsets[0] must be 0", fn);
   for (uint32_t s = 0; s < segment_count; ++s)
     BJ_REQUIRE(segment_offsets[s + 1] > segment_offsets[s], "%s: segment %u is empty (offsets must increase)", fn, s);
@@ -1201,6 +1204,7 @@ int32_t bajo_build_triangle_blas_set(bajo_ctx* ctx, const float* vertices, const
   bajo_bvh* h = set->store;
   h->node_width = opt.node_width;
   h->method = opt.method;
+  h->hp_threshold = opt.merging_threshold ? (uint32_t)opt.merging_threshold : 16u;
   h->wide_leaf = (uint32_t)opt.leaf_size;
   int32_t rc = build_segmented(ctx, h, vertices, segment_offsets, segment_count, timings);
   if (rc) {

@@ -216,6 +216,19 @@ def soup_grid_workload(ctx, rows: int = 64, cols: int = 64, ao_samples: int = 4)
     occluded = float(((o[:, 2].view(torch.int32) == -1) & (o[:, 7] == 0)).sum()) / max(live, 1)
     print(f"[bench] soup grid traces: closest {closest_ms:.2f} ms, AO {ao_ms:.2f} ms", file=sys.stderr, flush=True)
     bvh.free()
+    # H-PLOC at merging threshold 32 (bajo's GPU build on wave64: lists of up to 64 clusters per wave)
+    b32 = build_triangle_bvh(v, ctx=ctx, node_width=8, leaf_size=4, method=1, merging_threshold=32)
+    build32_ms = _timed(lambda: b32.rebuild(v), stream)
+    stages32 = b32.rebuild(v, timings=True)
+    st32 = torch.empty(5 * R, dtype=torch.int32, device=dev)
+    b32.launch_rays(rays, hits, R, stats=st32)
+    s32 = st32.view(-1, 5).to(torch.float64)
+    visits32, tests32 = float(s32[:, 0].mean()), float(s32[:, 3].mean())
+    del st32, s32
+    closest32_ms = _timed(lambda: b32.launch_rays(rays, hits, R), stream)
+    b32.free()
+    print(f"[bench] soup grid H-PLOC threshold 32: build {build32_ms:.2f} ms, closest {closest32_ms:.2f} ms",
+          file=sys.stderr, flush=True)
     node_bytes = NODE_BYTES[8]
     return {
         "workload": f"{rows}x{cols} grid of 4096-triangle soups (PCG32 streams), H-PLOC radius 8 + 8-wide "
@@ -244,6 +257,15 @@ def soup_grid_workload(ctx, rows: int = 64, cols: int = 64, ao_samples: int = 4)
             "note": "rays of primary misses carry t_max = -1 and count in the launch but not in mrays_s",
         },
         "rays_def": "origins uniform in the scene box, directions uniform on the sphere",
+        "hploc_threshold_32": {
+            "build_ms": build32_ms,
+            "build_mtris_s": n / (build32_ms / 1e3) / 1e6,
+            "build_stages_ms": {k: val / 1e6 for k, val in stages32.items()},
+            "closest_ms": closest32_ms,
+            "closest_mrays_s": R / (closest32_ms / 1e3) / 1e6,
+            "node_visits_per_ray": visits32,
+            "tri_tests_per_ray": tests32,
+        },
     }
 
 
@@ -882,6 +904,7 @@ def main() -> int:
                     help="run only the batched-environment workload and print its JSON")
     ap.add_argument("--target-only", action="store_true", help="run only the render-target workload")
     ap.add_argument("--guided-only", action="store_true", help="run only the learned-guiding room workload")
+    ap.add_argument("--soup-grid-only", action="store_true", help="run only the 16.7M soup-grid H-PLOC workload")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guiding room workload")
     ap.add_argument("--batched-render-only", action="store_true",
                     help="run only the batched render (without its autotuner and per-environment loop, for "
@@ -928,6 +951,9 @@ def main() -> int:
         return 0
     if args.guided_only:
         print(json.dumps({"guided": guided_workload(ctx)}), flush=True)
+        return 0
+    if args.soup_grid_only:
+        print(json.dumps({"soup_grid": soup_grid_workload(ctx)}), flush=True)
         return 0
     scene = scenes.icosphere_scene(N_SPHERE)
     method = {"lbvh": 0, "hploc": 1}[args.method]

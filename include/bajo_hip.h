@@ -50,8 +50,9 @@ extern "C" {
 /* Build methods, bvh/gpu GpuBvhBuildMethod (bvh/gpu/builder/__init__.mojo). */
 #define BAJO_BUILD_LBVH 0
 /* H-PLOC (bvh/gpu/builder/hploc_multi_wave.mojo:187-777): LBVH-guided
- * bottom-up clustering, search radius 8, merging threshold 16, with the
- * topology of bajo's CPU definition (bvh/cpu/builder/hploc.mojo:855-917).
+ * bottom-up clustering, search radius 8, merging threshold 16 (or 32,
+ * bajo_build_options.merging_threshold), with the topology of bajo's CPU
+ * definition (bvh/cpu/builder/hploc.mojo:855-917) at that threshold.
  * Leaves are then renumbered depth-first, so in the exported binary layout
  * every subtree covers a contiguous range of positions and `fence` is the
  * last one (bajo leaves FENCE unset for H-PLOC).  The root stays on the
@@ -76,7 +77,9 @@ typedef struct {
   int32_t method;     /* BAJO_BUILD_LBVH or BAJO_BUILD_HPLOC               */
   int32_t node_width; /* BAJO_WIDTH_BINARY, BAJO_WIDTH_WIDE4 or _WIDE8      */
   int32_t leaf_size;  /* width 8: most triangles per leaf child, 1..8; 0 = 4 */
-  int32_t reserved;   /* must be 0                                         */
+  int32_t merging_threshold; /* H-PLOC: 0 or 16 = bajo's CPU definition (bvh/cpu/builder/
+                                hploc.mojo:855-917), 32 = its GPU build on wave64 (WARP_SIZE / 2,
+                                bvh/gpu/builder/hploc_layout.mojo:6); LBVH ignores it */
 } bajo_build_options;
 
 typedef struct bajo_ctx bajo_ctx;

@@ -727,19 +727,20 @@ def hploc_binary(t: dict, numbering: np.ndarray | None = None) -> dict:
     return out
 
 
-def build_hploc(vertices: np.ndarray) -> dict:
+def build_hploc(vertices: np.ndarray, merging_threshold: int = HPLOC_MERGING_THRESHOLD) -> dict:
     """Oracle H-PLOC over a triangle soup: the LBVH's sorted keys, bajo's CPU
-    topology, and its depth-first binary layout (usable by trace_bvh2 and
+    topology at the given merging threshold (16, or 32 as bajo's GPU build on
+    wave64), and its depth-first binary layout (usable by trace_bvh2 and
     collapse_wide)."""
-    return build_hploc_bounds(triangle_bounds(f32(vertices)))
+    return build_hploc_bounds(triangle_bounds(f32(vertices)), merging_threshold)
 
 
-def build_hploc_bounds(leaf_bounds: np.ndarray) -> dict:
+def build_hploc_bounds(leaf_bounds: np.ndarray, merging_threshold: int = HPLOC_MERGING_THRESHOLD) -> dict:
     """Oracle H-PLOC over leaf boxes f32[6N]."""
     bounds = f32(leaf_bounds)
     lb = build_lbvh_bounds(bounds)
     codes = (lb["sorted_keys"] >> np.uint64(32)).astype(np.uint32)
-    t = hploc_topology(bounds, codes, lb["leaf_ids"])
+    t = hploc_topology(bounds, codes, lb["leaf_ids"], merging_threshold=merging_threshold)
     b = hploc_binary(t, hploc_lowest_leaf_numbering(t, lb["leaf_ids"]))
     b["scene_bounds"] = lb["scene_bounds"]
     b["topology"] = t

@@ -160,7 +160,7 @@ uint32_t orc_hploc_topology(const float *leaf_bounds6, const uint32_t *sorted_co
                             uint64_t n, int search_radius, int merging_threshold, uint32_t *parent, uint32_t *left,
                             uint32_t *right, uint32_t *leaf_id, uint32_t *leaf_count, float *bounds6,
                             uint64_t *stats6) {
-    if (n == 0 || search_radius < 1 || merging_threshold < 1 || merging_threshold > 16) return ORC_SENTINEL;
+    if (n == 0 || search_radius < 1 || merging_threshold < 1 || merging_threshold > 32) return ORC_SENTINEL;
     hp_state s;
     memset(&s, 0, sizeof(s));
     s.codes = sorted_codes; s.ids = sorted_ids; s.n = n;

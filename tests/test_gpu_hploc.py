@@ -85,11 +85,12 @@ def check_binary(b, v):
     assert seen.all()
 
 
-def device_hploc(v, node_width=2):
+def device_hploc(v, node_width=2, merging_threshold=0):
     from bajo_amd.bvh import build_triangle_bvh
 
     t = {}
-    bvh = build_triangle_bvh(to_dev(v), timings=t, node_width=node_width, method=HPLOC)
+    bvh = build_triangle_bvh(to_dev(v), timings=t, node_width=node_width, method=HPLOC,
+                             merging_threshold=merging_threshold)
     b = bvh.binary()
     return bvh, b, t
 
@@ -130,6 +131,41 @@ def test_hploc_topology_equals_reference(gpu_ctx, name, n):
             np.testing.assert_array_equal(got["sorted_keys"], (code_of[ids.astype(np.int64)] << np.uint64(32)) | ids)
     print(name, {k: val / 1e6 for k, val in t.items()})
     bvh.free()
+
+
+@pytest.mark.parametrize("name,n", [("bajo-257", 257), ("soup-65536", 65536), ("icosphere-1M", 1 << 20)])
+def test_hploc_threshold32_equals_reference(gpu_ctx, name, n):
+    """Merging threshold 32 (bajo's GPU build on wave64): two lists of up to 32 clusters fill the
+    wave; the layout equals the oracle's at the same threshold word for word."""
+    announce(f"hploc threshold 32 {name} n={n}")
+    if name.startswith("bajo"):
+        v = bajo_triangles(n)
+    elif name.startswith("icosphere"):
+        from bajo_amd import scenes
+
+        v = scenes.icosphere(n, frequency=230)
+    else:
+        v = soup(n, n)
+    bvh, got, t = device_hploc(v, merging_threshold=32)
+    want = oracle.build_hploc(v, merging_threshold=32)
+    if n <= 100_000:
+        check_binary(got, v)
+    assert got["root"] == want["root"]
+    for key in ("node_meta", "leaf_ids", "leaf_counts", "leaf_parent"):
+        np.testing.assert_array_equal(got[key], want[key], err_msg=key)
+    np.testing.assert_array_equal(got["node_bounds"].view(np.uint32), want["node_bounds"].view(np.uint32))
+    # a different tree from threshold 16's
+    sixteen = oracle.build_hploc(v)
+    assert not np.array_equal(sixteen["node_meta"], want["node_meta"])
+    print(name, "threshold 32", {k: val / 1e6 for k, val in t.items()})
+    bvh.free()
+
+
+def test_hploc_threshold_is_validated(gpu_ctx):
+    from bajo_amd.bvh import build_triangle_bvh
+
+    with pytest.raises(RuntimeError, match="merging_threshold"):
+        build_triangle_bvh(to_dev(soup(100, 1)), method=HPLOC, merging_threshold=24)
 
 
 def test_hploc_quality_not_worse_than_lbvh(gpu_ctx):

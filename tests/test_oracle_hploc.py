@@ -141,7 +141,7 @@ def test_quality_gate_against_lbvh():
 
 
 @pytest.mark.parametrize("n,seed,thr", [(1, 0, 16), (2, 1, 16), (3, 2, 2), (17, 3, 16), (33, 4, 16), (500, 5, 16),
-                                        (500, 6, 4), (4000, 7, 16)])
+                                        (500, 6, 4), (4000, 7, 16), (65, 8, 32), (4000, 9, 32)])
 def test_invariants_on_soups(n, seed, thr):
     v = soup(n, seed=seed)
     lb = oracle.build_lbvh(v)
@@ -216,3 +216,14 @@ def test_lowest_leaf_numbering(n, seed):
     np.testing.assert_array_equal(by_position["node_bounds"].reshape(-1, 12)[relabel],
                                   by_creation["node_bounds"].reshape(-1, 12))
     np.testing.assert_array_equal(by_position["leaf_parent"], relabel[by_creation["leaf_parent"]])
+
+
+def test_threshold_32_is_the_limit():
+    """Lists of two halves at threshold 32 fill 64 slots (one wave); 33 is refused."""
+    v = soup(200, seed=11)
+    lb = oracle.build_lbvh(v)
+    codes = (lb["sorted_keys"] >> np.uint64(32)).astype(np.uint32)
+    t = oracle.hploc_topology(oracle.triangle_bounds(v), codes, lb["leaf_ids"], merging_threshold=32)
+    validate(t, 200)
+    with pytest.raises(RuntimeError):
+        oracle.hploc_topology(oracle.triangle_bounds(v), codes, lb["leaf_ids"], merging_threshold=33)

@@ -905,6 +905,8 @@ def main() -> int:
     ap.add_argument("--target-only", action="store_true", help="run only the render-target workload")
     ap.add_argument("--guided-only", action="store_true", help="run only the learned-guiding room workload")
     ap.add_argument("--soup-grid-only", action="store_true", help="run only the 16.7M soup-grid H-PLOC workload")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline step (build + NEE + PATH renders) and its line: for kernel profiles")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guiding room workload")
     ap.add_argument("--batched-render-only", action="store_true",
                     help="run only the batched render (without its autotuner and per-environment loop, for "
@@ -1085,6 +1087,10 @@ def main() -> int:
         "nee_image_mean_rgb": img,
         "build_stages_ms": {k: v / 1e6 for k, v in stages.items()},
     }
+    if args.headline_only:
+        gscene.free()
+        print(json.dumps(line), flush=True)
+        return 0
     line["incoherent_trace"] = incoherent_trace(scene, ctx, args.node_width, args.leaf_size, method)
     gscene.free()
     if not args.no_guided:

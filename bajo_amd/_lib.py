@@ -106,7 +106,7 @@ class RenderSettings(ctypes.Structure):
         ("shadow_walk", c_uint32),
         ("sparse_shadow_rays", c_uint32),
         ("shadow_group", c_uint32),
-        ("reserved", c_uint32),
+        ("tail_group", c_uint32),
     ]
 
 

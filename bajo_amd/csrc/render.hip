@@ -66,6 +66,11 @@ constexpr uint32_t kSparseShadowRays = 8192;
 // lane per ray is fastest.
 constexpr uint32_t kDefaultShadowGroup = 16;
 constexpr uint32_t kGroupShadowRays = 1u << 20;
+// Lanes per path of the tail when the settings leave it at 0: 16 up to
+// kGroupTailPaths live paths (the tail takes over at a few thousand), one lane
+// per path above.
+constexpr uint32_t kDefaultTailGroup = 16;
+constexpr uint32_t kGroupTailPaths = 1u << 15;
 constexpr int kCounterWords = 16;
 constexpr uint32_t kStatusPathOverflow = 1, kStatusShadowOverflow = 3;
 constexpr int kMaxBounces = 64;
@@ -836,7 +841,7 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(k
 // shadow, advance and tail launches of the chunk do nothing.
 template <int kAlg, bool kWide, int kGeo>
 __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
-                                                        uint32_t depth, uint32_t threshold, uint32_t simds,
+                                                        uint32_t depth, uint32_t lo, uint32_t threshold, uint32_t simds,
                                                         unsigned long long* __restrict__ stats,
                                                         uint32_t* __restrict__ spill) {
   __shared__ uint32_t s_stack[kLdsStack * kRtBlock];
@@ -844,7 +849,7 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
   if (tag != 0 && tag != bounce + 1u) return;
   const uint64_t cap = q.capacity;
   const uint32_t active = q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap;
-  if (active > threshold) return;
+  if (active <= lo || active > threshold) return;  // (lo: the lane-group tail takes those)
   if (blockIdx.x == 0 && threadIdx.x == 0) q.counters[kTailDone] = bounce + 1u;
   const uint32_t lane = lane_id();
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -977,6 +982,174 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
       if (!ballot(alive || sh_live)) break;
     }
     if (mine && is_path) {
+#pragma unroll
+      for (int p = 0; p < kSamplePlanes; ++p) q.sample[plane_at<kSamplePlanes>(slot, p)] = acc[p];
+    }
+  }
+  wave_add(stats + kWorkBase + 0, (uint32_t)ext_visits);
+  wave_add(stats + kWorkBase + 1, (uint32_t)ext_leaves);
+  wave_add(stats + kWorkBase + 2, (uint32_t)sh_visits);
+  wave_add(stats + kWorkBase + 3, (uint32_t)sh_leaves);
+}
+
+// The tail with G lanes per path (triangle and batch scenes, 4- and 8-wide):
+// each group carries one path through the remaining bounces -- its extension
+// ray by GroupClosest (Walk8 on the group's leader for the rare near ties),
+// shading on the leader, then the bounce's shadow ray by GroupWalk -- so the
+// sample receives the wavefront's additions in its order (extension, then
+// shadow, bounce by bounce) and images equal the per-lane tail's.  The last
+// few thousand paths of a chunk walk long serial chains (over 100 node
+// visits for the slowest extension and shadow rays); a group shortens each
+// step's wait to one round of node loads and one of records.  Runs when the
+// chunk's live paths lie in (lo, hi]; the per-lane tail takes the others.
+template <int kAlg, int kGeo, int G>
+__global__ void __launch_bounds__(kRtBlock) tail_group_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
+                                                              uint32_t depth, uint32_t lo, uint32_t hi,
+                                                              unsigned long long* __restrict__ stats) {
+  constexpr int kGroups = kRtBlock / G;
+  static_assert(GroupWalk<G>::kStack <= 2 * GroupClosest<G>::kStack, "the walks share one LDS stack area");
+  __shared__ uint32_t s_stack[kGroups][2 * GroupClosest<G>::kStack];
+  __shared__ uint32_t s_recs[kGroups][GroupWalk<G>::kRecs];
+  const uint32_t tag = q.counters[kTailDone];
+  if (tag != 0 && tag != bounce + 1u) return;
+  const uint64_t cap = q.capacity;
+  const uint32_t active = q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap;
+  if (active <= lo || active > hi) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) q.counters[kTailDone] = bounce + 1u;
+  const uint32_t lane = lane_id(), grp = threadIdx.x / G, gl = lane % G;
+  const int lead = (int)((lane / G) * G);  // the group's leader lane in the wave
+  const bool leader = gl == 0;
+  uint32_t* stack = s_stack[grp];
+  uint32_t* recs = s_recs[grp];
+  const uint32_t groups = gridDim.x * kGroups;
+  uint64_t ext_visits = 0, ext_leaves = 0, sh_visits = 0, sh_leaves = 0;
+  // each wave takes 64 / G consecutive paths at a time
+  for (uint32_t g0 = (blockIdx.x * kRtBlock + (threadIdx.x & ~63u)) / G; g0 < active; g0 += groups) {
+    const uint32_t idx = g0 + lane / G;
+    bool alive = idx < active;
+    uint32_t path_id = 0, slot = 0;
+    bool delta = false;
+    float f[kPathPlanes];
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int p = 0; p < kPathPlanes; ++p) f[p] = 0.0f;
+    if (alive) {
+      const uint32_t idw = q.src_ids[idx];
+      path_id = idw & kPathIdMask;
+      delta = (idw & kPathDelta) != 0;
+#pragma unroll
+      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(idx, p)];
+      slot = path_id - q.sample_base;
+      if (leader) {
+#pragma unroll
+        for (int p = 0; p < kSamplePlanes; ++p) acc[p] = q.sample[plane_at<kSamplePlanes>(slot, p)];
+      }
+    }
+    WideView wv = sc.bvh.wide;
+    uint32_t first = 0;
+    if constexpr (kGeo == kGeoBatch) {
+      const uint32_t env = path_id / sc.env_paths;
+      wv.root = env;  // segment roots lead the node array
+      first = alive ? (uint32_t)sc.seg_off[env] : 0u;
+    }
+    for (uint32_t b = bounce; b < depth; ++b) {
+      if (b < kMaxBounces) wave_add(stats + 2 * b, (leader && alive) ? 1u : 0u);
+      // the extension ray
+      const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
+      GroupClosest<G> gc;
+      uint32_t ev = 0, et = 0;
+      if (alive) gc.begin(wv, ray, stack);
+      bool walking = alive;
+      while (__ballot(walking) != 0) {
+        if (walking && gc.step(wv, stack, recs, ev, et)) walking = false;
+      }
+      Hit h = miss(ray.tmax);
+      if (alive && leader) {
+        if (gc.near_tie()) {
+          // Walk8 decides; its stack lives in the group's (now free) LDS area
+          Stack fst{stack, nullptr, 0, 0, kMaxStack, 1};
+          TraceCounters tc;
+          h = traverse8<BAJO_TRACE_CLOSEST_HIT, true>(wv, ray, fst, &tc);
+          ev += tc.visits;
+          et += tc.leaves;
+        } else {
+          h = gc.finish(wv);
+        }
+        if constexpr (kGeo == kGeoBatch)
+          if (h.prim != kEmpty) h.prim += first;
+      }
+      if (leader) {
+        ext_visits += ev;
+        ext_leaves += et;
+      }
+      if (b < kMaxBounces) wave_max(stats + kMaxVisitBase + 2 * b, leader ? ev : 0u);
+      // shading on the leader
+      bool cast = false, next = false;
+      float sf[kShadowPlanes];
+#pragma unroll
+      for (int p = 0; p < kShadowPlanes; ++p) sf[p] = 0.0f;
+      if (alive && leader) {
+        const v3 T = mk(f[8], f[9], f[10]);
+        const Surface su = surface_of(sc, ray, h);
+        ShadowWork sw;
+        if (shade_shadow<kAlg>(sc, seed, b, path_id, T, ray.d, su, sw)) {
+          cast = true;
+#pragma unroll
+          for (int p = 0; p < kShadowPlanes; ++p) sf[p] = sw.f[p];
+        }
+        v3 add_c;
+        NextPath np;
+        const uint32_t sfl = shade_bounce<kAlg>(sc, seed, b, path_id, delta, ray, T, f[11], h.t, su, add_c, np);
+        if (sfl & kShadeAdd) {
+          acc[0] += add_c.x;
+          acc[1] += add_c.y;
+          acc[2] += add_c.z;
+        }
+        next = (sfl & kShadeNext) != 0;
+        if (next) {
+#pragma unroll
+          for (int p = 0; p < kPathPlanes; ++p) f[p] = np.f[p];
+          delta = (np.id & kPathDelta) != 0;
+        }
+      }
+      if (b < kMaxBounces) wave_add(stats + 2 * b + 1, (leader && cast) ? 1u : 0u);
+      // the bounce's shadow ray, traced by the whole group
+      cast = __shfl(cast, lead, 64);
+#pragma unroll
+      for (int p = 0; p < kShadowPlanes; ++p) sf[p] = __shfl(sf[p], lead, 64);
+      GroupWalk<G> gw;
+      uint32_t sv = 0, stt = 0;
+      bool occluded = false;
+      if (cast) gw.begin(wv, Ray{mk(sf[0], sf[1], sf[2]), mk(sf[4], sf[5], sf[6]), sf[3], sf[7]}, stack);
+      bool shadowing = cast;
+      while (__ballot(shadowing) != 0) {
+        if (shadowing) {
+          bool occ = false;
+          if (gw.step(wv, stack, recs, sv, stt, occ)) {
+            shadowing = false;
+            occluded = occ;
+          }
+        }
+      }
+      if (cast && leader && (kAlg == BAJO_RENDER_AO || !occluded)) {
+        const v3 cs = shadow_radiance<kAlg>(occluded, sf);
+        acc[0] += cs.x;
+        acc[1] += cs.y;
+        acc[2] += cs.z;
+      }
+      if (leader) {
+        sh_visits += sv;
+        sh_leaves += stt;
+      }
+      if (b < kMaxBounces) wave_max(stats + kMaxVisitBase + 2 * b + 1, leader ? sv : 0u);
+      // the next segment, from the leader (a path still going after the last bounce ends)
+      alive = __shfl(next, lead, 64) && b + 1 < depth;
+#pragma unroll
+      for (int p = 0; p < kPathPlanes; ++p) f[p] = __shfl(f[p], lead, 64);
+      delta = __shfl(delta, lead, 64);
+      if (__ballot(alive) == 0) break;
+    }
+    if (idx < active && leader) {
 #pragma unroll
       for (int p = 0; p < kSamplePlanes; ++p) q.sample[plane_at<kSamplePlanes>(slot, p)] = acc[p];
     }
@@ -1477,6 +1650,14 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   BJ_REQUIRE(st->shadow_group == 0 || st->shadow_group == 1 || st->shadow_group == 8 || st->shadow_group == 16 ||
                  st->shadow_group == 32,
              "bajo_render: shadow_group must be 0, 1, 8, 16 or 32 (got %u)", st->shadow_group);
+  // the tail with 16 lanes per path (triangle and batch scenes, 4- and 8-wide)
+  BJ_REQUIRE(st->tail_group == 0 || st->tail_group == 1 || st->tail_group == 16,
+             "bajo_render: tail_group must be 0, 1 or 16 (got %u)", st->tail_group);
+  constexpr bool kGroupTail = kWide && kGeo != kGeoMixed;
+  const uint32_t tail_group = st->tail_group ? st->tail_group : kDefaultTailGroup;
+  uint32_t group_tail_hi = 0;
+  if (kGroupTail && tail_group == 16)
+    group_tail_hi = st->tail_group ? tail_threshold : (kGroupTailPaths < tail_threshold ? kGroupTailPaths : tail_threshold);
   const bool adaptive_group = st->shadow_group == 0;
   const uint32_t shadow_group = adaptive_group ? kDefaultShadowGroup : st->shadow_group;
   // a resident grid for the lane-group kernel (six 256-thread blocks per CU)
@@ -1486,6 +1667,8 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   unsigned g_group = 6u * (unsigned)cu_count;
   if (g_group > g_sh) g_group = g_sh;
   if (g_group < 1) g_group = 1;
+  // the lane-group tail: four 256-thread blocks per CU (40 KB of LDS each)
+  const unsigned g_tgroup = 4u * (unsigned)cu_count;
   rc = ensure_arena(a, cap, shape.spill_threads);
   if (rc) return rc;
   hipStream_t stream = ctx->stream;
@@ -1552,8 +1735,17 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
       if (ev) { ev->ext.push_back(ev->make()); (void)hipEventRecord(ev->ext.back(), stream); }
       if constexpr (kTail) {
         if (b > 0) {
-          tail_kernel<kAlg, kWide, kGeo><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, tail_threshold,
-                                                                    simds, a.stats, a.spill);
+          // lane groups up to group_tail_hi live paths, one lane per path above
+          uint32_t lo = 0;
+          if constexpr (kGroupTail) {
+            if (group_tail_hi) {
+              tail_group_kernel<kAlg, kGroupTail ? kGeo : kGeoTriangles, 16><<<g_tgroup, kRtBlock, 0, stream>>>(
+                  sv, q, st->rng_seed, b, depth, 0u, group_tail_hi, a.stats);
+              lo = group_tail_hi;
+            }
+          }
+          tail_kernel<kAlg, kWide, kGeo><<<g_tail, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, depth, lo,
+                                                                    tail_threshold, simds, a.stats, a.spill);
           BJ_HIP(hipGetLastError());
         }
       }

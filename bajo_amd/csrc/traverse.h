@@ -258,15 +258,16 @@ struct Stack {
   uint32_t* spill;     // this thread's column: entry k at spill[k * spill_stride]
   uint64_t spill_stride;
   int sp;
-  int lds_depth = kLdsStack;  // entries held in LDS; deeper ones spill
+  int lds_depth = kLdsStack;    // entries held in LDS; deeper ones spill
+  int lds_stride = kTraceBlock;  // words between LDS entries (a column per lane of a block, or 1)
   BJ_D void push(uint32_t v) {
-    if (sp < lds_depth) lds[sp * kTraceBlock] = v;
+    if (sp < lds_depth) lds[sp * lds_stride] = v;
     else spill[(uint64_t)(sp - lds_depth) * spill_stride] = v;
     ++sp;
   }
   BJ_D uint32_t pop() {
     --sp;
-    return sp < lds_depth ? lds[sp * kTraceBlock] : spill[(uint64_t)(sp - lds_depth) * spill_stride];
+    return sp < lds_depth ? lds[sp * lds_stride] : spill[(uint64_t)(sp - lds_depth) * spill_stride];
   }
 };
 
@@ -1014,6 +1015,198 @@ struct GroupWalk {
     }
     wave_lds_sync();
     return sp == 0;
+  }
+};
+
+// Closest hit with G lanes per ray (the tail kernel's extension rays), equal
+// to the one-lane Walk8 result.  Walk8 accepts a triangle when its scaled
+// distance beats the best so far (ts < t_best * det), so which of two
+// triangles wins can depend on their test order when their distances lie
+// within rounding of each other; the group visits nodes in another order.
+// So the group keeps the two smallest candidate distances it has seen (t1
+// with its record, t2; candidates pass Walk8's own test against the ray's
+// t_max), culls only boxes entered beyond t1 * (1 + margin),
+// and reports a near tie when t2 <= t1 * (1 + margin); the caller then runs
+// Walk8 for that ray.  Without a near tie every pairwise comparison Walk8
+// makes has the exact outcome (the margin, 4e-6, is ten times the rounding of
+// the scaled test), so Walk8 ends at the unique nearest candidate, which is
+// t1's record; its hit record is formed by testing it again (the same
+// arithmetic that formed it in the walk).
+template <int G>
+struct GroupClosest {
+  static constexpr int kStack = 16 * G;  // entries of (node, entry distance); 18 levels fit, as GroupWalk
+  static constexpr int kRecs = 8 * G;
+  static constexpr float kMargin = 4e-6f;
+  Ray ray;
+  v3 rcp;
+  bool px, py, pz;
+  int sp;
+  uint32_t steps;
+  float t1, t2;
+  uint32_t best;  // record of t1 (kEmpty: no candidate)
+
+  BJ_D uint64_t group_mask() const {
+    const uint32_t lane = lane_id();
+    return (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << ((lane / G) * G);
+  }
+  BJ_D float cull() const { return fminf(ray.tmax, t1 * (1.0f + kMargin)); }
+
+  BJ_D void begin(const WideView& wv, const Ray& r, uint32_t* stack) {
+    ray = r;
+    rcp = rcp_dir(r.d);
+    px = rcp.x >= 0.0f;
+    py = rcp.y >= 0.0f;
+    pz = rcp.z >= 0.0f;
+    steps = 0;
+    t1 = t2 = kFltMax;
+    best = kEmpty;
+    sp = wv.node_count == 0 ? 0 : 1;
+    if (lane_id() % G == 0) {
+      stack[0] = wv.root;
+      stack[1] = __builtin_bit_cast(uint32_t, 0.0f);
+    }
+    wave_lds_sync();
+  }
+
+  // One step; true once the walk is finished (or ended by an error).
+  BJ_D bool step(const WideView& wv, uint32_t* stack, uint32_t* recs, uint32_t& visits, uint32_t& tests) {
+    if (sp == 0) return true;
+    const uint32_t gl = lane_id() % G, j = gl >> 3, i = gl & 7;
+    const uint64_t gm = group_mask();
+    const int nf = sp < G / 8 ? sp : G / 8;
+    const float limit = cull();
+    uint32_t node = 0;
+    bool mine = false;
+    if ((int)j < nf) {
+      node = stack[2 * (sp - 1 - (int)j)];
+      mine = __builtin_bit_cast(float, stack[2 * (sp - 1 - (int)j) + 1]) <= limit;
+    }
+    wave_lds_sync();
+    sp -= nf;
+    visits += (uint32_t)__builtin_popcountll(__ballot(mine && i == 0) & gm);
+    if ((__ballot(mine && node >= wv.node_count) & gm) != 0 || ++steps > wv.node_count) {
+      if (gl == 0) atomicOr(wv.status, (uint32_t)BAJO_STATUS_BAD_NODE);
+      sp = 0;
+      return true;
+    }
+    bool hit = false;
+    uint32_t ref = 0;
+    float tin = 0.0f;
+    if (mine) {
+      const uint4* np = reinterpret_cast<const uint4*>(wv.nodes + node);
+      const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+      ref = reinterpret_cast<const uint32_t*>(wv.nodes + node)[16 + i];
+      const uint32_t meta = w0.w;
+      const uint32_t n = meta >> 24;
+      const float sx = __builtin_bit_cast(float, (meta & 0xFFu) << 23);
+      const float sy = __builtin_bit_cast(float, ((meta >> 8) & 0xFFu) << 23);
+      const float sz = __builtin_bit_cast(float, ((meta >> 16) & 0xFFu) << 23);
+      const float ax = (__builtin_bit_cast(float, w0.x) - ray.o.x) * rcp.x;
+      const float ay = (__builtin_bit_cast(float, w0.y) - ray.o.y) * rcp.y;
+      const float az = (__builtin_bit_cast(float, w0.z) - ray.o.z) * rcp.z;
+      const float bx = sx * rcp.x, by = sy * rcp.y, bz = sz * rcp.z;
+      const bool hiw = i >= 4;
+      const int bt = (int)(i & 3u);
+      const uint32_t lox = hiw ? w1.y : w1.x, loy = hiw ? w1.w : w1.z, loz = hiw ? w2.y : w2.x;
+      const uint32_t hix = hiw ? w2.w : w2.z, hiy = hiw ? w3.y : w3.x, hiz = hiw ? w3.w : w3.z;
+      const uint32_t qnx = px ? lox : hix, qfx = px ? hix : lox;
+      const uint32_t qny = py ? loy : hiy, qfy = py ? hiy : loy;
+      const uint32_t qnz = pz ? loz : hiz, qfz = pz ? hiz : loz;
+      const float tnx = fmaf(ubyte_f(qnx, bt), bx, ax), tfx = fmaf(ubyte_f(qfx, bt), bx, ax);
+      const float tny = fmaf(ubyte_f(qny, bt), by, ay), tfy = fmaf(ubyte_f(qfy, bt), by, ay);
+      const float tnz = fmaf(ubyte_f(qnz, bt), bz, az), tfz = fmaf(ubyte_f(qfz, bt), bz, az);
+      tin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+      const float tout = fminf(fminf(tfx, tfy), fminf(tfz, limit));
+      hit = i < n && tin <= tout;
+    }
+    const bool is_leaf = hit && (ref & kLeafFlag) != 0;
+    const bool is_inner = hit && (ref & kLeafFlag) == 0;
+    const uint64_t inner = __ballot(is_inner) & gm;
+    const int n_inner = __builtin_popcountll(inner);
+    if (sp + n_inner > kStack) {
+      if (gl == 0) atomicOr(wv.status, (uint32_t)BAJO_STATUS_STACK_OVERFLOW);
+      sp = 0;
+      return true;
+    }
+    if (is_inner) {
+      const int at = sp + (int)mask_rank(inner);
+      stack[2 * at] = ref;
+      stack[2 * at + 1] = __builtin_bit_cast(uint32_t, tin);
+    }
+    sp += n_inner;
+    uint32_t start = 0, k = 0;
+    if (is_leaf) {
+      start = ref & kWideLeafStartMask;
+      const uint32_t e = start + ((ref >> 28) & 7u) + 1u;
+      k = (e > wv.prim_count ? wv.prim_count : e) - (start < wv.prim_count ? start : wv.prim_count);
+    }
+    uint32_t incl = k;
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off, G);
+      if (gl >= (uint32_t)off) incl += y;
+    }
+    const uint32_t total = __shfl(incl, G - 1, G);
+    for (uint32_t m = 0; m < k; ++m) recs[incl - k + m] = start + m;
+    wave_lds_sync();
+    tests += total;
+    for (uint32_t base = 0; base < total; base += G) {
+      float t = kFltMax;
+      uint32_t s = kEmpty;
+      if (base + gl < total) {
+        s = recs[base + gl];
+        const LeafRec rec = load_rec<false>(wv.tris, s);
+        const TriData tr{mk(rec.a.x, rec.a.y, rec.a.z), mk(rec.b.x, rec.b.y, rec.b.z),
+                         mk(rec.c.x, rec.c.y, rec.c.z), __builtin_bit_cast(uint32_t, rec.a.w)};
+        float ad, ts, us, vs;
+        if (tri_scaled(ray, tr, ray.tmax, ad, ts, us, vs)) {
+          const float inv = 1.0f / ad;
+          t = ts * inv;
+        } else {
+          s = kEmpty;
+        }
+      }
+      // the two smallest distances of this round (the lowest lane wins equal ones)
+      float m1 = t;
+      uint32_t r1 = s, l1 = gl;
+#pragma unroll
+      for (int off = 1; off < G; off <<= 1) {
+        const float om = __shfl_xor(m1, off, G);
+        const uint32_t orr = (uint32_t)__shfl_xor((int)r1, off, G);
+        const uint32_t ol = (uint32_t)__shfl_xor((int)l1, off, G);
+        if (om < m1 || (om == m1 && ol < l1)) {
+          m1 = om;
+          r1 = orr;
+          l1 = ol;
+        }
+      }
+      float m2 = gl == l1 ? kFltMax : t;
+#pragma unroll
+      for (int off = 1; off < G; off <<= 1) m2 = fminf(m2, __shfl_xor(m2, off, G));
+      if (r1 != kEmpty) {
+        if (m1 < t1) {
+          t2 = fminf(t1, m2);
+          t1 = m1;
+          best = r1;
+        } else {
+          t2 = fminf(t2, m1);
+        }
+      }
+    }
+    wave_lds_sync();
+    return sp == 0;
+  }
+
+  // After the walk: whether Walk8 must decide (a near tie at the nearest
+  // candidate), else the hit record of the nearest candidate.
+  BJ_D bool near_tie() const { return best != kEmpty && t2 <= t1 * (1.0f + kMargin); }
+  BJ_D Hit finish(const WideView& wv) const {
+    Closest c = no_closest(ray.tmax);
+    if (best != kEmpty) {
+      const LeafRec rec = load_rec<false>(wv.tris, best);
+      rec_closest<false>(rec, best, ray, c);
+    }
+    return closest_hit_p<false>(wv.tris, c, ray);
   }
 };
 

@@ -383,7 +383,11 @@ typedef struct bajo_scene bajo_scene;
  *   shadow_group: lanes per shadow ray above that threshold: 1 = one lane
  *     per ray (shadow_walk applies), 8, 16 or 32 = that many lanes per ray
  *     (one node per 8 lanes per step); 0 = 16 lanes up to 2^20 rays and one
- *     lane per ray beyond. */
+ *     lane per ray beyond;
+ *   tail_group: lanes per path in the tail kernel (4- and 8-wide triangle
+ *     and batch scenes): 1 = one lane per path, 16 = sixteen (extension rays
+ *     by a group walk equal to the one-lane walk, shadow rays by group any
+ *     hit); 0 = 16 up to 2^15 live paths, one beyond. */
 typedef struct {
   uint32_t image_width, image_height, samples_per_pixel, max_depth;
   uint64_t rng_seed;
@@ -394,7 +398,7 @@ typedef struct {
   uint32_t shadow_walk;
   uint32_t sparse_shadow_rays;
   uint32_t shadow_group;
-  uint32_t reserved; /* must be 0 */
+  uint32_t tail_group;
 } bajo_render_settings;
 
 /* Rays traced by one render (read back from device counters). */

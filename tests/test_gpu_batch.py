@@ -138,14 +138,15 @@ def test_autotuned_schedule_renders_the_same_image(gpu_ctx, batch):
     assert tuned.path_capacity == min((c for _, c in table), key=lambda c: table[("path_capacity", c)])
     # every knob of the schedule, each value on its own: images stay bit-identical
     space = {"extend_waves": [0, 4, 6, 8], "shadow_group": [0, 1, 8, 16, 32], "shadow_walk": [0, 1, 2, 3, 4],
-             "tail_paths": [0, 1, 64, 1 << 20], "path_capacity": [0, 1536]}
+             "tail_group": [0, 1, 16], "tail_paths": [0, 1, 64, 1 << 20], "path_capacity": [0, 1536]}
     for knob, values in space.items():
         for value in values:
             got = enqueue_render_gpu(replace(settings(), **{knob: value}), camera(), g, ALGS["MIS"]).numpy()
             np.testing.assert_array_equal(got, want, err_msg=f"{knob} = {value}")
     # the full search: a cached result, an image equal to the default schedule's
     best, full = tune_render(settings(), camera(), g, ALGS["MIS"], reps=1)
-    assert {k for k, _ in full} == {"extend_waves", "shadow_group", "shadow_walk", "tail_paths", "path_capacity"}
+    assert {k for k, _ in full} == {"extend_waves", "shadow_group", "shadow_walk", "tail_group", "tail_paths",
+                                    "path_capacity"}
     again, cached = tune_render(settings(), camera(), g, ALGS["MIS"], reps=1)
     assert again == best and cached == full
     np.testing.assert_array_equal(enqueue_render_gpu(best, camera(), g, ALGS["MIS"]).numpy(), want)
@@ -155,6 +156,8 @@ def test_autotuned_schedule_renders_the_same_image(gpu_ctx, batch):
         enqueue_render_gpu(replace(settings(), shadow_walk=5), camera(), g, ALGS["MIS"])
     with pytest.raises(RuntimeError, match="shadow_group"):
         enqueue_render_gpu(replace(settings(), shadow_group=4), camera(), g, ALGS["MIS"])
+    with pytest.raises(RuntimeError, match="tail_group"):
+        enqueue_render_gpu(replace(settings(), tail_group=8), camera(), g, ALGS["MIS"])
     g.free()
 
 

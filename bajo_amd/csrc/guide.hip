@@ -511,6 +511,9 @@ struct AdamArgs {
   float* m;
   float* v;
   uint8_t* pack;
+  float* ema;            // moving average of the weights and its weight images
+  uint8_t* infer_pack;
+  float ema_decay;
   const float* part;
   uint32_t chunks;
   const float* tile_part;
@@ -534,6 +537,11 @@ BJ_D float block_sum(const float* v, uint32_t count, float* s_tmp) {
   return total;
 }
 
+// Decay of the weights' moving average that renders sample from: the last
+// steps of a frame are noisy (lr 1e-2 on one frame's samples), and the
+// average over about 100 steps is the steadier guide.
+constexpr float kEmaDecay = 0.99f;
+
 __global__ void __launch_bounds__(256) guide_adam_kernel(AdamArgs aa) {
   __shared__ float s_tmp[256];
   const float wsum = block_sum(aa.tile_part, aa.tiles, s_tmp);
@@ -555,6 +563,9 @@ __global__ void __launch_bounds__(256) guide_adam_kernel(AdamArgs aa) {
   const float w = aa.params[p] - aa.lr * mh / (sqrtf(vh) + aa.eps);
   aa.params[p] = w;
   pack_param(aa.pack, sl, w);
+  const float e = aa.ema_decay * aa.ema[p] + (1.0f - aa.ema_decay) * w;
+  aa.ema[p] = e;
+  pack_param(aa.infer_pack, sl, e);
 }
 
 // ---- mixture sampling (tests and tools) ---------------------------------------------
@@ -610,7 +621,9 @@ int32_t upload_params(bajo_ctx* ctx, bajo_guide* g, const float* host) {
   BJ_HIP(hipMemcpyAsync(g->params, host, kGuideParams * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   BJ_HIP(hipMemsetAsync(g->adam_m, 0, kGuideParams * sizeof(float), ctx->stream));
   BJ_HIP(hipMemsetAsync(g->adam_v, 0, kGuideParams * sizeof(float), ctx->stream));
+  BJ_HIP(hipMemcpyAsync(g->ema, g->params, kGuideParams * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
   guide_pack_kernel<<<div_up(kGuideParams, 256), 256, 0, ctx->stream>>>(g->params, g->pack);
+  guide_pack_kernel<<<div_up(kGuideParams, 256), 256, 0, ctx->stream>>>(g->params, g->infer_pack);
   BJ_HIP(hipGetLastError());
   g->step = 0;
   return BAJO_OK;
@@ -619,10 +632,12 @@ int32_t upload_params(bajo_ctx* ctx, bajo_guide* g, const float* host) {
 void free_guide(bajo_guide* g) {
   if (!g) return;
   (void)hipFree(g->params); (void)hipFree(g->adam_m); (void)hipFree(g->adam_v); (void)hipFree(g->pack);
+  (void)hipFree(g->ema); (void)hipFree(g->infer_pack);
   (void)hipFree(g->x0t); (void)hipFree(g->h1t); (void)hipFree(g->h2t); (void)hipFree(g->h3t);
   (void)hipFree(g->d1t); (void)hipFree(g->d2t); (void)hipFree(g->d3t); (void)hipFree(g->dot);
   (void)hipFree(g->tile_part); (void)hipFree(g->grad_part);
   (void)hipFree(g->records); (void)hipFree(g->contrib); (void)hipFree(g->samples); (void)hipFree(g->sample_count);
+  (void)hipFree(g->block_counts);
   (void)hipFree(g->vertices); (void)hipFree(g->outputs);
   delete g;
 }
@@ -666,6 +681,8 @@ int32_t run_adam(hipStream_t stream, bajo_guide* g, uint64_t ldb, uint32_t chunk
   aa.m = g->adam_m;
   aa.v = g->adam_v;
   aa.pack = g->pack;
+  aa.ema = g->ema;
+  aa.infer_pack = g->infer_pack;
   aa.part = g->grad_part;
   aa.chunks = chunks;
   aa.tile_part = g->tile_part;
@@ -675,10 +692,14 @@ int32_t run_adam(hipStream_t stream, bajo_guide* g, uint64_t ldb, uint32_t chunk
   aa.beta2 = g->opt.beta2;
   aa.eps = g->opt.epsilon;
   aa.grad_out = grad_out;
+  aa.ema_decay = 0.0f;
   if (!grad_out) {
     g->step += 1;
     aa.bias1 = 1.0f - powf(g->opt.beta1, (float)g->step);
     aa.bias2 = 1.0f - powf(g->opt.beta2, (float)g->step);
+    // the average follows the first steps closely, then settles at kEmaDecay
+    const float warm = (1.0f + (float)g->step) / (10.0f + (float)g->step);
+    aa.ema_decay = warm < kEmaDecay ? warm : kEmaDecay;
   } else {
     aa.bias1 = aa.bias2 = 1.0f;
   }
@@ -704,57 +725,127 @@ int32_t guide_train_samples(hipStream_t stream, bajo_guide* g, const float* samp
 
 namespace {
 
-// Training samples of one frame: thread p walks path p's records; vertex k
-// receives the radiance added at bounces after it, over the throughput the
-// path carried out of it (per channel), as luminance over the density its
-// direction was drawn with.
+// The training sample of path p at bounce k, if it has one: the vertex's
+// inputs, the direction taken, and as weight the radiance added at bounces
+// after it over the throughput the path carried out of it (per channel), as
+// luminance over the density its direction was drawn with.
+BJ_D bool collect_sample(const float* __restrict__ records, const float* __restrict__ contrib, uint64_t p,
+                         uint32_t k, uint32_t depth, v3 lo, v3 inv, float* out) {
+  const float* r = records + (p * depth + k) * kRecordPlanes;
+  const float pdf = r[12];
+  if (!(pdf > 0.0f)) return false;
+  v3 L = mk(0.0f, 0.0f, 0.0f);
+  for (uint32_t j = k + 1; j < depth; ++j) {
+    const float* c = contrib + 3 * (p * depth + j);
+    L = add(L, mk(c[0], c[1], c[2]));
+  }
+  const v3 T = mk(r[13], r[14], r[15]);
+  L = mk(T.x > 0.0f ? L.x / T.x : 0.0f, T.y > 0.0f ? L.y / T.y : 0.0f, T.z > 0.0f ? L.z / T.z : 0.0f);
+  const float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
+  if (!(lum > 0.0f)) return false;
+  const GuideInput gi = guide_input(mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), mk(r[6], r[7], r[8]), lo, inv);
+#pragma unroll
+  for (int a = 0; a < kGuideInputs; ++a) out[a] = gi.v[a];
+  out[9] = r[9];
+  out[10] = r[10];
+  out[11] = r[11];
+  out[12] = lum / pdf;
+  return true;
+}
+
+BJ_D void bounds_frame(const float* __restrict__ bounds, v3& lo, v3& inv) {
+  lo = mk(bounds[0], bounds[1], bounds[2]);
+  const v3 ext = sub(mk(bounds[3], bounds[4], bounds[5]), lo);
+  inv = mk(ext.x > 0.0f ? 1.0f / ext.x : 0.0f, ext.y > 0.0f ? 1.0f / ext.y : 0.0f, ext.z > 0.0f ? 1.0f / ext.z : 0.0f);
+}
+
+// The frame's samples are stored in a fixed order -- block by block (256
+// paths each), bounce by bounce within a block, then path order -- so the
+// training batches, and the trained network, do not depend on the order in
+// which waves run.  Pass 1 counts each block's samples.
+__global__ void __launch_bounds__(256) guide_count_kernel(const float* __restrict__ records,
+                                                          const float* __restrict__ contrib, uint64_t paths,
+                                                          uint32_t depth, const float* __restrict__ bounds,
+                                                          uint32_t* __restrict__ block_counts) {
+  __shared__ uint32_t s_wave[4];
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  v3 lo, inv;
+  bounds_frame(bounds, lo, inv);
+  uint32_t mine = 0;
+  float out[kGuideSampleFloats];
+  if (p < paths)
+    for (uint32_t k = 0; k < depth; ++k) mine += collect_sample(records, contrib, p, k, depth, lo, inv, out) ? 1u : 0u;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) mine += (uint32_t)__shfl_xor((int)mine, off, 64);
+  if (lane_id() == 0) s_wave[threadIdx.x >> 6] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+}
+
+// Pass 2 (one block): exclusive prefix of the block counts, in place; the
+// total becomes the frame's sample count.
+__global__ void __launch_bounds__(1024) guide_offsets_kernel(uint32_t* __restrict__ block_counts, uint32_t blocks,
+                                                             uint32_t* __restrict__ count) {
+  __shared__ uint32_t s_wave[16];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < blocks; base += blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < blocks ? block_counts[i] : 0u;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
+      if (lane >= (uint32_t)off) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t before = s_carry;
+    for (uint32_t w = 0; w < wave; ++w) before += s_wave[w];
+    if (i < blocks) block_counts[i] = before + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += s_wave[w];
+      s_carry += t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) count[0] = s_carry;
+}
+
+// Pass 3: each block writes its samples from its offset, bounce by bounce,
+// in path order.
 __global__ void __launch_bounds__(256) guide_collect_kernel(const float* __restrict__ records,
                                                             const float* __restrict__ contrib, uint64_t paths,
                                                             uint32_t depth, const float* __restrict__ bounds,
-                                                            float* __restrict__ samples, uint32_t* __restrict__ count,
-                                                            uint64_t cap) {
+                                                            const uint32_t* __restrict__ block_offsets,
+                                                            float* __restrict__ samples, uint64_t cap) {
+  __shared__ uint32_t s_wave[4];
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const v3 lo = mk(bounds[0], bounds[1], bounds[2]);
-  const v3 ext = sub(mk(bounds[3], bounds[4], bounds[5]), lo);
-  const v3 inv = mk(ext.x > 0.0f ? 1.0f / ext.x : 0.0f, ext.y > 0.0f ? 1.0f / ext.y : 0.0f,
-                    ext.z > 0.0f ? 1.0f / ext.z : 0.0f);
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  v3 lo, inv;
+  bounds_frame(bounds, lo, inv);
+  uint64_t next = block_offsets[blockIdx.x];
   for (uint32_t k = 0; k < depth; ++k) {
-    bool want = false;
     float out[kGuideSampleFloats];
-    if (p < paths) {
-      const float* r = records + (p * depth + k) * kRecordPlanes;
-      const float pdf = r[12];
-      if (pdf > 0.0f) {
-        v3 L = mk(0.0f, 0.0f, 0.0f);
-        for (uint32_t j = k + 1; j < depth; ++j) {
-          const float* c = contrib + 3 * (p * depth + j);
-          L = add(L, mk(c[0], c[1], c[2]));
-        }
-        const v3 T = mk(r[13], r[14], r[15]);
-        L = mk(T.x > 0.0f ? L.x / T.x : 0.0f, T.y > 0.0f ? L.y / T.y : 0.0f, T.z > 0.0f ? L.z / T.z : 0.0f);
-        const float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
-        if (lum > 0.0f) {
-          const GuideInput gi = guide_input(mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), mk(r[6], r[7], r[8]), lo, inv);
-#pragma unroll
-          for (int a = 0; a < kGuideInputs; ++a) out[a] = gi.v[a];
-          out[9] = r[9];
-          out[10] = r[10];
-          out[11] = r[11];
-          out[12] = lum / pdf;
-          want = true;
-        }
-      }
-    }
+    const bool want = p < paths && collect_sample(records, contrib, p, k, depth, lo, inv, out);
     const uint64_t m = ballot(want);
-    if (m == 0) continue;
-    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-    uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(count, (uint32_t)__builtin_popcountll(m));
-    base = __shfl(base, (int)leader, 64);
-    const uint64_t slot = (uint64_t)base + mask_rank(m);
+    if (lane == 0) s_wave[wave] = (uint32_t)__builtin_popcountll(m);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t w = 0; w < 4; ++w) {
+      if (w < wave) before += s_wave[w];
+      total += s_wave[w];
+    }
+    const uint64_t slot = next + before + mask_rank(m);
     if (want && slot < cap)
 #pragma unroll
       for (int a = 0; a < kGuideSampleFloats; ++a) samples[slot * kGuideSampleFloats + a] = out[a];
+    next += total;
+    __syncthreads();
   }
 }
 
@@ -776,7 +867,7 @@ int32_t guide_infer_vertices(hipStream_t stream, const bajo_guide* g, const floa
   if (rc) return rc;
   const uint64_t need = (cap + 63) / 64;
   if ((uint64_t)grid > need) grid = (int)need;
-  guide_forward_kernel<VertexRows><<<grid, 256, 0, stream>>>(g->pack, VertexRows{vertex_fields, scene_bounds},
+  guide_forward_kernel<VertexRows><<<grid, 256, 0, stream>>>(g->infer_pack, VertexRows{vertex_fields, scene_bounds},
                                                              count_dev, cap, outputs);
   BJ_HIP(hipGetLastError());
   return BAJO_OK;
@@ -808,6 +899,15 @@ int32_t guide_prepare_frame(hipStream_t stream, bajo_guide* g, uint64_t paths, u
     BJ_HIP(hipMalloc(&g->samples, slots * kGuideSampleFloats * sizeof(float)));
     g->record_cap = g->sample_cap = slots;
   }
+  const uint64_t blocks = (paths + 255) / 256;
+  if (blocks > g->block_cap) {
+    BJ_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(g->block_counts);
+    g->block_counts = nullptr;
+    g->block_cap = 0;
+    BJ_HIP(hipMalloc(&g->block_counts, blocks * sizeof(uint32_t)));
+    g->block_cap = blocks;
+  }
   BJ_HIP(hipMemsetAsync(g->records, 0, slots * kRecordPlanes * sizeof(float), stream));
   BJ_HIP(hipMemsetAsync(g->contrib, 0, slots * 3 * sizeof(float), stream));
   BJ_HIP(hipMemsetAsync(g->sample_count, 0, sizeof(uint32_t), stream));
@@ -815,8 +915,13 @@ int32_t guide_prepare_frame(hipStream_t stream, bajo_guide* g, uint64_t paths, u
 }
 
 int32_t guide_collect(hipStream_t stream, bajo_guide* g, uint64_t paths, uint32_t depth, const float* scene_bounds) {
-  guide_collect_kernel<<<div_up(paths, 256), 256, 0, stream>>>(g->records, g->contrib, paths, depth, scene_bounds,
-                                                              g->samples, g->sample_count, g->sample_cap);
+  const unsigned blocks = div_up(paths, 256);
+  BJ_REQUIRE(blocks <= g->block_cap, "guide_collect: %u blocks, room for %llu", blocks,
+             (unsigned long long)g->block_cap);
+  guide_count_kernel<<<blocks, 256, 0, stream>>>(g->records, g->contrib, paths, depth, scene_bounds, g->block_counts);
+  guide_offsets_kernel<<<1, 1024, 0, stream>>>(g->block_counts, blocks, g->sample_count);
+  guide_collect_kernel<<<blocks, 256, 0, stream>>>(g->records, g->contrib, paths, depth, scene_bounds,
+                                                  g->block_counts, g->samples, g->sample_cap);
   BJ_HIP(hipGetLastError());
   return BAJO_OK;
 }
@@ -890,6 +995,8 @@ int32_t bajo_guide_create(bajo_ctx* ctx, const bajo_guide_options* options, bajo
   if (e == hipSuccess) e = hipMalloc(&g->adam_m, bj::kGuideParams * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&g->adam_v, bj::kGuideParams * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&g->pack, bj::kPackBytes);
+  if (e == hipSuccess) e = hipMalloc(&g->ema, bj::kGuideParams * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&g->infer_pack, bj::kPackBytes);
   if (e == hipSuccess) e = hipMalloc(&g->x0t, (size_t)bj::kX0Rows * cap * 2);
   if (e == hipSuccess) e = hipMalloc(&g->h1t, (size_t)bj::kHRows * cap * 2);
   if (e == hipSuccess) e = hipMalloc(&g->h2t, (size_t)bj::kHRows * cap * 2);
@@ -904,6 +1011,7 @@ int32_t bajo_guide_create(bajo_ctx* ctx, const bajo_guide_options* options, bajo
   if (e != hipSuccess) return fail(e, "hipMalloc(guide)");
   // padding entries of the weight image stay zero
   e = hipMemsetAsync(g->pack, 0, bj::kPackBytes, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(g->infer_pack, 0, bj::kPackBytes, ctx->stream);
   if (e == hipSuccess) e = hipMemsetAsync(g->sample_count, 0, 4 * sizeof(uint32_t), ctx->stream);
   if (e != hipSuccess) return fail(e, "hipMemsetAsync(guide)");
   std::vector<float> p;

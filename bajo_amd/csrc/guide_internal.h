@@ -12,6 +12,8 @@ struct bajo_guide {
   float* adam_m = nullptr;       // first moments
   float* adam_v = nullptr;       // second moments
   uint8_t* pack = nullptr;       // bf16 weight images (forward + transposed) and f32 biases
+  float* ema = nullptr;          // exponential moving average of the weights (what renders sample from)
+  uint8_t* infer_pack = nullptr; // the same images of `ema`
   // training batch buffers (bf16, batch-contiguous rows), for `batch_cap` samples
   uint32_t batch_cap = 0;
   uint16_t* x0t = nullptr;       // [224][batch_cap] encoded inputs, row 216 = 1
@@ -31,6 +33,8 @@ struct bajo_guide {
   uint64_t sample_cap = 0;
   float* samples = nullptr;      // kGuideSampleFloats per training sample
   uint32_t* sample_count = nullptr;
+  uint32_t* block_counts = nullptr;  // per collection block: samples, then their offset
+  uint64_t block_cap = 0;
   // rendering: the guide queue of one bounce and the network's outputs for it
   uint64_t vertex_cap = 0;
   float* vertices = nullptr;     // kVertexPlanes per entry, 64-entry tiles

@@ -141,3 +141,25 @@ def test_trained_guide_beats_plain_nee_at_equal_spp(gpu_ctx, room):
     assert w[main & near].sum() > 0.3 * w[main].sum()
     guide.free()
     g.free()
+
+
+def test_training_is_deterministic(gpu_ctx, room):
+    """Two guides trained on the same frames end bit-identical: the frame's samples are stored in a fixed
+    order and every reduction of the training step runs in a fixed order."""
+    from bajo_amd import scenes
+    from bajo_amd.guide import render_guided
+    from bajo_amd.rt import GpuScene
+
+    announce("guided training determinism")
+    g = GpuScene(room, gpu_ctx, node_width=8)
+    cam = scenes.room_camera()
+    a, _ = train_guide(gpu_ctx, g, cam, 6, w=64, h=36, steps_per_frame=4)
+    b, _ = train_guide(gpu_ctx, g, cam, 6, w=64, h=36, steps_per_frame=4)
+    np.testing.assert_array_equal(a.params(), b.params())
+    np.testing.assert_array_equal(a.training_samples(), b.training_samples())
+    ia, _ = render_guided(g, a, settings(32, 18, 2, 3), cam)
+    ib, _ = render_guided(g, b, settings(32, 18, 2, 3), cam)
+    np.testing.assert_array_equal(ia.cpu().numpy(), ib.cpu().numpy())
+    a.free()
+    b.free()
+    g.free()

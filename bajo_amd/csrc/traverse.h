@@ -907,47 +907,59 @@ struct GroupWalk {
   bool px, py, pz;
   int sp;
   uint32_t steps;
+  // The frontier of the next step, taken off the stack and loaded while the
+  // previous step's records were tested: how many nodes it holds, whether
+  // this lane's node is one of them, its first four 16-byte words and this
+  // lane's child reference.  Every step issues these loads at one place (a
+  // dummy node 0 where no node is taken), so they stay in flight across the
+  // record tests.
+  int nf;
+  bool mine;
+  uint4 w0, w1, w2, w3;
+  uint32_t ref;
 
   BJ_D uint64_t group_mask() const {
     const uint32_t lane = lane_id();
     return (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << ((lane / G) * G);
   }
 
+  // Starts a ray with the root as its first frontier (loaded here); a root
+  // outside the hierarchy ends the ray at once with a status word.
   BJ_D void begin(const WideView& wv, const Ray& r, uint32_t* stack) {
     ray = r;
     rcp = rcp_dir(r.d);
     px = rcp.x >= 0.0f;
     py = rcp.y >= 0.0f;
     pz = rcp.z >= 0.0f;
-    steps = 0;
-    sp = wv.node_count == 0 ? 0 : 1;
-    if (lane_id() % G == 0) stack[0] = wv.root;
-    wave_lds_sync();
+    steps = 1;
+    sp = 0;
+    const bool ok = wv.node_count != 0 && wv.root < wv.node_count;
+    if (wv.node_count != 0 && !ok && lane_id() % G == 0) atomicOr(wv.status, (uint32_t)BAJO_STATUS_BAD_NODE);
+    nf = ok ? 1 : 0;
+    mine = ok && (lane_id() % G) < 8;
+    if (wv.node_count != 0) load(wv, ok ? wv.root : 0u);
+  }
+
+  BJ_D void load(const WideView& wv, uint32_t node) {
+    const uint4* np = reinterpret_cast<const uint4*>(wv.nodes + node);
+    w0 = np[0];
+    w1 = np[1];
+    w2 = np[2];
+    w3 = np[3];
+    ref = reinterpret_cast<const uint32_t*>(wv.nodes + node)[16 + (lane_id() & 7u)];
   }
 
   // One step; returns true once the ray is finished (then *occluded says how).
   BJ_D bool step(const WideView& wv, uint32_t* stack, uint32_t* recs, uint32_t& visits, uint32_t& tests,
                  bool& occluded) {
     occluded = false;
-    if (sp == 0) return true;
+    if (nf == 0) return true;
+    visits += (uint32_t)nf;
     const uint32_t gl = lane_id() % G, j = gl >> 3, i = gl & 7;
     const uint64_t gm = group_mask();
-    const int nf = sp < G / 8 ? sp : G / 8;
-    const bool mine = (int)j < nf;
-    const uint32_t node = mine ? stack[sp - 1 - (int)j] : 0u;
-    wave_lds_sync();
-    sp -= nf;
-    visits += (uint32_t)nf;
-    if ((__ballot(mine && node >= wv.node_count) & gm) != 0 || ++steps > wv.node_count) {
-      if (gl == 0) atomicOr(wv.status, (uint32_t)BAJO_STATUS_BAD_NODE);
-      return true;
-    }
     bool hit = false;
-    uint32_t ref = 0;
+    const uint32_t cref = ref;
     if (mine) {
-      const uint4* np = reinterpret_cast<const uint4*>(wv.nodes + node);
-      const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
-      ref = reinterpret_cast<const uint32_t*>(wv.nodes + node)[16 + i];
       const uint32_t meta = w0.w;
       const uint32_t n = meta >> 24;
       const float sx = __builtin_bit_cast(float, (meta & 0xFFu) << 23);
@@ -972,23 +984,24 @@ struct GroupWalk {
       const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, ray.tmax));
       hit = i < n && tmin <= tmax;
     }
-    const bool is_leaf = hit && (ref & kLeafFlag) != 0;
-    const bool is_inner = hit && (ref & kLeafFlag) == 0;
+    const bool is_leaf = hit && (cref & kLeafFlag) != 0;
+    const bool is_inner = hit && (cref & kLeafFlag) == 0;
     // entered inner children onto the group's stack
     const uint64_t inner = __ballot(is_inner) & gm;
     const int n_inner = __builtin_popcountll(inner);
     if (sp + n_inner > kStack) {
       if (gl == 0) atomicOr(wv.status, (uint32_t)BAJO_STATUS_STACK_OVERFLOW);
       sp = 0;
+      nf = 0;
       return true;
     }
-    if (is_inner) stack[sp + (int)mask_rank(inner)] = ref;
+    if (is_inner) stack[sp + (int)mask_rank(inner)] = cref;
     sp += n_inner;
     // the entered leaves' records, one per lane
     uint32_t start = 0, k = 0;
     if (is_leaf) {
-      start = ref & kWideLeafStartMask;
-      const uint32_t e = start + ((ref >> 28) & 7u) + 1u;
+      start = cref & kWideLeafStartMask;
+      const uint32_t e = start + ((cref >> 28) & 7u) + 1u;
       k = (e > wv.prim_count ? wv.prim_count : e) - (start < wv.prim_count ? start : wv.prim_count);
     }
     uint32_t incl = k;
@@ -1001,20 +1014,46 @@ struct GroupWalk {
     for (uint32_t m = 0; m < k; ++m) recs[incl - k + m] = start + m;
     wave_lds_sync();
     tests += total;
-    for (uint32_t base = 0; base < total; base += G) {
+    // the first round of records is loaded, then the next frontier (up to G/8
+    // nodes off the stack), so its node loads are in flight while the
+    // records are tested
+    LeafRec rec;
+    const bool has = gl < total;
+    if (has) rec = load_rec<false>(wv.tris, recs[gl]);
+    nf = sp < G / 8 ? sp : G / 8;
+    mine = (int)j < nf;
+    // lanes without a node of their own load their group's first node (the
+    // same cache lines as its owners)
+    const int at = sp - 1 - (mine ? (int)j : 0);
+    const uint32_t top = stack[at > 0 ? at : 0];
+    wave_lds_sync();
+    sp -= nf;
+    const bool bad = (__ballot(mine && top >= wv.node_count) & gm) != 0 || (nf > 0 && ++steps > wv.node_count);
+    load(wv, mine && top < wv.node_count ? top : 0u);
+    if ((__ballot(has && rec_any<false>(rec, ray)) & gm) != 0) {
+      occluded = true;
+      nf = 0;
+      return true;
+    }
+    for (uint32_t base = G; base < total; base += G) {
       bool occ = false;
       if (base + gl < total) {
-        const LeafRec rec = load_rec<false>(wv.tris, recs[base + gl]);
-        occ = rec_any<false>(rec, ray);
+        const LeafRec r2 = load_rec<false>(wv.tris, recs[base + gl]);
+        occ = rec_any<false>(r2, ray);
       }
       if ((__ballot(occ) & gm) != 0) {
         occluded = true;
-        wave_lds_sync();
+        nf = 0;
         return true;
       }
     }
     wave_lds_sync();
-    return sp == 0;
+    if (bad) {
+      if (gl == 0) atomicOr(wv.status, (uint32_t)BAJO_STATUS_BAD_NODE);
+      nf = 0;
+      return true;
+    }
+    return nf == 0;
   }
 };
 

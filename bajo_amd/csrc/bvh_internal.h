@@ -98,6 +98,8 @@ struct bajo_bvh {
   uint32_t* seg_acc;      // 12 S ordered-int bounds accumulators
   float* seg_scene;       // 12 S: root box and centroid box of each segment
   uint32_t* seg_prim;     // n: input index of each sorted leaf
+  uint32_t* seg_chunk;    // ceil(n / 64): the segment holding position 64 c
+  bool seg_chunk_valid;   // seg_chunk describes the offsets last uploaded
   uint64_t seg_leaf_capacity;
   void* seg_sort_ws;
   size_t seg_sort_ws_bytes;

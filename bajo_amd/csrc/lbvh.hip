@@ -388,55 +388,111 @@ BJ_D uint32_t segment_of(const uint64_t* __restrict__ off, uint32_t S, uint64_t 
   return lo;
 }
 
+// The segment of every 64-position chunk's first position, so that a lookup
+// costs one table load and a short forward scan instead of a binary search
+// of log2(S) dependent loads (segments of 64 or more triangles need no step).
+__global__ void __launch_bounds__(kBlock) seg_chunk_kernel(const uint64_t* __restrict__ off, uint32_t S, uint64_t n,
+                                                           uint32_t* __restrict__ chunk) {
+  const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c * 64 < n) chunk[c] = segment_of(off, S, c * 64);
+}
+
+BJ_D uint32_t segment_at(const uint64_t* __restrict__ off, const uint32_t* __restrict__ chunk, uint64_t i) {
+  uint32_t s = chunk[i >> 6];
+  while (off[s + 1] <= i) ++s;
+  return s;
+}
+
 __global__ void __launch_bounds__(kBlock) seg_init_kernel(uint32_t* __restrict__ acc, uint32_t S) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < 12ull * S; w += stride)
     acc[w] = f2ord((w % 6) < 3 ? kFltMax : -kFltMax);
 }
 
-// Per-segment root and centroid boxes.  Lanes of a wave hold consecutive
-// triangles, so a wave's segments are contiguous runs: a segmented shuffle
-// reduction leaves each run's extrema in its first lane, which folds them
-// into the segment's accumulators (order-preserving integer atomics).
+// Per-segment root and centroid boxes.  Each wave walks a contiguous chunk of
+// kSegChunk triangles, 64 at a time (coalesced).  A segmented shuffle
+// reduction leaves each run of equal segments' extrema in its first lane;
+// the run that reaches the wave's last lane is carried into the next 64 (a
+// segment usually spans many of them) and every other run is folded into its
+// segment's accumulators by order-preserving integer atomics, so a wave
+// issues atomics about once per segment it touches instead of once per 64
+// triangles.  Extrema are taken on the order-preserving integer images of
+// the coordinates, so the result does not depend on the reduction order.
+constexpr uint32_t kSegChunk = 64 * 16;
+
+BJ_D void seg_flush(uint32_t* __restrict__ acc, uint32_t seg, const uint32_t (&r)[12]) {
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    if ((k % 6) < 3) atomicMin(&acc[12 * (uint64_t)seg + k], r[k]);
+    else atomicMax(&acc[12 * (uint64_t)seg + k], r[k]);
+  }
+}
+
 template <bool kBoxes>
 __global__ void __launch_bounds__(kBlock) seg_bounds_kernel(const float* __restrict__ src, uint64_t n,
-                                                            const uint64_t* __restrict__ off, uint32_t S,
+                                                            const uint64_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ chunk,
                                                             uint32_t* __restrict__ acc) {
   const uint32_t lane = lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
-    const uint64_t i = base + lane;
-    const bool active = i < n;
-    float r[12];
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const uint32_t lo_init = f2ord(kFltMax), hi_init = f2ord(-kFltMax);
+  for (uint64_t c0 = wave * kSegChunk; c0 < n; c0 += waves * kSegChunk) {
+    // the carried run: its segment and extrema (the same in every lane)
+    uint32_t carry_seg = kSentinel;
+    uint32_t carry[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) r[k] = ((k % 6) < 3) ? kFltMax : -kFltMax;
-    uint32_t seg = kSentinel;
-    if (active) {
-      seg = segment_of(off, S, i);
-      v3 lo, hi;
-      leaf_src<kBoxes>(src, i, lo, hi);
-      const v3 ce = scale(add(lo, hi), 0.5f);
-      r[0] = lo.x; r[1] = lo.y; r[2] = lo.z; r[3] = hi.x; r[4] = hi.y; r[5] = hi.z;
-      r[6] = ce.x; r[7] = ce.y; r[8] = ce.z; r[9] = ce.x; r[10] = ce.y; r[11] = ce.z;
-    }
+    for (int k = 0; k < 12; ++k) carry[k] = (k % 6) < 3 ? lo_init : hi_init;
+    for (uint32_t it = 0; it < kSegChunk; it += 64) {
+      const uint64_t i = c0 + it + lane;
+      if (c0 + it >= n) break;
+      const bool active = i < n;
+      uint32_t r[12];
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t os = __shfl_down(seg, d, 64);
-      const bool take = lane + d < 64 && os == seg;
-#pragma unroll
-      for (int k = 0; k < 12; ++k) {
-        const float o = __shfl_down(r[k], d, 64);
-        if (take) r[k] = ((k % 6) < 3) ? fmin_sel(r[k], o) : fmax_sel(r[k], o);
+      for (int k = 0; k < 12; ++k) r[k] = (k % 6) < 3 ? lo_init : hi_init;
+      uint32_t seg = kSentinel;
+      if (active) {
+        seg = segment_at(off, chunk, i);
+        v3 lo, hi;
+        leaf_src<kBoxes>(src, i, lo, hi);
+        const v3 ce = scale(add(lo, hi), 0.5f);
+        r[0] = f2ord(lo.x); r[1] = f2ord(lo.y); r[2] = f2ord(lo.z);
+        r[3] = f2ord(hi.x); r[4] = f2ord(hi.y); r[5] = f2ord(hi.z);
+        r[6] = r[9] = f2ord(ce.x);
+        r[7] = r[10] = f2ord(ce.y);
+        r[8] = r[11] = f2ord(ce.z);
       }
-    }
-    const uint32_t prev = __shfl_up(seg, 1, 64);
-    if (active && (lane == 0 || prev != seg)) {
 #pragma unroll
-      for (int k = 0; k < 12; ++k) {
-        if ((k % 6) < 3) atomicMin(&acc[12 * (uint64_t)seg + k], f2ord(r[k]));
-        else atomicMax(&acc[12 * (uint64_t)seg + k], f2ord(r[k]));
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t os = __shfl_down(seg, d, 64);
+        const bool take = lane + d < 64 && os == seg;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          const uint32_t o = __shfl_down(r[k], d, 64);
+          if (take) r[k] = (k % 6) < 3 ? (o < r[k] ? o : r[k]) : (o > r[k] ? o : r[k]);
+        }
       }
+      const uint32_t prev = __shfl_up(seg, 1, 64);
+      const bool head = active && (lane == 0 || prev != seg);
+      const uint64_t heads = __ballot(head);
+      const uint32_t last = 63u - (uint32_t)__builtin_clzll(heads);  // lane 0 is always a head here
+      // the first run continues the carried one, or the carried run is complete
+      if (lane == 0) {
+        if (seg == carry_seg) {
+#pragma unroll
+          for (int k = 0; k < 12; ++k)
+            r[k] = (k % 6) < 3 ? (carry[k] < r[k] ? carry[k] : r[k]) : (carry[k] > r[k] ? carry[k] : r[k]);
+        } else if (carry_seg != kSentinel) {
+          seg_flush(acc, carry_seg, carry);
+        }
+      }
+      // runs that end inside these 64 are complete
+      if (head && lane != last) seg_flush(acc, seg, r);
+      carry_seg = __shfl(seg, last, 64);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) carry[k] = __shfl(r[k], last, 64);
     }
+    if (lane == 0 && carry_seg != kSentinel) seg_flush(acc, carry_seg, carry);
   }
 }
 
@@ -445,7 +501,8 @@ __global__ void __launch_bounds__(kBlock) seg_bounds_kernel(const float* __restr
 // the order of the single-hierarchy key (code << 32 | index) within it.
 template <bool kBoxes>
 __global__ void __launch_bounds__(kBlock) seg_morton_kernel(const float* __restrict__ src, uint64_t n,
-                                                            const uint64_t* __restrict__ off, uint32_t S,
+                                                            const uint64_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ chunk, uint32_t S,
                                                             const uint32_t* __restrict__ acc,
                                                             float* __restrict__ scene, uint64_t* __restrict__ keys,
                                                             uint32_t* __restrict__ values) {
@@ -453,7 +510,7 @@ __global__ void __launch_bounds__(kBlock) seg_morton_kernel(const float* __restr
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t w = gtid; w < 12ull * S; w += stride) scene[w] = ord2f(acc[w]);
   for (uint64_t i = gtid; i < n; i += stride) {
-    const uint32_t seg = segment_of(off, S, i);
+    const uint32_t seg = segment_at(off, chunk, i);
     const uint32_t* a = acc + 12 * (uint64_t)seg;
     // bvh/gpu/builder/lbvh.mojo:46-52 with the segment's centroid box
     const v3 cmin = mk(ord2f(a[6]), ord2f(a[7]), ord2f(a[8]));
@@ -483,7 +540,8 @@ __global__ void __launch_bounds__(kBlock) seg_keys_kernel(uint64_t* __restrict__
 }
 
 __global__ void __launch_bounds__(kBlock) seg_karras_kernel(const uint64_t* __restrict__ keys, uint64_t n,
-                                                            const uint64_t* __restrict__ off, uint32_t S,
+                                                            const uint64_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ chunk,
                                                             BjNode* __restrict__ nodes,
                                                             uint32_t* __restrict__ leaf_parent,
                                                             uint32_t* __restrict__ leaf_counts,
@@ -491,7 +549,7 @@ __global__ void __launch_bounds__(kBlock) seg_karras_kernel(const uint64_t* __re
                                                             uint32_t* __restrict__ range_first) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j + 1 >= n) return;
-  const uint32_t seg = segment_of(off, S, j);
+  const uint32_t seg = segment_at(off, chunk, j);
   const uint64_t first = off[seg], count = off[seg + 1] - first;
   if (j - first + 1 >= count) return;  // the segment's unused last slot
   karras_node(keys + first, (int64_t)count, (int64_t)(j - first), nodes, leaf_parent, leaf_counts, flags, range_first,
@@ -609,8 +667,10 @@ void free_segment_staging(bajo_bvh* h) {
 void free_segments(bajo_bvh* h) {
   free_segment_staging(h);
   (void)hipFree(h->seg_off); (void)hipFree(h->seg_acc); (void)hipFree(h->seg_scene); (void)hipFree(h->seg_prim);
-  (void)hipFree(h->seg_sort_ws);
+  (void)hipFree(h->seg_chunk); (void)hipFree(h->seg_sort_ws);
   h->seg_off = nullptr; h->seg_acc = nullptr; h->seg_scene = nullptr; h->seg_prim = nullptr; h->seg_sort_ws = nullptr;
+  h->seg_chunk = nullptr;
+  h->seg_chunk_valid = false;
   h->seg_capacity = 0;
   h->seg_leaf_capacity = 0;
   h->seg_sort_ws_bytes = 0;
@@ -633,9 +693,13 @@ int32_t ensure_segment_capacity(bajo_bvh* h, uint64_t n, uint32_t S, size_t sort
   }
   if (n > h->seg_leaf_capacity) {
     (void)hipFree(h->seg_prim);
+    (void)hipFree(h->seg_chunk);
     h->seg_prim = nullptr;
+    h->seg_chunk = nullptr;
+    h->seg_chunk_valid = false;
     h->seg_leaf_capacity = 0;
     BJ_HIP(hipMalloc(&h->seg_prim, n * sizeof(uint32_t)));
+    BJ_HIP(hipMalloc(&h->seg_chunk, (n + 63) / 64 * sizeof(uint32_t)));
     h->seg_leaf_capacity = n;
   }
   if (sort_bytes > h->seg_sort_ws_bytes) {
@@ -791,6 +855,7 @@ int32_t upload_segment_offsets(hipStream_t st, bajo_bvh* h, const uint64_t* offs
   BJ_HIP(hipMemcpyAsync(h->seg_off, h->seg_off_host, bytes, hipMemcpyHostToDevice, st));
   BJ_HIP(hipEventRecord(h->seg_upload, st));
   h->seg_uploaded = S;
+  h->seg_chunk_valid = false;
   return BAJO_OK;
 }
 
@@ -864,13 +929,19 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
   if (stream_blocks < 1) stream_blocks = 1;
   unsigned seg_blocks = div_up(12ull * S, (uint64_t)kBlock);
   if (seg_blocks > 1024) seg_blocks = 1024;
+  if (!h->seg_chunk_valid) {
+    // once per set of offsets (rebuilds with the same offsets keep the table)
+    seg_chunk_kernel<<<div_up((n + 63) / 64, (uint64_t)kBlock), kBlock, 0, st>>>(h->seg_off, S, n, h->seg_chunk);
+    BJ_HIP(hipGetLastError());
+    h->seg_chunk_valid = true;
+  }
   ev.record(0, st);
   seg_init_kernel<<<seg_blocks, kBlock, 0, st>>>(h->seg_acc, S);
-  seg_bounds_kernel<false><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->seg_off, S, h->seg_acc);
+  seg_bounds_kernel<false><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->seg_off, h->seg_chunk, h->seg_acc);
   BJ_HIP(hipGetLastError());
   ev.record(1, st);
-  seg_morton_kernel<false><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->seg_off, S, h->seg_acc, h->seg_scene,
-                                                             h->keys, h->seg_prim);
+  seg_morton_kernel<false><<<stream_blocks, kBlock, 0, st>>>(vertices, n, h->seg_off, h->seg_chunk, S, h->seg_acc,
+                                                             h->seg_scene, h->keys, h->seg_prim);
   BJ_HIP(hipGetLastError());
   ev.record(2, st);
   rc = onesweep_run(ctx, plan, h->keys, h->seg_prim, h->seg_sort_ws, false);
@@ -881,7 +952,7 @@ int32_t build_segmented(bajo_ctx* ctx, bajo_bvh* h, const float* vertices, const
   const bool hploc = h->method == BAJO_BUILD_HPLOC && n > S;
   if (n > S) {
     // the LBVH writes its nodes directly; H-PLOC's guide is the same forest
-    seg_karras_kernel<<<div_up(n - 1, kBlock), kBlock, 0, st>>>(h->keys, n, h->seg_off, S,
+    seg_karras_kernel<<<div_up(n - 1, kBlock), kBlock, 0, st>>>(h->keys, n, h->seg_off, h->seg_chunk,
                                                                 hploc ? h->hp_guide : h->nodes, h->leaf_parent,
                                                                 h->leaf_counts, h->flags,
                                                                 hploc ? h->hp_first : nullptr);

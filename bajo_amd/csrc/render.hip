@@ -239,43 +239,78 @@ __global__ void init_counters_kernel(uint32_t* counters, uint32_t active, unsign
     for (uint32_t i = t; i < kStatsWords; i += blockDim.x) stats[i] = 0;
 }
 
-__global__ void __launch_bounds__(256) advance_kernel(uint32_t* counters, unsigned long long* stats, uint32_t bounce,
-                                                      uint32_t cap, const unsigned long long* ext_work,
-                                                      uint32_t ext_blocks, const unsigned long long* sh_work,
-                                                      uint32_t sh_blocks) {
-  __shared__ unsigned long long s_sum[6][256];
-  const uint32_t t = threadIdx.x;
+// Blocks of the partials reduction: each sums a strided share of the live
+// blocks' partials into one row of six sub-totals.
+constexpr uint32_t kSumBlocks = 64;
+
+// Sums (a, b, c, d) and maxima (ma, mc) over the 64 lanes of a wave.
+BJ_D void wave_reduce_work(unsigned long long (&v)[6]) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const unsigned long long o = __shfl_xor(v[k], off, 64);
+      v[k] = k < 4 ? v[k] + o : (o > v[k] ? o : v[k]);
+    }
+  }
+}
+
+// The per-block traversal partials of one bounce's extend and shadow
+// launches, reduced by kSumBlocks blocks into `sums` (kSumBlocks rows of 6).
+// Blocks past the live entries exited without writing their partials.
+__global__ void __launch_bounds__(256) partials_sum_kernel(const uint32_t* __restrict__ counters, uint32_t cap,
+                                                           const unsigned long long* __restrict__ ext_work,
+                                                           uint32_t ext_blocks,
+                                                           const unsigned long long* __restrict__ sh_work,
+                                                           uint32_t sh_blocks, unsigned long long* __restrict__ sums) {
+  __shared__ unsigned long long s_part[4][6];
   if (counters[kTailDone]) return;  // the tail kernel traced and counted the rest of the chunk
-  unsigned long long a = 0, b = 0, c = 0, d = 0, ma = 0, mc = 0;
-  // blocks past the live entries exit without writing their partials
   const uint32_t active = counters[kActive] < cap ? counters[kActive] : cap;
   const uint32_t shadows = counters[kShadow] < cap ? counters[kShadow] : cap;
   const uint32_t ext_live = (active + 255u) / 256u < ext_blocks ? (active + 255u) / 256u : ext_blocks;
   uint32_t sh_live = (shadows + 255u) / 256u < sh_blocks ? (shadows + 255u) / 256u : sh_blocks;
-  if (counters[kSparseShadow]) sh_live = counters[kSparseShadow];  // the one-ray-per-wave kernel's partials
-  for (uint32_t i = t; i < ext_live; i += blockDim.x) {
-    a += ext_work[kWorkStride * i];
-    b += ext_work[kWorkStride * i + 1];
-    ma = ext_work[kWorkStride * i + 2] > ma ? ext_work[kWorkStride * i + 2] : ma;
+  if (counters[kSparseShadow]) sh_live = counters[kSparseShadow];  // the lane-group kernels' partials
+  unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ext_live; i += stride) {
+    v[0] += ext_work[kWorkStride * i];
+    v[1] += ext_work[kWorkStride * i + 1];
+    v[4] = ext_work[kWorkStride * i + 2] > v[4] ? ext_work[kWorkStride * i + 2] : v[4];
   }
-  for (uint32_t i = t; i < sh_live; i += blockDim.x) {
-    c += sh_work[kWorkStride * i];
-    d += sh_work[kWorkStride * i + 1];
-    mc = sh_work[kWorkStride * i + 2] > mc ? sh_work[kWorkStride * i + 2] : mc;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sh_live; i += stride) {
+    v[2] += sh_work[kWorkStride * i];
+    v[3] += sh_work[kWorkStride * i + 1];
+    v[5] = sh_work[kWorkStride * i + 2] > v[5] ? sh_work[kWorkStride * i + 2] : v[5];
   }
-  s_sum[0][t] = a; s_sum[1][t] = b; s_sum[2][t] = c; s_sum[3][t] = d; s_sum[4][t] = ma; s_sum[5][t] = mc;
+  wave_reduce_work(v);
+  const uint32_t wave = threadIdx.x >> 6;
+  if (lane_id() == 0)
+    for (int k = 0; k < 6; ++k) s_part[wave][k] = v[k];
   __syncthreads();
-  if (t != 0) return;
-  for (int k = 0; k < 4; ++k) {
+  if (threadIdx.x < 6) {
+    const int k = (int)threadIdx.x;
     unsigned long long x = 0;
-    for (uint32_t i = 0; i < blockDim.x; ++i) x += s_sum[k][i];
-    stats[kWorkBase + k] += x;
+    for (int w = 0; w < 4; ++w) x = k < 4 ? x + s_part[w][k] : (s_part[w][k] > x ? s_part[w][k] : x);
+    sums[6 * blockIdx.x + k] = x;
   }
+}
+
+// One wave: folds the bounce's partial sums into the render statistics and
+// advances the queue counters to the next bounce.
+__global__ void __launch_bounds__(64) advance_kernel(uint32_t* counters, unsigned long long* stats, uint32_t bounce,
+                                                     uint32_t cap, const unsigned long long* __restrict__ sums) {
+  const uint32_t t = threadIdx.x;
+  if (counters[kTailDone]) return;  // the tail kernel traced and counted the rest of the chunk
+  unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
+  if (t < kSumBlocks)
+    for (int k = 0; k < 6; ++k) v[k] = sums[6 * t + k];
+  wave_reduce_work(v);
+  if (t != 0) return;
+  for (int k = 0; k < 4; ++k) stats[kWorkBase + k] += v[k];
   if (bounce < kMaxBounces) {
     for (int k = 0; k < 2; ++k) {
-      unsigned long long x = stats[kMaxVisitBase + 2 * bounce + k];
-      for (uint32_t i = 0; i < blockDim.x; ++i) x = s_sum[4 + k][i] > x ? s_sum[4 + k][i] : x;
-      stats[kMaxVisitBase + 2 * bounce + k] = x;
+      const unsigned long long x = stats[kMaxVisitBase + 2 * bounce + k];
+      stats[kMaxVisitBase + 2 * bounce + k] = v[4 + k] > x ? v[4 + k] : x;
     }
     stats[2 * bounce + 0] += counters[kActive];
     stats[2 * bounce + 1] += counters[kShadow];
@@ -1371,23 +1406,42 @@ __global__ void __launch_bounds__(kRtBlock) guide_sample_kernel(SceneView sc, Qu
   }
 }
 
+// Per-pixel average of its spp sample slots, summed in slot order.  A block's
+// pixels own one contiguous range of slots, which it stages through LDS in
+// chunks with coalesced loads; each thread then adds its own slots from LDS.
+constexpr uint32_t kResolveChunk = 2048;
+
 __global__ void __launch_bounds__(kRtBlock) resolve_kernel(const float* __restrict__ sample, float* __restrict__ pixels,
                                                            uint64_t cap, uint32_t pixel_base, uint32_t pixel_count,
                                                            uint32_t spp) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= pixel_count) return;
-  float r = 0.0f, g = 0.0f, b = 0.0f;
-  const uint64_t s0 = (uint64_t)p * spp;
-  for (uint64_t s = s0; s < s0 + spp; ++s) {
-    r += sample[plane_at<kSamplePlanes>((uint32_t)s, 0)];
-    g += sample[plane_at<kSamplePlanes>((uint32_t)s, 1)];
-    b += sample[plane_at<kSamplePlanes>((uint32_t)s, 2)];
+  __shared__ float s_buf[kSamplePlanes][kResolveChunk + kResolveChunk / 32];
+  const uint32_t p0 = blockIdx.x * blockDim.x;
+  const uint32_t p = p0 + threadIdx.x;
+  const uint32_t npix = pixel_count - p0 < blockDim.x ? pixel_count - p0 : blockDim.x;
+  const uint64_t base = (uint64_t)p0 * spp, total = (uint64_t)npix * spp;
+  const uint64_t mine_lo = (uint64_t)threadIdx.x * spp, mine_hi = mine_lo + spp;
+  float acc[kSamplePlanes] = {0.0f, 0.0f, 0.0f};
+  for (uint64_t c = 0; c < total; c += kResolveChunk) {
+    const uint32_t len = total - c < kResolveChunk ? (uint32_t)(total - c) : kResolveChunk;
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
+#pragma unroll
+      for (int pl = 0; pl < kSamplePlanes; ++pl)
+        s_buf[pl][i + i / 32] = sample[plane_at<kSamplePlanes>((uint32_t)(base + c + i), pl)];
+    __syncthreads();
+    const uint64_t lo = mine_lo > c ? mine_lo : c, hi = mine_hi < c + len ? mine_hi : c + len;
+    for (uint64_t s2 = lo; s2 < hi; ++s2) {
+      const uint32_t j = (uint32_t)(s2 - c);
+#pragma unroll
+      for (int pl = 0; pl < kSamplePlanes; ++pl) acc[pl] += s_buf[pl][j + j / 32];
+    }
+    __syncthreads();
   }
+  if (p >= pixel_count) return;
   const float k = 1.0f / (float)spp;
   const uint64_t o = 3ull * (pixel_base + p);
-  pixels[o + 0] = r * k;
-  pixels[o + 1] = g * k;
-  pixels[o + 2] = b * k;
+  pixels[o + 0] = acc[0] * k;
+  pixels[o + 1] = acc[1] * k;
+  pixels[o + 2] = acc[2] * k;
 }
 
 }  // namespace
@@ -1408,6 +1462,7 @@ struct RenderArena {
   uint64_t spill_threads = 0;
   unsigned long long* work_ext = nullptr;  // per-block traversal work partials
   unsigned long long* work_sh = nullptr;
+  unsigned long long* work_sums = nullptr;  // kSumBlocks rows of 6 sub-totals of the partials
   float* cam = nullptr;                    // 20 floats
   float* cams = nullptr;                   // 20 floats per environment (per-environment cameras)
   uint32_t cams_cap = 0;
@@ -1453,6 +1508,7 @@ void free_arena(RenderArena& a) {
   (void)hipFree(a.ids[0]); (void)hipFree(a.ids[1]); (void)hipFree(a.fields[0]); (void)hipFree(a.fields[1]);
   (void)hipFree(a.shadow_ids); (void)hipFree(a.shadow_fields); (void)hipFree(a.sample); (void)hipFree(a.counters);
   (void)hipFree(a.stats); (void)hipFree(a.spill); (void)hipFree(a.work_ext); (void)hipFree(a.work_sh);
+  (void)hipFree(a.work_sums);
   (void)hipFree(a.cam); (void)hipFree(a.cams);
   a = RenderArena();
 }
@@ -1510,6 +1566,7 @@ int32_t ensure_arena(RenderArena& a, uint64_t cap, uint64_t spill_threads) {
   BJ_HIP(hipMalloc(&a.spill, spill_threads * kSpillMax * sizeof(uint32_t)));
   BJ_HIP(hipMalloc(&a.work_ext, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
   BJ_HIP(hipMalloc(&a.work_sh, kWorkStride * (spill_threads / kRtBlock) * sizeof(unsigned long long)));
+  BJ_HIP(hipMalloc(&a.work_sums, 6 * kSumBlocks * sizeof(unsigned long long)));
   a.cap = cap;
   a.spill_threads = spill_threads;
   return BAJO_OK;
@@ -1843,8 +1900,9 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
         if (ev) { ev->sh.push_back(ev->make()); (void)hipEventRecord(ev->sh.back(), stream); }
       }
       const bool traced_shadows = kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS;
-      advance_kernel<<<1, 256, 0, stream>>>(a.counters, a.stats, b, (uint32_t)cap, a.work_ext, g_ext, a.work_sh,
-                                            traced_shadows ? g_sh : 0u);
+      partials_sum_kernel<<<kSumBlocks, 256, 0, stream>>>(a.counters, (uint32_t)cap, a.work_ext, g_ext, a.work_sh,
+                                                          traced_shadows ? g_sh : 0u, a.work_sums);
+      advance_kernel<<<1, 64, 0, stream>>>(a.counters, a.stats, b, (uint32_t)cap, a.work_sums);
     }
     const uint32_t pixel_count = chunk / st->samples_per_pixel;
     resolve_kernel<<<div_up(pixel_count, kRtBlock), kRtBlock, 0, stream>>>(

@@ -55,7 +55,7 @@ constexpr int kActive = 0, kNext = 1, kShade = 2, kShadow = 3, kStatus = 4, kFet
               kTailDone = 7;
 // guided renders: vertices queued for the guide this bounce (word 8 is the traversal status)
 constexpr int kVertex = 9;
-// blocks of work partials the one-ray-per-wave shadow kernel wrote this bounce (0: the per-lane kernel ran)
+// blocks of work partials the bounce's shadow kernel wrote (each sets it when it traces the bounce's rays)
 constexpr int kSparseShadow = 10;
 // shadow rays per bounce at or below which one wave traces each ray
 constexpr uint32_t kSparseShadowRays = 8192;
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) partials_sum_kernel(const uint32_t* __res
   const uint32_t shadows = counters[kShadow] < cap ? counters[kShadow] : cap;
   const uint32_t ext_live = (active + 255u) / 256u < ext_blocks ? (active + 255u) / 256u : ext_blocks;
   uint32_t sh_live = (shadows + 255u) / 256u < sh_blocks ? (shadows + 255u) / 256u : sh_blocks;
-  if (counters[kSparseShadow]) sh_live = counters[kSparseShadow];  // the lane-group kernels' partials
+  if (counters[kSparseShadow]) sh_live = counters[kSparseShadow];  // the shadow kernel's partials
   unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ext_live; i += stride) {
@@ -1202,7 +1202,10 @@ __global__ void __launch_bounds__(kRtBlock) tail_group_kernel(SceneView sc, Queu
 }
 
 // kWalk: the shadow walk form (traverse8_any: bit 0 two nodes per step, bit 1
-// leaf records four at a time).
+// leaf records four at a time).  One lane per ray, when the bounce's shadow
+// count exceeds `sparse_rays`.  A grid of the resident blocks whose waves take
+// 64 rays at a time from a cursor: a bounce that leaves the rays to another
+// kernel costs only the resident blocks' exits, not one block per arena entry.
 template <int kAlg, bool kWide, int kGeo, int kWalk = 0>
 __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t bounce,
                                                           uint32_t* __restrict__ spill,
@@ -1215,30 +1218,34 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
   const uint64_t cap = q.capacity;
   const uint32_t count =
       q.counters[kTailDone] ? 0u : (q.counters[kShadow] < cap ? q.counters[kShadow] : (uint32_t)cap);
-  uint64_t work_visits = 0, work_leaves = 0;
-  if ((uint64_t)blockIdx.x * kRtBlock >= count || count <= sparse_rays) return;
-  const uint32_t idx = (uint32_t)gtid;
-  if (idx < count) {
-    float f[kShadowPlanes];
+  if (count <= sparse_rays) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) q.counters[kSparseShadow] = gridDim.x;  // blocks of partials
+  uint32_t work_visits = 0, work_leaves = 0, ray_max = 0;  // a lane traces a few rays: 32 bits suffice
+  const uint32_t threads = gridDim.x * blockDim.x;
+  for (uint32_t idx = (uint32_t)gtid; idx < count; idx += threads) {
+    {
+      float f[kShadowPlanes];
 #pragma unroll
-    for (int p = 0; p < kShadowPlanes; ++p) f[p] = q.shadow_fields[plane_at<kShadowPlanes>(idx, p)];
-    const uint32_t sid = q.shadow_ids[idx];
-    const uint32_t slot = sid - q.sample_base;
-    const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
-    TraceCounters tc;
-    const Hit h = scene_trace<kTraceShadow + kWalk, kWide, kGeo>(sc, ray, st, &tc, false,
-                                                               kGeo == kGeoBatch ? sid / sc.env_paths : 0u);
-    work_visits += tc.visits;
-    work_leaves += tc.leaves;
-    const bool occluded = h.prim == kEmpty && h.t == 0.0f;
-    if (kAlg == BAJO_RENDER_AO || !occluded) {
-      const v3 c = shadow_radiance<kAlg>(occluded, f);
-      accumulate(q.sample, slot, c);
-      if constexpr (kAlg == BAJO_RENDER_GUIDED)
-        if (q.contrib) contrib_add(q.contrib, slot, q.depth, bounce, c);
+      for (int p = 0; p < kShadowPlanes; ++p) f[p] = q.shadow_fields[plane_at<kShadowPlanes>(idx, p)];
+      const uint32_t sid = q.shadow_ids[idx];
+      const uint32_t slot = sid - q.sample_base;
+      const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
+      TraceCounters tc;
+      const Hit h = scene_trace<kTraceShadow + kWalk, kWide, kGeo>(sc, ray, st, &tc, false,
+                                                                 kGeo == kGeoBatch ? sid / sc.env_paths : 0u);
+      work_visits += tc.visits;
+      work_leaves += tc.leaves;
+      ray_max = tc.visits > ray_max ? tc.visits : ray_max;
+      const bool occluded = h.prim == kEmpty && h.t == 0.0f;
+      if (kAlg == BAJO_RENDER_AO || !occluded) {
+        const v3 c = shadow_radiance<kAlg>(occluded, f);
+        accumulate(q.sample, slot, c);
+        if constexpr (kAlg == BAJO_RENDER_GUIDED)
+          if (q.contrib) contrib_add(q.contrib, slot, q.depth, bounce, c);
+      }
     }
   }
-  store_block_work(work, work_visits, work_leaves, (uint32_t)work_visits);
+  store_block_work(work, work_visits, work_leaves, ray_max);
 }
 
 // Shadow rays with G lanes per ray (GroupWalk) when the bounce's shadow count
@@ -1739,6 +1746,17 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   }
   if (g_group > g_sh) g_group = g_sh;
   if (g_group < 1) g_group = 1;
+  // the per-lane shadow kernel's resident grid, for the walk form this render uses
+  unsigned g_lane = 1;
+  if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
+    if (kWide && shadow_walk == 1) rc = grid_of(shadow_kernel<kAlg, kWide, kGeo, kWide ? 1 : 0>, &g_lane);
+    else if (kWide && shadow_walk == 2) rc = grid_of(shadow_kernel<kAlg, kWide, kGeo, kWide ? 2 : 0>, &g_lane);
+    else if (kWide && shadow_walk == 3) rc = grid_of(shadow_kernel<kAlg, kWide, kGeo, kWide ? 3 : 0>, &g_lane);
+    else rc = grid_of(shadow_kernel<kAlg, kWide, kGeo>, &g_lane);
+    if (rc) return rc;
+  }
+  if (g_lane > g_sh) g_lane = g_sh;
+  if (g_lane < 1) g_lane = 1;
   // The lane-group tail claims paths from a cursor after its first ones; its
   // grid is four 256-thread blocks per CU, twice the resident count (0.35 ms
   // for the 4M-triangle NEE frame's tail against 0.38 with the resident grid).
@@ -1870,16 +1888,16 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
         if (dense_group > 1 && !adaptive_group) {
           // the lane-group kernel takes every count above the sparse threshold
         } else if (kWide && shadow_walk == 1)
-          shadow_kernel<kAlg, kWide, kGeo, kWide ? 1 : 0><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
+          shadow_kernel<kAlg, kWide, kGeo, kWide ? 1 : 0><<<g_lane, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
                                                                                      a.work_sh, a.stats, per_lane_lo);
         else if (kWide && shadow_walk == 2)
-          shadow_kernel<kAlg, kWide, kGeo, kWide ? 2 : 0><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
+          shadow_kernel<kAlg, kWide, kGeo, kWide ? 2 : 0><<<g_lane, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
                                                                                      a.work_sh, a.stats, per_lane_lo);
         else if (kWide && shadow_walk == 3)
-          shadow_kernel<kAlg, kWide, kGeo, kWide ? 3 : 0><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
+          shadow_kernel<kAlg, kWide, kGeo, kWide ? 3 : 0><<<g_lane, kRtBlock, 0, stream>>>(sv, q, b, a.spill,
                                                                                      a.work_sh, a.stats, per_lane_lo);
         else
-          shadow_kernel<kAlg, kWide, kGeo><<<g_sh, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats,
+          shadow_kernel<kAlg, kWide, kGeo><<<g_lane, kRtBlock, 0, stream>>>(sv, q, b, a.spill, a.work_sh, a.stats,
                                                                           per_lane_lo);
         if constexpr (kSparse) {
           constexpr int kG = kSparse ? kGeo : kGeoTriangles;

@@ -163,12 +163,38 @@ __global__ void __launch_bounds__(kSortBlock) sort_bin_kernel(const K* __restric
     __hip_atomic_store(my_entry, kLookbackInc | count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else {
     __hip_atomic_store(my_entry, kLookbackAgg | count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Predecessors' entries are read kLookAhead at a time (their loads in
+    // flight together) and consumed in order: aggregates are added until an
+    // inclusive prefix, or the first entry not yet published, which is polled
+    // again.
+    constexpr int kLookAhead = 4;
     int64_t j = (int64_t)tile - 1;
     uint32_t spins = 0;
-    while (j >= 0) {
-      uint32_t v = __hip_atomic_load(lookback + (uint64_t)j * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t flag = v & ~kLookbackMask;
-      if (flag == 0) {
+    bool done = false;
+    while (!done && j >= 0) {
+      uint32_t v[kLookAhead];
+#pragma unroll
+      for (int k = 0; k < kLookAhead; ++k)
+        v[k] = j - k >= 0 ? __hip_atomic_load(lookback + (uint64_t)(j - k) * 256 + t, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT)
+                          : 0u;
+      int k = 0;
+      bool waiting = false;
+      for (; k < kLookAhead && j - k >= 0; ++k) {
+        const uint32_t flag = v[k] & ~kLookbackMask;
+        if (flag == 0) {
+          waiting = true;
+          break;
+        }
+        exclusive += v[k] & kLookbackMask;
+        if (flag == kLookbackInc) {
+          done = true;
+          break;
+        }
+      }
+      if (done) break;
+      j -= k;
+      if (waiting) {
         // Every predecessor holds an earlier ticket and is resident, so it
         // publishes; the cap only bounds the wait if the table was corrupted,
         // and giving up is reported through the status word.
@@ -177,11 +203,7 @@ __global__ void __launch_bounds__(kSortBlock) sort_bin_kernel(const K* __restric
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      exclusive += v & kLookbackMask;
-      if (flag == kLookbackInc) break;
-      --j;
     }
     __hip_atomic_store(my_entry, kLookbackInc | (exclusive + count), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }

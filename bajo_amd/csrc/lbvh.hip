@@ -289,6 +289,15 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
   const uint64_t block_last = block_first + blockDim.x - 1 < n - 1 ? block_first + blockDim.x - 1 : n - 1;
   uint32_t cur = kLeafFlag | (uint32_t)s;
   uint32_t parent = leaf_parent[s];
+  // A node's topology (parent, left, right, fence: its last 16 bytes) and leaf
+  // count were written by the topology launch and do not change here, so the
+  // next level's are loaded while this level's boxes are exchanged.
+  uint4 topo = make_uint4(kSentinel, kSentinel, kSentinel, 0u);
+  uint32_t count = 0;
+  if (parent < node_total) {
+    topo = reinterpret_cast<const uint4*>(nodes + parent)[3];
+    count = leaf_counts[parent];
+  }
   // A Karras hierarchy over 64-bit keys is at most 64 levels deep; the index
   // and depth checks only keep a corrupted hierarchy from leaving the arrays,
   // and a climb they stop raises BAJO_STATUS_REFIT_CORRUPT.
@@ -299,14 +308,21 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
       break;
     }
     BjNode* node = nodes + parent;
-    const uint32_t left = node->left, right = node->right;  // written by the topology launch
+    const uint32_t left = topo.y, right = topo.z;
     const bool is_left = cur == left;
     if (!is_left && cur != right) {
       atomicMax(status, (uint32_t)BAJO_STATUS_REFIT_CORRUPT);
       break;
     }
     // Karras node i is one end of its leaf range [fence + 1 - count, fence]
-    const uint64_t last = node->fence, first = last + 1 - leaf_counts[parent];
+    const uint64_t last = topo.w, first = last + 1 - count;
+    const uint32_t next = topo.x;
+    uint4 next_topo = make_uint4(kSentinel, kSentinel, kSentinel, 0u);
+    uint32_t next_count = 0;
+    if (next < node_total) {
+      next_topo = reinterpret_cast<const uint4*>(nodes + next)[3];
+      next_count = leaf_counts[next];
+    }
     v3 olo, ohi;
     if (first >= block_first && last <= block_last && parent >= block_first && parent - block_first < kBlock) {
       // every leaf below this node belongs to this block
@@ -364,7 +380,9 @@ __global__ void __launch_bounds__(kBlock) refit_kernel(const float* __restrict__
       hi = vmax(ohi, hi);
     }
     cur = parent;
-    parent = node->parent;
+    parent = next;
+    topo = next_topo;
+    count = next_count;
   }
   if (depth == 128) atomicMax(status, (uint32_t)BAJO_STATUS_REFIT_CORRUPT);
 }

@@ -333,18 +333,27 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
     store_wt(ids + i, (uint32_t)i);
   }
   publish_fence();
+  // The current guide node's topology (parent, left, right, fence: the last 16
+  // bytes of its record) and first leaf: written by the guide's launch and
+  // constant here, so a parent's are loaded together with the arrival atomic.
+  uint4 topo = make_uint4(kSentinel, 0u, 0u, 0u);
+  uint32_t topo_first = 0;
   for (int it = 0; it < kHpMaxLoop; ++it) {
     if (active && !pending) {
-      const uint32_t parent = (cur & kLeafFlag) ? leaf_parent[cur & kIndexMask] : guide[cur].parent;
+      const uint32_t parent = (cur & kLeafFlag) ? leaf_parent[cur & kIndexMask] : topo.x;
       if (parent == kSentinel || (uint64_t)parent >= n - 1) {
         active = false;
       } else {
+        const uint4 ptopo = reinterpret_cast<const uint4*>(guide + parent)[3];
+        const uint32_t pfirst = range_first[parent];
         const uint32_t old = __hip_atomic_fetch_add(flags + parent, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == 0) {
           active = false;
         } else {
           cur = parent;
           pending = true;
+          topo = ptopo;
+          topo_first = pfirst;
         }
       }
     }
@@ -354,12 +363,12 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
     // the left one) and keeps climbing.  Only merging nodes take the wave.
     if (pending) {
       const uint32_t g = cur;
-      const uint32_t first = range_first[g], last = guide[g].fence;
-      const uint32_t gl = guide[g].left, gr = guide[g].right;
+      const uint32_t first = topo_first, last = topo.w;
+      const uint32_t gl = topo.y, gr = topo.z;
       const uint32_t split = gl & kIndexMask;
       const uint32_t cl = (gl & kLeafFlag) ? 1u : load_co(counts + split);
       const uint32_t cr = (gr & kLeafFlag) ? 1u : load_co(counts + (gr & kIndexMask));
-      const bool root = guide[g].parent == kSentinel;
+      const bool root = topo.x == kSentinel;
       const bool sane = cl >= 1 && cr >= 1 && cl <= threshold && cr <= threshold && first <= split &&
                         split < last && (uint64_t)last < n && cl <= split - first + 1 && cr <= last - split;
       if (sane && !root && cl + cr <= threshold) {
@@ -382,7 +391,7 @@ __global__ void __launch_bounds__(kHpBlock) hploc_kernel(const float* __restrict
                    seg_roots, threshold);
       if ((int)lane_id() == leader) {
         pending = false;
-        if (guide[g].parent == kSentinel) active = false;
+        if (topo.x == kSentinel) active = false;  // the leader's own node g
       }
     }
     if (!ballot(active)) return;

@@ -744,7 +744,7 @@ BJ_D Hit scene_trace(const SceneView& sc, const Ray& ray, Stack& st, TraceCounte
   }
 }
 
-template <int kAlg, bool kWide, bool kFirst, int kGeo, int kWaves = 6, bool kLoop = false>
+template <int kAlg, bool kWide, bool kFirst, int kGeo, int kWaves = 6>
 __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) extend_kernel(SceneView sc, Queues q, uint64_t seed, uint32_t bounce,
                                                           PrimaryArgs pa, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
@@ -759,15 +759,12 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(k
   const bool tail_done = q.counters[kTailDone] != 0;
   const uint32_t active = tail_done ? 0u : (q.counters[kActive] < cap ? q.counters[kActive] : (uint32_t)cap);
   uint64_t work_visits = 0, work_leaves = 0;
-  uint32_t ray_max = 0;
   // One thread per queue entry; the grid covers the arena, and blocks past
   // the live entries exit at once (a persistent loop here made the compiler
-  // keep far more registers live, halving the resident waves).  kLoop: a
-  // grid of the resident blocks strides over the entries instead.
+  // keep far more registers live, halving the resident waves).
   if ((uint64_t)blockIdx.x * kRtBlock >= active) return;
-  const uint64_t stride = kLoop ? (uint64_t)gridDim.x * kRtBlock : ~0ull >> 1;
-  for (uint64_t blk = (uint64_t)blockIdx.x * kRtBlock; blk < active; blk += stride) {
-    const uint32_t idx = (uint32_t)(blk + threadIdx.x);
+  {
+    const uint32_t idx = (uint32_t)gtid;
     Surface su;
     su.found = false;
     Ray ray;
@@ -800,7 +797,6 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(k
       const Hit h = scene_trace<BAJO_TRACE_CLOSEST_HIT, kWide, kGeo>(sc, ray, st, &tc, false, env);
       work_visits += tc.visits;
       work_leaves += tc.leaves;
-      ray_max = tc.visits > ray_max ? tc.visits : ray_max;
       // throughput, pdf and id are read after the traversal rather than
       // held in registers through it
       if constexpr (!kFirst) {
@@ -868,7 +864,7 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(k
       }
     }
   }
-  store_block_work(work, work_visits, work_leaves, ray_max);
+  store_block_work(work, work_visits, work_leaves, (uint32_t)work_visits);
 }
 
 // The remaining bounces of a chunk once few paths are left.  Each thread
@@ -1750,14 +1746,6 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
   }
   if (g_group > g_sh) g_group = g_sh;
   if (g_group < 1) g_group = 1;
-  // Bounces after the second run the extension kernel on its resident grid
-  // (a grid-stride loop): their live paths are usually few, or none once the
-  // tail has taken over, and a grid over the whole arena cost about 14 us of
-  // empty block exits per launch.
-  unsigned g_ext_loop = 1;
-  rc = grid_of(extend_kernel<kAlg, kWide, false, kGeo, 4, true>, &g_ext_loop);
-  if (rc) return rc;
-  if (g_ext_loop > g_ext) g_ext_loop = g_ext;
   // the per-lane shadow kernel's resident grid, for the walk form this render uses
   unsigned g_lane = 1;
   if (kAlg != BAJO_RENDER_PATH && kAlg != BAJO_RENDER_NORMALS) {
@@ -1866,9 +1854,6 @@ int32_t run_render(bajo_ctx* ctx, bajo_scene* s, RenderArena& a, const bajo_rend
             sv, q, st->rng_seed, b, pa, a.spill, a.work_ext, a.stats);
       else if (kTunable && ext_waves == 8)
         extend_kernel<kAlg, kWide, false, kGeo, kTunable ? 8 : 6><<<g_ext, kRtBlock, 0, stream>>>(
-            sv, q, st->rng_seed, b, pa, a.spill, a.work_ext, a.stats);
-      else if (b >= 2)
-        extend_kernel<kAlg, kWide, false, kGeo, 4, true><<<g_ext_loop, kRtBlock, 0, stream>>>(
             sv, q, st->rng_seed, b, pa, a.spill, a.work_ext, a.stats);
       else
         extend_kernel<kAlg, kWide, false, kGeo><<<g_ext, kRtBlock, 0, stream>>>(sv, q, st->rng_seed, b, pa, a.spill,

@@ -1207,7 +1207,8 @@ __global__ void __launch_bounds__(kRtBlock) tail_group_kernel(SceneView sc, Queu
 // 64 rays at a time from a cursor: a bounce that leaves the rays to another
 // kernel costs only the resident blocks' exits, not one block per arena entry.
 template <int kAlg, bool kWide, int kGeo, int kWalk = 0>
-__global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q, uint32_t bounce,
+__global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(kWalk == 2 ? 5 : 1)))
+shadow_kernel(SceneView sc, Queues q, uint32_t bounce,
                                                           uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ work,
                                                           unsigned long long* __restrict__ stats,
@@ -1224,20 +1225,23 @@ __global__ void __launch_bounds__(kRtBlock) shadow_kernel(SceneView sc, Queues q
   const uint32_t threads = gridDim.x * blockDim.x;
   for (uint32_t idx = (uint32_t)gtid; idx < count; idx += threads) {
     {
+      // the ray's planes now, its radiance planes (8..10) and id after the walk
       float f[kShadowPlanes];
 #pragma unroll
-      for (int p = 0; p < kShadowPlanes; ++p) f[p] = q.shadow_fields[plane_at<kShadowPlanes>(idx, p)];
-      const uint32_t sid = q.shadow_ids[idx];
-      const uint32_t slot = sid - q.sample_base;
+      for (int p = 0; p < 8; ++p) f[p] = q.shadow_fields[plane_at<kShadowPlanes>(idx, p)];
+      uint32_t env = 0;
+      if constexpr (kGeo == kGeoBatch) env = q.shadow_ids[idx] / sc.env_paths;
       const Ray ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
       TraceCounters tc;
-      const Hit h = scene_trace<kTraceShadow + kWalk, kWide, kGeo>(sc, ray, st, &tc, false,
-                                                                 kGeo == kGeoBatch ? sid / sc.env_paths : 0u);
+      const Hit h = scene_trace<kTraceShadow + kWalk, kWide, kGeo>(sc, ray, st, &tc, false, env);
       work_visits += tc.visits;
       work_leaves += tc.leaves;
       ray_max = tc.visits > ray_max ? tc.visits : ray_max;
       const bool occluded = h.prim == kEmpty && h.t == 0.0f;
       if (kAlg == BAJO_RENDER_AO || !occluded) {
+#pragma unroll
+        for (int p = 8; p < kShadowPlanes; ++p) f[p] = q.shadow_fields[plane_at<kShadowPlanes>(idx, p)];
+        const uint32_t slot = q.shadow_ids[idx] - q.sample_base;
         const v3 c = shadow_radiance<kAlg>(occluded, f);
         accumulate(q.sample, slot, c);
         if constexpr (kAlg == BAJO_RENDER_GUIDED)

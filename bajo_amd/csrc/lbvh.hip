@@ -159,9 +159,40 @@ __global__ void __launch_bounds__(kBlock) morton_kernel(const float* __restrict_
 }
 
 // Karras common prefix over (code << 32 | sorted index), -1 outside [0, n).
-BJ_D int karras_delta(const uint64_t* __restrict__ keys, int64_t n, int64_t i, uint32_t code_i, int64_t j) {
+// The Morton codes (high key words) of a block's neighbourhood staged in LDS:
+// the range and split searches of a block's nodes stay mostly within a few
+// hundred positions of the block, and a code outside the window is read from
+// the keys.  Index j of a hierarchy (a segment's keys start at `base`) is
+// global position base + j.
+constexpr int64_t kCodeBefore = 512, kCodeAfter = 768;
+struct CodeWindow {
+  const uint32_t* lds;
+  int64_t lo, hi;  // global positions held in lds[0, hi - lo)
+  const uint64_t* keys;
+  int64_t base;
+  BJ_D uint32_t code(int64_t j) const {
+    const int64_t g = base + j;
+    return (g >= lo && g < hi) ? lds[g - lo] : (uint32_t)(keys[g] >> 32);
+  }
+};
+
+// Stages the codes of global positions [first - kCodeBefore, first + kCodeAfter)
+// of the n keys (clipped) for the block whose first position is `first`.
+BJ_D CodeWindow stage_codes(const uint64_t* __restrict__ keys, int64_t n, int64_t first, uint32_t* s_codes) {
+  CodeWindow cw;
+  cw.lo = first - kCodeBefore < 0 ? 0 : first - kCodeBefore;
+  cw.hi = first + kCodeAfter < n ? first + kCodeAfter : n;
+  for (int64_t g = cw.lo + threadIdx.x; g < cw.hi; g += blockDim.x) s_codes[g - cw.lo] = (uint32_t)(keys[g] >> 32);
+  __syncthreads();
+  cw.lds = s_codes;
+  cw.keys = keys;
+  cw.base = 0;
+  return cw;
+}
+
+BJ_D int karras_delta(const CodeWindow& keys, int64_t n, int64_t i, uint32_t code_i, int64_t j) {
   if (j < 0 || j >= n) return -1;
-  uint32_t code_j = (uint32_t)(keys[j] >> 32);
+  uint32_t code_j = keys.code(j);
   uint64_t x = ((uint64_t)(code_i ^ code_j) << 32) | (uint64_t)((uint32_t)i ^ (uint32_t)j);
   return x == 0 ? 64 : __builtin_clzll(x);
 }
@@ -171,11 +202,11 @@ BJ_D int karras_delta(const uint64_t* __restrict__ keys, int64_t n, int64_t i, u
 // sorted position, inner references node_base + the node (both bases are 0
 // for a single hierarchy; a segmented build places each segment's nodes and
 // leaves at its own bases).
-BJ_D void karras_node(const uint64_t* __restrict__ keys, int64_t n, int64_t i, BjNode* __restrict__ nodes,
+BJ_D void karras_node(const CodeWindow& keys, int64_t n, int64_t i, BjNode* __restrict__ nodes,
                       uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ leaf_counts,
                       uint32_t* __restrict__ flags, uint32_t* __restrict__ range_first, uint32_t node_base,
                       uint32_t leaf_base) {
-  const uint32_t ci = (uint32_t)(keys[i] >> 32);
+  const uint32_t ci = keys.code(i);
   // bvh/gpu/builder/lbvh.mojo:164-191: direction and range.
   int d_next = karras_delta(keys, n, i, ci, i + 1);
   int d_prev = karras_delta(keys, n, i, ci, i - 1);
@@ -192,7 +223,7 @@ BJ_D void karras_node(const uint64_t* __restrict__ keys, int64_t n, int64_t i, B
   int64_t j = i + l * d;
   int64_t first = i < j ? i : j, last = i < j ? j : i;
   // bvh/gpu/builder/lbvh.mojo:194-218: split by binary search.
-  const uint32_t cf = (uint32_t)(keys[first] >> 32);
+  const uint32_t cf = keys.code(first);
   int node_prefix = karras_delta(keys, n, first, cf, last);
   int64_t split = first, step = last - first;
   while (step > 1) {
@@ -234,10 +265,12 @@ __global__ void __launch_bounds__(kBlock) karras_kernel(const uint64_t* __restri
                                                         uint32_t* __restrict__ leaf_counts,
                                                         uint32_t* __restrict__ flags,
                                                         uint32_t* __restrict__ range_first) {
+  __shared__ uint32_t s_codes[kCodeBefore + kCodeAfter];
   const int64_t n = (int64_t)n_u;
+  const CodeWindow cw = stage_codes(keys, n, (int64_t)blockIdx.x * blockDim.x, s_codes);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
-  karras_node(keys, n, i, nodes, leaf_parent, leaf_counts, flags, range_first, 0u, 0u);
+  karras_node(cw, n, i, nodes, leaf_parent, leaf_counts, flags, range_first, 0u, 0u);
 }
 
 namespace {
@@ -565,12 +598,15 @@ __global__ void __launch_bounds__(kBlock) seg_karras_kernel(const uint64_t* __re
                                                             uint32_t* __restrict__ leaf_counts,
                                                             uint32_t* __restrict__ flags,
                                                             uint32_t* __restrict__ range_first) {
+  __shared__ uint32_t s_codes[kCodeBefore + kCodeAfter];
+  CodeWindow cw = stage_codes(keys, (int64_t)n, (int64_t)blockIdx.x * blockDim.x, s_codes);
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j + 1 >= n) return;
   const uint32_t seg = segment_at(off, chunk, j);
   const uint64_t first = off[seg], count = off[seg + 1] - first;
   if (j - first + 1 >= count) return;  // the segment's unused last slot
-  karras_node(keys + first, (int64_t)count, (int64_t)(j - first), nodes, leaf_parent, leaf_counts, flags, range_first,
+  cw.base = (int64_t)first;
+  karras_node(cw, (int64_t)count, (int64_t)(j - first), nodes, leaf_parent, leaf_counts, flags, range_first,
               (uint32_t)first, (uint32_t)first);
 }
 

@@ -19,10 +19,10 @@ Here six launch knobs of `RenderSettings` form the schedule:
   ahead or four at a time.  Shadow rays leave a surface, so their walks are
   long serial chains; the forms trade registers (occupancy) for memory-level
   parallelism within a ray.
-- ``tail_group``: lanes per path in the tail kernel (1 or 16).  Sixteen lanes
-  walk a path's extension ray (exactly as one lane would decide it) and its
-  shadow ray a node group at a time; one lane per path keeps more paths in
-  flight.
+- ``tail_group``: lanes per path in the tail kernel (1, 16, 32 or 64).  A
+  lane group walks a path's extension ray (exactly as one lane would decide
+  it) and its shadow ray a node group at a time (one node per eight lanes);
+  one lane per path keeps more paths in flight.
 - ``tail_paths``: the live-path count at which the tail kernel (one launch
   that finishes a chunk's remaining bounces with paired path / shadow lanes)
   takes over from the per-bounce launches.  Earlier saves launches on long
@@ -73,7 +73,7 @@ def default_space(total_paths: int, spp: int) -> dict[str, list[int]]:
         "extend_waves": [0, 4, 8],
         "shadow_group": [0, 1, 8, 32],
         "shadow_walk": [0, 1, 2, 4],
-        "tail_group": [0, 1],
+        "tail_group": [0, 1, 16, 64],
         "tail_paths": [0, 1 << 15, 1 << 17, 1 << 19, 1 << 21],
         "path_capacity": default_candidates(total_paths, spp),
     }

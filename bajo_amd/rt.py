@@ -57,7 +57,7 @@ class RenderSettings:
     shadow_walk: int = 0  # shadow any-hit walk: 1 one node, 2 two nodes, 3 / 4 with 4-record leaf loads; 0 = default
     sparse_shadow_rays: int = 0  # shadow rays per bounce traced one per wave at or below; 0 = 8192, 2**32-1 never
     shadow_group: int = 0  # lanes per shadow ray above that: 1, 8, 16 or 32; 0 = default
-    tail_group: int = 0  # lanes per path in the tail kernel: 1 or 16; 0 = default
+    tail_group: int = 0  # lanes per path in the tail kernel: 1, 16, 32 or 64; 0 = default
 
     def as_c(self, algorithm: int) -> _lib.RenderSettings:
         return _lib.RenderSettings(

@@ -385,9 +385,10 @@ typedef struct bajo_scene bajo_scene;
  *     (one node per 8 lanes per step); 0 = 16 lanes up to 2^20 rays and one
  *     lane per ray beyond;
  *   tail_group: lanes per path in the tail kernel (4- and 8-wide triangle
- *     and batch scenes): 1 = one lane per path, 16 = sixteen (extension rays
- *     by a group walk equal to the one-lane walk, shadow rays by group any
- *     hit); 0 = 16 up to 2^15 live paths, one beyond. */
+ *     and batch scenes): 1 = one lane per path, 16, 32 or 64 = that many
+ *     (extension rays by a group walk equal to the one-lane walk, shadow
+ *     rays by group any hit; one node per 8 lanes per step); 0 = 32 up to
+ *     2^15 live paths, one beyond. */
 typedef struct {
   uint32_t image_width, image_height, samples_per_pixel, max_depth;
   uint64_t rng_seed;

@@ -138,7 +138,7 @@ def test_autotuned_schedule_renders_the_same_image(gpu_ctx, batch):
     assert tuned.path_capacity == min((c for _, c in table), key=lambda c: table[("path_capacity", c)])
     # every knob of the schedule, each value on its own: images stay bit-identical
     space = {"extend_waves": [0, 4, 6, 8], "shadow_group": [0, 1, 8, 16, 32], "shadow_walk": [0, 1, 2, 3, 4],
-             "tail_group": [0, 1, 16], "tail_paths": [0, 1, 64, 1 << 20], "path_capacity": [0, 1536]}
+             "tail_group": [0, 1, 16, 32, 64], "tail_paths": [0, 1, 64, 1 << 20], "path_capacity": [0, 1536]}
     for knob, values in space.items():
         for value in values:
             got = enqueue_render_gpu(replace(settings(), **{knob: value}), camera(), g, ALGS["MIS"]).numpy()

@@ -1,7 +1,7 @@
 """The lane-group tail gives the per-lane tail's images bit for bit.
 
 The tail kernel carries a chunk's last paths through its remaining bounces.  With `tail_group = 16`
-each path gets 16 lanes: the extension ray through GroupClosest (which defers to Walk8 when two
+(or 32, 64) each path gets that many lanes: the extension ray through GroupClosest (which defers to Walk8 when two
 candidates lie within rounding of each other, since Walk8's winner can then depend on test order),
 shading on the group's leader, the shadow ray through GroupWalk.  Images must equal the one-lane
 tail's, and the ray counts the oracle's (bajo's wavefront protocol, test/rt/test_gpu.mojo:346-730).
@@ -63,7 +63,7 @@ def test_tail_groups_equal_per_lane_tail(gpu_ctx, width, alg, dup):
     img = ref.numpy().reshape(-1)
     rmse = float(np.sqrt(np.mean((img.astype(np.float64) - want) ** 2)))
     assert rmse <= 1e-3 and float(np.mean(img.view(np.uint32) == want.view(np.uint32))) > 0.99
-    for tg in (16, 0):
+    for tg in (16, 32, 64, 0):
         res = enqueue_render_gpu(replace(base, tail_group=tg), box_camera(), g, alg, with_stats=True)
         np.testing.assert_array_equal(res.numpy().view(np.uint32), ref.numpy().view(np.uint32),
                                       err_msg=f"tail_group {tg}")
@@ -85,7 +85,7 @@ def test_tail_groups_icosphere_and_batch(gpu_ctx):
     for alg in (PATH, NEE):
         base = RenderSettings(96, 54, 4, 2026, 5)
         imgs = [enqueue_render_gpu(replace(base, tail_group=tg, tail_paths=tp), cam, g, alg).numpy()
-                for tg, tp in ((1, 0), (16, 0), (16, 1 << 30), (0, 0))]
+                for tg, tp in ((1, 0), (16, 0), (16, 1 << 30), (32, 1 << 30), (64, 1 << 30), (0, 0))]
         for im in imgs[1:]:
             np.testing.assert_array_equal(im, imgs[0])
     g.free()
@@ -94,6 +94,7 @@ def test_tail_groups_icosphere_and_batch(gpu_ctx):
     bcam = Camera((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 0.41421356).flatten()
     base = RenderSettings(24, 16, 4, 4242, 3, tail_paths=1 << 30)
     a = enqueue_render_gpu(replace(base, tail_group=1), bcam, gb, NEE).numpy()
-    b = enqueue_render_gpu(replace(base, tail_group=16), bcam, gb, NEE).numpy()
-    np.testing.assert_array_equal(a, b)
+    for tg in (16, 32, 64):
+        b = enqueue_render_gpu(replace(base, tail_group=tg), bcam, gb, NEE).numpy()
+        np.testing.assert_array_equal(a, b, err_msg=f"tail_group {tg}")
     gb.free()

@@ -40,5 +40,5 @@ def test_autotune_space_covers_every_knob():
     sp = default_space(67_108_864, 4)
     assert tuple(sp) == KNOBS
     assert sp["extend_waves"] == [0, 4, 8] and sp["tail_paths"][0] == 0 and sp["shadow_walk"][0] == 0
-    assert sp["shadow_group"][0] == 0 and sp["tail_group"] == [0, 1]
+    assert sp["shadow_group"][0] == 0 and sp["tail_group"] == [0, 1, 16, 64]
     assert sp["path_capacity"] == default_candidates(67_108_864, 4)

@@ -1061,7 +1061,7 @@ def main() -> int:
             "traffic": traffic,
             "measured_hbm_gbs": traffic / (ext_ms_nee / 1e3) / 1e9 if traffic else None,
             "measured_hbm_frac": traffic / (ext_ms_nee / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None,
-            "limiter": "VALU instruction issue: SQ_INSTS_VALU x 4 cycles per wave64 instruction fills 0.90-1.03 of "
+            "limiter": "VALU instruction issue: SQ_INSTS_VALU x 4 cycles per wave64 instruction fills 0.91-1.04 of "
             "the 1,024 SIMDs' issue slots over the extend launches (profiles/extend_kernel_valu_counters.md); "
             "algorithmic bytes are mostly served from L2 / MALL, measured HBM traffic is far below them",
             "traffic_note": "HBM bytes per NEE render over the extend launches, rocprofv3 FETCH_SIZE x 2.0 + "

@@ -57,6 +57,16 @@ constexpr int kActive = 0, kNext = 1, kShade = 2, kShadow = 3, kStatus = 4, kFet
 constexpr int kVertex = 9;
 // blocks of work partials the bounce's shadow kernel wrote (each sets it when it traces the bounce's rays)
 constexpr int kSparseShadow = 10;
+// Next paths leaving their surface at a grazing angle are appended from the
+// back of the next queue, the others from the front; words kNextPair and
+// kNextPair + 1 count the two parts (one 64-bit atomic per wave reserves
+// both), and kFront holds the front part's length for the next bounce:
+// grazing rays walk many boxes along the surface, steep ones leave the
+// hierarchy after a few, so each wave of the next bounce gets walks of like
+// length.  A path's sample and random numbers do not depend on its queue
+// position.
+constexpr int kNextPair = 12, kFront = 14;
+constexpr float kGrazingCos = 0.5f;
 // shadow rays per bounce at or below which one wave traces each ray
 constexpr uint32_t kSparseShadowRays = 8192;
 // Lanes per shadow ray above that threshold when the settings leave it at 0:
@@ -235,7 +245,7 @@ __global__ void init_counters_kernel(uint32_t* counters, uint32_t active, unsign
   const uint32_t t = threadIdx.x;
   if (t < kCounterWords) {
     const bool status_word = t == kStatus || t == 8;
-    if (!status_word || first) counters[t] = t == kActive ? active : 0u;
+    if (!status_word || first) counters[t] = (t == kActive || t == kFront) ? active : 0u;
   }
   if (first)
     for (uint32_t i = t; i < kStatsWords; i += blockDim.x) stats[i] = 0;
@@ -319,9 +329,14 @@ __global__ void __launch_bounds__(64) advance_kernel(uint32_t* counters, unsigne
   }
   stats[kGuidedWord] += counters[kVertex];
   // appends past the arena were dropped (and flagged); never read past it
-  const uint32_t next = counters[kNext];
-  counters[kActive] = next < cap ? next : cap;
+  const uint32_t nf = counters[kNextPair], nb = counters[kNextPair + 1];
+  const uint32_t front = nf < cap ? nf : cap;
+  const uint32_t back = nb < cap - front ? nb : cap - front;
+  counters[kActive] = front + back;
+  counters[kFront] = front;
   counters[kNext] = 0;
+  counters[kNextPair] = 0;
+  counters[kNextPair + 1] = 0;
   counters[kShade] = 0;
   counters[kShadow] = 0;
   counters[kSparseShadow] = 0;
@@ -390,6 +405,40 @@ BJ_D uint32_t wave_reserve(uint32_t* counter, bool want) {
   if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__builtin_popcountll(m));
   base = __shfl(base, (int)leader, 64);
   return want ? base + mask_rank(m) : 0xFFFFFFFFu;
+}
+
+// The queue slot of a bounce's entry `idx`: the front part in append order,
+// then the back part from the last slot down.
+BJ_D uint32_t src_slot(uint32_t idx, uint32_t front, uint64_t cap) {
+  return idx < front ? idx : (uint32_t)(cap - 1u - (idx - front));
+}
+
+// Appends the next paths of a wave, grazing ones (|cos| to the surface normal
+// below kGrazingCos) from the back of the queue, with one atomic on the
+// packed pair of counts; returns the slot (the capacity when the queue is
+// full).
+BJ_D uint32_t reserve_next(const Queues& q, bool want, bool grazing) {
+  const uint64_t mf = ballot(want && !grazing), mb = ballot(want && grazing);
+  if ((mf | mb) == 0) return 0xFFFFFFFFu;
+  const uint32_t leader = (uint32_t)__builtin_ctzll(mf | mb);
+  uint32_t lo = 0, hi = 0;
+  if (lane_id() == leader) {
+    const unsigned long long inc =
+        (unsigned long long)__builtin_popcountll(mf) | ((unsigned long long)__builtin_popcountll(mb) << 32);
+    const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(q.counters + kNextPair), inc);
+    lo = (uint32_t)old;
+    hi = (uint32_t)(old >> 32);
+  }
+  lo = __shfl(lo, (int)leader, 64);
+  hi = __shfl(hi, (int)leader, 64);
+  if (!want) return 0xFFFFFFFFu;
+  const uint32_t cap = (uint32_t)q.capacity;
+  if (!grazing) {
+    const uint32_t ns = lo + mask_rank(mf);
+    return ns < cap ? ns : cap;
+  }
+  const uint32_t nb = hi + mask_rank(mb);
+  return nb < cap ? cap - 1u - nb : cap;
 }
 
 // rt/gpu/path_shading.mojo:261-403 (triangle lights only).
@@ -767,6 +816,7 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(k
   if ((uint64_t)blockIdx.x * kRtBlock >= active) return;
   {
     const uint32_t idx = (uint32_t)gtid;
+    const uint32_t sidx = kFirst ? idx : src_slot(idx, q.counters[kFront], cap);  // this entry's queue slot
     Surface su;
     su.found = false;
     Ray ray;
@@ -789,12 +839,12 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(k
         for (int p = 0; p < kSamplePlanes; ++p) q.sample[plane_at<kSamplePlanes>(idx, p)] = 0.0f;
       } else {
 #pragma unroll
-        for (int p = 0; p < 8; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(idx, p)];
+        for (int p = 0; p < 8; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(sidx, p)];
       }
       ray = Ray{mk(f[0], f[1], f[2]), mk(f[4], f[5], f[6]), f[3], f[7]};
       uint32_t env = 0;
       if constexpr (kGeo == kGeoBatch)
-        env = (kFirst ? q.sample_base + idx : (q.src_ids[idx] & kPathIdMask)) / sc.env_paths;
+        env = (kFirst ? q.sample_base + idx : (q.src_ids[sidx] & kPathIdMask)) / sc.env_paths;
       TraceCounters tc;
       const Hit h = scene_trace<BAJO_TRACE_CLOSEST_HIT, kWide, kGeo>(sc, ray, st, &tc, false, env);
       work_visits += tc.visits;
@@ -802,11 +852,11 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(k
       // throughput, pdf and id are read after the traversal rather than
       // held in registers through it
       if constexpr (!kFirst) {
-        const uint32_t idw = q.src_ids[idx];
+        const uint32_t idw = q.src_ids[sidx];
         path_id = idw & kPathIdMask;
         delta = (idw & kPathDelta) != 0;
-        T = mk(q.src_fields[plane_at<kPathPlanes>(idx, 8)], q.src_fields[plane_at<kPathPlanes>(idx, 9)], q.src_fields[plane_at<kPathPlanes>(idx, 10)]);
-        path_pdf = q.src_fields[plane_at<kPathPlanes>(idx, 11)];
+        T = mk(q.src_fields[plane_at<kPathPlanes>(sidx, 8)], q.src_fields[plane_at<kPathPlanes>(sidx, 9)], q.src_fields[plane_at<kPathPlanes>(sidx, 10)]);
+        path_pdf = q.src_fields[plane_at<kPathPlanes>(sidx, 11)];
       } else {
         path_id = q.sample_base + idx;
         delta = true;
@@ -855,7 +905,8 @@ __global__ void __launch_bounds__(kRtBlock) __attribute__((amdgpu_waves_per_eu(k
         }
       }
     }
-    const uint32_t ns = wave_reserve(q.counters + kNext, want_next);
+    const bool grazing = want_next && fabsf(dot(mk(np.f[4], np.f[5], np.f[6]), su.n)) < kGrazingCos;
+    const uint32_t ns = reserve_next(q, want_next, grazing);
     if (want_next) {
       if (ns < cap) {
         q.dst_ids[ns] = np.id;
@@ -919,11 +970,13 @@ __global__ void __launch_bounds__(kRtBlock) tail_kernel(SceneView sc, Queues q, 
     float acc[3] = {0.0f, 0.0f, 0.0f};
     uint32_t slot = 0;
     if (alive) {
-      const uint32_t idw = q.src_ids[idx];
+      // the queue's back part (grazing paths, the long ones) first
+      const uint32_t sidx = src_slot(active - 1u - idx, q.counters[kFront], cap);
+      const uint32_t idw = q.src_ids[sidx];
       path_id = idw & kPathIdMask;
       delta = (idw & kPathDelta) != 0;
 #pragma unroll
-      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(idx, p)];
+      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(sidx, p)];
       slot = path_id - q.sample_base;
 #pragma unroll
       for (int p = 0; p < kSamplePlanes; ++p) acc[p] = q.sample[plane_at<kSamplePlanes>(slot, p)];
@@ -1076,11 +1129,13 @@ __global__ void __launch_bounds__(kRtBlock) tail_group_kernel(SceneView sc, Queu
 #pragma unroll
     for (int p = 0; p < kPathPlanes; ++p) f[p] = 0.0f;
     if (alive) {
-      const uint32_t idw = q.src_ids[idx];
+      // the queue's back part (grazing paths, the long ones) first
+      const uint32_t sidx = src_slot(active - 1u - idx, q.counters[kFront], cap);
+      const uint32_t idw = q.src_ids[sidx];
       path_id = idw & kPathIdMask;
       delta = (idw & kPathDelta) != 0;
 #pragma unroll
-      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(idx, p)];
+      for (int p = 0; p < kPathPlanes; ++p) f[p] = q.src_fields[plane_at<kPathPlanes>(sidx, p)];
       slot = path_id - q.sample_base;
       if (leader) {
 #pragma unroll
@@ -1407,7 +1462,7 @@ __global__ void __launch_bounds__(kRtBlock) guide_sample_kernel(SceneView sc, Qu
       for (int k = 0; k < kRecordPlanes; ++k) r[k] = rf[k];
     }
   }
-  const uint32_t ns = wave_reserve(q.counters + kNext, want_next);
+  const uint32_t ns = reserve_next(q, want_next, false);
   if (want_next) {
     if (ns < cap) {
       q.dst_ids[ns] = np.id;
